@@ -1,0 +1,55 @@
+/*
+ * olpt_oracle.h - CPU restatement of the reference IPR hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's CPU-baseline leg may call it.
+ *
+ * Plain C11, FP64 throughout, operations ordered as in the reference so that
+ * results agree to the last bit wherever glibc is the common math library.
+ * Entry points take the same arguments as the reference harness
+ * (ref_harness.cpp) so that tests can run the two side by side.
+ * Return value: 0 ok, 1 capacity exceeded, 2 invalid argument,
+ * 3 unsupported configuration, 4 allocation failure.
+ */
+#ifndef OLPT_ORACLE_H
+#define OLPT_ORACLE_H
+
+#include <stdint.h>
+#include "olpt_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int orc_project(const olpt_camera *cam, const double *xyz, int n, double *uv);
+int orc_line_of_sight(const olpt_camera *cam, const double *uv, int n,
+                      double *out);
+int orc_triangulate(const double *lines, int n_line, double *xyz, double *err);
+int orc_polyfit(const double *x, const double *y, int n, int order,
+                double *coeff);
+int orc_otf_param(const olpt_otf *otf, int cam_id, const double *xyz, int n,
+                  double *out);
+int orc_find_peaks(const double *img, int H, int W, const olpt_ipr_params *p,
+                   int cap, double *xy, int32_t *n_out);
+int orc_stereo_match(const olpt_camera *cams, int n_cam, const int32_t *active,
+                     const double *xy, const int32_t *n_pts,
+                     const olpt_ipr_params *p, int cap_cand, int32_t *cand_ids,
+                     double *cand_e, int32_t *sel_idx, int cap_obj,
+                     double *obj_xyz, int32_t *obj_ids, int32_t *counts);
+int orc_shake(const olpt_camera *cams, int n_cam, const int32_t *active,
+              const double *imgs, int H, int W, const olpt_ipr_params *p,
+              const olpt_otf *otf, const double *xyz_in, int n_obj,
+              double *xyz_out, uint8_t *flags, double *score, double *res_out);
+int orc_residual(const olpt_camera *cams, int n_cam, const double *imgs, int H,
+                 int W, const olpt_ipr_params *p, const olpt_otf *otf,
+                 const double *xyz, int n_obj, double *res_out);
+int orc_run_ipr(const olpt_camera *cams, int n_cam, const double *imgs, int H,
+                int W, const olpt_ipr_params *p, const olpt_otf *otf, int cap,
+                double *obj_xyz, double *obj_uv, int cap_loop,
+                int32_t *loop_total, int32_t *counts);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
