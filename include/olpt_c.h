@@ -1,0 +1,138 @@
+/*
+ * olpt_c.h - C-ABI of the MI355X IPR library (libolpt_hip.so).
+ *
+ * Plain pointers and sizes, no C++ or torch types.  Pointers named d_* are
+ * device (HBM) pointers; everything else is host memory.  Entry points never
+ * throw: they return an olpt_status and keep a thread-local message for
+ * olpt_last_error().  Status codes 1-5 mirror the reference's ErrorCode
+ * (inc/error.hpp:41-48); codes >= 100 are GPU-side failures.
+ *
+ * A context owns one HIP stream (not re-entrant; use one context per
+ * thread/stream) and a library-owned device workspace sized by
+ * olpt_ctx_reserve().  Arithmetic precision of projection / residual /
+ * shake is OLPT_PREC_F32 (default) or OLPT_PREC_F64 (parity mode); line of
+ * sight, triangulation and the stereo tolerance tests are always FP64.
+ */
+#ifndef OLPT_C_H
+#define OLPT_C_H
+
+#include <stdint.h>
+#include "olpt_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct olpt_ctx olpt_ctx;
+typedef int32_t olpt_status;
+
+enum {
+  OLPT_OK = 0,
+  OLPT_E_INVALID_ARGUMENT = 1, /* ErrorCode::InvalidArgument */
+  OLPT_E_OUT_OF_RANGE = 2,     /* ErrorCode::OutOfRange */
+  OLPT_E_UNSUPPORTED = 3,      /* ErrorCode::UnsupportedType */
+  OLPT_E_IO = 4,               /* ErrorCode::IOfailure */
+  OLPT_E_NO_ENOUGH_DATA = 5,   /* ErrorCode::NoEnoughData */
+  OLPT_E_GPU = 100,            /* HIP runtime error */
+  OLPT_E_ALLOC = 101,          /* device allocation failed */
+  OLPT_E_CAPACITY = 102        /* a workspace capacity was exceeded */
+};
+
+enum { OLPT_U8 = 0, OLPT_U16 = 1, OLPT_F32 = 2, OLPT_F64 = 3 };
+enum { OLPT_PREC_F32 = 0, OLPT_PREC_F64 = 1 };
+
+/* ---- context -------------------------------------------------------- */
+/* stream: a hipStream_t, or NULL for the device's null stream. */
+olpt_status olpt_ctx_create(int device, void *stream, olpt_ctx **out);
+olpt_status olpt_ctx_destroy(olpt_ctx *ctx);
+/* Size the workspace: cameras, image size, per-camera 2D capacity, stereo
+ * candidate capacity, per-loop object capacity.  0 selects a default. */
+olpt_status olpt_ctx_reserve(olpt_ctx *ctx, int n_cam, int n_row, int n_col,
+                             int max_2d_per_cam, int max_candidates,
+                             int max_objects);
+olpt_status olpt_set_precision(olpt_ctx *ctx, int precision);
+/* Replaces Camera objects (Camera.h:78-174, pinhole model). */
+olpt_status olpt_set_cameras(olpt_ctx *ctx, const olpt_camera *cams, int n_cam);
+/* Replaces TracerConfig::_otf (OTF.h:33-78); arrays are copied. */
+olpt_status olpt_set_otf(olpt_ctx *ctx, const olpt_otf *otf);
+olpt_status olpt_sync(olpt_ctx *ctx);
+void *olpt_stream(olpt_ctx *ctx);
+const char *olpt_last_error(void);
+/* Device-side diagnostic counters of the last stereo/shake run:
+ * [0] candidate overflow, [1] LDS list overflow, [2] pool overflow,
+ * [3] prune replacement cases, [4] object overflow, [5] prune rounds
+ * exhausted, [6] checkMatch triangulation failures, [7] peak overflow. */
+olpt_status olpt_diagnostics(olpt_ctx *ctx, int32_t out[8]);
+
+/* ---- stages ---------------------------------------------------------- */
+/* ObjectFinder2D::findObject2D, tracer branch (ObjectFinder.cpp:37-111).
+ * One image (dtype OLPT_U8..F64, row-major n_row x n_col).  Writes (x, y)
+ * pairs in row-major scan order to d_xy (capacity cap) and the count to
+ * *n_out.  Returns OLPT_E_CAPACITY if more than cap peaks exist. */
+olpt_status olpt_find_peaks(olpt_ctx *ctx, const void *d_img, int dtype,
+                            int n_row, int n_col,
+                            const olpt_ipr_params *params, double *d_xy,
+                            int cap, int32_t *n_out);
+
+/* StereoMatch(cams, obj2d_list, cfg).match() (StereoMatch.cpp:65-178).
+ * d_xy: per camera, cap_2d (x, y) pairs ([n_cam][cap_2d][2]); n_pts: host
+ * counts; active: host flags (NULL = all active).  Outputs (device):
+ * the checked candidates in id-sorted order (d_cand_ids [n_cand][n_cam],
+ * d_cand_e), the selection flags d_sel [n_cand] (1 = kept by pruneMatch),
+ * the triangulated objects (d_obj_xyz [n_obj][3], d_obj_ids
+ * [n_obj][n_cam]).  counts (host) = {n_cand, n_sel, n_obj}.  Any of the
+ * candidate outputs may be NULL. */
+olpt_status olpt_stereo_match(olpt_ctx *ctx, const int32_t *active,
+                              const double *d_xy, int cap_2d,
+                              const int32_t *n_pts,
+                              const olpt_ipr_params *params,
+                              int32_t *d_cand_ids, double *d_cand_e,
+                              uint8_t *d_sel, int cap_cand, double *d_obj_xyz,
+                              int32_t *d_obj_ids, int cap_obj,
+                              int32_t *counts);
+
+/* Shake(cams, cfg).runShake(objs, imgs) (Shake.cpp:36-163) on n_obj
+ * objects, then calResidualImage of the unflagged objects (IPR mode,
+ * Shake.h:40-48).  d_imgs: [n_cam][n_row][n_col] of dtype.  Outputs:
+ * shaken positions, flags (Shake.h:19 bits), scores, and (optional)
+ * residual images d_res [n_cam][n_row][n_col] in the context precision. */
+olpt_status olpt_shake(olpt_ctx *ctx, const int32_t *active,
+                       const void *d_imgs, int dtype,
+                       const olpt_ipr_params *params, const double *d_xyz_in,
+                       int n_obj, double *d_xyz_out, uint8_t *d_flags,
+                       double *d_score, void *d_res);
+
+/* IPR-mode residual of given objects (Shake::calResidualImage, all cameras,
+ * clamped at 0) into d_res (context precision). */
+olpt_status olpt_residual(olpt_ctx *ctx, const void *d_imgs, int dtype,
+                          const olpt_ipr_params *params, const double *d_xyz,
+                          int n_obj, void *d_res);
+
+/* IPR::runIPR (IPR.cpp:195-306) on one frame, or on n_frames frames stored
+ * back to back (d_imgs [n_frames][n_cam][n_row][n_col]).  Per frame f the
+ * objects of all loops are written in loop order to d_obj_xyz
+ * [f][cap_obj_frame][3] and their final projections to d_obj_uv
+ * [f][cap_obj_frame][n_cam][2] (may be NULL); d_n_obj[f] holds the count.
+ * loop_total (host, optional, [n_frames][cap_loop]) receives the
+ * cumulative count after each IPR loop. */
+olpt_status olpt_ipr_run(olpt_ctx *ctx, const void *d_imgs, int dtype,
+                         int n_frames, const olpt_ipr_params *params,
+                         double *d_obj_xyz, double *d_obj_uv,
+                         int cap_obj_frame, int32_t *d_n_obj,
+                         int32_t *loop_total, int cap_loop);
+
+/* Capture olpt_ipr_run's kernel sequence for these exact arguments into a
+ * hipGraph owned by the context, and replay it.  Captured pointers stay
+ * fixed; replays overwrite the same outputs. */
+olpt_status olpt_ipr_graph_capture(olpt_ctx *ctx, const void *d_imgs,
+                                   int dtype, int n_frames,
+                                   const olpt_ipr_params *params,
+                                   double *d_obj_xyz, double *d_obj_uv,
+                                   int cap_obj_frame, int32_t *d_n_obj);
+olpt_status olpt_ipr_graph_launch(olpt_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OLPT_C_H */

@@ -1,0 +1,94 @@
+"""ctypes binding of libolpt_hip.so (include/olpt_c.h).
+
+The shared library is the product: if it is missing this module raises at
+import time - there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+from .synth import OlptCamera, OlptIprParams, OlptOtf
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libolpt_hip.so")
+
+OLPT_OK = 0
+OLPT_U8, OLPT_U16, OLPT_F32, OLPT_F64 = 0, 1, 2, 3
+OLPT_PREC_F32, OLPT_PREC_F64 = 0, 1
+
+# ErrorCode names of the reference (inc/error.hpp:41-48) + GPU codes
+STATUS_NAMES = {
+    1: "InvalidArgument", 2: "OutOfRange", 3: "UnsupportedType",
+    4: "IOfailure", 5: "NoEnoughData", 100: "GpuError",
+    101: "AllocationFailed", 102: "CapacityExceeded",
+}
+
+
+class FatalError(RuntimeError):
+    """Raised for a non-zero olpt_status (the reference raises FatalError
+    carrying an ErrorCode, inc/error.hpp:98-126)."""
+
+    def __init__(self, status: int, message: str):
+        self.status = status
+        super().__init__(f"[{STATUS_NAMES.get(status, status)}] {message}")
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_I32P = ctypes.POINTER(ctypes.c_int32)
+_DP = ctypes.POINTER(ctypes.c_double)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'`")
+    L = ctypes.CDLL(LIB_PATH)
+    L.olpt_last_error.restype = ctypes.c_char_p
+    L.olpt_ctx_create.argtypes = [_I, _P, ctypes.POINTER(_P)]
+    L.olpt_ctx_destroy.argtypes = [_P]
+    L.olpt_ctx_reserve.argtypes = [_P, _I, _I, _I, _I, _I, _I]
+    L.olpt_set_precision.argtypes = [_P, _I]
+    L.olpt_set_cameras.argtypes = [_P, ctypes.POINTER(OlptCamera), _I]
+    L.olpt_set_otf.argtypes = [_P, ctypes.POINTER(OlptOtf)]
+    L.olpt_sync.argtypes = [_P]
+    L.olpt_stream.argtypes = [_P]
+    L.olpt_stream.restype = _P
+    L.olpt_diagnostics.argtypes = [_P, _I32P]
+    L.olpt_find_peaks.argtypes = [_P, _P, _I, _I, _I, ctypes.POINTER(OlptIprParams),
+                                  _P, _I, _I32P]
+    L.olpt_stereo_match.argtypes = [_P, _I32P, _P, _I, _I32P,
+                                    ctypes.POINTER(OlptIprParams), _P, _P, _P, _I,
+                                    _P, _P, _I, _I32P]
+    L.olpt_shake.argtypes = [_P, _I32P, _P, _I, ctypes.POINTER(OlptIprParams), _P,
+                             _I, _P, _P, _P, _P]
+    L.olpt_residual.argtypes = [_P, _P, _I, ctypes.POINTER(OlptIprParams), _P, _I,
+                                _P]
+    L.olpt_ipr_run.argtypes = [_P, _P, _I, _I, ctypes.POINTER(OlptIprParams), _P,
+                               _P, _I, _P, _I32P, _I]
+    L.olpt_ipr_graph_capture.argtypes = [_P, _P, _I, _I,
+                                         ctypes.POINTER(OlptIprParams), _P, _P, _I,
+                                         _P]
+    L.olpt_ipr_graph_launch.argtypes = [_P]
+    for name in ("olpt_ctx_create", "olpt_ctx_destroy", "olpt_ctx_reserve",
+                 "olpt_set_precision", "olpt_set_cameras", "olpt_set_otf",
+                 "olpt_sync", "olpt_diagnostics", "olpt_find_peaks",
+                 "olpt_stereo_match", "olpt_shake", "olpt_residual",
+                 "olpt_ipr_run", "olpt_ipr_graph_capture",
+                 "olpt_ipr_graph_launch"):
+        getattr(L, name).restype = ctypes.c_int32
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status != OLPT_OK:
+        msg = lib().olpt_last_error().decode(errors="replace")
+        raise FatalError(status, msg)

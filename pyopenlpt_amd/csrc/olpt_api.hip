@@ -1,0 +1,945 @@
+// olpt_api.hip - C-ABI, device workspace and IPR orchestration.
+//
+// IPR::runIPR (IPR.cpp:195-306) becomes a fixed sequence of kernel launches
+// per loop: peaks -> stereo -> shake -> filter -> IPR-mode residual ->
+// append.  Every count stays in device memory, so the same sequence is what
+// olpt_ipr_graph_capture records into a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "olpt_c.h"
+#include "olpt_kernels.hpp"
+#include "olpt_scan.hpp"
+
+using namespace olpt;
+
+namespace {
+
+thread_local std::string g_err;
+
+olpt_status fail(olpt_status s, const std::string &msg) {
+  g_err = msg;
+  return s;
+}
+
+#define HIP_TRY(x)                                                             \
+  do {                                                                         \
+    hipError_t _e = (x);                                                       \
+    if (_e != hipSuccess)                                                      \
+      return fail(OLPT_E_GPU, std::string(#x ": ") + hipGetErrorString(_e));   \
+  } while (0)
+
+constexpr int kMaxRounds = 64;
+
+} // namespace
+
+struct olpt_ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  int prec = OLPT_PREC_F32;
+  int n_cam = 0, H = 0, W = 0, cap2d = 0, cap_cand = 0, cap_obj = 0;
+  int cams_set = 0, otf_set = 0;
+  olpt_camera cams[OLPT_MAX_CAM];
+  std::vector<void *> allocs;
+  // cameras / OTF on device
+  CamT<double> *d_cam64 = nullptr;
+  CamT<float> *d_cam32 = nullptr;
+  int otf_nx = 2, otf_ny = 2, otf_nz = 2;
+  double otf_bnd[6] = {0, 0, 0, 0, 0, 0};
+  double *d_otf64 = nullptr; // a | b | c | alpha, each n_cam*ng
+  float *d_otf32 = nullptr;
+  int otf_ng = 0;
+  // images (sized for FP64)
+  void *img_a = nullptr, *img_b = nullptr, *img_res = nullptr;
+  // peaks
+  uint64_t *pk_mask = nullptr;
+  int32_t *pk_cnt = nullptr, *pk_off = nullptr;
+  // stereo
+  StereoBuffers sb{};
+  double *xy = nullptr;
+  int32_t *npts = nullptr;
+  // shake
+  void *sh_uv = nullptr, *keep_uv = nullptr;
+  double *sh_xyz = nullptr, *keep_xyz = nullptr, *sh_score = nullptr;
+  uint8_t *sh_flag = nullptr;
+  int32_t *sh_n = nullptr, *n_keep = nullptr;
+  int32_t *bin_cnt = nullptr, *bin_start = nullptr, *bin_obj = nullptr, *obj_bin = nullptr,
+          *obj_rank = nullptr;
+  int bin_rows = 0, bin_cols = 0;
+  int32_t *g_cnt = nullptr, *g_start = nullptr, *g_obj = nullptr, *g_rank = nullptr,
+          *g_cellof = nullptr, *has_nb = nullptr, *sh_keep_off = nullptr;
+  int grid_n = 80;
+  void *rec = nullptr;
+  double *partial = nullptr;
+  // misc
+  int32_t *flags = nullptr;  // [16]
+  int32_t *scan_scratch = nullptr;
+  int32_t *totals = nullptr; // [64]
+  int32_t *loop_tot = nullptr; // [n_frames*cap_loop] device
+  int loop_tot_cap = 0;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool capturing = false;
+};
+
+namespace {
+
+olpt_status dalloc(olpt_ctx *c, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess)
+    return fail(OLPT_E_ALLOC, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+  c->allocs.push_back(*p);
+  return OLPT_OK;
+}
+
+template <typename P> olpt_status dalloc_t(olpt_ctx *c, P **p, size_t count) {
+  return dalloc(c, reinterpret_cast<void **>(p), count * sizeof(P));
+}
+
+void free_all(olpt_ctx *c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  for (void *p : c->allocs) (void)hipFree(p);
+  c->allocs.clear();
+}
+
+template <typename TIn, typename T>
+__global__ void k_convert(const TIn *__restrict__ src, long n, T *__restrict__ dst) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    dst[i] = (T)src[i];
+}
+
+template <typename T>
+olpt_status convert_image(const void *src, int dtype, long n, T *dst, hipStream_t st) {
+  const int threads = 256;
+  long blocks = (n + threads - 1) / threads;
+  if (blocks > 8192) blocks = 8192;
+  switch (dtype) {
+  case OLPT_U8: k_convert<uint8_t, T><<<blocks, threads, 0, st>>>((const uint8_t *)src, n, dst); break;
+  case OLPT_U16: k_convert<uint16_t, T><<<blocks, threads, 0, st>>>((const uint16_t *)src, n, dst); break;
+  case OLPT_F32: k_convert<float, T><<<blocks, threads, 0, st>>>((const float *)src, n, dst); break;
+  case OLPT_F64: k_convert<double, T><<<blocks, threads, 0, st>>>((const double *)src, n, dst); break;
+  default: return fail(OLPT_E_UNSUPPORTED, "unsupported image dtype");
+  }
+  return OLPT_OK;
+}
+
+__global__ void k_fill_i32(int32_t *p, long n, int32_t v) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+void fill_i32(int32_t *p, long n, int32_t v, hipStream_t st) {
+  long blocks = (n + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 4096) blocks = 4096;
+  k_fill_i32<<<blocks, 256, 0, st>>>(p, n, v);
+}
+
+// append kept objects of one loop to the frame output
+template <typename T>
+__global__ void k_append(const double *__restrict__ kx, const T *__restrict__ kuv,
+                         const int32_t *__restrict__ n_keep, int n_cam,
+                         double *__restrict__ ox, double *__restrict__ ouv, int cap,
+                         const int32_t *__restrict__ n_out) {
+  const int base = *n_out;
+  const int n = *n_keep;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int o = base + i;
+    if (o >= cap) break;
+    for (int d = 0; d < 3; ++d) ox[3L * o + d] = kx[3L * i + d];
+    if (ouv)
+      for (int k = 0; k < 2 * n_cam; ++k) ouv[((long)o * n_cam) * 2 + k] = (double)kuv[(long)i * n_cam * 2 + k];
+  }
+}
+
+__global__ void k_append_count(const int32_t *n_keep, int cap, int32_t *n_out,
+                               int32_t *loop_tot, int32_t *flags) {
+  if (threadIdx.x || blockIdx.x) return;
+  int t = *n_out + *n_keep;
+  if (t > cap) {
+    t = cap;
+    atomicOr(&flags[4], 1);
+  }
+  *n_out = t;
+  if (loop_tot) *loop_tot = t;
+}
+
+__global__ void k_sel_flags(const int8_t *state, const int32_t *n_cand, int cap,
+                            uint8_t *sel) {
+  const int M = min(*n_cand, cap);
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x)
+    sel[j] = state[j] == 1 ? 1 : 0;
+}
+
+__global__ void k_peak_overflow(const int32_t *pk_flag, int32_t *flags) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && pk_flag[0]) atomicOr(&flags[7], 1);
+}
+
+template <typename T> const CamT<T> *dcams(olpt_ctx *c);
+template <> const CamT<double> *dcams<double>(olpt_ctx *c) { return c->d_cam64; }
+template <> const CamT<float> *dcams<float>(olpt_ctx *c) { return c->d_cam32; }
+
+template <typename T> OtfT<T> dotf(olpt_ctx *c) {
+  OtfT<T> o;
+  o.nx = c->otf_nx;
+  o.ny = c->otf_ny;
+  o.nz = c->otf_nz;
+  o.ng = c->otf_ng;
+  for (int i = 0; i < 6; ++i) o.bnd[i] = (T)c->otf_bnd[i];
+  const long sz = (long)c->n_cam * c->otf_ng;
+  const T *base;
+  if constexpr (sizeof(T) == 8)
+    base = (const T *)c->d_otf64;
+  else
+    base = (const T *)c->d_otf32;
+  o.a = base;
+  o.b = base + sz;
+  o.c = base + 2 * sz;
+  o.alpha = base + 3 * sz;
+  return o;
+}
+
+template <typename T> ShakeBuffers<T> shake_buffers(olpt_ctx *c, T *img, T *img_next) {
+  ShakeBuffers<T> b{};
+  b.n_cam = c->n_cam;
+  b.H = c->H;
+  b.W = c->W;
+  b.cams = dcams<T>(c);
+  b.otf = dotf<T>(c);
+  b.img = img;
+  b.res = (T *)c->img_res;
+  b.img_next = img_next;
+  b.cap_obj = c->cap_obj;
+  b.xyz = c->sh_xyz;
+  b.uv = (T *)c->sh_uv;
+  b.score = c->sh_score;
+  b.flag = c->sh_flag;
+  b.n_obj = c->sh_n;
+  b.bin_px = 8;
+  b.bin_rows = c->bin_rows;
+  b.bin_cols = c->bin_cols;
+  b.bin_cnt = c->bin_cnt;
+  b.bin_start = c->bin_start;
+  b.bin_obj = c->bin_obj;
+  b.obj_bin = c->obj_bin;
+  b.obj_rank = c->obj_rank;
+  b.grid_n = c->grid_n;
+  b.g_cnt = c->g_cnt;
+  b.g_start = c->g_start;
+  b.g_obj = c->g_obj;
+  b.g_rank = c->g_rank;
+  b.g_cellof = c->g_cellof;
+  b.has_nb = c->has_nb;
+  b.rec = c->rec;
+  b.partial = c->partial;
+  b.keep_off = c->sh_keep_off;
+  b.n_keep = c->n_keep;
+  b.scan_scratch = c->scan_scratch;
+  b.totals = c->totals;
+  b.work = c->sb.work;
+  return b;
+}
+
+void set_grid(olpt_ctx *c, ShakeBuffers<float> *bf, ShakeBuffers<double> *bd,
+              const olpt_ipr_params *p) {
+  // repeated-object grid over the view volume (+ margin); cells >= tol_3d
+  double lo[3], ext = 0;
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = p->limit[2 * d] - 1.0;
+    ext = std::max(ext, p->limit[2 * d + 1] - p->limit[2 * d] + 2.0);
+  }
+  double cell = std::max(p->tol_3d_mm, ext / c->grid_n);
+  if (!(cell > 0)) cell = 1.0;
+  auto fill = [&](auto *b) {
+    if (!b) return;
+    for (int d = 0; d < 3; ++d) b->grid_lo[d] = lo[d];
+    b->grid_cell = cell;
+  };
+  fill(bf);
+  fill(bd);
+}
+
+StereoParams stereo_params(olpt_ctx *c, const int32_t *active, const olpt_ipr_params *p,
+                           double tol2d, olpt_status *st) {
+  StereoParams sp{};
+  sp.n_active = 0;
+  for (int k = 0; k < OLPT_MAX_CAM; ++k) sp.active[k] = 0;
+  for (int k = 0; k < c->n_cam; ++k) {
+    sp.active[k] = active ? (active[k] != 0) : 1;
+    sp.n_active += sp.active[k];
+  }
+  sp.tol2d = tol2d;
+  sp.tol3d = p->tol_3d_mm;
+  for (int i = 0; i < 6; ++i) sp.limit[i] = p->limit[i];
+  *st = OLPT_OK;
+  // StereoMatch ctor pre-checks (StereoMatch.cpp:27-47)
+  if (sp.n_active < 2) {
+    *st = fail(OLPT_E_INVALID_ARGUMENT, "preCheck: need at least 2 active cameras.");
+    return sp;
+  }
+  if (p->match_cam_count < 2 || p->match_cam_count > sp.n_active) {
+    *st = fail(OLPT_E_INVALID_ARGUMENT, "preCheck: match_cam_count must be within [2," +
+                                            std::to_string(sp.n_active) + "].");
+    return sp;
+  }
+  const int bc = std::min(std::max(2, p->match_cam_count), sp.n_active);
+  sp.build_count = bc;
+  if (bc != sp.n_active) {
+    // with check cameras the reference's checkMatch leaves e_check NaN
+    // (StereoMatch.cpp:522, :603-604) and match() aborts (:126-127)
+    *st = fail(OLPT_E_UNSUPPORTED,
+               "match_cam_count below the number of active cameras is undefined in the reference "
+               "(checkMatch leaves e_check NaN and match() aborts)");
+  }
+  return sp;
+}
+
+ShakeParams shake_params(olpt_ctx *c, const int32_t *active, const olpt_ipr_params *p) {
+  ShakeParams sp{};
+  for (int k = 0; k < OLPT_MAX_CAM; ++k) sp.active[k] = 0;
+  for (int k = 0; k < c->n_cam; ++k) sp.active[k] = active ? (active[k] != 0) : 1;
+  sp.shake_width = p->shake_width;
+  sp.shakewidth_min = p->shakewidth_min;
+  sp.thred_ghost = p->thred_ghost;
+  sp.tol3d = p->tol_3d_mm;
+  sp.r_px = p->radius_obj;
+  sp.n_shake_loop = p->n_shake_loop;
+  sp.ratio_augimg = p->ratio_augimg;
+  return sp;
+}
+
+olpt_status check_ready(olpt_ctx *c) {
+  if (!c) return fail(OLPT_E_INVALID_ARGUMENT, "null context");
+  if (!c->cams_set) return fail(OLPT_E_INVALID_ARGUMENT, "cameras not set");
+  if (!c->xy) return fail(OLPT_E_INVALID_ARGUMENT, "workspace not reserved");
+  return OLPT_OK;
+}
+
+olpt_status check_params(olpt_ctx *c, const olpt_ipr_params *p) {
+  if (!p) return fail(OLPT_E_INVALID_ARGUMENT, "null params");
+  if (p->radius_obj <= 0 || p->ratio_augimg <= 0)
+    return fail(OLPT_E_INVALID_ARGUMENT, "radius_obj and ratio_augimg must be positive");
+  const int box = (int)std::lround(2.0 * p->radius_obj);
+  const int roi = (int)std::lround(2.0 * p->radius_obj * p->ratio_augimg);
+  if (roi * roi > 4 * 64 || box > 24)
+    return fail(OLPT_E_UNSUPPORTED, "ROI larger than 16x16 pixels is not supported");
+  if (p->idmap_cell_px < 1) return fail(OLPT_E_INVALID_ARGUMENT, "idmap_cell_px must be >= 1");
+  if (p->idmap_cell_px != c->sb.cell_px)
+    return fail(OLPT_E_UNSUPPORTED, "idmap_cell_px differs from the reserved workspace (4)");
+  return OLPT_OK;
+}
+
+// One IPR loop on the active subset: img (loop input, T) -> img_next.
+template <typename T>
+olpt_status ipr_loop(olpt_ctx *c, T *img, T *img_next, const int32_t *active,
+                     const olpt_ipr_params *p, double tol2d, double *out_xyz,
+                     double *out_uv, int cap_out, int32_t *d_n_out, int32_t *d_loop_tot) {
+  hipStream_t st = c->st;
+  olpt_status s;
+  StereoParams sp = stereo_params(c, active, p, tol2d, &s);
+  if (s) return s;
+  CamList cl{0, {0}};
+  for (int k = 0; k < c->n_cam; ++k)
+    if (sp.active[k]) cl.id[cl.n++] = k;
+  const long HW = (long)c->H * c->W;
+  // 1) ObjectFinder2D on the active cameras (inactive slots stay empty)
+  fill_i32(c->npts, c->n_cam, 0, st);
+  launch_find_peaks<T>(img, HW, c->H, c->W, (double)p->min_obj_int, cl, c->pk_mask, c->pk_cnt,
+                       c->pk_off, c->scan_scratch, c->totals, c->xy, c->cap2d, c->npts,
+                       c->flags + 8, st);
+  k_peak_overflow<<<1, 1, 0, st>>>(c->flags + 8, c->flags);
+  // 2) StereoMatch
+  launch_stereo(c->sb, sp, kMaxRounds, st);
+  // 3) Shake the matched objects in place
+  ShakeBuffers<T> b = shake_buffers<T>(c, img, img_next);
+  b.xyz = c->sb.obj_xyz;
+  b.n_obj = c->sb.n_obj;
+  if constexpr (sizeof(T) == 8)
+    set_grid(c, nullptr, &b, p);
+  else
+    set_grid(c, &b, nullptr, p);
+  ShakeParams shp = shake_params(c, active, p);
+  launch_shake<T>(b, shp, st);
+  // 4) filterOutInvalid + calResidualImage (IPR mode)
+  launch_filter<T>(b, c->keep_xyz, (T *)c->keep_uv, c->n_keep, st);
+  launch_residual_ipr<T>(b, c->n_keep, c->keep_xyz, (T *)c->keep_uv, p->radius_obj, st);
+  // 5) append
+  if (out_xyz) {
+    k_append<T><<<256, 256, 0, st>>>(c->keep_xyz, (const T *)c->keep_uv, c->n_keep, c->n_cam,
+                                     out_xyz, out_uv, cap_out, d_n_out);
+    k_append_count<<<1, 1, 0, st>>>(c->n_keep, cap_out, d_n_out, d_loop_tot, c->flags);
+  }
+  HIP_TRY(hipGetLastError());
+  return OLPT_OK;
+}
+
+// IPR::runIPR on one frame already converted into img_a (T)
+template <typename T>
+olpt_status ipr_frame(olpt_ctx *c, const olpt_ipr_params *p, double *out_xyz, double *out_uv,
+                      int cap_out, int32_t *d_n_out, int32_t *d_loop_tot, int cap_loop,
+                      int *n_loop_out) {
+  const int n_cam = c->n_cam;
+  fill_i32(d_n_out, 1, 0, c->st);
+  int n_loop = 0;
+  if (n_cam < 2) { // IPR.cpp:200-204
+    *n_loop_out = 0;
+    return OLPT_OK;
+  }
+  T *img = (T *)c->img_a, *nxt = (T *)c->img_b;
+  int max_drop = std::min(p->n_cam_reduced, std::max(0, n_cam - 2));
+  olpt_ipr_params q = *p;
+  const int orig_mcc = p->match_cam_count;
+  for (int drop = 0; drop <= max_drop; ++drop) {
+    const int K = n_cam - drop;
+    const int loops = drop == 0 ? p->n_loop_ipr : p->n_loop_ipr_reduced;
+    q.match_cam_count = std::min(orig_mcc, K);
+    // lexicographic K-subsets (myMATH::generateCombinations, myMATH.cpp:39-71)
+    std::vector<int> comb(K);
+    for (int i = 0; i < K; ++i) comb[i] = i;
+    while (true) {
+      int32_t act[OLPT_MAX_CAM] = {0};
+      for (int i = 0; i < K; ++i) act[comb[i]] = 1;
+      double tol = q.tol_2d_px;
+      for (int loop = 0; loop < loops; ++loop) {
+        int32_t *lt = (d_loop_tot && n_loop < cap_loop) ? d_loop_tot + n_loop : nullptr;
+        olpt_status s = ipr_loop<T>(c, img, nxt, act, &q, tol, out_xyz, out_uv, cap_out,
+                                    d_n_out, lt);
+        if (s) return s;
+        std::swap(img, nxt);
+        ++n_loop;
+        tol = tol * 1.5; // IPR.cpp:291-292
+      }
+      int i = K - 1;
+      while (i >= 0 && comb[i] == n_cam - K + i) --i;
+      if (i < 0) break;
+      comb[i]++;
+      for (int j = i + 1; j < K; ++j) comb[j] = comb[j - 1] + 1;
+    }
+  }
+  *n_loop_out = n_loop;
+  return OLPT_OK;
+}
+
+size_t tsize(olpt_ctx *c) { return c->prec == OLPT_PREC_F64 ? 8 : 4; }
+
+} // namespace
+
+// ===========================================================================
+// extern "C"
+// ===========================================================================
+extern "C" {
+
+const char *olpt_last_error(void) { return g_err.c_str(); }
+
+olpt_status olpt_ctx_create(int device, void *stream, olpt_ctx **out) {
+  if (!out) return fail(OLPT_E_INVALID_ARGUMENT, "null out");
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(OLPT_E_INVALID_ARGUMENT, "bad device index");
+  HIP_TRY(hipSetDevice(device));
+  olpt_ctx *c = new olpt_ctx();
+  c->device = device;
+  c->st = (hipStream_t)stream;
+  *out = c;
+  return OLPT_OK;
+}
+
+olpt_status olpt_ctx_destroy(olpt_ctx *c) {
+  if (!c) return OLPT_OK;
+  (void)hipSetDevice(c->device);
+  if (c->st)
+    (void)hipStreamSynchronize(c->st);
+  else
+    (void)hipDeviceSynchronize();
+  free_all(c);
+  delete c;
+  return OLPT_OK;
+}
+
+void *olpt_stream(olpt_ctx *c) { return c ? (void *)c->st : nullptr; }
+
+olpt_status olpt_sync(olpt_ctx *c) {
+  if (!c) return fail(OLPT_E_INVALID_ARGUMENT, "null context");
+  HIP_TRY(hipStreamSynchronize(c->st));
+  return OLPT_OK;
+}
+
+olpt_status olpt_set_precision(olpt_ctx *c, int prec) {
+  if (!c || (prec != OLPT_PREC_F32 && prec != OLPT_PREC_F64))
+    return fail(OLPT_E_INVALID_ARGUMENT, "precision must be OLPT_PREC_F32 or OLPT_PREC_F64");
+  c->prec = prec;
+  return OLPT_OK;
+}
+
+olpt_status olpt_ctx_reserve(olpt_ctx *c, int n_cam, int H, int W, int max_2d, int max_cand,
+                             int max_obj) {
+  if (!c) return fail(OLPT_E_INVALID_ARGUMENT, "null context");
+  if (n_cam < 1 || n_cam > OLPT_MAX_CAM || H < 3 || W < 3)
+    return fail(OLPT_E_INVALID_ARGUMENT, "bad camera count or image size");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipDeviceSynchronize());
+  free_all(c);
+  c->n_cam = n_cam;
+  c->H = H;
+  c->W = W;
+  c->cap2d = max_2d > 0 ? max_2d : std::min(1 << 17, std::max(1024, H * W / 8));
+  c->cap_cand = max_cand > 0 ? max_cand : (1 << 21);
+  c->cap_obj = max_obj > 0 ? max_obj : (1 << 18);
+  const long HW = (long)H * W;
+  const int nc = n_cam, cap = c->cap2d, M = c->cap_cand, O = c->cap_obj;
+  olpt_status s;
+#define A(ptr, T, n) if ((s = dalloc_t(c, &(ptr), (size_t)(n)))) return s
+  A(c->d_cam64, CamT<double>, OLPT_MAX_CAM);
+  A(c->d_cam32, CamT<float>, OLPT_MAX_CAM);
+  void *tmp;
+  if ((s = dalloc(c, &tmp, 8 * (size_t)nc * HW))) return s;
+  c->img_a = tmp;
+  if ((s = dalloc(c, &tmp, 8 * (size_t)nc * HW))) return s;
+  c->img_b = tmp;
+  if ((s = dalloc(c, &tmp, 8 * (size_t)nc * HW))) return s;
+  c->img_res = tmp;
+  const long nchunk = peak_chunks(H, W);
+  A(c->pk_mask, uint64_t, nchunk * nc);
+  A(c->pk_cnt, int32_t, nchunk * nc);
+  A(c->pk_off, int32_t, nchunk * nc);
+  A(c->xy, double, (long)nc * cap * 2);
+  A(c->npts, int32_t, OLPT_MAX_CAM);
+  StereoBuffers &b = c->sb;
+  b = StereoBuffers{};
+  b.n_cam = nc;
+  b.cap = cap;
+  b.cams = c->d_cam64;
+  b.xy = c->xy;
+  b.npts = c->npts;
+  A(b.los, double, (long)nc * cap * 4);
+  b.cell_px = 4;
+  b.rows_cell = std::max(1, H / 4 + (H % 4 ? 1 : 0));
+  b.cols_cell = std::max(1, W / 4 + (W % 4 ? 1 : 0));
+  const long ncell = (long)b.rows_cell * b.cols_cell;
+  A(b.cell_cnt, int32_t, ncell * nc);
+  A(b.cell_start, int32_t, ncell * nc);
+  A(b.cell_pid, int32_t, (long)nc * cap);
+  A(b.pt_cell, int32_t, (long)nc * cap);
+  A(b.pt_rank, int32_t, (long)nc * cap);
+  b.cap_cand = M;
+  A(b.cand_ids, int32_t, (long)M * nc);
+  A(b.cand_e, double, M);
+  A(b.n_cand, int32_t, 1);
+  A(b.cand_ids_s, int32_t, (long)M * nc);
+  A(b.cand_e_s, double, M);
+  A(b.key_cnt, int32_t, cap);
+  A(b.key_start, int32_t, cap);
+  A(b.cand_rank, int32_t, M);
+  A(b.pt_count, int32_t, (long)nc * cap);
+  A(b.pt_start, int32_t, (long)nc * cap);
+  A(b.pt_list, int32_t, (long)M * nc);
+  A(b.flat_rank, int32_t, (long)M * nc);
+  A(b.pv, double, (long)M * nc);
+  A(b.pct, double, M);
+  A(b.state, int8_t, M);
+  A(b.undecided, int32_t, kMaxRounds);
+  A(b.keep, int32_t, M);
+  A(b.keep_off, int32_t, M);
+  A(b.tri_xyz, double, (long)M * 3);
+  b.cap_obj = O;
+  A(b.obj_xyz, double, (long)O * 3);
+  A(b.obj_ids, int32_t, (long)O * nc);
+  A(b.n_obj, int32_t, 1);
+  A(b.n_sel, int32_t, 1);
+  A(c->flags, int32_t, 16);
+  b.flags = c->flags;
+  A(b.work, int32_t, 16);
+  const long grid_cells = (long)c->grid_n * c->grid_n * c->grid_n;
+  long scan_n = std::max<long>(std::max<long>(M, (long)nc * cap), std::max<long>(ncell, grid_cells));
+  scan_n = std::max<long>(scan_n, nchunk);
+  A(c->scan_scratch, int32_t, scan_scratch_ints(scan_n, nc) + 64);
+  b.scan_scratch = c->scan_scratch;
+  A(c->totals, int32_t, 64);
+  b.totals = c->totals;
+  // shake
+  if ((s = dalloc(c, &c->sh_uv, 8 * (size_t)O * nc * 2))) return s;
+  if ((s = dalloc(c, &c->keep_uv, 8 * (size_t)O * nc * 2))) return s;
+  A(c->sh_xyz, double, (long)O * 3);
+  A(c->keep_xyz, double, (long)O * 3);
+  A(c->sh_score, double, O);
+  A(c->sh_flag, uint8_t, O);
+  A(c->sh_n, int32_t, 1);
+  A(c->n_keep, int32_t, 1);
+  c->bin_rows = (H + 7) / 8;
+  c->bin_cols = (W + 7) / 8;
+  const long nbin = (long)c->bin_rows * c->bin_cols;
+  A(c->bin_cnt, int32_t, nbin * nc);
+  A(c->bin_start, int32_t, nbin * nc);
+  A(c->bin_obj, int32_t, (long)O * nc);
+  A(c->obj_bin, int32_t, (long)O * nc);
+  A(c->obj_rank, int32_t, (long)O * nc);
+  A(c->g_cnt, int32_t, grid_cells);
+  A(c->g_start, int32_t, grid_cells);
+  A(c->g_obj, int32_t, O);
+  A(c->g_rank, int32_t, O);
+  A(c->g_cellof, int32_t, O);
+  A(c->has_nb, int32_t, O);
+  A(c->sh_keep_off, int32_t, O);
+  if ((s = dalloc(c, &c->rec, shake_rec_bytes(true) * (size_t)O * nc))) return s;
+  A(c->partial, double, 1024);
+#undef A
+  // re-upload cameras / OTF if they were set before
+  if (c->cams_set) {
+    CamT<double> h64[OLPT_MAX_CAM];
+    CamT<float> h32[OLPT_MAX_CAM];
+    for (int k = 0; k < c->n_cam; ++k) {
+      h64[k] = make_cam<double>(c->cams[k]);
+      h32[k] = make_cam<float>(c->cams[k]);
+    }
+    HIP_TRY(hipMemcpy(c->d_cam64, h64, sizeof(h64), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_cam32, h32, sizeof(h32), hipMemcpyHostToDevice));
+  }
+  c->otf_set = 0;
+  return OLPT_OK;
+}
+
+olpt_status olpt_set_cameras(olpt_ctx *c, const olpt_camera *cams, int n_cam) {
+  if (!c || !cams) return fail(OLPT_E_INVALID_ARGUMENT, "null argument");
+  if (!c->d_cam64) return fail(OLPT_E_INVALID_ARGUMENT, "workspace not reserved");
+  if (n_cam != c->n_cam) return fail(OLPT_E_INVALID_ARGUMENT, "camera count differs from reserve");
+  CamT<double> h64[OLPT_MAX_CAM];
+  CamT<float> h32[OLPT_MAX_CAM];
+  for (int k = 0; k < n_cam; ++k) {
+    const olpt_camera &q = cams[k];
+    if (q.n_row != c->H || q.n_col != c->W)
+      return fail(OLPT_E_INVALID_ARGUMENT, "camera image size differs from reserve");
+    if (q.is_distorted && q.n_dist != 4 && q.n_dist != 5 && q.n_dist != 8 && q.n_dist != 12)
+      return fail(OLPT_E_UNSUPPORTED, "number of distortion coefficients must be 4, 5, 8 or 12");
+    c->cams[k] = q;
+    h64[k] = make_cam<double>(q);
+    h32[k] = make_cam<float>(q);
+  }
+  HIP_TRY(hipMemcpy(c->d_cam64, h64, sizeof(CamT<double>) * n_cam, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_cam32, h32, sizeof(CamT<float>) * n_cam, hipMemcpyHostToDevice));
+  c->cams_set = 1;
+  return OLPT_OK;
+}
+
+olpt_status olpt_set_otf(olpt_ctx *c, const olpt_otf *o) {
+  if (!c || !o) return fail(OLPT_E_INVALID_ARGUMENT, "null argument");
+  if (o->n_cam != c->n_cam) return fail(OLPT_E_INVALID_ARGUMENT, "OTF camera count differs");
+  if (o->nx < 2 || o->ny < 2 || o->nz < 2)
+    return fail(OLPT_E_INVALID_ARGUMENT, "OTF grid number should be larger than 1");
+  const int ng = o->nx * o->ny * o->nz;
+  const long sz = (long)o->n_cam * ng;
+  std::vector<double> h(4 * sz);
+  std::vector<float> hf(4 * sz);
+  const double *src[4] = {o->a, o->b, o->c, o->alpha};
+  for (int q = 0; q < 4; ++q)
+    for (long i = 0; i < sz; ++i) {
+      h[q * sz + i] = src[q][i];
+      hf[q * sz + i] = (float)src[q][i];
+    }
+  if (c->otf_ng != ng || !c->d_otf64) {
+    olpt_status s;
+    if ((s = dalloc_t(c, &c->d_otf64, 4 * sz))) return s;
+    if ((s = dalloc_t(c, &c->d_otf32, 4 * sz))) return s;
+  }
+  HIP_TRY(hipMemcpy(c->d_otf64, h.data(), sizeof(double) * 4 * sz, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_otf32, hf.data(), sizeof(float) * 4 * sz, hipMemcpyHostToDevice));
+  c->otf_nx = o->nx;
+  c->otf_ny = o->ny;
+  c->otf_nz = o->nz;
+  c->otf_ng = ng;
+  for (int i = 0; i < 6; ++i) c->otf_bnd[i] = o->boundary[i];
+  c->otf_set = 1;
+  return OLPT_OK;
+}
+
+olpt_status olpt_diagnostics(olpt_ctx *c, int32_t out[8]) {
+  if (!c || !c->flags) return fail(OLPT_E_INVALID_ARGUMENT, "no workspace");
+  HIP_TRY(hipStreamSynchronize(c->st));
+  HIP_TRY(hipMemcpy(out, c->flags, 8 * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return OLPT_OK;
+}
+
+olpt_status olpt_find_peaks(olpt_ctx *c, const void *d_img, int dtype, int H, int W,
+                            const olpt_ipr_params *p, double *d_xy, int cap, int32_t *n_out) {
+  if (!c || !c->flags) return fail(OLPT_E_INVALID_ARGUMENT, "workspace not reserved");
+  if (!d_img || !d_xy || !p || !n_out) return fail(OLPT_E_INVALID_ARGUMENT, "null argument");
+  if (H != c->H || W != c->W) return fail(OLPT_E_INVALID_ARGUMENT, "image size differs from reserve");
+  CamList cl{1, {0}};
+  hipStream_t st = c->st;
+  fill_i32(c->flags + 8, 8, 0, st);
+  fill_i32(c->npts, OLPT_MAX_CAM, 0, st);
+  const double mi = (double)p->min_obj_int;
+  switch (dtype) {
+  case OLPT_U8: launch_find_peaks<uint8_t>((const uint8_t *)d_img, 0, H, W, mi, cl, c->pk_mask, c->pk_cnt, c->pk_off, c->scan_scratch, c->totals, d_xy, cap, c->npts, c->flags + 8, st); break;
+  case OLPT_U16: launch_find_peaks<uint16_t>((const uint16_t *)d_img, 0, H, W, mi, cl, c->pk_mask, c->pk_cnt, c->pk_off, c->scan_scratch, c->totals, d_xy, cap, c->npts, c->flags + 8, st); break;
+  case OLPT_F32: launch_find_peaks<float>((const float *)d_img, 0, H, W, mi, cl, c->pk_mask, c->pk_cnt, c->pk_off, c->scan_scratch, c->totals, d_xy, cap, c->npts, c->flags + 8, st); break;
+  case OLPT_F64: launch_find_peaks<double>((const double *)d_img, 0, H, W, mi, cl, c->pk_mask, c->pk_cnt, c->pk_off, c->scan_scratch, c->totals, d_xy, cap, c->npts, c->flags + 8, st); break;
+  default: return fail(OLPT_E_UNSUPPORTED, "unsupported image dtype");
+  }
+  HIP_TRY(hipGetLastError());
+  int32_t tot = 0;
+  HIP_TRY(hipMemcpyAsync(&tot, c->totals, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  *n_out = tot;
+  if (tot > cap) return fail(OLPT_E_CAPACITY, "more peaks than the output capacity");
+  return OLPT_OK;
+}
+
+olpt_status olpt_stereo_match(olpt_ctx *c, const int32_t *active, const double *d_xy, int cap_2d,
+                              const int32_t *n_pts, const olpt_ipr_params *p, int32_t *d_cand_ids,
+                              double *d_cand_e, uint8_t *d_sel, int cap_cand, double *d_obj_xyz,
+                              int32_t *d_obj_ids, int cap_obj, int32_t *counts) {
+  olpt_status s = check_ready(c);
+  if (s) return s;
+  if (!d_xy || !n_pts || !p || !counts) return fail(OLPT_E_INVALID_ARGUMENT, "null argument");
+  hipStream_t st = c->st;
+  for (int k = 0; k < c->n_cam; ++k)
+    if (n_pts[k] < 0 || n_pts[k] > c->cap2d || n_pts[k] > cap_2d)
+      return fail(OLPT_E_CAPACITY, "2D point count exceeds the workspace capacity");
+  HIP_TRY(hipMemcpy2DAsync(c->xy, sizeof(double) * 2 * c->cap2d, d_xy, sizeof(double) * 2 * cap_2d,
+                           sizeof(double) * 2 * std::min(cap_2d, c->cap2d), c->n_cam,
+                           hipMemcpyDeviceToDevice, st));
+  int32_t np[OLPT_MAX_CAM] = {0};
+  for (int k = 0; k < c->n_cam; ++k) np[k] = (active && !active[k]) ? 0 : n_pts[k];
+  HIP_TRY(hipMemcpyAsync(c->npts, np, sizeof(np), hipMemcpyHostToDevice, st));
+  StereoParams sp = stereo_params(c, active, p, p->tol_2d_px, &s);
+  if (s) return s;
+  fill_i32(c->flags, 8, 0, st);
+  launch_stereo(c->sb, sp, kMaxRounds, st);
+  HIP_TRY(hipGetLastError());
+  int32_t h[3];
+  HIP_TRY(hipMemcpyAsync(&h[0], c->sb.n_cand, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(&h[1], c->sb.n_sel, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(&h[2], c->sb.n_obj, 4, hipMemcpyDeviceToHost, st));
+  int32_t fl[8];
+  HIP_TRY(hipMemcpyAsync(fl, c->flags, sizeof(fl), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  counts[0] = h[0];
+  counts[1] = h[1];
+  counts[2] = h[2];
+  if (fl[0] || h[0] > c->cap_cand) return fail(OLPT_E_CAPACITY, "stereo candidate capacity exceeded");
+  if (fl[1]) return fail(OLPT_E_CAPACITY, "stereo LDS candidate list overflow");
+  if (fl[4]) return fail(OLPT_E_CAPACITY, "stereo object capacity exceeded");
+  if (fl[5]) return fail(OLPT_E_CAPACITY, "prune rounds exhausted");
+  const int M = h[0];
+  if (d_cand_ids && d_cand_e) {
+    if (M > cap_cand) return fail(OLPT_E_CAPACITY, "candidate output capacity too small");
+    HIP_TRY(hipMemcpyAsync(d_cand_ids, c->sb.cand_ids_s, sizeof(int32_t) * (size_t)M * c->n_cam,
+                           hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_cand_e, c->sb.cand_e_s, sizeof(double) * (size_t)M,
+                           hipMemcpyDeviceToDevice, st));
+  }
+  if (d_sel) {
+    if (M > cap_cand) return fail(OLPT_E_CAPACITY, "candidate output capacity too small");
+    k_sel_flags<<<256, 256, 0, st>>>(c->sb.state, c->sb.n_cand, c->cap_cand, d_sel);
+  }
+  if (d_obj_xyz) {
+    if (h[2] > cap_obj) return fail(OLPT_E_CAPACITY, "object output capacity too small");
+    HIP_TRY(hipMemcpyAsync(d_obj_xyz, c->sb.obj_xyz, sizeof(double) * 3 * (size_t)h[2],
+                           hipMemcpyDeviceToDevice, st));
+    if (d_obj_ids)
+      HIP_TRY(hipMemcpyAsync(d_obj_ids, c->sb.obj_ids, sizeof(int32_t) * (size_t)h[2] * c->n_cam,
+                             hipMemcpyDeviceToDevice, st));
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  return OLPT_OK;
+}
+
+} // extern "C"
+
+namespace {
+template <typename T>
+olpt_status shake_impl(olpt_ctx *c, const int32_t *active, const void *d_imgs, int dtype,
+                       const olpt_ipr_params *p, const double *d_xyz_in, int n_obj,
+                       double *d_xyz_out, uint8_t *d_flags, double *d_score, void *d_res) {
+  hipStream_t st = c->st;
+  const long n = (long)c->n_cam * c->H * c->W;
+  olpt_status s = convert_image<T>(d_imgs, dtype, n, (T *)c->img_a, st);
+  if (s) return s;
+  HIP_TRY(hipMemcpyAsync(c->sh_xyz, d_xyz_in, sizeof(double) * 3 * (size_t)n_obj,
+                         hipMemcpyDeviceToDevice, st));
+  int32_t nn = n_obj;
+  HIP_TRY(hipMemcpyAsync(c->sh_n, &nn, 4, hipMemcpyHostToDevice, st));
+  ShakeBuffers<T> b = shake_buffers<T>(c, (T *)c->img_a, (T *)c->img_b);
+  if constexpr (sizeof(T) == 8)
+    set_grid(c, nullptr, &b, p);
+  else
+    set_grid(c, &b, nullptr, p);
+  ShakeParams shp = shake_params(c, active, p);
+  launch_shake<T>(b, shp, st);
+  if (d_res) {
+    launch_filter<T>(b, c->keep_xyz, (T *)c->keep_uv, c->n_keep, st);
+    launch_residual_ipr<T>(b, c->n_keep, c->keep_xyz, (T *)c->keep_uv, p->radius_obj, st);
+    HIP_TRY(hipMemcpyAsync(d_res, c->img_b, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+  }
+  HIP_TRY(hipGetLastError());
+  if (d_xyz_out)
+    HIP_TRY(hipMemcpyAsync(d_xyz_out, c->sh_xyz, sizeof(double) * 3 * (size_t)n_obj,
+                           hipMemcpyDeviceToDevice, st));
+  if (d_flags)
+    HIP_TRY(hipMemcpyAsync(d_flags, c->sh_flag, (size_t)n_obj, hipMemcpyDeviceToDevice, st));
+  if (d_score)
+    HIP_TRY(hipMemcpyAsync(d_score, c->sh_score, sizeof(double) * (size_t)n_obj,
+                           hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return OLPT_OK;
+}
+
+template <typename T>
+olpt_status residual_impl(olpt_ctx *c, const void *d_imgs, int dtype, const olpt_ipr_params *p,
+                          const double *d_xyz, int n_obj, void *d_res) {
+  hipStream_t st = c->st;
+  const long n = (long)c->n_cam * c->H * c->W;
+  olpt_status s = convert_image<T>(d_imgs, dtype, n, (T *)c->img_a, st);
+  if (s) return s;
+  HIP_TRY(hipMemcpyAsync(c->keep_xyz, d_xyz, sizeof(double) * 3 * (size_t)n_obj,
+                         hipMemcpyDeviceToDevice, st));
+  int32_t nn = n_obj;
+  HIP_TRY(hipMemcpyAsync(c->n_keep, &nn, 4, hipMemcpyHostToDevice, st));
+  ShakeBuffers<T> b = shake_buffers<T>(c, (T *)c->img_a, (T *)c->img_b);
+  launch_residual_ipr<T>(b, c->n_keep, c->keep_xyz, (T *)c->keep_uv, p->radius_obj, st);
+  HIP_TRY(hipMemcpyAsync(d_res, c->img_b, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return OLPT_OK;
+}
+
+template <typename T>
+olpt_status ipr_run_impl(olpt_ctx *c, const void *d_imgs, int dtype, int n_frames,
+                         const olpt_ipr_params *p, double *d_obj_xyz, double *d_obj_uv, int cap,
+                         int32_t *d_n_obj, int32_t *loop_total, int cap_loop, bool sync) {
+  hipStream_t st = c->st;
+  const long n = (long)c->n_cam * c->H * c->W;
+  const size_t in_bytes = dtype == OLPT_U8 ? 1 : dtype == OLPT_U16 ? 2 : dtype == OLPT_F32 ? 4 : 8;
+  if (loop_total) {
+    const int need = std::max(1, n_frames * cap_loop);
+    if (c->loop_tot_cap < need) {
+      olpt_status s = dalloc_t(c, &c->loop_tot, need);
+      if (s) return s;
+      c->loop_tot_cap = need;
+    }
+  }
+  fill_i32(c->flags, 8, 0, st);
+  int n_loop = 0;
+  for (int f = 0; f < n_frames; ++f) {
+    const char *src = (const char *)d_imgs + (size_t)f * n * in_bytes;
+    olpt_status s = convert_image<T>(src, dtype, n, (T *)c->img_a, st);
+    if (s) return s;
+    s = ipr_frame<T>(c, p, d_obj_xyz + (size_t)f * cap * 3,
+                     d_obj_uv ? d_obj_uv + (size_t)f * cap * c->n_cam * 2 : nullptr, cap,
+                     d_n_obj + f, loop_total ? c->loop_tot + (size_t)f * cap_loop : nullptr,
+                     cap_loop, &n_loop);
+    if (s) return s;
+  }
+  if (!sync) return OLPT_OK;
+  int32_t fl[8];
+  HIP_TRY(hipMemcpyAsync(fl, c->flags, sizeof(fl), hipMemcpyDeviceToHost, st));
+  if (loop_total) {
+    HIP_TRY(hipMemcpyAsync(loop_total, c->loop_tot, sizeof(int32_t) * (size_t)n_frames * cap_loop,
+                           hipMemcpyDeviceToHost, st));
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  if (fl[7]) return fail(OLPT_E_CAPACITY, "2D peak capacity exceeded");
+  if (fl[0]) return fail(OLPT_E_CAPACITY, "stereo candidate capacity exceeded");
+  if (fl[1]) return fail(OLPT_E_CAPACITY, "stereo LDS candidate list overflow");
+  if (fl[4]) return fail(OLPT_E_CAPACITY, "object capacity exceeded");
+  if (fl[5]) return fail(OLPT_E_CAPACITY, "prune rounds exhausted");
+  (void)n_loop;
+  return OLPT_OK;
+}
+} // namespace
+
+extern "C" {
+
+olpt_status olpt_shake(olpt_ctx *c, const int32_t *active, const void *d_imgs, int dtype,
+                       const olpt_ipr_params *p, const double *d_xyz_in, int n_obj,
+                       double *d_xyz_out, uint8_t *d_flags, double *d_score, void *d_res) {
+  olpt_status s = check_ready(c);
+  if (s) return s;
+  if (!p || !d_imgs || (n_obj > 0 && !d_xyz_in)) return fail(OLPT_E_INVALID_ARGUMENT, "null argument");
+  if ((s = check_params(c, p))) return s;
+  if (!c->otf_set) return fail(OLPT_E_INVALID_ARGUMENT, "OTF not set");
+  if (n_obj < 0 || n_obj > c->cap_obj) return fail(OLPT_E_CAPACITY, "object count exceeds capacity");
+  if (c->prec == OLPT_PREC_F64)
+    return shake_impl<double>(c, active, d_imgs, dtype, p, d_xyz_in, n_obj, d_xyz_out, d_flags, d_score, d_res);
+  return shake_impl<float>(c, active, d_imgs, dtype, p, d_xyz_in, n_obj, d_xyz_out, d_flags, d_score, d_res);
+}
+
+olpt_status olpt_residual(olpt_ctx *c, const void *d_imgs, int dtype, const olpt_ipr_params *p,
+                          const double *d_xyz, int n_obj, void *d_res) {
+  olpt_status s = check_ready(c);
+  if (s) return s;
+  if (!p || !d_imgs || !d_res || (n_obj > 0 && !d_xyz)) return fail(OLPT_E_INVALID_ARGUMENT, "null argument");
+  if ((s = check_params(c, p))) return s;
+  if (!c->otf_set) return fail(OLPT_E_INVALID_ARGUMENT, "OTF not set");
+  if (n_obj < 0 || n_obj > c->cap_obj) return fail(OLPT_E_CAPACITY, "object count exceeds capacity");
+  if (c->prec == OLPT_PREC_F64) return residual_impl<double>(c, d_imgs, dtype, p, d_xyz, n_obj, d_res);
+  return residual_impl<float>(c, d_imgs, dtype, p, d_xyz, n_obj, d_res);
+}
+
+olpt_status olpt_ipr_run(olpt_ctx *c, const void *d_imgs, int dtype, int n_frames,
+                         const olpt_ipr_params *p, double *d_obj_xyz, double *d_obj_uv, int cap,
+                         int32_t *d_n_obj, int32_t *loop_total, int cap_loop) {
+  olpt_status s = check_ready(c);
+  if (s) return s;
+  if (!p || !d_imgs || !d_obj_xyz || !d_n_obj || n_frames < 1)
+    return fail(OLPT_E_INVALID_ARGUMENT, "null argument");
+  if ((s = check_params(c, p))) return s;
+  if (!c->otf_set) return fail(OLPT_E_INVALID_ARGUMENT, "OTF not set");
+  if (c->prec == OLPT_PREC_F64)
+    return ipr_run_impl<double>(c, d_imgs, dtype, n_frames, p, d_obj_xyz, d_obj_uv, cap, d_n_obj,
+                                loop_total, cap_loop, true);
+  return ipr_run_impl<float>(c, d_imgs, dtype, n_frames, p, d_obj_xyz, d_obj_uv, cap, d_n_obj,
+                             loop_total, cap_loop, true);
+}
+
+olpt_status olpt_ipr_graph_capture(olpt_ctx *c, const void *d_imgs, int dtype, int n_frames,
+                                   const olpt_ipr_params *p, double *d_obj_xyz, double *d_obj_uv,
+                                   int cap, int32_t *d_n_obj) {
+  olpt_status s = check_ready(c);
+  if (s) return s;
+  if (!p || !d_imgs || !d_obj_xyz || !d_n_obj || n_frames < 1)
+    return fail(OLPT_E_INVALID_ARGUMENT, "null argument");
+  if ((s = check_params(c, p))) return s;
+  if (!c->otf_set) return fail(OLPT_E_INVALID_ARGUMENT, "OTF not set");
+  if (!c->st) return fail(OLPT_E_INVALID_ARGUMENT, "graph capture needs a non-null stream");
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  HIP_TRY(hipStreamSynchronize(c->st));
+  HIP_TRY(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  if (c->prec == OLPT_PREC_F64)
+    s = ipr_run_impl<double>(c, d_imgs, dtype, n_frames, p, d_obj_xyz, d_obj_uv, cap, d_n_obj,
+                             nullptr, 0, false);
+  else
+    s = ipr_run_impl<float>(c, d_imgs, dtype, n_frames, p, d_obj_xyz, d_obj_uv, cap, d_n_obj,
+                            nullptr, 0, false);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->st, &g);
+  if (s) {
+    if (g) (void)hipGraphDestroy(g);
+    return s;
+  }
+  if (e != hipSuccess) return fail(OLPT_E_GPU, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  c->graph = g;
+  HIP_TRY(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+  return OLPT_OK;
+}
+
+olpt_status olpt_ipr_graph_launch(olpt_ctx *c) {
+  if (!c || !c->gexec) return fail(OLPT_E_INVALID_ARGUMENT, "no captured graph");
+  HIP_TRY(hipGraphLaunch(c->gexec, c->st));
+  return OLPT_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,370 @@
+// olpt_device.hpp - device-side geometry shared by the IPR kernels (gfx950).
+//
+// Camera projection / line of sight / triangulation / OTF interpolation /
+// ROI bounds, written for the GPU: templated on the arithmetic type T
+// (float for the fast path, double for the parity path) where the reference
+// computes in FP64, and always double where the tier design keeps FP64
+// (line-of-sight inversion of distortion, triangulation normal equations).
+// The library is compiled with -ffp-contract=off so FP64 results follow the
+// reference's rounding (the reference is built without FMA contraction).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "olpt_types.h"
+
+namespace olpt {
+
+constexpr int kWave = 64;
+constexpr int kMaxCam = OLPT_MAX_CAM;
+
+// --------------------------------------------------------------------------
+// Camera in the arithmetic type T (converted once on the host).
+// --------------------------------------------------------------------------
+template <typename T> struct CamT {
+  T fx, fy, cx, cy;
+  T k[12];
+  T R[9];
+  T Tv[3];
+  T Rinv[9];
+  T Tinv[3];
+  int32_t n_dist, is_distorted, n_row, n_col;
+};
+
+template <typename T> CamT<T> make_cam(const olpt_camera &c) {
+  CamT<T> o;
+  o.fx = (T)c.fx; o.fy = (T)c.fy; o.cx = (T)c.cx; o.cy = (T)c.cy;
+  for (int i = 0; i < 12; ++i) o.k[i] = (T)c.dist[i];
+  for (int i = 0; i < 9; ++i) { o.R[i] = (T)c.R[i]; o.Rinv[i] = (T)c.Rinv[i]; }
+  for (int i = 0; i < 3; ++i) { o.Tv[i] = (T)c.T[i]; o.Tinv[i] = (T)c.Tinv[i]; }
+  o.n_dist = c.is_distorted ? c.n_dist : 0;
+  o.is_distorted = c.is_distorted;
+  o.n_row = c.n_row;
+  o.n_col = c.n_col;
+  return o;
+}
+
+// Camera::distort (Camera.cpp:913-966): normalized (x,y) -> pixel (u,v)
+template <typename T>
+__device__ __forceinline__ void distort(const CamT<T> &c, T x, T y, T &u, T &v) {
+  if (c.is_distorted) {
+    const T *k = c.k;
+    const int nd = c.n_dist;
+    T r2 = x * x + y * y;
+    T r4 = r2 * r2;
+    T r6 = r4 * r2;
+    T a1 = T(2) * x * y;
+    T a2 = r2 + T(2) * x * x;
+    T a3 = r2 + T(2) * y * y;
+    T cd = T(1) + k[0] * r2 + k[1] * r4;
+    if (nd > 4) cd += k[4] * r6;
+    T ic = T(1);
+    if (nd > 5) ic = T(1) / (T(1) + k[5] * r2 + k[6] * r4 + k[7] * r6);
+    T xd = x * cd * ic + k[2] * a1 + k[3] * a2;
+    T yd = y * cd * ic + k[2] * a3 + k[3] * a1;
+    if (nd > 8) {
+      xd += k[8] * r2 + k[9] * r4;
+      yd += k[10] * r2 + k[11] * r4;
+    }
+    x = xd;
+    y = yd;
+  }
+  u = x * c.fx + c.cx;
+  v = y * c.fy + c.cy;
+}
+
+// Camera::project, pinhole branch = worldToUndistImg (Camera.cpp:680-689)
+// followed by distort.
+template <typename T>
+__device__ __forceinline__ void project(const CamT<T> &c, const T X[3], T &u,
+                                        T &v) {
+  T t[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    t[i] = c.R[3 * i] * X[0] + c.R[3 * i + 1] * X[1] + c.R[3 * i + 2] * X[2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) t[i] = t[i] + c.Tv[i];
+  const T x = t[2] != T(0) ? t[0] / t[2] : t[0];
+  const T y = t[2] != T(0) ? t[1] / t[2] : t[1];
+  distort(c, x, y, u, v);
+}
+
+// Camera::undistort (Camera.cpp:1018-1094) + pinholeLine (:1096-1108).
+// Returns the unit direction; the line passes through Tinv.  FP64 only.
+__device__ inline void line_of_sight(const CamT<double> &c, double u,
+                                     double v, double out[3]) {
+  double x = (u - c.cx) / c.fx;
+  double y = (v - c.cy) / c.fy;
+  double ux = x, uy = y;
+  if (c.is_distorted) {
+    const double *k = c.k;
+    const int nd = c.n_dist;
+    const double x0 = x, y0 = y;
+    double err = 1.7976931348623157e308;
+    int it = 0;
+    while (err > 1e-5 && it < 50) {
+      double r2 = x * x + y * y;
+      double a1 = 2 * x * y;
+      double a2 = r2 + 2 * x * x;
+      double a3 = r2 + 2 * y * y;
+      double ic;
+      if (nd == 4)
+        ic = 1 / (1 + (k[1] * r2 + k[0]) * r2);
+      else if (nd == 5)
+        ic = 1 / (1 + ((k[4] * r2 + k[1]) * r2 + k[0]) * r2);
+      else
+        ic = (1 + ((k[7] * r2 + k[6]) * r2 + k[5]) * r2) /
+             (1 + ((k[4] * r2 + k[1]) * r2 + k[0]) * r2);
+      if (ic < 0) {
+        ux = (u - c.cx) / c.fx;
+        uy = (v - c.cy) / c.fy;
+        break;
+      }
+      double dX = k[2] * a1 + k[3] * a2;
+      double dY = k[2] * a3 + k[3] * a1;
+      if (nd > 8) {
+        dX += k[8] * r2 + k[9] * r2 * r2;
+        dY += k[10] * r2 + k[11] * r2 * r2;
+      }
+      x = (x0 - dX) * ic;
+      y = (y0 - dY) * ic;
+      ux = x;
+      uy = y;
+      double pu, pv;
+      distort(c, ux, uy, pu, pv);
+      const double du = pu - u, dv = pv - v;
+      err = sqrt(du * du + dv * dv);
+      ++it;
+    }
+  }
+  double p[3] = {ux, uy, 1.0}, w[3], d[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    w[i] = c.Rinv[3 * i] * p[0] + c.Rinv[3 * i + 1] * p[1] + c.Rinv[3 * i + 2] * p[2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) w[i] = w[i] + c.Tinv[i];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) d[i] = w[i] - c.Tinv[i];
+  const double n = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) out[i] = d[i] / n;
+}
+
+// myMATH::gaussInverse for 3x3 (myMATH.h:183-289), incl. the signed row
+// maximum used for pivot scaling.
+__device__ inline void gauss_inverse3(const double A[9], double out[9]) {
+  double u[9];
+  int idx[3] = {0, 1, 2};
+  double mr[3];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) u[i] = A[i];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double m = u[3 * i];
+    if (u[3 * i + 1] > m) m = u[3 * i + 1];
+    if (u[3 * i + 2] > m) m = u[3 * i + 2];
+    mr[i] = m;
+  }
+  for (int y = 0; y < 2; ++y) {
+    double mp = 0;
+    int t = 0;
+    for (int x = y; x < 3; ++x) {
+      const double pv = fabs(u[3 * idx[x] + y] / mr[idx[x]]);
+      if (pv > mp) {
+        mp = pv;
+        t = x;
+      }
+    }
+    const int tmp = idx[t];
+    idx[t] = idx[y];
+    idx[y] = tmp;
+    for (int x = y + 1; x < 3; ++x) {
+      const double m = u[3 * idx[x] + y] / u[3 * idx[y] + y];
+      u[3 * idx[x] + y] = m;
+      for (int it = y + 1; it < 3; ++it) u[3 * idx[x] + it] -= m * u[3 * idx[y] + it];
+    }
+  }
+  double b[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  for (int col = 0; col < 3; ++col) {
+    for (int y = 0; y < 2; ++y)
+      for (int x = y + 1; x < 3; ++x) b[3 * idx[x] + col] -= u[3 * idx[x] + y] * b[3 * idx[y] + col];
+    out[6 + col] = b[3 * idx[2] + col] / u[3 * idx[2] + 2];
+    for (int x = 1; x > -1; --x) {
+      double z = b[3 * idx[x] + col];
+      for (int y = x + 1; y < 3; ++y) z -= u[3 * idx[x] + y] * out[3 * y + col];
+      out[3 * x + col] = z / u[3 * idx[x] + x];
+    }
+  }
+}
+
+// myMATH::triangulation (myMATH.cpp:357-419) over n lines through pt[i]
+// with unit direction u[i].  Returns false where the reference throws.
+__device__ inline bool triangulate(const double (*pt)[3], const double (*u)[3],
+                                   int n, double X[3], double &err) {
+  double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, P[3] = {0, 0, 0};
+  for (int i = 0; i < n; ++i) {
+    const double nx = u[i][0], ny = u[i][1], nz = u[i][2];
+    const double t[9] = {1 - nx * nx, -nx * ny, -nx * nz, -nx * ny, 1 - ny * ny,
+                         -ny * nz, -nx * nz, -ny * nz, 1 - nz * nz};
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      P[r] += t[3 * r] * pt[i][0] + t[3 * r + 1] * pt[i][1] + t[3 * r + 2] * pt[i][2];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) M[k] += t[k];
+  }
+  double inv[9];
+  gauss_inverse3(M, inv);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) X[r] = inv[3 * r] * P[0] + inv[3 * r + 1] * P[1] + inv[3 * r + 2] * P[2];
+  double e = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double d0 = X[0] - pt[i][0], d1 = X[1] - pt[i][1], d2 = X[2] - pt[i][2];
+    const double nr = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    const double dt = d0 * u[i][0] + d1 * u[i][1] + d2 * u[i][2];
+    double h = nr * nr - dt * dt;
+    if (h < 0.0 && h >= -1e-8)
+      h = 0.0;
+    else if (h < -1e-8)
+      return false;
+    else
+      h = sqrt(h);
+    e += h * h;
+  }
+  err = sqrt(e / n);
+  return true;
+}
+
+// --------------------------------------------------------------------------
+// OTF (OTF.cpp:273-338 + myMATH.cpp:131-214), arithmetic type T.
+// --------------------------------------------------------------------------
+template <typename T> struct OtfT {
+  int32_t nx, ny, nz, ng;
+  T bnd[6];
+  const T *a, *b, *c, *alpha; // [n_cam][ng]
+};
+
+template <typename T>
+__device__ __forceinline__ T trilin(const T lim[6], const T v[8], const T p[3]) {
+  T xd = (p[0] - lim[0]) / (lim[1] - lim[0]);
+  T yd = (p[1] - lim[2]) / (lim[3] - lim[2]);
+  T zd = (p[2] - lim[4]) / (lim[5] - lim[4]);
+  xd = fmax(T(0), fmin(T(1), xd));
+  yd = fmax(T(0), fmin(T(1), yd));
+  zd = fmax(T(0), fmin(T(1), zd));
+  const T c00 = v[0] * (1 - xd) + v[1] * xd;
+  const T c01 = v[3] * (1 - xd) + v[2] * xd;
+  const T c10 = v[4] * (1 - xd) + v[5] * xd;
+  const T c11 = v[7] * (1 - xd) + v[6] * xd;
+  const T c0 = c00 * (1 - yd) + c10 * yd;
+  const T c1 = c01 * (1 - yd) + c11 * yd;
+  return c0 * (1 - zd) + c1 * zd;
+}
+
+// -> {a, b, c, cos(alpha), sin(alpha)}
+template <typename T>
+__device__ inline void otf_param(const OtfT<T> &o, int cam, const T X[3], T out[5]) {
+  const T *B = o.bnd;
+  const T px = fmin(fmax(X[0], B[0]), B[1]);
+  const T py = fmin(fmax(X[1], B[2]), B[3]);
+  const T pz = fmin(fmax(X[2], B[4]), B[5]);
+  const T dx = (B[1] - B[0]) / (o.nx - 1);
+  const T dy = (B[3] - B[2]) / (o.ny - 1);
+  const T dz = (B[5] - B[4]) / (o.nz - 1);
+  int ix = (int)floor((px - B[0]) / dx), iy = (int)floor((py - B[2]) / dy),
+      iz = (int)floor((pz - B[4]) / dz);
+  ix = ix < 0 ? 0 : (ix > o.nx - 2 ? o.nx - 2 : ix);
+  iy = iy < 0 ? 0 : (iy > o.ny - 2 ? o.ny - 2 : iy);
+  iz = iz < 0 ? 0 : (iz > o.nz - 2 ? o.nz - 2 : iz);
+  const T gx = (B[1] - B[0]) / (o.nx - 1), gy = (B[3] - B[2]) / (o.ny - 1),
+          gz = (B[5] - B[4]) / (o.nz - 1);
+  const T lim[6] = {B[0] + gx * ix, B[0] + gx * (ix + 1), B[2] + gy * iy,
+                    B[2] + gy * (iy + 1), B[4] + gz * iz, B[4] + gz * (iz + 1)};
+  const int nyz = o.ny * o.nz;
+  const int g[8] = {ix * nyz + iy * o.nz + iz,           (ix + 1) * nyz + iy * o.nz + iz,
+                    (ix + 1) * nyz + iy * o.nz + iz + 1, ix * nyz + iy * o.nz + iz + 1,
+                    ix * nyz + (iy + 1) * o.nz + iz,     (ix + 1) * nyz + (iy + 1) * o.nz + iz,
+                    (ix + 1) * nyz + (iy + 1) * o.nz + iz + 1, ix * nyz + (iy + 1) * o.nz + iz + 1};
+  const T p[3] = {px, py, pz};
+  T va[8], vb[8], vc[8], vl[8];
+  const int base = cam * o.ng;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    va[k] = o.a[base + g[k]];
+    vb[k] = o.b[base + g[k]];
+    vc[k] = o.c[base + g[k]];
+    vl[k] = o.alpha[base + g[k]];
+  }
+  out[0] = trilin(lim, va, p);
+  out[1] = trilin(lim, vb, p);
+  out[2] = trilin(lim, vc, p);
+  const T al = trilin(lim, vl, p);
+  out[3] = cos(al);
+  out[4] = sin(al);
+}
+
+// Gaussian OTF intensity at pixel (col,row) for a particle at (u,v)
+// (TracerShakeStrategy::gaussIntensity, Shake.cpp:901-914).
+template <typename T>
+__device__ __forceinline__ T gauss_int(const T P[5], T u, T v, int row, int col) {
+  const T dx = T(col) - u, dy = T(row) - v;
+  const T xx = dx * P[3] + dy * P[4];
+  const T yy = -dx * P[4] + dy * P[3];
+  const T val = P[0] * exp(-(P[1] * xx * xx + P[2] * yy * yy));
+  return val > T(0) ? val : T(0);
+}
+
+// Shake::calROIBound (Shake.cpp:246-277).  [rmin,rmax) x [cmin,cmax);
+// empty is {1,0,1,0}.
+struct PRange {
+  int rmin, rmax, cmin, cmax;
+  __device__ bool empty() const { return rmax < rmin || cmax < cmin; }
+};
+
+template <typename T>
+__device__ __forceinline__ PRange roi_bound(int H, int W, T cx, T cy, T dx, T dy) {
+  PRange e{1, 0, 1, 0};
+  if (H <= 0 || W <= 0 || dx <= T(0) || dy <= T(0) || cx <= T(0) || cy <= T(0) ||
+      cx >= T(W) || cy >= T(H))
+    return e;
+  const int width = (int)round(T(2) * dx);
+  const int height = (int)round(T(2) * dy);
+  int cmin = (int)round(cx - T(width - 1) / T(2));
+  int rmin = (int)round(cy - T(height - 1) / T(2));
+  int cmax = cmin + width, rmax = rmin + height;
+  rmin = rmin > 0 ? rmin : 0;
+  cmin = cmin > 0 ? cmin : 0;
+  rmax = rmax < H ? rmax : H;
+  cmax = cmax < W ? cmax : W;
+  if (rmax < rmin || cmax < cmin) return e;
+  return PRange{rmin, rmax, cmin, cmax};
+}
+
+// --------------------------------------------------------------------------
+// Wavefront helpers (wave64).
+// --------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// number of set bits below this lane
+__device__ __forceinline__ int mbcnt(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+template <typename T> __device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+template <typename T> __device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    T o = __shfl_xor(v, off, kWave);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+} // namespace olpt

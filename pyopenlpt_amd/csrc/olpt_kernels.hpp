@@ -1,0 +1,159 @@
+// olpt_kernels.hpp - host-side launchers for the IPR kernels.  All counts
+// live in device memory; every launcher takes fixed grids sized from the
+// context's capacities, so a whole frame is a fixed kernel sequence that a
+// hipGraph can capture.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "olpt_device.hpp"
+
+namespace olpt {
+
+// Which cameras a stage works on (host-side list, passed by value).
+struct CamList {
+  int32_t n;
+  int32_t id[kMaxCam];
+};
+
+// ---------------- peaks (ObjectFinder2D::findTracer2D) --------------------
+// img: [n_cam][H][W] (element stride cam_stride); output xy as (x,y) double
+// pairs at xy[(cam*cap + i)*2]; npts[cam] = min(count, cap); overflow[0] is
+// set to 1 if any camera had more than cap peaks.  Work buffers:
+// mask/cnt/off hold (H-2)*ceil((W-2)/64) entries per listed camera.
+template <typename TIn>
+void launch_find_peaks(const TIn *img, long cam_stride, int H, int W,
+                       double min_int, CamList cams, uint64_t *mask,
+                       int32_t *cnt, int32_t *off, int32_t *scan_scratch,
+                       int32_t *totals, double *xy, int cap, int32_t *npts,
+                       int32_t *overflow, hipStream_t st);
+
+inline long peak_chunks(int H, int W) {
+  return (long)(H > 2 ? H - 2 : 0) * ((W - 2 + 63) / 64);
+}
+
+// ---------------- stereo matching (StereoMatch) ---------------------------
+struct StereoBuffers {
+  int n_cam;
+  int cap;               // per-camera 2D capacity
+  const CamT<double> *cams;
+  const double *xy;      // [n_cam][cap][2]
+  const int32_t *npts;   // [n_cam]
+  double *los;           // [n_cam][cap][4]  unit direction (w unused)
+  int cell_px, rows_cell, cols_cell;
+  int32_t *cell_cnt;     // [n_cam][ncell]
+  int32_t *cell_start;   // [n_cam][ncell]
+  int32_t *cell_pid;     // [n_cam][cap]
+  int32_t *pt_cell;      // [n_cam][cap]
+  int32_t *pt_rank;      // [n_cam][cap]
+  // candidates
+  int cap_cand;
+  int32_t *cand_ids;     // [cap_cand][n_cam] raw (emission order)
+  double *cand_e;
+  int32_t *n_cand;       // device counter
+  int32_t *cand_ids_s;   // sorted
+  double *cand_e_s;
+  int32_t *key_cnt;      // [cap] histogram on the primary camera
+  int32_t *key_start;    // [cap]
+  int32_t *cand_rank;    // [cap_cand]
+  // prune
+  int32_t *pt_count;     // [n_cam*cap] candidates per flat point
+  int32_t *pt_start;     // [n_cam*cap]
+  int32_t *pt_list;      // [cap_cand*n_cam]
+  int32_t *flat_rank;    // [cap_cand][n_cam]
+  double *pv;            // [cap_cand][n_cam] percentile per used point
+  double *pct;           // [cap_cand]
+  int8_t *state;         // [cap_cand] 0 undecided, 1 selected, 2 rejected
+  int32_t *undecided;    // [max_rounds] per-round undecided counts
+  int32_t *keep;         // [cap_cand] 1 if selected and triangulated
+  int32_t *keep_off;     // [cap_cand]
+  double *tri_xyz;       // [cap_cand][3]
+  // objects
+  int cap_obj;
+  double *obj_xyz;       // [cap_obj][3]
+  int32_t *obj_ids;      // [cap_obj][n_cam]
+  int32_t *n_obj;        // device
+  int32_t *n_sel;        // device
+  int32_t *flags;        // [8]: 0 cand overflow, 1 list overflow, 2 pool
+                         //      overflow, 3 replacement, 4 obj overflow,
+                         //      5 rounds exhausted, 6 triangulation throw
+  int32_t *work;         // dynamic work counter
+  int32_t *scan_scratch;
+  int32_t *totals;       // small scratch for scan totals
+};
+
+struct StereoParams {
+  int32_t active[kMaxCam];
+  int n_active;
+  int build_count;
+  double tol2d, tol3d;
+  double limit[6];
+};
+
+void launch_stereo(const StereoBuffers &b, const StereoParams &p,
+                   int max_rounds, hipStream_t st);
+
+// ---------------- shake (Shake::runShake) ---------------------------------
+template <typename T> struct ShakeBuffers {
+  int n_cam, H, W;
+  const CamT<T> *cams;
+  OtfT<T> otf;
+  T *img;             // [n_cam][H][W] loop input (orig for shake)
+  T *res;             // [n_cam][H][W] residual (work)
+  T *img_next;        // [n_cam][H][W] IPR-mode residual (next loop input)
+  int cap_obj;
+  double *xyz;        // [cap_obj][3]   in/out positions
+  T *uv;              // [cap_obj][n_cam][2]
+  double *score;      // [cap_obj]
+  uint8_t *flag;      // [cap_obj]
+  const int32_t *n_obj;
+  // binning of 4x4 projection boxes (residual gather)
+  int bin_px, bin_rows, bin_cols;
+  int32_t *bin_cnt, *bin_start, *bin_obj, *obj_bin, *obj_rank; // per cam
+  // repeated-object grid
+  int grid_n;         // cells per axis
+  double grid_lo[3], grid_cell;
+  int32_t *g_cnt, *g_start, *g_obj, *g_rank, *g_cellof;
+  int32_t *has_nb;
+  // per (object, camera) projection records, reduction partials
+  void *rec;          // [cap_obj][n_cam] records (sizeof depends on T)
+  double *partial;    // [2*256]
+  int32_t *keep_off;  // [cap_obj]
+  int32_t *n_keep;
+  int32_t *scan_scratch;
+  int32_t *totals;
+  int32_t *work;
+};
+
+struct ShakeParams {
+  int32_t active[kMaxCam];
+  double shake_width, shakewidth_min, thred_ghost, tol3d, r_px;
+  int n_shake_loop, ratio_augimg;
+};
+
+template <typename T>
+void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p,
+                  hipStream_t st);
+
+// IPR-mode residual of the kept objects: img_next = max(0, img - max proj)
+// over all cameras (Shake::calResidualImage, Shake.h:40-48).
+template <typename T>
+void launch_residual_ipr(const ShakeBuffers<T> &b, const int32_t *n_keep,
+                         const double *xyz_keep, T *uv_keep, double r_px,
+                         hipStream_t st);
+
+// order-preserving compaction of the objects with flag == 0 into
+// (xyz_keep, uv_keep); count to *n_keep
+template <typename T>
+void launch_filter(const ShakeBuffers<T> &b, double *xyz_keep, T *uv_keep,
+                   int32_t *n_keep, hipStream_t st);
+
+// size in bytes of one (object, camera) record
+size_t shake_rec_bytes(bool f64);
+
+template <typename T>
+void launch_convert_image(const void *src, int dtype, long n, T *dst,
+                          hipStream_t st);
+
+} // namespace olpt

@@ -1,0 +1,159 @@
+// olpt_peaks.hip - 2D tracer detection (ObjectFinder2D::findTracer2D,
+// ObjectFinder.cpp:37-111).
+//
+// Layout: one wave64 per "chunk" = 64 consecutive interior columns of one
+// interior row.  Pass 1 evaluates threshold + 4-neighbour local maximum
+// (myMATH::isLocalMax tie-break: up/left may equal, down/right must be
+// strictly lower, myMATH.h:398-439) with coalesced loads and writes the
+// 64-bit ballot mask and its popcount; the predicate includes the fit's own
+// rejections so counts and order match the reference exactly.  An exclusive
+// scan of the per-chunk counts gives each chunk its output offset, so pass 2
+// writes the peaks in the reference's row-major scan order with no atomics,
+// fitting the separable three-point log-parabola in FP64 (the reference's
+// absolute-coordinate formula, :67-97).
+#include "olpt_kernels.hpp"
+#include "olpt_scan.hpp"
+
+namespace olpt {
+
+constexpr int kPeakWaves = 4; // waves per workgroup
+
+template <typename TIn>
+__device__ __forceinline__ double pix(const TIn *p, long i) {
+  return (double)p[i];
+}
+
+__device__ __forceinline__ double safe_ln(double v) {
+  return log(v < 1e-4 ? 1e-4 : v);
+}
+
+// false where the reference rejects the peak (zero denominator or
+// non-finite centre, ObjectFinder.cpp:81-83, :95-97)
+template <typename TIn>
+__device__ __forceinline__ bool fit_peak(const TIn *p, long i, int W, int row,
+                                         int col, double &xc, double &yc) {
+  const int x1 = col - 1, x2 = col, x3 = col + 1;
+  const int y1 = row - 1, y2 = row, y3 = row + 1;
+  const double l1x = safe_ln(pix(p, i - 1));
+  const double l2 = safe_ln(pix(p, i));
+  const double l3x = safe_ln(pix(p, i + 1));
+  const double nx = l1x * (double)((x2 * x2) - (x3 * x3)) -
+                    l2 * (double)((x1 * x1) - (x3 * x3)) +
+                    l3x * (double)((x1 * x1) - (x2 * x2));
+  const double dx = l1x * (double)(x3 - x2) - l3x * (double)(x1 - x2) +
+                    l2 * (double)(x1 - x3);
+  if (dx == 0.0) return false;
+  xc = -0.5 * (nx / dx);
+  if (!isfinite(xc)) return false;
+  const double l1y = safe_ln(pix(p, i - W));
+  const double l3y = safe_ln(pix(p, i + W));
+  const double ny = l1y * (double)((y2 * y2) - (y3 * y3)) -
+                    l2 * (double)((y1 * y1) - (y3 * y3)) +
+                    l3y * (double)((y1 * y1) - (y2 * y2));
+  const double dy = l1y * (double)(y3 - y2) - l3y * (double)(y1 - y2) +
+                    l2 * (double)(y1 - y3);
+  if (dy == 0.0) return false;
+  yc = -0.5 * (ny / dy);
+  return isfinite(yc);
+}
+
+template <typename TIn>
+__global__ void __launch_bounds__(kPeakWaves * kWave)
+k_peak_mask(const TIn *__restrict__ img, long cam_stride, int H, int W,
+            double min_int, CamList cams, uint64_t *__restrict__ mask,
+            int32_t *__restrict__ cnt, long nchunk) {
+  const int s = blockIdx.y;
+  const long chunk = (long)blockIdx.x * kPeakWaves + threadIdx.x / kWave;
+  if (chunk >= nchunk) return;
+  const int cpr = (W - 2 + 63) / 64;
+  const int row = 1 + (int)(chunk / cpr);
+  const int col = 1 + (int)(chunk % cpr) * 64 + lane_id();
+  const TIn *p = img + cams.id[s] * cam_stride;
+  bool peak = false;
+  if (col < W - 1) {
+    const long i = (long)row * W + col;
+    const double v = pix(p, i);
+    if (!(v < min_int)) {
+      const double up = pix(p, i - W), lf = pix(p, i - 1);
+      const double dn = pix(p, i + W), rt = pix(p, i + 1);
+      peak = !(up > v) && !(lf > v) && !(dn >= v) && !(rt >= v);
+      double xc, yc;
+      if (peak) peak = fit_peak(p, i, W, row, col, xc, yc);
+    }
+  }
+  const uint64_t m = ballot(peak);
+  if (lane_id() == 0) {
+    mask[s * nchunk + chunk] = m;
+    cnt[s * nchunk + chunk] = __popcll(m);
+  }
+}
+
+template <typename TIn>
+__global__ void __launch_bounds__(kPeakWaves * kWave)
+k_peak_emit(const TIn *__restrict__ img, long cam_stride, int H, int W,
+            CamList cams, const uint64_t *__restrict__ mask,
+            const int32_t *__restrict__ off, long nchunk,
+            double *__restrict__ xy, int cap) {
+  const int s = blockIdx.y;
+  const long chunk = (long)blockIdx.x * kPeakWaves + threadIdx.x / kWave;
+  if (chunk >= nchunk) return;
+  const uint64_t m = mask[s * nchunk + chunk];
+  if (m == 0) return;
+  const int lane = lane_id();
+  if (!((m >> lane) & 1ull)) return;
+  const int cpr = (W - 2 + 63) / 64;
+  const int row = 1 + (int)(chunk / cpr);
+  const int col = 1 + (int)(chunk % cpr) * 64 + lane;
+  const int pos = off[s * nchunk + chunk] + mbcnt(m);
+  if (pos >= cap) return;
+  const int cam = cams.id[s];
+  const TIn *p = img + cam * cam_stride;
+  double xc = 0.0, yc = 0.0;
+  fit_peak(p, (long)row * W + col, W, row, col, xc, yc);
+  double *o = xy + ((long)cam * cap + pos) * 2;
+  o[0] = xc;
+  o[1] = yc;
+}
+
+// Per-camera counts and capacity check.
+__global__ void k_peak_finish(CamList cams, const int32_t *__restrict__ totals,
+                              int cap, int32_t *__restrict__ npts,
+                              int32_t *__restrict__ overflow) {
+  const int s = threadIdx.x;
+  if (s >= cams.n) return;
+  const int t = totals[s];
+  npts[cams.id[s]] = t < cap ? t : cap;
+  if (t > cap) atomicOr(&overflow[0], 1);
+}
+
+template <typename TIn>
+void launch_find_peaks(const TIn *img, long cam_stride, int H, int W,
+                       double min_int, CamList cams, uint64_t *mask,
+                       int32_t *cnt, int32_t *off, int32_t *scan_scratch,
+                       int32_t *totals, double *xy, int cap, int32_t *npts,
+                       int32_t *overflow, hipStream_t st) {
+  const long nchunk = peak_chunks(H, W);
+  if (nchunk <= 0 || cams.n <= 0) return;
+  dim3 g((unsigned)((nchunk + kPeakWaves - 1) / kPeakWaves), (unsigned)cams.n);
+  k_peak_mask<TIn><<<g, kPeakWaves * kWave, 0, st>>>(img, cam_stride, H, W,
+                                                     min_int, cams, mask, cnt,
+                                                     nchunk);
+  scan_exclusive(cnt, off, nchunk, cams.n, nchunk, totals, scan_scratch, st);
+  k_peak_emit<TIn><<<g, kPeakWaves * kWave, 0, st>>>(img, cam_stride, H, W, cams,
+                                                     mask, off, nchunk, xy, cap);
+  k_peak_finish<<<1, 64, 0, st>>>(cams, totals, cap, npts, overflow);
+}
+
+#define INST(T)                                                                \
+  template void launch_find_peaks<T>(const T *, long, int, int, double,        \
+                                     CamList, uint64_t *, int32_t *,           \
+                                     int32_t *, int32_t *, int32_t *,          \
+                                     double *, int, int32_t *, int32_t *,      \
+                                     hipStream_t);
+INST(uint8_t)
+INST(uint16_t)
+INST(float)
+INST(double)
+#undef INST
+
+} // namespace olpt

@@ -1,0 +1,102 @@
+// olpt_scan.hip - three-phase exclusive scan (tile sums, scan of tile sums,
+// tile-local scan plus carry).  Each tile is 4096 ints handled by one
+// 256-thread workgroup (16 contiguous ints per thread, LDS scan of the
+// per-thread partials), so a 4M-entry array is 1024 tiles.
+#include "olpt_scan.hpp"
+
+namespace olpt {
+
+__device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t *lds,
+                                                   int32_t &block_total) {
+  const int t = threadIdx.x;
+  lds[t] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    int32_t x = t >= off ? lds[t - off] : 0;
+    __syncthreads();
+    lds[t] += x;
+    __syncthreads();
+  }
+  const int32_t incl = lds[t];
+  block_total = lds[kScanThreads - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+k_tile_sums(const int32_t *__restrict__ in, long n, long stride,
+            int32_t *__restrict__ tsum, long tiles) {
+  __shared__ int32_t lds[kScanThreads];
+  const int seg = blockIdx.y;
+  const long tile = blockIdx.x;
+  const int32_t *p = in + seg * stride;
+  const long base = tile * kScanTile + (long)threadIdx.x * kScanPerThread;
+  int32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPerThread; ++k) {
+    const long i = base + k;
+    if (i < n) s += p[i];
+  }
+  int32_t tot;
+  block_excl_scan(s, lds, tot);
+  if (threadIdx.x == 0) tsum[seg * (tiles + 1) + tile] = tot;
+}
+
+// one workgroup per segment scans its tile sums in place
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_tile_sums(int32_t *__restrict__ tsum, long tiles,
+                 int32_t *__restrict__ total) {
+  __shared__ int32_t lds[kScanThreads];
+  const int seg = blockIdx.x;
+  int32_t *p = tsum + seg * (tiles + 1);
+  int32_t carry = 0;
+  for (long b = 0; b < tiles; b += kScanThreads) {
+    const long i = b + threadIdx.x;
+    const int32_t v = i < tiles ? p[i] : 0;
+    int32_t tot;
+    const int32_t ex = block_excl_scan(v, lds, tot);
+    if (i < tiles) p[i] = ex + carry;
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total) total[seg] = carry;
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+k_tile_scan(const int32_t *__restrict__ in, int32_t *__restrict__ out, long n,
+            long stride, const int32_t *__restrict__ tsum, long tiles) {
+  __shared__ int32_t lds[kScanThreads];
+  const int seg = blockIdx.y;
+  const long tile = blockIdx.x;
+  const int32_t *p = in + seg * stride;
+  int32_t *q = out + seg * stride;
+  const long base = tile * kScanTile + (long)threadIdx.x * kScanPerThread;
+  int32_t v[kScanPerThread];
+  int32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPerThread; ++k) {
+    const long i = base + k;
+    v[k] = i < n ? p[i] : 0;
+    s += v[k];
+  }
+  int32_t tot;
+  int32_t run = block_excl_scan(s, lds, tot) + tsum[seg * (tiles + 1) + tile];
+#pragma unroll
+  for (int k = 0; k < kScanPerThread; ++k) {
+    const long i = base + k;
+    if (i < n) q[i] = run;
+    run += v[k];
+  }
+}
+
+void scan_exclusive(const int32_t *in, int32_t *out, long n, int segments,
+                    long seg_stride, int32_t *total, int32_t *scratch,
+                    hipStream_t st) {
+  if (n <= 0 || segments <= 0) return;
+  const long tiles = (n + kScanTile - 1) / kScanTile;
+  dim3 g((unsigned)tiles, (unsigned)segments);
+  k_tile_sums<<<g, kScanThreads, 0, st>>>(in, n, seg_stride, scratch, tiles);
+  k_scan_tile_sums<<<segments, kScanThreads, 0, st>>>(scratch, tiles, total);
+  k_tile_scan<<<g, kScanThreads, 0, st>>>(in, out, n, seg_stride, scratch, tiles);
+}
+
+} // namespace olpt

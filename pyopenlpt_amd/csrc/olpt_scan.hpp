@@ -1,0 +1,28 @@
+// olpt_scan.hpp - exclusive prefix sums over int32 arrays (device-resident
+// sizes), used to turn per-chunk / per-cell / per-candidate counts into
+// output offsets without a host round trip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace olpt {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanPerThread = 16;
+constexpr int kScanTile = kScanThreads * kScanPerThread; // 4096
+
+// Scratch needed by scan_exclusive for n elements and `segments` segments.
+inline size_t scan_scratch_ints(long n, int segments) {
+  long tiles = (n + kScanTile - 1) / kScanTile;
+  return (size_t)(tiles + 1) * segments;
+}
+
+// Exclusive scan of `segments` arrays of length n (stride `seg_stride`):
+//   out[s][i] = sum_{k<i} in[s][k];  total[s] = sum of all.
+// in and out may alias.  total may be null.
+void scan_exclusive(const int32_t *in, int32_t *out, long n, int segments,
+                    long seg_stride, int32_t *total, int32_t *scratch,
+                    hipStream_t st);
+
+} // namespace olpt

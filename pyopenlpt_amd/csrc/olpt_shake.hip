@@ -1,0 +1,885 @@
+// olpt_shake.hip - Shake (tracer strategy) and residual images (Shake.cpp).
+//
+// Per shake iteration (Shake::runShake, Shake.cpp:36-163; Jacobi: every
+// object sees the residual built from the start-of-iteration positions):
+//   k_obj_project    2D projection of every object in every camera plus its
+//                    OTF parameters at the 3D centre (getOTFParam)
+//   k_bin_*          CSR bins of the objects' radius boxes (calROIBound with
+//                    r_px) keyed by the box's top-left 8x8-pixel cell
+//   k_resid_gather   16x16 pixel tile per workgroup: the objects whose boxes
+//                    can touch the tile are staged in LDS, every pixel takes
+//                    the maximum projection over them and writes
+//                    res = orig - max(0, max_k p_k)  (the reference's min
+//                    fusion, Shake.cpp:226-241) - no atomics on pixels
+//   k_shake          one wave64 per object: lanes own the pixels of the 8x8
+//                    augmented ROI of each camera (buildROIInfo, :279-328),
+//                    each residual evaluation (calShakeResidue, :968-1045)
+//                    is a lane-parallel squared difference plus a butterfly
+//                    reduction (bitwise identical in all lanes, so the
+//                    branchy per-axis search of shakeOneObject (:409-665)
+//                    stays wave-uniform); then calObjectScore (:668-782).
+// After the iterations: markRepeatedObj (:785-823) on a uniform grid with
+// the greedy pass over the (rare) objects that have a neighbour, the
+// mean-relative ghost threshold (:118-136), order-preserving filter
+// (IPR.cpp:41-63) and the IPR-mode residual for the next loop.
+#include "olpt_kernels.hpp"
+#include "olpt_scan.hpp"
+
+namespace olpt {
+
+constexpr int kTile = 16;         // residual tile edge (pixels)
+constexpr int kTileObjCap = 256;  // objects staged per tile batch
+
+static int grid_for(long n, int threads, int max_blocks) {
+  long g = (n + threads - 1) / threads;
+  if (g < 1) g = 1;
+  if (g > max_blocks) g = max_blocks;
+  return (int)g;
+}
+
+__global__ void k_zero_i32_s(int32_t *p, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+static void zero_i32(int32_t *p, long n, hipStream_t st) {
+  if (n <= 0) return;
+  k_zero_i32_s<<<grid_for(n, 256, 4096), 256, 0, st>>>(p, n);
+}
+
+// per (object, camera): u, v, box, OTF params
+template <typename T> struct ObjCamRec {
+  T u, v;
+  T P[5];
+  int32_t rmin, rmax, cmin, cmax;
+};
+
+template <typename T>
+__global__ void k_obj_project(ShakeBuffers<T> b, const double *__restrict__ xyz,
+                              const int32_t *__restrict__ n_ptr, ObjCamRec<T> *rec,
+                              T *uv_out, T r_px) {
+  const int n = *n_ptr;
+  const int nc = b.n_cam;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const T X[3] = {(T)xyz[3L * i], (T)xyz[3L * i + 1], (T)xyz[3L * i + 2]};
+    for (int k = 0; k < nc; ++k) {
+      ObjCamRec<T> r;
+      project(b.cams[k], X, r.u, r.v);
+      otf_param(b.otf, k, X, r.P);
+      const PRange pr = roi_bound<T>(b.H, b.W, r.u, r.v, r_px, r_px);
+      r.rmin = pr.rmin;
+      r.rmax = pr.rmax;
+      r.cmin = pr.cmin;
+      r.cmax = pr.cmax;
+      rec[(long)i * nc + k] = r;
+      if (uv_out) {
+        uv_out[((long)i * nc + k) * 2] = r.u;
+        uv_out[((long)i * nc + k) * 2 + 1] = r.v;
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_bin_count(ShakeBuffers<T> b, const int32_t *__restrict__ n_ptr,
+                            const ObjCamRec<T> *__restrict__ rec) {
+  const int n = *n_ptr;
+  const int nc = b.n_cam;
+  const long nbin = (long)b.bin_rows * b.bin_cols;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < (long)n * nc;
+       t += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(t / nc), k = (int)(t % nc);
+    const ObjCamRec<T> &r = rec[t];
+    int bin = -1, rank = 0;
+    if (!(r.rmax < r.rmin || r.cmax < r.cmin)) {
+      bin = (r.rmin / b.bin_px) * b.bin_cols + (r.cmin / b.bin_px);
+      rank = atomicAdd(&b.bin_cnt[k * nbin + bin], 1);
+    }
+    b.obj_bin[(long)k * b.cap_obj + i] = bin;
+    b.obj_rank[(long)k * b.cap_obj + i] = rank;
+  }
+}
+
+template <typename T>
+__global__ void k_bin_fill(ShakeBuffers<T> b, const int32_t *__restrict__ n_ptr) {
+  const int n = *n_ptr;
+  const int nc = b.n_cam;
+  const long nbin = (long)b.bin_rows * b.bin_cols;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < (long)n * nc;
+       t += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(t / nc), k = (int)(t % nc);
+    const int bin = b.obj_bin[(long)k * b.cap_obj + i];
+    if (bin < 0) continue;
+    b.bin_obj[(long)k * b.cap_obj + b.bin_start[k * nbin + bin] +
+              b.obj_rank[(long)k * b.cap_obj + i]] = i;
+  }
+}
+
+// mode 0: shake residual (active cameras, no clamp); mode 1: IPR output
+// (all cameras, clamp at 0).  The shake never reads an inactive camera's
+// residual, so mode 0 skips them.
+constexpr int kTileBinCap = 64;
+template <typename T>
+__global__ void __launch_bounds__(kTile *kTile)
+k_resid_gather(ShakeBuffers<T> b, const T *__restrict__ orig, T *__restrict__ out,
+               const ObjCamRec<T> *__restrict__ rec, CamList cams, int mode, int box_w) {
+  __shared__ ObjCamRec<T> s_obj[kTileObjCap];
+  __shared__ int32_t s_bstart[kTileBinCap], s_bpre[kTileBinCap + 1];
+  const int k = cams.id[blockIdx.z];
+  const int r0 = blockIdx.y * kTile, c0 = blockIdx.x * kTile;
+  const int tid = threadIdx.x;
+  const int row = r0 + tid / kTile, col = c0 + tid % kTile;
+  const long HW = (long)b.H * b.W;
+  const bool inside = row < b.H && col < b.W;
+  const long pix = (long)row * b.W + col;
+  const T o = inside ? orig[k * HW + pix] : T(0);
+  const long nbin = (long)b.bin_rows * b.bin_cols;
+  const int32_t *bstart = b.bin_start + k * nbin;
+  const int32_t *bcnt = b.bin_cnt + k * nbin;
+  const int32_t *bobj = b.bin_obj + (long)k * b.cap_obj;
+  // anchors (box top-left) that can cover this tile
+  const int ar0 = max(0, r0 - box_w + 1), ar1 = min(b.H - 1, r0 + kTile - 1);
+  const int ac0 = max(0, c0 - box_w + 1), ac1 = min(b.W - 1, c0 + kTile - 1);
+  const int br0 = ar0 / b.bin_px, br1 = ar1 / b.bin_px;
+  const int bc0 = ac0 / b.bin_px, bc1 = ac1 / b.bin_px;
+  const int nbc = bc1 - bc0 + 1;
+  const int nb = (br1 - br0 + 1) * nbc; // <= kTileBinCap (checked on the host)
+  if (tid < nb) {
+    const int bin = (br0 + tid / nbc) * b.bin_cols + bc0 + tid % nbc;
+    s_bstart[tid] = bstart[bin];
+    s_bpre[tid + 1] = bcnt[bin];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    s_bpre[0] = 0;
+    for (int q = 0; q < nb; ++q) s_bpre[q + 1] += s_bpre[q];
+  }
+  __syncthreads();
+  const int total = s_bpre[nb];
+  T maxp = T(0);
+  for (int base = 0; base < total; base += kTileObjCap) {
+    const int t = base + tid;
+    if (tid < kTileObjCap && t < total) {
+      int q = 0;
+      while (s_bpre[q + 1] <= t) ++q;
+      const int i = bobj[s_bstart[q] + (t - s_bpre[q])];
+      s_obj[tid] = rec[(long)i * b.n_cam + k];
+    }
+    __syncthreads();
+    const int n = min(kTileObjCap, total - base);
+    for (int u = 0; u < n; ++u) {
+      const ObjCamRec<T> &r = s_obj[u];
+      if (row >= r.rmin && row < r.rmax && col >= r.cmin && col < r.cmax) {
+        const T pv = gauss_int<T>(r.P, r.u, r.v, row, col);
+        if (pv > maxp) maxp = pv;
+      }
+    }
+    __syncthreads();
+  }
+  if (inside) {
+    T v = o - maxp;
+    if (mode == 1 && v < T(0)) v = T(0);
+    out[k * HW + pix] = v;
+  }
+}
+
+template <typename T>
+static void residual_pass(const ShakeBuffers<T> &b, const T *orig, T *out,
+                          const double *xyz, const int32_t *n_ptr,
+                          ObjCamRec<T> *rec, T *uv_out, CamList cams, int mode,
+                          T r_px, hipStream_t st) {
+  const int nc = b.n_cam;
+  const long nbin = (long)b.bin_rows * b.bin_cols;
+  k_obj_project<T><<<grid_for(b.cap_obj, 256, 2048), 256, 0, st>>>(b, xyz, n_ptr, rec, uv_out, r_px);
+  zero_i32(b.bin_cnt, nbin * nc, st);
+  k_bin_count<T><<<grid_for((long)b.cap_obj * nc, 256, 4096), 256, 0, st>>>(b, n_ptr, rec);
+  scan_exclusive(b.bin_cnt, b.bin_start, nbin, nc, nbin, nullptr, b.scan_scratch, st);
+  k_bin_fill<T><<<grid_for((long)b.cap_obj * nc, 256, 4096), 256, 0, st>>>(b, n_ptr);
+  const int box_w = (int)lround(2.0 * (double)r_px);
+  dim3 g((b.W + kTile - 1) / kTile, (b.H + kTile - 1) / kTile, cams.n);
+  k_resid_gather<T><<<g, kTile * kTile, 0, st>>>(b, orig, out, rec, cams, mode, box_w);
+}
+
+template <typename T>
+__global__ void k_copy_cam(const T *__restrict__ src, T *__restrict__ dst, long HW,
+                           CamList cams) {
+  const int k = cams.id[blockIdx.y];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < HW;
+       i += (long)gridDim.x * blockDim.x)
+    dst[k * HW + i] = src[k * HW + i];
+}
+
+// --------------------------------------------------------------------------
+// k_shake: wave per object
+// --------------------------------------------------------------------------
+template <typename T, int NC, int SLOTS> struct ShakeCtx {
+  int n_cam;
+  int active[NC];
+  PRange roi[NC];
+  int roiw[NC];
+  int npix[NC];
+  T aug[NC][SLOTS];   // this lane's augmented pixels
+  T r_px;
+};
+
+// one residual evaluation at X (calShakeResidue); wave-uniform result
+template <typename T, int NC, int SLOTS>
+__device__ T eval_residue(const ShakeBuffers<T> &b, const ShakeCtx<T, NC, SLOTS> &s,
+                          const T X[3], T *uv /* [NC][2] out */) {
+  const int lane = lane_id();
+  T acc = T(0);
+  int used = 0;
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    if (k >= s.n_cam) break;
+    T u, v;
+    project(b.cams[k], X, u, v);
+    uv[2 * k] = u;
+    uv[2 * k + 1] = v;
+    if (!s.active[k]) continue;
+    if (s.roi[k].rmax - s.roi[k].rmin <= 0 || s.roi[k].cmax - s.roi[k].cmin <= 0) continue;
+    T P[5];
+    otf_param(b.otf, k, X, P);
+    const int prmin = (int)floor(v - s.r_px), prmax = (int)ceil(v + s.r_px + T(1));
+    const int pcmin = (int)floor(u - s.r_px), pcmax = (int)ceil(u + s.r_px + T(1));
+    T sse = T(0);
+#pragma unroll
+    for (int sl = 0; sl < SLOTS; ++sl) {
+      const int idx = sl * kWave + lane;
+      if (idx < s.npix[k]) {
+        const int row = s.roi[k].rmin + idx / s.roiw[k];
+        const int col = s.roi[k].cmin + idx % s.roiw[k];
+        T pred = T(0);
+        if (row >= prmin && row < prmax && col >= pcmin && col < pcmax)
+          pred = gauss_int<T>(P, u, v, row, col);
+        const T d = s.aug[k][sl] - pred;
+        sse += d * d;
+      }
+    }
+    sse = wave_sum(sse);
+    acc += sse / T(s.npix[k]);
+    ++used;
+  }
+  return used > 1 ? acc / T(used) : T(INFINITY);
+}
+
+template <typename T>
+__device__ __forceinline__ bool in_roi_d(const PRange &r, T u, T v) {
+  return T(r.cmin) <= u && u < T(r.cmax) && T(r.rmin) <= v && v < T(r.rmax);
+}
+
+// median of up to NC values (uniform across lanes)
+template <typename T, int NC>
+__device__ T median_small(T *v, int n) {
+  if (n == 0) return T(0);
+  for (int i = 1; i < n; ++i) {
+    const T x = v[i];
+    int j = i - 1;
+    while (j >= 0 && v[j] > x) {
+      v[j + 1] = v[j];
+      --j;
+    }
+    v[j + 1] = x;
+  }
+  const int mid = n / 2;
+  T m = v[mid];
+  if ((n & 1) == 0) m = T(0.5) * (m + v[mid - 1]);
+  return m;
+}
+
+// myMATH::polyfit order 2 (myMATH.cpp:490-534): a = (inv(X'X) X') y, always
+// in FP64 - for offsets of ~1e-2 mm the normal matrix has a condition number
+// near 1e8, beyond FP32.
+__device__ void polyfit2(const double *x, const double *y, int n, double c[3]) {
+  double XtX[9];
+  for (int a = 0; a < 3; ++a)
+    for (int bb = 0; bb < 3; ++bb) {
+      double s = 0.0;
+      for (int i = 0; i < n; ++i) {
+        const double xa = a == 0 ? 1.0 : (a == 1 ? x[i] : x[i] * x[i]);
+        const double xb = bb == 0 ? 1.0 : (bb == 1 ? x[i] : x[i] * x[i]);
+        s += xa * xb;
+      }
+      XtX[3 * a + bb] = s;
+    }
+  double inv[9];
+  gauss_inverse3(XtX, inv);
+  for (int a = 0; a < 3; ++a) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) {
+      double t = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        const double xk = k == 0 ? 1.0 : (k == 1 ? x[i] : x[i] * x[i]);
+        t += inv[3 * a + k] * xk;
+      }
+      s += t * y[i];
+    }
+    c[a] = s;
+  }
+}
+
+template <typename T, int NC, int SLOTS>
+__global__ void __launch_bounds__(kWave)
+k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
+        T delta) {
+  const int n = *b.n_obj;
+  const int nc = b.n_cam;
+  const int lane = lane_id();
+  const long HW = (long)b.H * b.W;
+  const T r_px = (T)p.r_px;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    ShakeCtx<T, NC, SLOTS> s;
+    s.n_cam = nc;
+    s.r_px = r_px;
+    T X[3] = {(T)b.xyz[3L * i], (T)b.xyz[3L * i + 1], (T)b.xyz[3L * i + 2]};
+    T uv[2 * NC];
+    // buildROIInfo with the object's current projection
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      s.active[k] = (k < nc) ? p.active[k] : 0;
+      s.roi[k] = PRange{0, 0, 0, 0};
+      s.roiw[k] = 0;
+      s.npix[k] = 0;
+#pragma unroll
+      for (int sl = 0; sl < SLOTS; ++sl) s.aug[k][sl] = T(0);
+      if (k >= nc) continue;
+      const ObjCamRec<T> &r = rec[(long)i * nc + k];
+      uv[2 * k] = r.u;
+      uv[2 * k + 1] = r.v;
+      if (!s.active[k]) continue;
+      const T d = r_px > T(0) ? r_px : T(0);
+      const PRange roi = roi_bound<T>(b.H, b.W, r.u, r.v, d * (T)p.ratio_augimg,
+                                      d * (T)p.ratio_augimg);
+      s.roi[k] = roi;
+      if (roi.empty()) continue;
+      s.roiw[k] = roi.cmax - roi.cmin;
+      s.npix[k] = (roi.rmax - roi.rmin) * s.roiw[k];
+      const PRange rp{r.rmin, r.rmax, r.cmin, r.cmax};
+#pragma unroll
+      for (int sl = 0; sl < SLOTS; ++sl) {
+        const int idx = sl * kWave + lane;
+        if (idx < s.npix[k]) {
+          const int row = roi.rmin + idx / s.roiw[k];
+          const int col = roi.cmin + idx % s.roiw[k];
+          T a = b.res[k * HW + (long)row * b.W + col];
+          if (row >= rp.rmin && row < rp.rmax && col >= rp.cmin && col < rp.cmax)
+            a += gauss_int<T>(r.P, r.u, r.v, row, col);
+          s.aug[k][sl] = a > T(0) ? a : T(0);
+        }
+      }
+    }
+    // shakeOneObject; tmp keeps the last evaluated coordinate on each axis
+    T tmp[3] = {X[0], X[1], X[2]};
+    for (int ax = 0; ax < 3; ++ax) {
+      const T x0 = X[ax];
+      const T DL = -delta, DR = delta;
+      const T rel[3] = {DL, T(0), DR};
+      T f[3];
+      T uvs[3][2 * NC];
+      for (int j = 0; j < 3; ++j) {
+        tmp[ax] = x0 + rel[j];
+        f[j] = eval_residue<T, NC, SLOTS>(b, s, tmp, uvs[j]);
+      }
+      const T fL = f[0], f0 = f[1], fR = f[2];
+      const T tol = T(1e-3) * fmax(T(1), f0);
+      const bool mono_dec = (fL > f0 + tol) && (f0 > fR + tol);
+      const bool mono_inc = (fR > f0 + tol) && (f0 > fL + tol);
+      const bool mono = mono_dec || mono_inc;
+      int sgn = mono_dec ? 1 : (mono_inc ? -1 : 0);
+      const T kappa = (fL - T(2) * f0 + fR);
+      const T g = (fR - fL) / fmax(T(2) * delta, T(1e-12));
+      T cx[5] = {DL, T(0), DR, T(0), T(0)}, cf[5] = {fL, f0, fR, T(0), T(0)};
+      T cabs[5] = {x0 + DL, x0, x0 + DR, T(0), T(0)};
+      int ncand = 3;
+      bool skip_expand = false, allow_xhat = true;
+      if (kappa > T(0)) {
+        const T x3 = delta * (fL - fR) / (T(2) * kappa);
+        if (fabs(x3) <= T(0.95) * delta) {
+          const T a = kappa / (delta * delta + T(1e-12));
+          const T rho = (g * g) / (T(2) * a * fmax(f0, T(1)));
+          skip_expand = true;
+          if (rho < T(8e-4)) allow_xhat = false;
+        }
+      }
+      if (!skip_expand && mono) {
+        T Jv[NC], mg[NC];
+        int nJ = 0, nm = 0;
+        for (int k = 0; k < NC && k < nc; ++k) {
+          const bool ok0 = in_roi_d(s.roi[k], uvs[0][2 * k], uvs[0][2 * k + 1]);
+          const bool ok1 = in_roi_d(s.roi[k], uvs[1][2 * k], uvs[1][2 * k + 1]);
+          const bool ok2 = in_roi_d(s.roi[k], uvs[2][2 * k], uvs[2][2 * k + 1]);
+          if (ok0 && ok2) {
+            const T du = uvs[2][2 * k] - uvs[0][2 * k];
+            const T dv = uvs[2][2 * k + 1] - uvs[0][2 * k + 1];
+            Jv[nJ++] = hypot(du, dv) / fmax(T(2) * delta, T(1e-12));
+          }
+          if (ok1) {
+            const PRange &rr = s.roi[k];
+            const T mu = fmin(uvs[1][2 * k] - T(rr.cmin), T(rr.cmax) - T(1) - uvs[1][2 * k]);
+            const T mv = fmin(uvs[1][2 * k + 1] - T(rr.rmin), T(rr.rmax) - T(1) - uvs[1][2 * k + 1]);
+            mg[nm++] = fmin(mu, mv);
+          }
+        }
+        const T Jx = median_small<T, NC>(Jv, nJ);
+        const T mpx = median_small<T, NC>(mg, nm);
+        const T Droi = (Jx > T(0) && mpx > T(0)) ? (T(0.5) * mpx / Jx) : T(0);
+        T Dtry = T(0);
+        if (kappa > T(0)) {
+          const T x3 = delta * (fL - fR) / (T(2) * kappa);
+          if (fabs(x3) > delta) {
+            sgn = x3 > T(0) ? 1 : -1;
+            if (Jx > T(0) && fabs(x3) > (T(1) / Jx))
+              Dtry = T(1) / Jx;
+            else
+              Dtry = T(1.15) * fabs(x3);
+          }
+        }
+        if (Dtry <= T(0) && Jx > T(0)) Dtry = T(1) / Jx;
+        const T Dp = (Dtry > T(0) && Droi > T(0)) ? fmin(Dtry, Droi) : T(0);
+        if (Dp > T(0)) {
+          const T dxe = sgn >= 0 ? Dp : -Dp;
+          tmp[ax] = x0 + dxe;
+          T uvx[2 * NC];
+          // project for the in-ROI test before evaluating
+          bool out = false;
+          for (int k = 0; k < NC && k < nc; ++k) {
+            project(b.cams[k], tmp, uvx[2 * k], uvx[2 * k + 1]);
+            if (!in_roi_d(s.roi[k], uvx[2 * k], uvx[2 * k + 1])) out = true;
+          }
+          if (!out) {
+            cx[ncand] = dxe;
+            cf[ncand] = eval_residue<T, NC, SLOTS>(b, s, tmp, uvx);
+            cabs[ncand] = x0 + dxe;
+            ++ncand;
+          }
+        }
+      }
+      double dcx[5], dcf[5], coef[3];
+      for (int t = 0; t < ncand; ++t) {
+        dcx[t] = (double)cx[t];
+        dcf[t] = (double)cf[t];
+      }
+      polyfit2(dcx, dcf, ncand, coef);
+      if (allow_xhat && coef[2] > 0.0) {
+        const T xh = (T)(-coef[1] / (2.0 * coef[2]));
+        T xmin = cx[0], xmax = cx[0];
+        for (int t = 0; t < ncand; ++t) {
+          xmin = fmin(xmin, cx[t]);
+          xmax = fmax(xmax, cx[t]);
+        }
+        if (xh >= xmin && xh <= xmax) {
+          tmp[ax] = x0 + xh;
+          T uvx[2 * NC];
+          bool out = false;
+          for (int k = 0; k < NC && k < nc; ++k) {
+            project(b.cams[k], tmp, uvx[2 * k], uvx[2 * k + 1]);
+            if (!in_roi_d(s.roi[k], uvx[2 * k], uvx[2 * k + 1])) out = true;
+          }
+          if (!out) {
+            cx[ncand] = xh;
+            cf[ncand] = eval_residue<T, NC, SLOTS>(b, s, tmp, uvx);
+            cabs[ncand] = x0 + xh;
+            ++ncand;
+          }
+        }
+      }
+      int bi = 0;
+      T bf = cf[0];
+      for (int t = 1; t < ncand; ++t)
+        if (cf[t] < bf) {
+          bf = cf[t];
+          bi = t;
+        }
+      X[ax] = cabs[bi];
+    }
+    // final projection and calObjectScore
+    T P[NC][5];
+    for (int k = 0; k < NC && k < nc; ++k) {
+      project(b.cams[k], X, uv[2 * k], uv[2 * k + 1]);
+      otf_param(b.otf, k, X, P[k]);
+    }
+    T peak[NC], num[NC], den[NC];
+    int use[NC];
+    int n_used = 0;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      use[k] = 0;
+      peak[k] = num[k] = den[k] = T(0);
+      if (k >= nc || !s.active[k]) continue;
+      const T d = r_px > T(0) ? r_px : T(0);
+      if (d <= T(0)) continue;
+      const PRange rect = roi_bound<T>(b.H, b.W, uv[2 * k], uv[2 * k + 1], d, d);
+      if (rect.empty()) continue;
+      use[k] = 1;
+      ++n_used;
+      const int rw = rect.cmax - rect.cmin;
+      const int np = (rect.rmax - rect.rmin) * rw;
+      T pk = T(0), nm = T(0), dn = T(0);
+      for (int base = 0; base < np; base += kWave) {
+        const int idx = base + lane;
+        T meas = T(0), pred = T(0);
+        bool valid = idx < np;
+        int row = 0, col = 0;
+        if (valid) {
+          row = rect.rmin + idx / rw;
+          col = rect.cmin + idx % rw;
+        }
+        // augmented value if inside this camera's ROI, else residual
+        const PRange &ro = s.roi[k];
+        const bool inr = valid && ro.rmax - ro.rmin > 0 && ro.cmax - ro.cmin > 0 &&
+                         row >= ro.rmin && row < ro.rmax && col >= ro.cmin && col < ro.cmax;
+        const int aidx = inr ? (row - ro.rmin) * s.roiw[k] + (col - ro.cmin) : 0;
+        T av = T(0);
+#pragma unroll
+        for (int sl = 0; sl < SLOTS; ++sl) {
+          const T got = __shfl(s.aug[k][sl], aidx % kWave, kWave);
+          if (aidx / kWave == sl) av = got;
+        }
+        if (valid) {
+          meas = inr ? av : b.res[k * HW + (long)row * b.W + col];
+          pred = gauss_int<T>(P[k], uv[2 * k], uv[2 * k + 1], row, col);
+        }
+        const T mpos = meas > T(0) ? meas : T(0);
+        pk = fmax(pk, valid ? meas : T(0));
+        nm += valid ? mpos : T(0);
+        dn += valid ? pred : T(0);
+      }
+      peak[k] = wave_max(pk);
+      num[k] = wave_sum(nm);
+      den[k] = wave_sum(dn);
+    }
+    T score;
+    if (n_used < 2) {
+      score = T(1e-12);
+    } else {
+      if (n_used > 2) {
+        int mi = -1;
+        T mv = -T(INFINITY);
+        for (int k = 0; k < NC && k < nc; ++k)
+          if (use[k] && peak[k] > mv) {
+            mv = peak[k];
+            mi = k;
+          }
+        if (mi >= 0) use[mi] = 0;
+      }
+      T tn = T(0), td = T(0);
+      for (int k = 0; k < NC && k < nc; ++k)
+        if (use[k]) {
+          tn += num[k];
+          td += den[k];
+        }
+      T ratio = (td > T(1e-12)) ? (tn / td) : T(1);
+      ratio = fmax(T(2) / T(3), fmin(T(1.5), ratio));
+      score = sqrt(ratio);
+    }
+    if (lane == 0) {
+      b.xyz[3L * i] = (double)X[0];
+      b.xyz[3L * i + 1] = (double)X[1];
+      b.xyz[3L * i + 2] = (double)X[2];
+      b.score[i] = b.score[i] * (double)score;
+      for (int k = 0; k < nc && k < NC; ++k) {
+        b.uv[((long)i * nc + k) * 2] = uv[2 * k];
+        b.uv[((long)i * nc + k) * 2 + 1] = uv[2 * k + 1];
+      }
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// repeated objects + ghost threshold
+// --------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ int grid_cell_of(const ShakeBuffers<T> &b, const double *X,
+                                            int d) {
+  int c = (int)floor((X[d] - b.grid_lo[d]) / b.grid_cell);
+  return c < 0 ? 0 : (c >= b.grid_n ? b.grid_n - 1 : c);
+}
+
+template <typename T>
+__global__ void k_grid_count(ShakeBuffers<T> b) {
+  const int n = *b.n_obj;
+  const int G = b.grid_n;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double *X = b.xyz + 3L * i;
+    const int cell = (grid_cell_of(b, X, 0) * G + grid_cell_of(b, X, 1)) * G + grid_cell_of(b, X, 2);
+    b.g_cellof[i] = cell;
+    b.g_rank[i] = atomicAdd(&b.g_cnt[cell], 1);
+  }
+}
+
+template <typename T>
+__global__ void k_grid_fill(ShakeBuffers<T> b) {
+  const int n = *b.n_obj;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    b.g_obj[b.g_start[b.g_cellof[i]] + b.g_rank[i]] = i;
+}
+
+// nanoflann L2_Simple_Adaptor distance with diff = query - point
+__device__ __forceinline__ double dist2(const double *a, const double *c) {
+  const double d0 = a[0] - c[0], d1 = a[1] - c[1], d2 = a[2] - c[2];
+  return d0 * d0 + d1 * d1 + d2 * d2;
+}
+
+template <typename T>
+__device__ __forceinline__ int count_nb(const ShakeBuffers<T> &b, int i, double t2,
+                                        int8_t *mark, int mark_val) {
+  const double *X = b.xyz + 3L * i;
+  const int G = b.grid_n;
+  const int cx = grid_cell_of(b, X, 0), cy = grid_cell_of(b, X, 1), cz = grid_cell_of(b, X, 2);
+  int found = 0;
+  for (int dx = -1; dx <= 1; ++dx)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dz = -1; dz <= 1; ++dz) {
+        const int x = cx + dx, y = cy + dy, z = cz + dz;
+        if (x < 0 || y < 0 || z < 0 || x >= G || y >= G || z >= G) continue;
+        const int cell = (x * G + y) * G + z;
+        const int s0 = b.g_start[cell], ncell = b.g_cnt[cell];
+        for (int t = 0; t < ncell; ++t) {
+          const int j = b.g_obj[s0 + t];
+          if (j == i) continue;
+          if (dist2(X, b.xyz + 3L * j) < t2) {
+            ++found;
+            if (mark) mark[j] = (int8_t)mark_val;
+          }
+        }
+      }
+  return found;
+}
+
+template <typename T>
+__global__ void k_has_nb(ShakeBuffers<T> b, double t2) {
+  const int n = *b.n_obj;
+  // entries past n are cleared: has_nb is scanned over cap_obj
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.cap_obj;
+       i += gridDim.x * blockDim.x) {
+    if (i >= n) {
+      b.has_nb[i] = 0;
+      continue;
+    }
+    b.has_nb[i] = count_nb(b, i, t2, nullptr, 0) > 0 ? 1 : 0;
+    b.flag[i] = OLPT_FLAG_NONE;
+  }
+}
+
+// greedy pass (index order) over the objects that have a neighbour; the
+// last object is never a query (Shake.cpp:797: i < n_obj - 1).  Marks are
+// written as OLPT_FLAG_REPEATED into flag[].
+template <typename T>
+__global__ void k_repeat_greedy(ShakeBuffers<T> b, double t2) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int n = *b.n_obj;
+  const int m = b.totals[0];
+  int8_t *rep = (int8_t *)b.flag;
+  for (int q = 0; q < m; ++q) {
+    const int i = b.keep_off[q];
+    if (i >= n - 1) break;
+    if (rep[i]) continue;
+    count_nb(b, i, t2, rep, OLPT_FLAG_REPEATED);
+  }
+}
+
+// compact indices of objects with neighbours (order preserved)
+template <typename T>
+__global__ void k_nb_list(ShakeBuffers<T> b) {
+  const int n = *b.n_obj;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (b.has_nb[i]) b.keep_off[b.g_rank[i]] = i;
+}
+
+// deterministic two-level reduction of the non-repeated scores
+constexpr int kRedThreads = 256;
+constexpr int kRedBlocks = 256;
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) k_score_sum(ShakeBuffers<T> b) {
+  __shared__ double sd[kRedThreads];
+  __shared__ int sc[kRedThreads];
+  const int n = *b.n_obj;
+  double s = 0.0;
+  int c = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (b.flag[i] == OLPT_FLAG_NONE) {
+      s += b.score[i];
+      ++c;
+    }
+  sd[threadIdx.x] = s;
+  sc[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = kRedThreads / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      sd[threadIdx.x] += sd[threadIdx.x + o];
+      sc[threadIdx.x] += sc[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    b.partial[2 * blockIdx.x] = sd[0];
+    b.partial[2 * blockIdx.x + 1] = (double)sc[0];
+  }
+}
+
+// Ghost: score < thred_ghost * mean(non-repeated) or score == 0
+// (Shake.cpp:118-136); keep_off <- 1 for unflagged objects.
+template <typename T>
+__global__ void k_flags(ShakeBuffers<T> b, double thr) {
+  double sum = 0.0, cnt = 0.0;
+  for (int k = 0; k < kRedBlocks; ++k) {
+    sum += b.partial[2 * k];
+    cnt += b.partial[2 * k + 1];
+  }
+  const double mean = cnt > 0 ? sum / cnt : 0.0;
+  const int n = *b.n_obj;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint8_t f = b.flag[i];
+    const double sc = b.score[i];
+    if (sc < mean * thr || sc == 0.0) f |= OLPT_FLAG_GHOST;
+    b.flag[i] = f;
+  }
+}
+
+template <typename T>
+__global__ void k_init_score(ShakeBuffers<T> b) {
+  const int n = *b.n_obj;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    b.score[i] = 1.0;
+    b.flag[i] = 0;
+  }
+}
+
+template <typename T>
+__global__ void k_zero_score(ShakeBuffers<T> b) {
+  const int n = *b.n_obj;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    b.score[i] = 0.0;
+}
+
+template <typename T, int NC, int SLOTS>
+static void shake_iter(const ShakeBuffers<T> &b, const ShakeParams &p,
+                       const ObjCamRec<T> *rec, T delta, hipStream_t st) {
+  k_shake<T, NC, SLOTS><<<256 * 16, kWave, 0, st>>>(b, p, rec, delta);
+}
+
+template <typename T>
+void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st) {
+  const int nc = b.n_cam;
+  CamList act{0, {0}};
+  for (int k = 0; k < nc; ++k)
+    if (p.active[k]) act.id[act.n++] = k;
+  CamList inact{0, {0}};
+  for (int k = 0; k < nc; ++k)
+    if (!p.active[k]) inact.id[inact.n++] = k;
+  ObjCamRec<T> *rec = reinterpret_cast<ObjCamRec<T> *>(b.rec);
+  const T r_px = (T)p.r_px;
+  const int roi_w = (int)lround(2.0 * p.r_px * p.ratio_augimg);
+  const int roi_px = roi_w * roi_w;
+  const int gO = grid_for(b.cap_obj, 256, 2048);
+  k_init_score<T><<<gO, 256, 0, st>>>(b);
+  double delta = p.shake_width;
+  double dmin = p.shakewidth_min > 0 ? p.shakewidth_min : delta / 20;
+  for (int it = 0; it < p.n_shake_loop; ++it) {
+    // residual from the start-of-iteration positions (active cameras); the
+    // projection pass also refreshes uv (runShake reprojects first)
+    residual_pass<T>(b, b.img, b.res, b.xyz, b.n_obj, rec, b.uv, act, 0, r_px, st);
+    if (nc < 2) {
+      k_zero_score<T><<<gO, 256, 0, st>>>(b);
+    } else if (nc <= 4 && roi_px <= kWave) {
+      shake_iter<T, 4, 1>(b, p, rec, (T)delta, st);
+    } else if (nc <= 4 && roi_px <= 4 * kWave) {
+      shake_iter<T, 4, 4>(b, p, rec, (T)delta, st);
+    } else if (roi_px <= kWave) {
+      shake_iter<T, 8, 1>(b, p, rec, (T)delta, st);
+    } else {
+      shake_iter<T, 8, 4>(b, p, rec, (T)delta, st);
+    }
+    delta *= 0.5;
+    if (delta < dmin) delta = dmin;
+  }
+  if (p.n_shake_loop <= 0) {
+    k_obj_project<T><<<gO, 256, 0, st>>>(b, b.xyz, b.n_obj, rec, b.uv, r_px);
+  }
+  // markRepeatedObj
+  const double t2 = p.tol3d * p.tol3d;
+  const long ncell = (long)b.grid_n * b.grid_n * b.grid_n;
+  zero_i32(b.g_cnt, ncell, st);
+  k_grid_count<T><<<gO, 256, 0, st>>>(b);
+  scan_exclusive(b.g_cnt, b.g_start, ncell, 1, ncell, nullptr, b.scan_scratch, st);
+  k_grid_fill<T><<<gO, 256, 0, st>>>(b);
+  k_has_nb<T><<<gO, 256, 0, st>>>(b, t2);
+  scan_exclusive(b.has_nb, b.g_rank, b.cap_obj, 1, b.cap_obj, b.totals, b.scan_scratch, st);
+  k_nb_list<T><<<gO, 256, 0, st>>>(b);
+  k_repeat_greedy<T><<<1, 64, 0, st>>>(b, t2);
+  // ghost threshold: mean over non-repeated scores
+  k_score_sum<T><<<kRedBlocks, kRedThreads, 0, st>>>(b);
+  k_flags<T><<<gO, 256, 0, st>>>(b, p.thred_ghost);
+  (void)inact;
+}
+
+// --------------------------------------------------------------------------
+// filter + IPR-mode residual
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ void k_keep_write(ShakeBuffers<T> b, const int32_t *__restrict__ keep,
+                             const int32_t *__restrict__ off, double *xyz_keep,
+                             T *uv_keep) {
+  const int n = *b.n_obj;
+  const int nc = b.n_cam;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (!keep[i]) continue;
+    const int o = off[i];
+    for (int d = 0; d < 3; ++d) xyz_keep[3L * o + d] = b.xyz[3L * i + d];
+    for (int k = 0; k < nc; ++k) {
+      uv_keep[((long)o * nc + k) * 2] = b.uv[((long)i * nc + k) * 2];
+      uv_keep[((long)o * nc + k) * 2 + 1] = b.uv[((long)i * nc + k) * 2 + 1];
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_keep_flags(ShakeBuffers<T> b) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.cap_obj;
+       i += gridDim.x * blockDim.x)
+    b.has_nb[i] = (i < *b.n_obj && b.flag[i] == OLPT_FLAG_NONE) ? 1 : 0;
+}
+
+template <typename T>
+__global__ void k_keep_count(const int32_t *tot, int32_t *n_keep) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *n_keep = tot[0];
+}
+
+template <typename T>
+void launch_filter(const ShakeBuffers<T> &b, double *xyz_keep, T *uv_keep,
+                   int32_t *n_keep, hipStream_t st) {
+  const int gO = grid_for(b.cap_obj, 256, 2048);
+  k_keep_flags<T><<<gO, 256, 0, st>>>(b);
+  scan_exclusive(b.has_nb, b.keep_off, b.cap_obj, 1, b.cap_obj, b.totals,
+                 b.scan_scratch, st);
+  k_keep_write<T><<<gO, 256, 0, st>>>(b, b.has_nb, b.keep_off, xyz_keep, uv_keep);
+  k_keep_count<T><<<1, 64, 0, st>>>(b.totals, n_keep);
+}
+
+template <typename T>
+void launch_residual_ipr(const ShakeBuffers<T> &b, const int32_t *n_keep,
+                         const double *xyz_keep, T *uv_keep, double r_px,
+                         hipStream_t st) {
+  CamList all{0, {0}};
+  for (int k = 0; k < b.n_cam; ++k) all.id[all.n++] = k;
+  ObjCamRec<T> *rec = reinterpret_cast<ObjCamRec<T> *>(b.rec);
+  residual_pass<T>(b, b.img, b.img_next, xyz_keep, n_keep, rec, uv_keep, all, 1,
+                   (T)r_px, st);
+}
+
+size_t shake_rec_bytes(bool f64) {
+  return f64 ? sizeof(ObjCamRec<double>) : sizeof(ObjCamRec<float>);
+}
+
+#define INST(T)                                                                   \
+  template void launch_shake<T>(const ShakeBuffers<T> &, const ShakeParams &,     \
+                                hipStream_t);                                     \
+  template void launch_residual_ipr<T>(const ShakeBuffers<T> &, const int32_t *,  \
+                                       const double *, T *, double, hipStream_t); \
+  template void launch_filter<T>(const ShakeBuffers<T> &, double *, T *,          \
+                                 int32_t *, hipStream_t);
+INST(float)
+INST(double)
+#undef INST
+
+} // namespace olpt
