@@ -1,0 +1,221 @@
+#!/usr/bin/env python3
+"""IPR throughput benchmark on one MI355X.
+
+Workload: the ppp = 0.05 configuration - 4 pinhole cameras, 1024 x 1024
+uint16 images, 52,429 particles per frame drawn from 8 Gaussian clusters
+(sigma 5 mm) in a 40 mm cube, OTF a ~ U(800, 1200), b = c = 1.2, noise
+sigma 10; 4 IPR loops x 6 shake iterations.  A "step" runs IPR on a batch of
+frames (default 8) resident in HBM; value = reconstructed particles per
+second over the timed steps.
+
+One JSON line is printed on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "IPR particles/s at ppp=0.05, learned vs plain shake at equal 3D RMSE"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_frames(n_frames: int, ppp: float, size: int):
+    from pyopenlpt_amd import synth as S
+    frames = [S.workload_c3_frame(f, ppp=ppp, size=size) for f in range(n_frames)]
+    return frames
+
+
+def quality(xyz: np.ndarray, truth: np.ndarray, vox: float) -> dict:
+    """find_ghosts semantics (demo/openLPT_utils/ghost_eval.py:72-92): a
+    reconstruction within 1 voxel of a ground-truth particle is real."""
+    from scipy.spatial import cKDTree
+    if len(xyz) == 0:
+        return dict(rmse_mm=None, recon_rate=0.0, ghost_rate=None)
+    d, i = cKDTree(truth).query(xyz)
+    real = d < vox
+    return dict(rmse_mm=float(np.sqrt(np.mean(d[real] ** 2))) if real.any() else None,
+                recon_rate=float(len(np.unique(i[real])) / len(truth)),
+                ghost_rate=float(1.0 - real.mean()))
+
+
+CPU_CHILD = r"""
+import ctypes, json, os, sys, time
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import oracle as O
+from pyopenlpt_amd import synth as S
+ppp, size, kind = float(sys.argv[2]), int(sys.argv[3]), sys.argv[4]
+w = S.workload_c3_frame(0, ppp=ppp, size=size)
+H, W = w.images.shape[1:]
+cams = O.cams_array(w.cams)
+otf, keep = S.make_otf(4, w.otf_abca)
+imgs = w.images.astype(np.float64).copy()
+cap = 1 << 19
+ox = np.zeros((cap, 3)); ou = np.zeros((cap, 4, 2)); lt = np.zeros(64, np.int32)
+cnt = np.zeros(2, np.int32)
+if kind == "reference":
+    lib = O.load_ref(); fn = lib.ref_run_ipr
+else:
+    lib = O.load_port(); fn = lib.orc_run_ipr
+t = time.perf_counter()
+rc = fn(cams, 4, O.dptr(imgs), H, W, ctypes.byref(w.params), ctypes.byref(otf), cap,
+        O.dptr(ox), O.dptr(ou), 64, O.iptr(lt), O.iptr(cnt))
+el = time.perf_counter() - t
+print(json.dumps(dict(rc=int(rc), n=int(cnt[0]), seconds=el)))
+"""
+
+
+def cpu_baseline(ppp: float, size: int, threads: int) -> dict | None:
+    import oracle as O
+    kind = "reference" if O.ref_available() else "port"
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+    log(f"cpu baseline: {kind} IPR, 1 frame, {threads} threads")
+    r = subprocess.run([sys.executable, "-c", CPU_CHILD, ROOT, str(ppp), str(size), kind],
+                       capture_output=True, text=True, env=env, timeout=900)
+    if r.returncode != 0:
+        log("cpu baseline failed:", r.stderr[-2000:])
+        return None
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"value": round(res["n"] / res["seconds"], 1), "unit": "particles/s",
+            "cores": threads, "kind": kind,
+            "sample": (f"1 frame of the same workload (frame 0, {res['n']} objects, "
+                       f"{res['seconds']:.2f} s wall, IPR::runIPR only)")}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--frames", type=int, default=8, help="frames per step")
+    ap.add_argument("--ppp", type=float, default=0.05)
+    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--graph", action="store_true", help="replay a captured hipGraph")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); frames are "
+              "independent, so N GPUs would each run their own batch.", file=sys.stderr)
+        return 2
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args.ppp, args.size, args.cpu_threads)
+
+    import torch
+    from pyopenlpt_amd import synth as S
+    from pyopenlpt_amd.context import Context, to_device_image
+
+    log(f"generating {args.frames} frames (ppp {args.ppp}, {args.size}^2)")
+    frames = make_frames(args.frames, args.ppp, args.size)
+    w0 = frames[0]
+    H, W = w0.images.shape[1:]
+    batch = np.stack([f.images for f in frames])          # [F][4][H][W] uint16
+    dimg, code = to_device_image(batch)
+    otf, _keep = S.make_otf(4, w0.otf_abca)
+    ctx = Context(w0.cams, H, W, precision=args.precision)
+    ctx.set_otf(otf)
+    cap_obj = 1 << 17
+    outs = ctx.alloc_outputs(args.frames, cap_obj, dimg.device)
+
+    def step():
+        if args.graph:
+            ctx.graph_launch()
+        else:
+            ctx.ipr_run(dimg, code, w0.params, n_frames=args.frames, cap_obj=cap_obj,
+                        outputs=outs)
+
+    if args.graph:
+        ctx.graph_capture(dimg, code, w0.params, args.frames, outs, cap_obj)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # algorithmic bytes per stage launch (instrumented extra run, untimed)
+    ctx.stage_stats(True)
+    ctx.ipr_run(dimg, code, w0.params, n_frames=args.frames, cap_obj=cap_obj, outputs=outs)
+    stats = ctx.stage_stats(False)
+
+    ctx.stage_timing(not args.graph)
+    ctx.stage_times()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = e0.elapsed_time(e1)
+    times = ctx.stage_times() if not args.graph else {}
+    ms_per_step = ms_total / args.steps
+    n_obj = outs[2].cpu().numpy()
+    per_step = int(n_obj.sum())
+    value = per_step / (ms_per_step / 1000.0)
+    log(f"{args.steps} steps: {ms_per_step:.2f} ms/step (wall {1000 * wall / args.steps:.2f}), "
+        f"{per_step} particles/step -> {value:.0f} particles/s")
+
+    # quality over the batch
+    q = [quality(outs[0][f, : n_obj[f]].cpu().numpy(), frames[f].truth, frames[f].voxel_mm)
+         for f in range(args.frames)]
+    rmse = [x["rmse_mm"] for x in q if x["rmse_mm"] is not None]
+    qual = dict(rmse_mm=float(np.mean(rmse)) if rmse else None,
+                rmse_px=float(np.mean(rmse) / w0.voxel_mm) if rmse else None,
+                recon_rate=float(np.mean([x["recon_rate"] for x in q])),
+                ghost_rate=float(np.mean([x["ghost_rate"] for x in q])))
+
+    roofline = None
+    if times:
+        dom = max((k for k in times if times[k][1] > 0), key=lambda k: times[k][0])
+        ms_dom, calls = times[dom]
+        bytes_dom, launches = stats[dom]
+        if calls and launches:
+            per_launch_bytes = bytes_dom / launches
+            avg_ms = ms_dom / calls
+            achieved = per_launch_bytes / (avg_ms / 1000.0) / 1e9
+            roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                        "bytes_per_launch": int(per_launch_bytes),
+                        "avg_launch_ms": round(avg_ms, 4),
+                        "stage_ms_per_step": {k: round(v[0] / args.steps, 3)
+                                              for k, v in times.items() if v[1]}}
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "particles/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": args.precision,
+        "data": "synthetic",
+        "config": {"workload": f"ipr_ppp{args.ppp}_{args.size}sq_u16_4cam_clustered",
+                   "frames_per_step": args.frames, "particles_per_frame": len(w0.truth),
+                   "ipr_loops": int(w0.params.n_loop_ipr),
+                   "shake_loops": int(w0.params.n_shake_loop),
+                   "shake": "plain", "graph": bool(args.graph),
+                   "quality": qual},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

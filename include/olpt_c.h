@@ -64,6 +64,20 @@ const char *olpt_last_error(void);
  * exhausted, [6] checkMatch triangulation failures, [7] peak overflow. */
 olpt_status olpt_diagnostics(olpt_ctx *ctx, int32_t out[8]);
 
+/* Stage timing with HIP events on the context stream (eager runs only).
+ * Slots: 0 peaks, 1 stereo build, 2 sort+prune+triangulate, 3 shake
+ * kernel, 4 residual gather, 5 other.  olpt_stage_times returns the summed
+ * milliseconds and launch counts since the last read and resets them. */
+enum { OLPT_STAGE_PEAKS = 0, OLPT_STAGE_BUILD = 1, OLPT_STAGE_PRUNE = 2,
+       OLPT_STAGE_SHAKE = 3, OLPT_STAGE_RESIDUAL = 4, OLPT_STAGE_OTHER = 5 };
+olpt_status olpt_stage_timing(olpt_ctx *ctx, int enable);
+olpt_status olpt_stage_times(olpt_ctx *ctx, double ms[8], int32_t calls[8]);
+/* Algorithmic-byte accounting (synchronises once per IPR loop, so only for
+ * instrumented runs): returns the bytes and launch counts accumulated since
+ * the previous call, resets them, and switches accounting on or off. */
+olpt_status olpt_stage_stats(olpt_ctx *ctx, int enable, double bytes[8],
+                             int32_t launches[8]);
+
 /* ---- stages ---------------------------------------------------------- */
 /* ObjectFinder2D::findObject2D, tracer branch (ObjectFinder.cpp:37-111).
  * One image (dtype OLPT_U8..F64, row-major n_row x n_col).  Writes (x, y)

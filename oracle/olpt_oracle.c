@@ -247,6 +247,88 @@ static void gauss_inverse3(const double A[9], double out[9]) {
   }
 }
 
+/* myMATH::gaussInverse for n x n (myMATH.h:183-289), row-major in/out */
+static void gauss_inverse_n(const double *A, int n, double *out) {
+  double *u = (double *)malloc(sizeof(double) * (size_t)n * n);
+  double *b = (double *)malloc(sizeof(double) * (size_t)n * n);
+  double *mr = (double *)malloc(sizeof(double) * (size_t)n);
+  int *idx = (int *)malloc(sizeof(int) * (size_t)n);
+  memcpy(u, A, sizeof(double) * (size_t)n * n);
+  for (int i = 0; i < n; ++i) {
+    idx[i] = i;
+    double m = u[(size_t)i * n];
+    for (int j = 1; j < n; ++j)
+      if (u[(size_t)i * n + j] > m) m = u[(size_t)i * n + j];
+    mr[i] = m;
+  }
+  for (int y = 0; y < n - 1; ++y) {
+    double maxp = 0;
+    int t = 0;
+    for (int x = y; x < n; ++x) {
+      double pv = fabs(u[(size_t)idx[x] * n + y] / mr[idx[x]]);
+      if (pv > maxp) {
+        maxp = pv;
+        t = x;
+      }
+    }
+    int tmp = idx[t];
+    idx[t] = idx[y];
+    idx[y] = tmp;
+    for (int x = y + 1; x < n; ++x) {
+      double m = u[(size_t)idx[x] * n + y] / u[(size_t)idx[y] * n + y];
+      u[(size_t)idx[x] * n + y] = m;
+      for (int it = y + 1; it < n; ++it)
+        u[(size_t)idx[x] * n + it] -= m * u[(size_t)idx[y] * n + it];
+    }
+  }
+  for (int i = 0; i < n * n; ++i) b[i] = (i % (n + 1) == 0) ? 1.0 : 0.0;
+  memcpy(out, b, sizeof(double) * (size_t)n * n);
+  for (int col = 0; col < n; ++col) {
+    for (int y = 0; y < n - 1; ++y)
+      for (int x = y + 1; x < n; ++x)
+        b[(size_t)idx[x] * n + col] -= u[(size_t)idx[x] * n + y] * b[(size_t)idx[y] * n + col];
+    out[(size_t)(n - 1) * n + col] = b[(size_t)idx[n - 1] * n + col] / u[(size_t)idx[n - 1] * n + n - 1];
+    for (int x = n - 2; x > -1; --x) {
+      double z = b[(size_t)idx[x] * n + col];
+      for (int y = x + 1; y < n; ++y) z -= u[(size_t)idx[x] * n + y] * out[(size_t)y * n + col];
+      out[(size_t)x * n + col] = z / u[(size_t)idx[x] * n + x];
+    }
+  }
+  free(u);
+  free(b);
+  free(mr);
+  free(idx);
+}
+
+/* myMATH::polyfit of any order (myMATH.cpp:490-534) */
+static void polyfit_n(const double *x, const double *y, int n, int order, double *c) {
+  const int m = order + 1;
+  double *X = (double *)malloc(sizeof(double) * (size_t)n * m);
+  double *XtX = (double *)malloc(sizeof(double) * (size_t)m * m);
+  double *inv = (double *)malloc(sizeof(double) * (size_t)m * m);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < m; ++j) X[(size_t)i * m + j] = pow(x[i], (double)j);
+  for (int a = 0; a < m; ++a)
+    for (int b = 0; b < m; ++b) {
+      double s = 0;
+      for (int i = 0; i < n; ++i) s += X[(size_t)i * m + a] * X[(size_t)i * m + b];
+      XtX[a * m + b] = s;
+    }
+  gauss_inverse_n(XtX, m, inv);
+  for (int a = 0; a < m; ++a) {
+    double s = 0;
+    for (int i = 0; i < n; ++i) {
+      double t = 0;
+      for (int k = 0; k < m; ++k) t += inv[a * m + k] * X[(size_t)i * m + k];
+      s += t * y[i];
+    }
+    c[a] = s;
+  }
+  free(X);
+  free(XtX);
+  free(inv);
+}
+
 /* myMATH::triangulation (myMATH.cpp:357-419).  Returns -1 where the
  * reference throws (h < -1e-8). */
 static int triangulate(const line3 *const *L, int n, double X[3],
@@ -1796,9 +1878,21 @@ int orc_triangulate(const double *lines, int n_line, double *xyz, double *err) {
 
 int orc_polyfit(const double *x, const double *y, int n, int order,
                 double *coeff) {
-  if (order != 2) return ERR_UNSUP;
-  polyfit2(x, y, n, coeff);
+  if (order < 1) return ERR_ARG;
+  if (order == 2)
+    polyfit2(x, y, n, coeff);
+  else
+    polyfit_n(x, y, n, order, coeff);
   return OK;
+}
+
+int orc_trilinear(const double *lim, const double *v, const double *p, double *out) {
+  *out = trilinear(lim, v, p);
+  return OK;
+}
+
+int orc_is_local_max(const double *img, int H, int W, int row, int col) {
+  return is_local_max(img, H, W, row, col);
 }
 
 int orc_otf_param(const olpt_otf *otf, int cam_id, const double *xyz, int n,

@@ -27,6 +27,8 @@ int orc_line_of_sight(const olpt_camera *cam, const double *uv, int n,
 int orc_triangulate(const double *lines, int n_line, double *xyz, double *err);
 int orc_polyfit(const double *x, const double *y, int n, int order,
                 double *coeff);
+int orc_trilinear(const double *lim, const double *v, const double *p, double *out);
+int orc_is_local_max(const double *img, int H, int W, int row, int col);
 int orc_otf_param(const olpt_otf *otf, int cam_id, const double *xyz, int n,
                   double *out);
 int orc_find_peaks(const double *img, int H, int W, const olpt_ipr_params *p,

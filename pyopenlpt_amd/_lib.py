@@ -77,12 +77,16 @@ def lib() -> ctypes.CDLL:
                                          ctypes.POINTER(OlptIprParams), _P, _P, _I,
                                          _P]
     L.olpt_ipr_graph_launch.argtypes = [_P]
+    L.olpt_stage_timing.argtypes = [_P, _I]
+    L.olpt_stage_times.argtypes = [_P, _DP, _I32P]
+    L.olpt_stage_stats.argtypes = [_P, _I, _DP, _I32P]
     for name in ("olpt_ctx_create", "olpt_ctx_destroy", "olpt_ctx_reserve",
                  "olpt_set_precision", "olpt_set_cameras", "olpt_set_otf",
                  "olpt_sync", "olpt_diagnostics", "olpt_find_peaks",
                  "olpt_stereo_match", "olpt_shake", "olpt_residual",
                  "olpt_ipr_run", "olpt_ipr_graph_capture",
-                 "olpt_ipr_graph_launch"):
+                 "olpt_ipr_graph_launch", "olpt_stage_timing",
+                 "olpt_stage_times", "olpt_stage_stats"):
         getattr(L, name).restype = ctypes.c_int32
     _lib = L
     return L

@@ -105,6 +105,36 @@ class Context:
         check(self.L.olpt_diagnostics(self.h, out.ctypes.data_as(_lib._I32P)))
         return out
 
+    STAGES = ("peaks", "stereo_build", "prune", "shake", "residual", "other")
+
+    def stage_timing(self, enable: bool = True):
+        check(self.L.olpt_stage_timing(self.h, 1 if enable else 0))
+
+    def stage_times(self) -> dict:
+        ms = np.zeros(8)
+        calls = np.zeros(8, np.int32)
+        check(self.L.olpt_stage_times(self.h, ms.ctypes.data_as(_lib._DP),
+                                      calls.ctypes.data_as(_lib._I32P)))
+        return {name: (float(ms[i]), int(calls[i])) for i, name in enumerate(self.STAGES)}
+
+    def stage_stats(self, enable: bool) -> dict:
+        b = np.zeros(8)
+        n = np.zeros(8, np.int32)
+        check(self.L.olpt_stage_stats(self.h, 1 if enable else 0,
+                                      b.ctypes.data_as(_lib._DP),
+                                      n.ctypes.data_as(_lib._I32P)))
+        return {name: (float(b[i]), int(n[i])) for i, name in enumerate(self.STAGES)}
+
+    def graph_capture(self, imgs: torch.Tensor, dtype: int, params: OlptIprParams,
+                      n_frames: int, outputs, cap_obj: int):
+        ox, ouv, n_obj = outputs
+        check(self.L.olpt_ipr_graph_capture(self.h, ptr(imgs), dtype, n_frames,
+                                            ctypes.byref(params), ptr(ox), ptr(ouv),
+                                            cap_obj, ptr(n_obj)))
+
+    def graph_launch(self):
+        check(self.L.olpt_ipr_graph_launch(self.h))
+
     # -- stages ----------------------------------------------------------
     def find_peaks(self, img: torch.Tensor, dtype: int, params: OlptIprParams,
                    cap: int = 1 << 18) -> np.ndarray:

@@ -18,6 +18,10 @@
 
 using namespace olpt;
 
+namespace olpt {
+thread_local StageTimer *g_timer = nullptr;
+}
+
 namespace {
 
 thread_local std::string g_err;
@@ -82,6 +86,11 @@ struct olpt_ctx {
   int32_t *totals = nullptr; // [64]
   int32_t *loop_tot = nullptr; // [n_frames*cap_loop] device
   int loop_tot_cap = 0;
+  // stage timing and algorithmic-byte accounting
+  StageTimer *timer = nullptr;
+  bool stats = false;
+  double st_bytes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int32_t st_launch[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -102,6 +111,23 @@ olpt_status dalloc(olpt_ctx *c, void **p, size_t bytes) {
 template <typename P> olpt_status dalloc_t(olpt_ctx *c, P **p, size_t count) {
   return dalloc(c, reinterpret_cast<void **>(p), count * sizeof(P));
 }
+
+void timer_free(olpt_ctx *c) {
+  if (!c->timer) return;
+  for (int a = 0; a < 2; ++a)
+    for (int s = 0; s < 8; ++s)
+      for (int i = 0; i < 512; ++i) (void)hipEventDestroy(c->timer->ev[a][s][i]);
+  delete c->timer;
+  c->timer = nullptr;
+}
+
+// RAII: route stage events of this thread to the context's timer
+struct TimerScope {
+  TimerScope(olpt_ctx *c, bool capturing) {
+    g_timer = (!capturing && c->timer && c->timer->on) ? c->timer : nullptr;
+  }
+  ~TimerScope() { g_timer = nullptr; }
+};
 
 void free_all(olpt_ctx *c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -381,6 +407,39 @@ olpt_status ipr_loop(olpt_ctx *c, T *img, T *img_next, const int32_t *active,
     k_append_count<<<1, 1, 0, st>>>(c->n_keep, cap_out, d_n_out, d_loop_tot, c->flags);
   }
   HIP_TRY(hipGetLastError());
+  if (c->stats) {
+    // Algorithmic bytes per launch (what each stage must move at minimum):
+    //  peaks:    read the loop image of the active cameras + write the peaks
+    //  build:    read 2D points + LOS of the active cameras + the cell CSR,
+    //            write the checked candidates
+    //  prune:    read/write the candidates twice + point lists + objects
+    //  shake:    per iteration, the N x C x P^2 patch gathers of the
+    //            reference ROI (P = 2 round(r ratio)) + positions
+    //  residual: read the loop image and write the residual per camera
+    int32_t np[OLPT_MAX_CAM], cnt[3];
+    HIP_TRY(hipMemcpyAsync(np, c->npts, sizeof(np), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&cnt[0], c->sb.n_cand, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&cnt[1], c->sb.n_obj, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&cnt[2], c->n_keep, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    double n2d = 0;
+    for (int k = 0; k < c->n_cam; ++k) n2d += np[k];
+    const double s_t = sizeof(T);
+    const double na = cl.n, nc = c->n_cam;
+    const double ncell = (double)c->sb.rows_cell * c->sb.cols_cell;
+    const double M = cnt[0], N = cnt[1];
+    const double P = std::round(2.0 * p->radius_obj * p->ratio_augimg);
+    c->st_bytes[ST_PEAKS] += na * HW * s_t + n2d * 16.0;
+    c->st_launch[ST_PEAKS] += 1;
+    c->st_bytes[ST_BUILD] += n2d * 48.0 + na * ncell * 8.0 + M * (4.0 * nc + 8.0);
+    c->st_launch[ST_BUILD] += 1;
+    c->st_bytes[ST_PRUNE] += 2.0 * M * (4.0 * nc + 8.0) + 2.0 * M * na * 4.0 + N * (24.0 + 4.0 * nc);
+    c->st_launch[ST_PRUNE] += 1;
+    c->st_bytes[ST_SHAKE] += p->n_shake_loop * (N * na * P * P * s_t + N * 48.0);
+    c->st_launch[ST_SHAKE] += p->n_shake_loop;
+    c->st_bytes[ST_RESID] += p->n_shake_loop * (na * HW * 2.0 * s_t) + nc * HW * 2.0 * s_t;
+    c->st_launch[ST_RESID] += p->n_shake_loop + 1;
+  }
   return OLPT_OK;
 }
 
@@ -458,6 +517,7 @@ olpt_status olpt_ctx_create(int device, void *stream, olpt_ctx **out) {
 olpt_status olpt_ctx_destroy(olpt_ctx *c) {
   if (!c) return OLPT_OK;
   (void)hipSetDevice(c->device);
+  timer_free(c);
   if (c->st)
     (void)hipStreamSynchronize(c->st);
   else
@@ -594,6 +654,11 @@ olpt_status olpt_ctx_reserve(olpt_ctx *c, int n_cam, int H, int W, int max_2d, i
   if ((s = dalloc(c, &c->rec, shake_rec_bytes(true) * (size_t)O * nc))) return s;
   A(c->partial, double, 1024);
 #undef A
+  // device counters and flags start at zero (fresh allocations may hold
+  // another context's old values)
+  fill_i32(c->flags, 16, 0, nullptr);
+  fill_i32(c->npts, OLPT_MAX_CAM, 0, nullptr);
+  HIP_TRY(hipDeviceSynchronize());
   // re-upload cameras / OTF if they were set before
   if (c->cams_set) {
     CamT<double> h64[OLPT_MAX_CAM];
@@ -714,7 +779,7 @@ olpt_status olpt_stereo_match(olpt_ctx *c, const int32_t *active, const double *
   HIP_TRY(hipMemcpyAsync(c->npts, np, sizeof(np), hipMemcpyHostToDevice, st));
   StereoParams sp = stereo_params(c, active, p, p->tol_2d_px, &s);
   if (s) return s;
-  fill_i32(c->flags, 8, 0, st);
+  fill_i32(c->flags, 16, 0, st);
   launch_stereo(c->sb, sp, kMaxRounds, st);
   HIP_TRY(hipGetLastError());
   int32_t h[3];
@@ -818,6 +883,7 @@ olpt_status ipr_run_impl(olpt_ctx *c, const void *d_imgs, int dtype, int n_frame
                          const olpt_ipr_params *p, double *d_obj_xyz, double *d_obj_uv, int cap,
                          int32_t *d_n_obj, int32_t *loop_total, int cap_loop, bool sync) {
   hipStream_t st = c->st;
+  TimerScope ts(c, !sync);
   const long n = (long)c->n_cam * c->H * c->W;
   const size_t in_bytes = dtype == OLPT_U8 ? 1 : dtype == OLPT_U16 ? 2 : dtype == OLPT_F32 ? 4 : 8;
   if (loop_total) {
@@ -828,7 +894,7 @@ olpt_status ipr_run_impl(olpt_ctx *c, const void *d_imgs, int dtype, int n_frame
       c->loop_tot_cap = need;
     }
   }
-  fill_i32(c->flags, 8, 0, st);
+  fill_i32(c->flags, 16, 0, st);
   int n_loop = 0;
   for (int f = 0; f < n_frames; ++f) {
     const char *src = (const char *)d_imgs + (size_t)f * n * in_bytes;
@@ -933,6 +999,54 @@ olpt_status olpt_ipr_graph_capture(olpt_ctx *c, const void *d_imgs, int dtype, i
   if (e != hipSuccess) return fail(OLPT_E_GPU, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
   c->graph = g;
   HIP_TRY(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+  return OLPT_OK;
+}
+
+olpt_status olpt_stage_timing(olpt_ctx *c, int enable) {
+  if (!c) return fail(OLPT_E_INVALID_ARGUMENT, "null context");
+  if (enable && !c->timer) {
+    c->timer = new StageTimer();
+    for (int a = 0; a < 2; ++a)
+      for (int s = 0; s < 8; ++s)
+        for (int i = 0; i < 512; ++i) HIP_TRY(hipEventCreate(&c->timer->ev[a][s][i]));
+  }
+  if (c->timer) {
+    c->timer->on = enable != 0;
+    for (int s = 0; s < 8; ++s) c->timer->n[s] = 0;
+  }
+  return OLPT_OK;
+}
+
+olpt_status olpt_stage_times(olpt_ctx *c, double ms[8], int32_t calls[8]) {
+  if (!c) return fail(OLPT_E_INVALID_ARGUMENT, "null context");
+  HIP_TRY(hipStreamSynchronize(c->st));
+  for (int s = 0; s < 8; ++s) {
+    ms[s] = 0.0;
+    calls[s] = 0;
+    if (!c->timer) continue;
+    for (int i = 0; i < c->timer->n[s]; ++i) {
+      float t = 0.f;
+      HIP_TRY(hipEventElapsedTime(&t, c->timer->ev[0][s][i], c->timer->ev[1][s][i]));
+      ms[s] += t;
+    }
+    calls[s] = c->timer->n[s];
+    c->timer->n[s] = 0;
+  }
+  return OLPT_OK;
+}
+
+olpt_status olpt_stage_stats(olpt_ctx *c, int enable, double bytes[8], int32_t launches[8]) {
+  if (!c) return fail(OLPT_E_INVALID_ARGUMENT, "null context");
+  if (bytes && launches)
+    for (int s = 0; s < 8; ++s) {
+      bytes[s] = c->st_bytes[s];
+      launches[s] = c->st_launch[s];
+    }
+  for (int s = 0; s < 8; ++s) {
+    c->st_bytes[s] = 0;
+    c->st_launch[s] = 0;
+  }
+  c->stats = enable != 0;
   return OLPT_OK;
 }
 

@@ -11,6 +11,31 @@
 
 namespace olpt {
 
+// Optional stage timing with HIP events on the launch stream (eager runs
+// only).  Slots: see olpt_c.h (OLPT_STAGE_*).
+struct StageTimer {
+  bool on = false;
+  hipEvent_t ev[2][8][512];
+  int n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  void begin(int slot, hipStream_t st) {
+    if (on && n[slot] < 512) (void)hipEventRecord(ev[0][slot][n[slot]], st);
+  }
+  void end(int slot, hipStream_t st) {
+    if (on && n[slot] < 512) {
+      (void)hipEventRecord(ev[1][slot][n[slot]], st);
+      ++n[slot];
+    }
+  }
+};
+extern thread_local StageTimer *g_timer;
+inline void stage_begin(int slot, hipStream_t st) {
+  if (g_timer) g_timer->begin(slot, st);
+}
+inline void stage_end(int slot, hipStream_t st) {
+  if (g_timer) g_timer->end(slot, st);
+}
+enum { ST_PEAKS = 0, ST_BUILD = 1, ST_PRUNE = 2, ST_SHAKE = 3, ST_RESID = 4, ST_OTHER = 5 };
+
 // Which cameras a stage works on (host-side list, passed by value).
 struct CamList {
   int32_t n;

@@ -135,6 +135,7 @@ void launch_find_peaks(const TIn *img, long cam_stride, int H, int W,
   const long nchunk = peak_chunks(H, W);
   if (nchunk <= 0 || cams.n <= 0) return;
   dim3 g((unsigned)((nchunk + kPeakWaves - 1) / kPeakWaves), (unsigned)cams.n);
+  stage_begin(ST_PEAKS, st);
   k_peak_mask<TIn><<<g, kPeakWaves * kWave, 0, st>>>(img, cam_stride, H, W,
                                                      min_int, cams, mask, cnt,
                                                      nchunk);
@@ -142,6 +143,7 @@ void launch_find_peaks(const TIn *img, long cam_stride, int H, int W,
   k_peak_emit<TIn><<<g, kPeakWaves * kWave, 0, st>>>(img, cam_stride, H, W, cams,
                                                      mask, off, nchunk, xy, cap);
   k_peak_finish<<<1, 64, 0, st>>>(cams, totals, cap, npts, overflow);
+  stage_end(ST_PEAKS, st);
 }
 
 #define INST(T)                                                                \

@@ -197,7 +197,9 @@ static void residual_pass(const ShakeBuffers<T> &b, const T *orig, T *out,
   k_bin_fill<T><<<grid_for((long)b.cap_obj * nc, 256, 4096), 256, 0, st>>>(b, n_ptr);
   const int box_w = (int)lround(2.0 * (double)r_px);
   dim3 g((b.W + kTile - 1) / kTile, (b.H + kTile - 1) / kTile, cams.n);
+  stage_begin(ST_RESID, st);
   k_resid_gather<T><<<g, kTile * kTile, 0, st>>>(b, orig, out, rec, cams, mode, box_w);
+  stage_end(ST_RESID, st);
 }
 
 template <typename T>
@@ -755,7 +757,9 @@ __global__ void k_zero_score(ShakeBuffers<T> b) {
 template <typename T, int NC, int SLOTS>
 static void shake_iter(const ShakeBuffers<T> &b, const ShakeParams &p,
                        const ObjCamRec<T> *rec, T delta, hipStream_t st) {
+  stage_begin(ST_SHAKE, st);
   k_shake<T, NC, SLOTS><<<256 * 16, kWave, 0, st>>>(b, p, rec, delta);
+  stage_end(ST_SHAKE, st);
 }
 
 template <typename T>

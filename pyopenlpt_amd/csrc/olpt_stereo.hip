@@ -987,6 +987,7 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   // build
   const int NL = p.build_count - 2;
   const int waves = 256 * 12;
+  stage_begin(ST_BUILD, st);
   switch (NL < 0 ? 0 : NL) {
   case 0: k_build<0><<<waves, kWave, 0, st>>>(b, p); break;
   case 1: k_build<1><<<waves, kWave, 0, st>>>(b, p); break;
@@ -996,6 +997,8 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   case 5: k_build<5><<<waves, kWave, 0, st>>>(b, p); break;
   default: k_build<6><<<waves, kWave, 0, st>>>(b, p); break;
   }
+  stage_end(ST_BUILD, st);
+  stage_begin(ST_PRUNE, st);
   // sort
   const int gM = grid_for(b.cap_cand, T, 2048);
   zero_i32(b.key_cnt, b.cap, st);
@@ -1020,6 +1023,7 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
                  b.scan_scratch, st);
   k_obj_write<<<gM, T, 0, st>>>(b);
   k_obj_count<<<1, 64, 0, st>>>(b);
+  stage_end(ST_PRUNE, st);
 }
 
 } // namespace olpt

@@ -1,0 +1,130 @@
+"""GPU edge cases and error behaviour of the C-ABI."""
+import numpy as np
+import pytest
+import torch
+
+from common import GOLDEN_CASES, Case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def case(gpu_available):
+    return Case("golden_u8_256")
+
+
+def make_ctx(case, precision="f32", stream=None, **kw):
+    from pyopenlpt_amd.context import Context
+    ctx = Context(case.cams, case.H, case.W, precision=precision, stream=stream,
+                  max_2d=kw.get("max_2d", 1 << 15), max_cand=1 << 18, max_obj=1 << 15)
+    otf, keep = case.otf()
+    ctx.set_otf(otf)
+    ctx._keep = keep
+    return ctx
+
+
+def test_empty_images(case):
+    from pyopenlpt_amd.context import to_device_image
+    ctx = make_ctx(case)
+    d, code = to_device_image(np.zeros_like(case.images))
+    assert len(ctx.find_peaks(d[0].contiguous(), code, case.params)) == 0
+    (ox, _u, n_obj), lt = ctx.ipr_run(d, code, case.params, cap_obj=1 << 15)
+    assert int(n_obj[0].item()) == 0
+    assert (lt[0][:4] == 0).all()
+    ctx.close()
+
+
+def test_stereo_needs_two_active_cameras(case):
+    from pyopenlpt_amd._lib import FatalError
+    ctx = make_ctx(case)
+    with pytest.raises(FatalError) as e:
+        ctx.stereo_match(case.peaks(), case.params, active=[1, 0, 0, 0])
+    assert e.value.status == 1   # ErrorCode::InvalidArgument (StereoMatch.cpp:39-40)
+    ctx.close()
+
+
+def test_check_cameras_configuration_is_rejected(case):
+    """match_cam_count below the active count: the reference's checkMatch
+    leaves e_check NaN and match() aborts; the library refuses up front."""
+    from pyopenlpt_amd._lib import FatalError
+    ctx = make_ctx(case)
+    p = case.params
+    old = p.match_cam_count
+    p.match_cam_count = 3
+    try:
+        with pytest.raises(FatalError) as e:
+            ctx.stereo_match(case.peaks(), p)
+        assert e.value.status == 3
+    finally:
+        p.match_cam_count = old
+    ctx.close()
+
+
+def test_peak_capacity_error(case):
+    from pyopenlpt_amd._lib import FatalError
+    from pyopenlpt_amd.context import to_device_image
+    ctx = make_ctx(case)
+    d, code = to_device_image(case.images)
+    with pytest.raises(FatalError) as e:
+        ctx.find_peaks(d[0].contiguous(), code, case.params, cap=10)
+    assert e.value.status == 102
+    ctx.close()
+
+
+def test_reduced_subsets_match_reference():
+    """n_cam_reduced = 1 (4 subsets of 3 active cameras)."""
+    c = Case("golden_reduced_256")
+    from pyopenlpt_amd.context import to_device_image
+    ctx = make_ctx(c, "f64")
+    d, code = to_device_image(c.images)
+    (ox, _u, n_obj), lt = ctx.ipr_run(d, code, c.params, cap_obj=1 << 15)
+    loops = c.z["ipr_loops"]
+    assert len(loops) == 2 + 4 * 2
+    assert np.array_equal(lt[0][: len(loops)], loops)
+    ctx.close()
+
+
+def test_batched_frames_equal_single_frames(case):
+    from pyopenlpt_amd.context import to_device_image
+    cases = [Case(n) for n in ("golden_u8_256", "golden_reduced_256")]
+    cams_ok = all(np.array_equal(cases[0].z["cameras"], c.z["cameras"]) for c in cases)
+    if not cams_ok:
+        cases = [case, case]
+    imgs = np.stack([c.images for c in cases])
+    ctx = make_ctx(case, "f64")
+    d, code = to_device_image(imgs)
+    (ox, _u, n_obj), _lt = ctx.ipr_run(d, code, case.params, n_frames=2, cap_obj=1 << 14)
+    for f, c in enumerate(cases):
+        d1, _ = to_device_image(c.images)
+        (o1, _u1, n1), _ = ctx.ipr_run(d1, code, case.params, cap_obj=1 << 14)
+        k = int(n1[0].item())
+        assert int(n_obj[f].item()) == k
+        assert torch.equal(ox[f, :k], o1[0, :k])
+    ctx.close()
+
+
+def test_graph_replays_match_eager(case):
+    from pyopenlpt_amd.context import to_device_image
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        d, code = to_device_image(case.images)
+        ctx = make_ctx(case, "f32", stream=s)
+        outs_e = ctx.alloc_outputs(1, 1 << 14, d.device)
+        ctx.ipr_run(d, code, case.params, cap_obj=1 << 14, outputs=outs_e)
+        s.synchronize()
+        ref_n = int(outs_e[2][0].item())
+        ref_x = outs_e[0][0, :ref_n].clone()
+        outs_g = ctx.alloc_outputs(1, 1 << 14, d.device)
+        ctx.graph_capture(d, code, case.params, 1, outs_g, 1 << 14)
+        for rep in range(3):
+            outs_g[0].zero_()
+            outs_g[2].zero_()
+            ctx.graph_launch()
+            s.synchronize()
+            n = int(outs_g[2][0].item())
+            assert n == ref_n, rep
+            assert torch.equal(outs_g[0][0, :n], ref_x), rep
+            # eager work between replays
+            ctx.ipr_run(d, code, case.params, cap_obj=1 << 14, outputs=outs_e)
+            s.synchronize()
+    ctx.close()
