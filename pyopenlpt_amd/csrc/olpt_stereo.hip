@@ -6,14 +6,18 @@
 //   k_cell_*      IDMap as a CSR cell index per camera (StereoMatch.cpp:
 //                 1413-1438); within-cell order is irrelevant to the result
 //   k_build       one wave64 per reference observation, persistent waves with
-//                 dynamic work fetch.  Band enumeration is wave-cooperative:
-//                 lanes own cell rows of the strip intersection
+//                 dynamic work fetch.  The first epipolar band is enumerated
+//                 wave-cooperatively: lanes own cell rows of the strip
 //                 (computeStripIntersection, :1475-1566) and append accepted
-//                 points with ballot/mbcnt compaction into LDS; the back-
-//                 projection, triangulation, camera selection and checkMatch
-//                 tests then run lane-parallel over the compacted list.
-//                 The DFS over build cameras (buildMatch, :184-444) walks the
-//                 LDS level lists; the emitted set does not depend on order.
+//                 points with ballot/mbcnt compaction into LDS; back-
+//                 projection, triangulation and camera selection then run
+//                 lane-parallel over that list.  The deeper build cameras
+//                 (buildMatch's DFS, :184-444) are walked breadth-first with
+//                 one survivor per lane: each lane enumerates only the cell
+//                 rows inside the y-extent of its strip intersection (a
+//                 superset of the reference's non-empty spans), so a deeper
+//                 search costs a few cells instead of all 256 rows.  The
+//                 emitted set does not depend on visiting order.
 //   k_key_* / k_bucket_sort  lexicographic sort of the id vectors
 //                 (:153-156): counting sort on the first active camera's id,
 //                 then insertion sort inside each (small) bucket.
@@ -260,277 +264,193 @@ __device__ __forceinline__ bool in_limit(const double lim[6], const double X[3])
 // --------------------------------------------------------------------------
 // k_build: wave per reference observation
 // --------------------------------------------------------------------------
-constexpr int kRawCap = 1024;  // enumerated (pre back-projection) list
-constexpr int kL1Cap = 1024;   // survivors on the 2nd build camera
-constexpr int kLnCap = 256;    // survivors on deeper build cameras
+constexpr int kRawCap = 1024;   // level-1 band list (pre back-projection)
+constexpr int kLvlCap = 1024;   // survivors per deeper level
 
-template <int NL> struct BuildLds {
+// Survivors of level d are the 2D points chosen on the d-th build camera:
+// their camera, the next camera to search, and the parent entry (level
+// d-1) whose path they extend.  NLV = build_count - 2 non-terminal levels.
+template <int NLV> struct BuildLds {
   int32_t raw[kRawCap];
-  int32_t lvl_pid[NL > 0 ? kL1Cap + (NL - 1) * kLnCap : 1];
-  int8_t lvl_next[NL > 0 ? kL1Cap + (NL - 1) * kLnCap : 1];
-  // path (build order)
-  int32_t pcam[kMaxCam];
-  int32_t ppid[kMaxCam];
-  double ppt[kMaxCam][2];   // 2D point
-  double pu[kMaxCam][3];    // LOS unit direction
-  // strip lines on the current target camera
-  double la[kMaxCam][2], lU[kMaxCam][2], ln[kMaxCam][2], ld[kMaxCam];
+  int32_t pid[NLV > 0 ? NLV : 1][kLvlCap];
+  int16_t par[NLV > 0 ? NLV : 1][kLvlCap];
+  int8_t cam[NLV > 0 ? NLV : 1][kLvlCap];
+  int8_t nxt[NLV > 0 ? NLV : 1][kLvlCap];
+  int32_t n[NLV + 1];
   int32_t n_raw;
+  // reference observation and level-1 strip line
+  double la[2], lU[2], ln[2], ld;
 };
 
-__device__ __forceinline__ int lvl_base(int lvl) {
-  return lvl == 1 ? 0 : kL1Cap + (lvl - 2) * kLnCap;
-}
-__device__ __forceinline__ int lvl_cap(int lvl) { return lvl == 1 ? kL1Cap : kLnCap; }
+// Strip line of LOS (pt, u) on camera t, in the normal form the strip
+// intersection uses (IDMap::computeStripIntersection, :1475-1566).
+struct Strip {
+  double ax, ay;   // point on the line
+  double Ux, Uy;   // renormalised direction (IDMap::normalized)
+  double n0, n1, d;
+};
 
-// Enumerate points of camera t inside all k strips (the path's LOS
-// projected into t); result appended to L.raw.
-template <int NL>
-__device__ void enumerate_strip(const StereoBuffers &b, const StereoParams &p,
-                                BuildLds<NL> &L, int k, int t) {
-  const int lane = lane_id();
-  const CamT<double> *cams = b.cams;
-  if (lane < k) {
-    const int c = L.pcam[lane];
-    const Line2 l2 = make_line2d(cams[t], cams[c].Tinv, L.pu[lane]);
-    double Ux, Uy;
-    normalized(l2.ux, l2.uy, Ux, Uy);
-    L.la[lane][0] = l2.ax;
-    L.la[lane][1] = l2.ay;
-    L.lU[lane][0] = Ux;
-    L.lU[lane][1] = Uy;
-    const double n0 = -Uy, n1 = Ux;
-    L.ln[lane][0] = n0;
-    L.ln[lane][1] = n1;
-    L.ld[lane] = -(n0 * l2.ax + n1 * l2.ay);
-  }
-  if (lane == 0) L.n_raw = 0;
-  __syncthreads();
-  const int rows = b.rows_cell, cols = b.cols_cell;
-  const long ncell = (long)rows * cols;
-  const int32_t *cstart = b.cell_start + t * ncell;
-  const int32_t *ccnt = b.cell_cnt + t * ncell;
-  const int32_t *cpid = b.cell_pid + (long)t * b.cap;
-  const double *xy = b.xy + (long)t * b.cap * 2;
-  const double cell = (double)b.cell_px;
-  const double hw = 0.5 * b.cell_px, hh = 0.5 * b.cell_px;
-  const double to_index = 1.0 / b.cell_px;
-  const double tolg = p.tol2d + 1e-6;
-  const double tol2 = p.tol2d * p.tol2d;
-  for (int cy0 = 0; cy0 < rows; cy0 += kWave) {
-    const int cy = cy0 + lane;
-    int xa = 0, xb = cols - 1;
-    bool valid = cy < rows;
-    for (int l = 0; l < k && valid; ++l) {
-      const double n0 = L.ln[l][0], n1 = L.ln[l][1];
-      const double yc = (cy + 0.5) * cell;
-      const double C = n1 * yc + L.ld[l];
-      const double pe = fabs(n0) * hw + fabs(n1) * hh;
-      const double T = tolg + pe;
-      int a = 0, bb = cols - 1;
-      if (fabs(n0) < 1e-12) {
-        if (fabs(C) > T) valid = false;
-      } else {
-        double xl = (-T - C) / n0, xr = (T - C) / n0;
-        if (xl > xr) {
-          const double tt = xl;
-          xl = xr;
-          xr = tt;
-        }
-        a = (int)ceil(xl * to_index - 0.5 - 1e-9);
-        bb = (int)floor(xr * to_index - 0.5 + 1e-9);
-        a = a < 0 ? 0 : a;
-        bb = bb > cols - 1 ? cols - 1 : bb;
-        if (a > bb) valid = false;
-      }
-      if (valid) {
-        xa = a > xa ? a : xa;
-        xb = bb < xb ? bb : xb;
-      }
-    }
-    // walk the lane's cells / points one point per iteration
-    int cellid = valid ? cy * cols + xa : 0;
-    const int cell_end = valid ? cy * cols + xb : -1;
-    int kk = valid ? cstart[cellid] : 0, kend = valid ? kk + ccnt[cellid] : 0;
-    while (valid && kk == kend && cellid < cell_end) {
-      ++cellid;
-      kk = cstart[cellid];
-      kend = kk + ccnt[cellid];
-    }
-    bool has = valid && kk < kend;
-    while (__any(has)) {
-      bool acc = false;
-      int pid = -1;
-      if (has) {
-        pid = cpid[kk];
-        const double qx = xy[2 * pid], qy = xy[2 * pid + 1];
-        acc = true;
-        for (int l = 0; l < k; ++l) {
-          const double dx = qx - L.la[l][0], dy = qy - L.la[l][1];
-          const double z = dy * L.lU[l][0] - dx * L.lU[l][1];
-          if (z * z > tol2) {
-            acc = false;
-            break;
-          }
-        }
-        ++kk;
-        while (kk == kend && cellid < cell_end) {
-          ++cellid;
-          kk = cstart[cellid];
-          kend = kk + ccnt[cellid];
-        }
-        has = kk < kend;
-      }
-      const uint64_t m = ballot(acc);
-      if (m) {
-        const int base = L.n_raw;
-        const int pos = base + mbcnt(m);
-        if (acc) {
-          if (pos < kRawCap)
-            L.raw[pos] = pid;
-          else
-            atomicOr(&b.flags[1], 1);
-        }
-        __syncthreads();
-        if (lane == 0) L.n_raw = base + (int)__popcll(m);
-        __syncthreads();
-      }
-    }
-  }
-  __syncthreads();
-  if (lane == 0 && L.n_raw > kRawCap) L.n_raw = kRawCap;
-  __syncthreads();
+__device__ __forceinline__ Strip make_strip(const CamT<double> &ct, const double pt[3],
+                                            const double u[3]) {
+  const Line2 l2 = make_line2d(ct, pt, u);
+  Strip s;
+  s.ax = l2.ax;
+  s.ay = l2.ay;
+  normalized(l2.ux, l2.uy, s.Ux, s.Uy);
+  s.n0 = -s.Uy;
+  s.n1 = s.Ux;
+  s.d = -(s.n0 * l2.ax + s.n1 * l2.ay);
+  return s;
 }
 
-// Lane-parallel: back-projection (checkBackProjection :1206-1227), quick
-// triangulation (:397-407), then terminal checkMatch + emit, or next-camera
-// selection and append to level list `lvl`.
-template <int NL>
-__device__ void filter_extend(const StereoBuffers &b, const StereoParams &p,
-                              BuildLds<NL> &L, int k, int t, int lvl,
-                              int32_t *n_lvl) {
-  const int lane = lane_id();
+// Cell-row span of one row for one strip (same arithmetic as the
+// reference); returns false if the row is empty for this strip.
+__device__ __forceinline__ bool strip_span(const Strip &s, int cy, int cell_px, int cols,
+                                           double tolg, int &a, int &bb) {
+  const double cell = (double)cell_px;
+  const double hw = 0.5 * cell_px, hh = 0.5 * cell_px;
+  const double to_index = 1.0 / cell_px;
+  const double yc = (cy + 0.5) * cell;
+  const double C = s.n1 * yc + s.d;
+  const double pe = fabs(s.n0) * hw + fabs(s.n1) * hh;
+  const double T = tolg + pe;
+  a = 0;
+  bb = cols - 1;
+  if (fabs(s.n0) < 1e-12) return !(fabs(C) > T);
+  double xl = (-T - C) / s.n0, xr = (T - C) / s.n0;
+  if (xl > xr) {
+    const double tt = xl;
+    xl = xr;
+    xr = tt;
+  }
+  a = (int)ceil(xl * to_index - 0.5 - 1e-9);
+  bb = (int)floor(xr * to_index - 0.5 + 1e-9);
+  a = a < 0 ? 0 : a;
+  bb = bb > cols - 1 ? cols - 1 : bb;
+  return a <= bb;
+}
+
+// Rows whose cell centre can lie in both widened strips i, j: the
+// intersection is a parallelogram around the crossing of the centre lines;
+// its y-extent (+ one cell) bounds the non-empty rows.  Superset only.
+__device__ __forceinline__ void pair_rows(const Strip &si, const Strip &sj, int cell_px,
+                                          double tolg, int rows, int &r0, int &r1) {
+  const double det = si.n0 * sj.n1 - si.n1 * sj.n0;
+  if (!(fabs(det) >= 1e-6)) return;  // near-parallel: keep the current range
+  const double hw = 0.5 * cell_px;
+  const double Ti = tolg + (fabs(si.n0) + fabs(si.n1)) * hw;
+  const double Tj = tolg + (fabs(sj.n0) + fabs(sj.n1)) * hw;
+  // centre crossing: n_i.x = -d_i, n_j.x = -d_j
+  const double py = (si.n0 * (-sj.d) - sj.n0 * (-si.d)) / det;
+  // y-components of the unit offsets along each strip's normal band
+  const double wiy = -sj.n0 / det;  // moves n_i.x by 1, keeps n_j.x
+  const double wjy = si.n0 / det;
+  const double ext = fabs(Ti * wiy) + fabs(Tj * wjy);
+  // clamp in floating point before the integer conversion (far crossings)
+  const double lo = fmax(floor((py - ext) / cell_px) - 1.0, -1.0);
+  const double hi = fmin(floor((py + ext) / cell_px) + 1.0, (double)rows);
+  if (!(lo <= hi)) return;  // NaN geometry: keep the current range
+  if ((int)lo > r0) r0 = (int)lo;
+  if ((int)hi < r1) r1 = (int)hi;
+  if (r0 < 0) r0 = 0;
+  if (r1 > rows - 1) r1 = rows - 1;
+}
+
+template <int MAXB> struct Path {
+  int cam[MAXB];
+  int pid[MAXB];
+  double pt[MAXB][2];
+  double P[MAXB][3];  // LOS origin (camera centre)
+  double U[MAXB][3];  // LOS unit direction
+};
+
+template <int MAXB>
+__device__ __forceinline__ void path_push(Path<MAXB> &P, int k, const StereoBuffers &b,
+                                          int cam, int pid) {
+  P.cam[k] = cam;
+  P.pid[k] = pid;
+  const double *q = b.xy + ((long)cam * b.cap + pid) * 2;
+  P.pt[k][0] = q[0];
+  P.pt[k][1] = q[1];
+  const double *u = b.los + ((long)cam * b.cap + pid) * 4;
+  for (int c = 0; c < 3; ++c) {
+    P.P[k][c] = b.cams[cam].Tinv[c];
+    P.U[k][c] = u[c];
+  }
+}
+
+// Extend path (length k) by point pid on camera t: back-projection
+// (checkBackProjection, :1206-1227), quick triangulation (:397-407), then
+// either checkMatch + emission (terminal) or the next-camera choice.
+// Returns 0 reject, 1 emitted, 2 survivor (next camera in *next).
+template <int MAXB>
+__device__ int extend(const StereoBuffers &b, const StereoParams &p, Path<MAXB> &P, int k,
+                      int t, int pid, int *next) {
   const CamT<double> *cams = b.cams;
   const double tol2 = p.tol2d * p.tol2d;
-  const bool terminal = (k + 1 == p.build_count);
-  if (lane == 0 && !terminal) n_lvl[lvl] = 0;
-  __syncthreads();
-  const int n_raw = L.n_raw;
-  for (int i0 = 0; i0 < n_raw; i0 += kWave) {
-    const int i = i0 + lane;
-    bool keep = false;
-    int pid = -1, next = -1;
-    int32_t ids_out[kMaxCam];
-    double e_out = 0.0;
-    if (i < n_raw) {
-      pid = L.raw[i];
-      const double *q = b.xy + ((long)t * b.cap + pid) * 2;
-      const double *ut = b.los + ((long)t * b.cap + pid) * 4;
-      double u_t[3] = {ut[0], ut[1], ut[2]};
-      keep = true;
-      for (int j = 0; j < k && keep; ++j) {
-        const Line2 l2 = make_line2d(cams[L.pcam[j]], cams[t].Tinv, u_t);
-        const double d0 = L.ppt[j][0] - l2.ax, d1 = L.ppt[j][1] - l2.ay;
-        const double z = d1 * l2.ux - d0 * l2.uy;
-        if (z * z > tol2) keep = false;
-      }
-      if (keep) {
-        double pts[kMaxCam][3], us[kMaxCam][3];
-        for (int j = 0; j < k; ++j)
-          for (int c = 0; c < 3; ++c) {
-            pts[j][c] = cams[L.pcam[j]].Tinv[c];
-            us[j][c] = L.pu[j][c];
+  const double *ut = b.los + ((long)t * b.cap + pid) * 4;
+  const double u_t[3] = {ut[0], ut[1], ut[2]};
+  for (int j = 0; j < k; ++j) {
+    const Line2 l2 = make_line2d(cams[P.cam[j]], cams[t].Tinv, u_t);
+    const double d0 = P.pt[j][0] - l2.ax, d1 = P.pt[j][1] - l2.ay;
+    const double z = d1 * l2.ux - d0 * l2.uy;
+    if (z * z > tol2) return 0;
+  }
+  path_push<MAXB>(P, k, b, t, pid);
+  const int kk = k + 1;
+  {
+    const double th = min_parallax_deg(P.U, kk);
+    const double tq = tri_tol(p.tol3d, kk, p.build_count, th);
+    double X[3], e;
+    if (!triangulate(P.P, P.U, kk, X, e) || !(e <= tq)) return 0;
+  }
+  if (kk == p.build_count) {
+    // checkMatch, all-active branch (:530-540): LOS in camera order
+    double cp[MAXB][3], cu[MAXB][3];
+    int32_t ids[kMaxCam];
+    int n = 0;
+    for (int c = 0; c < b.n_cam; ++c) {
+      ids[c] = -1;
+      for (int j = 0; j < kk; ++j)
+        if (P.cam[j] == c) {
+          ids[c] = P.pid[j];
+          for (int d = 0; d < 3; ++d) {
+            cp[n][d] = P.P[j][d];
+            cu[n][d] = P.U[j][d];
           }
-        for (int c = 0; c < 3; ++c) {
-          pts[k][c] = cams[t].Tinv[c];
-          us[k][c] = u_t[c];
+          ++n;
         }
-        const double th = min_parallax_deg(us, k + 1);
-        const double tq = tri_tol(p.tol3d, k + 1, p.build_count, th);
-        double X[3], e;
-        if (!triangulate(pts, us, k + 1, X, e) || !(e <= tq)) keep = false;
-        if (keep && terminal) {
-          // checkMatch, all-active branch (:530-540): LOS in camera order
-          double cp[kMaxCam][3], cu[kMaxCam][3];
-          int n = 0;
-          for (int c = 0; c < b.n_cam; ++c) ids_out[c] = -1;
-          for (int j = 0; j < k; ++j) ids_out[L.pcam[j]] = L.ppid[j];
-          ids_out[t] = pid;
-          for (int c = 0; c < b.n_cam; ++c) {
-            if (ids_out[c] < 0) continue;
-            const double *uu = (c == t) ? u_t : nullptr;
-            int jj = -1;
-            for (int j = 0; j < k; ++j)
-              if (L.pcam[j] == c) jj = j;
-            for (int d = 0; d < 3; ++d) {
-              cp[n][d] = cams[c].Tinv[d];
-              cu[n][d] = uu ? uu[d] : L.pu[jj][d];
-            }
-            ++n;
-          }
-          double Xc[3];
-          if (!triangulate(cp, cu, n, Xc, e_out)) {
-            atomicOr(&b.flags[6], 1);
-            keep = false;
-          } else if (e_out > p.tol3d || !in_limit(p.limit, Xc)) {
-            keep = false;
-          }
-        } else if (keep) {
-          int rem[kMaxCam], nrem = 0;
-          for (int c = 0; c < b.n_cam; ++c) {
-            if (!p.active[c] || c == t) continue;
-            bool used = false;
-            for (int j = 0; j < k; ++j) used = used || (L.pcam[j] == c);
-            if (!used) rem[nrem++] = c;
-          }
-          next = select_next(cams, pts, us, k + 1, rem, nrem);
-          if (next < 0) keep = false;
-        }
-      }
     }
-    const uint64_t m = ballot(keep);
-    if (!m) continue;
-    if (terminal) {
-      int base = 0;
-      if (lane == 0) base = atomicAdd(b.n_cand, (int)__popcll(m));
-      base = __shfl(base, 0, kWave);
-      if (keep) {
-        const int slot = base + mbcnt(m);
-        if (slot < b.cap_cand) {
-          for (int c = 0; c < b.n_cam; ++c) b.cand_ids[(long)slot * b.n_cam + c] = ids_out[c];
-          b.cand_e[slot] = e_out;
-        } else {
-          atomicOr(&b.flags[0], 1);
-        }
-      }
+    double Xc[3], e;
+    if (!triangulate(cp, cu, n, Xc, e)) {
+      atomicOr(&b.flags[6], 1);
+      return 0;
+    }
+    if (e > p.tol3d || !in_limit(p.limit, Xc)) return 0;
+    const int slot = atomicAdd(b.n_cand, 1);
+    if (slot < b.cap_cand) {
+      for (int c = 0; c < b.n_cam; ++c) b.cand_ids[(long)slot * b.n_cam + c] = ids[c];
+      b.cand_e[slot] = e;
     } else {
-      const int base = n_lvl[lvl];
-      if (keep) {
-        const int pos = base + mbcnt(m);
-        if (pos < lvl_cap(lvl)) {
-          L.lvl_pid[lvl_base(lvl) + pos] = pid;
-          L.lvl_next[lvl_base(lvl) + pos] = (int8_t)next;
-        } else {
-          atomicOr(&b.flags[1], 1);
-        }
-      }
-      __syncthreads();
-      if (lane == 0) {
-        const int nn = base + (int)__popcll(m);
-        n_lvl[lvl] = nn < lvl_cap(lvl) ? nn : lvl_cap(lvl);
-      }
-      __syncthreads();
+      atomicOr(&b.flags[0], 1);
     }
+    return 1;
   }
-  __syncthreads();
+  int rem[kMaxCam], nrem = 0;
+  for (int c = 0; c < b.n_cam; ++c) {
+    if (!p.active[c]) continue;
+    bool used = false;
+    for (int j = 0; j < kk; ++j) used = used || (P.cam[j] == c);
+    if (!used) rem[nrem++] = c;
+  }
+  *next = select_next(cams, P.P, P.U, kk, rem, nrem);
+  return *next < 0 ? 0 : 2;
 }
 
-template <int NL>
+template <int MAXB>
 __global__ void __launch_bounds__(kWave) k_build(StereoBuffers b, StereoParams p) {
-  __shared__ BuildLds<NL> L;
-  __shared__ int32_t n_lvl[NL + 2];
-  __shared__ int32_t cursor[NL + 2];
-  __shared__ int32_t target[NL + 2];
+  constexpr int NLV = MAXB - 2;
+  __shared__ BuildLds<NLV> L;
   const int lane = lane_id();
   // reference camera: fewest observations among active cameras, lowest id
   // on ties (StereoMatch.cpp:70-85)
@@ -543,77 +463,172 @@ __global__ void __launch_bounds__(kWave) k_build(StereoBuffers b, StereoParams p
     }
   const int nref = best;
   const CamT<double> *cams = b.cams;
+  const int rows = b.rows_cell, cols = b.cols_cell;
+  const long ncell = (long)rows * cols;
+  const double tolg = p.tol2d + 1e-6;
+  const double tol2 = p.tol2d * p.tol2d;
   while (true) {
     int r = 0;
     if (lane == 0) r = atomicAdd(b.work, 1);
     r = __shfl(r, 0, kWave);
     if (r >= nref) break;
-    if (lane == 0) {
-      L.pcam[0] = ref;
-      L.ppid[0] = r;
-      const double *q = b.xy + ((long)ref * b.cap + r) * 2;
-      L.ppt[0][0] = q[0];
-      L.ppt[0][1] = q[1];
-      const double *u = b.los + ((long)ref * b.cap + r) * 4;
-      L.pu[0][0] = u[0];
-      L.pu[0][1] = u[1];
-      L.pu[0][2] = u[2];
-    }
-    __syncthreads();
+    Path<MAXB> P0;
+    path_push<MAXB>(P0, 0, b, ref, r);
     int rem[kMaxCam], nrem = 0;
     for (int c = 0; c < b.n_cam; ++c)
       if (p.active[c] && c != ref) rem[nrem++] = c;
     if (nrem == 0) continue;
-    const int t1 = select_second(cams, cams[ref].Tinv, L.pu[0], rem, nrem);
+    const int t1 = select_second(cams, P0.P[0], P0.U[0], rem, nrem);
     if (t1 < 0) continue;
-    enumerate_strip<NL>(b, p, L, 1, t1);
-    filter_extend<NL>(b, p, L, 1, t1, 1, n_lvl);
-    if (NL == 0) continue;
-    // DFS over level lists
-    int d = 1;
+    // ---- level 1: one band, wave-cooperative enumeration over cell rows
+    const Strip s1 = make_strip(cams[t1], P0.P[0], P0.U[0]);
     if (lane == 0) {
-      cursor[1] = 0;
-      target[1] = t1;
+      L.n_raw = 0;
+      for (int d = 0; d <= NLV; ++d) L.n[d] = 0;
     }
     __syncthreads();
-    while (d >= 1) {
-      if (cursor[d] >= n_lvl[d]) {
-        --d;
-        continue;
-      }
-      const int e = lvl_base(d) + cursor[d];
-      const int pid = L.lvl_pid[e];
-      const int nt = L.lvl_next[e];
-      const int tc = target[d];
-      __syncthreads();
-      if (lane == 0) {
-        cursor[d] = cursor[d] + 1;
-        L.pcam[d] = tc;
-        L.ppid[d] = pid;
-        const double *q = b.xy + ((long)tc * b.cap + pid) * 2;
-        L.ppt[d][0] = q[0];
-        L.ppt[d][1] = q[1];
-        const double *u = b.los + ((long)tc * b.cap + pid) * 4;
-        L.pu[d][0] = u[0];
-        L.pu[d][1] = u[1];
-        L.pu[d][2] = u[2];
-      }
-      __syncthreads();
-      const int k = d + 1;
-      enumerate_strip<NL>(b, p, L, k, nt);
-      filter_extend<NL>(b, p, L, k, nt, d + 1, n_lvl);
-      if (k + 1 < p.build_count && d + 1 <= NL && n_lvl[d + 1] > 0) {
-        if (lane == 0) {
-          cursor[d + 1] = 0;
-          target[d + 1] = nt;
+    {
+      const int32_t *cstart = b.cell_start + t1 * ncell;
+      const int32_t *ccnt = b.cell_cnt + t1 * ncell;
+      const int32_t *cpid = b.cell_pid + (long)t1 * b.cap;
+      const double *xy = b.xy + (long)t1 * b.cap * 2;
+      for (int cy0 = 0; cy0 < rows; cy0 += kWave) {
+        const int cy = cy0 + lane;
+        int xa = 0, xb = -1;
+        bool valid = cy < rows && strip_span(s1, cy, b.cell_px, cols, tolg, xa, xb);
+        int cellid = valid ? cy * cols + xa : 0;
+        const int cell_end = valid ? cy * cols + xb : -1;
+        int kk = valid ? cstart[cellid] : 0, kend = valid ? kk + ccnt[cellid] : 0;
+        while (valid && kk == kend && cellid < cell_end) {
+          ++cellid;
+          kk = cstart[cellid];
+          kend = kk + ccnt[cellid];
         }
-        __syncthreads();
-        ++d;
+        bool has = valid && kk < kend;
+        while (__any(has)) {
+          bool acc = false;
+          int pid = -1;
+          if (has) {
+            pid = cpid[kk];
+            const double dx = xy[2 * pid] - s1.ax, dy = xy[2 * pid + 1] - s1.ay;
+            const double z = dy * s1.Ux - dx * s1.Uy;
+            acc = !(z * z > tol2);
+            ++kk;
+            while (kk == kend && cellid < cell_end) {
+              ++cellid;
+              kk = cstart[cellid];
+              kend = kk + ccnt[cellid];
+            }
+            has = kk < kend;
+          }
+          const uint64_t m = ballot(acc);
+          if (m) {
+            const int base = L.n_raw;
+            const int pos = base + mbcnt(m);
+            if (acc) {
+              if (pos < kRawCap)
+                L.raw[pos] = pid;
+              else
+                atomicOr(&b.flags[1], 1);
+            }
+            __syncthreads();
+            if (lane == 0) L.n_raw = base + (int)__popcll(m);
+            __syncthreads();
+          }
+        }
       }
+    }
+    __syncthreads();
+    const int n_raw = L.n_raw < kRawCap ? L.n_raw : kRawCap;
+    // level-1 extension, lane-parallel over the band list
+    for (int i = lane; i < n_raw; i += kWave) {
+      int next = -1;
+      const int res = extend<MAXB>(b, p, P0, 1, t1, L.raw[i], &next);
+      if (res == 2 && NLV > 0) {
+        const int pos = atomicAdd(&L.n[0], 1);
+        if (pos < kLvlCap) {
+          L.pid[0][pos] = L.raw[i];
+          L.par[0][pos] = -1;
+          L.cam[0][pos] = (int8_t)t1;
+          L.nxt[0][pos] = (int8_t)next;
+        } else {
+          atomicOr(&b.flags[1], 1);
+        }
+      }
+    }
+    __syncthreads();
+    // ---- deeper levels: one survivor per lane, lane-serial enumeration of
+    // the (small) strip intersection on its next camera
+    for (int lv = 1; lv <= NLV; ++lv) {
+      const int n_in = L.n[lv - 1] < kLvlCap ? L.n[lv - 1] : kLvlCap;
+      for (int e = lane; e < n_in; e += kWave) {
+        // rebuild the path: ref + one entry per level
+        Path<MAXB> P = P0;
+        int chain[MAXB];
+        int q = e;
+        for (int d = lv - 1; d >= 0; --d) {
+          chain[d] = q;
+          q = L.par[d][q];
+        }
+        for (int d = 0; d < lv; ++d)
+          path_push<MAXB>(P, d + 1, b, L.cam[d][chain[d]], L.pid[d][chain[d]]);
+        const int k = lv + 1;              // path length
+        const int t = L.nxt[lv - 1][e];    // camera to search
+        Strip S[MAXB];
+        for (int j = 0; j < k; ++j) S[j] = make_strip(cams[t], P.P[j], P.U[j]);
+        int r0 = 0, r1 = rows - 1;
+        for (int j = 1; j < k; ++j) pair_rows(S[0], S[j], b.cell_px, tolg, rows, r0, r1);
+        const int32_t *cstart = b.cell_start + t * ncell;
+        const int32_t *ccnt = b.cell_cnt + t * ncell;
+        const int32_t *cpid = b.cell_pid + (long)t * b.cap;
+        const double *xy = b.xy + (long)t * b.cap * 2;
+        for (int cy = r0; cy <= r1; ++cy) {
+          int xa = 0, xb = cols - 1;
+          bool ok = true;
+          for (int j = 0; j < k && ok; ++j) {
+            int a, bb;
+            ok = strip_span(S[j], cy, b.cell_px, cols, tolg, a, bb);
+            if (ok) {
+              xa = a > xa ? a : xa;
+              xb = bb < xb ? bb : xb;
+              ok = xa <= xb;
+            }
+          }
+          if (!ok) continue;
+          for (int cx = xa; cx <= xb; ++cx) {
+            const int cell = cy * cols + cx;
+            const int s0 = cstart[cell], s1n = s0 + ccnt[cell];
+            for (int kk = s0; kk < s1n; ++kk) {
+              const int pid = cpid[kk];
+              const double qx = xy[2 * pid], qy = xy[2 * pid + 1];
+              bool in = true;
+              for (int j = 0; j < k && in; ++j) {
+                const double dx = qx - S[j].ax, dy = qy - S[j].ay;
+                const double z = dy * S[j].Ux - dx * S[j].Uy;
+                in = !(z * z > tol2);
+              }
+              if (!in) continue;
+              int next = -1;   // extend() only writes slot k of the path
+              const int res = extend<MAXB>(b, p, P, k, t, pid, &next);
+              if (res == 2 && lv < NLV) {
+                const int pos = atomicAdd(&L.n[lv], 1);
+                if (pos < kLvlCap) {
+                  L.pid[lv][pos] = pid;
+                  L.par[lv][pos] = (int16_t)e;
+                  L.cam[lv][pos] = (int8_t)t;
+                  L.nxt[lv][pos] = (int8_t)next;
+                } else {
+                  atomicOr(&b.flags[1], 1);
+                }
+              }
+            }
+          }
+        }
+      }
+      __syncthreads();
     }
   }
 }
-
 // --------------------------------------------------------------------------
 // lexicographic sort of candidate id vectors
 // --------------------------------------------------------------------------
@@ -985,17 +1000,16 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
                  b.scan_scratch, st);
   k_cell_fill<<<gc, T, 0, st>>>(b, p);
   // build
-  const int NL = p.build_count - 2;
   const int waves = 256 * 12;
   stage_begin(ST_BUILD, st);
-  switch (NL < 0 ? 0 : NL) {
-  case 0: k_build<0><<<waves, kWave, 0, st>>>(b, p); break;
-  case 1: k_build<1><<<waves, kWave, 0, st>>>(b, p); break;
+  switch (p.build_count) {
   case 2: k_build<2><<<waves, kWave, 0, st>>>(b, p); break;
   case 3: k_build<3><<<waves, kWave, 0, st>>>(b, p); break;
   case 4: k_build<4><<<waves, kWave, 0, st>>>(b, p); break;
   case 5: k_build<5><<<waves, kWave, 0, st>>>(b, p); break;
-  default: k_build<6><<<waves, kWave, 0, st>>>(b, p); break;
+  case 6: k_build<6><<<waves, kWave, 0, st>>>(b, p); break;
+  case 7: k_build<7><<<waves, kWave, 0, st>>>(b, p); break;
+  default: k_build<8><<<waves, kWave, 0, st>>>(b, p); break;
   }
   stage_end(ST_BUILD, st);
   stage_begin(ST_PRUNE, st);
