@@ -583,6 +583,7 @@ olpt_status olpt_ctx_reserve(olpt_ctx *c, int n_cam, int H, int W, int max_2d, i
   b.xy = c->xy;
   b.npts = c->npts;
   A(b.los, double, (long)nc * cap * 4);
+  A(b.lines, double, (long)nc * nc * cap * 4);
   b.cell_px = 4;
   b.rows_cell = std::max(1, H / 4 + (H % 4 ? 1 : 0));
   b.cols_cell = std::max(1, W / 4 + (W % 4 ? 1 : 0));

@@ -66,6 +66,7 @@ struct StereoBuffers {
   const double *xy;      // [n_cam][cap][2]
   const int32_t *npts;   // [n_cam]
   double *los;           // [n_cam][cap][4]  unit direction (w unused)
+  double *lines;         // [n_cam][n_cam][cap][4] LOS (c, pid) on camera t
   int cell_px, rows_cell, cols_cell;
   int32_t *cell_cnt;     // [n_cam][ncell]
   int32_t *cell_start;   // [n_cam][ncell]

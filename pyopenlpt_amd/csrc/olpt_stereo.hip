@@ -3,6 +3,7 @@
 // Stages (all counts device-resident, fixed grids):
 //   k_los         one FP64 line of sight per 2D point (Camera::lineOfSight is
 //                 deterministic, so it is computed once instead of per visit)
+//                 and its 2D line on every other camera (the line table)
 //   k_cell_*      IDMap as a CSR cell index per camera (StereoMatch.cpp:
 //                 1413-1438); within-cell order is irrelevant to the result
 //   k_build       one wave64 per reference observation, persistent waves with
@@ -12,12 +13,15 @@
 //                 points with ballot/mbcnt compaction into LDS; back-
 //                 projection, triangulation and camera selection then run
 //                 lane-parallel over that list.  The deeper build cameras
-//                 (buildMatch's DFS, :184-444) are walked breadth-first with
+//                 (buildMatch's DFS, :184-444) are walked level by level with
 //                 one survivor per lane: each lane enumerates only the cell
 //                 rows inside the y-extent of its strip intersection (a
-//                 superset of the reference's non-empty spans), so a deeper
-//                 search costs a few cells instead of all 256 rows.  The
-//                 emitted set does not depend on visiting order.
+//                 superset of the reference's non-empty spans).  Every level
+//                 first compacts (survivor, point) pairs into an LDS window,
+//                 then extends the window lane-parallel, so the expensive
+//                 tests run on full waves; full windows and lists are drained
+//                 depth-first, so no capacity bounds the search.  The emitted
+//                 set does not depend on visiting order.
 //   k_key_* / k_bucket_sort  lexicographic sort of the id vectors
 //                 (:153-156): counting sort on the first active camera's id,
 //                 then insertion sort inside each (small) bucket.
@@ -54,24 +58,6 @@ static void zero_i32(int32_t *p, long n, hipStream_t st) {
   long blocks = (n + 255) / 256;
   if (blocks > 4096) blocks = 4096;
   k_zero_i32<<<(unsigned)blocks, 256, 0, st>>>(p, n);
-}
-
-__global__ void k_los(StereoBuffers b, StereoParams p) {
-  const int cam = blockIdx.y;
-  if (!p.active[cam]) return;
-  const int n = b.npts[cam];
-  const CamT<double> c = b.cams[cam];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += gridDim.x * blockDim.x) {
-    const double *q = b.xy + ((long)cam * b.cap + i) * 2;
-    double u[3];
-    line_of_sight(c, q[0], q[1], u);
-    double *o = b.los + ((long)cam * b.cap + i) * 4;
-    o[0] = u[0];
-    o[1] = u[1];
-    o[2] = u[2];
-    o[3] = 0.0;
-  }
 }
 
 __global__ void k_cell_count(StereoBuffers b, StereoParams p) {
@@ -129,6 +115,33 @@ __device__ __forceinline__ Line2 make_line2d(const CamT<double> &c,
   return Line2{a[0], a[1], d0 / n, d1 / n};
 }
 
+__global__ void k_los(StereoBuffers b, StereoParams p) {
+  const int cam = blockIdx.y;
+  if (!p.active[cam]) return;
+  const int n = b.npts[cam];
+  const CamT<double> c = b.cams[cam];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    const double *q = b.xy + ((long)cam * b.cap + i) * 2;
+    double u[3];
+    line_of_sight(c, q[0], q[1], u);
+    double *o = b.los + ((long)cam * b.cap + i) * 4;
+    o[0] = u[0];
+    o[1] = u[1];
+    o[2] = u[2];
+    o[3] = 0.0;
+    for (int t = 0; t < b.n_cam; ++t) {
+      if (t == cam || !p.active[t]) continue;
+      const Line2 l = make_line2d(b.cams[t], c.Tinv, u);
+      double *w = b.lines + (((long)cam * b.n_cam + t) * b.cap + i) * 4;
+      w[0] = l.ax;
+      w[1] = l.ay;
+      w[2] = l.ux;
+      w[3] = l.uy;
+    }
+  }
+}
+
 // IDMap::normalized (StereoMatch.cpp:1442-1451)
 __device__ __forceinline__ void normalized(double ux, double uy, double &ox,
                                            double &oy) {
@@ -167,93 +180,50 @@ __device__ inline double tri_tol(double tol, int k, int m, double th_deg) {
   return tol * ad * aa;
 }
 
-// StereoMatch::selectSecondCameraByLineLength (:1307-1370)
-__device__ inline int select_second(const CamT<double> *cams, const double pt[3],
-                                    const double u[3], const int *rem, int nrem) {
-  int best = -1;
-  double best_len = INFINITY;
-  for (int ri = 0; ri < nrem; ++ri) {
-    const int cam = rem[ri];
-    const Line2 L = make_line2d(cams[cam], pt, u);
-    const double px = L.ax, py = L.ay, ux = L.ux, uy = L.uy;
-    const double W = (double)cams[cam].n_col, H = (double)cams[cam].n_row;
-    double qs[4][2];
-    int np = 0;
-    auto push_if = [&](double x, double y) {
-      if (x >= 0.0 && x <= W && y >= 0.0 && y <= H) {
-        qs[np][0] = x;
-        qs[np][1] = y;
-        ++np;
-      }
-    };
-    if (fabs(ux) > 1e-12) {
-      double t = (0.0 - px) / ux;
-      push_if(0.0, py + t * uy);
-      t = (W - 1 - px) / ux;
-      push_if(W - 1, py + t * uy);
+// Length of a 2D line's segment inside the image, as measured by
+// StereoMatch::selectSecondCameraByLineLength (:1307-1370); l = (ax, ay, ux, uy).
+__device__ inline double segment_in_image(const CamT<double> &cam, const double *l) {
+  const double px = l[0], py = l[1], ux = l[2], uy = l[3];
+  const double W = (double)cam.n_col, H = (double)cam.n_row;
+  double qs[4][2];
+  int np = 0;
+  auto push_if = [&](double x, double y) {
+    if (x >= 0.0 && x <= W && y >= 0.0 && y <= H) {
+      qs[np][0] = x;
+      qs[np][1] = y;
+      ++np;
     }
-    if (fabs(uy) > 1e-12) {
-      double t = (0.0 - py) / uy;
-      push_if(px + t * ux, 0.0);
-      t = (H - 1 - py) / uy;
-      push_if(px + t * ux, H - 1);
-    }
-    double uq[4][2];
-    int nu = 0;
-    for (int i = 0; i < np; ++i) {
-      bool dup = false;
-      for (int j = 0; j < nu; ++j)
-        if (fabs(uq[j][0] - qs[i][0]) < 1e-9 && fabs(uq[j][1] - qs[i][1]) < 1e-9) dup = true;
-      if (!dup) {
-        uq[nu][0] = qs[i][0];
-        uq[nu][1] = qs[i][1];
-        ++nu;
-      }
-    }
-    double len = 0.0;
-    if (nu >= 2) {
-      const double dx = uq[0][0] - uq[1][0], dy = uq[0][1] - uq[1][1];
-      len = sqrt(dx * dx + dy * dy);
-    }
-    if (len < best_len - 1e-9) {
-      best_len = len;
-      best = cam;
+  };
+  if (fabs(ux) > 1e-12) {
+    double t = (0.0 - px) / ux;
+    push_if(0.0, py + t * uy);
+    t = (W - 1 - px) / ux;
+    push_if(W - 1, py + t * uy);
+  }
+  if (fabs(uy) > 1e-12) {
+    double t = (0.0 - py) / uy;
+    push_if(px + t * ux, 0.0);
+    t = (H - 1 - py) / uy;
+    push_if(px + t * ux, H - 1);
+  }
+  double uq[4][2];
+  int nu = 0;
+  for (int i = 0; i < np; ++i) {
+    bool dup = false;
+    for (int j = 0; j < nu; ++j)
+      if (fabs(uq[j][0] - qs[i][0]) < 1e-9 && fabs(uq[j][1] - qs[i][1]) < 1e-9) dup = true;
+    if (!dup) {
+      uq[nu][0] = qs[i][0];
+      uq[nu][1] = qs[i][1];
+      ++nu;
     }
   }
-  return best;
-}
-
-// StereoMatch::selectNextCameraByMaxPairAngle (:1375-1411)
-__device__ inline int select_next(const CamT<double> *cams, const double (*pt)[3],
-                                  const double (*u)[3], int n, const int *rem,
-                                  int nrem) {
-  int best = -1;
-  double best_score = -1.0;
-  for (int ri = 0; ri < nrem; ++ri) {
-    const int cam = rem[ri];
-    double lx[kMaxCam], ly[kMaxCam];
-    for (int i = 0; i < n; ++i) {
-      const Line2 L = make_line2d(cams[cam], pt[i], u[i]);
-      lx[i] = L.ux;
-      ly[i] = L.uy;
-    }
-    double score = 0.0;
-    if (n >= 2) {
-      for (int i = 0; i < n; ++i) {
-        for (int j = i + 1; j < n; ++j) {
-          const double s = fabs(lx[i] * ly[j] - ly[i] * lx[j]);
-          if (s > score) score = s;
-          if (score >= 1.0 - 1e-12) break;
-        }
-        if (score >= 1.0 - 1e-12) break;
-      }
-    }
-    if (score > best_score) {
-      best_score = score;
-      best = cam;
-    }
+  double len = 0.0;
+  if (nu >= 2) {
+    const double dx = uq[0][0] - uq[1][0], dy = uq[0][1] - uq[1][1];
+    len = sqrt(dx * dx + dy * dy);
   }
-  return best;
+  return len;
 }
 
 __device__ __forceinline__ bool in_limit(const double lim[6], const double X[3]) {
@@ -264,25 +234,36 @@ __device__ __forceinline__ bool in_limit(const double lim[6], const double X[3])
 // --------------------------------------------------------------------------
 // k_build: wave per reference observation
 // --------------------------------------------------------------------------
-constexpr int kRawCap = 1024;   // level-1 band list (pre back-projection)
-constexpr int kLvlCap = 1024;   // survivors per deeper level
+// Line table: the 2D line of every observation's LOS on every other camera,
+// lines[(c * n_cam + t) * cap + pid] = makeLine2DFromLOS3D(cam t, LOS(c, pid))
+// (ax, ay, ux, uy).  The build's strips, back-projection tests and camera
+// choices all read lines of this form; each is a pure function of (c, t, pid),
+// so computing it once per pass gives the same values as recomputing it.
+__device__ __forceinline__ const double *line_of(const StereoBuffers &b, int c, int t,
+                                                 int pid) {
+  return b.lines + (((long)c * b.n_cam + t) * b.cap + pid) * 4;
+}
 
-// Survivors of level d are the 2D points chosen on the d-th build camera:
-// their camera, the next camera to search, and the parent entry (level
-// d-1) whose path they extend.  NLV = build_count - 2 non-terminal levels.
-template <int NLV> struct BuildLds {
-  int32_t raw[kRawCap];
-  int32_t pid[NLV > 0 ? NLV : 1][kLvlCap];
-  int16_t par[NLV > 0 ? NLV : 1][kLvlCap];
-  int8_t cam[NLV > 0 ? NLV : 1][kLvlCap];
-  int8_t nxt[NLV > 0 ? NLV : 1][kLvlCap];
-  int32_t n[NLV + 1];
-  int32_t n_raw;
-  // reference observation and level-1 strip line
-  double la[2], lU[2], ln[2], ld;
+constexpr int kWin = 512;   // (survivor, point) pair window per level
+constexpr int kList = 512;  // survivors per level list
+
+// Level LV (1 .. B-1) extends paths of length LV by one camera.  Level 1
+// starts from the reference observation; level LV >= 2 from the survivors of
+// level LV-1, held in list LV-2.  Windows and lists are drained depth-first
+// when full, so no capacity bounds the search.
+template <int B> struct BuildLds {
+  static constexpr int NL = B > 2 ? B - 2 : 1;
+  int32_t pair_pid[B - 1][kWin];
+  int16_t pair_e[B - 1][kWin];
+  int32_t l_pid[NL][kList];
+  int16_t l_par[NL][kList];
+  int8_t l_cam[NL][kList];
+  int8_t l_nxt[NL][kList];
+  int32_t n_pair[B - 1];
+  int32_t n_list[NL];
 };
 
-// Strip line of LOS (pt, u) on camera t, in the normal form the strip
+// Strip line of one LOS on the target camera in the normal form the strip
 // intersection uses (IDMap::computeStripIntersection, :1475-1566).
 struct Strip {
   double ax, ay;   // point on the line
@@ -290,20 +271,24 @@ struct Strip {
   double n0, n1, d;
 };
 
-__device__ __forceinline__ Strip make_strip(const CamT<double> &ct, const double pt[3],
-                                            const double u[3]) {
-  const Line2 l2 = make_line2d(ct, pt, u);
+__device__ __forceinline__ Strip strip_of(const double *l) {
   Strip s;
-  s.ax = l2.ax;
-  s.ay = l2.ay;
-  normalized(l2.ux, l2.uy, s.Ux, s.Uy);
+  s.ax = l[0];
+  s.ay = l[1];
+  normalized(l[2], l[3], s.Ux, s.Uy);
   s.n0 = -s.Uy;
   s.n1 = s.Ux;
-  s.d = -(s.n0 * l2.ax + s.n1 * l2.ay);
+  s.d = -(s.n0 * s.ax + s.n1 * s.ay);
   return s;
 }
 
-// Cell-row span of one row for one strip (same arithmetic as the
+__device__ __forceinline__ bool in_strip(const Strip &s, double qx, double qy, double tol2) {
+  const double dx = qx - s.ax, dy = qy - s.ay;
+  const double z = dy * s.Ux - dx * s.Uy;
+  return !(z * z > tol2);
+}
+
+// Cell-column span of one cell row for one strip (same arithmetic as the
 // reference); returns false if the row is empty for this strip.
 __device__ __forceinline__ bool strip_span(const Strip &s, int cy, int cell_px, int cols,
                                            double tolg, int &a, int &bb) {
@@ -342,8 +327,8 @@ __device__ __forceinline__ void pair_rows(const Strip &si, const Strip &sj, int 
   const double Tj = tolg + (fabs(sj.n0) + fabs(sj.n1)) * hw;
   // centre crossing: n_i.x = -d_i, n_j.x = -d_j
   const double py = (si.n0 * (-sj.d) - sj.n0 * (-si.d)) / det;
-  // y-components of the unit offsets along each strip's normal band
-  const double wiy = -sj.n0 / det;  // moves n_i.x by 1, keeps n_j.x
+  // y-change per unit of each strip's signed distance
+  const double wiy = -sj.n0 / det;
   const double wjy = si.n0 / det;
   const double ext = fabs(Ti * wiy) + fabs(Tj * wjy);
   // clamp in floating point before the integer conversion (far crossings)
@@ -356,101 +341,335 @@ __device__ __forceinline__ void pair_rows(const Strip &si, const Strip &sj, int 
   if (r1 > rows - 1) r1 = rows - 1;
 }
 
-template <int MAXB> struct Path {
-  int cam[MAXB];
-  int pid[MAXB];
-  double pt[MAXB][2];
-  double P[MAXB][3];  // LOS origin (camera centre)
-  double U[MAXB][3];  // LOS unit direction
+template <int B> struct Path {
+  int cam[B];
+  int pid[B];
+  double pt[B][2];
+  double P[B][3];  // LOS origin (camera centre)
+  double U[B][3];  // LOS unit direction
 };
 
-template <int MAXB>
-__device__ __forceinline__ void path_push(Path<MAXB> &P, int k, const StereoBuffers &b,
-                                          int cam, int pid) {
+template <int B>
+__device__ __forceinline__ void path_set(Path<B> &P, int k, const StereoBuffers &b, int cam,
+                                         int pid) {
   P.cam[k] = cam;
   P.pid[k] = pid;
   const double *q = b.xy + ((long)cam * b.cap + pid) * 2;
   P.pt[k][0] = q[0];
   P.pt[k][1] = q[1];
   const double *u = b.los + ((long)cam * b.cap + pid) * 4;
+#pragma unroll
   for (int c = 0; c < 3; ++c) {
     P.P[k][c] = b.cams[cam].Tinv[c];
     P.U[k][c] = u[c];
   }
 }
 
-// Extend path (length k) by point pid on camera t: back-projection
+// Path of survivor e of list K-2 (length K): slot 0 is the reference
+// observation, slot d the entry of list d-1 on the chain of parents.
+template <int B, int K>
+__device__ __forceinline__ void load_path(const StereoBuffers &b, const BuildLds<B> &L,
+                                          int ref, int r, int e, Path<B> &P) {
+  path_set<B>(P, 0, b, ref, r);
+  int q = e;
+#pragma unroll
+  for (int d = K - 1; d >= 1; --d) {
+    path_set<B>(P, d, b, L.l_cam[d - 1][q], L.l_pid[d - 1][q]);
+    q = L.l_par[d - 1][q];
+  }
+}
+
+// cos(6.001 deg): a smallest pairwise parallax above 6.001 deg (and below
+// 173.999 deg) makes calTriangulateTol's angle factor exactly 1.
+constexpr double kCosParallax = 0.9945200708509531;
+
+// Extend path P (length K) by point pid on camera t: back-projection
 // (checkBackProjection, :1206-1227), quick triangulation (:397-407), then
-// either checkMatch + emission (terminal) or the next-camera choice.
+// either checkMatch + emission (terminal) or the next-camera choice
+// (selectNextCameraByMaxPairAngle, :1375-1411).
 // Returns 0 reject, 1 emitted, 2 survivor (next camera in *next).
-template <int MAXB>
-__device__ int extend(const StereoBuffers &b, const StereoParams &p, Path<MAXB> &P, int k,
-                      int t, int pid, int *next) {
-  const CamT<double> *cams = b.cams;
+template <int B, int K>
+__device__ int extend(const StereoBuffers &b, const StereoParams &p, Path<B> &P, int t,
+                      int pid, int *next) {
+  constexpr int KK = K + 1;
   const double tol2 = p.tol2d * p.tol2d;
-  const double *ut = b.los + ((long)t * b.cap + pid) * 4;
-  const double u_t[3] = {ut[0], ut[1], ut[2]};
-  for (int j = 0; j < k; ++j) {
-    const Line2 l2 = make_line2d(cams[P.cam[j]], cams[t].Tinv, u_t);
-    const double d0 = P.pt[j][0] - l2.ax, d1 = P.pt[j][1] - l2.ay;
-    const double z = d1 * l2.ux - d0 * l2.uy;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const double *l = line_of(b, t, P.cam[j], pid);
+    const double d0 = P.pt[j][0] - l[0], d1 = P.pt[j][1] - l[1];
+    const double z = d1 * l[2] - d0 * l[3];
     if (z * z > tol2) return 0;
   }
-  path_push<MAXB>(P, k, b, t, pid);
-  const int kk = k + 1;
+  path_set<B>(P, K, b, t, pid);
   {
-    const double th = min_parallax_deg(P.U, kk);
-    const double tq = tri_tol(p.tol3d, kk, p.build_count, th);
-    double X[3], e;
-    if (!triangulate(P.P, P.U, kk, X, e) || !(e <= tq)) return 0;
-  }
-  if (kk == p.build_count) {
-    // checkMatch, all-active branch (:530-540): LOS in camera order
-    double cp[MAXB][3], cu[MAXB][3];
-    int32_t ids[kMaxCam];
-    int n = 0;
-    for (int c = 0; c < b.n_cam; ++c) {
-      ids[c] = -1;
-      for (int j = 0; j < kk; ++j)
-        if (P.cam[j] == c) {
-          ids[c] = P.pid[j];
-          for (int d = 0; d < 3; ++d) {
-            cp[n][d] = P.P[j][d];
-            cu[n][d] = P.U[j][d];
-          }
-          ++n;
-        }
+    // computeMinParallaxDeg: the smallest pairwise angle is the acos of the
+    // largest clamped dot product (acos is monotone)
+    double mx = -1.0;
+#pragma unroll
+    for (int i = 0; i < KK; ++i)
+#pragma unroll
+      for (int j = i + 1; j < KK; ++j) {
+        double dot = P.U[i][0] * P.U[j][0] + P.U[i][1] * P.U[j][1] + P.U[i][2] * P.U[j][2];
+        dot = fmax(-1.0, fmin(1.0, dot));
+        mx = fmax(mx, dot);
+      }
+    double tq;
+    if (mx < kCosParallax && mx > -kCosParallax) {
+      const int kk = KK > 2 ? KK : 2, mm = B > 2 ? B : 2;
+      tq = p.tol3d * sqrt((double)mm / (double)kk);
+    } else {
+      tq = tri_tol(p.tol3d, KK, B, acos(mx) * 180.0 / M_PI);
     }
+    double X[3], e;
+    if (!triangulate(P.P, P.U, KK, X, e) || !(e <= tq)) return 0;
+  }
+  if constexpr (KK == B) {
+    // checkMatch, all-active branch (:530-540): LOS in camera order
+    int rank[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      rank[j] = 0;
+#pragma unroll
+      for (int i = 0; i < B; ++i) rank[j] += P.cam[i] < P.cam[j] ? 1 : 0;
+    }
+    double cp[B][3], cu[B][3];
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+      for (int j = 0; j < B; ++j)
+        if (rank[j] == r) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            cp[r][d] = P.P[j][d];
+            cu[r][d] = P.U[j][d];
+          }
+        }
     double Xc[3], e;
-    if (!triangulate(cp, cu, n, Xc, e)) {
+    if (!triangulate(cp, cu, B, Xc, e)) {
       atomicOr(&b.flags[6], 1);
       return 0;
     }
     if (e > p.tol3d || !in_limit(p.limit, Xc)) return 0;
     const int slot = atomicAdd(b.n_cand, 1);
     if (slot < b.cap_cand) {
-      for (int c = 0; c < b.n_cam; ++c) b.cand_ids[(long)slot * b.n_cam + c] = ids[c];
+      for (int c = 0; c < b.n_cam; ++c) {
+        int id = -1;
+#pragma unroll
+        for (int j = 0; j < B; ++j) id = P.cam[j] == c ? P.pid[j] : id;
+        b.cand_ids[(long)slot * b.n_cam + c] = id;
+      }
       b.cand_e[slot] = e;
     } else {
       atomicOr(&b.flags[0], 1);
     }
     return 1;
+  } else {
+    unsigned rem = 0;
+    for (int c = 0; c < b.n_cam; ++c) rem |= p.active[c] ? (1u << c) : 0u;
+#pragma unroll
+    for (int j = 0; j < KK; ++j) rem &= ~(1u << P.cam[j]);
+    if (rem == 0) return 0;
+    if ((rem & (rem - 1)) == 0) {
+      // a single remaining camera: its score (>= 0) beats the initial -1
+      *next = __ffs(rem) - 1;
+      return 2;
+    }
+    int best = -1;
+    double best_score = -1.0;
+    for (int cam = 0; cam < b.n_cam; ++cam) {
+      if (!(rem & (1u << cam))) continue;
+      double lx[KK], ly[KK];
+#pragma unroll
+      for (int i = 0; i < KK; ++i) {
+        const double *l = line_of(b, P.cam[i], cam, P.pid[i]);
+        lx[i] = l[2];
+        ly[i] = l[3];
+      }
+      double score = 0.0;
+      bool stop = false;
+#pragma unroll
+      for (int i = 0; i < KK; ++i)
+#pragma unroll
+        for (int j = i + 1; j < KK; ++j) {
+          if (!stop) {
+            const double s = fabs(lx[i] * ly[j] - ly[i] * lx[j]);
+            if (s > score) score = s;
+            if (score >= 1.0 - 1e-12) stop = true;
+          }
+        }
+      if (score > best_score) {
+        best_score = score;
+        best = cam;
+      }
+    }
+    *next = best;
+    return best < 0 ? 0 : 2;
   }
-  int rem[kMaxCam], nrem = 0;
-  for (int c = 0; c < b.n_cam; ++c) {
-    if (!p.active[c]) continue;
-    bool used = false;
-    for (int j = 0; j < kk; ++j) used = used || (P.cam[j] == c);
-    if (!used) rem[nrem++] = c;
-  }
-  *next = select_next(cams, P.P, P.U, kk, rem, nrem);
-  return *next < 0 ? 0 : 2;
 }
 
-template <int MAXB>
+// Append a survivor of level LV to list LV-1.  The caller guarantees room.
+template <int B, int LV>
+__device__ __forceinline__ void push_survivor(BuildLds<B> &L, int pid, int par, int cam,
+                                              int nxt) {
+  const int pos = atomicAdd(&L.n_list[LV - 1], 1);
+  L.l_pid[LV - 1][pos] = pid;
+  L.l_par[LV - 1][pos] = (int16_t)par;
+  L.l_cam[LV - 1][pos] = (int8_t)cam;
+  L.l_nxt[LV - 1][pos] = (int8_t)nxt;
+}
+
+template <int B, int LV>
+__device__ void run_level(const StereoBuffers &b, const StereoParams &p, BuildLds<B> &L,
+                          int ref, int r);
+
+// Drain list LV-1 (the survivors of level LV) through level LV+1 if fewer
+// than a wave's worth of slots are left; `force` drains any remainder.
+template <int B, int LV>
+__device__ __forceinline__ void make_room(const StereoBuffers &b, const StereoParams &p,
+                                          BuildLds<B> &L, int ref, int r, bool force) {
+  if constexpr (LV + 1 < B) {
+    __syncthreads();
+    const int n = L.n_list[LV - 1];
+    if (force ? n > 0 : n + kWave > kList) {
+      run_level<B, LV + 1>(b, p, L, ref, r);
+      __syncthreads();
+      if (lane_id() == 0) L.n_list[LV - 1] = 0;
+      __syncthreads();
+    }
+  }
+}
+
+// Per-lane cursor over the contiguous point range of one cell row: within a
+// row, cells cx in [xa, xb] are consecutive in the CSR index.
+struct RowCursor {
+  int kk, kend;
+};
+
+__device__ __forceinline__ void row_range(const int32_t *cstart, const int32_t *ccnt,
+                                          int cy, int cols, int xa, int xb, RowCursor &c) {
+  const int c0 = cy * cols + xa, c1 = cy * cols + xb;
+  c.kk = cstart[c0];
+  c.kend = cstart[c1] + ccnt[c1];
+}
+
+// Levels >= 2: lane = survivor of list LV-2, enumerating the cell rows of
+// its K-strip intersection on its next camera; pairs are compacted into
+// the level's window, then extended lane-parallel.
+template <int B, int LV>
+__device__ void run_level(const StereoBuffers &b, const StereoParams &p, BuildLds<B> &L,
+                          int ref, int r) {
+  constexpr int K = LV;  // path length of the input survivors
+  const int lane = lane_id();
+  const int rows = b.rows_cell, cols = b.cols_cell;
+  const long ncell = (long)rows * cols;
+  const double tolg = p.tol2d + 1e-6;
+  const double tol2 = p.tol2d * p.tol2d;
+  __syncthreads();
+  const int n_in = L.n_list[LV - 2];
+  for (int c0 = 0; c0 < n_in; c0 += kWave) {
+    const int e = c0 + lane;
+    bool done = e >= n_in;
+    Strip S[K];
+    int t = 0, cy = 0, r1 = -1;
+    RowCursor cur{0, 0};
+    if (!done) {
+      t = L.l_nxt[LV - 2][e];
+      int cam[K], pidv[K];
+      cam[0] = ref;
+      pidv[0] = r;
+      int q = e;
+#pragma unroll
+      for (int d = K - 1; d >= 1; --d) {
+        cam[d] = L.l_cam[d - 1][q];
+        pidv[d] = L.l_pid[d - 1][q];
+        q = L.l_par[d - 1][q];
+      }
+#pragma unroll
+      for (int j = 0; j < K; ++j) S[j] = strip_of(line_of(b, cam[j], t, pidv[j]));
+      int r0 = 0;
+      r1 = rows - 1;
+#pragma unroll
+      for (int j = 1; j < K; ++j) pair_rows(S[0], S[j], b.cell_px, tolg, rows, r0, r1);
+      cy = r0 - 1;
+    }
+    const int32_t *cstart = b.cell_start + t * ncell;
+    const int32_t *ccnt = b.cell_cnt + t * ncell;
+    const int32_t *cpid = b.cell_pid + (long)t * b.cap;
+    const double *xy = b.xy + (long)t * b.cap * 2;
+    while (true) {
+      // fill the window
+      __syncthreads();
+      if (lane == 0) L.n_pair[LV - 1] = 0;
+      __syncthreads();
+      while (true) {
+        if (!done) {
+          while (cur.kk == cur.kend) {
+            ++cy;
+            if (cy > r1) {
+              done = true;
+              break;
+            }
+            int xa = 0, xb = cols - 1;
+            bool ok = true;
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+              int a, bb;
+              if (ok) {
+                ok = strip_span(S[j], cy, b.cell_px, cols, tolg, a, bb);
+                xa = a > xa ? a : xa;
+                xb = bb < xb ? bb : xb;
+                ok = ok && xa <= xb;
+              }
+            }
+            if (ok) row_range(cstart, ccnt, cy, cols, xa, xb, cur);
+          }
+          if (!done) {
+            const int pid = cpid[cur.kk];
+            const double qx = xy[2 * pid], qy = xy[2 * pid + 1];
+            bool acc = true;
+#pragma unroll
+            for (int j = 0; j < K; ++j) acc = acc && in_strip(S[j], qx, qy, tol2);
+            if (acc) {
+              const int pos = atomicAdd(&L.n_pair[LV - 1], 1);
+              if (pos < kWin) {
+                L.pair_pid[LV - 1][pos] = pid;
+                L.pair_e[LV - 1][pos] = (int16_t)e;
+                ++cur.kk;
+              }  // else: retried in the next window
+            } else {
+              ++cur.kk;
+            }
+          }
+        }
+        __syncthreads();
+        if (L.n_pair[LV - 1] >= kWin || !__any(!done)) break;
+      }
+      const int n = L.n_pair[LV - 1] < kWin ? L.n_pair[LV - 1] : kWin;
+      // extend the window's pairs lane-parallel
+      for (int i0 = 0; i0 < n; i0 += kWave) {
+        make_room<B, LV>(b, p, L, ref, r, false);
+        const int i = i0 + lane;
+        if (i < n) {
+          const int es = L.pair_e[LV - 1][i];
+          const int pid = L.pair_pid[LV - 1][i];
+          const int tt = L.l_nxt[LV - 2][es];
+          Path<B> P;
+          load_path<B, K>(b, L, ref, r, es, P);
+          int next = -1;
+          if (extend<B, K>(b, p, P, tt, pid, &next) == 2) {
+            if constexpr (LV + 1 < B) push_survivor<B, LV>(L, pid, es, tt, next);
+          }
+        }
+      }
+      if (!__any(!done)) break;
+    }
+  }
+  make_room<B, LV>(b, p, L, ref, r, true);
+}
+
+template <int B>
 __global__ void __launch_bounds__(kWave) k_build(StereoBuffers b, StereoParams p) {
-  constexpr int NLV = MAXB - 2;
-  __shared__ BuildLds<NLV> L;
+  __shared__ BuildLds<B> L;
   const int lane = lane_id();
   // reference camera: fewest observations among active cameras, lowest id
   // on ties (StereoMatch.cpp:70-85)
@@ -462,171 +681,90 @@ __global__ void __launch_bounds__(kWave) k_build(StereoBuffers b, StereoParams p
       ref = c;
     }
   const int nref = best;
-  const CamT<double> *cams = b.cams;
   const int rows = b.rows_cell, cols = b.cols_cell;
   const long ncell = (long)rows * cols;
   const double tolg = p.tol2d + 1e-6;
   const double tol2 = p.tol2d * p.tol2d;
+  if (lane == 0)
+    for (int d = 0; d < BuildLds<B>::NL; ++d) L.n_list[d] = 0;
   while (true) {
     int r = 0;
     if (lane == 0) r = atomicAdd(b.work, 1);
     r = __shfl(r, 0, kWave);
     if (r >= nref) break;
-    Path<MAXB> P0;
-    path_push<MAXB>(P0, 0, b, ref, r);
-    int rem[kMaxCam], nrem = 0;
-    for (int c = 0; c < b.n_cam; ++c)
-      if (p.active[c] && c != ref) rem[nrem++] = c;
-    if (nrem == 0) continue;
-    const int t1 = select_second(cams, P0.P[0], P0.U[0], rem, nrem);
-    if (t1 < 0) continue;
-    // ---- level 1: one band, wave-cooperative enumeration over cell rows
-    const Strip s1 = make_strip(cams[t1], P0.P[0], P0.U[0]);
-    if (lane == 0) {
-      L.n_raw = 0;
-      for (int d = 0; d <= NLV; ++d) L.n[d] = 0;
-    }
-    __syncthreads();
+    // selectSecondCameraByLineLength (:1307-1370): shortest in-image segment
+    int t1 = -1;
     {
-      const int32_t *cstart = b.cell_start + t1 * ncell;
-      const int32_t *ccnt = b.cell_cnt + t1 * ncell;
-      const int32_t *cpid = b.cell_pid + (long)t1 * b.cap;
-      const double *xy = b.xy + (long)t1 * b.cap * 2;
-      for (int cy0 = 0; cy0 < rows; cy0 += kWave) {
-        const int cy = cy0 + lane;
-        int xa = 0, xb = -1;
-        bool valid = cy < rows && strip_span(s1, cy, b.cell_px, cols, tolg, xa, xb);
-        int cellid = valid ? cy * cols + xa : 0;
-        const int cell_end = valid ? cy * cols + xb : -1;
-        int kk = valid ? cstart[cellid] : 0, kend = valid ? kk + ccnt[cellid] : 0;
-        while (valid && kk == kend && cellid < cell_end) {
-          ++cellid;
-          kk = cstart[cellid];
-          kend = kk + ccnt[cellid];
-        }
-        bool has = valid && kk < kend;
-        while (__any(has)) {
-          bool acc = false;
-          int pid = -1;
-          if (has) {
-            pid = cpid[kk];
-            const double dx = xy[2 * pid] - s1.ax, dy = xy[2 * pid + 1] - s1.ay;
-            const double z = dy * s1.Ux - dx * s1.Uy;
-            acc = !(z * z > tol2);
-            ++kk;
-            while (kk == kend && cellid < cell_end) {
-              ++cellid;
-              kk = cstart[cellid];
-              kend = kk + ccnt[cellid];
-            }
-            has = kk < kend;
-          }
-          const uint64_t m = ballot(acc);
-          if (m) {
-            const int base = L.n_raw;
-            const int pos = base + mbcnt(m);
-            if (acc) {
-              if (pos < kRawCap)
-                L.raw[pos] = pid;
-              else
-                atomicOr(&b.flags[1], 1);
-            }
-            __syncthreads();
-            if (lane == 0) L.n_raw = base + (int)__popcll(m);
-            __syncthreads();
-          }
+      double best_len = INFINITY;
+      for (int cam = 0; cam < b.n_cam; ++cam) {
+        if (!p.active[cam] || cam == ref) continue;
+        const double *l = line_of(b, ref, cam, r);
+        const double len = segment_in_image(b.cams[cam], l);
+        if (len < best_len - 1e-9) {
+          best_len = len;
+          t1 = cam;
         }
       }
     }
-    __syncthreads();
-    const int n_raw = L.n_raw < kRawCap ? L.n_raw : kRawCap;
-    // level-1 extension, lane-parallel over the band list
-    for (int i = lane; i < n_raw; i += kWave) {
-      int next = -1;
-      const int res = extend<MAXB>(b, p, P0, 1, t1, L.raw[i], &next);
-      if (res == 2 && NLV > 0) {
-        const int pos = atomicAdd(&L.n[0], 1);
-        if (pos < kLvlCap) {
-          L.pid[0][pos] = L.raw[i];
-          L.par[0][pos] = -1;
-          L.cam[0][pos] = (int8_t)t1;
-          L.nxt[0][pos] = (int8_t)next;
-        } else {
-          atomicOr(&b.flags[1], 1);
-        }
-      }
-    }
-    __syncthreads();
-    // ---- deeper levels: one survivor per lane, lane-serial enumeration of
-    // the (small) strip intersection on its next camera
-    for (int lv = 1; lv <= NLV; ++lv) {
-      const int n_in = L.n[lv - 1] < kLvlCap ? L.n[lv - 1] : kLvlCap;
-      for (int e = lane; e < n_in; e += kWave) {
-        // rebuild the path: ref + one entry per level
-        Path<MAXB> P = P0;
-        int chain[MAXB];
-        int q = e;
-        for (int d = lv - 1; d >= 0; --d) {
-          chain[d] = q;
-          q = L.par[d][q];
-        }
-        for (int d = 0; d < lv; ++d)
-          path_push<MAXB>(P, d + 1, b, L.cam[d][chain[d]], L.pid[d][chain[d]]);
-        const int k = lv + 1;              // path length
-        const int t = L.nxt[lv - 1][e];    // camera to search
-        Strip S[MAXB];
-        for (int j = 0; j < k; ++j) S[j] = make_strip(cams[t], P.P[j], P.U[j]);
-        int r0 = 0, r1 = rows - 1;
-        for (int j = 1; j < k; ++j) pair_rows(S[0], S[j], b.cell_px, tolg, rows, r0, r1);
-        const int32_t *cstart = b.cell_start + t * ncell;
-        const int32_t *ccnt = b.cell_cnt + t * ncell;
-        const int32_t *cpid = b.cell_pid + (long)t * b.cap;
-        const double *xy = b.xy + (long)t * b.cap * 2;
-        for (int cy = r0; cy <= r1; ++cy) {
-          int xa = 0, xb = cols - 1;
-          bool ok = true;
-          for (int j = 0; j < k && ok; ++j) {
-            int a, bb;
-            ok = strip_span(S[j], cy, b.cell_px, cols, tolg, a, bb);
-            if (ok) {
-              xa = a > xa ? a : xa;
-              xb = bb < xb ? bb : xb;
-              ok = xa <= xb;
-            }
-          }
-          if (!ok) continue;
-          for (int cx = xa; cx <= xb; ++cx) {
-            const int cell = cy * cols + cx;
-            const int s0 = cstart[cell], s1n = s0 + ccnt[cell];
-            for (int kk = s0; kk < s1n; ++kk) {
-              const int pid = cpid[kk];
-              const double qx = xy[2 * pid], qy = xy[2 * pid + 1];
-              bool in = true;
-              for (int j = 0; j < k && in; ++j) {
-                const double dx = qx - S[j].ax, dy = qy - S[j].ay;
-                const double z = dy * S[j].Ux - dx * S[j].Uy;
-                in = !(z * z > tol2);
-              }
-              if (!in) continue;
-              int next = -1;   // extend() only writes slot k of the path
-              const int res = extend<MAXB>(b, p, P, k, t, pid, &next);
-              if (res == 2 && lv < NLV) {
-                const int pos = atomicAdd(&L.n[lv], 1);
-                if (pos < kLvlCap) {
-                  L.pid[lv][pos] = pid;
-                  L.par[lv][pos] = (int16_t)e;
-                  L.cam[lv][pos] = (int8_t)t;
-                  L.nxt[lv][pos] = (int8_t)next;
-                } else {
-                  atomicOr(&b.flags[1], 1);
-                }
-              }
-            }
-          }
-        }
-      }
+    if (t1 < 0) continue;
+    const Strip s1 = strip_of(line_of(b, ref, t1, r));
+    const int32_t *cstart = b.cell_start + t1 * ncell;
+    const int32_t *ccnt = b.cell_cnt + t1 * ncell;
+    const int32_t *cpid = b.cell_pid + (long)t1 * b.cap;
+    const double *xy = b.xy + (long)t1 * b.cap * 2;
+    // level 1: one band over all cell rows, lane = cell row (stride 64)
+    int cy = lane - kWave;
+    bool done = false;
+    RowCursor cur{0, 0};
+    while (true) {
       __syncthreads();
+      if (lane == 0) L.n_pair[0] = 0;
+      __syncthreads();
+      while (true) {
+        if (!done) {
+          while (cur.kk == cur.kend) {
+            cy += kWave;
+            if (cy >= rows) {
+              done = true;
+              break;
+            }
+            int xa, xb;
+            if (strip_span(s1, cy, b.cell_px, cols, tolg, xa, xb))
+              row_range(cstart, ccnt, cy, cols, xa, xb, cur);
+          }
+          if (!done) {
+            const int pid = cpid[cur.kk];
+            if (in_strip(s1, xy[2 * pid], xy[2 * pid + 1], tol2)) {
+              const int pos = atomicAdd(&L.n_pair[0], 1);
+              if (pos < kWin) {
+                L.pair_pid[0][pos] = pid;
+                ++cur.kk;
+              }
+            } else {
+              ++cur.kk;
+            }
+          }
+        }
+        __syncthreads();
+        if (L.n_pair[0] >= kWin || !__any(!done)) break;
+      }
+      const int n = L.n_pair[0] < kWin ? L.n_pair[0] : kWin;
+      for (int i0 = 0; i0 < n; i0 += kWave) {
+        make_room<B, 1>(b, p, L, ref, r, false);
+        const int i = i0 + lane;
+        if (i < n) {
+          const int pid = L.pair_pid[0][i];
+          Path<B> P;
+          path_set<B>(P, 0, b, ref, r);
+          int next = -1;
+          if (extend<B, 1>(b, p, P, t1, pid, &next) == 2) {
+            if constexpr (B > 2) push_survivor<B, 1>(L, pid, -1, t1, next);
+          }
+        }
+      }
+      if (!__any(!done)) break;
     }
+    make_room<B, 1>(b, p, L, ref, r, true);
   }
 }
 // --------------------------------------------------------------------------
