@@ -357,7 +357,7 @@ olpt_status check_params(olpt_ctx *c, const olpt_ipr_params *p) {
   if (p->radius_obj <= 0 || p->ratio_augimg <= 0)
     return fail(OLPT_E_INVALID_ARGUMENT, "radius_obj and ratio_augimg must be positive");
   const int box = (int)std::lround(2.0 * p->radius_obj);
-  const int roi = (int)std::lround(2.0 * p->radius_obj * p->ratio_augimg);
+  const int roi = (int)std::floor(2.0 * p->radius_obj * p->ratio_augimg + 0.5 + 1e-3);
   if (roi * roi > 4 * 64 || box > 24)
     return fail(OLPT_E_UNSUPPORTED, "ROI larger than 16x16 pixels is not supported");
   if (p->idmap_cell_px < 1) return fail(OLPT_E_INVALID_ARGUMENT, "idmap_cell_px must be >= 1");

@@ -11,13 +11,15 @@
 //                    the maximum projection over them and writes
 //                    res = orig - max(0, max_k p_k)  (the reference's min
 //                    fusion, Shake.cpp:226-241) - no atomics on pixels
-//   k_shake          one wave64 per object: lanes own the pixels of the 8x8
-//                    augmented ROI of each camera (buildROIInfo, :279-328),
-//                    each residual evaluation (calShakeResidue, :968-1045)
-//                    is a lane-parallel squared difference plus a butterfly
-//                    reduction (bitwise identical in all lanes, so the
-//                    branchy per-axis search of shakeOneObject (:409-665)
-//                    stays wave-uniform); then calObjectScore (:668-782).
+//   k_shake          one wave64 per object, one lane group per camera: the
+//                    group owns the camera's projection, OTF parameters and
+//                    the pixels of its augmented ROI (buildROIInfo,
+//                    :279-328), so one residual evaluation (calShakeResidue,
+//                    :968-1045) covers every camera in a single pass; group
+//                    butterfly sums and an in-order cross-camera mean leave
+//                    the same value in every lane, so the branchy per-axis
+//                    search of shakeOneObject (:409-665) stays wave-uniform;
+//                    then calObjectScore (:668-782).
 // After the iterations: markRepeatedObj (:785-823) on a uniform grid with
 // the greedy pass over the (rare) objects that have a neighbour, the
 // mean-relative ghost threshold (:118-136), order-preserving filter
@@ -212,57 +214,27 @@ __global__ void k_copy_cam(const T *__restrict__ src, T *__restrict__ dst, long 
 }
 
 // --------------------------------------------------------------------------
-// k_shake: wave per object
+// k_shake: wave per object, lanes split by camera
 // --------------------------------------------------------------------------
-template <typename T, int NC, int SLOTS> struct ShakeCtx {
-  int n_cam;
-  int active[NC];
-  PRange roi[NC];
-  int roiw[NC];
-  int npix[NC];
-  T aug[NC][SLOTS];   // this lane's augmented pixels
-  T r_px;
-};
-
-// one residual evaluation at X (calShakeResidue); wave-uniform result
-template <typename T, int NC, int SLOTS>
-__device__ T eval_residue(const ShakeBuffers<T> &b, const ShakeCtx<T, NC, SLOTS> &s,
-                          const T X[3], T *uv /* [NC][2] out */) {
-  const int lane = lane_id();
-  T acc = T(0);
-  int used = 0;
+// Lane group k (kWave / NC lanes) owns camera k: its projection, OTF
+// parameters and SLOTS pixels of the camera's augmented ROI.  A residual
+// evaluation is one pass over all cameras at once; per-camera sums are
+// butterfly reductions inside the group (bitwise identical in its lanes),
+// and the cross-camera mean is summed in camera order from the group
+// leaders, so every lane holds the same value and the branchy per-axis
+// search of shakeOneObject stays wave-uniform.
+template <int LPG, typename T> __device__ __forceinline__ T group_sum(T v) {
 #pragma unroll
-  for (int k = 0; k < NC; ++k) {
-    if (k >= s.n_cam) break;
-    T u, v;
-    project(b.cams[k], X, u, v);
-    uv[2 * k] = u;
-    uv[2 * k + 1] = v;
-    if (!s.active[k]) continue;
-    if (s.roi[k].rmax - s.roi[k].rmin <= 0 || s.roi[k].cmax - s.roi[k].cmin <= 0) continue;
-    T P[5];
-    otf_param(b.otf, k, X, P);
-    const int prmin = (int)floor(v - s.r_px), prmax = (int)ceil(v + s.r_px + T(1));
-    const int pcmin = (int)floor(u - s.r_px), pcmax = (int)ceil(u + s.r_px + T(1));
-    T sse = T(0);
+  for (int off = LPG / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+template <int LPG, typename T> __device__ __forceinline__ T group_max(T v) {
 #pragma unroll
-    for (int sl = 0; sl < SLOTS; ++sl) {
-      const int idx = sl * kWave + lane;
-      if (idx < s.npix[k]) {
-        const int row = s.roi[k].rmin + idx / s.roiw[k];
-        const int col = s.roi[k].cmin + idx % s.roiw[k];
-        T pred = T(0);
-        if (row >= prmin && row < prmax && col >= pcmin && col < pcmax)
-          pred = gauss_int<T>(P, u, v, row, col);
-        const T d = s.aug[k][sl] - pred;
-        sse += d * d;
-      }
-    }
-    sse = wave_sum(sse);
-    acc += sse / T(s.npix[k]);
-    ++used;
+  for (int off = LPG / 2; off > 0; off >>= 1) {
+    const T o = __shfl_xor(v, off, kWave);
+    v = o > v ? o : v;
   }
-  return used > 1 ? acc / T(used) : T(INFINITY);
+  return v;
 }
 
 template <typename T>
@@ -320,67 +292,137 @@ __device__ void polyfit2(const double *x, const double *y, int n, double c[3]) {
   }
 }
 
+// Per-lane state of one object in one camera (the lane's group camera).
+template <typename T, int SLOTS> struct CamLane {
+  int k;           // camera of this lane's group
+  bool cam_ok;     // k < n_cam
+  bool act;        // camera active
+  PRange roi;      // augmented ROI (zero range for inactive cameras)
+  int roiw, npix;
+  T aug[SLOTS];    // this lane's augmented pixels
+};
+
+// calShakeResidue (Shake.cpp:968-1045) at Xe; (ue, ve) receive the lane
+// camera's projection.  Wave-uniform result.
+template <typename T, int NC, int SLOTS>
+__device__ __forceinline__ T eval_residue(const ShakeBuffers<T> &b, const CamLane<T, SLOTS> &c,
+                                          T r_px, const T Xe[3], T &ue, T &ve) {
+  constexpr int LPG = kWave / NC;
+  const int sub = lane_id() % LPG;
+  T sse = T(0);
+  ue = T(0);
+  ve = T(0);
+  if (c.cam_ok) project(b.cams[c.k], Xe, ue, ve);
+  const bool use = c.act && c.roi.rmax - c.roi.rmin > 0 && c.roi.cmax - c.roi.cmin > 0;
+  if (use) {
+    T P[5];
+    otf_param(b.otf, c.k, Xe, P);
+    const int prmin = (int)floor(ve - r_px), prmax = (int)ceil(ve + r_px + T(1));
+    const int pcmin = (int)floor(ue - r_px), pcmax = (int)ceil(ue + r_px + T(1));
+#pragma unroll
+    for (int sl = 0; sl < SLOTS; ++sl) {
+      const int idx = sl * LPG + sub;
+      if (idx < c.npix) {
+        const int row = c.roi.rmin + idx / c.roiw;
+        const int col = c.roi.cmin + idx % c.roiw;
+        T pred = T(0);
+        if (row >= prmin && row < prmax && col >= pcmin && col < pcmax)
+          pred = gauss_int<T>(P, ue, ve, row, col);
+        const T d = c.aug[sl] - pred;
+        sse += d * d;
+      }
+    }
+  }
+  sse = group_sum<LPG>(sse);
+  const T val = use ? sse / T(c.npix) : T(0);
+  const uint64_t um = ballot(use && sub == 0);
+  T acc = T(0);
+  int used = 0;
+#pragma unroll
+  for (int kk = 0; kk < NC; ++kk) {
+    const T vk = __shfl(val, kk * LPG, kWave);
+    if ((um >> (kk * LPG)) & 1ull) {
+      acc += vk;
+      ++used;
+    }
+  }
+  return used > 1 ? acc / T(used) : T(INFINITY);
+}
+
+// Whether the projection of Xt falls outside the ROI of any camera
+// (shakeOneObject's pre-evaluation check; every camera takes part).
+template <typename T, int NC, int SLOTS>
+__device__ __forceinline__ bool any_out(const ShakeBuffers<T> &b, const CamLane<T, SLOTS> &c,
+                                        const T Xt[3]) {
+  bool out = false;
+  if (c.cam_ok) {
+    T u, v;
+    project(b.cams[c.k], Xt, u, v);
+    out = !in_roi_d(c.roi, u, v);
+  }
+  return __any(out);
+}
+
 template <typename T, int NC, int SLOTS>
 __global__ void __launch_bounds__(kWave)
 k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
         T delta) {
+  constexpr int LPG = kWave / NC;
   const int n = *b.n_obj;
   const int nc = b.n_cam;
   const int lane = lane_id();
+  const int sub = lane % LPG;
   const long HW = (long)b.H * b.W;
   const T r_px = (T)p.r_px;
+  const T dpos = r_px > T(0) ? r_px : T(0);
+  CamLane<T, SLOTS> c;
+  c.k = lane / LPG;
+  c.cam_ok = c.k < nc;
+  c.act = c.cam_ok && p.active[c.k];
+  const int ksafe = c.cam_ok ? c.k : 0;
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
-    ShakeCtx<T, NC, SLOTS> s;
-    s.n_cam = nc;
-    s.r_px = r_px;
     T X[3] = {(T)b.xyz[3L * i], (T)b.xyz[3L * i + 1], (T)b.xyz[3L * i + 2]};
-    T uv[2 * NC];
-    // buildROIInfo with the object's current projection
+    // buildROIInfo (:279-328) with the object's current projection
+    c.roi = PRange{0, 0, 0, 0};
+    c.roiw = 0;
+    c.npix = 0;
 #pragma unroll
-    for (int k = 0; k < NC; ++k) {
-      s.active[k] = (k < nc) ? p.active[k] : 0;
-      s.roi[k] = PRange{0, 0, 0, 0};
-      s.roiw[k] = 0;
-      s.npix[k] = 0;
+    for (int sl = 0; sl < SLOTS; ++sl) c.aug[sl] = T(0);
+    if (c.act) {
+      const ObjCamRec<T> &r = rec[(long)i * nc + ksafe];
+      const PRange roi = roi_bound<T>(b.H, b.W, r.u, r.v, dpos * (T)p.ratio_augimg,
+                                      dpos * (T)p.ratio_augimg);
+      c.roi = roi;
+      if (!roi.empty()) {
+        c.roiw = roi.cmax - roi.cmin;
+        c.npix = (roi.rmax - roi.rmin) * c.roiw;
+        const T rP[5] = {r.P[0], r.P[1], r.P[2], r.P[3], r.P[4]};
 #pragma unroll
-      for (int sl = 0; sl < SLOTS; ++sl) s.aug[k][sl] = T(0);
-      if (k >= nc) continue;
-      const ObjCamRec<T> &r = rec[(long)i * nc + k];
-      uv[2 * k] = r.u;
-      uv[2 * k + 1] = r.v;
-      if (!s.active[k]) continue;
-      const T d = r_px > T(0) ? r_px : T(0);
-      const PRange roi = roi_bound<T>(b.H, b.W, r.u, r.v, d * (T)p.ratio_augimg,
-                                      d * (T)p.ratio_augimg);
-      s.roi[k] = roi;
-      if (roi.empty()) continue;
-      s.roiw[k] = roi.cmax - roi.cmin;
-      s.npix[k] = (roi.rmax - roi.rmin) * s.roiw[k];
-      const PRange rp{r.rmin, r.rmax, r.cmin, r.cmax};
-#pragma unroll
-      for (int sl = 0; sl < SLOTS; ++sl) {
-        const int idx = sl * kWave + lane;
-        if (idx < s.npix[k]) {
-          const int row = roi.rmin + idx / s.roiw[k];
-          const int col = roi.cmin + idx % s.roiw[k];
-          T a = b.res[k * HW + (long)row * b.W + col];
-          if (row >= rp.rmin && row < rp.rmax && col >= rp.cmin && col < rp.cmax)
-            a += gauss_int<T>(r.P, r.u, r.v, row, col);
-          s.aug[k][sl] = a > T(0) ? a : T(0);
+        for (int sl = 0; sl < SLOTS; ++sl) {
+          const int idx = sl * LPG + sub;
+          if (idx < c.npix) {
+            const int row = roi.rmin + idx / c.roiw;
+            const int col = roi.cmin + idx % c.roiw;
+            T a = b.res[ksafe * HW + (long)row * b.W + col];
+            if (row >= r.rmin && row < r.rmax && col >= r.cmin && col < r.cmax)
+              a += gauss_int<T>(rP, r.u, r.v, row, col);
+            c.aug[sl] = a > T(0) ? a : T(0);
+          }
         }
       }
     }
-    // shakeOneObject; tmp keeps the last evaluated coordinate on each axis
+    // shakeOneObject (:409-665); tmp keeps the last evaluated coordinate on
+    // each axis, as the reference's reused temporary object does
     T tmp[3] = {X[0], X[1], X[2]};
     for (int ax = 0; ax < 3; ++ax) {
       const T x0 = X[ax];
       const T DL = -delta, DR = delta;
       const T rel[3] = {DL, T(0), DR};
-      T f[3];
-      T uvs[3][2 * NC];
+      T f[3], ue[3], ve[3];
+#pragma unroll
       for (int j = 0; j < 3; ++j) {
         tmp[ax] = x0 + rel[j];
-        f[j] = eval_residue<T, NC, SLOTS>(b, s, tmp, uvs[j]);
+        f[j] = eval_residue<T, NC, SLOTS>(b, c, r_px, tmp, ue[j], ve[j]);
       }
       const T fL = f[0], f0 = f[1], fR = f[2];
       const T tol = T(1e-3) * fmax(T(1), f0);
@@ -404,23 +446,28 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
         }
       }
       if (!skip_expand && mono) {
+        // per-camera pixel speed and ROI margin, gathered in camera order
+        const bool ok0 = in_roi_d(c.roi, ue[0], ve[0]);
+        const bool ok1 = in_roi_d(c.roi, ue[1], ve[1]);
+        const bool ok2 = in_roi_d(c.roi, ue[2], ve[2]);
+        T Jk = T(0), mk = T(0);
+        if (ok0 && ok2)
+          Jk = hypot(ue[2] - ue[0], ve[2] - ve[0]) / fmax(T(2) * delta, T(1e-12));
+        if (ok1) {
+          const T mu = fmin(ue[1] - T(c.roi.cmin), T(c.roi.cmax) - T(1) - ue[1]);
+          const T mv = fmin(ve[1] - T(c.roi.rmin), T(c.roi.rmax) - T(1) - ve[1]);
+          mk = fmin(mu, mv);
+        }
+        const uint64_t mJ = ballot(c.cam_ok && sub == 0 && ok0 && ok2);
+        const uint64_t mM = ballot(c.cam_ok && sub == 0 && ok1);
         T Jv[NC], mg[NC];
         int nJ = 0, nm = 0;
-        for (int k = 0; k < NC && k < nc; ++k) {
-          const bool ok0 = in_roi_d(s.roi[k], uvs[0][2 * k], uvs[0][2 * k + 1]);
-          const bool ok1 = in_roi_d(s.roi[k], uvs[1][2 * k], uvs[1][2 * k + 1]);
-          const bool ok2 = in_roi_d(s.roi[k], uvs[2][2 * k], uvs[2][2 * k + 1]);
-          if (ok0 && ok2) {
-            const T du = uvs[2][2 * k] - uvs[0][2 * k];
-            const T dv = uvs[2][2 * k + 1] - uvs[0][2 * k + 1];
-            Jv[nJ++] = hypot(du, dv) / fmax(T(2) * delta, T(1e-12));
-          }
-          if (ok1) {
-            const PRange &rr = s.roi[k];
-            const T mu = fmin(uvs[1][2 * k] - T(rr.cmin), T(rr.cmax) - T(1) - uvs[1][2 * k]);
-            const T mv = fmin(uvs[1][2 * k + 1] - T(rr.rmin), T(rr.rmax) - T(1) - uvs[1][2 * k + 1]);
-            mg[nm++] = fmin(mu, mv);
-          }
+#pragma unroll
+        for (int kk = 0; kk < NC; ++kk) {
+          const T jv = __shfl(Jk, kk * LPG, kWave);
+          const T mv = __shfl(mk, kk * LPG, kWave);
+          if ((mJ >> (kk * LPG)) & 1ull) Jv[nJ++] = jv;
+          if ((mM >> (kk * LPG)) & 1ull) mg[nm++] = mv;
         }
         const T Jx = median_small<T, NC>(Jv, nJ);
         const T mpx = median_small<T, NC>(mg, nm);
@@ -441,16 +488,10 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
         if (Dp > T(0)) {
           const T dxe = sgn >= 0 ? Dp : -Dp;
           tmp[ax] = x0 + dxe;
-          T uvx[2 * NC];
-          // project for the in-ROI test before evaluating
-          bool out = false;
-          for (int k = 0; k < NC && k < nc; ++k) {
-            project(b.cams[k], tmp, uvx[2 * k], uvx[2 * k + 1]);
-            if (!in_roi_d(s.roi[k], uvx[2 * k], uvx[2 * k + 1])) out = true;
-          }
-          if (!out) {
+          if (!any_out<T, NC, SLOTS>(b, c, tmp)) {
+            T uu, vv;
             cx[ncand] = dxe;
-            cf[ncand] = eval_residue<T, NC, SLOTS>(b, s, tmp, uvx);
+            cf[ncand] = eval_residue<T, NC, SLOTS>(b, c, r_px, tmp, uu, vv);
             cabs[ncand] = x0 + dxe;
             ++ncand;
           }
@@ -471,15 +512,10 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
         }
         if (xh >= xmin && xh <= xmax) {
           tmp[ax] = x0 + xh;
-          T uvx[2 * NC];
-          bool out = false;
-          for (int k = 0; k < NC && k < nc; ++k) {
-            project(b.cams[k], tmp, uvx[2 * k], uvx[2 * k + 1]);
-            if (!in_roi_d(s.roi[k], uvx[2 * k], uvx[2 * k + 1])) out = true;
-          }
-          if (!out) {
+          if (!any_out<T, NC, SLOTS>(b, c, tmp)) {
+            T uu, vv;
             cx[ncand] = xh;
-            cf[ncand] = eval_residue<T, NC, SLOTS>(b, s, tmp, uvx);
+            cf[ncand] = eval_residue<T, NC, SLOTS>(b, c, r_px, tmp, uu, vv);
             cabs[ncand] = x0 + xh;
             ++ncand;
           }
@@ -494,61 +530,66 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
         }
       X[ax] = cabs[bi];
     }
-    // final projection and calObjectScore
-    T P[NC][5];
-    for (int k = 0; k < NC && k < nc; ++k) {
-      project(b.cams[k], X, uv[2 * k], uv[2 * k + 1]);
-      otf_param(b.otf, k, X, P[k]);
+    // final projection and calObjectScore (:668-782)
+    T uf = T(0), vf = T(0), Pf[5] = {T(0), T(0), T(0), T(0), T(0)};
+    if (c.cam_ok) {
+      project(b.cams[c.k], X, uf, vf);
+      otf_param(b.otf, c.k, X, Pf);
     }
+    bool usek = false;
+    PRange rect{1, 0, 1, 0};
+    if (c.act && dpos > T(0)) {
+      rect = roi_bound<T>(b.H, b.W, uf, vf, dpos, dpos);
+      usek = !rect.empty();
+    }
+    const int rw = usek ? rect.cmax - rect.cmin : 0;
+    const int np = usek ? (rect.rmax - rect.rmin) * rw : 0;
+    const int np_all = wave_max(np);
+    T pk = T(0), nm = T(0), dn = T(0);
+    for (int base = 0; base < np_all; base += LPG) {
+      const int idx = base + sub;
+      const bool valid = idx < np;
+      int row = 0, col = 0;
+      if (valid) {
+        row = rect.rmin + idx / rw;
+        col = rect.cmin + idx % rw;
+      }
+      // augmented value if inside this camera's ROI, else the residual
+      const PRange &ro = c.roi;
+      const bool inr = valid && ro.rmax - ro.rmin > 0 && ro.cmax - ro.cmin > 0 &&
+                       row >= ro.rmin && row < ro.rmax && col >= ro.cmin && col < ro.cmax;
+      const int aidx = inr ? (row - ro.rmin) * c.roiw + (col - ro.cmin) : 0;
+      const int src = (lane / LPG) * LPG + aidx % LPG;
+      T av = T(0);
+#pragma unroll
+      for (int sl = 0; sl < SLOTS; ++sl) {
+        const T got = __shfl(c.aug[sl], src, kWave);
+        if (aidx / LPG == sl) av = got;
+      }
+      T meas = T(0), pred = T(0);
+      if (valid) {
+        meas = inr ? av : b.res[ksafe * HW + (long)row * b.W + col];
+        pred = gauss_int<T>(Pf, uf, vf, row, col);
+      }
+      const T mpos = meas > T(0) ? meas : T(0);
+      pk = fmax(pk, valid ? meas : T(0));
+      nm += valid ? mpos : T(0);
+      dn += valid ? pred : T(0);
+    }
+    pk = group_max<LPG>(pk);
+    nm = group_sum<LPG>(nm);
+    dn = group_sum<LPG>(dn);
+    const uint64_t mu = ballot(usek && sub == 0);
     T peak[NC], num[NC], den[NC];
     int use[NC];
     int n_used = 0;
 #pragma unroll
-    for (int k = 0; k < NC; ++k) {
-      use[k] = 0;
-      peak[k] = num[k] = den[k] = T(0);
-      if (k >= nc || !s.active[k]) continue;
-      const T d = r_px > T(0) ? r_px : T(0);
-      if (d <= T(0)) continue;
-      const PRange rect = roi_bound<T>(b.H, b.W, uv[2 * k], uv[2 * k + 1], d, d);
-      if (rect.empty()) continue;
-      use[k] = 1;
-      ++n_used;
-      const int rw = rect.cmax - rect.cmin;
-      const int np = (rect.rmax - rect.rmin) * rw;
-      T pk = T(0), nm = T(0), dn = T(0);
-      for (int base = 0; base < np; base += kWave) {
-        const int idx = base + lane;
-        T meas = T(0), pred = T(0);
-        bool valid = idx < np;
-        int row = 0, col = 0;
-        if (valid) {
-          row = rect.rmin + idx / rw;
-          col = rect.cmin + idx % rw;
-        }
-        // augmented value if inside this camera's ROI, else residual
-        const PRange &ro = s.roi[k];
-        const bool inr = valid && ro.rmax - ro.rmin > 0 && ro.cmax - ro.cmin > 0 &&
-                         row >= ro.rmin && row < ro.rmax && col >= ro.cmin && col < ro.cmax;
-        const int aidx = inr ? (row - ro.rmin) * s.roiw[k] + (col - ro.cmin) : 0;
-        T av = T(0);
-#pragma unroll
-        for (int sl = 0; sl < SLOTS; ++sl) {
-          const T got = __shfl(s.aug[k][sl], aidx % kWave, kWave);
-          if (aidx / kWave == sl) av = got;
-        }
-        if (valid) {
-          meas = inr ? av : b.res[k * HW + (long)row * b.W + col];
-          pred = gauss_int<T>(P[k], uv[2 * k], uv[2 * k + 1], row, col);
-        }
-        const T mpos = meas > T(0) ? meas : T(0);
-        pk = fmax(pk, valid ? meas : T(0));
-        nm += valid ? mpos : T(0);
-        dn += valid ? pred : T(0);
-      }
-      peak[k] = wave_max(pk);
-      num[k] = wave_sum(nm);
-      den[k] = wave_sum(dn);
+    for (int kk = 0; kk < NC; ++kk) {
+      peak[kk] = __shfl(pk, kk * LPG, kWave);
+      num[kk] = __shfl(nm, kk * LPG, kWave);
+      den[kk] = __shfl(dn, kk * LPG, kWave);
+      use[kk] = (int)((mu >> (kk * LPG)) & 1ull);
+      n_used += use[kk];
     }
     T score;
     if (n_used < 2) {
@@ -557,18 +598,22 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
       if (n_used > 2) {
         int mi = -1;
         T mv = -T(INFINITY);
-        for (int k = 0; k < NC && k < nc; ++k)
-          if (use[k] && peak[k] > mv) {
-            mv = peak[k];
-            mi = k;
+#pragma unroll
+        for (int kk = 0; kk < NC; ++kk)
+          if (use[kk] && peak[kk] > mv) {
+            mv = peak[kk];
+            mi = kk;
           }
-        if (mi >= 0) use[mi] = 0;
+#pragma unroll
+        for (int kk = 0; kk < NC; ++kk)
+          if (kk == mi) use[kk] = 0;
       }
       T tn = T(0), td = T(0);
-      for (int k = 0; k < NC && k < nc; ++k)
-        if (use[k]) {
-          tn += num[k];
-          td += den[k];
+#pragma unroll
+      for (int kk = 0; kk < NC; ++kk)
+        if (use[kk]) {
+          tn += num[kk];
+          td += den[kk];
         }
       T ratio = (td > T(1e-12)) ? (tn / td) : T(1);
       ratio = fmax(T(2) / T(3), fmin(T(1.5), ratio));
@@ -579,10 +624,10 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
       b.xyz[3L * i + 1] = (double)X[1];
       b.xyz[3L * i + 2] = (double)X[2];
       b.score[i] = b.score[i] * (double)score;
-      for (int k = 0; k < nc && k < NC; ++k) {
-        b.uv[((long)i * nc + k) * 2] = uv[2 * k];
-        b.uv[((long)i * nc + k) * 2 + 1] = uv[2 * k + 1];
-      }
+    }
+    if (c.cam_ok && sub == 0) {
+      b.uv[((long)i * nc + c.k) * 2] = uf;
+      b.uv[((long)i * nc + c.k) * 2 + 1] = vf;
     }
   }
 }
@@ -773,7 +818,9 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
     if (!p.active[k]) inact.id[inact.n++] = k;
   ObjCamRec<T> *rec = reinterpret_cast<ObjCamRec<T> *>(b.rec);
   const T r_px = (T)p.r_px;
-  const int roi_w = (int)lround(2.0 * p.r_px * p.ratio_augimg);
+  // ROI edge round(2 r ratio); the margin covers FP32 rounding of the
+  // device-side product at a .5 boundary, so the slots always hold the ROI
+  const int roi_w = (int)std::floor(2.0 * p.r_px * p.ratio_augimg + 0.5 + 1e-3);
   const int roi_px = roi_w * roi_w;
   const int gO = grid_for(b.cap_obj, 256, 2048);
   k_init_score<T><<<gO, 256, 0, st>>>(b);
@@ -785,14 +832,14 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
     residual_pass<T>(b, b.img, b.res, b.xyz, b.n_obj, rec, b.uv, act, 0, r_px, st);
     if (nc < 2) {
       k_zero_score<T><<<gO, 256, 0, st>>>(b);
-    } else if (nc <= 4 && roi_px <= kWave) {
-      shake_iter<T, 4, 1>(b, p, rec, (T)delta, st);
-    } else if (nc <= 4 && roi_px <= 4 * kWave) {
+    } else if (nc <= 4 && roi_px <= 64) {
       shake_iter<T, 4, 4>(b, p, rec, (T)delta, st);
-    } else if (roi_px <= kWave) {
-      shake_iter<T, 8, 1>(b, p, rec, (T)delta, st);
+    } else if (nc <= 4) {
+      shake_iter<T, 4, 16>(b, p, rec, (T)delta, st);
+    } else if (roi_px <= 64) {
+      shake_iter<T, 8, 8>(b, p, rec, (T)delta, st);
     } else {
-      shake_iter<T, 8, 4>(b, p, rec, (T)delta, st);
+      shake_iter<T, 8, 32>(b, p, rec, (T)delta, st);
     }
     delta *= 0.5;
     if (delta < dmin) delta = dmin;
