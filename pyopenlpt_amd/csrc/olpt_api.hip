@@ -547,6 +547,8 @@ olpt_status olpt_ctx_reserve(olpt_ctx *c, int n_cam, int H, int W, int max_2d, i
   if (!c) return fail(OLPT_E_INVALID_ARGUMENT, "null context");
   if (n_cam < 1 || n_cam > OLPT_MAX_CAM || H < 3 || W < 3)
     return fail(OLPT_E_INVALID_ARGUMENT, "bad camera count or image size");
+  if (max_cand >= (1 << 27))
+    return fail(OLPT_E_INVALID_ARGUMENT, "max_cand must be below 2^27");
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipDeviceSynchronize());
   free_all(c);
@@ -602,9 +604,17 @@ olpt_status olpt_ctx_reserve(olpt_ctx *c, int n_cam, int H, int W, int max_2d, i
   A(b.key_cnt, int32_t, cap);
   A(b.key_start, int32_t, cap);
   A(b.cand_rank, int32_t, M);
+  A(b.bucket_idx, int32_t, M);
   A(b.pt_count, int32_t, (long)nc * cap);
   A(b.pt_start, int32_t, (long)nc * cap);
   A(b.pt_list, int32_t, (long)M * nc);
+  A(b.pt_of, int32_t, (long)M * nc);
+  A(b.pt_sorted, int32_t, (long)M * nc);
+  A(b.pt_occ, int32_t, (long)M * nc);
+  A(b.ppos, int32_t, (long)M * nc);
+  A(b.pt_flag, int32_t, (long)nc * cap);
+  A(b.head, int32_t, (long)nc * cap);
+  A(b.n_entries, int32_t, 1);
   A(b.flat_rank, int32_t, (long)M * nc);
   A(b.pv, double, (long)M * nc);
   A(b.pct, double, M);

@@ -83,10 +83,18 @@ struct StereoBuffers {
   int32_t *key_cnt;      // [cap] histogram on the primary camera
   int32_t *key_start;    // [cap]
   int32_t *cand_rank;    // [cap_cand]
+  int32_t *bucket_idx;   // [cap_cand] raw candidate index, grouped by key
   // prune
   int32_t *pt_count;     // [n_cam*cap] candidates per flat point
   int32_t *pt_start;     // [n_cam*cap]
-  int32_t *pt_list;      // [cap_cand*n_cam]
+  int32_t *pt_list;      // [cap_cand*n_cam] (candidate << 3 | slot)
+  int32_t *pt_of;        // [cap_cand*n_cam] flat point of each list entry
+  int32_t *pt_sorted;    // [cap_cand*n_cam] lists in priority order
+  int32_t *pt_occ;       // [cap_cand*n_cam] position occupancy (rank check)
+  int32_t *ppos;         // [cap_cand][n_cam] position in the point's priority list
+  int32_t *pt_flag;      // [n_cam*cap] bit0/bit1: sequential percentile/priority
+  int32_t *head;         // [n_cam*cap] first entry not known to be rejected
+  int32_t *n_entries;    // device: total list entries
   int32_t *flat_rank;    // [cap_cand][n_cam]
   double *pv;            // [cap_cand][n_cam] percentile per used point
   double *pct;           // [cap_cand]

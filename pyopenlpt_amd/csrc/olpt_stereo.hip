@@ -22,19 +22,26 @@
 //                 tests run on full waves; full windows and lists are drained
 //                 depth-first, so no capacity bounds the search.  The emitted
 //                 set does not depend on visiting order.
-//   k_key_* / k_bucket_sort  lexicographic sort of the id vectors
+//   k_key_* / k_bucket_rank  lexicographic sort of the id vectors
 //                 (:153-156): counting sort on the first active camera's id,
-//                 then insertion sort inside each (small) bucket.
-//   k_flat / k_pct / k_mis_round / k_replace_check / k_tri  pruneMatch
-//                 (:631-980) + triangulateMatch (:985-1050).  The greedy
-//                 packing in (pct, e_check, index) order equals the greedy
-//                 maximal independent set of the conflict graph under that
-//                 priority, computed in rounds: a candidate is selected once
-//                 every higher-priority conflicting candidate is rejected and
-//                 rejected once one of them is selected.  The reference's
-//                 replacement branch is checked afterwards for every rejected
-//                 candidate (flags[3] counts cases where it would have fired;
-//                 with the sort order it cannot).
+//                 then each candidate's place inside its (small) bucket by
+//                 counting the members that precede it.
+//   k_flat / k_pct_* / k_prio_* / k_mis_round / k_replace_check / k_tri
+//                 pruneMatch (:631-980) + triangulateMatch (:985-1050).
+//                 Per-point orders (by e_check for the percentiles, by the
+//                 greedy's priority for the packing) are rank counts, one
+//                 thread per list entry; ranks that form a permutation are
+//                 the unique order, and the rare point where they do not
+//                 (eq_eps ties) is redone sequentially.  The greedy packing
+//                 in (pct, e_check, index) order equals the greedy maximal
+//                 independent set of the conflict graph under that priority,
+//                 computed in rounds over the priority-sorted point lists: a
+//                 candidate is selected once every entry ahead of it on its
+//                 points is rejected and rejected once one of them is
+//                 selected.  The reference's replacement branch is checked
+//                 afterwards for every rejected candidate (flags[3] counts
+//                 cases where it would have fired; with the sort order it
+//                 cannot).
 #include "olpt_kernels.hpp"
 #include "olpt_scan.hpp"
 
@@ -790,10 +797,7 @@ __global__ void k_key_scatter(StereoBuffers b, StereoParams p) {
   const int pc = primary_cam(p, b.n_cam);
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
     const int key = b.cand_ids[(long)j * b.n_cam + pc];
-    const int pos = b.key_start[key] + b.cand_rank[j];
-    for (int c = 0; c < b.n_cam; ++c)
-      b.cand_ids_s[(long)pos * b.n_cam + c] = b.cand_ids[(long)j * b.n_cam + c];
-    b.cand_e_s[pos] = b.cand_e[j];
+    b.bucket_idx[b.key_start[key] + b.cand_rank[j]] = j;
   }
 }
 
@@ -803,29 +807,31 @@ __device__ __forceinline__ bool ids_less(const int32_t *a, const int32_t *c, int
   return false;
 }
 
-__global__ void k_bucket_sort(StereoBuffers b, StereoParams p) {
+// Final position of each candidate = bucket start + number of bucket members
+// that precede it lexicographically (equal vectors, which the build never
+// emits, by emission index so positions stay a permutation).
+__global__ void k_bucket_rank(StereoBuffers b, StereoParams p) {
+  const int M = min(*b.n_cand, b.cap_cand);
   const int pc = primary_cam(p, b.n_cam);
-  const int nkey = b.npts[pc];
   const int nc = b.n_cam;
-  for (int key = blockIdx.x * blockDim.x + threadIdx.x; key < nkey;
-       key += gridDim.x * blockDim.x) {
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
+    int32_t idj[kMaxCam];
+    for (int c = 0; c < nc; ++c) idj[c] = b.cand_ids[(long)j * nc + c];
+    const int key = idj[pc];
     const int s = b.key_start[key], n = b.key_cnt[key];
-    if (n < 2) continue;
-    int32_t *ids = b.cand_ids_s + (long)s * nc;
-    double *e = b.cand_e_s + s;
-    for (int i = 1; i < n; ++i) {
-      int32_t cur[kMaxCam];
-      for (int c = 0; c < nc; ++c) cur[c] = ids[(long)i * nc + c];
-      const double ce = e[i];
-      int j = i - 1;
-      while (j >= 0 && ids_less(cur, ids + (long)j * nc, nc)) {
-        for (int c = 0; c < nc; ++c) ids[(long)(j + 1) * nc + c] = ids[(long)j * nc + c];
-        e[j + 1] = e[j];
-        --j;
-      }
-      for (int c = 0; c < nc; ++c) ids[(long)(j + 1) * nc + c] = cur[c];
-      e[j + 1] = ce;
+    int r = 0;
+    for (int t = 0; t < n; ++t) {
+      const int m = b.bucket_idx[s + t];
+      if (m == j) continue;
+      const int32_t *idm = b.cand_ids + (long)m * nc;
+      int cmp = 0;
+      for (int c = 0; c < nc && cmp == 0; ++c)
+        cmp = idm[c] < idj[c] ? -1 : (idm[c] > idj[c] ? 1 : 0);
+      r += (cmp < 0 || (cmp == 0 && m < j)) ? 1 : 0;
     }
+    const int pos = s + r;
+    for (int c = 0; c < nc; ++c) b.cand_ids_s[(long)pos * nc + c] = idj[c];
+    b.cand_e_s[pos] = b.cand_e[j];
   }
 }
 
@@ -861,6 +867,11 @@ __device__ __forceinline__ bool less_order(const double *pct, const double *e,
   return aj < bj;
 }
 
+// Point lists: entry = (candidate << 3) | slot, slot = rank of the camera
+// among the candidate's used cameras (camera order).
+__device__ __forceinline__ int ent_cand(int32_t v) { return v >> 3; }
+__device__ __forceinline__ int ent_slot(int32_t v) { return v & 7; }
+
 __global__ void k_flat(StereoBuffers b, StereoParams p) {
   const int M = min(*b.n_cand, b.cap_cand);
   const int nc = b.n_cam;
@@ -887,46 +898,72 @@ __global__ void k_pt_fill(StereoBuffers b, StereoParams p) {
       const int pid = b.cand_ids_s[(long)j * nc + c];
       if (pid < 0) continue;
       const long f = (long)c * b.cap + pid;
-      b.pt_list[b.pt_start[f] + b.flat_rank[(long)j * nc + k]] = j;
+      const int q = b.pt_start[f] + b.flat_rank[(long)j * nc + k];
+      b.pt_list[q] = (j << 3) | k;
+      b.pt_of[q] = (int32_t)f;
       ++k;
     }
   }
 }
 
-// slot of camera c among the candidate's used cameras (camera order)
-__device__ __forceinline__ int flat_slot(const StereoBuffers &b,
-                                         const StereoParams &p, int j, int cam) {
-  int k = 0;
-  for (int c = 0; c < cam; ++c)
-    if (p.active[c] && b.cand_ids_s[(long)j * b.n_cam + c] >= 0) ++k;
-  return k;
+__global__ void k_zero_entries(StereoBuffers b) {
+  const int n = *b.n_entries;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+    b.pt_occ[q] = 0;
 }
 
 // Percentile of each candidate on each used 2D point (StereoMatch.cpp:
-// 727-800): stable sort of the point's candidates by e_check (eq_eps ties by
-// index), average rank for eq_eps blocks, rank/(n-1).
-__global__ void k_pct(StereoBuffers b, StereoParams p) {
+// 727-800): the point's candidates sorted by e_check (eq_eps ties by
+// index), eq_eps blocks take their average rank, pv = rank / (n - 1).
+// One thread per list entry counts the entries below it.  An entry with no
+// eq_eps partner forms a block of its own at that rank; a point holding any
+// eq_eps pair is flagged and redone by k_pct_fallback.
+__global__ void k_pct_rank(StereoBuffers b) {
+  const int ne = *b.n_entries;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ne; q += gridDim.x * blockDim.x) {
+    const int f = b.pt_of[q];
+    const int s0 = b.pt_start[f], n = b.pt_count[f];
+    const int32_t v = b.pt_list[q];
+    const int j = ent_cand(v);
+    const double ej = b.cand_e_s[j];
+    int r = 0;
+    bool tie = false;
+    for (int t = 0; t < n; ++t) {
+      const int m = ent_cand(b.pt_list[s0 + t]);
+      if (m == j) continue;
+      const double em = b.cand_e_s[m];
+      if (eq_eps(em, ej))
+        tie = true;
+      else
+        r += em < ej ? 1 : 0;
+    }
+    if (tie) {
+      atomicOr(&b.pt_flag[f], 1);
+      continue;
+    }
+    const double denom = (double)(n - 1);
+    const double avg_rank = 0.5 * (double)(r + r);
+    b.pv[(long)j * b.n_cam + ent_slot(v)] = (denom > 0.0) ? (avg_rank / denom) : 0.0;
+  }
+}
+
+// Sequential statement of the percentile for flagged points: insertion sort
+// by (e eq_eps, index), then eq_eps blocks from each block's first entry.
+__global__ void k_pct_fallback(StereoBuffers b) {
   const long total = (long)b.n_cam * b.cap;
   for (long f = blockIdx.x * (long)blockDim.x + threadIdx.x; f < total;
        f += (long)gridDim.x * blockDim.x) {
+    if (!(b.pt_flag[f] & 1)) continue;
     const int n = b.pt_count[f];
-    if (n == 0) continue;
-    const int cam = (int)(f / b.cap);
     int32_t *lst = b.pt_list + b.pt_start[f];
-    if (n == 1) {
-      const int j = lst[0];
-      b.pv[(long)j * b.n_cam + flat_slot(b, p, j, cam)] = 0.0;
-      continue;
-    }
-    // insertion sort by (e eq_eps, index)
     for (int i = 1; i < n; ++i) {
-      const int cur = lst[i];
-      const double ce = b.cand_e_s[cur];
+      const int32_t cur = lst[i];
+      const double ce = b.cand_e_s[ent_cand(cur)];
       int k = i - 1;
       while (k >= 0) {
-        const int o = lst[k];
-        const double oe = b.cand_e_s[o];
-        const bool less = !eq_eps(ce, oe) ? (ce < oe) : (cur < o);
+        const int32_t o = lst[k];
+        const double oe = b.cand_e_s[ent_cand(o)];
+        const bool less = !eq_eps(ce, oe) ? (ce < oe) : (ent_cand(cur) < ent_cand(o));
         if (!less) break;
         lst[k + 1] = o;
         --k;
@@ -936,15 +973,13 @@ __global__ void k_pct(StereoBuffers b, StereoParams p) {
     int k = 0;
     while (k < n) {
       int k0 = k, k1 = k + 1;
-      const double e0 = b.cand_e_s[lst[k0]];
-      while (k1 < n && eq_eps(b.cand_e_s[lst[k1]], e0)) ++k1;
+      const double e0 = b.cand_e_s[ent_cand(lst[k0])];
+      while (k1 < n && eq_eps(b.cand_e_s[ent_cand(lst[k1])], e0)) ++k1;
       const double avg_rank = 0.5 * (double)(k0 + (k1 - 1));
       const double denom = (double)(n - 1);
       const double pv = (denom > 0.0) ? (avg_rank / denom) : 0.0;
-      for (int t = k0; t < k1; ++t) {
-        const int j = lst[t];
-        b.pv[(long)j * b.n_cam + flat_slot(b, p, j, cam)] = pv;
-      }
+      for (int t = k0; t < k1; ++t)
+        b.pv[(long)ent_cand(lst[t]) * b.n_cam + ent_slot(lst[t])] = pv;
       k = k1;
     }
   }
@@ -966,8 +1001,57 @@ __global__ void k_pct_final(StereoBuffers b, StereoParams p) {
   }
 }
 
-// One round of the priority greedy.  In-place updates are safe: a decided
-// state never changes, and an undecided neighbour only delays a decision.
+// Each point's candidates in the greedy's priority order (the global sort
+// of StereoMatch.cpp:820-851 restricted to the point): position = number of
+// entries that precede the entry.  A point whose positions collide (the
+// pairwise order is not transitive there) is flagged for k_prio_fallback.
+__global__ void k_prio_rank(StereoBuffers b) {
+  const int ne = *b.n_entries;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ne; q += gridDim.x * blockDim.x) {
+    const int f = b.pt_of[q];
+    const int s0 = b.pt_start[f], n = b.pt_count[f];
+    const int32_t v = b.pt_list[q];
+    const int j = ent_cand(v);
+    int r = 0;
+    for (int t = 0; t < n; ++t) {
+      const int m = ent_cand(b.pt_list[s0 + t]);
+      if (m != j && less_order(b.pct, b.cand_e_s, m, j)) ++r;
+    }
+    b.ppos[(long)j * b.n_cam + ent_slot(v)] = r;
+    b.pt_sorted[s0 + r] = v;
+    if (atomicAdd(&b.pt_occ[s0 + r], 1) != 0) atomicOr(&b.pt_flag[f], 2);
+  }
+}
+
+__global__ void k_prio_fallback(StereoBuffers b) {
+  const long total = (long)b.n_cam * b.cap;
+  for (long f = blockIdx.x * (long)blockDim.x + threadIdx.x; f < total;
+       f += (long)gridDim.x * blockDim.x) {
+    if (!(b.pt_flag[f] & 2)) continue;
+    const int n = b.pt_count[f];
+    const int s0 = b.pt_start[f];
+    int32_t *lst = b.pt_sorted + s0;
+    for (int i = 0; i < n; ++i) lst[i] = b.pt_list[s0 + i];
+    for (int i = 1; i < n; ++i) {
+      const int32_t cur = lst[i];
+      int k = i - 1;
+      while (k >= 0 && less_order(b.pct, b.cand_e_s, ent_cand(cur), ent_cand(lst[k]))) {
+        lst[k + 1] = lst[k];
+        --k;
+      }
+      lst[k + 1] = cur;
+    }
+    for (int i = 0; i < n; ++i)
+      b.ppos[(long)ent_cand(lst[i]) * b.n_cam + ent_slot(lst[i])] = i;
+  }
+}
+
+// One round of the priority greedy over the priority-sorted point lists.
+// head[f] only moves past rejected entries, so every entry before it is
+// rejected: a candidate is selected when on each of its points it is the
+// head, and rejected when a selected entry precedes it on any point.
+// In-place updates are safe: decided states never change, and an
+// undecided entry ahead only delays a decision.
 __global__ void k_mis_round(StereoBuffers b, StereoParams p, int round) {
   if (round > 0 && b.undecided[round - 1] == 0) return;
   const int M = min(*b.n_cand, b.cap_cand);
@@ -976,22 +1060,28 @@ __global__ void k_mis_round(StereoBuffers b, StereoParams p, int round) {
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
     if (b.state[j] != 0) continue;
     bool out = false, blocked = false;
+    int k = 0;
     for (int c = 0; c < nc && !out; ++c) {
       if (!p.active[c]) continue;
       const int pid = b.cand_ids_s[(long)j * nc + c];
       if (pid < 0) continue;
       const long f = (long)c * b.cap + pid;
-      const int s0 = b.pt_start[f], n = b.pt_count[f];
-      for (int t = 0; t < n; ++t) {
-        const int m = b.pt_list[s0 + t];
-        if (m == j) continue;
-        if (!less_order(b.pct, b.cand_e_s, m, j)) continue;
-        const int sm = __atomic_load_n(&b.state[m], __ATOMIC_RELAXED);
-        if (sm == 1) {
+      const int pos = b.ppos[(long)j * nc + k];
+      ++k;
+      const int s0 = b.pt_start[f];
+      const int h0 = __atomic_load_n(&b.head[f], __ATOMIC_RELAXED);
+      int h = h0, st = 2;
+      while (h < pos) {
+        st = __atomic_load_n(&b.state[ent_cand(b.pt_sorted[s0 + h])], __ATOMIC_RELAXED);
+        if (st != 2) break;
+        ++h;
+      }
+      if (h > h0) atomicMax(&b.head[f], h);
+      if (h < pos) {
+        if (st == 1)
           out = true;
-          break;
-        }
-        if (sm == 0) blocked = true;
+        else
+          blocked = true;
       }
     }
     if (out)
@@ -1011,25 +1101,30 @@ __global__ void k_rounds_check(StereoBuffers b, int max_rounds) {
 }
 
 // Replacement branch of the reference (StereoMatch.cpp:906-968), evaluated
-// for each rejected candidate against the owners it met at its turn.
+// for each rejected candidate against the owners it met at its turn: the
+// selected entries that precede it on its points (at most one per point).
 __global__ void k_replace_check(StereoBuffers b, StereoParams p) {
   const int M = min(*b.n_cand, b.cap_cand);
   const int nc = b.n_cam;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
     if (b.state[j] != 2) continue;
     int S[kMaxCam], ns = 0;
+    int k = 0;
     for (int c = 0; c < nc; ++c) {
       if (!p.active[c]) continue;
       const int pid = b.cand_ids_s[(long)j * nc + c];
       if (pid < 0) continue;
       const long f = (long)c * b.cap + pid;
-      const int s0 = b.pt_start[f], n = b.pt_count[f];
-      for (int t = 0; t < n; ++t) {
-        const int m = b.pt_list[s0 + t];
-        if (m == j || b.state[m] != 1 || !less_order(b.pct, b.cand_e_s, m, j)) continue;
+      const int pos = b.ppos[(long)j * nc + k];
+      ++k;
+      const int s0 = b.pt_start[f];
+      for (int h = b.head[f]; h < pos; ++h) {
+        const int m = ent_cand(b.pt_sorted[s0 + h]);
+        if (b.state[m] != 1) continue;
         bool seen = false;
         for (int q = 0; q < ns; ++q) seen = seen || S[q] == m;
-        if (!seen && ns < kMaxCam) S[ns++] = m;
+        if (!seen) S[ns++] = m;
+        break;
       }
     }
     double ap = 0.0, ae = 0.0;
@@ -1157,16 +1252,26 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   k_key_hist<<<gM, T, 0, st>>>(b, p);
   scan_exclusive(b.key_cnt, b.key_start, b.cap, 1, b.cap, nullptr, b.scan_scratch, st);
   k_key_scatter<<<gM, T, 0, st>>>(b, p);
-  k_bucket_sort<<<grid_for(b.cap, T, 1024), T, 0, st>>>(b, p);
+  k_bucket_rank<<<gM, T, 0, st>>>(b, p);
   // prune
   const long nflat = (long)b.n_cam * b.cap;
   zero_i32(b.pt_count, nflat, st);
+  zero_i32(b.pt_flag, nflat, st);
+  zero_i32(b.head, nflat, st);
   k_flat<<<gM, T, 0, st>>>(b, p);
-  scan_exclusive(b.pt_count, b.pt_start, nflat, 1, nflat, nullptr, b.scan_scratch, st);
+  scan_exclusive(b.pt_count, b.pt_start, nflat, 1, nflat, b.n_entries, b.scan_scratch, st);
   k_pt_fill<<<gM, T, 0, st>>>(b, p);
-  k_pct<<<grid_for(nflat, T, 2048), T, 0, st>>>(b, p);
+  const int gE = grid_for((long)b.cap_cand * 4, T, 4096);
+  k_zero_entries<<<gE, T, 0, st>>>(b);
+  k_pct_rank<<<gE, T, 0, st>>>(b);
+  k_pct_fallback<<<grid_for(nflat, T, 2048), T, 0, st>>>(b);
   k_pct_final<<<gM, T, 0, st>>>(b, p);
-  for (int r = 0; r < max_rounds; ++r) k_mis_round<<<gM, T, 0, st>>>(b, p, r);
+  k_prio_rank<<<gE, T, 0, st>>>(b);
+  k_prio_fallback<<<grid_for(nflat, T, 2048), T, 0, st>>>(b);
+  // the first rounds decide almost every candidate; later rounds (usually
+  // empty, returning at once) use a small grid
+  for (int r = 0; r < max_rounds; ++r)
+    k_mis_round<<<r < 16 ? gM : 64, T, 0, st>>>(b, p, r);
   k_rounds_check<<<1, 1, 0, st>>>(b, max_rounds);
   k_replace_check<<<gM, T, 0, st>>>(b, p);
   // triangulate + compact in candidate order
