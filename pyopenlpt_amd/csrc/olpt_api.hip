@@ -586,6 +586,12 @@ olpt_status olpt_ctx_reserve(olpt_ctx *c, int n_cam, int H, int W, int max_2d, i
   b.npts = c->npts;
   A(b.los, double, (long)nc * cap * 4);
   A(b.lines, double, (long)nc * nc * cap * 4);
+  A(b.t1, int8_t, cap);
+  b.cap_pairs = std::max<long>(1L << 22, 16L * M);
+  A(b.pairs, int2, b.cap_pairs);
+  A(b.n_pairs, int32_t, 1);
+  A(b.surv, SurvRec, std::max(1, nc - 2) * b.cap_pairs);
+  A(b.n_surv, int32_t, kMaxCam);
   b.cell_px = 4;
   b.rows_cell = std::max(1, H / 4 + (H % 4 ? 1 : 0));
   b.cols_cell = std::max(1, W / 4 + (W % 4 ? 1 : 0));

@@ -13,6 +13,7 @@
 #include <stdint.h>
 
 #include "olpt_types.h"
+#include "olpt_gi3.hpp"
 
 namespace olpt {
 
@@ -152,58 +153,17 @@ __device__ inline void line_of_sight(const CamT<double> &c, double u,
 }
 
 // myMATH::gaussInverse for 3x3 (myMATH.h:183-289), incl. the signed row
-// maximum used for pivot scaling.
-__device__ inline void gauss_inverse3(const double A[9], double out[9]) {
-  double u[9];
-  int idx[3] = {0, 1, 2};
-  double mr[3];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) u[i] = A[i];
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    double m = u[3 * i];
-    if (u[3 * i + 1] > m) m = u[3 * i + 1];
-    if (u[3 * i + 2] > m) m = u[3 * i + 2];
-    mr[i] = m;
-  }
-  for (int y = 0; y < 2; ++y) {
-    double mp = 0;
-    int t = 0;
-    for (int x = y; x < 3; ++x) {
-      const double pv = fabs(u[3 * idx[x] + y] / mr[idx[x]]);
-      if (pv > mp) {
-        mp = pv;
-        t = x;
-      }
-    }
-    const int tmp = idx[t];
-    idx[t] = idx[y];
-    idx[y] = tmp;
-    for (int x = y + 1; x < 3; ++x) {
-      const double m = u[3 * idx[x] + y] / u[3 * idx[y] + y];
-      u[3 * idx[x] + y] = m;
-      for (int it = y + 1; it < 3; ++it) u[3 * idx[x] + it] -= m * u[3 * idx[y] + it];
-    }
-  }
-  double b[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-  for (int col = 0; col < 3; ++col) {
-    for (int y = 0; y < 2; ++y)
-      for (int x = y + 1; x < 3; ++x) b[3 * idx[x] + col] -= u[3 * idx[x] + y] * b[3 * idx[y] + col];
-    out[6 + col] = b[3 * idx[2] + col] / u[3 * idx[2] + 2];
-    for (int x = 1; x > -1; --x) {
-      double z = b[3 * idx[x] + col];
-      for (int y = x + 1; y < 3; ++y) z -= u[3 * idx[x] + y] * out[3 * y + col];
-      out[3 * x + col] = z / u[3 * idx[x] + x];
-    }
-  }
-}
+// maximum used for pivot scaling: gauss_inverse3 in olpt_gi3.hpp.
 
 // myMATH::triangulation (myMATH.cpp:357-419) over n lines through pt[i]
 // with unit direction u[i].  Returns false where the reference throws.
-__device__ inline bool triangulate(const double (*pt)[3], const double (*u)[3],
-                                   int n, double X[3], double &err) {
+// Fixed line count: every loop unrolls, so the lines stay in registers.
+template <int N>
+__device__ __forceinline__ bool triangulate_n(const double (*pt)[3], const double (*u)[3],
+                                              double X[3], double &err) {
   double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, P[3] = {0, 0, 0};
-  for (int i = 0; i < n; ++i) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
     const double nx = u[i][0], ny = u[i][1], nz = u[i][2];
     const double t[9] = {1 - nx * nx, -nx * ny, -nx * nz, -nx * ny, 1 - ny * ny,
                          -ny * nz, -nx * nz, -ny * nz, 1 - nz * nz};
@@ -218,7 +178,9 @@ __device__ inline bool triangulate(const double (*pt)[3], const double (*u)[3],
 #pragma unroll
   for (int r = 0; r < 3; ++r) X[r] = inv[3 * r] * P[0] + inv[3 * r + 1] * P[1] + inv[3 * r + 2] * P[2];
   double e = 0.0;
-  for (int i = 0; i < n; ++i) {
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
     const double d0 = X[0] - pt[i][0], d1 = X[1] - pt[i][1], d2 = X[2] - pt[i][2];
     const double nr = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
     const double dt = d0 * u[i][0] + d1 * u[i][1] + d2 * u[i][2];
@@ -226,13 +188,27 @@ __device__ inline bool triangulate(const double (*pt)[3], const double (*u)[3],
     if (h < 0.0 && h >= -1e-8)
       h = 0.0;
     else if (h < -1e-8)
-      return false;
+      ok = false;  // the reference throws here
     else
       h = sqrt(h);
     e += h * h;
   }
-  err = sqrt(e / n);
-  return true;
+  err = sqrt(e / N);
+  return ok;
+}
+
+__device__ __forceinline__ bool triangulate(const double (*pt)[3], const double (*u)[3],
+                                            int n, double X[3], double &err) {
+  switch (n) {
+  case 2: return triangulate_n<2>(pt, u, X, err);
+  case 3: return triangulate_n<3>(pt, u, X, err);
+  case 4: return triangulate_n<4>(pt, u, X, err);
+  case 5: return triangulate_n<5>(pt, u, X, err);
+  case 6: return triangulate_n<6>(pt, u, X, err);
+  case 7: return triangulate_n<7>(pt, u, X, err);
+  case 8: return triangulate_n<8>(pt, u, X, err);
+  default: return false;
+  }
 }
 
 // --------------------------------------------------------------------------

@@ -59,6 +59,17 @@ inline long peak_chunks(int H, int W) {
 }
 
 // ---------------- stereo matching (StereoMatch) ---------------------------
+// A partial match that survived one build level: its point, the camera it
+// lies on, the camera to search next and the partial match it extends (the
+// reference observation for level 1).
+struct SurvRec {
+  int32_t parent;
+  int32_t pid;
+  int8_t cam;
+  int8_t next;
+  int16_t pad;
+};
+
 struct StereoBuffers {
   int n_cam;
   int cap;               // per-camera 2D capacity
@@ -67,6 +78,13 @@ struct StereoBuffers {
   const int32_t *npts;   // [n_cam]
   double *los;           // [n_cam][cap][4]  unit direction (w unused)
   double *lines;         // [n_cam][n_cam][cap][4] LOS (c, pid) on camera t
+  // build
+  int8_t *t1;            // [cap] second build camera per reference observation
+  long cap_pairs;
+  int2 *pairs;           // [cap_pairs] (partial match, point) work list
+  int32_t *n_pairs;      // device
+  SurvRec *surv;         // [n_cam - 2][cap_pairs] survivors per level
+  int32_t *n_surv;       // [kMaxCam] device
   int cell_px, rows_cell, cols_cell;
   int32_t *cell_cnt;     // [n_cam][ncell]
   int32_t *cell_start;   // [n_cam][ncell]

@@ -6,22 +6,20 @@
 //                 and its 2D line on every other camera (the line table)
 //   k_cell_*      IDMap as a CSR cell index per camera (StereoMatch.cpp:
 //                 1413-1438); within-cell order is irrelevant to the result
-//   k_build       one wave64 per reference observation, persistent waves with
-//                 dynamic work fetch.  The first epipolar band is enumerated
-//                 wave-cooperatively: lanes own cell rows of the strip
-//                 (computeStripIntersection, :1475-1566) and append accepted
-//                 points with ballot/mbcnt compaction into LDS; back-
-//                 projection, triangulation and camera selection then run
-//                 lane-parallel over that list.  The deeper build cameras
-//                 (buildMatch's DFS, :184-444) are walked level by level with
-//                 one survivor per lane: each lane enumerates only the cell
-//                 rows inside the y-extent of its strip intersection (a
-//                 superset of the reference's non-empty spans).  Every level
-//                 first compacts (survivor, point) pairs into an LDS window,
-//                 then extends the window lane-parallel, so the expensive
-//                 tests run on full waves; full windows and lists are drained
-//                 depth-first, so no capacity bounds the search.  The emitted
-//                 set does not depend on visiting order.
+//   k_b_*         buildMatch's depth-first search (:184-444) run breadth-
+//                 first over all reference observations at once, one kernel
+//                 per step and one thread per work item: k_b_second picks
+//                 each observation's second camera, k_b_band enumerates its
+//                 epipolar band (thread per observation x cell row of the
+//                 strip, computeStripIntersection :1475-1566), k_b_extend
+//                 runs back-projection, triangulation and camera selection
+//                 on every (partial match, point) pair, k_b_enum enumerates
+//                 the strip intersection of every surviving partial match
+//                 on its next camera, but only the cell rows inside the
+//                 y-extent of the intersection (a superset of the
+//                 reference's non-empty spans).  Lists grow by wave-
+//                 aggregated appends; the emitted set does not depend on
+//                 visiting order.
 //   k_key_* / k_bucket_rank  lexicographic sort of the id vectors
 //                 (:153-156): counting sort on the first active camera's id,
 //                 then each candidate's place inside its (small) bucket by
@@ -239,7 +237,7 @@ __device__ __forceinline__ bool in_limit(const double lim[6], const double X[3])
 }
 
 // --------------------------------------------------------------------------
-// k_build: wave per reference observation
+// build: breadth-first over all reference observations
 // --------------------------------------------------------------------------
 // Line table: the 2D line of every observation's LOS on every other camera,
 // lines[(c * n_cam + t) * cap + pid] = makeLine2DFromLOS3D(cam t, LOS(c, pid))
@@ -250,25 +248,6 @@ __device__ __forceinline__ const double *line_of(const StereoBuffers &b, int c, 
                                                  int pid) {
   return b.lines + (((long)c * b.n_cam + t) * b.cap + pid) * 4;
 }
-
-constexpr int kWin = 512;   // (survivor, point) pair window per level
-constexpr int kList = 512;  // survivors per level list
-
-// Level LV (1 .. B-1) extends paths of length LV by one camera.  Level 1
-// starts from the reference observation; level LV >= 2 from the survivors of
-// level LV-1, held in list LV-2.  Windows and lists are drained depth-first
-// when full, so no capacity bounds the search.
-template <int B> struct BuildLds {
-  static constexpr int NL = B > 2 ? B - 2 : 1;
-  int32_t pair_pid[B - 1][kWin];
-  int16_t pair_e[B - 1][kWin];
-  int32_t l_pid[NL][kList];
-  int16_t l_par[NL][kList];
-  int8_t l_cam[NL][kList];
-  int8_t l_nxt[NL][kList];
-  int32_t n_pair[B - 1];
-  int32_t n_list[NL];
-};
 
 // Strip line of one LOS on the target camera in the normal form the strip
 // intersection uses (IDMap::computeStripIntersection, :1475-1566).
@@ -348,16 +327,84 @@ __device__ __forceinline__ void pair_rows(const Strip &si, const Strip &sj, int 
   if (r1 > rows - 1) r1 = rows - 1;
 }
 
-template <int B> struct Path {
-  int cam[B];
-  int pid[B];
-  double pt[B][2];
-  double P[B][3];  // LOS origin (camera centre)
-  double U[B][3];  // LOS unit direction
+// Per-lane cursor over the contiguous point range of one cell row: within a
+// row, cells cx in [xa, xb] are consecutive in the CSR index.
+struct RowCursor {
+  int kk, kend;
 };
 
-template <int B>
-__device__ __forceinline__ void path_set(Path<B> &P, int k, const StereoBuffers &b, int cam,
+__device__ __forceinline__ void row_range(const int32_t *cstart, const int32_t *ccnt,
+                                          int cy, int cols, int xa, int xb, RowCursor &c) {
+  const int c0 = cy * cols + xa, c1 = cy * cols + xb;
+  c.kk = cstart[c0];
+  c.kend = cstart[c1] + ccnt[c1];
+}
+
+// Block queue: a workgroup collects its output records in LDS and claims
+// room in the global list with one atomic per flush.  Global atomics on one
+// address serialise across the XCDs' L2 slices, so a per-wave atomic on the
+// list counter is what limits a list-building kernel; per-block batches
+// also make the global writes contiguous.  All threads of the block call
+// bq_push (at most one record each) and bq_flush in the same order.
+template <typename Rec, int CAP> struct BlockQueue {
+  Rec buf[CAP];
+  int n, base;
+};
+
+template <typename Rec, int CAP>
+__device__ __forceinline__ void bq_init(BlockQueue<Rec, CAP> &q) {
+  if (threadIdx.x == 0) q.n = 0;
+  __syncthreads();
+}
+
+// Room for one record per thread is guaranteed by bq_flush's threshold.
+template <typename Rec, int CAP>
+__device__ __forceinline__ void bq_push(BlockQueue<Rec, CAP> &q, bool want, const Rec &r) {
+  if (want) q.buf[atomicAdd(&q.n, 1)] = r;
+}
+
+// Flush when fewer than blockDim free slots remain (or any, with `all`).
+template <typename Rec, int CAP, typename Out>
+__device__ __forceinline__ void bq_flush(BlockQueue<Rec, CAP> &q, int32_t *counter, long cap,
+                                         int32_t *overflow, bool all, Out out) {
+  __syncthreads();
+  const int n = q.n;
+  if (all ? n > 0 : n > CAP - (int)blockDim.x) {
+    if (threadIdx.x == 0) q.base = atomicAdd(counter, n);
+    __syncthreads();
+    const int base = q.base;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      if ((long)base + i < cap)
+        out((long)base + i, q.buf[i]);
+      else
+        atomicOr(overflow, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) q.n = 0;
+    __syncthreads();
+  }
+}
+
+struct CandRec {
+  int32_t ids[kMaxCam];
+  double e;
+};
+
+constexpr int kQueue = 2048;      // pair queue capacity (records)
+constexpr int kSurvQueue = 1024;
+constexpr int kCandQueue = 512;
+
+// Path of a partial match in build order: slot 0 is the reference
+// observation, slot k the point chosen on the k-th build camera.
+struct Path {
+  int cam[kMaxCam];
+  int pid[kMaxCam];
+  double pt[kMaxCam][2];
+  double P[kMaxCam][3];  // LOS origin (camera centre)
+  double U[kMaxCam][3];  // LOS unit direction
+};
+
+__device__ __forceinline__ void path_set(Path &P, int k, const StereoBuffers &b, int cam,
                                          int pid) {
   P.cam[k] = cam;
   P.pid[k] = pid;
@@ -372,18 +419,38 @@ __device__ __forceinline__ void path_set(Path<B> &P, int k, const StereoBuffers 
   }
 }
 
-// Path of survivor e of list K-2 (length K): slot 0 is the reference
-// observation, slot d the entry of list d-1 on the chain of parents.
-template <int B, int K>
-__device__ __forceinline__ void load_path(const StereoBuffers &b, const BuildLds<B> &L,
-                                          int ref, int r, int e, Path<B> &P) {
-  path_set<B>(P, 0, b, ref, r);
-  int q = e;
+__device__ __forceinline__ const SurvRec &surv_at(const StereoBuffers &b, int level, int s) {
+  return b.surv[(long)(level - 1) * b.cap_pairs + s];
+}
+
+// Cameras and points of survivor s of level K-1 (a path of length K).
+template <int K>
+__device__ __forceinline__ void path_ids(const StereoBuffers &b, int ref, int s,
+                                         int cam[kMaxCam], int pid[kMaxCam]) {
 #pragma unroll
-  for (int d = K - 1; d >= 1; --d) {
-    path_set<B>(P, d, b, L.l_cam[d - 1][q], L.l_pid[d - 1][q]);
-    q = L.l_par[d - 1][q];
+  for (int k = K - 1; k >= 1; --k) {
+    const SurvRec r = surv_at(b, k, s);
+    cam[k] = r.cam;
+    pid[k] = r.pid;
+    s = r.parent;
   }
+  cam[0] = ref;
+  pid[0] = s;
+}
+
+// reference camera: fewest observations among active cameras, lowest id on
+// ties (StereoMatch.cpp:70-85)
+__device__ __forceinline__ int ref_camera(const StereoBuffers &b, const StereoParams &p,
+                                          int &nref) {
+  int ref = -1;
+  int best = 0x7fffffff;
+  for (int c = 0; c < b.n_cam; ++c)
+    if (p.active[c] && b.npts[c] < best) {
+      best = b.npts[c];
+      ref = c;
+    }
+  nref = best;
+  return ref;
 }
 
 // cos(6.001 deg): a smallest pairwise parallax above 6.001 deg (and below
@@ -392,13 +459,15 @@ constexpr double kCosParallax = 0.9945200708509531;
 
 // Extend path P (length K) by point pid on camera t: back-projection
 // (checkBackProjection, :1206-1227), quick triangulation (:397-407), then
-// either checkMatch + emission (terminal) or the next-camera choice
+// either checkMatch (terminal) or the next-camera choice
 // (selectNextCameraByMaxPairAngle, :1375-1411).
-// Returns 0 reject, 1 emitted, 2 survivor (next camera in *next).
-template <int B, int K>
-__device__ int extend(const StereoBuffers &b, const StereoParams &p, Path<B> &P, int t,
-                      int pid, int *next) {
+// Returns 0 reject, 1 complete match (checkMatch error in *e_out),
+// 2 survivor (next camera in *next).
+template <int K>
+__device__ int extend(const StereoBuffers &b, const StereoParams &p, Path &P, int t,
+                      int pid, int *next, double *e_out) {
   constexpr int KK = K + 1;
+  const int B = p.build_count;
   const double tol2 = p.tol2d * p.tol2d;
 #pragma unroll
   for (int j = 0; j < K; ++j) {
@@ -407,7 +476,7 @@ __device__ int extend(const StereoBuffers &b, const StereoParams &p, Path<B> &P,
     const double z = d1 * l[2] - d0 * l[3];
     if (z * z > tol2) return 0;
   }
-  path_set<B>(P, K, b, t, pid);
+  path_set(P, K, b, t, pid);
   {
     // computeMinParallaxDeg: the smallest pairwise angle is the acos of the
     // largest clamped dot product (acos is monotone)
@@ -428,22 +497,22 @@ __device__ int extend(const StereoBuffers &b, const StereoParams &p, Path<B> &P,
       tq = tri_tol(p.tol3d, KK, B, acos(mx) * 180.0 / M_PI);
     }
     double X[3], e;
-    if (!triangulate(P.P, P.U, KK, X, e) || !(e <= tq)) return 0;
+    if (!triangulate_n<KK>(P.P, P.U, X, e) || !(e <= tq)) return 0;
   }
-  if constexpr (KK == B) {
+  if (KK == B) {
     // checkMatch, all-active branch (:530-540): LOS in camera order
-    int rank[B];
+    int rank[KK];
 #pragma unroll
-    for (int j = 0; j < B; ++j) {
+    for (int j = 0; j < KK; ++j) {
       rank[j] = 0;
 #pragma unroll
-      for (int i = 0; i < B; ++i) rank[j] += P.cam[i] < P.cam[j] ? 1 : 0;
+      for (int i = 0; i < KK; ++i) rank[j] += P.cam[i] < P.cam[j] ? 1 : 0;
     }
-    double cp[B][3], cu[B][3];
+    double cp[KK][3], cu[KK][3];
 #pragma unroll
-    for (int r = 0; r < B; ++r)
+    for (int r = 0; r < KK; ++r)
 #pragma unroll
-      for (int j = 0; j < B; ++j)
+      for (int j = 0; j < KK; ++j)
         if (rank[j] == r) {
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
@@ -452,147 +521,148 @@ __device__ int extend(const StereoBuffers &b, const StereoParams &p, Path<B> &P,
           }
         }
     double Xc[3], e;
-    if (!triangulate(cp, cu, B, Xc, e)) {
+    if (!triangulate_n<KK>(cp, cu, Xc, e)) {
       atomicOr(&b.flags[6], 1);
       return 0;
     }
     if (e > p.tol3d || !in_limit(p.limit, Xc)) return 0;
-    const int slot = atomicAdd(b.n_cand, 1);
-    if (slot < b.cap_cand) {
-      for (int c = 0; c < b.n_cam; ++c) {
-        int id = -1;
-#pragma unroll
-        for (int j = 0; j < B; ++j) id = P.cam[j] == c ? P.pid[j] : id;
-        b.cand_ids[(long)slot * b.n_cam + c] = id;
-      }
-      b.cand_e[slot] = e;
-    } else {
-      atomicOr(&b.flags[0], 1);
-    }
+    *e_out = e;
     return 1;
-  } else {
-    unsigned rem = 0;
-    for (int c = 0; c < b.n_cam; ++c) rem |= p.active[c] ? (1u << c) : 0u;
+  }
+  unsigned rem = 0;
+  for (int c = 0; c < b.n_cam; ++c) rem |= p.active[c] ? (1u << c) : 0u;
 #pragma unroll
-    for (int j = 0; j < KK; ++j) rem &= ~(1u << P.cam[j]);
-    if (rem == 0) return 0;
-    if ((rem & (rem - 1)) == 0) {
-      // a single remaining camera: its score (>= 0) beats the initial -1
-      *next = __ffs(rem) - 1;
-      return 2;
+  for (int j = 0; j < KK; ++j) rem &= ~(1u << P.cam[j]);
+  if (rem == 0) return 0;
+  if ((rem & (rem - 1)) == 0) {
+    // a single remaining camera: its score (>= 0) beats the initial -1
+    *next = __ffs(rem) - 1;
+    return 2;
+  }
+  int best = -1;
+  double best_score = -1.0;
+  for (int cam = 0; cam < b.n_cam; ++cam) {
+    if (!(rem & (1u << cam))) continue;
+    double lx[KK], ly[KK];
+#pragma unroll
+    for (int i = 0; i < KK; ++i) {
+      const double *l = line_of(b, P.cam[i], cam, P.pid[i]);
+      lx[i] = l[2];
+      ly[i] = l[3];
     }
-    int best = -1;
-    double best_score = -1.0;
-    for (int cam = 0; cam < b.n_cam; ++cam) {
-      if (!(rem & (1u << cam))) continue;
-      double lx[KK], ly[KK];
+    double score = 0.0;
+    bool stop = false;
 #pragma unroll
-      for (int i = 0; i < KK; ++i) {
-        const double *l = line_of(b, P.cam[i], cam, P.pid[i]);
-        lx[i] = l[2];
-        ly[i] = l[3];
-      }
-      double score = 0.0;
-      bool stop = false;
+    for (int i = 0; i < KK; ++i)
 #pragma unroll
-      for (int i = 0; i < KK; ++i)
-#pragma unroll
-        for (int j = i + 1; j < KK; ++j) {
-          if (!stop) {
-            const double s = fabs(lx[i] * ly[j] - ly[i] * lx[j]);
-            if (s > score) score = s;
-            if (score >= 1.0 - 1e-12) stop = true;
-          }
+      for (int j = i + 1; j < KK; ++j) {
+        if (!stop) {
+          const double s = fabs(lx[i] * ly[j] - ly[i] * lx[j]);
+          if (s > score) score = s;
+          if (score >= 1.0 - 1e-12) stop = true;
         }
-      if (score > best_score) {
-        best_score = score;
-        best = cam;
+      }
+    if (score > best_score) {
+      best_score = score;
+      best = cam;
+    }
+  }
+  *next = best;
+  return best < 0 ? 0 : 2;
+}
+
+// Second build camera of every reference observation
+// (selectSecondCameraByLineLength, :1307-1370): shortest in-image segment.
+__global__ void __launch_bounds__(256) k_b_second(StereoBuffers b, StereoParams p) {
+  int nref;
+  const int ref = ref_camera(b, p, nref);
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nref;
+       r += gridDim.x * blockDim.x) {
+    int t1 = -1;
+    double best_len = INFINITY;
+    for (int cam = 0; cam < b.n_cam; ++cam) {
+      if (!p.active[cam] || cam == ref) continue;
+      const double len = segment_in_image(b.cams[cam], line_of(b, ref, cam, r));
+      if (len < best_len - 1e-9) {
+        best_len = len;
+        t1 = cam;
       }
     }
-    *next = best;
-    return best < 0 ? 0 : 2;
+    b.t1[r] = (int8_t)t1;
   }
 }
 
-// Append a survivor of level LV to list LV-1.  The caller guarantees room.
-template <int B, int LV>
-__device__ __forceinline__ void push_survivor(BuildLds<B> &L, int pid, int par, int cam,
-                                              int nxt) {
-  const int pos = atomicAdd(&L.n_list[LV - 1], 1);
-  L.l_pid[LV - 1][pos] = pid;
-  L.l_par[LV - 1][pos] = (int16_t)par;
-  L.l_cam[LV - 1][pos] = (int8_t)cam;
-  L.l_nxt[LV - 1][pos] = (int8_t)nxt;
-}
-
-template <int B, int LV>
-__device__ void run_level(const StereoBuffers &b, const StereoParams &p, BuildLds<B> &L,
-                          int ref, int r);
-
-// Drain list LV-1 (the survivors of level LV) through level LV+1 if fewer
-// than a wave's worth of slots are left; `force` drains any remainder.
-template <int B, int LV>
-__device__ __forceinline__ void make_room(const StereoBuffers &b, const StereoParams &p,
-                                          BuildLds<B> &L, int ref, int r, bool force) {
-  if constexpr (LV + 1 < B) {
-    __syncthreads();
-    const int n = L.n_list[LV - 1];
-    if (force ? n > 0 : n + kWave > kList) {
-      run_level<B, LV + 1>(b, p, L, ref, r);
-      __syncthreads();
-      if (lane_id() == 0) L.n_list[LV - 1] = 0;
-      __syncthreads();
+// Level-1 band: one thread per (reference observation, cell row of its
+// second camera); accepted points become (r, pid) pairs.
+__global__ void __launch_bounds__(256) k_b_band(StereoBuffers b, StereoParams p) {
+  int nref;
+  const int ref = ref_camera(b, p, nref);
+  const int rows = b.rows_cell, cols = b.cols_cell;
+  const long ncell = (long)rows * cols;
+  const long total = (long)nref * rows;
+  const double tolg = p.tol2d + 1e-6;
+  const double tol2 = p.tol2d * p.tol2d;
+  const long stride = (long)gridDim.x * blockDim.x;
+  __shared__ BlockQueue<int2, kQueue> q;
+  bq_init(q);
+  auto out = [&](long i, const int2 &v) { b.pairs[i] = v; };
+  for (long base = (long)blockIdx.x * blockDim.x; base < total; base += stride) {
+    const long item = base + threadIdx.x;
+    const int r = (int)(item / rows), cy = (int)(item % rows);
+    const int t1 = item < total ? (int)b.t1[r] : -1;
+    RowCursor cur{0, 0};
+    Strip s{};
+    const double *xy = b.xy;
+    const int32_t *cpid = b.cell_pid;
+    if (t1 >= 0) {
+      s = strip_of(line_of(b, ref, t1, r));
+      int xa, xb;
+      if (strip_span(s, cy, b.cell_px, cols, tolg, xa, xb))
+        row_range(b.cell_start + t1 * ncell, b.cell_cnt + t1 * ncell, cy, cols, xa, xb, cur);
+      xy += (long)t1 * b.cap * 2;
+      cpid += (long)t1 * b.cap;
+    }
+    while (__syncthreads_or(cur.kk < cur.kend)) {
+      bool acc = false;
+      int pid = 0;
+      if (cur.kk < cur.kend) {
+        pid = cpid[cur.kk++];
+        acc = in_strip(s, xy[2 * pid], xy[2 * pid + 1], tol2);
+      }
+      bq_push(q, acc, make_int2(r, pid));
+      bq_flush(q, b.n_pairs, b.cap_pairs, &b.flags[1], false, out);
     }
   }
+  bq_flush(q, b.n_pairs, b.cap_pairs, &b.flags[1], true, out);
 }
 
-// Per-lane cursor over the contiguous point range of one cell row: within a
-// row, cells cx in [xa, xb] are consecutive in the CSR index.
-struct RowCursor {
-  int kk, kend;
-};
-
-__device__ __forceinline__ void row_range(const int32_t *cstart, const int32_t *ccnt,
-                                          int cy, int cols, int xa, int xb, RowCursor &c) {
-  const int c0 = cy * cols + xa, c1 = cy * cols + xb;
-  c.kk = cstart[c0];
-  c.kend = cstart[c1] + ccnt[c1];
-}
-
-// Levels >= 2: lane = survivor of list LV-2, enumerating the cell rows of
-// its K-strip intersection on its next camera; pairs are compacted into
-// the level's window, then extended lane-parallel.
-template <int B, int LV>
-__device__ void run_level(const StereoBuffers &b, const StereoParams &p, BuildLds<B> &L,
-                          int ref, int r) {
-  constexpr int K = LV;  // path length of the input survivors
-  const int lane = lane_id();
+// Deeper bands: one thread per survivor of level K-1 (a path of length K),
+// enumerating the cell rows of its K-strip intersection on its next camera.
+template <int K>
+__global__ void __launch_bounds__(256) k_b_enum(StereoBuffers b, StereoParams p) {
+  int nref;
+  const int ref = ref_camera(b, p, nref);
   const int rows = b.rows_cell, cols = b.cols_cell;
   const long ncell = (long)rows * cols;
   const double tolg = p.tol2d + 1e-6;
   const double tol2 = p.tol2d * p.tol2d;
-  __syncthreads();
-  const int n_in = L.n_list[LV - 2];
-  for (int c0 = 0; c0 < n_in; c0 += kWave) {
-    const int e = c0 + lane;
-    bool done = e >= n_in;
+  const int n_in = min(b.n_surv[K - 1], (int)b.cap_pairs);
+  const int stride = gridDim.x * blockDim.x;
+  __shared__ BlockQueue<int2, kQueue> q;
+  bq_init(q);
+  auto out = [&](long i, const int2 &v) { b.pairs[i] = v; };
+  for (int base = blockIdx.x * blockDim.x; base < n_in; base += stride) {
+    const int s = base + threadIdx.x;
+    bool live = s < n_in;
     Strip S[K];
     int t = 0, cy = 0, r1 = -1;
     RowCursor cur{0, 0};
-    if (!done) {
-      t = L.l_nxt[LV - 2][e];
-      int cam[K], pidv[K];
-      cam[0] = ref;
-      pidv[0] = r;
-      int q = e;
+    if (live) {
+      t = surv_at(b, K - 1, s).next;
+      int cam[kMaxCam], pid[kMaxCam];
+      path_ids<K>(b, ref, s, cam, pid);
 #pragma unroll
-      for (int d = K - 1; d >= 1; --d) {
-        cam[d] = L.l_cam[d - 1][q];
-        pidv[d] = L.l_pid[d - 1][q];
-        q = L.l_par[d - 1][q];
-      }
-#pragma unroll
-      for (int j = 0; j < K; ++j) S[j] = strip_of(line_of(b, cam[j], t, pidv[j]));
+      for (int j = 0; j < K; ++j) S[j] = strip_of(line_of(b, cam[j], t, pid[j]));
       int r0 = 0;
       r1 = rows - 1;
 #pragma unroll
@@ -603,177 +673,110 @@ __device__ void run_level(const StereoBuffers &b, const StereoParams &p, BuildLd
     const int32_t *ccnt = b.cell_cnt + t * ncell;
     const int32_t *cpid = b.cell_pid + (long)t * b.cap;
     const double *xy = b.xy + (long)t * b.cap * 2;
-    while (true) {
-      // fill the window
-      __syncthreads();
-      if (lane == 0) L.n_pair[LV - 1] = 0;
-      __syncthreads();
-      while (true) {
-        if (!done) {
-          while (cur.kk == cur.kend) {
-            ++cy;
-            if (cy > r1) {
-              done = true;
-              break;
-            }
-            int xa = 0, xb = cols - 1;
-            bool ok = true;
-#pragma unroll
-            for (int j = 0; j < K; ++j) {
-              int a, bb;
-              if (ok) {
-                ok = strip_span(S[j], cy, b.cell_px, cols, tolg, a, bb);
-                xa = a > xa ? a : xa;
-                xb = bb < xb ? bb : xb;
-                ok = ok && xa <= xb;
-              }
-            }
-            if (ok) row_range(cstart, ccnt, cy, cols, xa, xb, cur);
+    while (__syncthreads_or(live)) {
+      bool acc = false;
+      int pid = 0;
+      if (live) {
+        while (cur.kk == cur.kend) {
+          ++cy;
+          if (cy > r1) {
+            live = false;
+            break;
           }
-          if (!done) {
-            const int pid = cpid[cur.kk];
-            const double qx = xy[2 * pid], qy = xy[2 * pid + 1];
-            bool acc = true;
+          int xa = 0, xb = cols - 1;
+          bool ok = true;
 #pragma unroll
-            for (int j = 0; j < K; ++j) acc = acc && in_strip(S[j], qx, qy, tol2);
-            if (acc) {
-              const int pos = atomicAdd(&L.n_pair[LV - 1], 1);
-              if (pos < kWin) {
-                L.pair_pid[LV - 1][pos] = pid;
-                L.pair_e[LV - 1][pos] = (int16_t)e;
-                ++cur.kk;
-              }  // else: retried in the next window
-            } else {
-              ++cur.kk;
+          for (int j = 0; j < K; ++j) {
+            int a, bb;
+            if (ok) {
+              ok = strip_span(S[j], cy, b.cell_px, cols, tolg, a, bb);
+              xa = a > xa ? a : xa;
+              xb = bb < xb ? bb : xb;
+              ok = ok && xa <= xb;
             }
           }
+          if (ok) row_range(cstart, ccnt, cy, cols, xa, xb, cur);
         }
-        __syncthreads();
-        if (L.n_pair[LV - 1] >= kWin || !__any(!done)) break;
-      }
-      const int n = L.n_pair[LV - 1] < kWin ? L.n_pair[LV - 1] : kWin;
-      // extend the window's pairs lane-parallel
-      for (int i0 = 0; i0 < n; i0 += kWave) {
-        make_room<B, LV>(b, p, L, ref, r, false);
-        const int i = i0 + lane;
-        if (i < n) {
-          const int es = L.pair_e[LV - 1][i];
-          const int pid = L.pair_pid[LV - 1][i];
-          const int tt = L.l_nxt[LV - 2][es];
-          Path<B> P;
-          load_path<B, K>(b, L, ref, r, es, P);
-          int next = -1;
-          if (extend<B, K>(b, p, P, tt, pid, &next) == 2) {
-            if constexpr (LV + 1 < B) push_survivor<B, LV>(L, pid, es, tt, next);
-          }
+        if (live) {
+          pid = cpid[cur.kk++];
+          const double qx = xy[2 * pid], qy = xy[2 * pid + 1];
+          acc = true;
+#pragma unroll
+          for (int j = 0; j < K; ++j) acc = acc && in_strip(S[j], qx, qy, tol2);
         }
       }
-      if (!__any(!done)) break;
+      bq_push(q, acc, make_int2(s, pid));
+      bq_flush(q, b.n_pairs, b.cap_pairs, &b.flags[1], false, out);
     }
   }
-  make_room<B, LV>(b, p, L, ref, r, true);
+  bq_flush(q, b.n_pairs, b.cap_pairs, &b.flags[1], true, out);
 }
 
-template <int B>
-__global__ void __launch_bounds__(kWave) k_build(StereoBuffers b, StereoParams p) {
-  __shared__ BuildLds<B> L;
-  const int lane = lane_id();
-  // reference camera: fewest observations among active cameras, lowest id
-  // on ties (StereoMatch.cpp:70-85)
-  int ref = -1;
-  int best = 0x7fffffff;
-  for (int c = 0; c < b.n_cam; ++c)
-    if (p.active[c] && b.npts[c] < best) {
-      best = b.npts[c];
-      ref = c;
-    }
-  const int nref = best;
-  const int rows = b.rows_cell, cols = b.cols_cell;
-  const long ncell = (long)rows * cols;
-  const double tolg = p.tol2d + 1e-6;
-  const double tol2 = p.tol2d * p.tol2d;
-  if (lane == 0)
-    for (int d = 0; d < BuildLds<B>::NL; ++d) L.n_list[d] = 0;
-  while (true) {
-    int r = 0;
-    if (lane == 0) r = atomicAdd(b.work, 1);
-    r = __shfl(r, 0, kWave);
-    if (r >= nref) break;
-    // selectSecondCameraByLineLength (:1307-1370): shortest in-image segment
-    int t1 = -1;
-    {
-      double best_len = INFINITY;
-      for (int cam = 0; cam < b.n_cam; ++cam) {
-        if (!p.active[cam] || cam == ref) continue;
-        const double *l = line_of(b, ref, cam, r);
-        const double len = segment_in_image(b.cams[cam], l);
-        if (len < best_len - 1e-9) {
-          best_len = len;
-          t1 = cam;
-        }
+// Extension of every (path of length K, point) pair: a survivor goes to
+// level K, a complete match (K + 1 == build_count) to the candidate list.
+template <int K>
+__global__ void __launch_bounds__(256) k_b_extend(StereoBuffers b, StereoParams p) {
+  int nref;
+  const int ref = ref_camera(b, p, nref);
+  const int n = min(*b.n_pairs, (int)b.cap_pairs);
+  const bool terminal = K + 1 == p.build_count;
+  const int stride = gridDim.x * blockDim.x;
+  __shared__ BlockQueue<CandRec, kCandQueue> qc;
+  __shared__ BlockQueue<SurvRec, kSurvQueue> qs;
+  bq_init(qc);
+  bq_init(qs);
+  const int nc = b.n_cam;
+  auto out_c = [&](long i, const CandRec &v) {
+    for (int c = 0; c < nc; ++c) b.cand_ids[i * nc + c] = v.ids[c];
+    b.cand_e[i] = v.e;
+  };
+  SurvRec *surv_out = b.surv + (long)(K - 1) * b.cap_pairs;
+  auto out_s = [&](long i, const SurvRec &v) { surv_out[i] = v; };
+  for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    const int q = base + threadIdx.x;
+    int res = 0, next = -1, t = 0;
+    double e = 0.0;
+    Path P;
+    int2 pr = make_int2(0, 0);
+    if (q < n) {
+      pr = b.pairs[q];
+      int cam[kMaxCam], pid[kMaxCam];
+      if constexpr (K == 1) {
+        cam[0] = ref;
+        pid[0] = pr.x;
+        t = b.t1[pr.x];
+      } else {
+        path_ids<K>(b, ref, pr.x, cam, pid);
+        t = surv_at(b, K - 1, pr.x).next;
       }
+#pragma unroll
+      for (int j = 0; j < K; ++j) path_set(P, j, b, cam[j], pid[j]);
+      res = extend<K>(b, p, P, t, pr.y, &next, &e);
     }
-    if (t1 < 0) continue;
-    const Strip s1 = strip_of(line_of(b, ref, t1, r));
-    const int32_t *cstart = b.cell_start + t1 * ncell;
-    const int32_t *ccnt = b.cell_cnt + t1 * ncell;
-    const int32_t *cpid = b.cell_pid + (long)t1 * b.cap;
-    const double *xy = b.xy + (long)t1 * b.cap * 2;
-    // level 1: one band over all cell rows, lane = cell row (stride 64)
-    int cy = lane - kWave;
-    bool done = false;
-    RowCursor cur{0, 0};
-    while (true) {
-      __syncthreads();
-      if (lane == 0) L.n_pair[0] = 0;
-      __syncthreads();
-      while (true) {
-        if (!done) {
-          while (cur.kk == cur.kend) {
-            cy += kWave;
-            if (cy >= rows) {
-              done = true;
-              break;
-            }
-            int xa, xb;
-            if (strip_span(s1, cy, b.cell_px, cols, tolg, xa, xb))
-              row_range(cstart, ccnt, cy, cols, xa, xb, cur);
-          }
-          if (!done) {
-            const int pid = cpid[cur.kk];
-            if (in_strip(s1, xy[2 * pid], xy[2 * pid + 1], tol2)) {
-              const int pos = atomicAdd(&L.n_pair[0], 1);
-              if (pos < kWin) {
-                L.pair_pid[0][pos] = pid;
-                ++cur.kk;
-              }
-            } else {
-              ++cur.kk;
-            }
-          }
+    if (terminal) {
+      CandRec cr;
+      if (res == 1) {
+        for (int c = 0; c < nc; ++c) {
+          int id = -1;
+#pragma unroll
+          for (int j = 0; j <= K; ++j) id = P.cam[j] == c ? P.pid[j] : id;
+          cr.ids[c] = id;
         }
-        __syncthreads();
-        if (L.n_pair[0] >= kWin || !__any(!done)) break;
+        cr.e = e;
       }
-      const int n = L.n_pair[0] < kWin ? L.n_pair[0] : kWin;
-      for (int i0 = 0; i0 < n; i0 += kWave) {
-        make_room<B, 1>(b, p, L, ref, r, false);
-        const int i = i0 + lane;
-        if (i < n) {
-          const int pid = L.pair_pid[0][i];
-          Path<B> P;
-          path_set<B>(P, 0, b, ref, r);
-          int next = -1;
-          if (extend<B, 1>(b, p, P, t1, pid, &next) == 2) {
-            if constexpr (B > 2) push_survivor<B, 1>(L, pid, -1, t1, next);
-          }
-        }
-      }
-      if (!__any(!done)) break;
+      bq_push(qc, res == 1, cr);
+      bq_flush(qc, b.n_cand, b.cap_cand, &b.flags[0], false, out_c);
+    } else {
+      bq_push(qs, res == 2, SurvRec{pr.x, pr.y, (int8_t)t, (int8_t)next, 0});
+      bq_flush(qs, &b.n_surv[K], b.cap_pairs, &b.flags[1], false, out_s);
     }
-    make_room<B, 1>(b, p, L, ref, r, true);
   }
+  if (terminal)
+    bq_flush(qc, b.n_cand, b.cap_cand, &b.flags[0], true, out_c);
+  else
+    bq_flush(qs, &b.n_surv[K], b.cap_pairs, &b.flags[1], true, out_s);
 }
+
 // --------------------------------------------------------------------------
 // lexicographic sort of candidate id vectors
 // --------------------------------------------------------------------------
@@ -1147,7 +1150,7 @@ __global__ void k_replace_check(StereoBuffers b, StereoParams p) {
 }
 
 // triangulateMatch: camera-order LOS, FP64, gate on tol_3d
-__global__ void k_tri(StereoBuffers b, StereoParams p) {
+__global__ void __launch_bounds__(256) k_tri(StereoBuffers b, StereoParams p) {
   const int M = min(*b.n_cand, b.cap_cand);
   const int nc = b.n_cam;
   int nsel = 0;
@@ -1215,6 +1218,13 @@ static int grid_for(long n, int threads, int max_blocks) {
   return (int)g;
 }
 
+template <int K>
+static void build_level(const StereoBuffers &b, const StereoParams &p, hipStream_t st) {
+  zero_i32(b.n_pairs, 1, st);
+  k_b_enum<K><<<4096, 256, 0, st>>>(b, p);
+  k_b_extend<K><<<4096, 256, 0, st>>>(b, p);
+}
+
 void launch_stereo(const StereoBuffers &b, const StereoParams &p,
                    int max_rounds, hipStream_t st) {
   const long ncell = (long)b.rows_cell * b.cols_cell;
@@ -1232,18 +1242,21 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   scan_exclusive(b.cell_cnt, b.cell_start, ncell, b.n_cam, ncell, nullptr,
                  b.scan_scratch, st);
   k_cell_fill<<<gc, T, 0, st>>>(b, p);
-  // build
-  const int waves = 256 * 12;
+  // build: level 1 from every reference observation, then one enumerate +
+  // extend pair of kernels per deeper build camera
+  const int G = 4096;
   stage_begin(ST_BUILD, st);
-  switch (p.build_count) {
-  case 2: k_build<2><<<waves, kWave, 0, st>>>(b, p); break;
-  case 3: k_build<3><<<waves, kWave, 0, st>>>(b, p); break;
-  case 4: k_build<4><<<waves, kWave, 0, st>>>(b, p); break;
-  case 5: k_build<5><<<waves, kWave, 0, st>>>(b, p); break;
-  case 6: k_build<6><<<waves, kWave, 0, st>>>(b, p); break;
-  case 7: k_build<7><<<waves, kWave, 0, st>>>(b, p); break;
-  default: k_build<8><<<waves, kWave, 0, st>>>(b, p); break;
-  }
+  zero_i32(b.n_surv, kMaxCam, st);
+  zero_i32(b.n_pairs, 1, st);
+  k_b_second<<<grid_for(b.cap, T, 1024), T, 0, st>>>(b, p);
+  k_b_band<<<G, T, 0, st>>>(b, p);
+  k_b_extend<1><<<G, T, 0, st>>>(b, p);
+  if (p.build_count > 2) build_level<2>(b, p, st);
+  if (p.build_count > 3) build_level<3>(b, p, st);
+  if (p.build_count > 4) build_level<4>(b, p, st);
+  if (p.build_count > 5) build_level<5>(b, p, st);
+  if (p.build_count > 6) build_level<6>(b, p, st);
+  if (p.build_count > 7) build_level<7>(b, p, st);
   stage_end(ST_BUILD, st);
   stage_begin(ST_PRUNE, st);
   // sort
