@@ -105,6 +105,8 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
     ap.add_argument("--graph", action="store_true", help="replay a captured hipGraph")
+    ap.add_argument("--streams", type=int, default=4,
+                    help="frames of a step run concurrently on this many HIP streams")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
@@ -129,6 +131,7 @@ def main() -> int:
     otf, _keep = S.make_otf(4, w0.otf_abca)
     ctx = Context(w0.cams, H, W, precision=args.precision)
     ctx.set_otf(otf)
+    ctx.set_concurrency(args.streams)
     cap_obj = 1 << 17
     outs = ctx.alloc_outputs(args.frames, cap_obj, dimg.device)
 
@@ -208,6 +211,7 @@ def main() -> int:
                    "ipr_loops": int(w0.params.n_loop_ipr),
                    "shake_loops": int(w0.params.n_shake_loop),
                    "shake": "plain", "graph": bool(args.graph),
+                   "streams": args.streams,
                    "quality": qual},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -51,6 +51,13 @@ olpt_status olpt_ctx_reserve(olpt_ctx *ctx, int n_cam, int n_row, int n_col,
                              int max_2d_per_cam, int max_candidates,
                              int max_objects);
 olpt_status olpt_set_precision(olpt_ctx *ctx, int precision);
+/* No counterpart in the reference (its IPR runs one frame per call on host
+ * threads).  olpt_ipr_run spreads the frames of one call over n_streams
+ * workspaces, each with its own HIP stream, so that one frame's small
+ * kernels overlap another frame's work; 1 (default) keeps every frame on
+ * the context stream.  Each extra stream holds a full workspace, as sized
+ * by olpt_ctx_reserve.  n_streams: 1..8. */
+olpt_status olpt_set_concurrency(olpt_ctx *ctx, int n_streams);
 /* Replaces Camera objects (Camera.h:78-174, pinhole model). */
 olpt_status olpt_set_cameras(olpt_ctx *ctx, const olpt_camera *cams, int n_cam);
 /* Replaces TracerConfig::_otf (OTF.h:33-78); arrays are copied. */

@@ -56,6 +56,7 @@ def lib() -> ctypes.CDLL:
     L.olpt_ctx_destroy.argtypes = [_P]
     L.olpt_ctx_reserve.argtypes = [_P, _I, _I, _I, _I, _I, _I]
     L.olpt_set_precision.argtypes = [_P, _I]
+    L.olpt_set_concurrency.argtypes = [_P, _I]
     L.olpt_set_cameras.argtypes = [_P, ctypes.POINTER(OlptCamera), _I]
     L.olpt_set_otf.argtypes = [_P, ctypes.POINTER(OlptOtf)]
     L.olpt_sync.argtypes = [_P]
@@ -81,7 +82,8 @@ def lib() -> ctypes.CDLL:
     L.olpt_stage_times.argtypes = [_P, _DP, _I32P]
     L.olpt_stage_stats.argtypes = [_P, _I, _DP, _I32P]
     for name in ("olpt_ctx_create", "olpt_ctx_destroy", "olpt_ctx_reserve",
-                 "olpt_set_precision", "olpt_set_cameras", "olpt_set_otf",
+                 "olpt_set_precision", "olpt_set_concurrency", "olpt_set_cameras",
+                 "olpt_set_otf",
                  "olpt_sync", "olpt_diagnostics", "olpt_find_peaks",
                  "olpt_stereo_match", "olpt_shake", "olpt_residual",
                  "olpt_ipr_run", "olpt_ipr_graph_capture",

@@ -83,6 +83,11 @@ class Context:
         check(self.L.olpt_set_precision(
             self.h, _lib.OLPT_PREC_F64 if precision == "f64" else _lib.OLPT_PREC_F32))
 
+    def set_concurrency(self, n_streams: int):
+        """Spread the frames of one ipr_run over n_streams workspaces, each
+        on its own HIP stream (1 = every frame on the context stream)."""
+        check(self.L.olpt_set_concurrency(self.h, int(n_streams)))
+
     @property
     def real_dtype(self):
         return torch.float64 if self.precision == "f64" else torch.float32

@@ -42,22 +42,12 @@ constexpr int kMaxRounds = 64;
 
 } // namespace
 
-struct olpt_ctx {
-  int device = 0;
-  hipStream_t st = nullptr;
-  int prec = OLPT_PREC_F32;
-  int n_cam = 0, H = 0, W = 0, cap2d = 0, cap_cand = 0, cap_obj = 0;
-  int cams_set = 0, otf_set = 0;
-  olpt_camera cams[OLPT_MAX_CAM];
-  std::vector<void *> allocs;
-  // cameras / OTF on device
-  CamT<double> *d_cam64 = nullptr;
-  CamT<float> *d_cam32 = nullptr;
-  int otf_nx = 2, otf_ny = 2, otf_nz = 2;
-  double otf_bnd[6] = {0, 0, 0, 0, 0, 0};
-  double *d_otf64 = nullptr; // a | b | c | alpha, each n_cam*ng
-  float *d_otf32 = nullptr;
-  int otf_ng = 0;
+// Per-stream workspace: everything one frame's IPR writes.  Frames of one
+// olpt_ipr_run call are spread over the workspaces, each on its own stream,
+// so the many small kernels of one frame overlap with another frame's work.
+struct Workspace {
+  hipStream_t st = nullptr;     // own stream (workspace 0: the context stream)
+  hipEvent_t done = nullptr;
   // images (sized for FP64)
   void *img_a = nullptr, *img_b = nullptr, *img_res = nullptr;
   // peaks
@@ -74,16 +64,42 @@ struct olpt_ctx {
   int32_t *sh_n = nullptr, *n_keep = nullptr;
   int32_t *bin_cnt = nullptr, *bin_start = nullptr, *bin_obj = nullptr, *obj_bin = nullptr,
           *obj_rank = nullptr;
-  int bin_rows = 0, bin_cols = 0;
   int32_t *g_cnt = nullptr, *g_start = nullptr, *g_obj = nullptr, *g_rank = nullptr,
           *g_cellof = nullptr, *has_nb = nullptr, *sh_keep_off = nullptr;
-  int grid_n = 80;
   void *rec = nullptr;
   double *partial = nullptr;
   // misc
   int32_t *flags = nullptr;  // [16]
   int32_t *scan_scratch = nullptr;
   int32_t *totals = nullptr; // [64]
+};
+
+constexpr int kMaxWorkspaces = 8;
+
+struct olpt_ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  int prec = OLPT_PREC_F32;
+  int n_cam = 0, H = 0, W = 0, cap2d = 0, cap_cand = 0, cap_obj = 0;
+  int cams_set = 0, otf_set = 0;
+  olpt_camera cams[OLPT_MAX_CAM];
+  std::vector<void *> allocs;
+  // cameras / OTF on device
+  CamT<double> *d_cam64 = nullptr;
+  CamT<float> *d_cam32 = nullptr;
+  int otf_nx = 2, otf_ny = 2, otf_nz = 2;
+  double otf_bnd[6] = {0, 0, 0, 0, 0, 0};
+  double *d_otf64 = nullptr; // a | b | c | alpha, each n_cam*ng
+  float *d_otf32 = nullptr;
+  int otf_ng = 0;
+  int bin_rows = 0, bin_cols = 0;
+  int grid_n = 80;
+  // workspaces
+  Workspace ws[kMaxWorkspaces];
+  int n_ws = 0;          // allocated
+  int n_ws_want = 1;     // workspaces to keep allocated
+  int n_ws_active = 1;   // workspaces olpt_ipr_run spreads frames over
+  hipEvent_t ev_fork = nullptr;
   int32_t *loop_tot = nullptr; // [n_frames*cap_loop] device
   int loop_tot_cap = 0;
   // stage timing and algorithmic-byte accounting
@@ -128,6 +144,8 @@ struct TimerScope {
   }
   ~TimerScope() { g_timer = nullptr; }
 };
+
+void free_streams(olpt_ctx *c);
 
 void free_all(olpt_ctx *c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -236,7 +254,8 @@ template <typename T> OtfT<T> dotf(olpt_ctx *c) {
   return o;
 }
 
-template <typename T> ShakeBuffers<T> shake_buffers(olpt_ctx *c, T *img, T *img_next) {
+template <typename T>
+ShakeBuffers<T> shake_buffers(olpt_ctx *c, Workspace &w, T *img, T *img_next) {
   ShakeBuffers<T> b{};
   b.n_cam = c->n_cam;
   b.H = c->H;
@@ -244,36 +263,36 @@ template <typename T> ShakeBuffers<T> shake_buffers(olpt_ctx *c, T *img, T *img_
   b.cams = dcams<T>(c);
   b.otf = dotf<T>(c);
   b.img = img;
-  b.res = (T *)c->img_res;
+  b.res = (T *)w.img_res;
   b.img_next = img_next;
   b.cap_obj = c->cap_obj;
-  b.xyz = c->sh_xyz;
-  b.uv = (T *)c->sh_uv;
-  b.score = c->sh_score;
-  b.flag = c->sh_flag;
-  b.n_obj = c->sh_n;
+  b.xyz = w.sh_xyz;
+  b.uv = (T *)w.sh_uv;
+  b.score = w.sh_score;
+  b.flag = w.sh_flag;
+  b.n_obj = w.sh_n;
   b.bin_px = 8;
   b.bin_rows = c->bin_rows;
   b.bin_cols = c->bin_cols;
-  b.bin_cnt = c->bin_cnt;
-  b.bin_start = c->bin_start;
-  b.bin_obj = c->bin_obj;
-  b.obj_bin = c->obj_bin;
-  b.obj_rank = c->obj_rank;
+  b.bin_cnt = w.bin_cnt;
+  b.bin_start = w.bin_start;
+  b.bin_obj = w.bin_obj;
+  b.obj_bin = w.obj_bin;
+  b.obj_rank = w.obj_rank;
   b.grid_n = c->grid_n;
-  b.g_cnt = c->g_cnt;
-  b.g_start = c->g_start;
-  b.g_obj = c->g_obj;
-  b.g_rank = c->g_rank;
-  b.g_cellof = c->g_cellof;
-  b.has_nb = c->has_nb;
-  b.rec = c->rec;
-  b.partial = c->partial;
-  b.keep_off = c->sh_keep_off;
-  b.n_keep = c->n_keep;
-  b.scan_scratch = c->scan_scratch;
-  b.totals = c->totals;
-  b.work = c->sb.work;
+  b.g_cnt = w.g_cnt;
+  b.g_start = w.g_start;
+  b.g_obj = w.g_obj;
+  b.g_rank = w.g_rank;
+  b.g_cellof = w.g_cellof;
+  b.has_nb = w.has_nb;
+  b.rec = w.rec;
+  b.partial = w.partial;
+  b.keep_off = w.sh_keep_off;
+  b.n_keep = w.n_keep;
+  b.scan_scratch = w.scan_scratch;
+  b.totals = w.totals;
+  b.work = w.sb.work;
   return b;
 }
 
@@ -348,7 +367,7 @@ ShakeParams shake_params(olpt_ctx *c, const int32_t *active, const olpt_ipr_para
 olpt_status check_ready(olpt_ctx *c) {
   if (!c) return fail(OLPT_E_INVALID_ARGUMENT, "null context");
   if (!c->cams_set) return fail(OLPT_E_INVALID_ARGUMENT, "cameras not set");
-  if (!c->xy) return fail(OLPT_E_INVALID_ARGUMENT, "workspace not reserved");
+  if (!c->n_ws || !c->ws[0].xy) return fail(OLPT_E_INVALID_ARGUMENT, "workspace not reserved");
   return OLPT_OK;
 }
 
@@ -361,17 +380,17 @@ olpt_status check_params(olpt_ctx *c, const olpt_ipr_params *p) {
   if (roi * roi > 4 * 64 || box > 24)
     return fail(OLPT_E_UNSUPPORTED, "ROI larger than 16x16 pixels is not supported");
   if (p->idmap_cell_px < 1) return fail(OLPT_E_INVALID_ARGUMENT, "idmap_cell_px must be >= 1");
-  if (p->idmap_cell_px != c->sb.cell_px)
+  if (p->idmap_cell_px != c->ws[0].sb.cell_px)
     return fail(OLPT_E_UNSUPPORTED, "idmap_cell_px differs from the reserved workspace (4)");
   return OLPT_OK;
 }
 
 // One IPR loop on the active subset: img (loop input, T) -> img_next.
 template <typename T>
-olpt_status ipr_loop(olpt_ctx *c, T *img, T *img_next, const int32_t *active,
+olpt_status ipr_loop(olpt_ctx *c, Workspace &w, T *img, T *img_next, const int32_t *active,
                      const olpt_ipr_params *p, double tol2d, double *out_xyz,
                      double *out_uv, int cap_out, int32_t *d_n_out, int32_t *d_loop_tot) {
-  hipStream_t st = c->st;
+  hipStream_t st = w.st;
   olpt_status s;
   StereoParams sp = stereo_params(c, active, p, tol2d, &s);
   if (s) return s;
@@ -380,17 +399,17 @@ olpt_status ipr_loop(olpt_ctx *c, T *img, T *img_next, const int32_t *active,
     if (sp.active[k]) cl.id[cl.n++] = k;
   const long HW = (long)c->H * c->W;
   // 1) ObjectFinder2D on the active cameras (inactive slots stay empty)
-  fill_i32(c->npts, c->n_cam, 0, st);
-  launch_find_peaks<T>(img, HW, c->H, c->W, (double)p->min_obj_int, cl, c->pk_mask, c->pk_cnt,
-                       c->pk_off, c->scan_scratch, c->totals, c->xy, c->cap2d, c->npts,
-                       c->flags + 8, st);
-  k_peak_overflow<<<1, 1, 0, st>>>(c->flags + 8, c->flags);
+  fill_i32(w.npts, c->n_cam, 0, st);
+  launch_find_peaks<T>(img, HW, c->H, c->W, (double)p->min_obj_int, cl, w.pk_mask, w.pk_cnt,
+                       w.pk_off, w.scan_scratch, w.totals, w.xy, c->cap2d, w.npts,
+                       w.flags + 8, st);
+  k_peak_overflow<<<1, 1, 0, st>>>(w.flags + 8, w.flags);
   // 2) StereoMatch
-  launch_stereo(c->sb, sp, kMaxRounds, st);
+  launch_stereo(w.sb, sp, kMaxRounds, st);
   // 3) Shake the matched objects in place
-  ShakeBuffers<T> b = shake_buffers<T>(c, img, img_next);
-  b.xyz = c->sb.obj_xyz;
-  b.n_obj = c->sb.n_obj;
+  ShakeBuffers<T> b = shake_buffers<T>(c, w, img, img_next);
+  b.xyz = w.sb.obj_xyz;
+  b.n_obj = w.sb.n_obj;
   if constexpr (sizeof(T) == 8)
     set_grid(c, nullptr, &b, p);
   else
@@ -398,13 +417,13 @@ olpt_status ipr_loop(olpt_ctx *c, T *img, T *img_next, const int32_t *active,
   ShakeParams shp = shake_params(c, active, p);
   launch_shake<T>(b, shp, st);
   // 4) filterOutInvalid + calResidualImage (IPR mode)
-  launch_filter<T>(b, c->keep_xyz, (T *)c->keep_uv, c->n_keep, st);
-  launch_residual_ipr<T>(b, c->n_keep, c->keep_xyz, (T *)c->keep_uv, p->radius_obj, st);
+  launch_filter<T>(b, w.keep_xyz, (T *)w.keep_uv, w.n_keep, st);
+  launch_residual_ipr<T>(b, w.n_keep, w.keep_xyz, (T *)w.keep_uv, p->radius_obj, st);
   // 5) append
   if (out_xyz) {
-    k_append<T><<<256, 256, 0, st>>>(c->keep_xyz, (const T *)c->keep_uv, c->n_keep, c->n_cam,
+    k_append<T><<<256, 256, 0, st>>>(w.keep_xyz, (const T *)w.keep_uv, w.n_keep, c->n_cam,
                                      out_xyz, out_uv, cap_out, d_n_out);
-    k_append_count<<<1, 1, 0, st>>>(c->n_keep, cap_out, d_n_out, d_loop_tot, c->flags);
+    k_append_count<<<1, 1, 0, st>>>(w.n_keep, cap_out, d_n_out, d_loop_tot, w.flags);
   }
   HIP_TRY(hipGetLastError());
   if (c->stats) {
@@ -417,16 +436,16 @@ olpt_status ipr_loop(olpt_ctx *c, T *img, T *img_next, const int32_t *active,
     //            reference ROI (P = 2 round(r ratio)) + positions
     //  residual: read the loop image and write the residual per camera
     int32_t np[OLPT_MAX_CAM], cnt[3];
-    HIP_TRY(hipMemcpyAsync(np, c->npts, sizeof(np), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(&cnt[0], c->sb.n_cand, 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(&cnt[1], c->sb.n_obj, 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(&cnt[2], c->n_keep, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(np, w.npts, sizeof(np), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&cnt[0], w.sb.n_cand, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&cnt[1], w.sb.n_obj, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&cnt[2], w.n_keep, 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     double n2d = 0;
     for (int k = 0; k < c->n_cam; ++k) n2d += np[k];
     const double s_t = sizeof(T);
     const double na = cl.n, nc = c->n_cam;
-    const double ncell = (double)c->sb.rows_cell * c->sb.cols_cell;
+    const double ncell = (double)w.sb.rows_cell * w.sb.cols_cell;
     const double M = cnt[0], N = cnt[1];
     const double P = std::round(2.0 * p->radius_obj * p->ratio_augimg);
     c->st_bytes[ST_PEAKS] += na * HW * s_t + n2d * 16.0;
@@ -445,17 +464,17 @@ olpt_status ipr_loop(olpt_ctx *c, T *img, T *img_next, const int32_t *active,
 
 // IPR::runIPR on one frame already converted into img_a (T)
 template <typename T>
-olpt_status ipr_frame(olpt_ctx *c, const olpt_ipr_params *p, double *out_xyz, double *out_uv,
-                      int cap_out, int32_t *d_n_out, int32_t *d_loop_tot, int cap_loop,
-                      int *n_loop_out) {
+olpt_status ipr_frame(olpt_ctx *c, Workspace &w, const olpt_ipr_params *p, double *out_xyz,
+                      double *out_uv, int cap_out, int32_t *d_n_out, int32_t *d_loop_tot,
+                      int cap_loop, int *n_loop_out) {
   const int n_cam = c->n_cam;
-  fill_i32(d_n_out, 1, 0, c->st);
+  fill_i32(d_n_out, 1, 0, w.st);
   int n_loop = 0;
   if (n_cam < 2) { // IPR.cpp:200-204
     *n_loop_out = 0;
     return OLPT_OK;
   }
-  T *img = (T *)c->img_a, *nxt = (T *)c->img_b;
+  T *img = (T *)w.img_a, *nxt = (T *)w.img_b;
   int max_drop = std::min(p->n_cam_reduced, std::max(0, n_cam - 2));
   olpt_ipr_params q = *p;
   const int orig_mcc = p->match_cam_count;
@@ -472,7 +491,7 @@ olpt_status ipr_frame(olpt_ctx *c, const olpt_ipr_params *p, double *out_xyz, do
       double tol = q.tol_2d_px;
       for (int loop = 0; loop < loops; ++loop) {
         int32_t *lt = (d_loop_tot && n_loop < cap_loop) ? d_loop_tot + n_loop : nullptr;
-        olpt_status s = ipr_loop<T>(c, img, nxt, act, &q, tol, out_xyz, out_uv, cap_out,
+        olpt_status s = ipr_loop<T>(c, w, img, nxt, act, &q, tol, out_xyz, out_uv, cap_out,
                                     d_n_out, lt);
         if (s) return s;
         std::swap(img, nxt);
@@ -523,6 +542,7 @@ olpt_status olpt_ctx_destroy(olpt_ctx *c) {
   else
     (void)hipDeviceSynchronize();
   free_all(c);
+  free_streams(c);
   delete c;
   return OLPT_OK;
 }
@@ -542,48 +562,37 @@ olpt_status olpt_set_precision(olpt_ctx *c, int prec) {
   return OLPT_OK;
 }
 
-olpt_status olpt_ctx_reserve(olpt_ctx *c, int n_cam, int H, int W, int max_2d, int max_cand,
-                             int max_obj) {
-  if (!c) return fail(OLPT_E_INVALID_ARGUMENT, "null context");
-  if (n_cam < 1 || n_cam > OLPT_MAX_CAM || H < 3 || W < 3)
-    return fail(OLPT_E_INVALID_ARGUMENT, "bad camera count or image size");
-  if (max_cand >= (1 << 27))
-    return fail(OLPT_E_INVALID_ARGUMENT, "max_cand must be below 2^27");
-  HIP_TRY(hipSetDevice(c->device));
-  HIP_TRY(hipDeviceSynchronize());
-  free_all(c);
-  c->n_cam = n_cam;
-  c->H = H;
-  c->W = W;
-  c->cap2d = max_2d > 0 ? max_2d : std::min(1 << 17, std::max(1024, H * W / 8));
-  c->cap_cand = max_cand > 0 ? max_cand : (1 << 21);
-  c->cap_obj = max_obj > 0 ? max_obj : (1 << 18);
+} // extern "C"
+
+namespace {
+
+// All buffers of one workspace, sized from the context's reserve.
+olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
+  const int H = c->H, W = c->W;
   const long HW = (long)H * W;
-  const int nc = n_cam, cap = c->cap2d, M = c->cap_cand, O = c->cap_obj;
+  const int nc = c->n_cam, cap = c->cap2d, M = c->cap_cand, O = c->cap_obj;
   olpt_status s;
 #define A(ptr, T, n) if ((s = dalloc_t(c, &(ptr), (size_t)(n)))) return s
-  A(c->d_cam64, CamT<double>, OLPT_MAX_CAM);
-  A(c->d_cam32, CamT<float>, OLPT_MAX_CAM);
   void *tmp;
   if ((s = dalloc(c, &tmp, 8 * (size_t)nc * HW))) return s;
-  c->img_a = tmp;
+  w.img_a = tmp;
   if ((s = dalloc(c, &tmp, 8 * (size_t)nc * HW))) return s;
-  c->img_b = tmp;
+  w.img_b = tmp;
   if ((s = dalloc(c, &tmp, 8 * (size_t)nc * HW))) return s;
-  c->img_res = tmp;
+  w.img_res = tmp;
   const long nchunk = peak_chunks(H, W);
-  A(c->pk_mask, uint64_t, nchunk * nc);
-  A(c->pk_cnt, int32_t, nchunk * nc);
-  A(c->pk_off, int32_t, nchunk * nc);
-  A(c->xy, double, (long)nc * cap * 2);
-  A(c->npts, int32_t, OLPT_MAX_CAM);
-  StereoBuffers &b = c->sb;
+  A(w.pk_mask, uint64_t, nchunk * nc);
+  A(w.pk_cnt, int32_t, nchunk * nc);
+  A(w.pk_off, int32_t, nchunk * nc);
+  A(w.xy, double, (long)nc * cap * 2);
+  A(w.npts, int32_t, OLPT_MAX_CAM);
+  StereoBuffers &b = w.sb;
   b = StereoBuffers{};
   b.n_cam = nc;
   b.cap = cap;
   b.cams = c->d_cam64;
-  b.xy = c->xy;
-  b.npts = c->npts;
+  b.xy = w.xy;
+  b.npts = w.npts;
   A(b.los, double, (long)nc * cap * 4);
   A(b.lines, double, (long)nc * nc * cap * 4);
   A(b.t1, int8_t, cap);
@@ -634,48 +643,115 @@ olpt_status olpt_ctx_reserve(olpt_ctx *c, int n_cam, int H, int W, int max_2d, i
   A(b.obj_ids, int32_t, (long)O * nc);
   A(b.n_obj, int32_t, 1);
   A(b.n_sel, int32_t, 1);
-  A(c->flags, int32_t, 16);
-  b.flags = c->flags;
+  A(w.flags, int32_t, 16);
+  b.flags = w.flags;
   A(b.work, int32_t, 16);
   const long grid_cells = (long)c->grid_n * c->grid_n * c->grid_n;
   long scan_n = std::max<long>(std::max<long>(M, (long)nc * cap), std::max<long>(ncell, grid_cells));
   scan_n = std::max<long>(scan_n, nchunk);
-  A(c->scan_scratch, int32_t, scan_scratch_ints(scan_n, nc) + 64);
-  b.scan_scratch = c->scan_scratch;
-  A(c->totals, int32_t, 64);
-  b.totals = c->totals;
+  A(w.scan_scratch, int32_t, scan_scratch_ints(scan_n, nc) + 64);
+  b.scan_scratch = w.scan_scratch;
+  A(w.totals, int32_t, 64);
+  b.totals = w.totals;
   // shake
-  if ((s = dalloc(c, &c->sh_uv, 8 * (size_t)O * nc * 2))) return s;
-  if ((s = dalloc(c, &c->keep_uv, 8 * (size_t)O * nc * 2))) return s;
-  A(c->sh_xyz, double, (long)O * 3);
-  A(c->keep_xyz, double, (long)O * 3);
-  A(c->sh_score, double, O);
-  A(c->sh_flag, uint8_t, O);
-  A(c->sh_n, int32_t, 1);
-  A(c->n_keep, int32_t, 1);
-  c->bin_rows = (H + 7) / 8;
-  c->bin_cols = (W + 7) / 8;
+  if ((s = dalloc(c, &w.sh_uv, 8 * (size_t)O * nc * 2))) return s;
+  if ((s = dalloc(c, &w.keep_uv, 8 * (size_t)O * nc * 2))) return s;
+  A(w.sh_xyz, double, (long)O * 3);
+  A(w.keep_xyz, double, (long)O * 3);
+  A(w.sh_score, double, O);
+  A(w.sh_flag, uint8_t, O);
+  A(w.sh_n, int32_t, 1);
+  A(w.n_keep, int32_t, 1);
   const long nbin = (long)c->bin_rows * c->bin_cols;
-  A(c->bin_cnt, int32_t, nbin * nc);
-  A(c->bin_start, int32_t, nbin * nc);
-  A(c->bin_obj, int32_t, (long)O * nc);
-  A(c->obj_bin, int32_t, (long)O * nc);
-  A(c->obj_rank, int32_t, (long)O * nc);
-  A(c->g_cnt, int32_t, grid_cells);
-  A(c->g_start, int32_t, grid_cells);
-  A(c->g_obj, int32_t, O);
-  A(c->g_rank, int32_t, O);
-  A(c->g_cellof, int32_t, O);
-  A(c->has_nb, int32_t, O);
-  A(c->sh_keep_off, int32_t, O);
-  if ((s = dalloc(c, &c->rec, shake_rec_bytes(true) * (size_t)O * nc))) return s;
-  A(c->partial, double, 1024);
+  A(w.bin_cnt, int32_t, nbin * nc);
+  A(w.bin_start, int32_t, nbin * nc);
+  A(w.bin_obj, int32_t, (long)O * nc);
+  A(w.obj_bin, int32_t, (long)O * nc);
+  A(w.obj_rank, int32_t, (long)O * nc);
+  A(w.g_cnt, int32_t, grid_cells);
+  A(w.g_start, int32_t, grid_cells);
+  A(w.g_obj, int32_t, O);
+  A(w.g_rank, int32_t, O);
+  A(w.g_cellof, int32_t, O);
+  A(w.has_nb, int32_t, O);
+  A(w.sh_keep_off, int32_t, O);
+  if ((s = dalloc(c, &w.rec, shake_rec_bytes(true) * (size_t)O * nc))) return s;
+  A(w.partial, double, 1024);
 #undef A
   // device counters and flags start at zero (fresh allocations may hold
   // another context's old values)
-  fill_i32(c->flags, 16, 0, nullptr);
-  fill_i32(c->npts, OLPT_MAX_CAM, 0, nullptr);
+  fill_i32(w.flags, 16, 0, nullptr);
+  fill_i32(w.npts, OLPT_MAX_CAM, 0, nullptr);
+  return OLPT_OK;
+}
+
+void free_streams(olpt_ctx *c) {
+  for (int i = 1; i < kMaxWorkspaces; ++i) {
+    if (c->ws[i].st) (void)hipStreamDestroy(c->ws[i].st);
+    c->ws[i].st = nullptr;
+  }
+  for (int i = 0; i < kMaxWorkspaces; ++i) {
+    if (c->ws[i].done) (void)hipEventDestroy(c->ws[i].done);
+    c->ws[i].done = nullptr;
+  }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  c->ev_fork = nullptr;
+}
+
+// Allocate workspaces up to the requested concurrency; extra ones get a
+// stream of their own (default priority, non-blocking) and a join event.
+olpt_status ensure_workspaces(olpt_ctx *c) {
+  if (!c->cap2d) return OLPT_OK;
+  if (!c->ev_fork) HIP_TRY(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  while (c->n_ws < c->n_ws_want) {
+    Workspace &w = c->ws[c->n_ws];
+    if (c->n_ws == 0)
+      w.st = c->st;
+    else if (!w.st)
+      HIP_TRY(hipStreamCreateWithFlags(&w.st, hipStreamNonBlocking));
+    if (!w.done) HIP_TRY(hipEventCreateWithFlags(&w.done, hipEventDisableTiming));
+    olpt_status s = alloc_workspace(c, w);
+    if (s) return s;
+    ++c->n_ws;
+  }
   HIP_TRY(hipDeviceSynchronize());
+  return OLPT_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+olpt_status olpt_ctx_reserve(olpt_ctx *c, int n_cam, int H, int W, int max_2d, int max_cand,
+                             int max_obj) {
+  if (!c) return fail(OLPT_E_INVALID_ARGUMENT, "null context");
+  if (n_cam < 1 || n_cam > OLPT_MAX_CAM || H < 3 || W < 3)
+    return fail(OLPT_E_INVALID_ARGUMENT, "bad camera count or image size");
+  if (max_cand >= (1 << 27))
+    return fail(OLPT_E_INVALID_ARGUMENT, "max_cand must be below 2^27");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipDeviceSynchronize());
+  free_all(c);
+  c->n_cam = n_cam;
+  c->H = H;
+  c->W = W;
+  c->cap2d = max_2d > 0 ? max_2d : std::min(1 << 17, std::max(1024, H * W / 8));
+  c->cap_cand = max_cand > 0 ? max_cand : (1 << 21);
+  c->cap_obj = max_obj > 0 ? max_obj : (1 << 18);
+  c->bin_rows = (H + 7) / 8;
+  c->bin_cols = (W + 7) / 8;
+  olpt_status s;
+  if ((s = dalloc_t(c, &c->d_cam64, OLPT_MAX_CAM))) return s;
+  if ((s = dalloc_t(c, &c->d_cam32, OLPT_MAX_CAM))) return s;
+  for (int i = 0; i < kMaxWorkspaces; ++i) {
+    hipStream_t keep_st = c->ws[i].st;
+    hipEvent_t keep_ev = c->ws[i].done;
+    c->ws[i] = Workspace{};
+    c->ws[i].st = i == 0 ? c->st : keep_st;
+    c->ws[i].done = keep_ev;
+  }
+  c->n_ws = 0;
+  if ((s = ensure_workspaces(c))) return s;
   // re-upload cameras / OTF if they were set before
   if (c->cams_set) {
     CamT<double> h64[OLPT_MAX_CAM];
@@ -688,7 +764,22 @@ olpt_status olpt_ctx_reserve(olpt_ctx *c, int n_cam, int H, int W, int max_2d, i
     HIP_TRY(hipMemcpy(c->d_cam32, h32, sizeof(h32), hipMemcpyHostToDevice));
   }
   c->otf_set = 0;
+  c->d_otf64 = nullptr;
+  c->d_otf32 = nullptr;
+  c->otf_ng = 0;
+  c->loop_tot = nullptr;
+  c->loop_tot_cap = 0;
   return OLPT_OK;
+}
+
+olpt_status olpt_set_concurrency(olpt_ctx *c, int n_streams) {
+  if (!c) return fail(OLPT_E_INVALID_ARGUMENT, "null context");
+  if (n_streams < 1 || n_streams > kMaxWorkspaces)
+    return fail(OLPT_E_INVALID_ARGUMENT, "n_streams must be within [1, 8]");
+  if (c->capturing) return fail(OLPT_E_INVALID_ARGUMENT, "cannot change concurrency while capturing");
+  if (n_streams > c->n_ws_want) c->n_ws_want = n_streams;
+  c->n_ws_active = n_streams;
+  return ensure_workspaces(c);
 }
 
 olpt_status olpt_set_cameras(olpt_ctx *c, const olpt_camera *cams, int n_cam) {
@@ -745,32 +836,38 @@ olpt_status olpt_set_otf(olpt_ctx *c, const olpt_otf *o) {
 }
 
 olpt_status olpt_diagnostics(olpt_ctx *c, int32_t out[8]) {
-  if (!c || !c->flags) return fail(OLPT_E_INVALID_ARGUMENT, "no workspace");
-  HIP_TRY(hipStreamSynchronize(c->st));
-  HIP_TRY(hipMemcpy(out, c->flags, 8 * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (!c || !c->n_ws) return fail(OLPT_E_INVALID_ARGUMENT, "no workspace");
+  HIP_TRY(hipDeviceSynchronize());
+  for (int i = 0; i < 8; ++i) out[i] = 0;
+  for (int k = 0; k < c->n_ws; ++k) {
+    int32_t f[8];
+    HIP_TRY(hipMemcpy(f, c->ws[k].flags, sizeof(f), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 8; ++i) out[i] = (i == 3) ? out[i] + f[i] : (out[i] | f[i]);
+  }
   return OLPT_OK;
 }
 
 olpt_status olpt_find_peaks(olpt_ctx *c, const void *d_img, int dtype, int H, int W,
                             const olpt_ipr_params *p, double *d_xy, int cap, int32_t *n_out) {
-  if (!c || !c->flags) return fail(OLPT_E_INVALID_ARGUMENT, "workspace not reserved");
+  if (!c || !c->n_ws) return fail(OLPT_E_INVALID_ARGUMENT, "workspace not reserved");
   if (!d_img || !d_xy || !p || !n_out) return fail(OLPT_E_INVALID_ARGUMENT, "null argument");
+  Workspace &w = c->ws[0];
   if (H != c->H || W != c->W) return fail(OLPT_E_INVALID_ARGUMENT, "image size differs from reserve");
   CamList cl{1, {0}};
   hipStream_t st = c->st;
-  fill_i32(c->flags + 8, 8, 0, st);
-  fill_i32(c->npts, OLPT_MAX_CAM, 0, st);
+  fill_i32(w.flags + 8, 8, 0, st);
+  fill_i32(w.npts, OLPT_MAX_CAM, 0, st);
   const double mi = (double)p->min_obj_int;
   switch (dtype) {
-  case OLPT_U8: launch_find_peaks<uint8_t>((const uint8_t *)d_img, 0, H, W, mi, cl, c->pk_mask, c->pk_cnt, c->pk_off, c->scan_scratch, c->totals, d_xy, cap, c->npts, c->flags + 8, st); break;
-  case OLPT_U16: launch_find_peaks<uint16_t>((const uint16_t *)d_img, 0, H, W, mi, cl, c->pk_mask, c->pk_cnt, c->pk_off, c->scan_scratch, c->totals, d_xy, cap, c->npts, c->flags + 8, st); break;
-  case OLPT_F32: launch_find_peaks<float>((const float *)d_img, 0, H, W, mi, cl, c->pk_mask, c->pk_cnt, c->pk_off, c->scan_scratch, c->totals, d_xy, cap, c->npts, c->flags + 8, st); break;
-  case OLPT_F64: launch_find_peaks<double>((const double *)d_img, 0, H, W, mi, cl, c->pk_mask, c->pk_cnt, c->pk_off, c->scan_scratch, c->totals, d_xy, cap, c->npts, c->flags + 8, st); break;
+  case OLPT_U8: launch_find_peaks<uint8_t>((const uint8_t *)d_img, 0, H, W, mi, cl, w.pk_mask, w.pk_cnt, w.pk_off, w.scan_scratch, w.totals, d_xy, cap, w.npts, w.flags + 8, st); break;
+  case OLPT_U16: launch_find_peaks<uint16_t>((const uint16_t *)d_img, 0, H, W, mi, cl, w.pk_mask, w.pk_cnt, w.pk_off, w.scan_scratch, w.totals, d_xy, cap, w.npts, w.flags + 8, st); break;
+  case OLPT_F32: launch_find_peaks<float>((const float *)d_img, 0, H, W, mi, cl, w.pk_mask, w.pk_cnt, w.pk_off, w.scan_scratch, w.totals, d_xy, cap, w.npts, w.flags + 8, st); break;
+  case OLPT_F64: launch_find_peaks<double>((const double *)d_img, 0, H, W, mi, cl, w.pk_mask, w.pk_cnt, w.pk_off, w.scan_scratch, w.totals, d_xy, cap, w.npts, w.flags + 8, st); break;
   default: return fail(OLPT_E_UNSUPPORTED, "unsupported image dtype");
   }
   HIP_TRY(hipGetLastError());
   int32_t tot = 0;
-  HIP_TRY(hipMemcpyAsync(&tot, c->totals, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(&tot, w.totals, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   *n_out = tot;
   if (tot > cap) return fail(OLPT_E_CAPACITY, "more peaks than the output capacity");
@@ -785,52 +882,53 @@ olpt_status olpt_stereo_match(olpt_ctx *c, const int32_t *active, const double *
   if (s) return s;
   if (!d_xy || !n_pts || !p || !counts) return fail(OLPT_E_INVALID_ARGUMENT, "null argument");
   hipStream_t st = c->st;
+  Workspace &w = c->ws[0];
   for (int k = 0; k < c->n_cam; ++k)
     if (n_pts[k] < 0 || n_pts[k] > c->cap2d || n_pts[k] > cap_2d)
       return fail(OLPT_E_CAPACITY, "2D point count exceeds the workspace capacity");
-  HIP_TRY(hipMemcpy2DAsync(c->xy, sizeof(double) * 2 * c->cap2d, d_xy, sizeof(double) * 2 * cap_2d,
+  HIP_TRY(hipMemcpy2DAsync(w.xy, sizeof(double) * 2 * c->cap2d, d_xy, sizeof(double) * 2 * cap_2d,
                            sizeof(double) * 2 * std::min(cap_2d, c->cap2d), c->n_cam,
                            hipMemcpyDeviceToDevice, st));
   int32_t np[OLPT_MAX_CAM] = {0};
   for (int k = 0; k < c->n_cam; ++k) np[k] = (active && !active[k]) ? 0 : n_pts[k];
-  HIP_TRY(hipMemcpyAsync(c->npts, np, sizeof(np), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(w.npts, np, sizeof(np), hipMemcpyHostToDevice, st));
   StereoParams sp = stereo_params(c, active, p, p->tol_2d_px, &s);
   if (s) return s;
-  fill_i32(c->flags, 16, 0, st);
-  launch_stereo(c->sb, sp, kMaxRounds, st);
+  fill_i32(w.flags, 16, 0, st);
+  launch_stereo(w.sb, sp, kMaxRounds, st);
   HIP_TRY(hipGetLastError());
   int32_t h[3];
-  HIP_TRY(hipMemcpyAsync(&h[0], c->sb.n_cand, 4, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(&h[1], c->sb.n_sel, 4, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(&h[2], c->sb.n_obj, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(&h[0], w.sb.n_cand, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(&h[1], w.sb.n_sel, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(&h[2], w.sb.n_obj, 4, hipMemcpyDeviceToHost, st));
   int32_t fl[8];
-  HIP_TRY(hipMemcpyAsync(fl, c->flags, sizeof(fl), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(fl, w.flags, sizeof(fl), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   counts[0] = h[0];
   counts[1] = h[1];
   counts[2] = h[2];
   if (fl[0] || h[0] > c->cap_cand) return fail(OLPT_E_CAPACITY, "stereo candidate capacity exceeded");
-  if (fl[1]) return fail(OLPT_E_CAPACITY, "stereo LDS candidate list overflow");
+  if (fl[1]) return fail(OLPT_E_CAPACITY, "stereo build list capacity exceeded");
   if (fl[4]) return fail(OLPT_E_CAPACITY, "stereo object capacity exceeded");
   if (fl[5]) return fail(OLPT_E_CAPACITY, "prune rounds exhausted");
   const int M = h[0];
   if (d_cand_ids && d_cand_e) {
     if (M > cap_cand) return fail(OLPT_E_CAPACITY, "candidate output capacity too small");
-    HIP_TRY(hipMemcpyAsync(d_cand_ids, c->sb.cand_ids_s, sizeof(int32_t) * (size_t)M * c->n_cam,
+    HIP_TRY(hipMemcpyAsync(d_cand_ids, w.sb.cand_ids_s, sizeof(int32_t) * (size_t)M * c->n_cam,
                            hipMemcpyDeviceToDevice, st));
-    HIP_TRY(hipMemcpyAsync(d_cand_e, c->sb.cand_e_s, sizeof(double) * (size_t)M,
+    HIP_TRY(hipMemcpyAsync(d_cand_e, w.sb.cand_e_s, sizeof(double) * (size_t)M,
                            hipMemcpyDeviceToDevice, st));
   }
   if (d_sel) {
     if (M > cap_cand) return fail(OLPT_E_CAPACITY, "candidate output capacity too small");
-    k_sel_flags<<<256, 256, 0, st>>>(c->sb.state, c->sb.n_cand, c->cap_cand, d_sel);
+    k_sel_flags<<<256, 256, 0, st>>>(w.sb.state, w.sb.n_cand, c->cap_cand, d_sel);
   }
   if (d_obj_xyz) {
     if (h[2] > cap_obj) return fail(OLPT_E_CAPACITY, "object output capacity too small");
-    HIP_TRY(hipMemcpyAsync(d_obj_xyz, c->sb.obj_xyz, sizeof(double) * 3 * (size_t)h[2],
+    HIP_TRY(hipMemcpyAsync(d_obj_xyz, w.sb.obj_xyz, sizeof(double) * 3 * (size_t)h[2],
                            hipMemcpyDeviceToDevice, st));
     if (d_obj_ids)
-      HIP_TRY(hipMemcpyAsync(d_obj_ids, c->sb.obj_ids, sizeof(int32_t) * (size_t)h[2] * c->n_cam,
+      HIP_TRY(hipMemcpyAsync(d_obj_ids, w.sb.obj_ids, sizeof(int32_t) * (size_t)h[2] * c->n_cam,
                              hipMemcpyDeviceToDevice, st));
   }
   HIP_TRY(hipStreamSynchronize(st));
@@ -845,14 +943,15 @@ olpt_status shake_impl(olpt_ctx *c, const int32_t *active, const void *d_imgs, i
                        const olpt_ipr_params *p, const double *d_xyz_in, int n_obj,
                        double *d_xyz_out, uint8_t *d_flags, double *d_score, void *d_res) {
   hipStream_t st = c->st;
+  Workspace &w = c->ws[0];
   const long n = (long)c->n_cam * c->H * c->W;
-  olpt_status s = convert_image<T>(d_imgs, dtype, n, (T *)c->img_a, st);
+  olpt_status s = convert_image<T>(d_imgs, dtype, n, (T *)w.img_a, st);
   if (s) return s;
-  HIP_TRY(hipMemcpyAsync(c->sh_xyz, d_xyz_in, sizeof(double) * 3 * (size_t)n_obj,
+  HIP_TRY(hipMemcpyAsync(w.sh_xyz, d_xyz_in, sizeof(double) * 3 * (size_t)n_obj,
                          hipMemcpyDeviceToDevice, st));
   int32_t nn = n_obj;
-  HIP_TRY(hipMemcpyAsync(c->sh_n, &nn, 4, hipMemcpyHostToDevice, st));
-  ShakeBuffers<T> b = shake_buffers<T>(c, (T *)c->img_a, (T *)c->img_b);
+  HIP_TRY(hipMemcpyAsync(w.sh_n, &nn, 4, hipMemcpyHostToDevice, st));
+  ShakeBuffers<T> b = shake_buffers<T>(c, w, (T *)w.img_a, (T *)w.img_b);
   if constexpr (sizeof(T) == 8)
     set_grid(c, nullptr, &b, p);
   else
@@ -860,18 +959,18 @@ olpt_status shake_impl(olpt_ctx *c, const int32_t *active, const void *d_imgs, i
   ShakeParams shp = shake_params(c, active, p);
   launch_shake<T>(b, shp, st);
   if (d_res) {
-    launch_filter<T>(b, c->keep_xyz, (T *)c->keep_uv, c->n_keep, st);
-    launch_residual_ipr<T>(b, c->n_keep, c->keep_xyz, (T *)c->keep_uv, p->radius_obj, st);
-    HIP_TRY(hipMemcpyAsync(d_res, c->img_b, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+    launch_filter<T>(b, w.keep_xyz, (T *)w.keep_uv, w.n_keep, st);
+    launch_residual_ipr<T>(b, w.n_keep, w.keep_xyz, (T *)w.keep_uv, p->radius_obj, st);
+    HIP_TRY(hipMemcpyAsync(d_res, w.img_b, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
   }
   HIP_TRY(hipGetLastError());
   if (d_xyz_out)
-    HIP_TRY(hipMemcpyAsync(d_xyz_out, c->sh_xyz, sizeof(double) * 3 * (size_t)n_obj,
+    HIP_TRY(hipMemcpyAsync(d_xyz_out, w.sh_xyz, sizeof(double) * 3 * (size_t)n_obj,
                            hipMemcpyDeviceToDevice, st));
   if (d_flags)
-    HIP_TRY(hipMemcpyAsync(d_flags, c->sh_flag, (size_t)n_obj, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_flags, w.sh_flag, (size_t)n_obj, hipMemcpyDeviceToDevice, st));
   if (d_score)
-    HIP_TRY(hipMemcpyAsync(d_score, c->sh_score, sizeof(double) * (size_t)n_obj,
+    HIP_TRY(hipMemcpyAsync(d_score, w.sh_score, sizeof(double) * (size_t)n_obj,
                            hipMemcpyDeviceToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));
   return OLPT_OK;
@@ -881,16 +980,17 @@ template <typename T>
 olpt_status residual_impl(olpt_ctx *c, const void *d_imgs, int dtype, const olpt_ipr_params *p,
                           const double *d_xyz, int n_obj, void *d_res) {
   hipStream_t st = c->st;
+  Workspace &w = c->ws[0];
   const long n = (long)c->n_cam * c->H * c->W;
-  olpt_status s = convert_image<T>(d_imgs, dtype, n, (T *)c->img_a, st);
+  olpt_status s = convert_image<T>(d_imgs, dtype, n, (T *)w.img_a, st);
   if (s) return s;
-  HIP_TRY(hipMemcpyAsync(c->keep_xyz, d_xyz, sizeof(double) * 3 * (size_t)n_obj,
+  HIP_TRY(hipMemcpyAsync(w.keep_xyz, d_xyz, sizeof(double) * 3 * (size_t)n_obj,
                          hipMemcpyDeviceToDevice, st));
   int32_t nn = n_obj;
-  HIP_TRY(hipMemcpyAsync(c->n_keep, &nn, 4, hipMemcpyHostToDevice, st));
-  ShakeBuffers<T> b = shake_buffers<T>(c, (T *)c->img_a, (T *)c->img_b);
-  launch_residual_ipr<T>(b, c->n_keep, c->keep_xyz, (T *)c->keep_uv, p->radius_obj, st);
-  HIP_TRY(hipMemcpyAsync(d_res, c->img_b, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(w.n_keep, &nn, 4, hipMemcpyHostToDevice, st));
+  ShakeBuffers<T> b = shake_buffers<T>(c, w, (T *)w.img_a, (T *)w.img_b);
+  launch_residual_ipr<T>(b, w.n_keep, w.keep_xyz, (T *)w.keep_uv, p->radius_obj, st);
+  HIP_TRY(hipMemcpyAsync(d_res, w.img_b, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));
   return OLPT_OK;
 }
@@ -911,29 +1011,48 @@ olpt_status ipr_run_impl(olpt_ctx *c, const void *d_imgs, int dtype, int n_frame
       c->loop_tot_cap = need;
     }
   }
-  fill_i32(c->flags, 16, 0, st);
+  // frames round-robin over the workspaces; the other streams start after
+  // the work already queued on the context stream and join back at the end
+  const int nw = std::max(1, std::min({c->n_ws_active, c->n_ws, n_frames}));
+  if (nw > 1) {
+    HIP_TRY(hipEventRecord(c->ev_fork, st));
+    for (int k = 1; k < nw; ++k) HIP_TRY(hipStreamWaitEvent(c->ws[k].st, c->ev_fork, 0));
+  }
+  for (int k = 0; k < nw; ++k) fill_i32(c->ws[k].flags, 16, 0, c->ws[k].st);
   int n_loop = 0;
   for (int f = 0; f < n_frames; ++f) {
+    Workspace &w = c->ws[f % nw];
     const char *src = (const char *)d_imgs + (size_t)f * n * in_bytes;
-    olpt_status s = convert_image<T>(src, dtype, n, (T *)c->img_a, st);
+    olpt_status s = convert_image<T>(src, dtype, n, (T *)w.img_a, w.st);
     if (s) return s;
-    s = ipr_frame<T>(c, p, d_obj_xyz + (size_t)f * cap * 3,
+    s = ipr_frame<T>(c, w, p, d_obj_xyz + (size_t)f * cap * 3,
                      d_obj_uv ? d_obj_uv + (size_t)f * cap * c->n_cam * 2 : nullptr, cap,
                      d_n_obj + f, loop_total ? c->loop_tot + (size_t)f * cap_loop : nullptr,
                      cap_loop, &n_loop);
     if (s) return s;
   }
+  if (nw > 1) {
+    for (int k = 1; k < nw; ++k) {
+      HIP_TRY(hipEventRecord(c->ws[k].done, c->ws[k].st));
+      HIP_TRY(hipStreamWaitEvent(st, c->ws[k].done, 0));
+    }
+  }
   if (!sync) return OLPT_OK;
-  int32_t fl[8];
-  HIP_TRY(hipMemcpyAsync(fl, c->flags, sizeof(fl), hipMemcpyDeviceToHost, st));
+  int32_t fl[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  std::vector<int32_t> fw(8 * nw);
+  for (int k = 0; k < nw; ++k)
+    HIP_TRY(hipMemcpyAsync(fw.data() + 8 * k, c->ws[k].flags, 8 * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, st));
   if (loop_total) {
     HIP_TRY(hipMemcpyAsync(loop_total, c->loop_tot, sizeof(int32_t) * (size_t)n_frames * cap_loop,
                            hipMemcpyDeviceToHost, st));
   }
   HIP_TRY(hipStreamSynchronize(st));
+  for (int k = 0; k < nw; ++k)
+    for (int i = 0; i < 8; ++i) fl[i] |= fw[8 * k + i];
   if (fl[7]) return fail(OLPT_E_CAPACITY, "2D peak capacity exceeded");
   if (fl[0]) return fail(OLPT_E_CAPACITY, "stereo candidate capacity exceeded");
-  if (fl[1]) return fail(OLPT_E_CAPACITY, "stereo LDS candidate list overflow");
+  if (fl[1]) return fail(OLPT_E_CAPACITY, "stereo build list capacity exceeded");
   if (fl[4]) return fail(OLPT_E_CAPACITY, "object capacity exceeded");
   if (fl[5]) return fail(OLPT_E_CAPACITY, "prune rounds exhausted");
   (void)n_loop;

@@ -84,16 +84,22 @@ def test_reduced_subsets_match_reference():
     ctx.close()
 
 
-def test_batched_frames_equal_single_frames(case):
+@pytest.mark.parametrize("streams", [1, 3])
+def test_batched_frames_equal_single_frames(case, streams):
+    """Frames of one call, on one stream or spread over several workspaces
+    and streams, give the single-frame results."""
     from pyopenlpt_amd.context import to_device_image
     cases = [Case(n) for n in ("golden_u8_256", "golden_reduced_256")]
     cams_ok = all(np.array_equal(cases[0].z["cameras"], c.z["cameras"]) for c in cases)
     if not cams_ok:
         cases = [case, case]
+    cases = cases + cases[::-1] + cases[:1]          # 5 frames: uneven over 3 streams
     imgs = np.stack([c.images for c in cases])
     ctx = make_ctx(case, "f64")
+    ctx.set_concurrency(streams)
     d, code = to_device_image(imgs)
-    (ox, _u, n_obj), _lt = ctx.ipr_run(d, code, case.params, n_frames=2, cap_obj=1 << 14)
+    (ox, _u, n_obj), _lt = ctx.ipr_run(d, code, case.params, n_frames=len(cases),
+                                       cap_obj=1 << 14)
     for f, c in enumerate(cases):
         d1, _ = to_device_image(c.images)
         (o1, _u1, n1), _ = ctx.ipr_run(d1, code, case.params, cap_obj=1 << 14)
@@ -126,5 +132,36 @@ def test_graph_replays_match_eager(case):
             assert torch.equal(outs_g[0][0, :n], ref_x), rep
             # eager work between replays
             ctx.ipr_run(d, code, case.params, cap_obj=1 << 14, outputs=outs_e)
+            s.synchronize()
+    ctx.close()
+
+
+def test_graph_with_streams_matches_eager(case):
+    """A captured multi-frame run that forks over two streams replays to the
+    eager result, with eager work between replays."""
+    from pyopenlpt_amd.context import to_device_image
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        imgs = np.stack([case.images, case.images[:, ::-1, :].copy(), case.images])
+        d, code = to_device_image(imgs)
+        ctx = make_ctx(case, "f32", stream=s)
+        ctx.set_concurrency(2)
+        outs_e = ctx.alloc_outputs(3, 1 << 14, d.device)
+        ctx.ipr_run(d, code, case.params, n_frames=3, cap_obj=1 << 14, outputs=outs_e)
+        s.synchronize()
+        ref_n = outs_e[2].clone()
+        ref_x = outs_e[0].clone()
+        outs_g = ctx.alloc_outputs(3, 1 << 14, d.device)
+        ctx.graph_capture(d, code, case.params, 3, outs_g, 1 << 14)
+        for rep in range(3):
+            outs_g[0].zero_()
+            outs_g[2].zero_()
+            ctx.graph_launch()
+            s.synchronize()
+            assert torch.equal(outs_g[2], ref_n), rep
+            for f in range(3):
+                n = int(ref_n[f].item())
+                assert torch.equal(outs_g[0][f, :n], ref_x[f, :n]), (rep, f)
+            ctx.ipr_run(d, code, case.params, n_frames=3, cap_obj=1 << 14, outputs=outs_e)
             s.synchronize()
     ctx.close()
