@@ -216,13 +216,15 @@ __global__ void k_copy_cam(const T *__restrict__ src, T *__restrict__ dst, long 
 // --------------------------------------------------------------------------
 // k_shake: wave per object, lanes split by camera
 // --------------------------------------------------------------------------
-// Lane group k (kWave / NC lanes) owns camera k: its projection, OTF
-// parameters and SLOTS pixels of the camera's augmented ROI.  A residual
-// evaluation is one pass over all cameras at once; per-camera sums are
-// butterfly reductions inside the group (bitwise identical in its lanes),
-// and the cross-camera mean is summed in camera order from the group
-// leaders, so every lane holds the same value and the branchy per-axis
-// search of shakeOneObject stays wave-uniform.
+// A wave holds OPW objects; each object's lanes split into NC groups and
+// group k owns camera k: its projection, OTF parameters and SLOTS pixels of
+// the camera's augmented ROI.  A residual evaluation is one pass over all
+// cameras at once; per-camera sums are butterfly reductions inside the
+// group (bitwise identical in its lanes), and the cross-camera mean is
+// summed in camera order from the group leaders, so every lane of an object
+// holds the same value and the branchy per-axis search of shakeOneObject
+// stays uniform per object.  Several objects per wave share the per-camera
+// projection and OTF work of a lane over more pixels.
 template <int LPG, typename T> __device__ __forceinline__ T group_sum(T v) {
 #pragma unroll
   for (int off = LPG / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
@@ -294,6 +296,8 @@ __device__ void polyfit2(const double *x, const double *y, int n, double c[3]) {
 
 // Per-lane state of one object in one camera (the lane's group camera).
 template <typename T, int SLOTS> struct CamLane {
+  int obase;       // first lane of this lane's object
+  uint64_t omask;  // the object's lanes
   int k;           // camera of this lane's group
   bool cam_ok;     // k < n_cam
   bool act;        // camera active
@@ -304,10 +308,10 @@ template <typename T, int SLOTS> struct CamLane {
 
 // calShakeResidue (Shake.cpp:968-1045) at Xe; (ue, ve) receive the lane
 // camera's projection.  Wave-uniform result.
-template <typename T, int NC, int SLOTS>
+template <typename T, int NC, int OPW, int SLOTS>
 __device__ __forceinline__ T eval_residue(const ShakeBuffers<T> &b, const CamLane<T, SLOTS> &c,
                                           T r_px, const T Xe[3], T &ue, T &ve) {
-  constexpr int LPG = kWave / NC;
+  constexpr int LPG = kWave / OPW / NC;
   const int sub = lane_id() % LPG;
   T sse = T(0);
   ue = T(0);
@@ -335,12 +339,12 @@ __device__ __forceinline__ T eval_residue(const ShakeBuffers<T> &b, const CamLan
   }
   sse = group_sum<LPG>(sse);
   const T val = use ? sse / T(c.npix) : T(0);
-  const uint64_t um = ballot(use && sub == 0);
+  const uint64_t um = ballot(use && sub == 0) >> c.obase;
   T acc = T(0);
   int used = 0;
 #pragma unroll
   for (int kk = 0; kk < NC; ++kk) {
-    const T vk = __shfl(val, kk * LPG, kWave);
+    const T vk = __shfl(val, c.obase + kk * LPG, kWave);
     if ((um >> (kk * LPG)) & 1ull) {
       acc += vk;
       ++used;
@@ -351,7 +355,7 @@ __device__ __forceinline__ T eval_residue(const ShakeBuffers<T> &b, const CamLan
 
 // Whether the projection of Xt falls outside the ROI of any camera
 // (shakeOneObject's pre-evaluation check; every camera takes part).
-template <typename T, int NC, int SLOTS>
+template <typename T, int SLOTS>
 __device__ __forceinline__ bool any_out(const ShakeBuffers<T> &b, const CamLane<T, SLOTS> &c,
                                         const T Xt[3]) {
   bool out = false;
@@ -360,14 +364,15 @@ __device__ __forceinline__ bool any_out(const ShakeBuffers<T> &b, const CamLane<
     project(b.cams[c.k], Xt, u, v);
     out = !in_roi_d(c.roi, u, v);
   }
-  return __any(out);
+  return (ballot(out) & c.omask) != 0;
 }
 
-template <typename T, int NC, int SLOTS>
+template <typename T, int NC, int OPW, int SLOTS>
 __global__ void __launch_bounds__(kWave)
 k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
         T delta) {
-  constexpr int LPG = kWave / NC;
+  constexpr int LPO = kWave / OPW;   // lanes per object
+  constexpr int LPG = LPO / NC;      // lanes per camera
   const int n = *b.n_obj;
   const int nc = b.n_cam;
   const int lane = lane_id();
@@ -376,11 +381,13 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
   const T r_px = (T)p.r_px;
   const T dpos = r_px > T(0) ? r_px : T(0);
   CamLane<T, SLOTS> c;
-  c.k = lane / LPG;
+  c.obase = (lane / LPO) * LPO;
+  c.omask = (LPO == kWave ? ~0ull : ((1ull << LPO) - 1)) << c.obase;
+  c.k = (lane % LPO) / LPG;
   c.cam_ok = c.k < nc;
   c.act = c.cam_ok && p.active[c.k];
   const int ksafe = c.cam_ok ? c.k : 0;
-  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+  for (int i = blockIdx.x * OPW + lane / LPO; i < n; i += gridDim.x * OPW) {
     T X[3] = {(T)b.xyz[3L * i], (T)b.xyz[3L * i + 1], (T)b.xyz[3L * i + 2]};
     // buildROIInfo (:279-328) with the object's current projection
     c.roi = PRange{0, 0, 0, 0};
@@ -422,7 +429,7 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         tmp[ax] = x0 + rel[j];
-        f[j] = eval_residue<T, NC, SLOTS>(b, c, r_px, tmp, ue[j], ve[j]);
+        f[j] = eval_residue<T, NC, OPW, SLOTS>(b, c, r_px, tmp, ue[j], ve[j]);
       }
       const T fL = f[0], f0 = f[1], fR = f[2];
       const T tol = T(1e-3) * fmax(T(1), f0);
@@ -458,14 +465,14 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
           const T mv = fmin(ve[1] - T(c.roi.rmin), T(c.roi.rmax) - T(1) - ve[1]);
           mk = fmin(mu, mv);
         }
-        const uint64_t mJ = ballot(c.cam_ok && sub == 0 && ok0 && ok2);
-        const uint64_t mM = ballot(c.cam_ok && sub == 0 && ok1);
+        const uint64_t mJ = ballot(c.cam_ok && sub == 0 && ok0 && ok2) >> c.obase;
+        const uint64_t mM = ballot(c.cam_ok && sub == 0 && ok1) >> c.obase;
         T Jv[NC], mg[NC];
         int nJ = 0, nm = 0;
 #pragma unroll
         for (int kk = 0; kk < NC; ++kk) {
-          const T jv = __shfl(Jk, kk * LPG, kWave);
-          const T mv = __shfl(mk, kk * LPG, kWave);
+          const T jv = __shfl(Jk, c.obase + kk * LPG, kWave);
+          const T mv = __shfl(mk, c.obase + kk * LPG, kWave);
           if ((mJ >> (kk * LPG)) & 1ull) Jv[nJ++] = jv;
           if ((mM >> (kk * LPG)) & 1ull) mg[nm++] = mv;
         }
@@ -488,10 +495,10 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
         if (Dp > T(0)) {
           const T dxe = sgn >= 0 ? Dp : -Dp;
           tmp[ax] = x0 + dxe;
-          if (!any_out<T, NC, SLOTS>(b, c, tmp)) {
+          if (!any_out<T, SLOTS>(b, c, tmp)) {
             T uu, vv;
             cx[ncand] = dxe;
-            cf[ncand] = eval_residue<T, NC, SLOTS>(b, c, r_px, tmp, uu, vv);
+            cf[ncand] = eval_residue<T, NC, OPW, SLOTS>(b, c, r_px, tmp, uu, vv);
             cabs[ncand] = x0 + dxe;
             ++ncand;
           }
@@ -512,10 +519,10 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
         }
         if (xh >= xmin && xh <= xmax) {
           tmp[ax] = x0 + xh;
-          if (!any_out<T, NC, SLOTS>(b, c, tmp)) {
+          if (!any_out<T, SLOTS>(b, c, tmp)) {
             T uu, vv;
             cx[ncand] = xh;
-            cf[ncand] = eval_residue<T, NC, SLOTS>(b, c, r_px, tmp, uu, vv);
+            cf[ncand] = eval_residue<T, NC, OPW, SLOTS>(b, c, r_px, tmp, uu, vv);
             cabs[ncand] = x0 + xh;
             ++ncand;
           }
@@ -544,7 +551,7 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
     }
     const int rw = usek ? rect.cmax - rect.cmin : 0;
     const int np = usek ? (rect.rmax - rect.rmin) * rw : 0;
-    const int np_all = wave_max(np);
+    const int np_all = group_max<LPO>(np);
     T pk = T(0), nm = T(0), dn = T(0);
     for (int base = 0; base < np_all; base += LPG) {
       const int idx = base + sub;
@@ -559,7 +566,7 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
       const bool inr = valid && ro.rmax - ro.rmin > 0 && ro.cmax - ro.cmin > 0 &&
                        row >= ro.rmin && row < ro.rmax && col >= ro.cmin && col < ro.cmax;
       const int aidx = inr ? (row - ro.rmin) * c.roiw + (col - ro.cmin) : 0;
-      const int src = (lane / LPG) * LPG + aidx % LPG;
+      const int src = (lane / LPG) * LPG + aidx % LPG;   // same camera group
       T av = T(0);
 #pragma unroll
       for (int sl = 0; sl < SLOTS; ++sl) {
@@ -579,15 +586,15 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
     pk = group_max<LPG>(pk);
     nm = group_sum<LPG>(nm);
     dn = group_sum<LPG>(dn);
-    const uint64_t mu = ballot(usek && sub == 0);
+    const uint64_t mu = ballot(usek && sub == 0) >> c.obase;
     T peak[NC], num[NC], den[NC];
     int use[NC];
     int n_used = 0;
 #pragma unroll
     for (int kk = 0; kk < NC; ++kk) {
-      peak[kk] = __shfl(pk, kk * LPG, kWave);
-      num[kk] = __shfl(nm, kk * LPG, kWave);
-      den[kk] = __shfl(dn, kk * LPG, kWave);
+      peak[kk] = __shfl(pk, c.obase + kk * LPG, kWave);
+      num[kk] = __shfl(nm, c.obase + kk * LPG, kWave);
+      den[kk] = __shfl(dn, c.obase + kk * LPG, kWave);
       use[kk] = (int)((mu >> (kk * LPG)) & 1ull);
       n_used += use[kk];
     }
@@ -619,7 +626,7 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
       ratio = fmax(T(2) / T(3), fmin(T(1.5), ratio));
       score = sqrt(ratio);
     }
-    if (lane == 0) {
+    if (lane == c.obase) {
       b.xyz[3L * i] = (double)X[0];
       b.xyz[3L * i + 1] = (double)X[1];
       b.xyz[3L * i + 2] = (double)X[2];
@@ -799,13 +806,18 @@ __global__ void k_zero_score(ShakeBuffers<T> b) {
     b.score[i] = 0.0;
 }
 
-template <typename T, int NC, int SLOTS>
+template <typename T, int NC, int OPW, int SLOTS>
 static void shake_iter(const ShakeBuffers<T> &b, const ShakeParams &p,
                        const ObjCamRec<T> *rec, T delta, hipStream_t st) {
   stage_begin(ST_SHAKE, st);
-  k_shake<T, NC, SLOTS><<<256 * 16, kWave, 0, st>>>(b, p, rec, delta);
+  k_shake<T, NC, OPW, SLOTS><<<256 * 16, kWave, 0, st>>>(b, p, rec, delta);
   stage_end(ST_SHAKE, st);
 }
+
+// Objects per wave for up to 4 cameras and an 8x8 ROI: 4 in FP32 (measured
+// fastest; the per-camera projection and OTF work covers 16 pixels per
+// lane), 2 in FP64, whose doubled registers would otherwise spill.
+template <typename T> constexpr int shake_opw() { return sizeof(T) == 4 ? 4 : 2; }
 
 template <typename T>
 void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st) {
@@ -833,13 +845,13 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
     if (nc < 2) {
       k_zero_score<T><<<gO, 256, 0, st>>>(b);
     } else if (nc <= 4 && roi_px <= 64) {
-      shake_iter<T, 4, 4>(b, p, rec, (T)delta, st);
+      shake_iter<T, 4, shake_opw<T>(), 4 * shake_opw<T>()>(b, p, rec, (T)delta, st);
     } else if (nc <= 4) {
-      shake_iter<T, 4, 16>(b, p, rec, (T)delta, st);
+      shake_iter<T, 4, 1, 16>(b, p, rec, (T)delta, st);
     } else if (roi_px <= 64) {
-      shake_iter<T, 8, 8>(b, p, rec, (T)delta, st);
+      shake_iter<T, 8, 1, 8>(b, p, rec, (T)delta, st);
     } else {
-      shake_iter<T, 8, 32>(b, p, rec, (T)delta, st);
+      shake_iter<T, 8, 1, 32>(b, p, rec, (T)delta, st);
     }
     delta *= 0.5;
     if (delta < dmin) delta = dmin;
