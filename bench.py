@@ -95,6 +95,18 @@ def cpu_baseline(ppp: float, size: int, threads: int) -> dict | None:
                        f"{res['seconds']:.2f} s wall, IPR::runIPR only)")}
 
 
+def traffic_per_launch(workload: str):
+    """HBM bytes per k_shake launch from profiles/k_shake__<workload>.pmc.md
+    (written by `python -m pyopenlpt_amd.prof pmc`), or None."""
+    path = os.path.join(ROOT, "profiles", f"k_shake__{workload}.pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.strip().startswith("traffic_bytes_per_dispatch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +120,11 @@ def main() -> int:
     ap.add_argument("--streams", type=int, default=4,
                     help="frames of a step run concurrently on this many HIP streams")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--roofline-steps", type=int, default=1,
+                    help="single-stream steps timed per k_shake launch for the roofline")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="after the run, an elementwise kernel that reads and writes this many "
+                         "MiB (counter calibration under rocprofv3 --pmc)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
 
@@ -122,6 +139,7 @@ def main() -> int:
     from pyopenlpt_amd import synth as S
     from pyopenlpt_amd.context import Context, to_device_image
 
+    workload = f"ipr_ppp{args.ppp}_{args.size}sq_u16_4cam_clustered"
     log(f"generating {args.frames} frames (ppp {args.ppp}, {args.size}^2)")
     frames = make_frames(args.frames, args.ppp, args.size)
     w0 = frames[0]
@@ -153,8 +171,6 @@ def main() -> int:
     ctx.ipr_run(dimg, code, w0.params, n_frames=args.frames, cap_obj=cap_obj, outputs=outs)
     stats = ctx.stage_stats(False)
 
-    ctx.stage_timing(not args.graph)
-    ctx.stage_times()
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -166,13 +182,30 @@ def main() -> int:
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_total = e0.elapsed_time(e1)
-    times = ctx.stage_times() if not args.graph else {}
     ms_per_step = ms_total / args.steps
     n_obj = outs[2].cpu().numpy()
     per_step = int(n_obj.sum())
     value = per_step / (ms_per_step / 1000.0)
     log(f"{args.steps} steps: {ms_per_step:.2f} ms/step (wall {1000 * wall / args.steps:.2f}), "
         f"{per_step} particles/step -> {value:.0f} particles/s")
+
+    # Roofline pass: the kernel's own launch time.  Concurrent streams share
+    # the GPU, so per-launch event intervals of the throughput run include
+    # waiting on the other streams; this timed region runs the same step on
+    # one stream with HIP events around every k_shake launch.
+    times = {}
+    if not args.graph and args.roofline_steps > 0:
+        ctx.set_concurrency(1)
+        ctx.stage_timing(True)
+        ctx.stage_times()
+        torch.cuda.synchronize()
+        for _ in range(args.roofline_steps):
+            ctx.ipr_run(dimg, code, w0.params, n_frames=args.frames, cap_obj=cap_obj,
+                        outputs=outs)
+        torch.cuda.synchronize()
+        times = ctx.stage_times()
+        ctx.stage_timing(False)
+        ctx.set_concurrency(args.streams)
 
     # quality over the batch
     q = [quality(outs[0][f, : n_obj[f]].cpu().numpy(), frames[f].truth, frames[f].voxel_mm)
@@ -183,22 +216,37 @@ def main() -> int:
                 recon_rate=float(np.mean([x["recon_rate"] for x in q])),
                 ghost_rate=float(np.mean([x["ghost_rate"] for x in q])))
 
+    # Roofline of the dominant kernel: k_shake (the "shake" stage brackets
+    # exactly its launches).  Algorithmic bytes per launch: the per-object
+    # ROI patch gathers N x C x P^2 x s (P = 8, s = 4 bytes in FP32) plus
+    # positions; HBM traffic per launch from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE summary of this workload, when present.
     roofline = None
-    if times:
-        dom = max((k for k in times if times[k][1] > 0), key=lambda k: times[k][0])
-        ms_dom, calls = times[dom]
-        bytes_dom, launches = stats[dom]
-        if calls and launches:
-            per_launch_bytes = bytes_dom / launches
-            avg_ms = ms_dom / calls
-            achieved = per_launch_bytes / (avg_ms / 1000.0) / 1e9
-            roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3),
-                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-                        "bytes_per_launch": int(per_launch_bytes),
-                        "avg_launch_ms": round(avg_ms, 4),
-                        "stage_ms_per_step": {k: round(v[0] / args.steps, 3)
-                                              for k, v in times.items() if v[1]}}
+    if times and times.get("shake", (0, 0))[1] and stats.get("shake", (0, 0))[1]:
+        ms_dom, calls = times["shake"]
+        bytes_dom, launches = stats["shake"]
+        per_launch_bytes = bytes_dom / launches
+        avg_ms = ms_dom / calls
+        achieved = per_launch_bytes / (avg_ms / 1000.0) / 1e9
+        roofline = {"bound": "hbm", "kernel": "k_shake", "achieved": round(achieved, 3),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 6),
+                    "traffic": traffic_per_launch(workload),
+                    "bytes_per_launch": int(per_launch_bytes),
+                    "avg_launch_ms": round(avg_ms, 4),
+                    "measured": f"{args.roofline_steps} single-stream step(s), HIP events "
+                                "around each k_shake launch",
+                    "stage_ms_per_step_1stream": {k: round(v[0] / args.roofline_steps, 3)
+                                                  for k, v in times.items() if v[1]}}
+
+    if args.copy_calib_mib > 0:
+        nel = args.copy_calib_mib * (1 << 20) // 4
+        src = torch.ones(nel, dtype=torch.float32, device=dimg.device)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        log(f"calibration: 3 x neg over {args.copy_calib_mib} MiB")
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "particles/s",
@@ -206,7 +254,7 @@ def main() -> int:
         "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": args.precision,
         "data": "synthetic",
-        "config": {"workload": f"ipr_ppp{args.ppp}_{args.size}sq_u16_4cam_clustered",
+        "config": {"workload": workload,
                    "frames_per_step": args.frames, "particles_per_frame": len(w0.truth),
                    "ipr_loops": int(w0.params.n_loop_ipr),
                    "shake_loops": int(w0.params.n_shake_loop),

@@ -132,7 +132,7 @@ void timer_free(olpt_ctx *c) {
   if (!c->timer) return;
   for (int a = 0; a < 2; ++a)
     for (int s = 0; s < 8; ++s)
-      for (int i = 0; i < 512; ++i) (void)hipEventDestroy(c->timer->ev[a][s][i]);
+      for (int i = 0; i < kStageEvents; ++i) (void)hipEventDestroy(c->timer->ev[a][s][i]);
   delete c->timer;
   c->timer = nullptr;
 }
@@ -1144,7 +1144,7 @@ olpt_status olpt_stage_timing(olpt_ctx *c, int enable) {
     c->timer = new StageTimer();
     for (int a = 0; a < 2; ++a)
       for (int s = 0; s < 8; ++s)
-        for (int i = 0; i < 512; ++i) HIP_TRY(hipEventCreate(&c->timer->ev[a][s][i]));
+        for (int i = 0; i < kStageEvents; ++i) HIP_TRY(hipEventCreate(&c->timer->ev[a][s][i]));
   }
   if (c->timer) {
     c->timer->on = enable != 0;

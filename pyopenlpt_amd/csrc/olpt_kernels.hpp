@@ -13,15 +13,16 @@ namespace olpt {
 
 // Optional stage timing with HIP events on the launch stream (eager runs
 // only).  Slots: see olpt_c.h (OLPT_STAGE_*).
+constexpr int kStageEvents = 2048;  // timed occurrences per stage
 struct StageTimer {
   bool on = false;
-  hipEvent_t ev[2][8][512];
+  hipEvent_t ev[2][8][kStageEvents];
   int n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   void begin(int slot, hipStream_t st) {
-    if (on && n[slot] < 512) (void)hipEventRecord(ev[0][slot][n[slot]], st);
+    if (on && n[slot] < kStageEvents) (void)hipEventRecord(ev[0][slot][n[slot]], st);
   }
   void end(int slot, hipStream_t st) {
-    if (on && n[slot] < 512) {
+    if (on && n[slot] < kStageEvents) {
       (void)hipEventRecord(ev[1][slot][n[slot]], st);
       ++n[slot];
     }
