@@ -66,7 +66,7 @@ olpt_status olpt_sync(olpt_ctx *ctx);
 void *olpt_stream(olpt_ctx *ctx);
 const char *olpt_last_error(void);
 /* Device-side diagnostic counters of the last stereo/shake run:
- * [0] candidate overflow, [1] LDS list overflow, [2] pool overflow,
+ * [0] candidate overflow, [1] build list overflow, [2] unused,
  * [3] prune replacement cases, [4] object overflow, [5] prune rounds
  * exhausted, [6] checkMatch triangulation failures, [7] peak overflow. */
 olpt_status olpt_diagnostics(olpt_ctx *ctx, int32_t out[8]);
