@@ -152,6 +152,36 @@ olpt_status olpt_ipr_graph_capture(olpt_ctx *ctx, const void *d_imgs,
                                    int cap_obj_frame, int32_t *d_n_obj);
 olpt_status olpt_ipr_graph_launch(olpt_ctx *ctx);
 
+/* ---- learned shake proposal ------------------------------------------
+ * No counterpart in the reference.  A BF16 MLP on MFMA (FP32 accumulate):
+ * 4 cameras x 11x11 augmented residual patches (484 inputs, zero-padded to
+ * 512) -> 256 -> 256 -> 256 (GELU) -> (dx, dy, dz in voxels, ghost logit).
+ * Parameters are one FP32 array of olpt_mlp_num_params() values: W1
+ * [512][256], b1 [256], W2 [256][256], b2, W3 [256][256], b3, W4 [256][32],
+ * b4 [32]; rows 484..511 of W1 and columns 4..31 of W4 / b4 are padding and
+ * are kept at zero.  A handle owns its stream use and training buffers for
+ * batches of up to max_batch rows. */
+typedef struct olpt_mlp olpt_mlp;
+int32_t olpt_mlp_num_params(void);
+olpt_status olpt_mlp_create(int device, void *stream, int max_batch, olpt_mlp **out);
+olpt_status olpt_mlp_destroy(olpt_mlp *mlp);
+olpt_status olpt_mlp_set_params(olpt_mlp *mlp, const float *params, int32_t n);
+olpt_status olpt_mlp_get_params(olpt_mlp *mlp, float *params, int32_t n);
+/* d_y [n][4] = MLP(d_x [n][512] bf16), device buffers, on the handle's
+ * stream. */
+olpt_status olpt_mlp_infer(olpt_mlp *mlp, const void *d_x, int32_t n, float *d_y);
+/* One Adam step (beta 0.9 / 0.999) on d_x [n][512] bf16 with targets d_t
+ * [n][4] = (dx, dy, dz in voxels, ghost label 0/1): loss = mean over rows
+ * of (1 - g) |y_xyz - t_xyz|^2 + ghost_weight * BCE(y_ghost, g).
+ * *loss_out (host, may be NULL) receives the batch loss. */
+olpt_status olpt_mlp_train_step(olpt_mlp *mlp, const void *d_x, const float *d_t,
+                                int32_t n, float lr, float ghost_weight,
+                                float *loss_out);
+/* Gradient of the last train step's loss (parameter layout); a step with
+ * lr = 0 leaves the parameters unchanged. */
+olpt_status olpt_mlp_get_grads(olpt_mlp *mlp, float *grads, int32_t n);
+const char *olpt_mlp_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

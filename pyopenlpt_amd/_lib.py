@@ -81,6 +81,21 @@ def lib() -> ctypes.CDLL:
     L.olpt_stage_timing.argtypes = [_P, _I]
     L.olpt_stage_times.argtypes = [_P, _DP, _I32P]
     L.olpt_stage_stats.argtypes = [_P, _I, _DP, _I32P]
+    _F = ctypes.c_float
+    _FP = ctypes.POINTER(ctypes.c_float)
+    L.olpt_mlp_num_params.restype = ctypes.c_int32
+    L.olpt_mlp_last_error.restype = ctypes.c_char_p
+    L.olpt_mlp_create.argtypes = [_I, _P, _I, ctypes.POINTER(_P)]
+    L.olpt_mlp_destroy.argtypes = [_P]
+    L.olpt_mlp_set_params.argtypes = [_P, _FP, ctypes.c_int32]
+    L.olpt_mlp_get_params.argtypes = [_P, _FP, ctypes.c_int32]
+    L.olpt_mlp_get_grads.argtypes = [_P, _FP, ctypes.c_int32]
+    L.olpt_mlp_infer.argtypes = [_P, _P, ctypes.c_int32, _P]
+    L.olpt_mlp_train_step.argtypes = [_P, _P, _P, ctypes.c_int32, _F, _F, _FP]
+    for name in ("olpt_mlp_create", "olpt_mlp_destroy", "olpt_mlp_set_params",
+                 "olpt_mlp_get_params", "olpt_mlp_get_grads", "olpt_mlp_infer",
+                 "olpt_mlp_train_step"):
+        getattr(L, name).restype = ctypes.c_int32
     for name in ("olpt_ctx_create", "olpt_ctx_destroy", "olpt_ctx_reserve",
                  "olpt_set_precision", "olpt_set_concurrency", "olpt_set_cameras",
                  "olpt_set_otf",
@@ -97,4 +112,10 @@ def lib() -> ctypes.CDLL:
 def check(status: int) -> None:
     if status != OLPT_OK:
         msg = lib().olpt_last_error().decode(errors="replace")
+        raise FatalError(status, msg)
+
+
+def check_mlp(status: int) -> None:
+    if status != OLPT_OK:
+        msg = lib().olpt_mlp_last_error().decode(errors="replace")
         raise FatalError(status, msg)

@@ -1,0 +1,123 @@
+"""Learned shake proposal: the BF16 MLP (olpt_mlp_* in include/olpt_c.h).
+
+No counterpart in the reference.  ShakeNet wraps one olpt_mlp handle: its
+parameters (FP32 master copy, He-initialised here), inference and Adam
+training steps run as HIP kernels on MFMA; torch only provides device
+buffers and the stream.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check_mlp
+
+N_IN, K0, H, N_OUT, N4 = 484, 512, 256, 4, 32
+
+
+def param_layout() -> dict:
+    """Name -> (offset, shape) in the flat FP32 parameter array."""
+    out, off = {}, 0
+    for name, shape in (("W1", (K0, H)), ("b1", (H,)), ("W2", (H, H)), ("b2", (H,)),
+                        ("W3", (H, H)), ("b3", (H,)), ("W4", (H, N4)), ("b4", (N4,))):
+        out[name] = (off, shape)
+        off += int(np.prod(shape))
+    return out
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-normal hidden weights, a small output layer, zero biases; padding
+    (rows 484.. of W1, columns 4.. of W4) zero."""
+    rng = np.random.default_rng(seed)
+    lay = param_layout()
+    p = np.zeros(sum(int(np.prod(s)) for _, s in lay.values()), np.float32)
+
+    def put(name, arr):
+        o, s = lay[name]
+        p[o:o + arr.size] = arr.reshape(-1)
+
+    w1 = np.zeros((K0, H), np.float32)
+    w1[:N_IN] = rng.normal(0, np.sqrt(2.0 / N_IN), (N_IN, H))
+    put("W1", w1)
+    put("W2", rng.normal(0, np.sqrt(2.0 / H), (H, H)).astype(np.float32))
+    put("W3", rng.normal(0, np.sqrt(2.0 / H), (H, H)).astype(np.float32))
+    w4 = np.zeros((H, N4), np.float32)
+    w4[:, :N_OUT] = rng.normal(0, 0.1 * np.sqrt(1.0 / H), (H, N_OUT))
+    put("W4", w4)
+    return p
+
+
+class ShakeNet:
+    """One olpt_mlp handle on torch's current stream of `device`."""
+
+    def __init__(self, max_batch: int = 8192, device: int = 0, seed: int = 0,
+                 stream: torch.cuda.Stream | None = None):
+        self.L = _lib.lib()
+        self.n_params = int(self.L.olpt_mlp_num_params())
+        self.max_batch = max_batch
+        self.stream = stream if stream is not None else torch.cuda.current_stream(device)
+        h = ctypes.c_void_p()
+        check_mlp(self.L.olpt_mlp_create(device, ctypes.c_void_p(self.stream.cuda_stream),
+                                         max_batch, ctypes.byref(h)))
+        self.h = h
+        self.set_params(init_params(seed))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.olpt_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _fp(self, a: np.ndarray):
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check_mlp(self.L.olpt_mlp_set_params(self.h, self._fp(p), p.size))
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, np.float32)
+        check_mlp(self.L.olpt_mlp_get_params(self.h, self._fp(p), p.size))
+        return p
+
+    def grads(self) -> np.ndarray:
+        g = np.zeros(self.n_params, np.float32)
+        check_mlp(self.L.olpt_mlp_get_grads(self.h, self._fp(g), g.size))
+        return g
+
+    def infer(self, x: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """x [n][512] bfloat16 (device) -> [n][4] float32."""
+        assert x.dtype == torch.bfloat16 and x.shape[1] == K0 and x.is_contiguous()
+        n = x.shape[0]
+        if out is None:
+            out = torch.empty((n, N_OUT), dtype=torch.float32, device=x.device)
+        check_mlp(self.L.olpt_mlp_infer(self.h, ctypes.c_void_p(x.data_ptr()), n,
+                                        ctypes.c_void_p(out.data_ptr())))
+        return out
+
+    def train_step(self, x: torch.Tensor, t: torch.Tensor, lr: float,
+                   ghost_weight: float = 1.0) -> float:
+        """One Adam step; t [n][4] = (dx, dy, dz in voxels, ghost 0/1)."""
+        assert x.dtype == torch.bfloat16 and x.shape[1] == K0 and x.is_contiguous()
+        assert t.dtype == torch.float32 and t.shape == (x.shape[0], 4) and t.is_contiguous()
+        loss = ctypes.c_float()
+        check_mlp(self.L.olpt_mlp_train_step(self.h, ctypes.c_void_p(x.data_ptr()),
+                                             ctypes.c_void_p(t.data_ptr()), x.shape[0],
+                                             float(lr), float(ghost_weight),
+                                             ctypes.byref(loss)))
+        return float(loss.value)
+
+    def save(self, path: str):
+        np.savez_compressed(path, params=self.params())
+
+    def load(self, path: str):
+        with np.load(path, allow_pickle=False) as z:
+            self.set_params(z["params"])
