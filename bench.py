@@ -4,9 +4,13 @@
 Workload: the ppp = 0.05 configuration - 4 pinhole cameras, 1024 x 1024
 uint16 images, 52,429 particles per frame drawn from 8 Gaussian clusters
 (sigma 5 mm) in a 40 mm cube, OTF a ~ U(800, 1200), b = c = 1.2, noise
-sigma 10; 4 IPR loops x 6 shake iterations.  A "step" runs IPR on a batch of
-64 frames resident in HBM, spread over 4 HIP streams; value = reconstructed
-particles per second over the timed steps.
+sigma 10; 4 IPR loops.  A "step" runs IPR on a batch of 64 frames resident
+in HBM, spread over 4 HIP streams.  Two legs are timed: plain shake (the
+reference schedule, 6 iterations from 0.25 voxel) and the learned proposal
+(the BF16 MLP trained here on separately seeded frames, then 1 plain
+iteration from 0.03 voxel).  value = reconstructed particles per second of
+the learned leg when its 3D RMSE against ground truth is not above the plain
+leg's, else of the plain leg; both legs are reported in config.
 
 One JSON line is printed on stdout; progress goes to stderr.
 """
@@ -136,6 +140,10 @@ def main() -> int:
                     help="after the run, an elementwise kernel that reads and writes this many "
                          "MiB (counter calibration under rocprofv3 --pmc)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--shake", default="both", choices=["plain", "both"],
+                    help="both: also train the learned proposal and time its leg")
+    ap.add_argument("--train-particles", type=int, default=6_000_000)
+    ap.add_argument("--train-steps", type=int, default=6000)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -166,42 +174,54 @@ def main() -> int:
     ctx.set_concurrency(args.streams)
     cap_obj = 1 << 17
     outs = ctx.alloc_outputs(args.frames, cap_obj, dimg.device)
+    stream = torch.cuda.current_stream()
 
-    def step():
+    def timed_leg(params, label):
+        """Warm-up, then args.steps timed steps with events on the context
+        stream; returns ms/step, particles/step and quality over the batch."""
+        def step():
+            if args.graph:
+                ctx.graph_launch()
+            else:
+                ctx.ipr_run(dimg, code, params, n_frames=args.frames, cap_obj=cap_obj,
+                            outputs=outs)
         if args.graph:
-            ctx.graph_launch()
-        else:
-            ctx.ipr_run(dimg, code, w0.params, n_frames=args.frames, cap_obj=cap_obj,
-                        outputs=outs)
+            ctx.graph_capture(dimg, code, params, args.frames, outs, cap_obj)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(args.steps):
+            step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = e0.elapsed_time(e1) / args.steps
+        n_obj = outs[2].cpu().numpy()
+        per_step = int(n_obj.sum())
+        rate = per_step / (ms / 1000.0)
+        log(f"{label}: {args.steps} steps: {ms:.2f} ms/step (wall {1000 * wall / args.steps:.2f}), "
+            f"{per_step} particles/step -> {rate:.0f} particles/s")
+        q = [quality(outs[0][f, : n_obj[f]].cpu().numpy(), frames[f].truth, frames[f].voxel_mm)
+             for f in range(args.frames)]
+        rmse = [x["rmse_mm"] for x in q if x["rmse_mm"] is not None]
+        qual = dict(rmse_mm=float(np.mean(rmse)) if rmse else None,
+                    rmse_px=float(np.mean(rmse) / w0.voxel_mm) if rmse else None,
+                    recon_rate=float(np.mean([x["recon_rate"] for x in q])),
+                    ghost_rate=float(np.mean([x["ghost_rate"] for x in q])))
+        return dict(particles_per_s=round(rate, 1), ms_per_step=round(ms, 3),
+                    particles_per_step=per_step, shake_loops=int(params.n_shake_loop),
+                    quality=qual)
 
-    if args.graph:
-        ctx.graph_capture(dimg, code, w0.params, args.frames, outs, cap_obj)
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
+    # ---- plain shake: the reference schedule (6 iterations from 0.25 vox)
+    plain = timed_leg(w0.params, "plain")
 
     # algorithmic bytes per stage launch (instrumented extra run, untimed)
     ctx.stage_stats(True)
     ctx.ipr_run(dimg, code, w0.params, n_frames=args.frames, cap_obj=cap_obj, outputs=outs)
     stats = ctx.stage_stats(False)
-
-    stream = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        step()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_total = e0.elapsed_time(e1)
-    ms_per_step = ms_total / args.steps
-    n_obj = outs[2].cpu().numpy()
-    per_step = int(n_obj.sum())
-    value = per_step / (ms_per_step / 1000.0)
-    log(f"{args.steps} steps: {ms_per_step:.2f} ms/step (wall {1000 * wall / args.steps:.2f}), "
-        f"{per_step} particles/step -> {value:.0f} particles/s")
 
     # Roofline pass: the kernel's own launch time.  Concurrent streams share
     # the GPU, so per-launch event intervals of the throughput run include
@@ -221,14 +241,42 @@ def main() -> int:
         ctx.stage_timing(False)
         ctx.set_concurrency(args.streams)
 
-    # quality over the batch
-    q = [quality(outs[0][f, : n_obj[f]].cpu().numpy(), frames[f].truth, frames[f].voxel_mm)
-         for f in range(args.frames)]
-    rmse = [x["rmse_mm"] for x in q if x["rmse_mm"] is not None]
-    qual = dict(rmse_mm=float(np.mean(rmse)) if rmse else None,
-                rmse_px=float(np.mean(rmse) / w0.voxel_mm) if rmse else None,
-                recon_rate=float(np.mean([x["recon_rate"] for x in q])),
-                ghost_rate=float(np.mean([x["ghost_rate"] for x in q])))
+    # ---- learned proposal + 1 plain iteration (from 0.03 vox)
+    learned = None
+    if args.shake == "both":
+        from pyopenlpt_amd.learned import ShakeNet, make_training_set, train
+        t0 = time.perf_counter()
+        tctx = Context(w0.cams, H, W, precision="f32")
+        tctx.set_otf(otf)
+        X, T, tstats = make_training_set(tctx, w0.params, w0.voxel_mm,
+                                         n_particles=args.train_particles, seed=9001)
+        net = ShakeNet(max_batch=8192, seed=0)
+        losses = train(net, X, T, steps=args.train_steps, batch=8192, lr=3e-3, ghost_weight=0.5)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        del X, T
+        tctx.close()
+        log(f"learned proposal: {tstats['samples']} samples from {tstats['frames']} frames, "
+            f"{args.train_steps} steps, loss {np.mean(losses[:10]):.3f} -> "
+            f"{np.mean(losses[-50:]):.3f}, {t_train:.1f} s")
+        lp = S.OlptIprParams.from_buffer_copy(w0.params)
+        lp.n_shake_loop = 1
+        lp.shake_width = 0.03 * w0.voxel_mm
+        ctx.set_shake_model(net, w0.voxel_mm, 0.6)
+        learned = timed_leg(lp, "learned")
+        ctx.set_shake_model(None)
+        learned["train"] = dict(samples=tstats["samples"], frames=tstats["frames"],
+                                particles=tstats["particles"], steps=args.train_steps,
+                                seconds=round(t_train, 2), seeds="9001.. (eval frames 1234..)")
+
+    # the metric: learned vs plain at equal 3D RMSE (learned counts when its
+    # RMSE is not above plain's)
+    use_learned = (learned is not None and learned["quality"]["rmse_mm"] is not None and
+                   plain["quality"]["rmse_mm"] is not None and
+                   learned["quality"]["rmse_mm"] <= plain["quality"]["rmse_mm"])
+    best = learned if use_learned else plain
+    value = best["particles_per_s"]
+    ms_per_step = best["ms_per_step"]
 
     # Roofline of the dominant kernel: k_shake (the "shake" stage brackets
     # exactly its launches).  Algorithmic bytes per launch: the per-object
@@ -248,8 +296,8 @@ def main() -> int:
                     "traffic": traffic_per_launch(workload),
                     "bytes_per_launch": int(per_launch_bytes),
                     "avg_launch_ms": round(avg_ms, 4),
-                    "measured": f"{args.roofline_steps} single-stream step(s), HIP events "
-                                "around each k_shake launch",
+                    "measured": f"{args.roofline_steps} single-stream step(s) of the plain leg, "
+                                "HIP events around each k_shake launch",
                     "stage_ms_per_step_1stream": {k: round(v[0] / args.roofline_steps, 3)
                                                   for k, v in times.items() if v[1]}}
 
@@ -271,10 +319,11 @@ def main() -> int:
         "config": {"workload": workload,
                    "frames_per_step": args.frames, "particles_per_frame": len(w0.truth),
                    "ipr_loops": int(w0.params.n_loop_ipr),
-                   "shake_loops": int(w0.params.n_shake_loop),
-                   "shake": "plain", "graph": bool(args.graph),
-                   "streams": args.streams,
-                   "quality": qual},
+                   "shake": "learned" if use_learned else "plain",
+                   "graph": bool(args.graph), "streams": args.streams,
+                   "plain": plain, "learned": learned,
+                   "learned_over_plain": (round(learned["particles_per_s"] / plain["particles_per_s"], 3)
+                                          if learned else None)},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

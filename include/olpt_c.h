@@ -177,6 +177,31 @@ olpt_status olpt_mlp_infer(olpt_mlp *mlp, const void *d_x, int32_t n, float *d_y
 olpt_status olpt_mlp_train_step(olpt_mlp *mlp, const void *d_x, const float *d_t,
                                 int32_t n, float lr, float ghost_weight,
                                 float *loss_out);
+/* Use mlp (NULL: plain shake) in the shake of every IPR loop and olpt_shake
+ * call of this context: before the plain iterations each object takes the
+ * network's step (clamped to +-max_step_vox voxels per axis, voxel_mm per
+ * voxel) when it lowers the shake residue (calShakeResidue, Shake.cpp:
+ * 968-1045) on the object's ROI; the plain iterations then run as
+ * params->n_shake_loop / shake_width say.  4 cameras, ROI up to 8x8.  The
+ * network is not owned; a previously captured graph keeps the old setting. */
+olpt_status olpt_set_shake_model(olpt_ctx *ctx, const olpt_mlp *mlp,
+                                 double voxel_mm, double max_step_vox);
+/* Network inputs of n_obj objects at d_xyz on images d_imgs: residual of
+ * the active cameras from all n_obj objects (as at the start of a shake
+ * iteration), then per camera the 11x11 window of max(0, residual + own
+ * projection) around the rounded projection, divided by the OTF amplitude:
+ * d_x [n_obj][512] bf16 (4 x 121 values, then zeros). */
+olpt_status olpt_shake_patches(olpt_ctx *ctx, const int32_t *active,
+                               const void *d_imgs, int dtype,
+                               const olpt_ipr_params *params,
+                               const double *d_xyz, int n_obj, void *d_x);
+/* Render n_obj particles: max-fusion of their OTF projections over the box
+ * of half-size r_px (calROIBound), amplitude d_amp[i] in place of the OTF
+ * amplitude when d_amp is not NULL, into d_out [n_cam][n_row][n_col] in the
+ * context precision (the training images of the learned proposal). */
+olpt_status olpt_render(olpt_ctx *ctx, const double *d_xyz, const double *d_amp,
+                        int n_obj, double r_px, void *d_out);
+
 /* Gradient of the last train step's loss (parameter layout); a step with
  * lr = 0 leaves the parameters unchanged. */
 olpt_status olpt_mlp_get_grads(olpt_mlp *mlp, float *grads, int32_t n);

@@ -92,6 +92,11 @@ def lib() -> ctypes.CDLL:
     L.olpt_mlp_get_grads.argtypes = [_P, _FP, ctypes.c_int32]
     L.olpt_mlp_infer.argtypes = [_P, _P, ctypes.c_int32, _P]
     L.olpt_mlp_train_step.argtypes = [_P, _P, _P, ctypes.c_int32, _F, _F, _FP]
+    L.olpt_set_shake_model.argtypes = [_P, _P, ctypes.c_double, ctypes.c_double]
+    L.olpt_render.argtypes = [_P, _P, _P, _I, ctypes.c_double, _P]
+    L.olpt_shake_patches.argtypes = [_P, _I32P, _P, _I, ctypes.POINTER(OlptIprParams), _P, _I, _P]
+    for name in ("olpt_set_shake_model", "olpt_render", "olpt_shake_patches"):
+        getattr(L, name).restype = ctypes.c_int32
     for name in ("olpt_mlp_create", "olpt_mlp_destroy", "olpt_mlp_set_params",
                  "olpt_mlp_get_params", "olpt_mlp_get_grads", "olpt_mlp_infer",
                  "olpt_mlp_train_step"):

@@ -151,7 +151,8 @@ class Context:
         return xy[: n.value].cpu().numpy()
 
     def stereo_match(self, pts: list[np.ndarray], params: OlptIprParams,
-                     active=None, cap_cand: int = 1 << 21, cap_obj: int = 1 << 18):
+                     active=None, cap_cand: int = 1 << 21, cap_obj: int = 1 << 18,
+                     candidates: bool = True):
         cap2d = max(1, max(len(p) for p in pts))
         xy = np.zeros((self.n_cam, cap2d, 2))
         for k, p in enumerate(pts):
@@ -160,9 +161,12 @@ class Context:
         npts = np.array([len(p) for p in pts], np.int32)
         act = None if active is None else np.asarray(active, np.int32)
         dev = d_xy.device
-        cid = torch.empty((cap_cand, self.n_cam), dtype=torch.int32, device=dev)
-        ce = torch.empty(cap_cand, dtype=torch.float64, device=dev)
-        sel = torch.empty(cap_cand, dtype=torch.uint8, device=dev)
+        if candidates:
+            cid = torch.empty((cap_cand, self.n_cam), dtype=torch.int32, device=dev)
+            ce = torch.empty(cap_cand, dtype=torch.float64, device=dev)
+            sel = torch.empty(cap_cand, dtype=torch.uint8, device=dev)
+        else:
+            cid = ce = sel = None
         ox = torch.empty((cap_obj, 3), dtype=torch.float64, device=dev)
         oi = torch.empty((cap_obj, self.n_cam), dtype=torch.int32, device=dev)
         counts = np.zeros(3, np.int32)
@@ -172,6 +176,9 @@ class Context:
             ptr(cid), ptr(ce), ptr(sel), cap_cand, ptr(ox), ptr(oi), cap_obj,
             counts.ctypes.data_as(_lib._I32P)))
         M, _, no = (int(x) for x in counts)
+        if not candidates:
+            return dict(obj_xyz=ox[:no].cpu().numpy(), obj_ids=oi[:no].cpu().numpy(),
+                        counts=counts)
         return dict(cand_ids=cid[:M].cpu().numpy(), cand_e=ce[:M].cpu().numpy(),
                     sel=sel[:M].cpu().numpy().astype(bool), obj_xyz=ox[:no].cpu().numpy(),
                     obj_ids=oi[:no].cpu().numpy(), counts=counts)
@@ -201,6 +208,32 @@ class Context:
         check(self.L.olpt_residual(self.h, ptr(imgs), dtype, ctypes.byref(params),
                                    ptr(d_in), len(xyz), ptr(d_res)))
         return d_res.cpu().numpy()
+
+    # -- learned shake proposal ----------------------------------------------
+    def set_shake_model(self, net, voxel_mm: float = 0.0, max_step_vox: float = 0.5):
+        """Use a learned.ShakeNet before the plain shake iterations (None:
+        plain shake only)."""
+        self._net = net
+        check(self.L.olpt_set_shake_model(self.h, net.h if net is not None else None,
+                                          float(voxel_mm), float(max_step_vox)))
+
+    def render(self, xyz: torch.Tensor, amp: torch.Tensor | None = None,
+               r_px: float = 2.5) -> torch.Tensor:
+        """Max-fusion rendering of particles xyz [n][3] (float64, device)."""
+        out = torch.empty((self.n_cam, self.H, self.W), dtype=self.real_dtype, device=xyz.device)
+        check(self.L.olpt_render(self.h, ptr(xyz), ptr(amp), xyz.shape[0], float(r_px), ptr(out)))
+        return out
+
+    def shake_patches(self, imgs: torch.Tensor, dtype: int, params: OlptIprParams,
+                      xyz: torch.Tensor, active=None) -> torch.Tensor:
+        """Network inputs [n][512] (bfloat16) of objects xyz [n][3] (device)."""
+        n = xyz.shape[0]
+        out = torch.zeros((max(n, 1), 512), dtype=torch.bfloat16, device=xyz.device)
+        act = None if active is None else np.asarray(active, np.int32)
+        check(self.L.olpt_shake_patches(self.h, None if act is None else act.ctypes.data_as(_lib._I32P),
+                                        ptr(imgs), dtype, ctypes.byref(params), ptr(xyz), n,
+                                        ptr(out)))
+        return out[:n]
 
     def ipr_run(self, imgs: torch.Tensor, dtype: int, params: OlptIprParams,
                 n_frames: int = 1, cap_obj: int = 1 << 18, cap_loop: int = 64,

@@ -68,6 +68,8 @@ struct Workspace {
           *g_cellof = nullptr, *has_nb = nullptr, *sh_keep_off = nullptr;
   void *rec = nullptr;
   double *partial = nullptr;
+  void *patches = nullptr;   // learned proposal: bf16 [cap_obj][512]
+  float *prop = nullptr;     // [cap_obj][4]
   // misc
   int32_t *flags = nullptr;  // [16]
   int32_t *scan_scratch = nullptr;
@@ -94,6 +96,9 @@ struct olpt_ctx {
   int otf_ng = 0;
   int bin_rows = 0, bin_cols = 0;
   int grid_n = 80;
+  // learned shake proposal (not owned)
+  const olpt_mlp *mlp = nullptr;
+  double mlp_voxel_mm = 0.0, mlp_max_step_vox = 0.5;
   // workspaces
   Workspace ws[kMaxWorkspaces];
   int n_ws = 0;          // allocated
@@ -293,6 +298,8 @@ ShakeBuffers<T> shake_buffers(olpt_ctx *c, Workspace &w, T *img, T *img_next) {
   b.scan_scratch = w.scan_scratch;
   b.totals = w.totals;
   b.work = w.sb.work;
+  b.patches = c->mlp ? w.patches : nullptr;
+  b.prop = c->mlp ? w.prop : nullptr;
   return b;
 }
 
@@ -361,6 +368,9 @@ ShakeParams shake_params(olpt_ctx *c, const int32_t *active, const olpt_ipr_para
   sp.r_px = p->radius_obj;
   sp.n_shake_loop = p->n_shake_loop;
   sp.ratio_augimg = p->ratio_augimg;
+  sp.mlp = c->mlp;
+  sp.voxel_mm = c->mlp_voxel_mm;
+  sp.max_step_vox = c->mlp_max_step_vox;
   return sp;
 }
 
@@ -379,6 +389,8 @@ olpt_status check_params(olpt_ctx *c, const olpt_ipr_params *p) {
   const int roi = (int)std::floor(2.0 * p->radius_obj * p->ratio_augimg + 0.5 + 1e-3);
   if (roi * roi > 4 * 64 || box > 24)
     return fail(OLPT_E_UNSUPPORTED, "ROI larger than 16x16 pixels is not supported");
+  if (c->mlp && roi > 8)
+    return fail(OLPT_E_UNSUPPORTED, "the learned shake proposal supports ROIs up to 8x8 pixels");
   if (p->idmap_cell_px < 1) return fail(OLPT_E_INVALID_ARGUMENT, "idmap_cell_px must be >= 1");
   if (p->idmap_cell_px != c->ws[0].sb.cell_px)
     return fail(OLPT_E_UNSUPPORTED, "idmap_cell_px differs from the reserved workspace (4)");
@@ -685,6 +697,14 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   return OLPT_OK;
 }
 
+olpt_status alloc_learned(olpt_ctx *c, Workspace &w) {
+  if (w.patches) return OLPT_OK;
+  olpt_status s;
+  if ((s = dalloc(c, &w.patches, (size_t)c->cap_obj * 512 * 2))) return s;
+  if ((s = dalloc_t(c, &w.prop, (size_t)c->cap_obj * 4))) return s;
+  return OLPT_OK;
+}
+
 void free_streams(olpt_ctx *c) {
   for (int i = 1; i < kMaxWorkspaces; ++i) {
     if (c->ws[i].st) (void)hipStreamDestroy(c->ws[i].st);
@@ -712,6 +732,7 @@ olpt_status ensure_workspaces(olpt_ctx *c) {
     if (!w.done) HIP_TRY(hipEventCreateWithFlags(&w.done, hipEventDisableTiming));
     olpt_status s = alloc_workspace(c, w);
     if (s) return s;
+    if (c->mlp && (s = alloc_learned(c, w))) return s;
     ++c->n_ws;
   }
   HIP_TRY(hipDeviceSynchronize());
@@ -1190,6 +1211,95 @@ olpt_status olpt_ipr_graph_launch(olpt_ctx *c) {
   if (!c || !c->gexec) return fail(OLPT_E_INVALID_ARGUMENT, "no captured graph");
   HIP_TRY(hipGraphLaunch(c->gexec, c->st));
   return OLPT_OK;
+}
+
+} // extern "C"
+
+// ===========================================================================
+// learned shake proposal, rendering and network inputs
+// ===========================================================================
+namespace {
+template <typename T>
+olpt_status render_impl(olpt_ctx *c, const double *d_xyz, const double *d_amp, int n, double r_px,
+                        void *d_out) {
+  hipStream_t st = c->st;
+  Workspace &w = c->ws[0];
+  if (n > 0)
+    HIP_TRY(hipMemcpyAsync(w.keep_xyz, d_xyz, sizeof(double) * 3 * (size_t)n,
+                           hipMemcpyDeviceToDevice, st));
+  fill_i32(w.n_keep, 1, n, st);
+  ShakeBuffers<T> b = shake_buffers<T>(c, w, (T *)w.img_a, (T *)w.img_b);
+  launch_render<T>(b, w.keep_xyz, d_amp, w.n_keep, r_px, (T *)d_out, st);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  return OLPT_OK;
+}
+
+template <typename T>
+olpt_status patches_impl(olpt_ctx *c, const int32_t *active, const void *d_imgs, int dtype,
+                         const olpt_ipr_params *p, const double *d_xyz, int n, void *d_x) {
+  hipStream_t st = c->st;
+  Workspace &w = c->ws[0];
+  const long npx = (long)c->n_cam * c->H * c->W;
+  olpt_status s = convert_image<T>(d_imgs, dtype, npx, (T *)w.img_a, st);
+  if (s) return s;
+  if (n > 0)
+    HIP_TRY(hipMemcpyAsync(w.sh_xyz, d_xyz, sizeof(double) * 3 * (size_t)n,
+                           hipMemcpyDeviceToDevice, st));
+  fill_i32(w.sh_n, 1, n, st);
+  ShakeBuffers<T> b = shake_buffers<T>(c, w, (T *)w.img_a, (T *)w.img_b);
+  ShakeParams shp = shake_params(c, active, p);
+  launch_patches<T>(b, shp, d_x, st);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  return OLPT_OK;
+}
+} // namespace
+
+extern "C" {
+
+olpt_status olpt_set_shake_model(olpt_ctx *c, const olpt_mlp *mlp, double voxel_mm,
+                                 double max_step_vox) {
+  if (!c) return fail(OLPT_E_INVALID_ARGUMENT, "null context");
+  if (mlp && c->n_cam != 4)
+    return fail(OLPT_E_UNSUPPORTED, "the learned shake proposal takes exactly 4 cameras");
+  if (mlp && !(voxel_mm > 0)) return fail(OLPT_E_INVALID_ARGUMENT, "voxel_mm must be positive");
+  c->mlp = mlp;
+  c->mlp_voxel_mm = voxel_mm;
+  c->mlp_max_step_vox = max_step_vox > 0 ? max_step_vox : 0.5;
+  if (mlp)
+    for (int k = 0; k < c->n_ws; ++k) {
+      olpt_status s = alloc_learned(c, c->ws[k]);
+      if (s) return s;
+    }
+  return OLPT_OK;
+}
+
+olpt_status olpt_render(olpt_ctx *c, const double *d_xyz, const double *d_amp, int n_obj,
+                        double r_px, void *d_out) {
+  olpt_status s = check_ready(c);
+  if (s) return s;
+  if (!d_out || (n_obj > 0 && !d_xyz)) return fail(OLPT_E_INVALID_ARGUMENT, "null argument");
+  if (n_obj < 0 || n_obj > c->cap_obj) return fail(OLPT_E_CAPACITY, "object count exceeds capacity");
+  if (!c->otf_set) return fail(OLPT_E_INVALID_ARGUMENT, "OTF not set");
+  if (!(r_px > 0) || std::lround(2.0 * r_px) > 24)
+    return fail(OLPT_E_INVALID_ARGUMENT, "r_px must be in (0, 12]");
+  if (c->prec == OLPT_PREC_F64) return render_impl<double>(c, d_xyz, d_amp, n_obj, r_px, d_out);
+  return render_impl<float>(c, d_xyz, d_amp, n_obj, r_px, d_out);
+}
+
+olpt_status olpt_shake_patches(olpt_ctx *c, const int32_t *active, const void *d_imgs, int dtype,
+                               const olpt_ipr_params *p, const double *d_xyz, int n_obj,
+                               void *d_x) {
+  olpt_status s = check_ready(c);
+  if (s) return s;
+  if (!p || !d_imgs || !d_x || (n_obj > 0 && !d_xyz)) return fail(OLPT_E_INVALID_ARGUMENT, "null argument");
+  if ((s = check_params(c, p))) return s;
+  if (!c->otf_set) return fail(OLPT_E_INVALID_ARGUMENT, "OTF not set");
+  if (c->n_cam != 4) return fail(OLPT_E_UNSUPPORTED, "the learned shake proposal takes exactly 4 cameras");
+  if (n_obj < 0 || n_obj > c->cap_obj) return fail(OLPT_E_CAPACITY, "object count exceeds capacity");
+  if (c->prec == OLPT_PREC_F64) return patches_impl<double>(c, active, d_imgs, dtype, p, d_xyz, n_obj, d_x);
+  return patches_impl<float>(c, active, d_imgs, dtype, p, d_xyz, n_obj, d_x);
 }
 
 } // extern "C"

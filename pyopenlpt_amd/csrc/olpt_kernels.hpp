@@ -177,11 +177,16 @@ template <typename T> struct ShakeBuffers {
   int32_t *scan_scratch;
   int32_t *totals;
   int32_t *work;
+  // learned proposal (optional)
+  void *patches;      // [cap_obj][512] bf16 network input
+  float *prop;        // [cap_obj][4] network output
 };
 
 struct ShakeParams {
   int32_t active[kMaxCam];
   double shake_width, shakewidth_min, thred_ghost, tol3d, r_px;
+  const void *mlp;            // learned proposal network (olpt_mlp), or null
+  double voxel_mm, max_step_vox;
   int n_shake_loop, ratio_augimg;
 };
 
@@ -204,6 +209,19 @@ void launch_filter(const ShakeBuffers<T> &b, double *xyz_keep, T *uv_keep,
 
 // size in bytes of one (object, camera) record
 size_t shake_rec_bytes(bool f64);
+
+// Learned shake proposal: network inputs of the objects (residual from their
+// positions, then the 4 x 11 x 11 patches, bf16 [cap_obj][512]).
+template <typename T>
+void launch_patches(const ShakeBuffers<T> &b, const ShakeParams &p, void *X, hipStream_t st);
+// Render objects (max-fusion of their projections; amp, if given, replaces
+// the OTF amplitude per object) into out [n_cam][H][W].
+template <typename T>
+void launch_render(const ShakeBuffers<T> &b, const double *xyz, const double *amp,
+                   const int32_t *n_ptr, double r_px, T *out, hipStream_t st);
+// MLP forward on a device-resident row count (olpt_mlp.hip).
+void mlp_infer_device(const void *mlp, const void *X, const int32_t *d_n, int cap, float *Y,
+                      hipStream_t st);
 
 template <typename T>
 void launch_convert_image(const void *src, int dtype, long n, T *dst,

@@ -100,13 +100,16 @@ struct TrainBuf {
 
 // Forward pass of kRows rows.  Hidden activations stay in LDS; in training
 // mode pre-activations and transposed inputs are kept for the backward pass.
+// The row count is n_host, or *n_dev when given; blocks stride over the
+// 64-row tiles.
 template <bool TRAIN>
-__global__ void __launch_bounds__(256) k_fwd(Net net, const bf16 *X, int n, float *Y, TrainBuf tb) {
+__global__ void __launch_bounds__(256) k_fwd(Net net, const bf16 *X, int n_host, const int32_t *n_dev,
+                                             float *Y, TrainBuf tb) {
   __shared__ bf16 hA[kRows * kLdsLd], hB[kRows * kLdsLd];
-  const int m0 = blockIdx.x * kRows;
+  const int n = n_dev ? *n_dev : n_host;
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  for (int m0 = blockIdx.x * kRows; m0 < n; m0 += gridDim.x * kRows) {
   const int rows = min(kRows, n - m0);
-  if (rows <= 0) return;
   if (TRAIN) {
     // transposed input for the first layer's weight gradient
     for (int i = threadIdx.x; i < kRows * K0; i += 256) {
@@ -156,6 +159,8 @@ __global__ void __launch_bounds__(256) k_fwd(Net net, const bf16 *X, int n, floa
       const int row = w * 32 + acc_row(j), col = l % 32;
       if (col < NOut && row < rows) Y[(long)(m0 + row) * NOut + col] = acc[0][0][j] + net.b4[col];
     }
+  }
+  __syncthreads();  // the next tile overwrites the LDS activations
   }
 }
 
@@ -342,6 +347,20 @@ void refresh(olpt_mlp *m) {
 }
 }  // namespace
 
+namespace olpt {
+// Forward pass for rows counted on the device (inside an IPR run, on the
+// caller's stream; inference only reads the weights).
+void mlp_infer_device(const void *mlp, const void *X, const int32_t *d_n, int cap, float *Y,
+                      hipStream_t st) {
+  const olpt_mlp *m = static_cast<const olpt_mlp *>(mlp);
+  int blocks = (cap + kRows - 1) / kRows;
+  if (blocks > 1024) blocks = 1024;
+  if (blocks < 1) blocks = 1;
+  TrainBuf none{};
+  k_fwd<false><<<blocks, 256, 0, st>>>(net_of(m), (const bf16 *)X, 0, d_n, Y, none);
+}
+}  // namespace olpt
+
 extern "C" {
 
 const char *olpt_mlp_last_error(void) { return g_mlp_err.c_str(); }
@@ -449,7 +468,8 @@ olpt_status olpt_mlp_infer(olpt_mlp *m, const void *d_x, int32_t n, float *d_y) 
   if (!m || (n > 0 && (!d_x || !d_y))) return mfail(OLPT_E_INVALID_ARGUMENT, "null argument");
   if (n <= 0) return OLPT_OK;
   TrainBuf none{};
-  k_fwd<false><<<(n + kRows - 1) / kRows, 256, 0, m->st>>>(net_of(m), (const bf16 *)d_x, n, d_y, none);
+  k_fwd<false><<<(n + kRows - 1) / kRows, 256, 0, m->st>>>(net_of(m), (const bf16 *)d_x, n, nullptr,
+                                                         d_y, none);
   MLP_TRY(hipGetLastError());
   return OLPT_OK;
 }
@@ -466,7 +486,7 @@ olpt_status olpt_mlp_train_step(olpt_mlp *m, const void *d_x, const float *d_t, 
   const int nb = (n + kRows - 1) / kRows;
   k_zero<<<512, 256, 0, st>>>(m->G, kParams);
   k_zero<<<1, 64, 0, st>>>(m->loss, 4);
-  k_fwd<true><<<nb, 256, 0, st>>>(net_of(m), (const bf16 *)d_x, n, m->Y, tb);
+  k_fwd<true><<<nb, 256, 0, st>>>(net_of(m), (const bf16 *)d_x, n, nullptr, m->Y, tb);
   k_loss<<<256, 256, 0, st>>>(m->Y, d_t, n, tb, ghost_weight, m->loss);
   // The backward pass covers the nb*kRows rows the forward pass wrote (rows
   // past n hold zeros there); batch chunks of the weight gradient tile them

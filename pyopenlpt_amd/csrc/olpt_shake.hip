@@ -118,7 +118,7 @@ __global__ void k_bin_fill(ShakeBuffers<T> b, const int32_t *__restrict__ n_ptr)
 }
 
 // mode 0: shake residual (active cameras, no clamp); mode 1: IPR output
-// (all cameras, clamp at 0).  The shake never reads an inactive camera's
+// (all cameras, clamp at 0); mode 2: rendered image max(0, max_k p_k).  The shake never reads an inactive camera's
 // residual, so mode 0 skips them.
 constexpr int kTileBinCap = 64;
 template <typename T>
@@ -181,18 +181,29 @@ k_resid_gather(ShakeBuffers<T> b, const T *__restrict__ orig, T *__restrict__ ou
   if (inside) {
     T v = o - maxp;
     if (mode == 1 && v < T(0)) v = T(0);
+    if (mode == 2) v = maxp;   // rendering: max-fusion of the projections
     out[k * HW + pix] = v;
   }
+}
+
+// per-object amplitude in place of the OTF's a (rendering)
+template <typename T>
+__global__ void k_set_amp(ObjCamRec<T> *rec, const double *__restrict__ amp,
+                          const int32_t *__restrict__ n_ptr, int nc) {
+  const long n = (long)*n_ptr * nc;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+    rec[t].P[0] = (T)amp[t / nc];
 }
 
 template <typename T>
 static void residual_pass(const ShakeBuffers<T> &b, const T *orig, T *out,
                           const double *xyz, const int32_t *n_ptr,
                           ObjCamRec<T> *rec, T *uv_out, CamList cams, int mode,
-                          T r_px, hipStream_t st) {
+                          T r_px, hipStream_t st, const double *amp = nullptr) {
   const int nc = b.n_cam;
   const long nbin = (long)b.bin_rows * b.bin_cols;
   k_obj_project<T><<<grid_for(b.cap_obj, 256, 2048), 256, 0, st>>>(b, xyz, n_ptr, rec, uv_out, r_px);
+  if (amp) k_set_amp<T><<<grid_for((long)b.cap_obj * nc, 256, 2048), 256, 0, st>>>(rec, amp, n_ptr, nc);
   zero_i32(b.bin_cnt, nbin * nc, st);
   k_bin_count<T><<<grid_for((long)b.cap_obj * nc, 256, 4096), 256, 0, st>>>(b, n_ptr, rec);
   scan_exclusive(b.bin_cnt, b.bin_start, nbin, nc, nbin, nullptr, b.scan_scratch, st);
@@ -367,6 +378,45 @@ __device__ __forceinline__ bool any_out(const ShakeBuffers<T> &b, const CamLane<
   return (ballot(out) & c.omask) != 0;
 }
 
+// buildROIInfo (Shake.cpp:279-328) for object i with its current
+// projection: this lane's camera ROI (ratio_augimg x the object radius) and
+// its SLOTS pixels of aug = max(0, residual + own projection in the box).
+template <typename T, int LPG, int SLOTS>
+__device__ __forceinline__ void build_roi(const ShakeBuffers<T> &b, const ShakeParams &p,
+                                          const ObjCamRec<T> *__restrict__ rec, int i,
+                                          CamLane<T, SLOTS> &c) {
+  const int sub = lane_id() % LPG;
+  const long HW = (long)b.H * b.W;
+  const T r_px = (T)p.r_px;
+  const T dpos = r_px > T(0) ? r_px : T(0);
+  c.roi = PRange{0, 0, 0, 0};
+  c.roiw = 0;
+  c.npix = 0;
+#pragma unroll
+  for (int sl = 0; sl < SLOTS; ++sl) c.aug[sl] = T(0);
+  if (!c.act) return;
+  const ObjCamRec<T> &r = rec[(long)i * b.n_cam + c.k];
+  const PRange roi = roi_bound<T>(b.H, b.W, r.u, r.v, dpos * (T)p.ratio_augimg,
+                                  dpos * (T)p.ratio_augimg);
+  c.roi = roi;
+  if (roi.empty()) return;
+  c.roiw = roi.cmax - roi.cmin;
+  c.npix = (roi.rmax - roi.rmin) * c.roiw;
+  const T rP[5] = {r.P[0], r.P[1], r.P[2], r.P[3], r.P[4]};
+#pragma unroll
+  for (int sl = 0; sl < SLOTS; ++sl) {
+    const int idx = sl * LPG + sub;
+    if (idx < c.npix) {
+      const int row = roi.rmin + idx / c.roiw;
+      const int col = roi.cmin + idx % c.roiw;
+      T a = b.res[c.k * HW + (long)row * b.W + col];
+      if (row >= r.rmin && row < r.rmax && col >= r.cmin && col < r.cmax)
+        a += gauss_int<T>(rP, r.u, r.v, row, col);
+      c.aug[sl] = a > T(0) ? a : T(0);
+    }
+  }
+}
+
 template <typename T, int NC, int OPW, int SLOTS>
 __global__ void __launch_bounds__(kWave)
 k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
@@ -389,35 +439,7 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
   const int ksafe = c.cam_ok ? c.k : 0;
   for (int i = blockIdx.x * OPW + lane / LPO; i < n; i += gridDim.x * OPW) {
     T X[3] = {(T)b.xyz[3L * i], (T)b.xyz[3L * i + 1], (T)b.xyz[3L * i + 2]};
-    // buildROIInfo (:279-328) with the object's current projection
-    c.roi = PRange{0, 0, 0, 0};
-    c.roiw = 0;
-    c.npix = 0;
-#pragma unroll
-    for (int sl = 0; sl < SLOTS; ++sl) c.aug[sl] = T(0);
-    if (c.act) {
-      const ObjCamRec<T> &r = rec[(long)i * nc + ksafe];
-      const PRange roi = roi_bound<T>(b.H, b.W, r.u, r.v, dpos * (T)p.ratio_augimg,
-                                      dpos * (T)p.ratio_augimg);
-      c.roi = roi;
-      if (!roi.empty()) {
-        c.roiw = roi.cmax - roi.cmin;
-        c.npix = (roi.rmax - roi.rmin) * c.roiw;
-        const T rP[5] = {r.P[0], r.P[1], r.P[2], r.P[3], r.P[4]};
-#pragma unroll
-        for (int sl = 0; sl < SLOTS; ++sl) {
-          const int idx = sl * LPG + sub;
-          if (idx < c.npix) {
-            const int row = roi.rmin + idx / c.roiw;
-            const int col = roi.cmin + idx % c.roiw;
-            T a = b.res[ksafe * HW + (long)row * b.W + col];
-            if (row >= r.rmin && row < r.rmax && col >= r.cmin && col < r.cmax)
-              a += gauss_int<T>(rP, r.u, r.v, row, col);
-            c.aug[sl] = a > T(0) ? a : T(0);
-          }
-        }
-      }
-    }
+    build_roi<T, LPG, SLOTS>(b, p, rec, i, c);
     // shakeOneObject (:409-665); tmp keeps the last evaluated coordinate on
     // each axis, as the reference's reused temporary object does
     T tmp[3] = {X[0], X[1], X[2]};
@@ -640,6 +662,85 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
 }
 
 // --------------------------------------------------------------------------
+// learned shake proposal
+// --------------------------------------------------------------------------
+// Network input of every object: per camera the 11x11 window of the
+// augmented residual max(0, residual + own projection in its box) centred on
+// the rounded projection, divided by the OTF amplitude there; 4 x 121 values
+// in camera order, zero outside the image and for inactive cameras, then 28
+// zeros of padding (bf16 [n][512]).
+constexpr int kPatch = 11, kPatchHalf = 5, kPatchIn = 512;
+
+template <typename T>
+__global__ void k_patch_gather(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
+                               __bf16 *__restrict__ X) {
+  const long n = *b.n_obj;
+  const int nc = b.n_cam;
+  const long HW = (long)b.H * b.W;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n * kPatchIn;
+       t += (long)gridDim.x * blockDim.x) {
+    const long i = t / kPatchIn;
+    const int c = (int)(t % kPatchIn);
+    float v = 0.0f;
+    const int k = c / (kPatch * kPatch), pix = c % (kPatch * kPatch);
+    if (k < 4 && k < nc && p.active[k]) {
+      const ObjCamRec<T> &r = rec[i * nc + k];
+      if (fabs((double)r.u) < 1e6 && fabs((double)r.v) < 1e6) {
+        const int cu = (int)round((double)r.u), cv = (int)round((double)r.v);
+        const int row = cv + pix / kPatch - kPatchHalf, col = cu + pix % kPatch - kPatchHalf;
+        if (row >= 0 && row < b.H && col >= 0 && col < b.W) {
+          T a = b.res[k * HW + (long)row * b.W + col];
+          if (row >= r.rmin && row < r.rmax && col >= r.cmin && col < r.cmax)
+            a += gauss_int<T>(r.P, r.u, r.v, row, col);
+          a = a > T(0) ? a : T(0);
+          const T amp = r.P[0] > T(1e-6) ? r.P[0] : T(1e-6);
+          v = (float)(a / amp);
+        }
+      }
+    }
+    X[t] = (__bf16)v;
+  }
+}
+
+// Take the network's step X + clamp(dX) * voxel when it lowers the shake
+// residue (calShakeResidue on the same ROI); otherwise keep X.
+template <typename T, int NC, int OPW, int SLOTS>
+__global__ void __launch_bounds__(kWave)
+k_propose(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
+          const float *__restrict__ prop) {
+  constexpr int LPO = kWave / OPW;
+  const int n = *b.n_obj;
+  const int lane = lane_id();
+  const T r_px = (T)p.r_px;
+  CamLane<T, SLOTS> c;
+  c.obase = (lane / LPO) * LPO;
+  c.omask = (LPO == kWave ? ~0ull : ((1ull << LPO) - 1)) << c.obase;
+  c.k = (lane % LPO) / (LPO / NC);
+  c.cam_ok = c.k < b.n_cam;
+  c.act = c.cam_ok && p.active[c.k];
+  const T lim = (T)p.max_step_vox, vox = (T)p.voxel_mm;
+  for (int i = blockIdx.x * OPW + lane / LPO; i < n; i += gridDim.x * OPW) {
+    T X[3] = {(T)b.xyz[3L * i], (T)b.xyz[3L * i + 1], (T)b.xyz[3L * i + 2]};
+    build_roi<T, LPO / NC, SLOTS>(b, p, rec, i, c);
+    T ue, ve;
+    const T f0 = eval_residue<T, NC, OPW, SLOTS>(b, c, r_px, X, ue, ve);
+    T Xn[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      T s = (T)prop[4L * i + d];
+      s = s > lim ? lim : (s < -lim ? -lim : s);
+      Xn[d] = X[d] + s * vox;
+    }
+    const T f1 = eval_residue<T, NC, OPW, SLOTS>(b, c, r_px, Xn, ue, ve);
+    if (f1 < f0 && lane == c.obase) {
+      b.xyz[3L * i] = (double)Xn[0];
+      b.xyz[3L * i + 1] = (double)Xn[1];
+      b.xyz[3L * i + 2] = (double)Xn[2];
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
 // repeated objects + ghost threshold
 // --------------------------------------------------------------------------
 template <typename T>
@@ -836,6 +937,15 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
   const int roi_px = roi_w * roi_w;
   const int gO = grid_for(b.cap_obj, 256, 2048);
   k_init_score<T><<<gO, 256, 0, st>>>(b);
+  if (p.mlp && b.patches && b.prop && nc == 4 && roi_px <= 64) {
+    // learned proposal from the matched positions, then the plain iterations
+    residual_pass<T>(b, b.img, b.res, b.xyz, b.n_obj, rec, b.uv, act, 0, r_px, st);
+    k_patch_gather<T><<<grid_for((long)b.cap_obj * kPatchIn, 256, 8192), 256, 0, st>>>(
+        b, p, rec, (__bf16 *)b.patches);
+    mlp_infer_device(p.mlp, b.patches, b.n_obj, b.cap_obj, b.prop, st);
+    constexpr int O = shake_opw<T>();
+    k_propose<T, 4, O, 4 * O><<<256 * 16, kWave, 0, st>>>(b, p, rec, b.prop);
+  }
   double delta = p.shake_width;
   double dmin = p.shakewidth_min > 0 ? p.shakewidth_min : delta / 20;
   for (int it = 0; it < p.n_shake_loop; ++it) {
@@ -930,6 +1040,26 @@ void launch_residual_ipr(const ShakeBuffers<T> &b, const int32_t *n_keep,
                    (T)r_px, st);
 }
 
+template <typename T>
+void launch_patches(const ShakeBuffers<T> &b, const ShakeParams &p, void *X, hipStream_t st) {
+  CamList act{0, {0}};
+  for (int k = 0; k < b.n_cam; ++k)
+    if (p.active[k]) act.id[act.n++] = k;
+  ObjCamRec<T> *rec = reinterpret_cast<ObjCamRec<T> *>(b.rec);
+  residual_pass<T>(b, b.img, b.res, b.xyz, b.n_obj, rec, b.uv, act, 0, (T)p.r_px, st);
+  k_patch_gather<T><<<grid_for((long)b.cap_obj * kPatchIn, 256, 8192), 256, 0, st>>>(
+      b, p, rec, (__bf16 *)X);
+}
+
+template <typename T>
+void launch_render(const ShakeBuffers<T> &b, const double *xyz, const double *amp,
+                   const int32_t *n_ptr, double r_px, T *out, hipStream_t st) {
+  CamList all{0, {0}};
+  for (int k = 0; k < b.n_cam; ++k) all.id[all.n++] = k;
+  ObjCamRec<T> *rec = reinterpret_cast<ObjCamRec<T> *>(b.rec);
+  residual_pass<T>(b, out, out, xyz, n_ptr, rec, nullptr, all, 2, (T)r_px, st, amp);
+}
+
 size_t shake_rec_bytes(bool f64) {
   return f64 ? sizeof(ObjCamRec<double>) : sizeof(ObjCamRec<float>);
 }
@@ -940,7 +1070,12 @@ size_t shake_rec_bytes(bool f64) {
   template void launch_residual_ipr<T>(const ShakeBuffers<T> &, const int32_t *,  \
                                        const double *, T *, double, hipStream_t); \
   template void launch_filter<T>(const ShakeBuffers<T> &, double *, T *,          \
-                                 int32_t *, hipStream_t);
+                                 int32_t *, hipStream_t);                          \
+  template void launch_patches<T>(const ShakeBuffers<T> &, const ShakeParams &,   \
+                                  void *, hipStream_t);                            \
+  template void launch_render<T>(const ShakeBuffers<T> &, const double *,         \
+                                 const double *, const int32_t *, double, T *,     \
+                                 hipStream_t);
 INST(float)
 INST(double)
 #undef INST

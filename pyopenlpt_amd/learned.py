@@ -121,3 +121,81 @@ class ShakeNet:
     def load(self, path: str):
         with np.load(path, allow_pickle=False) as z:
             self.set_params(z["params"])
+
+
+# ---------------------------------------------------------------------------
+# training data: frames drawn by the projection kernel
+# ---------------------------------------------------------------------------
+def training_frame(ctx, params, rng: np.random.Generator, vox: float,
+                   ppp_range=(0.01, 0.08), jitter_vox: float = 0.5, noise: float = 10.0):
+    """One synthetic frame rendered on the GPU (olpt_render, 5x5 boxes,
+    amplitudes U(800, 1200), Gaussian noise, clip, truncate), matched by the
+    library's peaks + stereo; every matched object becomes a sample.
+
+    Objects within 1 voxel of a true particle are real: their input position
+    is moved by U(-jitter, jitter) voxels per axis and the target is the
+    offset back to the truth (voxels).  The others are ghosts (target 0,
+    label 1).  Returns (X bf16 [n][512], T float32 [n][4], n_real, n_truth).
+    """
+    from scipy.spatial import cKDTree
+    from . import synth as S
+    from ._lib import OLPT_F32
+    H, W = ctx.H, ctx.W
+    n_truth = int(rng.uniform(*ppp_range) * H * W)
+    seed = int(rng.integers(1 << 30))
+    truth = (S.clustered_particles(n_truth, seed) if rng.random() < 0.5
+             else S.uniform_particles(n_truth, seed))
+    amp = rng.uniform(800.0, 1200.0, n_truth)
+    dev = torch.device("cuda", ctx.device)
+    d_xyz = torch.from_numpy(np.ascontiguousarray(truth, np.float64)).to(dev)
+    d_amp = torch.from_numpy(amp).to(dev)
+    img = ctx.render(d_xyz, d_amp, r_px=2.5).float()
+    g = torch.Generator(device=dev).manual_seed(seed)
+    img = torch.floor(torch.clamp(img + noise * torch.randn(img.shape, device=dev, generator=g),
+                                  0.0, 65535.0)).contiguous()
+    pts = [ctx.find_peaks(img[k].contiguous(), OLPT_F32, params) for k in range(ctx.n_cam)]
+    obj = ctx.stereo_match(pts, params, candidates=False)["obj_xyz"]
+    if len(obj) == 0:
+        return None
+    d, idx = cKDTree(truth).query(obj)
+    real = d < vox
+    jit = rng.uniform(-jitter_vox, jitter_vox, obj.shape) * vox
+    x_in = obj + np.where(real[:, None], jit, 0.0)
+    T = np.zeros((len(obj), 4), np.float32)
+    T[real, :3] = (truth[idx[real]] - x_in[real]) / vox
+    T[~real, 3] = 1.0
+    X = ctx.shake_patches(img, OLPT_F32, params,
+                          torch.from_numpy(np.ascontiguousarray(x_in)).to(dev))
+    return X, torch.from_numpy(T).to(dev), int(real.sum()), n_truth
+
+
+def make_training_set(ctx, params, vox: float, n_particles: int = 2_000_000, seed: int = 9001,
+                      **kw):
+    """Frames until n_particles true particles have been drawn."""
+    rng = np.random.default_rng(seed)
+    xs, ts, n_seen, n_real = [], [], 0, 0
+    while n_seen < n_particles:
+        r = training_frame(ctx, params, rng, vox, **kw)
+        if r is None:
+            continue
+        xs.append(r[0])
+        ts.append(r[1])
+        n_real += r[2]
+        n_seen += r[3]
+    return torch.cat(xs), torch.cat(ts), dict(samples=sum(len(x) for x in xs), real=n_real,
+                                              particles=n_seen, frames=len(xs))
+
+
+def train(net: ShakeNet, X: torch.Tensor, T: torch.Tensor, steps: int, batch: int = 8192,
+          lr: float = 1e-3, ghost_weight: float = 0.5, seed: int = 0) -> list[float]:
+    """Adam on random batches; the learning rate decays linearly to 10%."""
+    g = torch.Generator(device=X.device).manual_seed(seed)
+    n = X.shape[0]
+    losses = []
+    for s in range(steps):
+        idx = torch.randint(0, n, (min(batch, n),), device=X.device, generator=g)
+        xb = X[idx].contiguous()
+        tb = T[idx].contiguous()
+        rate = lr * (1.0 - 0.9 * s / max(1, steps - 1))
+        losses.append(net.train_step(xb, tb, rate, ghost_weight))
+    return losses
