@@ -199,3 +199,70 @@ def train(net: ShakeNet, X: torch.Tensor, T: torch.Tensor, steps: int, batch: in
         rate = lr * (1.0 - 0.9 * s / max(1, steps - 1))
         losses.append(net.train_step(xb, tb, rate, ghost_weight))
     return losses
+
+
+# ---------------------------------------------------------------------------
+# MLP throughput on MFMA (python -m pyopenlpt_amd.learned)
+# ---------------------------------------------------------------------------
+MACS_PER_ROW = K0 * H + 2 * H * H + H * N4   # padded GEMM shapes the kernels run
+
+
+def _bench(argv=None) -> int:
+    import argparse
+    import json
+    import time
+    ap = argparse.ArgumentParser(description="time the proposal MLP's training steps and "
+                                             "inference (BF16 MFMA, FP32 accumulate)")
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--infer-rows", type=int, default=40960)
+    args = ap.parse_args(argv)
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev).manual_seed(0)
+    net = ShakeNet(max_batch=args.batch, seed=0)
+    x = torch.zeros((args.batch, K0), device=dev)
+    x[:, :N_IN] = torch.rand((args.batch, N_IN), device=dev, generator=g)
+    x = x.to(torch.bfloat16).contiguous()
+    t = torch.zeros((args.batch, 4), device=dev)
+    t[:, :3] = 0.3 * torch.randn((args.batch, 3), device=dev, generator=g)
+    t[:, 3] = (torch.rand(args.batch, device=dev, generator=g) < 0.3).float()
+    t = t.contiguous()
+    for _ in range(10):
+        net.train_step(x, t, 1e-4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        net.train_step(x, t, 1e-4)
+    torch.cuda.synchronize()
+    train_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # forward 2 B MACs, backward 4 B MACs less the unused input-gradient GEMM
+    train_flop = 6.0 * args.batch * MACS_PER_ROW - 2.0 * args.batch * K0 * H
+    xi = torch.zeros((args.infer_rows, K0), device=dev)
+    xi[:, :N_IN] = torch.rand((args.infer_rows, N_IN), device=dev, generator=g)
+    xi = xi.to(torch.bfloat16).contiguous()
+    y = net.infer(xi)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        net.infer(xi, y)
+    e1.record()
+    torch.cuda.synchronize()
+    infer_ms = e0.elapsed_time(e1) / 20
+    infer_flop = 2.0 * args.infer_rows * MACS_PER_ROW
+    peak = 2500.0  # dense BF16 TFLOP/s
+    out = {"train_ms_per_step": round(train_ms, 4),
+           "train_tflops": round(train_flop / train_ms / 1e9, 2),
+           "infer_ms": round(infer_ms, 4), "infer_rows": args.infer_rows,
+           "infer_tflops": round(infer_flop / infer_ms / 1e9, 2),
+           "peak_tflops_bf16_dense": peak, "batch": args.batch}
+    out["train_frac"] = round(out["train_tflops"] / peak, 4)
+    out["infer_frac"] = round(out["infer_tflops"] / peak, 4)
+    print(json.dumps(out), flush=True)
+    net.close()
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(_bench())
