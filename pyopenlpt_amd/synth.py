@@ -300,6 +300,20 @@ def make_otf(n_cam: int, abca=(125.0, 1.5, 1.5, 0.0), half: float = 20.0,
     return o, arrs
 
 
+def make_otf_grid(n_cam: int, shape: tuple, boundary, a, b, c, alpha):
+    """OTF on an nx x ny x nz grid from per-node arrays [n_cam][nx*ny*nz]
+    (node index ix*ny*nz + iy*nz + iz).  Returns (struct, keepalive arrays)."""
+    nx, ny, nz = shape
+    ng = nx * ny * nz
+    arrs = [np.ascontiguousarray(np.asarray(v, np.float64).reshape(n_cam * ng)) for v in (a, b, c, alpha)]
+    o = OlptOtf()
+    o.n_cam, o.nx, o.ny, o.nz = n_cam, nx, ny, nz
+    for i, v in enumerate(boundary):
+        o.boundary[i] = v
+    o.a, o.b, o.c, o.alpha = [x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for x in arrs]
+    return o, arrs
+
+
 def workload_c1(seed: int = 1234, n: int = 2000, size: int = 512) -> Workload:
     """CPU-parity case: 512^2 u8, 2000 uniform particles, a=125, b=c=1.5,
     noise sigma 2, k1 = 1e-8 (loads as undistorted), 4 loops x 4 shakes."""

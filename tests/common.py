@@ -58,6 +58,12 @@ class Case:
         return p
 
     def otf(self):
+        """The case's OTF: a full grid when the fixture stores one, else the
+        uniform 2x2x2 grid of otf_abca."""
+        if "otf_a" in self.z:
+            shape = tuple(int(v) for v in self.z["otf_shape"])
+            return S.make_otf_grid(self.n_cam, shape, self.z["otf_boundary"], self.z["otf_a"],
+                                   self.z["otf_b"], self.z["otf_c"], self.z["otf_alpha"])
         return S.make_otf(self.n_cam, self.abca)
 
     def peaks(self):
@@ -98,10 +104,20 @@ def oracle_stereo(cams, pts, params, active=None):
                     obj_ids=oi[:no])
 
 
+def _otf(n_cam, otf):
+    """An OlptOtf (or (struct, keepalive) pair) as given, or the uniform grid
+    of an (a, b, c, alpha) tuple."""
+    if isinstance(otf, tuple) and len(otf) == 2 and isinstance(otf[0], S.OlptOtf):
+        return otf
+    if isinstance(otf, S.OlptOtf):
+        return otf, None
+    return S.make_otf(n_cam, otf)
+
+
 def oracle_shake(cams, images, params, abca, xyz, active=None, residual=True):
     L = O.load_port()
     n_cam, H, W = images.shape
-    otf, _k = S.make_otf(n_cam, abca)
+    otf, _k = _otf(n_cam, abca)
     n = len(xyz)
     xo = np.zeros((max(n, 1), 3))
     fl = np.zeros(max(n, 1), np.uint8)
@@ -120,7 +136,7 @@ def oracle_shake(cams, images, params, abca, xyz, active=None, residual=True):
 def oracle_ipr(cams, images, params, abca):
     L = O.load_port()
     n_cam, H, W = images.shape
-    otf, _k = S.make_otf(n_cam, abca)
+    otf, _k = _otf(n_cam, abca)
     cap = 1 << 19
     ox = np.zeros((cap, 3))
     ou = np.zeros((cap, n_cam, 2))

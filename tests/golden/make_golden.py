@@ -14,7 +14,8 @@ frames and stores inputs and stage outputs as compressed .npz fixtures:
                                  fixed sample of pixels)
   ipr_xyz / ipr_uv / ipr_loops   IPR::runIPR
 
-Usage: python tests/golden/make_golden.py  (needs the reference checkout)
+Usage: python tests/golden/make_golden.py [case ...]  (needs the reference checkout;
+no names = every case)
 """
 from __future__ import annotations
 
@@ -57,17 +58,33 @@ def params_to_array(p) -> np.ndarray:
     return np.array(v)
 
 
-def run_case(name: str, w, otf_abca, *, reduced: int = 0, sample: int = 4096):
+WANT = set(sys.argv[1:])   # case names to write; empty = all
+
+
+def run_case(name: str, w, otf_abca, *, reduced: int = 0, sample: int = 4096, grid=None):
+    """`grid` = dict(shape, boundary, a, b, c, alpha) gives a full OTF grid
+    (arrays [n_cam][nx*ny*nz]); otherwise the uniform 2x2x2 grid of otf_abca."""
+    if WANT and name not in WANT:
+        return
     lib = O.load_ref()
     p = w.params
     p.n_cam_reduced = reduced
     H, W = w.images.shape[1:]
     n_cam = len(w.cams)
     cams = O.cams_array(w.cams)
-    otf, _keep = S.make_otf(n_cam, otf_abca)
+    if grid is None:
+        otf, _keep = S.make_otf(n_cam, otf_abca)
+    else:
+        otf, _keep = S.make_otf_grid(n_cam, grid["shape"], grid["boundary"], grid["a"],
+                                     grid["b"], grid["c"], grid["alpha"])
     imgs = w.images.astype(np.float64).copy()
     out = dict(images=w.images, cameras=cams_to_array(w.cams), params=params_to_array(p),
                otf_abca=np.array(otf_abca, dtype=np.float64), truth=w.truth)
+    if grid is not None:
+        out.update(otf_shape=np.array(grid["shape"], np.int32),
+                   otf_boundary=np.array(grid["boundary"], np.float64),
+                   **{f"otf_{k}": np.asarray(grid[k], np.float64).reshape(n_cam, -1)
+                      for k in ("a", "b", "c", "alpha")})
     # peaks
     pts = []
     for c in range(n_cam):
@@ -178,6 +195,23 @@ def main():
     w.params.n_loop_ipr = 2
     w.params.n_loop_ipr_reduced = 2
     run_case("golden_reduced_256", w, (125.0, 1.5, 1.5, 0.0), reduced=1)
+    # non-uniform OTF: a 3 x 4 x 2 grid (unequal axes, so a swapped index
+    # order changes the interpolated values) whose box [-18,18] x [-20,20] x
+    # [-15,15] is smaller than the particle volume (clamping at the box);
+    # a, b, c and alpha vary per camera and node, alpha up to +-0.3 rad
+    shape = (3, 4, 2)
+    ng = int(np.prod(shape))
+    g = np.random.default_rng(51)
+    grid = dict(shape=shape, boundary=(-18.0, 18.0, -20.0, 20.0, -15.0, 15.0),
+                a=g.uniform(105.0, 145.0, (4, ng)), b=g.uniform(1.2, 1.8, (4, ng)),
+                c=g.uniform(1.2, 1.8, (4, ng)), alpha=g.uniform(-0.3, 0.3, (4, ng)))
+    cams = S.make_rig(256, 256, max_intensity=255)
+    pts = S.uniform_particles(600, 52)
+    img = S.render(cams, pts, g.uniform(105.0, 145.0, 600), 1.5, 1.5, 0.0, 2.0, 53, np.uint8)
+    p = S.default_params(vox, min_obj_int=30)
+    run_case("golden_otfgrid_256", S.Workload("otfgrid", cams, img, pts, p,
+                                              (125.0, 1.5, 1.5, 0.0), (-20, 20) * 3, vox),
+             (125.0, 1.5, 1.5, 0.0), grid=grid)
 
 
 if __name__ == "__main__":

@@ -112,9 +112,8 @@ def test_residual_matches_oracle(case):
     got = ctx.residual(d, code, case.params, xyz)
     import ctypes
     import oracle as O
-    from pyopenlpt_amd import synth as S
     L = O.load_port()
-    otf, _k = S.make_otf(case.n_cam, case.abca)
+    otf, _k = case.otf()
     want = np.zeros((case.n_cam, case.H, case.W))
     L.orc_residual(O.cams_array(case.cams), case.n_cam,
                    O.dptr(case.images.astype(np.float64)), case.H, case.W,
@@ -151,10 +150,15 @@ def test_run_ipr_f32_counts(case):
 # ---------------------------------------------------------------------------
 # full-size workload (1024^2 u16, 20,000 particles) against the oracle
 # ---------------------------------------------------------------------------
-@pytest.fixture(scope="module")
-def c2(gpu_available):
+@pytest.fixture(scope="module", params=[0, 1], ids=["all_cams", "reduced1"])
+def c2(request, gpu_available):
+    """n_cam_reduced = 1 adds two passes over each of the four 3-camera
+    subsets after the four all-camera loops (IPR.cpp:194-240);
+    on this workload they lift the count from 18,231 to 19,731."""
     from pyopenlpt_amd import synth as S
     w = S.workload_c2()
+    w.params.n_cam_reduced = request.param
+    w.params.n_loop_ipr_reduced = 2
     xyz, uv, loops = oracle_ipr(w.cams, w.images, w.params, w.otf_abca)
     return w, xyz, loops
 

@@ -31,7 +31,7 @@ def test_stereo_bit_exact(case):
 
 
 def test_shake_bit_exact(case):
-    xo, fl, sc, res = oracle_shake(case.cams, case.images, case.params, case.abca,
+    xo, fl, sc, res = oracle_shake(case.cams, case.images, case.params, case.otf(),
                                    case.z["sm_obj_xyz"])
     assert np.array_equal(xo, case.z["sh_xyz"])
     assert np.array_equal(fl, case.z["sh_flags"])
@@ -44,7 +44,7 @@ def test_shake_bit_exact(case):
 def test_shake_planted_flags(case):
     if "shf_flags" not in case.z:
         pytest.skip("no planted-flag case")
-    xo, fl, sc, _ = oracle_shake(case.cams, case.images, case.params, case.abca,
+    xo, fl, sc, _ = oracle_shake(case.cams, case.images, case.params, case.otf(),
                                  case.z["shf_xyz_in"], residual=False)
     assert np.array_equal(fl, case.z["shf_flags"])
     assert (fl & 1).sum() > 0 and (fl & 2).sum() > 0   # both Ghost and Repeated occur
@@ -53,7 +53,7 @@ def test_shake_planted_flags(case):
 
 
 def test_run_ipr_bit_exact(case):
-    xyz, uv, loops = oracle_ipr(case.cams, case.images, case.params, case.abca)
+    xyz, uv, loops = oracle_ipr(case.cams, case.images, case.params, case.otf())
     assert np.array_equal(loops, case.z["ipr_loops"])
     assert np.array_equal(xyz, case.z["ipr_xyz"])
     assert np.array_equal(uv, case.z["ipr_uv"])
