@@ -207,6 +207,80 @@ olpt_status olpt_render(olpt_ctx *ctx, const double *d_xyz, const double *d_amp,
 olpt_status olpt_mlp_get_grads(olpt_mlp *mlp, float *grads, int32_t n);
 const char *olpt_mlp_last_error(void);
 
+/* ---- shake-the-box tracking (STB) ----------------------------------- */
+/* STBParam + PFParam + the frame range of BasicSetting (Config.h:14-37,
+ * :47-60, :98-101).  Lengths in mm. */
+typedef struct olpt_stb_params {
+  int32_t frame_start, frame_end;
+  int32_t n_initial_frames;        /* [4], at least 4 (Config.cpp:197-199) */
+  double radius_search_obj_mm;     /* initial-phase linking radius */
+  double radius_search_track_mm;   /* convergence-phase neighbour radius */
+  double limit[6];                 /* view volume, as olpt_ipr_params.limit */
+  int32_t pf_nx, pf_ny, pf_nz;     /* predictive-field grid */
+  double pf_r_mm;                  /* predictive-field search half-width */
+  int32_t pf_nbin[3];              /* [11 11 11] displacement bins */
+  int32_t pf_smooth;               /* [1] Gaussian smoothing of the field */
+  double pf_sigma[3];              /* [1 1 1] grid nodes */
+} olpt_stb_params;
+
+typedef struct olpt_stb olpt_stb;
+enum { OLPT_TRACK_SHORT_ACTIVE = 0, OLPT_TRACK_LONG_ACTIVE = 1,
+       OLPT_TRACK_LONG_INACTIVE = 2, OLPT_TRACK_EXIT = 3 };
+
+/* STB(setting, "Tracer", cfg) (STB.cpp:6-38) on a context whose cameras
+ * and OTF are set.  ipr: the IPR / stereo / shake parameters of the object
+ * configuration.  output_path (may be NULL or "") receives the files the
+ * reference writes: PredField_<f>_<f+1>.csv, InitialTrack/ and
+ * ConvergeTrack/ track files (STB.cpp:9-11, :124-157, :66-89). */
+olpt_status olpt_stb_create(olpt_ctx *ctx, const olpt_stb_params *stb,
+                            const olpt_ipr_params *ipr, const char *output_path,
+                            olpt_stb **out);
+olpt_status olpt_stb_destroy(olpt_stb *stb);
+/* STB::processFrame (STB.cpp:40-89): d_imgs [n_cam][n_row][n_col]. */
+olpt_status olpt_stb_process_frame(olpt_stb *stb, int frame_id, const void *d_imgs,
+                                   int dtype);
+/* Size of one track list (OLPT_TRACK_*): tracks and total points. */
+olpt_status olpt_stb_track_count(olpt_stb *stb, int which, int32_t *n_tracks,
+                                 int32_t *n_points);
+/* The list's points, track by track in list order: track index, frame id,
+ * world position (host buffers of cap rows). */
+olpt_status olpt_stb_get_tracks(olpt_stb *stb, int which, int32_t cap, int32_t *track,
+                                int32_t *frame, double *xyz);
+/* Replace one list by n_rows points grouped by track index, as
+ * STB::loadTracks builds them (STB.cpp:1006-1050). */
+olpt_status olpt_stb_set_tracks(olpt_stb *stb, int which, int32_t n_rows,
+                                const int32_t *track, const int32_t *frame,
+                                const double *xyz);
+/* STB::saveTracksAll / loadTracksAll (STB.cpp:926-1005, :1052-1076). */
+olpt_status olpt_stb_save_tracks(olpt_stb *stb, const char *folder, int frame);
+olpt_status olpt_stb_load_tracks(olpt_stb *stb, const char *folder, int frame);
+
+/* STB::predictNext (STB.cpp:233-316) of n_tr tracks given by their last
+ * len_i = min(length, 6) points: d_pos [n_tr][6][3] (oldest first), d_len
+ * [n_tr]; d_out [n_tr][3]; d_ok[i] = 0 where the track has fewer than 3
+ * points (the reference returns null). */
+olpt_status olpt_stb_predict(olpt_ctx *ctx, const double *d_pos, const int32_t *d_len,
+                             int n_tr, double *d_out, int32_t *d_ok);
+/* STB::checkLinearFit (STB.cpp:874-924): d_pos [n_tr][4][3] holds the last
+ * min(length, 4) points, oldest first, d_len the track lengths. */
+olpt_status olpt_stb_linear_fit(olpt_ctx *ctx, int n_initial_frames, const double *d_pos,
+                                const int32_t *d_len, int n_tr, int32_t *d_ok);
+/* PredField::calPredField (PredField.cpp:21-86) between two object sets,
+ * then PredField::getDisp (:91-165) at n_q query points.  d_field
+ * [nx*ny*nz][3] (may be NULL); d_disp [n_q][3]. */
+olpt_status olpt_pred_field(olpt_ctx *ctx, const olpt_stb_params *stb,
+                            const double *d_prev, int n_prev, const double *d_curr,
+                            int n_curr, double *d_field, const double *d_q, int n_q,
+                            double *d_disp);
+/* OTF::estimateUniformOTFFromImage (OTF.cpp:16-144) for every camera from
+ * n_img frames d_frames [n_img][n_cam][n_row][n_col]: peaks of each image
+ * (at most 1000, shuffled as the reference does), a = min(mean + 2 std,
+ * max) of the peak pixels, (b, c) by least squares of -ln(I/a) over the
+ * peaks' (2r+1)^2 windows, alpha = 0.  abca [n_cam][4] (host); ok[k] = 0
+ * where an image of camera k had no peak (its row is left untouched). */
+olpt_status olpt_estimate_otf(olpt_ctx *ctx, const void *d_frames, int dtype, int n_img,
+                              const olpt_ipr_params *params, double *abca, int32_t *ok);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes
 import os
 
-from .synth import OlptCamera, OlptIprParams, OlptOtf
+from .synth import OlptCamera, OlptIprParams, OlptOtf, OlptStbParams
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libolpt_hip.so")
@@ -96,6 +96,25 @@ def lib() -> ctypes.CDLL:
     L.olpt_render.argtypes = [_P, _P, _P, _I, ctypes.c_double, _P]
     L.olpt_shake_patches.argtypes = [_P, _I32P, _P, _I, ctypes.POINTER(OlptIprParams), _P, _I, _P]
     for name in ("olpt_set_shake_model", "olpt_render", "olpt_shake_patches"):
+        getattr(L, name).restype = ctypes.c_int32
+    _SP = ctypes.POINTER(OlptStbParams)
+    _IP = ctypes.POINTER(OlptIprParams)
+    L.olpt_stb_create.argtypes = [_P, _SP, _IP, ctypes.c_char_p, ctypes.POINTER(_P)]
+    L.olpt_stb_destroy.argtypes = [_P]
+    L.olpt_stb_process_frame.argtypes = [_P, _I, _P, _I]
+    L.olpt_stb_track_count.argtypes = [_P, _I, _I32P, _I32P]
+    L.olpt_stb_get_tracks.argtypes = [_P, _I, ctypes.c_int32, _I32P, _I32P, _DP]
+    L.olpt_stb_set_tracks.argtypes = [_P, _I, ctypes.c_int32, _I32P, _I32P, _DP]
+    L.olpt_stb_save_tracks.argtypes = [_P, ctypes.c_char_p, _I]
+    L.olpt_stb_load_tracks.argtypes = [_P, ctypes.c_char_p, _I]
+    L.olpt_stb_predict.argtypes = [_P, _P, _P, _I, _P, _P]
+    L.olpt_stb_linear_fit.argtypes = [_P, _I, _P, _P, _I, _P]
+    L.olpt_pred_field.argtypes = [_P, _SP, _P, _I, _P, _I, _P, _P, _I, _P]
+    L.olpt_estimate_otf.argtypes = [_P, _P, _I, _I, _IP, _DP, _I32P]
+    for name in ("olpt_stb_create", "olpt_stb_destroy", "olpt_stb_process_frame",
+                 "olpt_stb_track_count", "olpt_stb_get_tracks", "olpt_stb_set_tracks",
+                 "olpt_stb_save_tracks", "olpt_stb_load_tracks", "olpt_stb_predict",
+                 "olpt_stb_linear_fit", "olpt_pred_field", "olpt_estimate_otf"):
         getattr(L, name).restype = ctypes.c_int32
     for name in ("olpt_mlp_create", "olpt_mlp_destroy", "olpt_mlp_set_params",
                  "olpt_mlp_get_params", "olpt_mlp_get_grads", "olpt_mlp_infer",

@@ -19,7 +19,7 @@ ARCH = os.environ.get("OLPT_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["olpt_scan.hip", "olpt_peaks.hip", "olpt_stereo.hip",
-           "olpt_shake.hip", "olpt_api.hip", "olpt_mlp.hip"]
+           "olpt_shake.hip", "olpt_api.hip", "olpt_mlp.hip", "olpt_stb.hip"]
 
 CFLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

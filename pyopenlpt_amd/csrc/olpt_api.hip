@@ -525,6 +525,22 @@ size_t tsize(olpt_ctx *c) { return c->prec == OLPT_PREC_F64 ? 8 : 4; }
 
 } // namespace
 
+namespace olpt {
+bool ctx_info(const olpt_ctx *c, CtxInfo *out) {
+  if (!c || !c->cams_set || !c->n_ws || !c->d_cam64) return false;
+  out->st = c->st;
+  out->prec = c->prec;
+  out->n_cam = c->n_cam;
+  out->H = c->H;
+  out->W = c->W;
+  out->cap_obj = c->cap_obj;
+  out->cams = c->cams;
+  out->d_cam64 = c->d_cam64;
+  return true;
+}
+int32_t set_error(int32_t s, const char *msg) { return fail(s, msg); }
+} // namespace olpt
+
 // ===========================================================================
 // extern "C"
 // ===========================================================================

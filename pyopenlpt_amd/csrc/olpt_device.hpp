@@ -48,7 +48,7 @@ template <typename T> CamT<T> make_cam(const olpt_camera &c) {
 
 // Camera::distort (Camera.cpp:913-966): normalized (x,y) -> pixel (u,v)
 template <typename T>
-__device__ __forceinline__ void distort(const CamT<T> &c, T x, T y, T &u, T &v) {
+__host__ __device__ __forceinline__ void distort(const CamT<T> &c, T x, T y, T &u, T &v) {
   if (c.is_distorted) {
     const T *k = c.k;
     const int nd = c.n_dist;
@@ -78,8 +78,8 @@ __device__ __forceinline__ void distort(const CamT<T> &c, T x, T y, T &u, T &v) 
 // Camera::project, pinhole branch = worldToUndistImg (Camera.cpp:680-689)
 // followed by distort.
 template <typename T>
-__device__ __forceinline__ void project(const CamT<T> &c, const T X[3], T &u,
-                                        T &v) {
+__host__ __device__ __forceinline__ void project(const CamT<T> &c, const T X[3], T &u,
+                                                 T &v) {
   T t[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)

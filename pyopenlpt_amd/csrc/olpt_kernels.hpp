@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "olpt_c.h"
 #include "olpt_device.hpp"
 
 namespace olpt {
@@ -226,5 +227,17 @@ void mlp_infer_device(const void *mlp, const void *X, const int32_t *d_n, int ca
 template <typename T>
 void launch_convert_image(const void *src, int dtype, long n, T *dst,
                           hipStream_t st);
+
+// What modules built on a context (tracking, OTF estimate) read from it.
+struct CtxInfo {
+  hipStream_t st;
+  int prec, n_cam, H, W, cap_obj;
+  const olpt_camera *cams;     // host copies
+  const CamT<double> *d_cam64; // device copies (FP64)
+};
+// False when the context has no workspace or cameras yet.
+bool ctx_info(const olpt_ctx *c, CtxInfo *out);
+// Set the thread-local message read by olpt_last_error and return s.
+int32_t set_error(int32_t s, const char *msg);
 
 } // namespace olpt

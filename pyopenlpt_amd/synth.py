@@ -69,6 +69,22 @@ class OlptIprParams(ctypes.Structure):
 # ---------------------------------------------------------------------------
 
 
+class OlptStbParams(ctypes.Structure):
+    """olpt_stb_params (include/olpt_c.h): STBParam + PFParam + frame range."""
+    _fields_ = [
+        ("frame_start", ctypes.c_int32), ("frame_end", ctypes.c_int32),
+        ("n_initial_frames", ctypes.c_int32),
+        ("radius_search_obj_mm", ctypes.c_double),
+        ("radius_search_track_mm", ctypes.c_double),
+        ("limit", ctypes.c_double * 6),
+        ("pf_nx", ctypes.c_int32), ("pf_ny", ctypes.c_int32), ("pf_nz", ctypes.c_int32),
+        ("pf_r_mm", ctypes.c_double),
+        ("pf_nbin", ctypes.c_int32 * 3),
+        ("pf_smooth", ctypes.c_int32),
+        ("pf_sigma", ctypes.c_double * 3),
+    ]
+
+
 @dataclasses.dataclass
 class PinholeCamera:
     n_row: int
@@ -358,3 +374,55 @@ def workload_c3_frame(frame: int, seed: int = 1234, ppp: float = 0.05,
                        min_obj_int=240)
     return Workload(f"c3_1024_u16_ppp{ppp}_f{frame}", cams, img, pts, p,
                     (1000.0, 1.2, 1.2, 0.0), (-20, 20) * 3, vox)
+
+
+# ---------------------------------------------------------------------------
+# Tracking sequences: particles advected by an ABC flow
+# ---------------------------------------------------------------------------
+def abc_velocity(p: np.ndarray, A: float = 1.0, B: float = 1.0, C: float = 1.0,
+                 period_mm: float = 40.0) -> np.ndarray:
+    """Arnold-Beltrami-Childress flow with period 2*pi <-> period_mm."""
+    k = 2.0 * math.pi / period_mm
+    x, y, z = p[:, 0], p[:, 1], p[:, 2]
+    return np.stack([A * np.sin(k * z) + C * np.cos(k * y),
+                     B * np.sin(k * x) + A * np.cos(k * z),
+                     C * np.sin(k * y) + B * np.cos(k * x)], axis=1)
+
+
+def abc_sequence(n: int, n_frames: int, seed: int, max_disp_mm: float,
+                 half: float = 20.0) -> list[tuple[np.ndarray, np.ndarray]]:
+    """Positions [n][3] and particle ids per frame: RK4 steps of the ABC
+    flow scaled so that no particle moves more than max_disp_mm per frame
+    (|u| <= 2 sqrt(3)); a particle leaving the cube is replaced by a new one
+    (new id) at a uniform random position."""
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(-half, half, (n, 3))
+    ids = np.arange(n)
+    next_id = n
+    dt = max_disp_mm / (2.0 * math.sqrt(3.0))
+    out = []
+    for _ in range(n_frames):
+        out.append((p.copy(), ids.copy()))
+        k1 = abc_velocity(p)
+        k2 = abc_velocity(p + 0.5 * dt * k1)
+        k3 = abc_velocity(p + 0.5 * dt * k2)
+        k4 = abc_velocity(p + dt * k3)
+        p = p + dt / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
+        gone = np.any(np.abs(p) > half, axis=1)
+        m = int(gone.sum())
+        if m:
+            p[gone] = rng.uniform(-half, half, (m, 3))
+            ids[gone] = np.arange(next_id, next_id + m)
+            next_id += m
+    return out
+
+
+def render_sequence(cams, seq, amp_range=(105.0, 145.0), b=1.5, c=1.5, noise=2.0,
+                    seed=0, dtype=np.uint8) -> np.ndarray:
+    """Frames [F][n_cam][H][W]: each particle keeps its amplitude (drawn per
+    id) across frames."""
+    rng = np.random.default_rng(seed)
+    max_id = max(int(ids.max()) for _, ids in seq) + 1
+    amp = rng.uniform(amp_range[0], amp_range[1], max_id)
+    return np.stack([render(cams, p, amp[ids], b, c, 0.0, noise, seed + 1 + f, dtype)
+                     for f, (p, ids) in enumerate(seq)])

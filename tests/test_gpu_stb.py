@@ -1,0 +1,134 @@
+"""Shake-the-box tracking on the GPU against the reference's own C++
+(golden vectors written by tests/golden/make_golden_stb.py from the compiled
+reference: STB.cpp, PredField.cpp, Track.cpp driven by its config reader).
+
+Bars:
+* predictNext: bit-exact positions (same FP64 operations in the same order).
+* checkLinearFit: identical decisions.
+* PredField: field and getDisp within 1e-9 mm; the only differences are
+  the last bits of the device exp/log in the displacement histogram.
+* The whole run (config files -> OTF estimate -> 10 frames of STB): the OTF
+  estimate within 1e-12 relative; in FP64 mode the four track lists with the
+  same tracks, frames and positions (<= 1e-9 mm); in FP32 mode (shake and
+  residual in FP32) long-track counts within 2% and positions of matching
+  points within one shake step.
+"""
+import ast
+import os
+
+import numpy as np
+import pytest
+
+from common import Case
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                      "stb_abc_256.npz")
+LISTS = ("short_active", "long_active", "long_inactive", "exit")
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return dict(np.load(GOLDEN))
+
+
+@pytest.fixture(scope="module")
+def ctx(gpu_available, gold):
+    from pyopenlpt_amd.context import Context
+    cams = [Case._cam(r) for r in gold["cameras"]]
+    c = Context(cams, cams[0].n_row, cams[0].n_col, precision="f64")
+    yield c
+    c.close()
+
+
+def test_predict_next_bit_exact(ctx, gold):
+    from pyopenlpt_amd.stb import predict_next
+    pos, lens = gold["pn_pos"], gold["pn_len"]
+    win = np.zeros((len(lens), 6, 3))
+    m = np.minimum(lens, 6)
+    for i in range(len(lens)):
+        win[i, : m[i]] = pos[i, lens[i] - m[i]: lens[i]]
+    pred, ok = predict_next(ctx, win, m)
+    assert np.array_equal(ok, gold["pn_ok"])
+    sel = ok == 1
+    assert np.array_equal(pred[sel], gold["pn_pred"][sel]), np.abs(pred[sel] - gold["pn_pred"][sel]).max()
+
+
+def test_linear_fit_decisions(ctx, gold):
+    from pyopenlpt_amd.stb import check_linear_fit
+    pos, lens = gold["lf_pos"], gold["lf_len"]
+    win = np.zeros((len(lens), 4, 3))
+    for i in range(len(lens)):
+        m = min(4, lens[i])
+        win[i, :m] = pos[i, lens[i] - m: lens[i]]
+    ok = check_linear_fit(ctx, 4, win, lens)
+    assert np.array_equal(ok, gold["lf_ok"])
+    assert 0 < ok.sum() < len(ok)
+
+
+def test_pred_field(ctx, gold):
+    from pyopenlpt_amd import synth as S
+    from pyopenlpt_amd.stb import PredField
+    tr = ast.literal_eval(str(gold["tracer"]))
+    vox = float(gold["voxel_mm"])
+    sp = S.OlptStbParams()
+    for i, v in enumerate((-20, 20, -20, 20, -20, 20)):
+        sp.limit[i] = v
+    sp.pf_nx, sp.pf_ny, sp.pf_nz = tr["pf_grid"]
+    sp.pf_r_mm = tr["pf_r_vox"] * vox
+    for i in range(3):
+        sp.pf_nbin[i] = 11
+        sp.pf_sigma[i] = 1.0
+    sp.pf_smooth = 1
+    field, disp = PredField(ctx, sp).calPredField(gold["pf_prev"], gold["pf_curr"], gold["pf_q"])
+    assert np.abs(field - gold["pf_field"]).max() <= 1e-9
+    assert np.abs(disp - gold["pf_disp"]).max() <= 1e-9
+    assert np.abs(gold["pf_field"]).max() > 0.1   # a non-trivial field
+
+
+def _run(tmp_path, gold, precision):
+    from pyopenlpt_amd import config as C
+    from pyopenlpt_amd.stb import STB
+    cams = [Case._cam(r) for r in gold["cameras"]]
+    tr = ast.literal_eval(str(gold["tracer"]))
+    cfg = C.write_run_folder(str(tmp_path), cams, gold["frames"],
+                             axis_limit=(-20, 20, -20, 20, -20, 20),
+                             voxel_to_mm=float(gold["voxel_mm"]), tracer=tr)
+    setting = C.BasicSetting()
+    assert setting.readConfig(cfg)
+    stb = STB(setting, "Tracer", setting.object_config_paths[0], precision=precision)
+    ios = [C.ImageIO("", p) for p in setting.image_file_paths]
+    for f in range(setting.frame_start, setting.frame_end + 1):
+        stb.process_frame(f, np.stack([io.loadImg(f) for io in ios]))
+    return stb
+
+
+def test_stb_run_f64_matches_reference(tmp_path, gpu_available, gold):
+    stb = _run(tmp_path, gold, "f64")
+    est = np.stack([stb.cfg.otf.a[:, 0], stb.cfg.otf.b[:, 0], stb.cfg.otf.c[:, 0],
+                    stb.cfg.otf.alpha[:, 0]], 1)
+    assert np.allclose(est, gold["otf_abca_est"], rtol=1e-12, atol=0)
+    for which, name in enumerate(LISTS):
+        r = stb.rows(which)
+        assert r["n_tracks"] == int(gold[f"tr_{name}_n"]), name
+        assert np.array_equal(r["track"], gold[f"tr_{name}_track"]), name
+        assert np.array_equal(r["frame"], gold[f"tr_{name}_frame"]), name
+        if len(r["xyz"]):
+            assert np.abs(r["xyz"] - gold[f"tr_{name}_xyz"]).max() <= 1e-9, name
+    # files the reference writes at the last frame
+    out = os.path.join(str(tmp_path), "out", "ConvergeTrack")
+    assert os.path.exists(os.path.join(out, "LongTrackActive_9.csv"))
+    from pyopenlpt_amd import config as C
+    t = C.read_tracks(os.path.join(out, "LongTrackActive_9.csv"))
+    assert np.array_equal(t["track"], gold["tr_long_active_track"])
+    assert np.abs(t["xyz"] - gold["tr_long_active_xyz"]).max() <= 5e-7   # 6 decimals
+    stb.close()
+
+
+def test_stb_run_f32_close(tmp_path, gpu_available, gold):
+    stb = _run(tmp_path, gold, "f32")
+    r = stb.rows(1)
+    want = int(gold["tr_long_active_n"])
+    assert abs(r["n_tracks"] - want) <= max(2, int(0.02 * want))
+    stb.close()
