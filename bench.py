@@ -121,8 +121,167 @@ def traffic_per_launch(workload: str):
     return None
 
 
+# ---------------------------------------------------------------------------
+# tracking workload (--workload stb)
+# ---------------------------------------------------------------------------
+STB_TRACER = dict(radius_search_obj_vox=10.0, n_initial_frames=4, radius_search_track_vox=30.0,
+                  shake_width_vox=0.25, pf_grid=(51, 51, 51), pf_r_vox=25.0, n_loop_ipr=4,
+                  n_shake_loop=4, thred_ghost=0.1, tol_2d_px=1.0, tol_3d_vox=1.0,
+                  n_cam_reduced=0, n_loop_ipr_reduced=2, min_obj_int=240, radius_obj=2.0)
+
+
+def _stb_frame(args):
+    from pyopenlpt_amd import synth as S
+    cams, p, amp, seed = args
+    return S.render(cams, p, amp, 1.2, 1.2, 0.0, 10.0, seed, np.uint16)
+
+
+def make_stb_frames(n_part: int, n_frames: int, size: int, seed: int = 1234):
+    """ABC-flow sequence (A = B = C = 1, period 40 mm, RK4, at most 3 px per
+    frame, particles leaving the cube re-seeded), rendered as the C2 frames
+    (u16, a ~ U(800, 1200) per particle, b = c = 1.2, noise 10)."""
+    import concurrent.futures as cf
+    from pyopenlpt_amd import synth as S
+    vox = 40.0 / (0.95 * size)
+    cams = S.make_rig(size, size, max_intensity=65535)
+    seq = S.abc_sequence(n_part, n_frames, seed, max_disp_mm=3.0 * vox)
+    rng = np.random.default_rng(seed + 5)
+    amp = rng.uniform(800.0, 1200.0, max(int(ids.max()) for _, ids in seq) + 1)
+    jobs = [(cams, p, amp[ids], seed + 100 + f) for f, (p, ids) in enumerate(seq)]
+    workers = max(1, min(16, os.cpu_count() or 1, n_frames))
+    with cf.ProcessPoolExecutor(max_workers=workers) as ex:
+        frames = np.stack(list(ex.map(_stb_frame, jobs)))
+    return vox, cams, seq, frames
+
+
+STB_CPU_CHILD = r"""
+import ctypes, json, sys, time
+sys.path.insert(0, sys.argv[1])
+import oracle as O
+L = O.load_ref()
+L.ref_stb_open.restype = ctypes.c_void_p
+L.ref_stb_open.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int]
+L.ref_stb_process.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+h = L.ref_stb_open(sys.argv[2].encode(), None, 1)
+assert h
+k = int(sys.argv[3])
+t = time.perf_counter()
+for f in range(k):
+    assert L.ref_stb_process(ctypes.c_void_p(h), f, 1) == 0
+print(json.dumps(dict(frames=k, seconds=time.perf_counter() - t)))
+"""
+
+
+def track_quality(stb, seq, vox: float) -> dict:
+    """Tracks of at least 4 points (all four lists) against the ground
+    truth: a point is matched to the nearest true particle of its frame
+    within 1 voxel; a track is correct when all its points match the same
+    particle.  Also the share of the last frame's particles carried by an
+    active long track."""
+    from scipy.spatial import cKDTree
+    trees = [cKDTree(p) for p, _ in seq]
+    n_tr = n_ok = n_pts = 0
+    for which in range(4):
+        r = stb.rows(which)
+        if not len(r["track"]):
+            continue
+        order = np.argsort(r["track"], kind="stable")
+        tid, fr, xyz = r["track"][order], r["frame"][order], r["xyz"][order]
+        cuts = np.flatnonzero(np.diff(tid)) + 1
+        for a, b in zip(np.r_[0, cuts], np.r_[cuts, len(tid)]):
+            if b - a < 4:
+                continue
+            ids = []
+            for f, q in zip(fr[a:b], xyz[a:b]):
+                d, i = trees[f].query(q)
+                ids.append(seq[f][1][i] if d < vox else -1)
+            n_tr += 1
+            n_pts += b - a
+            n_ok += int(ids[0] >= 0 and all(v == ids[0] for v in ids))
+    last = stb.rows(1)
+    lf = len(seq) - 1
+    sel = last["frame"] == lf
+    d, i = trees[lf].query(last["xyz"][sel]) if sel.any() else (np.zeros(0), np.zeros(0, int))
+    covered = len(np.unique(i[d < vox])) / len(seq[lf][0])
+    return dict(tracks_min4=n_tr, correct_frac=round(n_ok / max(1, n_tr), 4),
+                mean_len=round(n_pts / max(1, n_tr), 2), last_frame_coverage=round(covered, 4))
+
+
+def stb_main(args) -> int:
+    import shutil
+    import tempfile
+    import torch
+    from pyopenlpt_amd import config as C
+    from pyopenlpt_amd.context import to_device_image
+    from pyopenlpt_amd.stb import STB
+    n_frames = args.frames if args.frames_set else 100
+    log(f"tracking workload: {args.stb_particles} particles, {n_frames} frames, {args.size}^2")
+    vox, cams, seq, frames = make_stb_frames(args.stb_particles, n_frames, args.size)
+    tmp = tempfile.mkdtemp(prefix="olpt_stb_")
+    try:
+        k_cpu = max(5, args.stb_cpu_frames)
+        cfg = C.write_run_folder(tmp, cams, frames, axis_limit=(-20, 20, -20, 20, -20, 20),
+                                 voxel_to_mm=vox, tracer=STB_TRACER,
+                                 n_tiff=max(5, k_cpu if not args.no_cpu_baseline else 5))
+        d_frames = [to_device_image(frames[f]) for f in range(n_frames)]
+        torch.cuda.synchronize()
+        setting = C.BasicSetting()
+        setting.readConfig(cfg)
+        stb = STB(setting, "Tracer", setting.object_config_paths[0], precision=args.precision,
+                  write_files=False)
+        stb.ctx.set_concurrency(1)
+        t0 = time.perf_counter()
+        t_k = None
+        for f in range(n_frames):
+            stb.process_frame(f, d_frames[f])
+            if f + 1 == k_cpu:
+                torch.cuda.synchronize()
+                t_k = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        q = track_quality(stb, seq, vox)
+        counts = {n: stb.rows(w)["n_tracks"] for w, n in enumerate(
+            ("short_active", "long_active", "long_inactive", "exit"))}
+        stb.close()
+        cpu = None
+        if not args.no_cpu_baseline:
+            import oracle as O
+            if O.ref_available():
+                env = dict(os.environ, OMP_NUM_THREADS=str(args.cpu_threads), OMP_PROC_BIND="close")
+                log(f"cpu baseline: reference STB, frames 0..{k_cpu - 1}, {args.cpu_threads} threads")
+                r = subprocess.run([sys.executable, "-c", STB_CPU_CHILD, ROOT, cfg, str(k_cpu)],
+                                   capture_output=True, text=True, env=env, timeout=1800)
+                if r.returncode == 0:
+                    res = json.loads(r.stdout.strip().splitlines()[-1])
+                    cpu = {"value": round(res["frames"] / res["seconds"], 4), "unit": "frames/s",
+                           "cores": args.cpu_threads, "kind": "reference",
+                           "sample": (f"frames 0-{k_cpu - 1} of the same sequence (4 initial-phase + "
+                                      f"{k_cpu - 4} convergence frames), STB::processFrame wall "
+                                      f"{res['seconds']:.2f} s"),
+                           "gpu_same_frames_per_s": round(k_cpu / t_k, 3) if t_k else None}
+                else:
+                    log("cpu baseline failed:", r.stderr[-2000:])
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    out = {"metric": "STB frames/s (tracking, C5 ABC flow)", "value": round(n_frames / wall, 3),
+           "unit": "frames/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+           "ms_per_step": round(1000 * wall / n_frames, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": args.precision,
+           "data": "synthetic",
+           "config": {"workload": f"stb_abc_{args.stb_particles}p_{n_frames}f_{args.size}sq_u16_4cam",
+                      "frames": n_frames, "particles": args.stb_particles, "tracks": counts,
+                      "quality": q, "step": "one frame of STB::processFrame, images in HBM"},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="ipr", choices=["ipr", "stb"],
+                    help="ipr: the headline IPR throughput; stb: shake-the-box tracking")
+    ap.add_argument("--stb-particles", type=int, default=30000)
+    ap.add_argument("--stb-cpu-frames", type=int, default=6)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -150,6 +309,9 @@ def main() -> int:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); frames are "
               "independent, so N GPUs would each run their own batch.", file=sys.stderr)
         return 2
+    args.frames_set = "--frames" in sys.argv
+    if args.workload == "stb":
+        return stb_main(args)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.ppp, args.size, args.cpu_threads)
 

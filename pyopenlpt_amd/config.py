@@ -466,11 +466,14 @@ def write_tracer_config(path: str, *, radius_search_obj_vox, n_initial_frames,
 
 
 def write_run_folder(folder: str, cams, frames: np.ndarray, *, axis_limit, voxel_to_mm,
-                     tracer: dict, fps: int = 1, n_thread: int = 0) -> str:
+                     tracer: dict, fps: int = 1, n_thread: int = 0,
+                     n_tiff: int | None = None) -> str:
     """Everything a run reads, under folder: camera files, one TIFF per
     camera and frame with its image list, tracerConfig.txt and config.txt
     (output folder: folder/out/).  frames: [F][n_cam][H][W] uint8/uint16.
-    Returns the path of config.txt (all paths in it are absolute)."""
+    n_tiff: write only the first n_tiff frames' TIFFs (the lists still name
+    every frame), for runs that take later frames from memory.  Returns the
+    path of config.txt (all paths in it are absolute)."""
     from .camfile import save_camera
     folder = os.path.abspath(folder)
     os.makedirs(os.path.join(folder, "camFile"), exist_ok=True)
@@ -484,7 +487,8 @@ def write_run_folder(folder: str, cams, frames: np.ndarray, *, axis_limit, voxel
         names = []
         for f in range(frames.shape[0]):
             pth = os.path.join(d, f"img{f:05d}.tif")
-            write_tiff(pth, frames[f, k])
+            if n_tiff is None or f < n_tiff:
+                write_tiff(pth, frames[f, k])
             names.append(pth)
         lf = os.path.join(folder, "imgFile", f"cam{k + 1}ImageNames.txt")
         with open(lf, "w") as fh:

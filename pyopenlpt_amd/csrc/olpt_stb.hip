@@ -179,40 +179,90 @@ __global__ void k_in_view(const double *__restrict__ xyz, int n, const CamT<doub
 }
 
 // |p - q| <= tol for some track end q (Pt3D::norm: sum of squares in axis
-// order, Matrix.hpp:255-261).  Track ends staged through LDS.
+// order, Matrix.hpp:255-261).  Grid: x over objects, y over chunks of
+// track ends staged through LDS; a hit sets rep[i] = 1 (rep cleared first).
+constexpr int kRepChunk = 1024;
 __global__ void __launch_bounds__(256)
 k_repeat(const double *__restrict__ obj, int n, const double *__restrict__ ends, int m, double tol,
-         int8_t *__restrict__ rep) {
-  __shared__ double s[256 * 3];
+         int32_t *__restrict__ rep) {
+  __shared__ double s[kRepChunk * 3];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double p[3] = {0, 0, 0};
-  if (i < n) p[0] = obj[3L * i], p[1] = obj[3L * i + 1], p[2] = obj[3L * i + 2];
-  bool hit = false;
-  for (int base = 0; base < m; base += 256) {
-    __syncthreads();
-    const int j = base + threadIdx.x;
-    if (j < m)
-      for (int a = 0; a < 3; ++a) s[3 * threadIdx.x + a] = ends[3L * j + a];
-    __syncthreads();
-    const int cnt = min(256, m - base);
-    if (i < n && !hit)
-      for (int t = 0; t < cnt; ++t) {
-        const double d0 = p[0] - s[3 * t], d1 = p[1] - s[3 * t + 1], d2 = p[2] - s[3 * t + 2];
-        double r = 0.0;
-        r += d0 * d0;
-        r += d1 * d1;
-        r += d2 * d2;
-        if (sqrt(r) <= tol) {
-          hit = true;
-          break;
-        }
-      }
+  const int base = blockIdx.y * kRepChunk;
+  const int cnt = min(kRepChunk, m - base);
+  for (int t = threadIdx.x; t < cnt * 3; t += blockDim.x) s[t] = ends[3L * base + t];
+  __syncthreads();
+  if (i >= n) return;
+  const double p0 = obj[3L * i], p1 = obj[3L * i + 1], p2 = obj[3L * i + 2];
+  for (int t = 0; t < cnt; ++t) {
+    const double d0 = p0 - s[3 * t], d1 = p1 - s[3 * t + 1], d2 = p2 - s[3 * t + 2];
+    double r = 0.0;
+    r += d0 * d0;
+    r += d1 * d1;
+    r += d2 * d2;
+    if (sqrt(r) <= tol) {
+      rep[i] = 1;
+      return;
+    }
   }
-  if (i < n) rep[i] = hit ? 1 : 0;
+}
+
+// Block-wide sum / nearest (fixed reduction tree: deterministic).
+constexpr int kLinkThreads = 256;
+__device__ double block_sum(double v, double *sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int w = threadIdx.x / 64;
+  __syncthreads();
+  if (threadIdx.x % 64 == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int k = 0; k < kLinkThreads / 64; ++k) t += sh[k];
+  return t;
 }
 
 // Nearest object to q with squared distance < r2 (nanoflann L2_Simple:
-// ((0 + dx^2) + dy^2) + dz^2 with d = q - p), or kUnlinked.
+// ((0 + dx^2) + dy^2) + dz^2 with d = q - p), or kUnlinked; ties go to the
+// lower index.  Called by all threads of the block.
+__device__ int block_nearest(const double *__restrict__ obj, int n, const double q[3], double r2,
+                             double *shd, int *shi) {
+  double bd = r2;
+  int best = kUnlinked;
+  for (int j = threadIdx.x; j < n; j += kLinkThreads) {
+    const double d0 = q[0] - obj[3L * j], d1 = q[1] - obj[3L * j + 1], d2 = q[2] - obj[3L * j + 2];
+    double d = 0.0;
+    d += d0 * d0;
+    d += d1 * d1;
+    d += d2 * d2;
+    if (d < bd) {
+      bd = d;
+      best = j;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double od = __shfl_xor(bd, off, 64);
+    const int oi = __shfl_xor(best, off, 64);
+    if (oi != kUnlinked && (best == kUnlinked || od < bd || (od == bd && oi < best))) {
+      bd = od;
+      best = oi;
+    }
+  }
+  const int w = threadIdx.x / 64;
+  __syncthreads();
+  if (threadIdx.x % 64 == 0) {
+    shd[w] = bd;
+    shi[w] = best;
+  }
+  __syncthreads();
+  bd = r2;
+  best = kUnlinked;
+  for (int k = 0; k < kLinkThreads / 64; ++k)
+    if (shi[k] != kUnlinked && (best == kUnlinked || shd[k] < bd || (shd[k] == bd && shi[k] < best))) {
+      bd = shd[k];
+      best = shi[k];
+    }
+  return best;
+}
+
+// Nearest object, one thread (initial-phase linking, one thread per track).
 __device__ int nearest(const double *__restrict__ obj, int n, const double q[3], double r2) {
   int best = kUnlinked;
   double bd = r2;
@@ -230,16 +280,18 @@ __device__ int nearest(const double *__restrict__ obj, int n, const double q[3],
   return best;
 }
 
-// linkShortTrack (STB.cpp:743-872).  nb_pt: each long-active track's
-// point before its last one, rounded to float as the reference's track
-// cloud hands it to the k-d tree (Track.h:56-57); nb_disp: its last
-// displacement in FP64; nb_len: its length.
-__global__ void k_link_short(const double *__restrict__ last, int n_sa,
-                             const double *__restrict__ nb_pt, const double *__restrict__ nb_disp,
-                             const int32_t *__restrict__ nb_len, int n_la,
-                             const double *__restrict__ obj, int n_obj, double base_r, int n_iter,
-                             int32_t *__restrict__ link) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// linkShortTrack (STB.cpp:743-872), one workgroup per short track.  nb_pt:
+// each long-active track's point before its last one, rounded to float as
+// the reference's track cloud hands it to the k-d tree (Track.h:56-57);
+// nb_disp: its last displacement in FP64; nb_len: its length.
+__global__ void __launch_bounds__(kLinkThreads)
+k_link_short(const double *__restrict__ last, int n_sa, const double *__restrict__ nb_pt,
+             const double *__restrict__ nb_disp, const int32_t *__restrict__ nb_len, int n_la,
+             const double *__restrict__ obj, int n_obj, double base_r, int n_iter,
+             int32_t *__restrict__ link) {
+  __shared__ double shd[kLinkThreads / 64];
+  __shared__ int shi[kLinkThreads / 64];
+  const int i = blockIdx.x;
   if (i >= n_sa) return;
   const double p[3] = {last[3L * i], last[3L * i + 1], last[3L * i + 2]};
   int id = kUnlinked;
@@ -249,9 +301,8 @@ __global__ void k_link_short(const double *__restrict__ last, int n_sa,
       factor = step == 0 ? 1.0 : factor * 1.1;
       const double r_track = (3.0 * base_r) * factor;
       const double r2 = r_track * r_track;
-      int cnt = 0;
-      double mean_d2 = 0.0;
-      for (int j = 0; j < n_la; ++j) {
+      double c_loc = 0.0, s_loc = 0.0;
+      for (int j = threadIdx.x; j < n_la; j += kLinkThreads) {
         const double d0 = p[0] - nb_pt[3L * j], d1 = p[1] - nb_pt[3L * j + 1],
                      d2 = p[2] - nb_pt[3L * j + 2];
         double d = 0.0;
@@ -259,20 +310,22 @@ __global__ void k_link_short(const double *__restrict__ last, int n_sa,
         d += d1 * d1;
         d += d2 * d2;
         if (d < r2) {
-          mean_d2 += d;
-          ++cnt;
+          s_loc += d;
+          c_loc += 1.0;
         }
       }
+      const double cnt = block_sum(c_loc, shd);
+      const double sum_d2 = block_sum(s_loc, shd);
       double vel[3] = {0.0, 0.0, 0.0};
-      if (cnt > 0) {
-        mean_d2 /= (double)cnt;
+      if (cnt > 0.0) {
+        const double mean_d2 = sum_d2 / cnt;
         double L2 = (0.8 * 0.8) * mean_d2;
         const double R = base_r * factor;
         const double lo = (0.1 * R) * (0.1 * R), hi = (0.8 * R) * (0.8 * R);
         L2 = L2 < lo ? lo : (hi < L2 ? hi : L2);
         const double beta2 = 1.0 / (L2 + 1e-12);
-        double sw = 0.0, sd[3] = {0.0, 0.0, 0.0};
-        for (int j = 0; j < n_la; ++j) {
+        double sw = 0.0, sd0 = 0.0, sd1 = 0.0, sd2 = 0.0;
+        for (int j = threadIdx.x; j < n_la; j += kLinkThreads) {
           const double d0 = p[0] - nb_pt[3L * j], d1 = p[1] - nb_pt[3L * j + 1],
                        d2 = p[2] - nb_pt[3L * j + 2];
           double d = 0.0;
@@ -282,28 +335,32 @@ __global__ void k_link_short(const double *__restrict__ last, int n_sa,
           if (!(d < r2) || nb_len[j] < 2) continue;
           const double w = 1.0 / (1.0 + d * beta2);
           sw += w;
-          sd[0] += w * nb_disp[3L * j];
-          sd[1] += w * nb_disp[3L * j + 1];
-          sd[2] += w * nb_disp[3L * j + 2];
+          sd0 += w * nb_disp[3L * j];
+          sd1 += w * nb_disp[3L * j + 1];
+          sd2 += w * nb_disp[3L * j + 2];
         }
+        sw = block_sum(sw, shd);
+        sd0 = block_sum(sd0, shd);
+        sd1 = block_sum(sd1, shd);
+        sd2 = block_sum(sd2, shd);
         if (sw > 0.0) {
           const double inv = 1.0 / sw;
-          vel[0] = sd[0] * inv;
-          vel[1] = sd[1] * inv;
-          vel[2] = sd[2] * inv;
+          vel[0] = sd0 * inv;
+          vel[1] = sd1 * inv;
+          vel[2] = sd2 * inv;
         }
       }
       const double est[3] = {p[0] + vel[0], p[1] + vel[1], p[2] + vel[2]};
       const double r_obj = base_r * factor;
-      id = nearest(obj, n_obj, est, r_obj * r_obj);
+      id = block_nearest(obj, n_obj, est, r_obj * r_obj, shd, shi);
       if (id != kUnlinked) break;
-      if (cnt == 0) {
-        id = nearest(obj, n_obj, p, r_obj * r_obj);
+      if (cnt == 0.0) {
+        id = block_nearest(obj, n_obj, p, r_obj * r_obj, shd, shi);
         if (id != kUnlinked) break;
       }
     }
   }
-  link[i] = id;
+  if (threadIdx.x == 0) link[i] = id;
 }
 
 // Predictive-field grid and bins (PredField::setGrid, PredField.cpp:168-194).
@@ -1202,13 +1259,15 @@ olpt_status conv_phase(olpt_stb *s, int frame, const void *d_imgs, int dtype) {
     for (size_t i = 0; i < la.size(); ++i) ends[i] = la[i].p.back();
     STB_CHECK(upload_pts(s->ends, ends, st));
     STB_CHECK(upload_pts(s->cand_d, objs, st));
-    STB_CHECK(s->rep.need(objs.size()));
-    k_repeat<<<blocks_for((long)objs.size()), 256, 0, st>>>(
-        s->cand_d.as<double>(), (int)objs.size(), s->ends.as<double>(), (int)ends.size(),
-        s->ip.tol_3d_mm, s->rep.as<int8_t>());
+    const int no = (int)objs.size(), ne = (int)ends.size();
+    STB_CHECK(s->rep.need(sizeof(int32_t) * no));
+    k_fill_i32_stb<<<blocks_for(no), 256, 0, st>>>(s->rep.as<int32_t>(), no, 0);
+    k_repeat<<<dim3((no + 255) / 256, (ne + kRepChunk - 1) / kRepChunk), 256, 0, st>>>(
+        s->cand_d.as<double>(), no, s->ends.as<double>(), ne, s->ip.tol_3d_mm,
+        s->rep.as<int32_t>());
     STB_TRY(hipGetLastError());
-    std::vector<int8_t> rep(objs.size());
-    STB_TRY(hipMemcpyAsync(rep.data(), s->rep.p, objs.size(), hipMemcpyDeviceToHost, st));
+    std::vector<int32_t> rep(no);
+    STB_TRY(hipMemcpyAsync(rep.data(), s->rep.p, sizeof(int32_t) * no, hipMemcpyDeviceToHost, st));
     STB_TRY(hipStreamSynchronize(st));
     std::vector<StbPt> kept;
     for (size_t i = 0; i < objs.size(); ++i)
@@ -1240,7 +1299,7 @@ olpt_status conv_phase(olpt_stb *s, int frame, const void *d_imgs, int dtype) {
                                hipMemcpyHostToDevice, st));
       STB_CHECK(upload_pts(s->cand_d, objs, st));
       STB_CHECK(s->link.need(sizeof(int32_t) * n_sa));
-      k_link_short<<<blocks_for(n_sa, 64), 64, 0, st>>>(
+      k_link_short<<<n_sa, kLinkThreads, 0, st>>>(
           s->last.as<double>(), n_sa, s->nb_pt.as<double>(), s->nb_disp.as<double>(),
           s->nb_len.as<int32_t>(), (int)la.size(), s->cand_d.as<double>(), n_obj,
           s->sp.radius_search_track_mm, 5, s->link.as<int32_t>());

@@ -24,6 +24,7 @@ import sys
 
 
 def _short(name: str) -> str:
+    name = name.replace("(anonymous namespace)::", "")
     name = re.sub(r"\(.*", "", name)
     return name[:110]
 
