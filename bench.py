@@ -352,6 +352,7 @@ def main() -> int:
         for _ in range(args.warmup):
             step()
         torch.cuda.synchronize()
+        ctx.stage_times()      # reset the host-issue counter
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t0 = time.perf_counter()
         e0.record(stream)
@@ -360,12 +361,14 @@ def main() -> int:
         e1.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        h_ms, h_calls = ctx.stage_times()["host_issue"]
         ms = e0.elapsed_time(e1) / args.steps
         n_obj = outs[2].cpu().numpy()
         per_step = int(n_obj.sum())
         rate = per_step / (ms / 1000.0)
-        log(f"{label}: {args.steps} steps: {ms:.2f} ms/step (wall {1000 * wall / args.steps:.2f}), "
-            f"{per_step} particles/step -> {rate:.0f} particles/s")
+        log(f"{label}: {args.steps} steps: {ms:.2f} ms/step (wall {1000 * wall / args.steps:.2f}, "
+            f"host issue {h_ms / max(1, h_calls):.1f}), {per_step} particles/step -> "
+            f"{rate:.0f} particles/s")
         q = [quality(outs[0][f, : n_obj[f]].cpu().numpy(), frames[f].truth, frames[f].voxel_mm)
              for f in range(args.frames)]
         rmse = [x["rmse_mm"] for x in q if x["rmse_mm"] is not None]
@@ -374,6 +377,7 @@ def main() -> int:
                     recon_rate=float(np.mean([x["recon_rate"] for x in q])),
                     ghost_rate=float(np.mean([x["ghost_rate"] for x in q])))
         return dict(particles_per_s=round(rate, 1), ms_per_step=round(ms, 3),
+                    host_issue_ms_per_step=round(h_ms / h_calls, 3) if h_calls else None,
                     particles_per_step=per_step, shake_loops=int(params.n_shake_loop),
                     quality=qual)
 

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -113,9 +114,14 @@ struct olpt_ctx {
   double st_bytes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int32_t st_launch[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   // graph
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  // one captured graph per workspace (per stream)
+  hipGraph_t graph[kMaxWorkspaces] = {};
+  hipGraphExec_t gexec[kMaxWorkspaces] = {};
+  int n_graph = 0;
   bool capturing = false;
+  // host time spent issuing olpt_ipr_run's work (not waiting for it)
+  double host_ms = 0.0;
+  int32_t host_calls = 0;
 };
 
 namespace {
@@ -152,11 +158,18 @@ struct TimerScope {
 
 void free_streams(olpt_ctx *c);
 
+void free_graphs(olpt_ctx *c) {
+  for (int k = 0; k < kMaxWorkspaces; ++k) {
+    if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
+    if (c->graph[k]) (void)hipGraphDestroy(c->graph[k]);
+    c->gexec[k] = nullptr;
+    c->graph[k] = nullptr;
+  }
+  c->n_graph = 0;
+}
+
 void free_all(olpt_ctx *c) {
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
-  c->gexec = nullptr;
-  c->graph = nullptr;
+  free_graphs(c);
   for (void *p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
 }
@@ -1056,16 +1069,21 @@ olpt_status ipr_run_impl(olpt_ctx *c, const void *d_imgs, int dtype, int n_frame
     for (int k = 1; k < nw; ++k) HIP_TRY(hipStreamWaitEvent(c->ws[k].st, c->ev_fork, 0));
   }
   for (int k = 0; k < nw; ++k) fill_i32(c->ws[k].flags, 16, 0, c->ws[k].st);
-  int n_loop = 0;
-  for (int f = 0; f < n_frames; ++f) {
+  // frame f goes to workspace f % nw, in frame order on that stream
+  auto run_frame = [&](int f, int *n_loop) -> olpt_status {
     Workspace &w = c->ws[f % nw];
     const char *src = (const char *)d_imgs + (size_t)f * n * in_bytes;
     olpt_status s = convert_image<T>(src, dtype, n, (T *)w.img_a, w.st);
     if (s) return s;
-    s = ipr_frame<T>(c, w, p, d_obj_xyz + (size_t)f * cap * 3,
-                     d_obj_uv ? d_obj_uv + (size_t)f * cap * c->n_cam * 2 : nullptr, cap,
-                     d_n_obj + f, loop_total ? c->loop_tot + (size_t)f * cap_loop : nullptr,
-                     cap_loop, &n_loop);
+    return ipr_frame<T>(c, w, p, d_obj_xyz + (size_t)f * cap * 3,
+                        d_obj_uv ? d_obj_uv + (size_t)f * cap * c->n_cam * 2 : nullptr, cap,
+                        d_n_obj + f, loop_total ? c->loop_tot + (size_t)f * cap_loop : nullptr,
+                        cap_loop, n_loop);
+  };
+  int n_loop = 0;
+  const auto t_issue = std::chrono::steady_clock::now();
+  for (int f = 0; f < n_frames; ++f) {
+    olpt_status s = run_frame(f, &n_loop);
     if (s) return s;
   }
   if (nw > 1) {
@@ -1075,6 +1093,8 @@ olpt_status ipr_run_impl(olpt_ctx *c, const void *d_imgs, int dtype, int n_frame
     }
   }
   if (!sync) return OLPT_OK;
+  c->host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_issue).count();
+  c->host_calls += 1;
   int32_t fl[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   std::vector<int32_t> fw(8 * nw);
   for (int k = 0; k < nw; ++k)
@@ -1093,6 +1113,30 @@ olpt_status ipr_run_impl(olpt_ctx *c, const void *d_imgs, int dtype, int n_frame
   if (fl[4]) return fail(OLPT_E_CAPACITY, "object capacity exceeded");
   if (fl[5]) return fail(OLPT_E_CAPACITY, "prune rounds exhausted");
   (void)n_loop;
+  return OLPT_OK;
+}
+
+// The frames of workspace k (f = k, k + nw, ...) on its stream, for capture:
+// the same kernel sequence olpt_ipr_run issues for them, without host reads.
+template <typename T>
+olpt_status capture_slot(olpt_ctx *c, int k, int nw, const void *d_imgs, int dtype, int n_frames,
+                         const olpt_ipr_params *p, double *d_obj_xyz, double *d_obj_uv, int cap,
+                         int32_t *d_n_obj) {
+  TimerScope ts(c, true);
+  const long n = (long)c->n_cam * c->H * c->W;
+  const size_t in_bytes = dtype == OLPT_U8 ? 1 : dtype == OLPT_U16 ? 2 : dtype == OLPT_F32 ? 4 : 8;
+  Workspace &w = c->ws[k];
+  fill_i32(w.flags, 16, 0, w.st);
+  int n_loop = 0;
+  for (int f = k; f < n_frames; f += nw) {
+    const char *src = (const char *)d_imgs + (size_t)f * n * in_bytes;
+    olpt_status s = convert_image<T>(src, dtype, n, (T *)w.img_a, w.st);
+    if (s) return s;
+    s = ipr_frame<T>(c, w, p, d_obj_xyz + (size_t)f * cap * 3,
+                     d_obj_uv ? d_obj_uv + (size_t)f * cap * c->n_cam * 2 : nullptr, cap,
+                     d_n_obj + f, nullptr, 0, &n_loop);
+    if (s) return s;
+  }
   return OLPT_OK;
 }
 } // namespace
@@ -1151,27 +1195,42 @@ olpt_status olpt_ipr_graph_capture(olpt_ctx *c, const void *d_imgs, int dtype, i
   if ((s = check_params(c, p))) return s;
   if (!c->otf_set) return fail(OLPT_E_INVALID_ARGUMENT, "OTF not set");
   if (!c->st) return fail(OLPT_E_INVALID_ARGUMENT, "graph capture needs a non-null stream");
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
-  c->gexec = nullptr;
-  c->graph = nullptr;
-  HIP_TRY(hipStreamSynchronize(c->st));
-  HIP_TRY(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-  if (c->prec == OLPT_PREC_F64)
-    s = ipr_run_impl<double>(c, d_imgs, dtype, n_frames, p, d_obj_xyz, d_obj_uv, cap, d_n_obj,
-                             nullptr, 0, false);
-  else
-    s = ipr_run_impl<float>(c, d_imgs, dtype, n_frames, p, d_obj_xyz, d_obj_uv, cap, d_n_obj,
-                            nullptr, 0, false);
-  hipGraph_t g = nullptr;
-  hipError_t e = hipStreamEndCapture(c->st, &g);
-  if (s) {
-    if (g) (void)hipGraphDestroy(g);
-    return s;
+  if (c->stats) return fail(OLPT_E_INVALID_ARGUMENT, "byte accounting cannot be captured");
+  free_graphs(c);
+  // One graph per workspace: the frames f = k, k + nw, ... that
+  // olpt_ipr_run would send to workspace k, captured on its stream.  The
+  // graphs are launched on their own streams, so frames overlap as in the
+  // eager run while the host submits one graph instead of ~1000 kernels
+  // per frame.
+  const int nw = std::max(1, std::min({c->n_ws_active, c->n_ws, n_frames}));
+  for (int k = 0; k < nw; ++k) HIP_TRY(hipStreamSynchronize(c->ws[k].st));
+  c->capturing = true;
+  for (int k = 0; k < nw; ++k) {
+    hipStream_t st = c->ws[k].st;
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    if (c->prec == OLPT_PREC_F64)
+      s = capture_slot<double>(c, k, nw, d_imgs, dtype, n_frames, p, d_obj_xyz, d_obj_uv, cap, d_n_obj);
+    else
+      s = capture_slot<float>(c, k, nw, d_imgs, dtype, n_frames, p, d_obj_xyz, d_obj_uv, cap, d_n_obj);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (s || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      c->capturing = false;
+      free_graphs(c);
+      if (s) return s;
+      return fail(OLPT_E_GPU, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
+    c->graph[k] = g;
+    c->n_graph = k + 1;
+    hipError_t ei = hipGraphInstantiate(&c->gexec[k], g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      c->capturing = false;
+      free_graphs(c);
+      return fail(OLPT_E_GPU, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
   }
-  if (e != hipSuccess) return fail(OLPT_E_GPU, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-  c->graph = g;
-  HIP_TRY(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+  c->capturing = false;
   return OLPT_OK;
 }
 
@@ -1205,6 +1264,10 @@ olpt_status olpt_stage_times(olpt_ctx *c, double ms[8], int32_t calls[8]) {
     calls[s] = c->timer->n[s];
     c->timer->n[s] = 0;
   }
+  ms[OLPT_STAGE_HOST] = c->host_ms;
+  calls[OLPT_STAGE_HOST] = c->host_calls;
+  c->host_ms = 0.0;
+  c->host_calls = 0;
   return OLPT_OK;
 }
 
@@ -1224,8 +1287,17 @@ olpt_status olpt_stage_stats(olpt_ctx *c, int enable, double bytes[8], int32_t l
 }
 
 olpt_status olpt_ipr_graph_launch(olpt_ctx *c) {
-  if (!c || !c->gexec) return fail(OLPT_E_INVALID_ARGUMENT, "no captured graph");
-  HIP_TRY(hipGraphLaunch(c->gexec, c->st));
+  if (!c || !c->n_graph) return fail(OLPT_E_INVALID_ARGUMENT, "no captured graph");
+  const int ng = c->n_graph;
+  if (ng > 1) {
+    HIP_TRY(hipEventRecord(c->ev_fork, c->st));
+    for (int k = 1; k < ng; ++k) HIP_TRY(hipStreamWaitEvent(c->ws[k].st, c->ev_fork, 0));
+  }
+  for (int k = 0; k < ng; ++k) HIP_TRY(hipGraphLaunch(c->gexec[k], c->ws[k].st));
+  for (int k = 1; k < ng; ++k) {
+    HIP_TRY(hipEventRecord(c->ws[k].done, c->ws[k].st));
+    HIP_TRY(hipStreamWaitEvent(c->st, c->ws[k].done, 0));
+  }
   return OLPT_OK;
 }
 

@@ -1055,52 +1055,96 @@ __global__ void k_prio_fallback(StereoBuffers b) {
 // head, and rejected when a selected entry precedes it on any point.
 // In-place updates are safe: decided states never change, and an
 // undecided entry ahead only delays a decision.
+// One candidate's turn: selected (1) when every entry ahead of it on each of
+// its points is rejected, rejected (2) when one is selected, else still
+// undecided (0).  States and heads are read with device-coherent loads.
+__device__ __forceinline__ int mis_decide(const StereoBuffers &b, const StereoParams &p, int j) {
+  const int nc = b.n_cam;
+  bool out = false, blocked = false;
+  int k = 0;
+  for (int c = 0; c < nc && !out; ++c) {
+    if (!p.active[c]) continue;
+    const int pid = b.cand_ids_s[(long)j * nc + c];
+    if (pid < 0) continue;
+    const long f = (long)c * b.cap + pid;
+    const int pos = b.ppos[(long)j * nc + k];
+    ++k;
+    const int s0 = b.pt_start[f];
+    const int h0 = __atomic_load_n(&b.head[f], __ATOMIC_RELAXED);
+    int h = h0, st = 2;
+    while (h < pos) {
+      st = __atomic_load_n(&b.state[ent_cand(b.pt_sorted[s0 + h])], __ATOMIC_RELAXED);
+      if (st != 2) break;
+      ++h;
+    }
+    if (h > h0) atomicMax(&b.head[f], h);
+    if (h < pos) {
+      if (st == 1)
+        out = true;
+      else
+        blocked = true;
+    }
+  }
+  if (out) {
+    b.state[j] = 2;
+    return 2;
+  }
+  if (!blocked) {
+    b.state[j] = 1;
+    return 1;
+  }
+  return 0;
+}
+
 __global__ void k_mis_round(StereoBuffers b, StereoParams p, int round) {
   if (round > 0 && b.undecided[round - 1] == 0) return;
   const int M = min(*b.n_cand, b.cap_cand);
-  const int nc = b.n_cam;
   int und = 0;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
     if (b.state[j] != 0) continue;
-    bool out = false, blocked = false;
-    int k = 0;
-    for (int c = 0; c < nc && !out; ++c) {
-      if (!p.active[c]) continue;
-      const int pid = b.cand_ids_s[(long)j * nc + c];
-      if (pid < 0) continue;
-      const long f = (long)c * b.cap + pid;
-      const int pos = b.ppos[(long)j * nc + k];
-      ++k;
-      const int s0 = b.pt_start[f];
-      const int h0 = __atomic_load_n(&b.head[f], __ATOMIC_RELAXED);
-      int h = h0, st = 2;
-      while (h < pos) {
-        st = __atomic_load_n(&b.state[ent_cand(b.pt_sorted[s0 + h])], __ATOMIC_RELAXED);
-        if (st != 2) break;
-        ++h;
-      }
-      if (h > h0) atomicMax(&b.head[f], h);
-      if (h < pos) {
-        if (st == 1)
-          out = true;
-        else
-          blocked = true;
-      }
-    }
-    if (out)
-      b.state[j] = 2;
-    else if (!blocked)
-      b.state[j] = 1;
-    else
-      ++und;
+    if (mis_decide(b, p, j) == 0) ++und;
   }
   und = wave_sum(und);
   if (lane_id() == 0 && und) atomicAdd(&b.undecided[round], und);
 }
 
-__global__ void k_rounds_check(StereoBuffers b, int max_rounds) {
-  if (threadIdx.x == 0 && blockIdx.x == 0 && b.undecided[max_rounds - 1] > 0)
-    atomicOr(&b.flags[5], 1);
+// After kMisFull rounds over all candidates the few still undecided are
+// listed (in keep_off, free until the final compaction; count in
+// undecided[kMisFull]) and one workgroup runs the remaining rounds over the
+// list, a barrier between rounds.  The decisions are the same fixed point
+// (each depends only on entries ahead of it), so the result is unchanged.
+constexpr int kMisFull = 10;
+constexpr int kMisFinishThreads = 1024;
+constexpr int kMisFinishRounds = 1 << 14;
+
+__global__ void k_mis_compact(StereoBuffers b) {
+  if (b.undecided[kMisFull - 1] == 0) return;
+  const int M = min(*b.n_cand, b.cap_cand);
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x)
+    if (b.state[j] == 0) b.keep_off[atomicAdd(&b.undecided[kMisFull], 1)] = j;
+}
+
+__global__ void __launch_bounds__(kMisFinishThreads)
+k_mis_finish(StereoBuffers b, StereoParams p) {
+  __shared__ int s_und[kMisFinishThreads / 64];
+  const int n = b.undecided[kMisFull];
+  if (n == 0) return;
+  for (int r = 0; r < kMisFinishRounds; ++r) {
+    int und = 0;
+    for (int i = threadIdx.x; i < n; i += kMisFinishThreads) {
+      const int j = b.keep_off[i];
+      if (__atomic_load_n(&b.state[j], __ATOMIC_RELAXED) != 0) continue;
+      if (mis_decide(b, p, j) == 0) ++und;
+    }
+    und = wave_sum(und);
+    if (lane_id() == 0) s_und[threadIdx.x / 64] = und;
+    __syncthreads();
+    int tot = 0;
+    for (int w = 0; w < kMisFinishThreads / 64; ++w) tot += s_und[w];
+    __syncthreads();
+    if (tot == 0) return;
+  }
+  if (threadIdx.x == 0) atomicOr(&b.flags[5], 1);  // rounds exhausted
 }
 
 // Replacement branch of the reference (StereoMatch.cpp:906-968), evaluated
@@ -1281,11 +1325,11 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   k_pct_final<<<gM, T, 0, st>>>(b, p);
   k_prio_rank<<<gE, T, 0, st>>>(b);
   k_prio_fallback<<<grid_for(nflat, T, 2048), T, 0, st>>>(b);
-  // the first rounds decide almost every candidate; later rounds (usually
-  // empty, returning at once) use a small grid
-  for (int r = 0; r < max_rounds; ++r)
-    k_mis_round<<<r < 16 ? gM : 64, T, 0, st>>>(b, p, r);
-  k_rounds_check<<<1, 1, 0, st>>>(b, max_rounds);
+  // the first rounds decide almost every candidate; one workgroup finishes
+  // the few left
+  for (int r = 0; r < kMisFull; ++r) k_mis_round<<<gM, T, 0, st>>>(b, p, r);
+  k_mis_compact<<<gM, T, 0, st>>>(b);
+  k_mis_finish<<<1, kMisFinishThreads, 0, st>>>(b, p);
   k_replace_check<<<gM, T, 0, st>>>(b, p);
   // triangulate + compact in candidate order
   k_tri<<<gM, T, 0, st>>>(b, p);
