@@ -358,9 +358,17 @@ __device__ __forceinline__ void bq_init(BlockQueue<Rec, CAP> &q) {
 }
 
 // Room for one record per thread is guaranteed by bq_flush's threshold.
+// One LDS atomic per wave: the wave's pushing lanes take consecutive slots
+// in lane order.
 template <typename Rec, int CAP>
 __device__ __forceinline__ void bq_push(BlockQueue<Rec, CAP> &q, bool want, const Rec &r) {
-  if (want) q.buf[atomicAdd(&q.n, 1)] = r;
+  const uint64_t m = ballot(want);
+  if (m == 0) return;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  int base = 0;
+  if (lane_id() == leader) base = atomicAdd(&q.n, __popcll(m));
+  base = __shfl(base, leader, kWave);
+  if (want) q.buf[base + mbcnt(m)] = r;
 }
 
 // Flush when fewer than blockDim free slots remain (or any, with `all`).
