@@ -1,5 +1,6 @@
 /*
- * olpt_oracle.c - CPU restatement of the reference IPR hot path (FP64).
+ * olpt_oracle.c - CPU restatement of the reference IPR hot path (FP64) and
+ * of the tracking functions predictNext, checkLinearFit and the predictive field.
  *
  * TEST INFRASTRUCTURE ONLY.  This is the oracle the HIP library is checked
  * against; it is never linked into the product.  Each function cites the
@@ -16,6 +17,10 @@
  *   stereo match / prune      compiled reference on seeded frames
  *   shake / flags / residual  compiled reference on seeded frames
  *   runIPR                    compiled reference, per-loop counts
+ *   STB predictNext,          compiled reference (predictNext / checkLinearFit
+ *   checkLinearFit,           / calPredField golden vectors, bit-exact), the
+ *   predictive field          one-step LMS closed form, exact recovery of
+ *                             isolated displacements
  * Exact duplicates at zero distance in the repeat search are resolved by
  * nanoflann's unstable distance sort in the reference; here every neighbour
  * other than the query itself is marked ("parity unpinned" for that tie).
@@ -2036,4 +2041,285 @@ int orc_run_ipr(const olpt_camera *cams, int n_cam, const double *imgs, int H,
   free(xyz.v);
   free(uv.v);
   return rc;
+}
+
+
+/* ======================================================================
+ * Tracking (STB) pieces
+ * ==================================================================== */
+
+/* STB::predictNext (STB.cpp:233-316) for tracks of len[i] points stored at
+ * pos[i][0..len[i]) (row stride `stride` points): normalised LMS fit of the
+ * last sample from the previous `order` samples, iterated while
+ * |error| > 1e-8 (SMALLNUMBER) for at most 5000 steps (WIENER_MAX_ITER),
+ * then the one-step-ahead prediction.  ok[i] = 0 for tracks under 3 points
+ * (the reference returns null). */
+int orc_stb_predict(const double *pos, const int32_t *len, int n_tr, int stride, double *out,
+                    int32_t *ok) {
+  for (int i = 0; i < n_tr; ++i) {
+    const int n = len[i];
+    if (n < 3) {
+      ok[i] = 0;
+      out[3 * i] = out[3 * i + 1] = out[3 * i + 2] = 0.0;
+      continue;
+    }
+    ok[i] = 1;
+    const int order = n < 6 ? n - 1 : 5;
+    const int start = n - 1 - order;
+    for (int axis = 0; axis < 3; ++axis) {
+      double series[6], filter[5];
+      for (int j = 0; j <= order; ++j)
+        series[j] = pos[((size_t)i * stride + start + j) * 3 + axis];
+      int shifted = 0;
+      if (fabs(series[order]) < 1.0) {
+        shifted = 1;
+        for (int j = 0; j <= order; ++j) series[j] += 10.0;
+      }
+      double denom = 0.0;
+      for (int j = 0; j < order; ++j) denom += series[j] * series[j];
+      const double step = (denom > 1e-12) ? (1.0 / denom) : 0.0;
+      for (int j = 0; j < order; ++j) filter[j] = 0.0;
+      double prediction = 0.0;
+      double error = series[order] - prediction;
+      int iter = 0;
+      while (fabs(error) > 1e-8 && iter < 5000 && step > 0.0) {
+        prediction = 0.0;
+        for (int j = 0; j < order; ++j) {
+          filter[j] += step * series[j] * error;
+          prediction += filter[j] * series[j];
+        }
+        error = series[order] - prediction;
+        ++iter;
+      }
+      if (order >= 1 && step == 0.0) {
+        prediction = series[order] + (series[order] - series[order - 1]);
+      } else {
+        prediction = 0.0;
+        for (int j = 0; j < order; ++j) prediction += filter[j] * series[j + 1];
+      }
+      if (shifted) prediction -= 10.0;
+      out[3 * i + axis] = prediction;
+    }
+  }
+  return 0;
+}
+
+/* STB::checkLinearFit (STB.cpp:874-924): a straight-line least-squares fit
+ * of the last min(n_init, 4) points per axis through the Matrix products
+ * inverse(X'X) X' y (Matrix.hpp:392-408 summation order); the track passes
+ * when the fitted last point lies within 5e-2 mm (MAX_ERR_LINEARFIT) of the
+ * measured one. */
+int orc_stb_linear_fit(int n_init, const double *pos, const int32_t *len, int n_tr, int stride,
+                       int32_t *ok) {
+  const int n_pts = n_init > 4 ? 4 : n_init;
+  double X[8], Xt[8], XtX[4] = {0, 0, 0, 0}, inv[4], K[8];
+  for (int i = 0; i < n_pts; ++i) {
+    X[2 * i] = 1;
+    X[2 * i + 1] = i;
+    Xt[i] = 1;
+    Xt[n_pts + i] = i;
+  }
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int k = 0; k < n_pts; ++k) XtX[2 * a + b] += Xt[a * n_pts + k] * X[2 * k + b];
+  gauss_inverse_n(XtX, 2, inv);
+  for (int a = 0; a < 2; ++a)
+    for (int j = 0; j < n_pts; ++j) {
+      K[a * n_pts + j] = 0.0;
+      for (int k = 0; k < 2; ++k) K[a * n_pts + j] += inv[2 * a + k] * Xt[k * n_pts + j];
+    }
+  for (int t = 0; t < n_tr; ++t) {
+    const int L = len[t];
+    if (L < n_pts) {
+      ok[t] = 0;
+      continue;
+    }
+    double est[3];
+    for (int a = 0; a < 3; ++a) {
+      double c0 = 0.0, c1 = 0.0;
+      for (int j = 0; j < n_pts; ++j)
+        c0 += K[j] * pos[((size_t)t * stride + L - n_pts + j) * 3 + a];
+      for (int j = 0; j < n_pts; ++j)
+        c1 += K[n_pts + j] * pos[((size_t)t * stride + L - n_pts + j) * 3 + a];
+      est[a] = c0 + (n_pts - 1) * c1;
+    }
+    double d2 = 0.0;
+    for (int a = 0; a < 3; ++a)
+      d2 += pow(est[a] - pos[((size_t)t * stride + L - 1) * 3 + a], 2);
+    ok[t] = sqrt(d2) > 5e-2 ? 0 : 1;
+  }
+  return 0;
+}
+
+/* PredField::fitPeakBinLocGauss (PredField.cpp:345-371) */
+static double pf_fit_gauss(double y1, double v1, double y2, double v2, double y3, double v3) {
+  const double l1 = v1 < LOGSMALLNUMBER ? log(LOGSMALLNUMBER) : log(v1);
+  const double l2 = v2 < LOGSMALLNUMBER ? log(LOGSMALLNUMBER) : log(v2);
+  const double l3 = v3 < LOGSMALLNUMBER ? log(LOGSMALLNUMBER) : log(v3);
+  double yc = -0.5 * ((l1 * (double)((y2 * y2) - (y3 * y3))) - (l2 * (double)((y1 * y1) - (y3 * y3))) +
+                      (l3 * (double)((y1 * y1) - (y2 * y2)))) /
+              ((l1 * (double)(y3 - y2)) - (l3 * (double)(y1 - y2)) + (l2 * (double)(y1 - y3)));
+  if (!isfinite(yc)) yc = y2;
+  return yc;
+}
+
+/* PredField::calPredField (PredField.cpp:21-86) + smoothDispField
+ * (:374-485), then getDisp (:91-165) at n_q points.  lim = view volume,
+ * grid nx x ny x nz (node id x*ny*nz + y*nz + z), half-width r, nbin bins
+ * per axis; field [nx*ny*nz][3]. */
+int orc_pred_field(const double *lim, int nx, int ny, int nz, double r, const int32_t *nbin,
+                   int smooth, const double *sigma, const double *prev, int n_prev,
+                   const double *curr, int n_curr, double *field, const double *q, int n_q,
+                   double *disp) {
+  const int ng = nx * ny * nz;
+  const int nb = nbin[0] * nbin[1] * nbin[2];
+  const int nbv[3] = {nbin[0], nbin[1], nbin[2]};
+  const int nn[3] = {nx, ny, nz};
+  double m[3], c[3];
+  for (int a = 0; a < 3; ++a) {
+    m[a] = (nbv[a] - 1) / (4 * r);
+    c[a] = (double)((nbv[a] - 1) / 2);
+  }
+  double *pdf = (double *)malloc(sizeof(double) * (size_t)nb);
+  int *lp = (int *)malloc(sizeof(int) * (size_t)(n_prev > 0 ? n_prev : 1));
+  int *lc = (int *)malloc(sizeof(int) * (size_t)(n_curr > 0 ? n_curr : 1));
+  if (!pdf || !lp || !lc) return 1;
+  for (int xi = 0; xi < nx; ++xi)
+    for (int yi = 0; yi < ny; ++yi)
+      for (int zi = 0; zi < nz; ++zi) {
+        const int g = xi * ny * nz + yi * nz + zi;
+        const double node[3] = {linspace_at(lim[0], lim[1], nx, xi),
+                                linspace_at(lim[2], lim[3], ny, yi),
+                                linspace_at(lim[4], lim[5], nz, zi)};
+        int np = 0, nc = 0;
+        for (int i = 0; i < n_prev; ++i) {
+          const double dx = prev[3 * i] - node[0], dy = prev[3 * i + 1] - node[1],
+                       dz = prev[3 * i + 2] - node[2];
+          if (dx < r && dx > -r && dy < r && dy > -r && dz < r && dz > -r) lp[np++] = i;
+        }
+        for (int i = 0; i < n_curr; ++i) {
+          const double dx = curr[3 * i] - node[0], dy = curr[3 * i + 1] - node[1],
+                       dz = curr[3 * i + 2] - node[2];
+          if (dx < r && dx > -r && dy < r && dy > -r && dz < r && dz > -r) lc[nc++] = i;
+        }
+        if (nc == 0 || np == 0) {
+          field[3 * g] = field[3 * g + 1] = field[3 * g + 2] = 0;
+          continue;
+        }
+        for (int b = 0; b < nb; ++b) pdf[b] = 0;
+        for (int j = 0; j < nc; ++j)
+          for (int k = 0; k < np; ++k) {
+            double d[3], bin[3];
+            int br[3];
+            for (int a = 0; a < 3; ++a) {
+              d[a] = curr[3 * lc[j] + a] - prev[3 * lp[k] + a];
+              bin[a] = m[a] * d[a] + c[a];
+              br[a] = (int)round(bin[a]);
+            }
+            for (int i = br[0] - 1 < 0 ? 0 : br[0] - 1; i <= (nbv[0] - 1 < br[0] + 1 ? nbv[0] - 1 : br[0] + 1); ++i) {
+              const double xx = i - bin[0];
+              for (int jj = br[1] - 1 < 0 ? 0 : br[1] - 1; jj <= (nbv[1] - 1 < br[1] + 1 ? nbv[1] - 1 : br[1] + 1); ++jj) {
+                const double yy = jj - bin[1];
+                for (int kk = br[2] - 1 < 0 ? 0 : br[2] - 1; kk <= (nbv[2] - 1 < br[2] + 1 ? nbv[2] - 1 : br[2] + 1); ++kk) {
+                  const double zz = kk - bin[2];
+                  pdf[(i * nbv[1] + jj) * nbv[2] + kk] += exp(-(xx * xx + yy * yy + zz * zz));
+                }
+              }
+            }
+          }
+        int best = 0;
+        for (int b = 1; b < nb; ++b)
+          if (pdf[b] > pdf[best]) best = b;
+        int pk[3] = {-1, -1, -1};
+        if (pdf[best] > 1e-1) {
+          pk[0] = best / (nbv[1] * nbv[2]);
+          pk[1] = (best - pk[0] * nbv[1] * nbv[2]) / nbv[2];
+          pk[2] = best % nbv[2];
+        }
+        for (int a = 0; a < 3; ++a) {
+          double v;
+          if (pk[a] < 0) v = 0.0;
+          else if (pk[a] == 0) v = -2 * r;
+          else if (pk[a] == nbv[a] - 1) v = 2 * r;
+          else {
+            int lo[3] = {pk[0], pk[1], pk[2]}, hi[3] = {pk[0], pk[1], pk[2]};
+            lo[a] -= 1;
+            hi[a] += 1;
+            const double loc = pf_fit_gauss(pk[a] - 1, pdf[(lo[0] * nbv[1] + lo[1]) * nbv[2] + lo[2]],
+                                            pk[a], pdf[(pk[0] * nbv[1] + pk[1]) * nbv[2] + pk[2]],
+                                            pk[a] + 1, pdf[(hi[0] * nbv[1] + hi[1]) * nbv[2] + hi[2]]);
+            v = (loc - c[a]) / m[a];
+          }
+          field[3 * g + a] = v;
+        }
+      }
+  free(pdf);
+  free(lp);
+  free(lc);
+  if (smooth) {
+    double *tmp = (double *)malloc(sizeof(double) * 3 * (size_t)ng);
+    if (!tmp) return 1;
+    for (int a = 0; a < 3; ++a) {
+      const int rad = (int)round(3 * sigma[a]);
+      double kern[64], sum = 0.0;
+      if (2 * rad + 1 > 64) {
+        free(tmp);
+        return 1;
+      }
+      for (int i = 0; i < 2 * rad + 1; ++i) {
+        const int x = i - rad;
+        kern[i] = exp(-(x * x) / (2 * sigma[a] * sigma[a]));
+        sum += kern[i];
+      }
+      for (int i = 0; i < 2 * rad + 1; ++i) kern[i] /= sum;
+      for (int dir = 0; dir < 3; ++dir) {
+        memcpy(tmp, field, sizeof(double) * 3 * (size_t)ng);
+        for (int x = 0; x < nx; ++x)
+          for (int y = 0; y < ny; ++y)
+            for (int z = 0; z < nz; ++z) {
+              const int p3[3] = {x, y, z};
+              double acc = 0.0;
+              for (int k = -rad; k <= rad; ++k) {
+                const int idx = p3[dir] + k;
+                if (idx >= 0 && idx < nn[dir]) {
+                  const int sx = dir == 0 ? idx : x, sy = dir == 1 ? idx : y, sz = dir == 2 ? idx : z;
+                  acc += tmp[3 * (sx * ny * nz + sy * nz + sz) + a] * kern[k + rad];
+                }
+              }
+              field[3 * (x * ny * nz + y * nz + z) + a] = acc;
+            }
+      }
+    }
+    free(tmp);
+  }
+  const double dx = (lim[1] - lim[0]) / (nx - 1), dy = (lim[3] - lim[2]) / (ny - 1),
+               dz = (lim[5] - lim[4]) / (nz - 1);
+  for (int i = 0; i < n_q; ++i) {
+    const double px = fmin(fmax(q[3 * i], lim[0]), lim[1]);
+    const double py = fmin(fmax(q[3 * i + 1], lim[2]), lim[3]);
+    const double pz = fmin(fmax(q[3 * i + 2], lim[4]), lim[5]);
+    int xi = (int)floor((px - lim[0]) / dx), yi = (int)floor((py - lim[2]) / dy),
+        zi = (int)floor((pz - lim[4]) / dz);
+    xi = xi < 0 ? 0 : xi;
+    yi = yi < 0 ? 0 : yi;
+    zi = zi < 0 ? 0 : zi;
+    xi = xi > nx - 2 ? nx - 2 : xi;
+    yi = yi > ny - 2 ? ny - 2 : yi;
+    zi = zi > nz - 2 ? nz - 2 : zi;
+    const double cell[6] = {linspace_at(lim[0], lim[1], nx, xi), linspace_at(lim[0], lim[1], nx, xi + 1),
+                            linspace_at(lim[2], lim[3], ny, yi), linspace_at(lim[2], lim[3], ny, yi + 1),
+                            linspace_at(lim[4], lim[5], nz, zi), linspace_at(lim[4], lim[5], nz, zi + 1)};
+#define G3(a, b, cc) ((a) * ny * nz + (b) * nz + (cc))
+    const int ids[8] = {G3(xi, yi, zi),         G3(xi + 1, yi, zi),     G3(xi + 1, yi, zi + 1),
+                        G3(xi, yi, zi + 1),     G3(xi, yi + 1, zi),     G3(xi + 1, yi + 1, zi),
+                        G3(xi + 1, yi + 1, zi + 1), G3(xi, yi + 1, zi + 1)};
+#undef G3
+    const double pt[3] = {px, py, pz};
+    for (int a = 0; a < 3; ++a) {
+      double v[8];
+      for (int k = 0; k < 8; ++k) v[k] = field[3 * ids[k] + a];
+      disp[3 * i + a] = trilinear(cell, v, pt);
+    }
+  }
+  return 0;
 }

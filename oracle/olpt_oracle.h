@@ -50,6 +50,16 @@ int orc_run_ipr(const olpt_camera *cams, int n_cam, const double *imgs, int H,
                 double *obj_xyz, double *obj_uv, int cap_loop,
                 int32_t *loop_total, int32_t *counts);
 
+/* tracking (STB) pieces */
+int orc_stb_predict(const double *pos, const int32_t *len, int n_tr, int stride, double *out,
+                    int32_t *ok);
+int orc_stb_linear_fit(int n_init, const double *pos, const int32_t *len, int n_tr, int stride,
+                       int32_t *ok);
+int orc_pred_field(const double *lim, int nx, int ny, int nz, double r, const int32_t *nbin,
+                   int smooth, const double *sigma, const double *prev, int n_prev,
+                   const double *curr, int n_curr, double *field, const double *q, int n_q,
+                   double *disp);
+
 #ifdef __cplusplus
 }
 #endif

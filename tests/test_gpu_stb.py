@@ -132,3 +132,87 @@ def test_stb_run_f32_close(tmp_path, gpu_available, gold):
     want = int(gold["tr_long_active_n"])
     assert abs(r["n_tracks"] - want) <= max(2, int(0.02 * want))
     stb.close()
+
+
+# ---------------------------------------------------------------------------
+# larger inputs against the oracle's restatements (oracle/olpt_oracle.c,
+# pinned to the same golden vectors by tests/test_oracle_stb.py)
+# ---------------------------------------------------------------------------
+def _tracks(n, max_len, seed):
+    g = np.random.default_rng(seed)
+    lens = g.integers(2, max_len + 1, n).astype(np.int32)
+    t = np.arange(max_len)[None, :, None]
+    pos = (g.uniform(-20, 20, (n, 1, 3)) + g.normal(0, 0.3, (n, 1, 3)) * t +
+           g.normal(0, 0.02, (n, 1, 3)) * t * t + g.normal(0, 0.01, (n, max_len, 3)))
+    pos[::13, :, 1] = g.uniform(-0.9, 0.9, (len(pos[::13]), 1))   # shifted branch
+    return pos, lens
+
+
+def test_predict_next_vs_oracle_100k(ctx):
+    import oracle as O
+    from pyopenlpt_amd.stb import predict_next
+    pos, lens = _tracks(100_000, 12, 21)
+    want = np.zeros((len(lens), 3))
+    wok = np.zeros(len(lens), np.int32)
+    O.load_port().orc_stb_predict(O.dptr(pos), O.iptr(lens), len(lens), pos.shape[1],
+                                  O.dptr(want), O.iptr(wok))
+    m = np.minimum(lens, 6)
+    win = np.zeros((len(lens), 6, 3))
+    for k in range(2, 13):
+        sel = lens == k
+        mm = min(k, 6)
+        win[sel, :mm] = pos[sel, k - mm:k]
+    got, ok = predict_next(ctx, win, m)
+    assert np.array_equal(ok, wok)
+    assert np.array_equal(got[ok == 1], want[wok == 1])
+
+
+def test_linear_fit_vs_oracle_100k(ctx):
+    import oracle as O
+    from pyopenlpt_amd.stb import check_linear_fit
+    pos, lens = _tracks(100_000, 10, 22)
+    want = np.zeros(len(lens), np.int32)
+    O.load_port().orc_stb_linear_fit(4, O.dptr(pos), O.iptr(lens), len(lens), pos.shape[1],
+                                     O.iptr(want))
+    win = np.zeros((len(lens), 4, 3))
+    for k in range(2, 11):
+        sel = lens == k
+        mm = min(k, 4)
+        win[sel, :mm] = pos[sel, k - mm:k]
+    got = check_linear_fit(ctx, 4, win, lens)
+    assert np.array_equal(got, want)
+    assert 0.2 < want.mean() < 0.9
+
+
+def test_pred_field_vs_oracle_31cube(ctx):
+    import ctypes
+    import oracle as O
+    from pyopenlpt_amd import synth as S
+    from pyopenlpt_amd.stb import PredField
+    g = np.random.default_rng(23)
+    prev = g.uniform(-20, 20, (5000, 3))
+    curr = prev + np.stack([0.3 * np.sin(prev[:, 2] / 6), 0.2 * np.cos(prev[:, 0] / 7),
+                            0.1 * np.ones(len(prev))], 1) + g.normal(0, 0.01, prev.shape)
+    curr[::10] = g.uniform(-20, 20, (len(curr[::10]), 3))
+    q = g.uniform(-21, 21, (2000, 3))
+    lim = np.array([-20.0, 20, -20, 20, -20, 20])
+    r = 1.5
+    sp = S.OlptStbParams()
+    for i, v in enumerate(lim):
+        sp.limit[i] = v
+    sp.pf_nx = sp.pf_ny = sp.pf_nz = 31
+    sp.pf_r_mm = r
+    for i in range(3):
+        sp.pf_nbin[i] = 11
+        sp.pf_sigma[i] = 1.0
+    sp.pf_smooth = 1
+    field, disp = PredField(ctx, sp).calPredField(prev, curr, q)
+    wf = np.zeros((31 ** 3, 3))
+    wd = np.zeros((len(q), 3))
+    O.load_port().orc_pred_field(O.dptr(lim), 31, 31, 31, ctypes.c_double(r),
+                                 O.iptr(np.array([11, 11, 11], np.int32)), 1,
+                                 O.dptr(np.ones(3)), O.dptr(prev), len(prev), O.dptr(curr),
+                                 len(curr), O.dptr(wf), O.dptr(q), len(q), O.dptr(wd))
+    assert np.abs(field - wf).max() <= 1e-9
+    assert np.abs(disp - wd).max() <= 1e-9
+    assert np.abs(wf).max() > 0.1
