@@ -281,12 +281,19 @@ __device__ inline void otf_param(const OtfT<T> &o, int cam, const T X[3], T out[
 
 // Gaussian OTF intensity at pixel (col,row) for a particle at (u,v)
 // (TracerShakeStrategy::gaussIntensity, Shake.cpp:901-914).
+// exp of the Gaussian intensity: FP64 keeps the correctly rounded exp (the
+// reference's); FP32 uses the hardware exp2 (v_exp_f32) on x log2(e), whose
+// relative error (~1e-6 over the ROI's exponent range) is far below the
+// FP32 loop's own residual rounding.
+__device__ __forceinline__ double gexp(double x) { return exp(x); }
+__device__ __forceinline__ float gexp(float x) { return __expf(x); }
+
 template <typename T>
 __device__ __forceinline__ T gauss_int(const T P[5], T u, T v, int row, int col) {
   const T dx = T(col) - u, dy = T(row) - v;
   const T xx = dx * P[3] + dy * P[4];
   const T yy = -dx * P[4] + dy * P[3];
-  const T val = P[0] * exp(-(P[1] * xx * xx + P[2] * yy * yy));
+  const T val = P[0] * gexp(-(P[1] * xx * xx + P[2] * yy * yy));
   return val > T(0) ? val : T(0);
 }
 
