@@ -12,7 +12,6 @@ from __future__ import annotations
 
 import ctypes
 import dataclasses
-import os
 import sys
 
 import numpy as np
@@ -109,12 +108,12 @@ class STB:
         return dict(n_tracks=nt.value, track=tid[:n], frame=fr[:n], xyz=xyz[:n])
 
     def _tracks(self, which: int) -> list[Track]:
+        """Rows come track by track in list order (every track non-empty)."""
         r = self.rows(which)
-        out = []
-        for i in range(r["n_tracks"]):
-            m = r["track"] == i
-            out.append(Track(r["frame"][m].tolist(), r["xyz"][m]))
-        return out
+        tid = r["track"]
+        bounds = np.r_[0, np.flatnonzero(np.diff(tid)) + 1, len(tid)]
+        return [Track(r["frame"][a:b].tolist(), r["xyz"][a:b])
+                for a, b in zip(bounds[:-1], bounds[1:])]
 
     @property
     def _short_track_active(self):
