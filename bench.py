@@ -292,6 +292,10 @@ def main() -> int:
     ap.add_argument("--graph", action="store_true", help="replay a captured hipGraph")
     ap.add_argument("--streams", type=int, default=4,
                     help="frames of a step run concurrently on this many HIP streams")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes instead of timing candidates "
+                         "on the first frames (untimed) and keeping the fastest")
+    ap.add_argument("--autotune-frames", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--roofline-steps", type=int, default=1,
                     help="single-stream steps timed per k_shake launch for the roofline")
@@ -337,6 +341,16 @@ def main() -> int:
     cap_obj = 1 << 17
     outs = ctx.alloc_outputs(args.frames, cap_obj, dimg.device)
     stream = torch.cuda.current_stream()
+    tune_log = None
+    if not args.no_autotune:
+        nt = min(args.frames, args.autotune_frames)
+        t0 = time.perf_counter()
+        best, tune_log = ctx.autotune(dimg[:nt], code, w0.params, nt)
+        for t, sec in tune_log:
+            log(f"  {t}: {1000 * sec:.1f} ms")
+        log(f"autotune over {len(tune_log)} shapes on {nt} frames "
+            f"({time.perf_counter() - t0:.1f} s): {best}")
+    tuning = ctx.get_tuning()
 
     def timed_leg(params, label):
         """Warm-up, then args.steps timed steps with events on the context
@@ -405,7 +419,7 @@ def main() -> int:
         torch.cuda.synchronize()
         times = ctx.stage_times()
         ctx.stage_timing(False)
-        ctx.set_concurrency(args.streams)
+        ctx.set_concurrency(tuning["n_streams"])
 
     # ---- learned proposal + 1 plain iteration (from 0.03 vox)
     learned = None
@@ -487,7 +501,8 @@ def main() -> int:
                    "frames_per_step": args.frames, "particles_per_frame": len(w0.truth),
                    "ipr_loops": int(w0.params.n_loop_ipr),
                    "shake": "learned" if use_learned else "plain",
-                   "graph": bool(args.graph), "streams": args.streams,
+                   "graph": bool(args.graph), "streams": tuning["n_streams"],
+                   "tuning": tuning, "autotuned": not args.no_autotune,
                    "plain": plain, "learned": learned,
                    "learned_over_plain": (round(learned["particles_per_s"] / plain["particles_per_s"], 3)
                                           if learned else None)},

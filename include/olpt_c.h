@@ -58,6 +58,25 @@ olpt_status olpt_set_precision(olpt_ctx *ctx, int precision);
  * the context stream.  Each extra stream holds a full workspace, as sized
  * by olpt_ctx_reserve.  n_streams: 1..8. */
 olpt_status olpt_set_concurrency(olpt_ctx *ctx, int n_streams);
+/* No counterpart in the reference.  Launch shapes that change how the work
+ * is scheduled, never what it computes: every kernel they size walks its
+ * work with a grid-stride loop, and every result whose order depends on
+ * scheduling is ranked afterwards.  shake_blocks: one-wave workgroups of
+ * the shake and proposal kernels (default 4096); build_blocks: 256-thread
+ * workgroups of the stereo build kernels (default 4096); prune_blocks: cap
+ * on the 256-thread workgroups of the sort/prune/triangulate kernels
+ * (default 2048); n_streams: as olpt_set_concurrency.  Each field must lie
+ * in its range (blocks 64..65536, streams 1..8); a captured graph is
+ * dropped.  Python's Context.autotune times candidates on the caller's
+ * frames and keeps the fastest. */
+typedef struct {
+  int32_t n_streams;
+  int32_t shake_blocks;
+  int32_t build_blocks;
+  int32_t prune_blocks;
+} olpt_tuning;
+olpt_status olpt_set_tuning(olpt_ctx *ctx, const olpt_tuning *tuning);
+olpt_status olpt_get_tuning(olpt_ctx *ctx, olpt_tuning *tuning);
 /* Replaces Camera objects (Camera.h:78-174, pinhole model). */
 olpt_status olpt_set_cameras(olpt_ctx *ctx, const olpt_camera *cams, int n_cam);
 /* Replaces TracerConfig::_otf (OTF.h:33-78); arrays are copied. */

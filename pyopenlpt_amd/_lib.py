@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes
 import os
 
-from .synth import OlptCamera, OlptIprParams, OlptOtf, OlptStbParams
+from .synth import OlptCamera, OlptIprParams, OlptOtf, OlptStbParams, OlptTuning
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libolpt_hip.so")
@@ -57,6 +57,8 @@ def lib() -> ctypes.CDLL:
     L.olpt_ctx_reserve.argtypes = [_P, _I, _I, _I, _I, _I, _I]
     L.olpt_set_precision.argtypes = [_P, _I]
     L.olpt_set_concurrency.argtypes = [_P, _I]
+    L.olpt_set_tuning.argtypes = [_P, ctypes.POINTER(OlptTuning)]
+    L.olpt_get_tuning.argtypes = [_P, ctypes.POINTER(OlptTuning)]
     L.olpt_set_cameras.argtypes = [_P, ctypes.POINTER(OlptCamera), _I]
     L.olpt_set_otf.argtypes = [_P, ctypes.POINTER(OlptOtf)]
     L.olpt_sync.argtypes = [_P]
@@ -121,7 +123,8 @@ def lib() -> ctypes.CDLL:
                  "olpt_mlp_train_step"):
         getattr(L, name).restype = ctypes.c_int32
     for name in ("olpt_ctx_create", "olpt_ctx_destroy", "olpt_ctx_reserve",
-                 "olpt_set_precision", "olpt_set_concurrency", "olpt_set_cameras",
+                 "olpt_set_precision", "olpt_set_concurrency", "olpt_set_tuning",
+                 "olpt_get_tuning", "olpt_set_cameras",
                  "olpt_set_otf",
                  "olpt_sync", "olpt_diagnostics", "olpt_find_peaks",
                  "olpt_stereo_match", "olpt_shake", "olpt_residual",

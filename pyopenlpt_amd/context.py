@@ -12,7 +12,7 @@ import torch
 
 from . import _lib
 from ._lib import check
-from .synth import OlptCamera, OlptIprParams, OlptOtf
+from .synth import OlptCamera, OlptIprParams, OlptOtf, OlptTuning
 
 _DTYPE_CODE = {torch.uint8: _lib.OLPT_U8, torch.int16: _lib.OLPT_U16,
                torch.float32: _lib.OLPT_F32, torch.float64: _lib.OLPT_F64}
@@ -87,6 +87,68 @@ class Context:
         """Spread the frames of one ipr_run over n_streams workspaces, each
         on its own HIP stream (1 = every frame on the context stream)."""
         check(self.L.olpt_set_concurrency(self.h, int(n_streams)))
+
+    def set_tuning(self, **kw):
+        """Change launch shapes (olpt_set_tuning): any of n_streams,
+        shake_blocks, build_blocks, prune_blocks; the rest stay."""
+        t = self.get_tuning()
+        for k, v in kw.items():
+            if k not in t:
+                raise KeyError(k)
+            t[k] = int(v)
+        check(self.L.olpt_set_tuning(self.h, ctypes.byref(OlptTuning(**t))))
+
+    def get_tuning(self) -> dict:
+        t = OlptTuning()
+        check(self.L.olpt_get_tuning(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    TUNING_SPACE = {
+        "n_streams": (1, 2, 4, 6, 8),
+        "shake_blocks": (1024, 2048, 4096, 8192, 16384),
+        "build_blocks": (1024, 2048, 4096, 8192),
+        "prune_blocks": (128, 256, 512, 1024, 2048, 4096),
+    }
+
+    def autotune(self, imgs: torch.Tensor, dtype: int, params: OlptIprParams,
+                 n_frames: int, space: dict | None = None, reps: int = 2,
+                 passes: int = 1) -> tuple[dict, list]:
+        """Time olpt_ipr_run on the caller's frames over launch shapes and
+        keep the fastest (coordinate descent: one knob at a time, the others
+        held at the best so far).  The shapes do not change results, only
+        speed.  Returns (best tuning, [(tuning, seconds per call)])."""
+        import time
+        space = space or self.TUNING_SPACE
+        outputs = self.alloc_outputs(n_frames, 1 << 18, imgs.device)
+        log = []
+
+        def measure(t):
+            self.set_tuning(**t)
+            self.ipr_run(imgs, dtype, params, n_frames, outputs=outputs)
+            torch.cuda.synchronize()
+            best = float("inf")
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                self.ipr_run(imgs, dtype, params, n_frames, outputs=outputs)
+                torch.cuda.synchronize()
+                best = min(best, time.perf_counter() - t0)
+            log.append((dict(t), best))
+            return best
+
+        best_t = self.get_tuning()
+        best_s = measure(best_t)
+        for _ in range(passes):
+            for knob, values in space.items():
+                for v in values:
+                    if v == best_t[knob]:
+                        continue
+                    cand = dict(best_t, **{knob: v})
+                    sec = measure(cand)
+                    if sec < best_s:
+                        best_t, best_s = cand, sec
+        self.set_tuning(**best_t)
+        del outputs
+        return best_t, log
 
     @property
     def real_dtype(self):

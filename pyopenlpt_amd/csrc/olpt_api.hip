@@ -122,6 +122,8 @@ struct olpt_ctx {
   // host time spent issuing olpt_ipr_run's work (not waiting for it)
   double host_ms = 0.0;
   int32_t host_calls = 0;
+  // launch shapes (olpt_set_tuning)
+  olpt_tuning tune = {1, 4096, 4096, 2048};
 };
 
 namespace {
@@ -275,6 +277,7 @@ template <typename T> OtfT<T> dotf(olpt_ctx *c) {
 template <typename T>
 ShakeBuffers<T> shake_buffers(olpt_ctx *c, Workspace &w, T *img, T *img_next) {
   ShakeBuffers<T> b{};
+  b.grid_shake = c->tune.shake_blocks;
   b.n_cam = c->n_cam;
   b.H = c->H;
   b.W = c->W;
@@ -694,6 +697,8 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   b.scan_scratch = w.scan_scratch;
   A(w.totals, int32_t, 64);
   b.totals = w.totals;
+  b.grid_build = c->tune.build_blocks;
+  b.grid_prune = c->tune.prune_blocks;
   // shake
   if ((s = dalloc(c, &w.sh_uv, 8 * (size_t)O * nc * 2))) return s;
   if ((s = dalloc(c, &w.keep_uv, 8 * (size_t)O * nc * 2))) return s;
@@ -830,6 +835,31 @@ olpt_status olpt_set_concurrency(olpt_ctx *c, int n_streams) {
   if (n_streams > c->n_ws_want) c->n_ws_want = n_streams;
   c->n_ws_active = n_streams;
   return ensure_workspaces(c);
+}
+
+olpt_status olpt_set_tuning(olpt_ctx *c, const olpt_tuning *t) {
+  if (!c || !t) return fail(OLPT_E_INVALID_ARGUMENT, "null argument");
+  if (c->capturing) return fail(OLPT_E_INVALID_ARGUMENT, "cannot change tuning while capturing");
+  const int32_t blk[3] = {t->shake_blocks, t->build_blocks, t->prune_blocks};
+  for (int i = 0; i < 3; ++i)
+    if (blk[i] < 64 || blk[i] > 65536)
+      return fail(OLPT_E_INVALID_ARGUMENT, "block counts must be within [64, 65536]");
+  if (t->n_streams < 1 || t->n_streams > kMaxWorkspaces)
+    return fail(OLPT_E_INVALID_ARGUMENT, "n_streams must be within [1, 8]");
+  free_graphs(c);
+  c->tune = *t;
+  for (int k = 0; k < c->n_ws; ++k) {
+    c->ws[k].sb.grid_build = t->build_blocks;
+    c->ws[k].sb.grid_prune = t->prune_blocks;
+  }
+  return olpt_set_concurrency(c, t->n_streams);
+}
+
+olpt_status olpt_get_tuning(olpt_ctx *c, olpt_tuning *t) {
+  if (!c || !t) return fail(OLPT_E_INVALID_ARGUMENT, "null argument");
+  *t = c->tune;
+  t->n_streams = c->n_ws_active;
+  return OLPT_OK;
 }
 
 olpt_status olpt_set_cameras(olpt_ctx *c, const olpt_camera *cams, int n_cam) {

@@ -135,6 +135,8 @@ struct StereoBuffers {
   int32_t *work;         // dynamic work counter
   int32_t *scan_scratch;
   int32_t *totals;       // small scratch for scan totals
+  int grid_build;        // workgroups of the build kernels
+  int grid_prune;        // cap on workgroups of the sort/prune kernels
 };
 
 struct StereoParams {
@@ -181,6 +183,7 @@ template <typename T> struct ShakeBuffers {
   // learned proposal (optional)
   void *patches;      // [cap_obj][512] bf16 network input
   float *prop;        // [cap_obj][4] network output
+  int grid_shake;     // one-wave workgroups of the shake/proposal kernels
 };
 
 struct ShakeParams {

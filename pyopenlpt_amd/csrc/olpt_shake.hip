@@ -911,7 +911,7 @@ template <typename T, int NC, int OPW, int SLOTS>
 static void shake_iter(const ShakeBuffers<T> &b, const ShakeParams &p,
                        const ObjCamRec<T> *rec, T delta, hipStream_t st) {
   stage_begin(ST_SHAKE, st);
-  k_shake<T, NC, OPW, SLOTS><<<256 * 16, kWave, 0, st>>>(b, p, rec, delta);
+  k_shake<T, NC, OPW, SLOTS><<<b.grid_shake, kWave, 0, st>>>(b, p, rec, delta);
   stage_end(ST_SHAKE, st);
 }
 
@@ -944,7 +944,7 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
         b, p, rec, (__bf16 *)b.patches);
     mlp_infer_device(p.mlp, b.patches, b.n_obj, b.cap_obj, b.prop, st);
     constexpr int O = shake_opw<T>();
-    k_propose<T, 4, O, 4 * O><<<256 * 16, kWave, 0, st>>>(b, p, rec, b.prop);
+    k_propose<T, 4, O, 4 * O><<<b.grid_shake, kWave, 0, st>>>(b, p, rec, b.prop);
   }
   double delta = p.shake_width;
   double dmin = p.shakewidth_min > 0 ? p.shakewidth_min : delta / 20;

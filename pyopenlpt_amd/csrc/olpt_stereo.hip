@@ -1273,8 +1273,8 @@ static int grid_for(long n, int threads, int max_blocks) {
 template <int K>
 static void build_level(const StereoBuffers &b, const StereoParams &p, hipStream_t st) {
   zero_i32(b.n_pairs, 1, st);
-  k_b_enum<K><<<4096, 256, 0, st>>>(b, p);
-  k_b_extend<K><<<4096, 256, 0, st>>>(b, p);
+  k_b_enum<K><<<b.grid_build, 256, 0, st>>>(b, p);
+  k_b_extend<K><<<b.grid_build, 256, 0, st>>>(b, p);
 }
 
 void launch_stereo(const StereoBuffers &b, const StereoParams &p,
@@ -1296,7 +1296,7 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   k_cell_fill<<<gc, T, 0, st>>>(b, p);
   // build: level 1 from every reference observation, then one enumerate +
   // extend pair of kernels per deeper build camera
-  const int G = 4096;
+  const int G = b.grid_build;
   stage_begin(ST_BUILD, st);
   zero_i32(b.n_surv, kMaxCam, st);
   zero_i32(b.n_pairs, 1, st);
@@ -1312,7 +1312,7 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   stage_end(ST_BUILD, st);
   stage_begin(ST_PRUNE, st);
   // sort
-  const int gM = grid_for(b.cap_cand, T, 2048);
+  const int gM = grid_for(b.cap_cand, T, b.grid_prune);
   zero_i32(b.key_cnt, b.cap, st);
   k_key_hist<<<gM, T, 0, st>>>(b, p);
   scan_exclusive(b.key_cnt, b.key_start, b.cap, 1, b.cap, nullptr, b.scan_scratch, st);

@@ -165,3 +165,51 @@ def test_graph_with_streams_matches_eager(case):
             ctx.ipr_run(d, code, case.params, n_frames=3, cap_obj=1 << 14, outputs=outs_e)
             s.synchronize()
     ctx.close()
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_tuning_leaves_results_unchanged(case, precision):
+    """Launch shapes (olpt_set_tuning) change the schedule only: the IPR
+    objects, their order and the loop counts are identical under each."""
+    from pyopenlpt_amd.context import to_device_image
+    imgs = np.stack([case.images, case.images[:, ::-1, :].copy(), case.images])
+    d, code = to_device_image(imgs)
+    ctx = make_ctx(case, precision)
+    ref = None
+    for t in ({}, dict(shake_blocks=64, build_blocks=64, prune_blocks=64),
+              dict(n_streams=3, shake_blocks=16384, build_blocks=65536, prune_blocks=8192),
+              dict(n_streams=2, shake_blocks=1000, build_blocks=777, prune_blocks=129)):
+        ctx.set_tuning(**t)
+        (ox, _u, n_obj), lt = ctx.ipr_run(d, code, case.params, n_frames=3, cap_obj=1 << 14)
+        torch.cuda.synchronize()
+        got = (n_obj.cpu(), [ox[f, :int(n_obj[f].item())].cpu() for f in range(3)], lt)
+        if ref is None:
+            ref = got
+            assert int(ref[0].sum()) > 0
+            continue
+        assert torch.equal(got[0], ref[0]), t
+        for f in range(3):
+            assert torch.equal(got[1][f], ref[1][f]), (t, f)
+        assert np.array_equal(got[2], ref[2]), t
+    assert ctx.get_tuning() == dict(n_streams=2, shake_blocks=1000, build_blocks=777,
+                                    prune_blocks=129)
+    ctx.close()
+
+
+def test_tuning_rejects_out_of_range_and_autotune_keeps_a_candidate(case):
+    from pyopenlpt_amd._lib import FatalError
+    from pyopenlpt_amd.context import to_device_image
+    ctx = make_ctx(case, "f32")
+    before = ctx.get_tuning()
+    for bad in (dict(shake_blocks=0), dict(build_blocks=1 << 20), dict(n_streams=9)):
+        with pytest.raises(FatalError) as e:
+            ctx.set_tuning(**bad)
+        assert e.value.status == 1
+    assert ctx.get_tuning() == before
+    d, code = to_device_image(np.stack([case.images, case.images]))
+    space = {"n_streams": (1, 2), "shake_blocks": (1024, 4096), "prune_blocks": (512, 2048)}
+    best, log = ctx.autotune(d, code, case.params, 2, space=space, reps=1)
+    assert ctx.get_tuning() == best
+    assert len(log) == 4
+    assert all(best[k] in v for k, v in space.items())
+    ctx.close()
