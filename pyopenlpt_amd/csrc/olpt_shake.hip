@@ -305,6 +305,15 @@ __device__ void polyfit2(const double *x, const double *y, int n, double c[3]) {
   }
 }
 
+// Carry column overflow into rows for a row-major walk of a ROI of width
+// w >= 1: (ro, co) becomes the (row, column) offset of pixel ro * w + co.
+__device__ __forceinline__ void roi_wrap(int w, int &ro, int &co) {
+  while (co >= w) {
+    co -= w;
+    ++ro;
+  }
+}
+
 // Per-lane state of one object in one camera (the lane's group camera).
 template <typename T, int SLOTS> struct CamLane {
   int obase;       // first lane of this lane's object
@@ -334,18 +343,24 @@ __device__ __forceinline__ T eval_residue(const ShakeBuffers<T> &b, const CamLan
     otf_param(b.otf, c.k, Xe, P);
     const int prmin = (int)floor(ve - r_px), prmax = (int)ceil(ve + r_px + T(1));
     const int pcmin = (int)floor(ue - r_px), pcmax = (int)ceil(ue + r_px + T(1));
+    // pixel sl * LPG + sub of the ROI in row-major order (roiw >= 1 here),
+    // stepped LPG columns at a time instead of divided out per slot
+    int ro = 0, co = sub;
+    roi_wrap(c.roiw, ro, co);
 #pragma unroll
     for (int sl = 0; sl < SLOTS; ++sl) {
       const int idx = sl * LPG + sub;
       if (idx < c.npix) {
-        const int row = c.roi.rmin + idx / c.roiw;
-        const int col = c.roi.cmin + idx % c.roiw;
+        const int row = c.roi.rmin + ro;
+        const int col = c.roi.cmin + co;
         T pred = T(0);
         if (row >= prmin && row < prmax && col >= pcmin && col < pcmax)
           pred = gauss_int<T>(P, ue, ve, row, col);
         const T d = c.aug[sl] - pred;
         sse += d * d;
       }
+      co += LPG;
+      roi_wrap(c.roiw, ro, co);
     }
   }
   sse = group_sum<LPG>(sse);
@@ -402,13 +417,16 @@ __device__ __forceinline__ void build_roi(const ShakeBuffers<T> &b, const ShakeP
   if (roi.empty()) return;
   c.roiw = roi.cmax - roi.cmin;
   c.npix = (roi.rmax - roi.rmin) * c.roiw;
+  if (c.npix <= 0) return;
   const T rP[5] = {r.P[0], r.P[1], r.P[2], r.P[3], r.P[4]};
+  int ro = 0, co = sub;
+  roi_wrap(c.roiw, ro, co);
 #pragma unroll
-  for (int sl = 0; sl < SLOTS; ++sl) {
+  for (int sl = 0; sl < SLOTS; ++sl, co += LPG, roi_wrap(c.roiw, ro, co)) {
     const int idx = sl * LPG + sub;
     if (idx < c.npix) {
-      const int row = roi.rmin + idx / c.roiw;
-      const int col = roi.cmin + idx % c.roiw;
+      const int row = roi.rmin + ro;
+      const int col = roi.cmin + co;
       T a = b.res[c.k * HW + (long)row * b.W + col];
       if (row >= r.rmin && row < r.rmax && col >= r.cmin && col < r.cmax)
         a += gauss_int<T>(rP, r.u, r.v, row, col);

@@ -345,7 +345,8 @@ __device__ __forceinline__ void row_range(const int32_t *cstart, const int32_t *
 // address serialise across the XCDs' L2 slices, so a per-wave atomic on the
 // list counter is what limits a list-building kernel; per-block batches
 // also make the global writes contiguous.  All threads of the block call
-// bq_push (at most one record each) and bq_flush in the same order.
+// bq_push (at most R records each between flushes) and bq_flush<R> in the
+// same order.
 template <typename Rec, int CAP> struct BlockQueue {
   Rec buf[CAP];
   int n, base;
@@ -357,7 +358,7 @@ __device__ __forceinline__ void bq_init(BlockQueue<Rec, CAP> &q) {
   __syncthreads();
 }
 
-// Room for one record per thread is guaranteed by bq_flush's threshold.
+// Room for R records per thread is guaranteed by bq_flush<R>'s threshold.
 // One LDS atomic per wave: the wave's pushing lanes take consecutive slots
 // in lane order.
 template <typename Rec, int CAP>
@@ -371,13 +372,14 @@ __device__ __forceinline__ void bq_push(BlockQueue<Rec, CAP> &q, bool want, cons
   if (want) q.buf[base + mbcnt(m)] = r;
 }
 
-// Flush when fewer than blockDim free slots remain (or any, with `all`).
-template <typename Rec, int CAP, typename Out>
+// Flush when fewer than R * blockDim free slots remain (or any, with `all`).
+template <int R = 1, typename Rec, int CAP, typename Out>
 __device__ __forceinline__ void bq_flush(BlockQueue<Rec, CAP> &q, int32_t *counter, long cap,
                                          int32_t *overflow, bool all, Out out) {
+  static_assert(R * 256 <= CAP / 2, "queue too small for R records per thread of 256");
   __syncthreads();
   const int n = q.n;
-  if (all ? n > 0 : n > CAP - (int)blockDim.x) {
+  if (all ? n > 0 : n > CAP - R * (int)blockDim.x) {
     if (threadIdx.x == 0) q.base = atomicAdd(counter, n);
     __syncthreads();
     const int base = q.base;
@@ -399,6 +401,7 @@ struct CandRec {
 };
 
 constexpr int kQueue = 2048;      // pair queue capacity (records)
+constexpr int kPtBatch = 4;       // points tested per thread between flushes
 constexpr int kSurvQueue = 1024;
 constexpr int kCandQueue = 512;
 
@@ -630,15 +633,18 @@ __global__ void __launch_bounds__(256) k_b_band(StereoBuffers b, StereoParams p)
       xy += (long)t1 * b.cap * 2;
       cpid += (long)t1 * b.cap;
     }
+    // kPtBatch points per barrier: their loads are in flight together
     while (__syncthreads_or(cur.kk < cur.kend)) {
-      bool acc = false;
-      int pid = 0;
-      if (cur.kk < cur.kend) {
-        pid = cpid[cur.kk++];
-        acc = in_strip(s, xy[2 * pid], xy[2 * pid + 1], tol2);
+      int pid[kPtBatch];
+#pragma unroll
+      for (int u = 0; u < kPtBatch; ++u) pid[u] = cur.kk + u < cur.kend ? cpid[cur.kk + u] : -1;
+      cur.kk = min(cur.kk + kPtBatch, cur.kend);
+#pragma unroll
+      for (int u = 0; u < kPtBatch; ++u) {
+        const bool acc = pid[u] >= 0 && in_strip(s, xy[2 * pid[u]], xy[2 * pid[u] + 1], tol2);
+        bq_push(q, acc, make_int2(r, pid[u]));
       }
-      bq_push(q, acc, make_int2(r, pid));
-      bq_flush(q, b.n_pairs, b.cap_pairs, &b.flags[1], false, out);
+      bq_flush<kPtBatch>(q, b.n_pairs, b.cap_pairs, &b.flags[1], false, out);
     }
   }
   bq_flush(q, b.n_pairs, b.cap_pairs, &b.flags[1], true, out);
@@ -681,11 +687,14 @@ __global__ void __launch_bounds__(256) k_b_enum(StereoBuffers b, StereoParams p)
     const int32_t *ccnt = b.cell_cnt + t * ncell;
     const int32_t *cpid = b.cell_pid + (long)t * b.cap;
     const double *xy = b.xy + (long)t * b.cap * 2;
+    // kPtBatch points per barrier (across cell rows): their loads are in
+    // flight together
     while (__syncthreads_or(live)) {
-      bool acc = false;
-      int pid = 0;
-      if (live) {
-        while (cur.kk == cur.kend) {
+      int pid[kPtBatch];
+#pragma unroll
+      for (int u = 0; u < kPtBatch; ++u) {
+        pid[u] = -1;
+        while (live && cur.kk == cur.kend) {
           ++cy;
           if (cy > r1) {
             live = false;
@@ -705,16 +714,20 @@ __global__ void __launch_bounds__(256) k_b_enum(StereoBuffers b, StereoParams p)
           }
           if (ok) row_range(cstart, ccnt, cy, cols, xa, xb, cur);
         }
-        if (live) {
-          pid = cpid[cur.kk++];
-          const double qx = xy[2 * pid], qy = xy[2 * pid + 1];
+        if (live) pid[u] = cpid[cur.kk++];
+      }
+#pragma unroll
+      for (int u = 0; u < kPtBatch; ++u) {
+        bool acc = false;
+        if (pid[u] >= 0) {
+          const double qx = xy[2 * pid[u]], qy = xy[2 * pid[u] + 1];
           acc = true;
 #pragma unroll
           for (int j = 0; j < K; ++j) acc = acc && in_strip(S[j], qx, qy, tol2);
         }
+        bq_push(q, acc, make_int2(s, pid[u]));
       }
-      bq_push(q, acc, make_int2(s, pid));
-      bq_flush(q, b.n_pairs, b.cap_pairs, &b.flags[1], false, out);
+      bq_flush<kPtBatch>(q, b.n_pairs, b.cap_pairs, &b.flags[1], false, out);
     }
   }
   bq_flush(q, b.n_pairs, b.cap_pairs, &b.flags[1], true, out);
