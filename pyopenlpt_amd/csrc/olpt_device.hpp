@@ -23,6 +23,17 @@ constexpr int kMaxCam = OLPT_MAX_CAM;
 // --------------------------------------------------------------------------
 // Camera in the arithmetic type T (converted once on the host).
 // --------------------------------------------------------------------------
+// a * b + c.  FP32 fuses it (one instruction, one rounding); FP64 keeps the
+// separately rounded product and sum of the reference's expressions (the
+// build turns off contraction), so FP64 results do not change where it is
+// used in place of the written-out form.
+__host__ __device__ __forceinline__ float mad_t(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+__host__ __device__ __forceinline__ double mad_t(double a, double b, double c) {
+  return a * b + c;
+}
+
 template <typename T> struct CamT {
   T fx, fy, cx, cy;
   T k[12];
@@ -71,8 +82,8 @@ __host__ __device__ __forceinline__ void distort(const CamT<T> &c, T x, T y, T &
     x = xd;
     y = yd;
   }
-  u = x * c.fx + c.cx;
-  v = y * c.fy + c.cy;
+  u = mad_t(x, c.fx, c.cx);
+  v = mad_t(y, c.fy, c.cy);
 }
 
 // Camera::project, pinhole branch = worldToUndistImg (Camera.cpp:680-689)
@@ -83,7 +94,7 @@ __host__ __device__ __forceinline__ void project(const CamT<T> &c, const T X[3],
   T t[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
-    t[i] = c.R[3 * i] * X[0] + c.R[3 * i + 1] * X[1] + c.R[3 * i + 2] * X[2];
+    t[i] = mad_t(c.R[3 * i + 2], X[2], mad_t(c.R[3 * i + 1], X[1], c.R[3 * i] * X[0]));
 #pragma unroll
   for (int i = 0; i < 3; ++i) t[i] = t[i] + c.Tv[i];
   const T x = t[2] != T(0) ? t[0] / t[2] : t[0];
@@ -220,21 +231,29 @@ template <typename T> struct OtfT {
   const T *a, *b, *c, *alpha; // [n_cam][ng]
 };
 
+// myMATH::triLinearInterp (myMATH.cpp:131-214) in two parts: the clamped
+// cell fractions of p (shared by the four OTF parameters), then the
+// interpolation of one parameter's 8 corner values.
 template <typename T>
-__device__ __forceinline__ T trilin(const T lim[6], const T v[8], const T p[3]) {
+__device__ __forceinline__ void trilin_frac(const T lim[6], const T p[3], T d[3]) {
   T xd = (p[0] - lim[0]) / (lim[1] - lim[0]);
   T yd = (p[1] - lim[2]) / (lim[3] - lim[2]);
   T zd = (p[2] - lim[4]) / (lim[5] - lim[4]);
-  xd = fmax(T(0), fmin(T(1), xd));
-  yd = fmax(T(0), fmin(T(1), yd));
-  zd = fmax(T(0), fmin(T(1), zd));
-  const T c00 = v[0] * (1 - xd) + v[1] * xd;
-  const T c01 = v[3] * (1 - xd) + v[2] * xd;
-  const T c10 = v[4] * (1 - xd) + v[5] * xd;
-  const T c11 = v[7] * (1 - xd) + v[6] * xd;
-  const T c0 = c00 * (1 - yd) + c10 * yd;
-  const T c1 = c01 * (1 - yd) + c11 * yd;
-  return c0 * (1 - zd) + c1 * zd;
+  d[0] = fmax(T(0), fmin(T(1), xd));
+  d[1] = fmax(T(0), fmin(T(1), yd));
+  d[2] = fmax(T(0), fmin(T(1), zd));
+}
+
+template <typename T>
+__device__ __forceinline__ T trilin(const T v[8], const T d[3]) {
+  const T xd = d[0], yd = d[1], zd = d[2];
+  const T c00 = mad_t(v[0], 1 - xd, v[1] * xd);
+  const T c01 = mad_t(v[3], 1 - xd, v[2] * xd);
+  const T c10 = mad_t(v[4], 1 - xd, v[5] * xd);
+  const T c11 = mad_t(v[7], 1 - xd, v[6] * xd);
+  const T c0 = mad_t(c00, 1 - yd, c10 * yd);
+  const T c1 = mad_t(c01, 1 - yd, c11 * yd);
+  return mad_t(c0, 1 - zd, c1 * zd);
 }
 
 // -> {a, b, c, cos(alpha), sin(alpha)}
@@ -271,12 +290,21 @@ __device__ inline void otf_param(const OtfT<T> &o, int cam, const T X[3], T out[
     vc[k] = o.c[base + g[k]];
     vl[k] = o.alpha[base + g[k]];
   }
-  out[0] = trilin(lim, va, p);
-  out[1] = trilin(lim, vb, p);
-  out[2] = trilin(lim, vc, p);
-  const T al = trilin(lim, vl, p);
-  out[3] = cos(al);
-  out[4] = sin(al);
+  T d[3];
+  trilin_frac(lim, p, d);
+  out[0] = trilin(va, d);
+  out[1] = trilin(vb, d);
+  out[2] = trilin(vc, d);
+  const T al = trilin(vl, d);
+  // an unrotated OTF (alpha 0 everywhere) skips the trigonometry:
+  // cos(0) = 1 and sin(0) = 0 exactly
+  if (__builtin_expect(al == T(0), 1)) {
+    out[3] = T(1);
+    out[4] = T(0);
+  } else {
+    out[3] = cos(al);
+    out[4] = sin(al);
+  }
 }
 
 // Gaussian OTF intensity at pixel (col,row) for a particle at (u,v)
@@ -291,9 +319,9 @@ __device__ __forceinline__ float gexp(float x) { return __expf(x); }
 template <typename T>
 __device__ __forceinline__ T gauss_int(const T P[5], T u, T v, int row, int col) {
   const T dx = T(col) - u, dy = T(row) - v;
-  const T xx = dx * P[3] + dy * P[4];
-  const T yy = -dx * P[4] + dy * P[3];
-  const T val = P[0] * gexp(-(P[1] * xx * xx + P[2] * yy * yy));
+  const T xx = mad_t(dx, P[3], dy * P[4]);
+  const T yy = mad_t(dy, P[3], -dx * P[4]);
+  const T val = P[0] * gexp(-mad_t(P[1] * xx, xx, P[2] * yy * yy));
   return val > T(0) ? val : T(0);
 }
 

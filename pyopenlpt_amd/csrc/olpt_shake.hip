@@ -357,7 +357,7 @@ __device__ __forceinline__ T eval_residue(const ShakeBuffers<T> &b, const CamLan
         if (row >= prmin && row < prmax && col >= pcmin && col < pcmax)
           pred = gauss_int<T>(P, ue, ve, row, col);
         const T d = c.aug[sl] - pred;
-        sse += d * d;
+        sse = mad_t(d, d, sse);
       }
       co += LPG;
       roi_wrap(c.roiw, ro, co);
