@@ -305,13 +305,15 @@ __device__ void polyfit2(const double *x, const double *y, int n, double c[3]) {
   }
 }
 
-// Carry column overflow into rows for a row-major walk of a ROI of width
-// w >= 1: (ro, co) becomes the (row, column) offset of pixel ro * w + co.
-__device__ __forceinline__ void roi_wrap(int w, int &ro, int &co) {
-  while (co >= w) {
-    co -= w;
-    ++ro;
-  }
+// ROI pixel layout of a camera group: LPG lanes x SLOTS slots cover a
+// PW x PW box (PW = sqrt(LPG * SLOTS), a multiple of LPG) in row-major
+// order, so slot sl of lane sub holds ROI pixel (ro, co) = (sl * LPG / PW,
+// sl * LPG % PW + sub): a compile-time row and a lane column.  ROIs
+// narrower or shorter than PW leave the slots beyond them empty.
+constexpr int roi_pitch(int n) {
+  int r = 1;
+  while ((r + 1) * (r + 1) <= n) ++r;
+  return r;
 }
 
 // Per-lane state of one object in one camera (the lane's group camera).
@@ -332,6 +334,8 @@ template <typename T, int NC, int OPW, int SLOTS>
 __device__ __forceinline__ T eval_residue(const ShakeBuffers<T> &b, const CamLane<T, SLOTS> &c,
                                           T r_px, const T Xe[3], T &ue, T &ve) {
   constexpr int LPG = kWave / OPW / NC;
+  constexpr int PW = roi_pitch(LPG * SLOTS);
+  static_assert(PW * PW == LPG * SLOTS && PW % LPG == 0, "ROI slots must tile a square");
   const int sub = lane_id() % LPG;
   T sse = T(0);
   ue = T(0);
@@ -343,24 +347,25 @@ __device__ __forceinline__ T eval_residue(const ShakeBuffers<T> &b, const CamLan
     otf_param(b.otf, c.k, Xe, P);
     const int prmin = (int)floor(ve - r_px), prmax = (int)ceil(ve + r_px + T(1));
     const int pcmin = (int)floor(ue - r_px), pcmax = (int)ceil(ue + r_px + T(1));
-    // pixel sl * LPG + sub of the ROI in row-major order (roiw >= 1 here),
-    // stepped LPG columns at a time instead of divided out per slot
-    int ro = 0, co = sub;
-    roi_wrap(c.roiw, ro, co);
+    const int roih = c.roi.rmax - c.roi.rmin;
 #pragma unroll
     for (int sl = 0; sl < SLOTS; ++sl) {
-      const int idx = sl * LPG + sub;
-      if (idx < c.npix) {
-        const int row = c.roi.rmin + ro;
-        const int col = c.roi.cmin + co;
-        T pred = T(0);
-        if (row >= prmin && row < prmax && col >= pcmin && col < pcmax)
-          pred = gauss_int<T>(P, ue, ve, row, col);
-        const T d = c.aug[sl] - pred;
-        sse = mad_t(d, d, sse);
+      const int ro = sl * LPG / PW, co = sl * LPG % PW + sub;
+      const int row = c.roi.rmin + ro;
+      const int col = c.roi.cmin + co;
+      const bool valid = co < c.roiw && ro < roih;
+      const bool win = row >= prmin && row < prmax && col >= pcmin && col < pcmax;
+      T pred = T(0);
+      if constexpr (sizeof(T) == 4) {
+        // FP32: evaluated everywhere and selected, no branches per pixel
+        const T g = gauss_int<T>(P, ue, ve, row, col);
+        pred = win ? g : T(0);
+      } else {
+        if (win) pred = gauss_int<T>(P, ue, ve, row, col);
       }
-      co += LPG;
-      roi_wrap(c.roiw, ro, co);
+      // empty slots add an exact zero
+      const T d = valid ? c.aug[sl] - pred : T(0);
+      sse = mad_t(d, d, sse);
     }
   }
   sse = group_sum<LPG>(sse);
@@ -416,15 +421,15 @@ __device__ __forceinline__ void build_roi(const ShakeBuffers<T> &b, const ShakeP
   c.roi = roi;
   if (roi.empty()) return;
   c.roiw = roi.cmax - roi.cmin;
-  c.npix = (roi.rmax - roi.rmin) * c.roiw;
+  const int roih = roi.rmax - roi.rmin;
+  c.npix = roih * c.roiw;
   if (c.npix <= 0) return;
+  constexpr int PW = roi_pitch(LPG * SLOTS);
   const T rP[5] = {r.P[0], r.P[1], r.P[2], r.P[3], r.P[4]};
-  int ro = 0, co = sub;
-  roi_wrap(c.roiw, ro, co);
 #pragma unroll
-  for (int sl = 0; sl < SLOTS; ++sl, co += LPG, roi_wrap(c.roiw, ro, co)) {
-    const int idx = sl * LPG + sub;
-    if (idx < c.npix) {
+  for (int sl = 0; sl < SLOTS; ++sl) {
+    const int ro = sl * LPG / PW, co = sl * LPG % PW + sub;
+    if (co < c.roiw && ro < roih) {
       const int row = roi.rmin + ro;
       const int col = roi.cmin + co;
       T a = b.res[c.k * HW + (long)row * b.W + col];
@@ -605,7 +610,7 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
       const PRange &ro = c.roi;
       const bool inr = valid && ro.rmax - ro.rmin > 0 && ro.cmax - ro.cmin > 0 &&
                        row >= ro.rmin && row < ro.rmax && col >= ro.cmin && col < ro.cmax;
-      const int aidx = inr ? (row - ro.rmin) * c.roiw + (col - ro.cmin) : 0;
+      const int aidx = inr ? (row - ro.rmin) * roi_pitch(LPG * SLOTS) + (col - ro.cmin) : 0;
       const int src = (lane / LPG) * LPG + aidx % LPG;   // same camera group
       T av = T(0);
 #pragma unroll
