@@ -255,6 +255,7 @@ struct Strip {
   double ax, ay;   // point on the line
   double Ux, Uy;   // renormalised direction (IDMap::normalized)
   double n0, n1, d;
+  double inv_n0;   // 1 / n0 (strip_span's column bounds)
 };
 
 __device__ __forceinline__ Strip strip_of(const double *l) {
@@ -265,6 +266,7 @@ __device__ __forceinline__ Strip strip_of(const double *l) {
   s.n0 = -s.Uy;
   s.n1 = s.Ux;
   s.d = -(s.n0 * s.ax + s.n1 * s.ay);
+  s.inv_n0 = 1.0 / s.n0;
   return s;
 }
 
@@ -274,8 +276,12 @@ __device__ __forceinline__ bool in_strip(const Strip &s, double qx, double qy, d
   return !(z * z > tol2);
 }
 
-// Cell-column span of one cell row for one strip (same arithmetic as the
-// reference); returns false if the row is empty for this strip.
+// Cell-column span of one cell row for one strip (the reference's
+// arithmetic, with the division by n0 taken as a product with its
+// reciprocal); returns false if the row is empty for this strip.  The span
+// only chooses which cells are visited: it is widened by the cell extent
+// and 1e-6 px, and every point in it meets the exact in_strip test, so the
+// reciprocal's last-bit rounding cannot change which points pass.
 __device__ __forceinline__ bool strip_span(const Strip &s, int cy, int cell_px, int cols,
                                            double tolg, int &a, int &bb) {
   const double cell = (double)cell_px;
@@ -288,7 +294,7 @@ __device__ __forceinline__ bool strip_span(const Strip &s, int cy, int cell_px, 
   a = 0;
   bb = cols - 1;
   if (fabs(s.n0) < 1e-12) return !(fabs(C) > T);
-  double xl = (-T - C) / s.n0, xr = (T - C) / s.n0;
+  double xl = (-T - C) * s.inv_n0, xr = (T - C) * s.inv_n0;
   if (xl > xr) {
     const double tt = xl;
     xl = xr;
