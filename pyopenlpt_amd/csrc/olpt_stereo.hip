@@ -408,6 +408,11 @@ struct CandRec {
 
 constexpr int kQueue = 2048;      // pair queue capacity (records)
 constexpr int kPtBatch = 4;       // points tested per thread between flushes
+// The rank kernels (bucket, percentile and priority ranks) compare one
+// element with every other member of its list; they take kRankBatch members
+// per step so that the members' dependent loads are in flight together.
+// Counts do not depend on the order of the comparisons.
+constexpr int kRankBatch = 4;
 constexpr int kSurvQueue = 1024;
 constexpr int kCandQueue = 512;
 
@@ -850,14 +855,37 @@ __global__ void k_bucket_rank(StereoBuffers b, StereoParams p) {
     const int key = idj[pc];
     const int s = b.key_start[key], n = b.key_cnt[key];
     int r = 0;
-    for (int t = 0; t < n; ++t) {
-      const int m = b.bucket_idx[s + t];
-      if (m == j) continue;
-      const int32_t *idm = b.cand_ids + (long)m * nc;
-      int cmp = 0;
-      for (int c = 0; c < nc && cmp == 0; ++c)
-        cmp = idm[c] < idj[c] ? -1 : (idm[c] > idj[c] ? 1 : 0);
-      r += (cmp < 0 || (cmp == 0 && m < j)) ? 1 : 0;
+    if (nc == 4) {
+      // four cameras: the members' id vectors are loaded whole, a batch at
+      // a time, and compared without branches
+      for (int t0 = 0; t0 < n; t0 += kRankBatch) {
+        int m[kRankBatch];
+        int4 im[kRankBatch];
+#pragma unroll
+        for (int u = 0; u < kRankBatch; ++u) m[u] = t0 + u < n ? b.bucket_idx[s + t0 + u] : j;
+#pragma unroll
+        for (int u = 0; u < kRankBatch; ++u)
+          im[u] = *reinterpret_cast<const int4 *>(b.cand_ids + (long)m[u] * 4);
+#pragma unroll
+        for (int u = 0; u < kRankBatch; ++u) {
+          const int32_t a[4] = {im[u].x, im[u].y, im[u].z, im[u].w};
+          int cmp = 0;
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            cmp = cmp != 0 ? cmp : (a[c] < idj[c] ? -1 : (a[c] > idj[c] ? 1 : 0));
+          r += m[u] != j && (cmp < 0 || (cmp == 0 && m[u] < j)) ? 1 : 0;
+        }
+      }
+    } else {
+      for (int t = 0; t < n; ++t) {
+        const int m = b.bucket_idx[s + t];
+        if (m == j) continue;
+        const int32_t *idm = b.cand_ids + (long)m * nc;
+        int cmp = 0;
+        for (int c = 0; c < nc && cmp == 0; ++c)
+          cmp = idm[c] < idj[c] ? -1 : (idm[c] > idj[c] ? 1 : 0);
+        r += (cmp < 0 || (cmp == 0 && m < j)) ? 1 : 0;
+      }
     }
     const int pos = s + r;
     for (int c = 0; c < nc; ++c) b.cand_ids_s[(long)pos * nc + c] = idj[c];
@@ -874,9 +902,8 @@ __device__ __forceinline__ bool eq_eps(double a, double c) {
 }
 
 // order of StereoMatch.cpp:820-851: pct (eq_eps), e_check (eq_eps), index
-__device__ __forceinline__ bool less_order(const double *pct, const double *e,
-                                           int aj, int bj) {
-  const double pa = pct[aj], pb = pct[bj];
+__device__ __forceinline__ bool less_order_v(double pa, double ea, int aj, double pb, double eb,
+                                             int bj) {
   const bool af = isfinite(pa), bf = isfinite(pb);
   if (af && bf) {
     if (!eq_eps(pa, pb)) return pa < pb;
@@ -885,7 +912,6 @@ __device__ __forceinline__ bool less_order(const double *pct, const double *e,
   } else if (bf) {
     return false;
   }
-  const double ea = e[aj], eb = e[bj];
   const bool aef = isfinite(ea), bef = isfinite(eb);
   if (aef && bef) {
     if (!eq_eps(ea, eb)) return ea < eb;
@@ -895,6 +921,11 @@ __device__ __forceinline__ bool less_order(const double *pct, const double *e,
     return false;
   }
   return aj < bj;
+}
+
+__device__ __forceinline__ bool less_order(const double *pct, const double *e,
+                                           int aj, int bj) {
+  return less_order_v(pct[aj], e[aj], aj, pct[bj], e[bj], bj);
 }
 
 // Point lists: entry = (candidate << 3) | slot, slot = rank of the camera
@@ -958,14 +989,22 @@ __global__ void k_pct_rank(StereoBuffers b) {
     const double ej = b.cand_e_s[j];
     int r = 0;
     bool tie = false;
-    for (int t = 0; t < n; ++t) {
-      const int m = ent_cand(b.pt_list[s0 + t]);
-      if (m == j) continue;
-      const double em = b.cand_e_s[m];
-      if (eq_eps(em, ej))
-        tie = true;
-      else
-        r += em < ej ? 1 : 0;
+    for (int t0 = 0; t0 < n; t0 += kRankBatch) {
+      int m[kRankBatch];
+      double em[kRankBatch];
+#pragma unroll
+      for (int u = 0; u < kRankBatch; ++u)
+        m[u] = t0 + u < n ? ent_cand(b.pt_list[s0 + t0 + u]) : j;
+#pragma unroll
+      for (int u = 0; u < kRankBatch; ++u) em[u] = b.cand_e_s[m[u]];
+#pragma unroll
+      for (int u = 0; u < kRankBatch; ++u) {
+        if (m[u] == j) continue;
+        if (eq_eps(em[u], ej))
+          tie = true;
+        else
+          r += em[u] < ej ? 1 : 0;
+      }
     }
     if (tie) {
       atomicOr(&b.pt_flag[f], 1);
@@ -1042,10 +1081,22 @@ __global__ void k_prio_rank(StereoBuffers b) {
     const int s0 = b.pt_start[f], n = b.pt_count[f];
     const int32_t v = b.pt_list[q];
     const int j = ent_cand(v);
+    const double pj = b.pct[j], ej = b.cand_e_s[j];
     int r = 0;
-    for (int t = 0; t < n; ++t) {
-      const int m = ent_cand(b.pt_list[s0 + t]);
-      if (m != j && less_order(b.pct, b.cand_e_s, m, j)) ++r;
+    for (int t0 = 0; t0 < n; t0 += kRankBatch) {
+      int m[kRankBatch];
+      double pm[kRankBatch], em[kRankBatch];
+#pragma unroll
+      for (int u = 0; u < kRankBatch; ++u)
+        m[u] = t0 + u < n ? ent_cand(b.pt_list[s0 + t0 + u]) : j;
+#pragma unroll
+      for (int u = 0; u < kRankBatch; ++u) {
+        pm[u] = b.pct[m[u]];
+        em[u] = b.cand_e_s[m[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < kRankBatch; ++u)
+        r += m[u] != j && less_order_v(pm[u], em[u], m[u], pj, ej, j) ? 1 : 0;
     }
     b.ppos[(long)j * b.n_cam + ent_slot(v)] = r;
     b.pt_sorted[s0 + r] = v;
@@ -1098,11 +1149,27 @@ __device__ __forceinline__ int mis_decide(const StereoBuffers &b, const StereoPa
     ++k;
     const int s0 = b.pt_start[f];
     const int h0 = __atomic_load_n(&b.head[f], __ATOMIC_RELAXED);
+    // first entry ahead of pos that is not rejected, kRankBatch entries per
+    // step (their list and state loads in flight together).  States only
+    // leave 0 for a final 1 or 2, so a batch's snapshot at worst defers a
+    // decision to a later round.
     int h = h0, st = 2;
     while (h < pos) {
-      st = __atomic_load_n(&b.state[ent_cand(b.pt_sorted[s0 + h])], __ATOMIC_RELAXED);
-      if (st != 2) break;
-      ++h;
+      int sv[kRankBatch];
+#pragma unroll
+      for (int u = 0; u < kRankBatch; ++u) {
+        const int e = h + u < pos ? b.pt_sorted[s0 + h + u] : -1;
+        sv[u] = e >= 0 ? (int)__atomic_load_n(&b.state[ent_cand(e)], __ATOMIC_RELAXED) : 2;
+      }
+      int adv = kRankBatch;
+#pragma unroll
+      for (int u = kRankBatch - 1; u >= 0; --u)
+        if (sv[u] != 2) adv = u;
+      h = min(h + adv, pos);
+      if (adv < kRankBatch) {
+        st = sv[adv];
+        break;
+      }
     }
     if (h > h0) atomicMax(&b.head[f], h);
     if (h < pos) {
