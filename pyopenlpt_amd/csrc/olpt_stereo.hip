@@ -614,14 +614,16 @@ __global__ void __launch_bounds__(256) k_b_second(StereoBuffers b, StereoParams 
   }
 }
 
-// Level-1 band: one thread per (reference observation, cell row of its
-// second camera); accepted points become (r, pid) pairs.
+// Level-1 band: one thread per (reference observation, run of kBandRows
+// cell rows of its second camera); accepted points become (r, pid) pairs.
+constexpr int kBandRows = 4;
 __global__ void __launch_bounds__(256) k_b_band(StereoBuffers b, StereoParams p) {
   int nref;
   const int ref = ref_camera(b, p, nref);
   const int rows = b.rows_cell, cols = b.cols_cell;
+  const int nrun = (rows + kBandRows - 1) / kBandRows;
   const long ncell = (long)rows * cols;
-  const long total = (long)nref * rows;
+  const long total = (long)nref * nrun;
   const double tolg = p.tol2d + 1e-6;
   const double tol2 = p.tol2d * p.tol2d;
   const long stride = (long)gridDim.x * blockDim.x;
@@ -630,26 +632,41 @@ __global__ void __launch_bounds__(256) k_b_band(StereoBuffers b, StereoParams p)
   auto out = [&](long i, const int2 &v) { b.pairs[i] = v; };
   for (long base = (long)blockIdx.x * blockDim.x; base < total; base += stride) {
     const long item = base + threadIdx.x;
-    const int r = (int)(item / rows), cy = (int)(item % rows);
+    const int r = (int)(item / nrun);
     const int t1 = item < total ? (int)b.t1[r] : -1;
+    int cy = (int)(item % nrun) * kBandRows - 1;
+    const int cy_end = min(cy + kBandRows, rows - 1);
+    bool live = t1 >= 0;
     RowCursor cur{0, 0};
     Strip s{};
     const double *xy = b.xy;
     const int32_t *cpid = b.cell_pid;
-    if (t1 >= 0) {
+    const int32_t *cstart = b.cell_start, *ccnt = b.cell_cnt;
+    if (live) {
       s = strip_of(line_of(b, ref, t1, r));
-      int xa, xb;
-      if (strip_span(s, cy, b.cell_px, cols, tolg, xa, xb))
-        row_range(b.cell_start + t1 * ncell, b.cell_cnt + t1 * ncell, cy, cols, xa, xb, cur);
       xy += (long)t1 * b.cap * 2;
       cpid += (long)t1 * b.cap;
+      cstart += t1 * ncell;
+      ccnt += t1 * ncell;
     }
-    // kPtBatch points per barrier: their loads are in flight together
-    while (__syncthreads_or(cur.kk < cur.kend)) {
+    // kPtBatch points per barrier (across the run's rows): their loads are
+    // in flight together
+    while (__syncthreads_or(live)) {
       int pid[kPtBatch];
 #pragma unroll
-      for (int u = 0; u < kPtBatch; ++u) pid[u] = cur.kk + u < cur.kend ? cpid[cur.kk + u] : -1;
-      cur.kk = min(cur.kk + kPtBatch, cur.kend);
+      for (int u = 0; u < kPtBatch; ++u) {
+        pid[u] = -1;
+        while (live && cur.kk == cur.kend) {
+          if (++cy > cy_end) {
+            live = false;
+            break;
+          }
+          int xa, xb;
+          if (strip_span(s, cy, b.cell_px, cols, tolg, xa, xb))
+            row_range(cstart, ccnt, cy, cols, xa, xb, cur);
+        }
+        if (live) pid[u] = cpid[cur.kk++];
+      }
 #pragma unroll
       for (int u = 0; u < kPtBatch; ++u) {
         const bool acc = pid[u] >= 0 && in_strip(s, xy[2 * pid[u]], xy[2 * pid[u] + 1], tol2);
