@@ -115,14 +115,18 @@ class Context:
                  passes: int = 1) -> tuple[dict, list]:
         """Time olpt_ipr_run on the caller's frames over launch shapes and
         keep the fastest (coordinate descent: one knob at a time, the others
-        held at the best so far).  The shapes do not change results, only
+        held at the best so far; each shape is timed once).  The shapes do not change results, only
         speed.  Returns (best tuning, [(tuning, seconds per call)])."""
         import time
         space = space or self.TUNING_SPACE
         outputs = self.alloc_outputs(n_frames, 1 << 18, imgs.device)
         log = []
+        seen = {}
 
         def measure(t):
+            key = tuple(sorted(t.items()))
+            if key in seen:          # a shape is timed once
+                return seen[key]
             self.set_tuning(**t)
             self.ipr_run(imgs, dtype, params, n_frames, outputs=outputs)
             torch.cuda.synchronize()
@@ -133,6 +137,7 @@ class Context:
                 torch.cuda.synchronize()
                 best = min(best, time.perf_counter() - t0)
             log.append((dict(t), best))
+            seen[key] = best
             return best
 
         best_t = self.get_tuning()

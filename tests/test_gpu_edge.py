@@ -210,6 +210,7 @@ def test_tuning_rejects_out_of_range_and_autotune_keeps_a_candidate(case):
     space = {"n_streams": (1, 2), "shake_blocks": (1024, 4096), "prune_blocks": (512, 2048)}
     best, log = ctx.autotune(d, code, case.params, 2, space=space, reps=1)
     assert ctx.get_tuning() == best
-    assert len(log) == 4
+    # the start, then each knob's other value, each shape timed once
+    assert len(log) == 4 and len({tuple(sorted(t.items())) for t, _ in log}) == 4
     assert all(best[k] in v for k, v in space.items())
     ctx.close()
