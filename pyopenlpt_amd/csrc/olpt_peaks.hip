@@ -1,8 +1,9 @@
 // olpt_peaks.hip - 2D tracer detection (ObjectFinder2D::findTracer2D,
 // ObjectFinder.cpp:37-111).
 //
-// Layout: one wave64 per "chunk" = 64 consecutive interior columns of one
-// interior row.  Pass 1 evaluates threshold + 4-neighbour local maximum
+// Layout: a "chunk" is 64 consecutive interior columns of one interior row;
+// one wave64 takes up to 16 chunks of a row.  Pass 1 evaluates threshold +
+// 4-neighbour local maximum
 // (myMATH::isLocalMax tie-break: up/left may equal, down/right must be
 // strictly lower, myMATH.h:398-439) with coalesced loads and writes the
 // 64-bit ballot mask and its popcount; the predicate includes the fit's own
@@ -57,32 +58,109 @@ __device__ __forceinline__ bool fit_peak(const TIn *p, long i, int W, int row,
   return isfinite(yc);
 }
 
+// A wave takes a group of up to kPeakGroup consecutive chunks of one row.
+// Candidates (threshold + local maximum) are found a chunk at a time with
+// coalesced loads; the separable fit - five FP64 logarithms and two
+// divisions, needed for its rejections and for the centre - then runs on
+// the group's candidates packed densely into lanes, so a wave fits ~64
+// candidates per pass instead of the two or three a single chunk holds.
+constexpr int kPeakGroup = 16;
+
+// index of the r-th (0-based) set bit of m (r < popcount(m))
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int r) {
+  int pos = 0;
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1) {
+    const uint64_t lo = m & ((1ull << w) - 1ull);
+    const int c = __popcll(lo);
+    if (r >= c) {
+      r -= c;
+      m >>= w;
+      pos += w;
+    } else {
+      m = lo;
+    }
+  }
+  return pos;
+}
+
+// the (chunk k, bit) of candidate t of a group with masks cm and exclusive
+// prefix counts pre (wave-uniform)
+__device__ __forceinline__ void group_locate(const uint64_t cm[kPeakGroup],
+                                             const int pre[kPeakGroup + 1], int g, int t,
+                                             int &k, int &bit) {
+  k = 0;
+#pragma unroll
+  for (int q = 1; q < kPeakGroup; ++q)
+    if (q < g && pre[q] <= t) k = q;
+  uint64_t m = 0;
+  int base = 0;
+#pragma unroll
+  for (int q = 0; q < kPeakGroup; ++q)
+    if (q == k) {
+      m = cm[q];
+      base = pre[q];
+    }
+  bit = nth_set_bit(m, t - base);
+}
+
 template <typename TIn>
 __global__ void __launch_bounds__(kPeakWaves * kWave)
 k_peak_mask(const TIn *__restrict__ img, long cam_stride, int H, int W,
             double min_int, CamList cams, uint64_t *__restrict__ mask,
             int32_t *__restrict__ cnt, long nchunk) {
+  __shared__ unsigned long long s_vm[kPeakWaves][kPeakGroup];
   const int s = blockIdx.y;
-  const long chunk = (long)blockIdx.x * kPeakWaves + threadIdx.x / kWave;
-  if (chunk >= nchunk) return;
+  const int wv = threadIdx.x / kWave;
+  const int lane = lane_id();
   const int cpr = (W - 2 + 63) / 64;
-  const int row = 1 + (int)(chunk / cpr);
-  const int col = 1 + (int)(chunk % cpr) * 64 + lane_id();
+  const int gpr = (cpr + kPeakGroup - 1) / kPeakGroup;
+  const long grp = (long)blockIdx.x * kPeakWaves + wv;
+  if (grp >= (long)(H - 2) * gpr) return;
+  const int row = 1 + (int)(grp / gpr);
+  const int c0 = (int)(grp % gpr) * kPeakGroup;
+  const int g = min(kPeakGroup, cpr - c0);
   const TIn *p = img + cams.id[s] * cam_stride;
-  bool peak = false;
-  if (col < W - 1) {
-    const long i = (long)row * W + col;
-    const double v = pix(p, i);
-    if (!(v < min_int)) {
-      const double up = pix(p, i - W), lf = pix(p, i - 1);
-      const double dn = pix(p, i + W), rt = pix(p, i + 1);
-      peak = !(up > v) && !(lf > v) && !(dn >= v) && !(rt >= v);
+  uint64_t cm[kPeakGroup];
+  int pre[kPeakGroup + 1];
+  pre[0] = 0;
+#pragma unroll
+  for (int k = 0; k < kPeakGroup; ++k) {
+    bool cand = false;
+    const int col = 1 + (c0 + k) * 64 + lane;
+    if (k < g && col < W - 1) {
+      const long i = (long)row * W + col;
+      const double v = pix(p, i);
+      if (!(v < min_int)) {
+        const double up = pix(p, i - W), lf = pix(p, i - 1);
+        const double dn = pix(p, i + W), rt = pix(p, i + 1);
+        cand = !(up > v) && !(lf > v) && !(dn >= v) && !(rt >= v);
+      }
+    }
+    cm[k] = ballot(cand);
+    pre[k + 1] = pre[k] + __popcll(cm[k]);
+  }
+  if (lane < kPeakGroup) s_vm[wv][lane] = 0ull;
+  __builtin_amdgcn_wave_barrier();
+  // the fit's rejections (ObjectFinder.cpp:81-83, :95-97) on the packed
+  // candidates; the survivors' bits go back into their chunk masks
+  const int total = pre[kPeakGroup];
+  for (int t0 = 0; t0 < total; t0 += kWave) {
+    const int t = t0 + lane;
+    if (t < total) {
+      int k, bit;
+      group_locate(cm, pre, g, t, k, bit);
+      const int col = 1 + (c0 + k) * 64 + bit;
       double xc, yc;
-      if (peak) peak = fit_peak(p, i, W, row, col, xc, yc);
+      if (fit_peak(p, (long)row * W + col, W, row, col, xc, yc))
+        atomicOr(&s_vm[wv][k], 1ull << bit);
     }
   }
-  const uint64_t m = ballot(peak);
-  if (lane_id() == 0) {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (lane < g) {
+    const uint64_t m = s_vm[wv][lane];
+    const long chunk = (long)(row - 1) * cpr + c0 + lane;
     mask[s * nchunk + chunk] = m;
     cnt[s * nchunk + chunk] = __popcll(m);
   }
@@ -95,24 +173,47 @@ k_peak_emit(const TIn *__restrict__ img, long cam_stride, int H, int W,
             const int32_t *__restrict__ off, long nchunk,
             double *__restrict__ xy, int cap) {
   const int s = blockIdx.y;
-  const long chunk = (long)blockIdx.x * kPeakWaves + threadIdx.x / kWave;
-  if (chunk >= nchunk) return;
-  const uint64_t m = mask[s * nchunk + chunk];
-  if (m == 0) return;
+  const int wv = threadIdx.x / kWave;
   const int lane = lane_id();
-  if (!((m >> lane) & 1ull)) return;
   const int cpr = (W - 2 + 63) / 64;
-  const int row = 1 + (int)(chunk / cpr);
-  const int col = 1 + (int)(chunk % cpr) * 64 + lane;
-  const int pos = off[s * nchunk + chunk] + mbcnt(m);
-  if (pos >= cap) return;
+  const int gpr = (cpr + kPeakGroup - 1) / kPeakGroup;
+  const long grp = (long)blockIdx.x * kPeakWaves + wv;
+  if (grp >= (long)(H - 2) * gpr) return;
+  const int row = 1 + (int)(grp / gpr);
+  const int c0 = (int)(grp % gpr) * kPeakGroup;
+  const int g = min(kPeakGroup, cpr - c0);
+  const long chunk0 = s * nchunk + (long)(row - 1) * cpr + c0;
+  uint64_t cm[kPeakGroup];
+  int pre[kPeakGroup + 1];
+  pre[0] = 0;
+#pragma unroll
+  for (int k = 0; k < kPeakGroup; ++k) {
+    cm[k] = k < g ? mask[chunk0 + k] : 0ull;
+    pre[k + 1] = pre[k] + __popcll(cm[k]);
+  }
+  const int total = pre[kPeakGroup];
+  if (total == 0) return;
   const int cam = cams.id[s];
   const TIn *p = img + cam * cam_stride;
-  double xc = 0.0, yc = 0.0;
-  fit_peak(p, (long)row * W + col, W, row, col, xc, yc);
-  double *o = xy + ((long)cam * cap + pos) * 2;
-  o[0] = xc;
-  o[1] = yc;
+  for (int t0 = 0; t0 < total; t0 += kWave) {
+    const int t = t0 + lane;
+    if (t >= total) continue;
+    int k, bit;
+    group_locate(cm, pre, g, t, k, bit);
+    // rank inside the chunk = t - pre[k]: the reference's row-major order
+    int base = 0;
+#pragma unroll
+    for (int q = 0; q < kPeakGroup; ++q)
+      if (q == k) base = pre[q];
+    const int pos = off[chunk0 + k] + (t - base);
+    if (pos >= cap) continue;
+    const int col = 1 + (c0 + k) * 64 + bit;
+    double xc = 0.0, yc = 0.0;
+    fit_peak(p, (long)row * W + col, W, row, col, xc, yc);
+    double *o = xy + ((long)cam * cap + pos) * 2;
+    o[0] = xc;
+    o[1] = yc;
+  }
 }
 
 // Per-camera counts and capacity check.
@@ -134,7 +235,9 @@ void launch_find_peaks(const TIn *img, long cam_stride, int H, int W,
                        int32_t *overflow, hipStream_t st) {
   const long nchunk = peak_chunks(H, W);
   if (nchunk <= 0 || cams.n <= 0) return;
-  dim3 g((unsigned)((nchunk + kPeakWaves - 1) / kPeakWaves), (unsigned)cams.n);
+  const int cpr = (W - 2 + 63) / 64;
+  const long ngroup = (long)(H - 2) * ((cpr + kPeakGroup - 1) / kPeakGroup);
+  dim3 g((unsigned)((ngroup + kPeakWaves - 1) / kPeakWaves), (unsigned)cams.n);
   stage_begin(ST_PEAKS, st);
   k_peak_mask<TIn><<<g, kPeakWaves * kWave, 0, st>>>(img, cam_stride, H, W,
                                                      min_int, cams, mask, cnt,
