@@ -261,6 +261,12 @@ template <typename T> OtfT<T> dotf(olpt_ctx *c) {
   o.nz = c->otf_nz;
   o.ng = c->otf_ng;
   for (int i = 0; i < 6; ++i) o.bnd[i] = (T)c->otf_bnd[i];
+  const int n[3] = {c->otf_nx, c->otf_ny, c->otf_nz};
+  for (int a = 0; a < 3; ++a) {
+    // the same T operations as the device's (B[2a+1] - B[2a]) / (n - 1)
+    o.step[a] = (o.bnd[2 * a + 1] - o.bnd[2 * a]) / (T)(n[a] - 1);
+    o.inv_step[a] = (T)1 / o.step[a];
+  }
   const long sz = (long)c->n_cam * c->otf_ng;
   const T *base;
   if constexpr (sizeof(T) == 8)

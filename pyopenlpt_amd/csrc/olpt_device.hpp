@@ -228,6 +228,9 @@ __device__ __forceinline__ bool triangulate(const double (*pt)[3], const double 
 template <typename T> struct OtfT {
   int32_t nx, ny, nz, ng;
   T bnd[6];
+  // grid spacing (bnd[2a+1] - bnd[2a]) / (n_a - 1), rounded in T exactly as
+  // the device would, and its reciprocal (FP32 lookups only)
+  T step[3], inv_step[3];
   const T *a, *b, *c, *alpha; // [n_cam][ng]
 };
 
@@ -263,16 +266,23 @@ __device__ inline void otf_param(const OtfT<T> &o, int cam, const T X[3], T out[
   const T px = fmin(fmax(X[0], B[0]), B[1]);
   const T py = fmin(fmax(X[1], B[2]), B[3]);
   const T pz = fmin(fmax(X[2], B[4]), B[5]);
-  const T dx = (B[1] - B[0]) / (o.nx - 1);
-  const T dy = (B[3] - B[2]) / (o.ny - 1);
-  const T dz = (B[5] - B[4]) / (o.nz - 1);
-  int ix = (int)floor((px - B[0]) / dx), iy = (int)floor((py - B[2]) / dy),
-      iz = (int)floor((pz - B[4]) / dz);
+  const T gx = o.step[0], gy = o.step[1], gz = o.step[2];
+  int ix, iy, iz;
+  if constexpr (sizeof(T) == 4) {
+    // FP32: products with the reciprocal spacing (the cell index can differ
+    // from the division's only for a point within rounding of a cell face,
+    // where the interpolation is continuous)
+    ix = (int)floor((px - B[0]) * o.inv_step[0]);
+    iy = (int)floor((py - B[2]) * o.inv_step[1]);
+    iz = (int)floor((pz - B[4]) * o.inv_step[2]);
+  } else {
+    ix = (int)floor((px - B[0]) / gx);
+    iy = (int)floor((py - B[2]) / gy);
+    iz = (int)floor((pz - B[4]) / gz);
+  }
   ix = ix < 0 ? 0 : (ix > o.nx - 2 ? o.nx - 2 : ix);
   iy = iy < 0 ? 0 : (iy > o.ny - 2 ? o.ny - 2 : iy);
   iz = iz < 0 ? 0 : (iz > o.nz - 2 ? o.nz - 2 : iz);
-  const T gx = (B[1] - B[0]) / (o.nx - 1), gy = (B[3] - B[2]) / (o.ny - 1),
-          gz = (B[5] - B[4]) / (o.nz - 1);
   const T lim[6] = {B[0] + gx * ix, B[0] + gx * (ix + 1), B[2] + gy * iy,
                     B[2] + gy * (iy + 1), B[4] + gz * iz, B[4] + gz * (iz + 1)};
   const int nyz = o.ny * o.nz;
@@ -291,7 +301,14 @@ __device__ inline void otf_param(const OtfT<T> &o, int cam, const T X[3], T out[
     vl[k] = o.alpha[base + g[k]];
   }
   T d[3];
-  trilin_frac(lim, p, d);
+  if constexpr (sizeof(T) == 4) {
+    // FP32: the cell edge taken as the grid spacing
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      d[a] = fmax(T(0), fmin(T(1), (p[a] - lim[2 * a]) * o.inv_step[a]));
+  } else {
+    trilin_frac(lim, p, d);
+  }
   out[0] = trilin(va, d);
   out[1] = trilin(vb, d);
   out[2] = trilin(vc, d);
