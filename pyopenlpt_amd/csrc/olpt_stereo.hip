@@ -255,7 +255,7 @@ struct Strip {
   double ax, ay;   // point on the line
   double Ux, Uy;   // renormalised direction (IDMap::normalized)
   double n0, n1, d;
-  double inv_n0;   // 1 / n0 (strip_span's column bounds)
+  double inv_n0;   // 1 / n0 (the row spans' column bounds)
 };
 
 __device__ __forceinline__ Strip strip_of(const double *l) {
@@ -276,32 +276,47 @@ __device__ __forceinline__ bool in_strip(const Strip &s, double qx, double qy, d
   return !(z * z > tol2);
 }
 
-// Cell-column span of one cell row for one strip (the reference's
-// arithmetic, with the division by n0 taken as a product with its
-// reciprocal); returns false if the row is empty for this strip.  The span
-// only chooses which cells are visited: it is widened by the cell extent
-// and 1e-6 px, and every point in it meets the exact in_strip test, so the
-// reciprocal's last-bit rounding cannot change which points pass.
-__device__ __forceinline__ bool strip_span(const Strip &s, int cy, int cell_px, int cols,
-                                           double tolg, int &a, int &bb) {
+// Cell-column span of one cell row for one strip: the cells whose centre
+// lies within tol + 1e-6 px plus the cell's own extent along the strip
+// normal (the reference's IDMap::computeStripIntersection arithmetic).  The
+// row-independent terms are taken once per strip (StripRows) and a row costs
+// one FMA for the centre line's offset and a product with the normal's
+// reciprocal.  A span only chooses which cells are visited: widened by the
+// cell extent and 1e-6 px, it holds every cell with a point that passes the
+// exact FP64 in_strip test, so its own last-bit rounding cannot change which
+// points pass.
+struct StripRows {
+  double C0, dC, T, inv_n0, to_index;
+  bool flat;  // |n0| < 1e-12: the strip runs along the rows
+};
+
+__device__ __forceinline__ StripRows strip_rows(const Strip &s, int cell_px, double tolg) {
+  StripRows r;
   const double cell = (double)cell_px;
   const double hw = 0.5 * cell_px, hh = 0.5 * cell_px;
-  const double to_index = 1.0 / cell_px;
-  const double yc = (cy + 0.5) * cell;
-  const double C = s.n1 * yc + s.d;
-  const double pe = fabs(s.n0) * hw + fabs(s.n1) * hh;
-  const double T = tolg + pe;
+  r.to_index = 1.0 / cell_px;
+  r.C0 = s.n1 * (0.5 * cell) + s.d;
+  r.dC = s.n1 * cell;
+  r.T = tolg + (fabs(s.n0) * hw + fabs(s.n1) * hh);
+  r.flat = fabs(s.n0) < 1e-12;
+  r.inv_n0 = s.inv_n0;
+  return r;
+}
+
+// returns false if row cy holds no cell of the strip
+__device__ __forceinline__ bool row_span(const StripRows &r, int cy, int cols, int &a, int &bb) {
+  const double C = fma((double)cy, r.dC, r.C0);
   a = 0;
   bb = cols - 1;
-  if (fabs(s.n0) < 1e-12) return !(fabs(C) > T);
-  double xl = (-T - C) * s.inv_n0, xr = (T - C) * s.inv_n0;
+  if (r.flat) return !(fabs(C) > r.T);
+  double xl = (-r.T - C) * r.inv_n0, xr = (r.T - C) * r.inv_n0;
   if (xl > xr) {
     const double tt = xl;
     xl = xr;
     xr = tt;
   }
-  a = (int)ceil(xl * to_index - 0.5 - 1e-9);
-  bb = (int)floor(xr * to_index - 0.5 + 1e-9);
+  a = (int)ceil(xl * r.to_index - 0.5 - 1e-9);
+  bb = (int)floor(xr * r.to_index - 0.5 + 1e-9);
   a = a < 0 ? 0 : a;
   bb = bb > cols - 1 ? cols - 1 : bb;
   return a <= bb;
@@ -639,11 +654,13 @@ __global__ void __launch_bounds__(256) k_b_band(StereoBuffers b, StereoParams p)
     bool live = t1 >= 0;
     RowCursor cur{0, 0};
     Strip s{};
+    StripRows sr{};
     const double *xy = b.xy;
     const int32_t *cpid = b.cell_pid;
     const int32_t *cstart = b.cell_start, *ccnt = b.cell_cnt;
     if (live) {
       s = strip_of(line_of(b, ref, t1, r));
+      sr = strip_rows(s, b.cell_px, tolg);
       xy += (long)t1 * b.cap * 2;
       cpid += (long)t1 * b.cap;
       cstart += t1 * ncell;
@@ -662,7 +679,7 @@ __global__ void __launch_bounds__(256) k_b_band(StereoBuffers b, StereoParams p)
             break;
           }
           int xa, xb;
-          if (strip_span(s, cy, b.cell_px, cols, tolg, xa, xb))
+          if (row_span(sr, cy, cols, xa, xb))
             row_range(cstart, ccnt, cy, cols, xa, xb, cur);
         }
         if (live) pid[u] = cpid[cur.kk++];
@@ -697,6 +714,7 @@ __global__ void __launch_bounds__(256) k_b_enum(StereoBuffers b, StereoParams p)
     const int s = base + threadIdx.x;
     bool live = s < n_in;
     Strip S[K];
+    StripRows SR[K];
     int t = 0, cy = 0, r1 = -1;
     RowCursor cur{0, 0};
     if (live) {
@@ -704,7 +722,10 @@ __global__ void __launch_bounds__(256) k_b_enum(StereoBuffers b, StereoParams p)
       int cam[kMaxCam], pid[kMaxCam];
       path_ids<K>(b, ref, s, cam, pid);
 #pragma unroll
-      for (int j = 0; j < K; ++j) S[j] = strip_of(line_of(b, cam[j], t, pid[j]));
+      for (int j = 0; j < K; ++j) {
+        S[j] = strip_of(line_of(b, cam[j], t, pid[j]));
+        SR[j] = strip_rows(S[j], b.cell_px, tolg);
+      }
       int r0 = 0;
       r1 = rows - 1;
 #pragma unroll
@@ -734,7 +755,7 @@ __global__ void __launch_bounds__(256) k_b_enum(StereoBuffers b, StereoParams p)
           for (int j = 0; j < K; ++j) {
             int a, bb;
             if (ok) {
-              ok = strip_span(S[j], cy, b.cell_px, cols, tolg, a, bb);
+              ok = row_span(SR[j], cy, cols, a, bb);
               xa = a > xa ? a : xa;
               xb = bb < xb ? bb : xb;
               ok = ok && xa <= xb;
