@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -95,6 +96,10 @@ struct olpt_ctx {
   double *d_otf64 = nullptr; // a | b | c | alpha, each n_cam*ng
   float *d_otf32 = nullptr;
   int otf_ng = 0;
+  // an OTF with the same (a, b, c, alpha) at every grid node of each camera:
+  // per camera (a, b, c, cos alpha, sin alpha) in FP32 for the FP32 lookups
+  int otf_uniform = 0;
+  float *d_otf_uni32 = nullptr;
   int bin_rows = 0, bin_cols = 0;
   int grid_n = 80;
   // learned shake proposal (not owned)
@@ -273,6 +278,13 @@ template <typename T> OtfT<T> dotf(olpt_ctx *c) {
     base = (const T *)c->d_otf64;
   else
     base = (const T *)c->d_otf32;
+  if constexpr (sizeof(T) == 4) {
+    o.uniform = c->otf_uniform;
+    o.uni = (const T *)c->d_otf_uni32;
+  } else {
+    o.uniform = 0;
+    o.uni = nullptr;
+  }
   o.a = base;
   o.b = base + sz;
   o.c = base + 2 * sz;
@@ -828,6 +840,8 @@ olpt_status olpt_ctx_reserve(olpt_ctx *c, int n_cam, int H, int W, int max_2d, i
   c->d_otf64 = nullptr;
   c->d_otf32 = nullptr;
   c->otf_ng = 0;
+  c->otf_uniform = 0;
+  c->d_otf_uni32 = nullptr;
   c->loop_tot = nullptr;
   c->loop_tot_cap = 0;
   return OLPT_OK;
@@ -912,6 +926,29 @@ olpt_status olpt_set_otf(olpt_ctx *c, const olpt_otf *o) {
   }
   HIP_TRY(hipMemcpy(c->d_otf64, h.data(), sizeof(double) * 4 * sz, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->d_otf32, hf.data(), sizeof(float) * 4 * sz, hipMemcpyHostToDevice));
+  bool uniform = true;
+  for (int q = 0; q < 4 && uniform; ++q)
+    for (int k = 0; k < o->n_cam && uniform; ++k)
+      for (int g = 1; g < ng && uniform; ++g)
+        uniform = hf[q * sz + (long)k * ng + g] == hf[q * sz + (long)k * ng];
+  if (uniform) {
+    if (!c->d_otf_uni32) {
+      olpt_status s;
+      if ((s = dalloc_t(c, &c->d_otf_uni32, 5 * OLPT_MAX_CAM))) return s;
+    }
+    float u[5 * OLPT_MAX_CAM] = {};
+    for (int k = 0; k < o->n_cam; ++k) {
+      const long i = (long)k * ng;
+      const float al = hf[3 * sz + i];
+      u[5 * k + 0] = hf[i];
+      u[5 * k + 1] = hf[sz + i];
+      u[5 * k + 2] = hf[2 * sz + i];
+      u[5 * k + 3] = al == 0.0f ? 1.0f : std::cos(al);
+      u[5 * k + 4] = al == 0.0f ? 0.0f : std::sin(al);
+    }
+    HIP_TRY(hipMemcpy(c->d_otf_uni32, u, sizeof(u), hipMemcpyHostToDevice));
+  }
+  c->otf_uniform = uniform ? 1 : 0;
   c->otf_nx = o->nx;
   c->otf_ny = o->ny;
   c->otf_nz = o->nz;

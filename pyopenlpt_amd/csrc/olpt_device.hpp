@@ -231,6 +231,10 @@ template <typename T> struct OtfT {
   // grid spacing (bnd[2a+1] - bnd[2a]) / (n_a - 1), rounded in T exactly as
   // the device would, and its reciprocal (FP32 lookups only)
   T step[3], inv_step[3];
+  // FP32 only: the grid holds one (a, b, c, alpha) per camera; uni is
+  // [n_cam][5] = (a, b, c, cos alpha, sin alpha)
+  int32_t uniform;
+  const T *uni;
   const T *a, *b, *c, *alpha; // [n_cam][ng]
 };
 
@@ -262,6 +266,15 @@ __device__ __forceinline__ T trilin(const T v[8], const T d[3]) {
 // -> {a, b, c, cos(alpha), sin(alpha)}
 template <typename T>
 __device__ inline void otf_param(const OtfT<T> &o, int cam, const T X[3], T out[5]) {
+  if constexpr (sizeof(T) == 4) {
+    // a uniform grid interpolates to its node values wherever X lies
+    if (o.uniform) {
+      const T *u = o.uni + 5 * cam;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) out[q] = u[q];
+      return;
+    }
+  }
   const T *B = o.bnd;
   const T px = fmin(fmax(X[0], B[0]), B[1]);
   const T py = fmin(fmax(X[1], B[2]), B[3]);
