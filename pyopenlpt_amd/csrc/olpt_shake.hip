@@ -348,6 +348,8 @@ __device__ __forceinline__ T eval_residue(const ShakeBuffers<T> &b, const CamLan
     const int prmin = (int)floor(ve - r_px), prmax = (int)ceil(ve + r_px + T(1));
     const int pcmin = (int)floor(ue - r_px), pcmax = (int)ceil(ue + r_px + T(1));
     const int roih = c.roi.rmax - c.roi.rmin;
+    // FP32: exp2 with log2(e) folded into b and c once per evaluation
+    const T b2 = P[1] * T(1.4426950408889634), c2 = P[2] * T(1.4426950408889634);
 #pragma unroll
     for (int sl = 0; sl < SLOTS; ++sl) {
       const int ro = sl * LPG / PW, co = sl * LPG % PW + sub;
@@ -357,9 +359,16 @@ __device__ __forceinline__ T eval_residue(const ShakeBuffers<T> &b, const CamLan
       const bool win = row >= prmin && row < prmax && col >= pcmin && col < pcmax;
       T pred = T(0);
       if constexpr (sizeof(T) == 4) {
-        // FP32: evaluated everywhere and selected, no branches per pixel
-        const T g = gauss_int<T>(P, ue, ve, row, col);
-        pred = win ? g : T(0);
+        // FP32: the intensity only where some lane of the wave needs it (a
+        // wave-uniform branch: the ROI's edge rows mostly lie outside every
+        // object's window), selected per lane without branches
+        if (ballot(win) != 0) {
+          const T dx = T(col) - ue, dy = T(row) - ve;
+          const T xx = mad_t(dx, P[3], dy * P[4]);
+          const T yy = mad_t(dy, P[3], -dx * P[4]);
+          const T g = P[0] * __builtin_amdgcn_exp2f(-mad_t(b2 * xx, xx, c2 * yy * yy));
+          pred = win && g > T(0) ? g : T(0);
+        }
       } else {
         if (win) pred = gauss_int<T>(P, ue, ve, row, col);
       }
