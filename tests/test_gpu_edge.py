@@ -214,3 +214,31 @@ def test_tuning_rejects_out_of_range_and_autotune_keeps_a_candidate(case):
     assert len(log) == 4 and len({tuple(sorted(t.items())) for t, _ in log}) == 4
     assert all(best[k] in v for k, v in space.items())
     ctx.close()
+
+
+@pytest.mark.parametrize("shape", [(40, 2100), (37, 1090), (9, 1027), (24, 4200), (6, 67),
+                                   (3, 3)])
+def test_peaks_match_oracle_on_odd_widths(shape):
+    """Rows of several 64-pixel chunks per 16-chunk group, partial last groups
+    and chunks, and tiny images: the GPU peaks are the oracle's, in order
+    (dense bright pixels with plateaus and equal neighbours)."""
+    from common import oracle_peaks
+    from pyopenlpt_amd import synth as S
+    from pyopenlpt_amd.context import Context, to_device_image
+    H, W = shape
+    rng = np.random.default_rng(H * 7919 + W)
+    img = rng.integers(0, 60, (H, W)).astype(np.uint16)
+    n = max(1, H * W // 12)
+    img[rng.integers(0, H, n), rng.integers(0, W, n)] = rng.integers(100, 3000, n).astype(np.uint16)
+    img[rng.integers(0, H, n // 3), rng.integers(0, W, n // 3)] = 500
+    p = S.default_params(0.04, min_obj_int=50)
+    want = oracle_peaks(img.astype(np.float64), p)
+    cap = max(64, H * W // 2)
+    ctx = Context(S.make_rig(H, W, n_cam=2), H, W, precision="f64", max_2d=cap,
+                  max_cand=1 << 12, max_obj=1 << 12)
+    d, code = to_device_image(img)
+    got = ctx.find_peaks(d, code, p, cap=cap)
+    assert got.shape == want.shape
+    if len(want):
+        assert np.abs(got - want).max() <= 1e-9
+    ctx.close()
