@@ -216,3 +216,61 @@ def test_pred_field_vs_oracle_31cube(ctx):
     assert np.abs(field - wf).max() <= 1e-9
     assert np.abs(disp - wd).max() <= 1e-9
     assert np.abs(wf).max() > 0.1
+
+
+def test_tracks_save_load_resume(tmp_path, gpu_available, gold):
+    """saveTracksAll after frame 6, loadTracksAll into a fresh STB (the
+    reference's load_track resume; like STB::loadTracksAll, STB.cpp:1052-1076,
+    it restores the two active lists): the loaded lists equal the saved ones
+    to the files' 6 decimals, and frames 7-9 on both STBs end with the same
+    active tracks and frames, and the same number of tracks leaving the
+    active lists.  The loaded points carry the files' rounding (5e-7 mm) into
+    later predictions, which can flip a branch of the shake search, so later
+    positions agree within a tenth of the shake width."""
+    from pyopenlpt_amd import config as C
+    from pyopenlpt_amd.stb import STB
+    cams = [Case._cam(r) for r in gold["cameras"]]
+    tr = ast.literal_eval(str(gold["tracer"]))
+    cfg = C.write_run_folder(str(tmp_path / "run"), cams, gold["frames"],
+                             axis_limit=(-20, 20, -20, 20, -20, 20),
+                             voxel_to_mm=float(gold["voxel_mm"]), tracer=tr)
+    setting = C.BasicSetting()
+    assert setting.readConfig(cfg)
+    ios = [C.ImageIO("", p) for p in setting.image_file_paths]
+    imgs = {f: np.stack([io.loadImg(f) for io in ios])
+            for f in range(setting.frame_start, setting.frame_end + 1)}
+    a = STB(setting, "Tracer", setting.object_config_paths[0], precision="f64",
+            write_files=False)
+    for f in range(0, 7):
+        a.process_frame(f, imgs[f])
+    save = str(tmp_path / "saved") + os.sep
+    os.makedirs(save, exist_ok=True)
+    a.saveTracksAll(save, 6)
+    left_a = a.rows(2)["n_tracks"] + a.rows(3)["n_tracks"]
+    b = STB(setting, "Tracer", setting.object_config_paths[0], precision="f64",
+            write_files=False)
+    b.loadTracksAll(save, 6)
+    for which in (0, 1):
+        ra, rb = a.rows(which), b.rows(which)
+        assert ra["n_tracks"] == rb["n_tracks"], LISTS[which]
+        assert np.array_equal(ra["track"], rb["track"]), LISTS[which]
+        assert np.array_equal(ra["frame"], rb["frame"]), LISTS[which]
+        if len(ra["xyz"]):
+            assert np.abs(ra["xyz"] - rb["xyz"]).max() <= 5.1e-7, LISTS[which]
+    assert a.rows(1)["n_tracks"] > 50
+    for f in range(7, 10):
+        a.process_frame(f, imgs[f])
+        b.process_frame(f, imgs[f])
+    tol = 0.1 * tr["shake_width_vox"] * float(gold["voxel_mm"])
+    for which in (0, 1):
+        ra, rb = a.rows(which), b.rows(which)
+        assert ra["n_tracks"] == rb["n_tracks"], LISTS[which]
+        assert np.array_equal(ra["frame"], rb["frame"]), LISTS[which]
+        if len(ra["xyz"]):
+            d = np.abs(ra["xyz"] - rb["xyz"]).max(1)
+            assert d.max() <= tol, LISTS[which]
+            assert np.median(d) <= 1e-6, LISTS[which]
+    left_after = a.rows(2)["n_tracks"] + a.rows(3)["n_tracks"]
+    assert left_after - left_a == b.rows(2)["n_tracks"] + b.rows(3)["n_tracks"]
+    a.close()
+    b.close()
