@@ -63,9 +63,9 @@ olpt_status olpt_set_concurrency(olpt_ctx *ctx, int n_streams);
  * work with a grid-stride loop, and every result whose order depends on
  * scheduling is ranked afterwards.  shake_blocks: one-wave workgroups of
  * the shake and proposal kernels (default 4096); build_blocks: 256-thread
- * workgroups of the stereo build kernels (default 4096); prune_blocks: cap
+ * workgroups of the stereo build kernels (default 1024); prune_blocks: cap
  * on the 256-thread workgroups of the sort/prune/triangulate kernels
- * (default 2048); n_streams: as olpt_set_concurrency.  Each field must lie
+ * (default 512); n_streams: as olpt_set_concurrency.  Each field must lie
  * in its range (blocks 64..65536, streams 1..8); a captured graph is
  * dropped.  Python's Context.autotune times candidates on the caller's
  * frames and keeps the fastest. */

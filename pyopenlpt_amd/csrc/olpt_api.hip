@@ -128,7 +128,7 @@ struct olpt_ctx {
   double host_ms = 0.0;
   int32_t host_calls = 0;
   // launch shapes (olpt_set_tuning)
-  olpt_tuning tune = {1, 4096, 4096, 2048};
+  olpt_tuning tune = {1, 4096, 1024, 512};
 };
 
 namespace {
