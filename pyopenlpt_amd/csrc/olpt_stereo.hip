@@ -736,19 +736,29 @@ __global__ void __launch_bounds__(256) k_b_enum(StereoBuffers b, StereoParams p)
     const int32_t *ccnt = b.cell_cnt + t * ncell;
     const int32_t *cpid = b.cell_pid + (long)t * b.cap;
     const double *xy = b.xy + (long)t * b.cap * 2;
-    // kPtBatch points per barrier (across cell rows): their loads are in
-    // flight together
+    // per barrier: up to kPtBatch points of the current row segment (their
+    // loads in flight together), then at most one cell row advanced
     while (__syncthreads_or(live)) {
       int pid[kPtBatch];
 #pragma unroll
+      for (int u = 0; u < kPtBatch; ++u)
+        pid[u] = live && cur.kk + u < cur.kend ? cpid[cur.kk + u] : -1;
+      if (live) cur.kk = min(cur.kk + kPtBatch, cur.kend);
+#pragma unroll
       for (int u = 0; u < kPtBatch; ++u) {
-        pid[u] = -1;
-        while (live && cur.kk == cur.kend) {
-          ++cy;
-          if (cy > r1) {
-            live = false;
-            break;
-          }
+        bool acc = false;
+        if (pid[u] >= 0) {
+          const double qx = xy[2 * pid[u]], qy = xy[2 * pid[u] + 1];
+          acc = true;
+#pragma unroll
+          for (int j = 0; j < K; ++j) acc = acc && in_strip(S[j], qx, qy, tol2);
+        }
+        bq_push(q, acc, make_int2(s, pid[u]));
+      }
+      if (live && cur.kk == cur.kend) {
+        if (++cy > r1) {
+          live = false;
+        } else {
           int xa = 0, xb = cols - 1;
           bool ok = true;
 #pragma unroll
@@ -763,18 +773,6 @@ __global__ void __launch_bounds__(256) k_b_enum(StereoBuffers b, StereoParams p)
           }
           if (ok) row_range(cstart, ccnt, cy, cols, xa, xb, cur);
         }
-        if (live) pid[u] = cpid[cur.kk++];
-      }
-#pragma unroll
-      for (int u = 0; u < kPtBatch; ++u) {
-        bool acc = false;
-        if (pid[u] >= 0) {
-          const double qx = xy[2 * pid[u]], qy = xy[2 * pid[u] + 1];
-          acc = true;
-#pragma unroll
-          for (int j = 0; j < K; ++j) acc = acc && in_strip(S[j], qx, qy, tol2);
-        }
-        bq_push(q, acc, make_int2(s, pid[u]));
       }
       bq_flush<kPtBatch>(q, b.n_pairs, b.cap_pairs, &b.flags[1], false, out);
     }
