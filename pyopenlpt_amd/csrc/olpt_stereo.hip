@@ -666,28 +666,26 @@ __global__ void __launch_bounds__(256) k_b_band(StereoBuffers b, StereoParams p)
       cstart += t1 * ncell;
       ccnt += t1 * ncell;
     }
-    // kPtBatch points per barrier (across the run's rows): their loads are
-    // in flight together
+    // per barrier: up to kPtBatch points of the current row segment (their
+    // loads in flight together), then at most one cell row advanced
     while (__syncthreads_or(live)) {
       int pid[kPtBatch];
 #pragma unroll
-      for (int u = 0; u < kPtBatch; ++u) {
-        pid[u] = -1;
-        while (live && cur.kk == cur.kend) {
-          if (++cy > cy_end) {
-            live = false;
-            break;
-          }
-          int xa, xb;
-          if (row_span(sr, cy, cols, xa, xb))
-            row_range(cstart, ccnt, cy, cols, xa, xb, cur);
-        }
-        if (live) pid[u] = cpid[cur.kk++];
-      }
+      for (int u = 0; u < kPtBatch; ++u)
+        pid[u] = live && cur.kk + u < cur.kend ? cpid[cur.kk + u] : -1;
+      if (live) cur.kk = min(cur.kk + kPtBatch, cur.kend);
 #pragma unroll
       for (int u = 0; u < kPtBatch; ++u) {
         const bool acc = pid[u] >= 0 && in_strip(s, xy[2 * pid[u]], xy[2 * pid[u] + 1], tol2);
         bq_push(q, acc, make_int2(r, pid[u]));
+      }
+      if (live && cur.kk == cur.kend) {
+        if (++cy > cy_end) {
+          live = false;
+        } else {
+          int xa, xb;
+          if (row_span(sr, cy, cols, xa, xb)) row_range(cstart, ccnt, cy, cols, xa, xb, cur);
+        }
       }
       bq_flush<kPtBatch>(q, b.n_pairs, b.cap_pairs, &b.flags[1], false, out);
     }
