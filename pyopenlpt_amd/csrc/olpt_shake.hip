@@ -473,8 +473,15 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
     T X[3] = {(T)b.xyz[3L * i], (T)b.xyz[3L * i + 1], (T)b.xyz[3L * i + 2]};
     build_roi<T, LPG, SLOTS>(b, p, rec, i, c);
     // shakeOneObject (:409-665); tmp keeps the last evaluated coordinate on
-    // each axis, as the reference's reused temporary object does
+    // each axis, as the reference's reused temporary object does.  The
+    // centre sample of an axis is tmp itself (the axis's own coordinate is
+    // still X[ax]), so when the previous axis ended on an evaluation of tmp
+    // its residue and projection are reused: the evaluation is a pure
+    // function of the point and the object's fixed ROI, so the value is the
+    // one a repeated evaluation would give, bit for bit.
     T tmp[3] = {X[0], X[1], X[2]};
+    bool have_last = false;  // (f_last, u_last, v_last) evaluated at tmp
+    T f_last = T(0), u_last = T(0), v_last = T(0);
     for (int ax = 0; ax < 3; ++ax) {
       const T x0 = X[ax];
       const T DL = -delta, DR = delta;
@@ -483,8 +490,18 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         tmp[ax] = x0 + rel[j];
-        f[j] = eval_residue<T, NC, OPW, SLOTS>(b, c, r_px, tmp, ue[j], ve[j]);
+        if (j == 1 && have_last) {
+          f[1] = f_last;
+          ue[1] = u_last;
+          ve[1] = v_last;
+        } else {
+          f[j] = eval_residue<T, NC, OPW, SLOTS>(b, c, r_px, tmp, ue[j], ve[j]);
+        }
       }
+      have_last = true;
+      f_last = f[2];
+      u_last = ue[2];
+      v_last = ve[2];
       const T fL = f[0], f0 = f[1], fR = f[2];
       const T tol = T(1e-3) * fmax(T(1), f0);
       const bool mono_dec = (fL > f0 + tol) && (f0 > fR + tol);
@@ -549,10 +566,11 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
         if (Dp > T(0)) {
           const T dxe = sgn >= 0 ? Dp : -Dp;
           tmp[ax] = x0 + dxe;
-          if (!any_out<T, SLOTS>(b, c, tmp)) {
-            T uu, vv;
+          have_last = !any_out<T, SLOTS>(b, c, tmp);
+          if (have_last) {
             cx[ncand] = dxe;
-            cf[ncand] = eval_residue<T, NC, OPW, SLOTS>(b, c, r_px, tmp, uu, vv);
+            cf[ncand] = eval_residue<T, NC, OPW, SLOTS>(b, c, r_px, tmp, u_last, v_last);
+            f_last = cf[ncand];
             cabs[ncand] = x0 + dxe;
             ++ncand;
           }
@@ -573,10 +591,11 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
         }
         if (xh >= xmin && xh <= xmax) {
           tmp[ax] = x0 + xh;
-          if (!any_out<T, SLOTS>(b, c, tmp)) {
-            T uu, vv;
+          have_last = !any_out<T, SLOTS>(b, c, tmp);
+          if (have_last) {
             cx[ncand] = xh;
-            cf[ncand] = eval_residue<T, NC, OPW, SLOTS>(b, c, r_px, tmp, uu, vv);
+            cf[ncand] = eval_residue<T, NC, OPW, SLOTS>(b, c, r_px, tmp, u_last, v_last);
+            f_last = cf[ncand];
             cabs[ncand] = x0 + xh;
             ++ncand;
           }
