@@ -6,11 +6,12 @@
 //                    OTF parameters at the 3D centre (getOTFParam)
 //   k_bin_*          CSR bins of the objects' radius boxes (calROIBound with
 //                    r_px) keyed by the box's top-left 8x8-pixel cell
-//   k_resid_gather   16x16 pixel tile per workgroup: the objects whose boxes
-//                    can touch the tile are staged in LDS, every pixel takes
-//                    the maximum projection over them and writes
-//                    res = orig - max(0, max_k p_k)  (the reference's min
-//                    fusion, Shake.cpp:226-241) - no atomics on pixels
+//   k_resid_gather   32x32 pixel tile per workgroup: the objects whose boxes
+//                    can touch the tile are staged in LDS and scatter their
+//                    box rows into an LDS image by integer max; every pixel
+//                    then writes res = orig - max(0, max_k p_k)  (the
+//                    reference's min fusion, Shake.cpp:226-241) - no
+//                    atomics on HBM
 //   k_shake          one wave64 per object, one lane group per camera: the
 //                    group owns the camera's projection, OTF parameters and
 //                    the pixels of its augmented ROI (buildROIInfo,
@@ -29,7 +30,6 @@
 
 namespace olpt {
 
-constexpr int kTile = 16;         // residual tile edge (pixels)
 constexpr int kTileObjCap = 256;  // objects staged per tile batch
 
 static int grid_for(long n, int threads, int max_blocks) {
@@ -118,71 +118,120 @@ __global__ void k_bin_fill(ShakeBuffers<T> b, const int32_t *__restrict__ n_ptr)
 }
 
 // mode 0: shake residual (active cameras, no clamp); mode 1: IPR output
-// (all cameras, clamp at 0); mode 2: rendered image max(0, max_k p_k).  The shake never reads an inactive camera's
-// residual, so mode 0 skips them.
-constexpr int kTileBinCap = 64;
+// (all cameras, clamp at 0); mode 2: rendered image max(0, max_k p_k).  The
+// shake never reads an inactive camera's residual, so mode 0 skips them.
+//
+// calResidueImage (Shake.cpp:165-245) subtracts from each pixel the largest
+// projection of the objects whose boxes cover it.  One workgroup owns a
+// kResTile x kResTile tile of one camera: it stages the objects whose box
+// anchors lie in the bins that can reach the tile, then each (object, box
+// row) work item writes its row's projection values inside the tile into an
+// LDS image with an integer max.  The values are >= 0 and the IEEE bit
+// patterns of non-negative numbers order as unsigned integers, so the max is
+// exact and independent of order: every pixel ends with the same value a
+// per-pixel loop over the objects gives, at box-area rather than
+// tile-objects work per pixel.
+constexpr int kTileBinCap = 64;    // bins a tile can reach: 7 x 7 for boxes <= 24 px
+constexpr int kResTile = 32;       // residual tile edge (pixels)
+constexpr int kResThreads = 256;
+static_assert(kTileBinCap == kWave, "the bin prefix is one wave's scan");
+
+template <typename T> struct MaxBits;
+template <> struct MaxBits<float> {
+  using U = uint32_t;
+  static __device__ U enc(float v) { return __float_as_uint(v); }
+  static __device__ float dec(U u) { return __uint_as_float(u); }
+};
+template <> struct MaxBits<double> {
+  using U = unsigned long long;
+  static __device__ U enc(double v) { return (U)__double_as_longlong(v); }
+  static __device__ double dec(U u) { return __longlong_as_double((long long)u); }
+};
+
 template <typename T>
-__global__ void __launch_bounds__(kTile *kTile)
+__global__ void __launch_bounds__(kResThreads)
 k_resid_gather(ShakeBuffers<T> b, const T *__restrict__ orig, T *__restrict__ out,
                const ObjCamRec<T> *__restrict__ rec, CamList cams, int mode, int box_w) {
+  using U = typename MaxBits<T>::U;
+  constexpr int NPX = kResTile * kResTile;
+  __shared__ U s_img[NPX];
   __shared__ ObjCamRec<T> s_obj[kTileObjCap];
   __shared__ int32_t s_bstart[kTileBinCap], s_bpre[kTileBinCap + 1];
   const int k = cams.id[blockIdx.z];
-  const int r0 = blockIdx.y * kTile, c0 = blockIdx.x * kTile;
+  const int r0 = blockIdx.y * kResTile, c0 = blockIdx.x * kResTile;
   const int tid = threadIdx.x;
-  const int row = r0 + tid / kTile, col = c0 + tid % kTile;
   const long HW = (long)b.H * b.W;
-  const bool inside = row < b.H && col < b.W;
-  const long pix = (long)row * b.W + col;
-  const T o = inside ? orig[k * HW + pix] : T(0);
   const long nbin = (long)b.bin_rows * b.bin_cols;
   const int32_t *bstart = b.bin_start + k * nbin;
   const int32_t *bcnt = b.bin_cnt + k * nbin;
   const int32_t *bobj = b.bin_obj + (long)k * b.cap_obj;
-  // anchors (box top-left) that can cover this tile
-  const int ar0 = max(0, r0 - box_w + 1), ar1 = min(b.H - 1, r0 + kTile - 1);
-  const int ac0 = max(0, c0 - box_w + 1), ac1 = min(b.W - 1, c0 + kTile - 1);
+  for (int q = tid; q < NPX; q += kResThreads) s_img[q] = 0;
+  // anchors (box top-left) that can cover this tile, and their bins
+  const int ar0 = max(0, r0 - box_w + 1), ar1 = min(b.H - 1, r0 + kResTile - 1);
+  const int ac0 = max(0, c0 - box_w + 1), ac1 = min(b.W - 1, c0 + kResTile - 1);
   const int br0 = ar0 / b.bin_px, br1 = ar1 / b.bin_px;
   const int bc0 = ac0 / b.bin_px, bc1 = ac1 / b.bin_px;
   const int nbc = bc1 - bc0 + 1;
-  const int nb = (br1 - br0 + 1) * nbc; // <= kTileBinCap (checked on the host)
-  if (tid < nb) {
-    const int bin = (br0 + tid / nbc) * b.bin_cols + bc0 + tid % nbc;
-    s_bstart[tid] = bstart[bin];
-    s_bpre[tid + 1] = bcnt[bin];
-  }
-  __syncthreads();
-  if (tid == 0) {
-    s_bpre[0] = 0;
-    for (int q = 0; q < nb; ++q) s_bpre[q + 1] += s_bpre[q];
+  const int nb = (br1 - br0 + 1) * nbc;  // <= kTileBinCap (box_w <= 24 checked on the host)
+  if (tid < kWave) {
+    int cnt = 0;
+    if (tid < nb) {
+      const int bin = (br0 + tid / nbc) * b.bin_cols + bc0 + tid % nbc;
+      s_bstart[tid] = bstart[bin];
+      cnt = bcnt[bin];
+    }
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int o = __shfl_up(cnt, off, kWave);
+      if (tid >= off) cnt += o;
+    }
+    s_bpre[tid + 1] = cnt;
+    if (tid == 0) s_bpre[0] = 0;
   }
   __syncthreads();
   const int total = s_bpre[nb];
-  T maxp = T(0);
+  const float inv_bw = 1.0f / (float)box_w;  // items < 256 * 24: the quotient is exact
   for (int base = 0; base < total; base += kTileObjCap) {
-    const int t = base + tid;
-    if (tid < kTileObjCap && t < total) {
-      int q = 0;
-      while (s_bpre[q + 1] <= t) ++q;
-      const int i = bobj[s_bstart[q] + (t - s_bpre[q])];
+    const int n = min(kTileObjCap, total - base);
+    if (tid < n) {
+      const int t = base + tid;
+      int lo = 0, hi = nb - 1;  // the last bin q with s_bpre[q] <= t
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_bpre[mid] <= t)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      const int i = bobj[s_bstart[lo] + (t - s_bpre[lo])];
       s_obj[tid] = rec[(long)i * b.n_cam + k];
     }
     __syncthreads();
-    const int n = min(kTileObjCap, total - base);
-    for (int u = 0; u < n; ++u) {
+    for (int it = tid; it < n * box_w; it += kResThreads) {
+      const int u = (int)(((float)it + 0.5f) * inv_bw);
       const ObjCamRec<T> &r = s_obj[u];
-      if (row >= r.rmin && row < r.rmax && col >= r.cmin && col < r.cmax) {
-        const T pv = gauss_int<T>(r.P, r.u, r.v, row, col);
-        if (pv > maxp) maxp = pv;
+      const int row = r.rmin + (it - u * box_w);
+      if (row < r.rmax && row >= r0 && row < r0 + kResTile) {
+        const int cb = max(r.cmin, c0), ce = min(r.cmax, c0 + kResTile);
+        U *dst = s_img + (row - r0) * kResTile - c0;
+        for (int col = cb; col < ce; ++col) {
+          const T pv = gauss_int<T>(r.P, r.u, r.v, row, col);
+          if (pv > T(0)) atomicMax(dst + col, MaxBits<T>::enc(pv));
+        }
       }
     }
     __syncthreads();
   }
-  if (inside) {
-    T v = o - maxp;
-    if (mode == 1 && v < T(0)) v = T(0);
-    if (mode == 2) v = maxp;   // rendering: max-fusion of the projections
-    out[k * HW + pix] = v;
+  for (int q = tid; q < NPX; q += kResThreads) {
+    const int row = r0 + q / kResTile, col = c0 + q % kResTile;
+    if (row < b.H && col < b.W) {
+      const long pix = (long)row * b.W + col;
+      const T maxp = MaxBits<T>::dec(s_img[q]);
+      T v = orig[k * HW + pix] - maxp;
+      if (mode == 1 && v < T(0)) v = T(0);
+      if (mode == 2) v = maxp;  // rendering: max-fusion of the projections
+      out[k * HW + pix] = v;
+    }
   }
 }
 
@@ -209,9 +258,9 @@ static void residual_pass(const ShakeBuffers<T> &b, const T *orig, T *out,
   scan_exclusive(b.bin_cnt, b.bin_start, nbin, nc, nbin, nullptr, b.scan_scratch, st);
   k_bin_fill<T><<<grid_for((long)b.cap_obj * nc, 256, 4096), 256, 0, st>>>(b, n_ptr);
   const int box_w = (int)lround(2.0 * (double)r_px);
-  dim3 g((b.W + kTile - 1) / kTile, (b.H + kTile - 1) / kTile, cams.n);
+  dim3 g((b.W + kResTile - 1) / kResTile, (b.H + kResTile - 1) / kResTile, cams.n);
   stage_begin(ST_RESID, st);
-  k_resid_gather<T><<<g, kTile * kTile, 0, st>>>(b, orig, out, rec, cams, mode, box_w);
+  k_resid_gather<T><<<g, kResThreads, 0, st>>>(b, orig, out, rec, cams, mode, box_w);
   stage_end(ST_RESID, st);
 }
 
