@@ -1,32 +1,42 @@
 // olpt_scan.hip - three-phase exclusive scan (tile sums, scan of tile sums,
 // tile-local scan plus carry).  Each tile is 4096 ints handled by one
-// 256-thread workgroup (16 contiguous ints per thread, LDS scan of the
+// 256-thread workgroup (16 contiguous ints per thread, wave-shuffle scan of the
 // per-thread partials), so a 4M-entry array is 1024 tiles.
 #include "olpt_scan.hpp"
 
 namespace olpt {
 
+// Workgroup exclusive scan: a shuffle scan inside each wave64, then every
+// thread adds the totals of the waves before its own (kScanThreads / 64
+// entries in LDS).  lds needs kScanThreads / 64 ints.
+constexpr int kScanWaves = kScanThreads / 64;
 __device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t *lds,
                                                    int32_t &block_total) {
-  const int t = threadIdx.x;
-  lds[t] = v;
-  __syncthreads();
-  for (int off = 1; off < kScanThreads; off <<= 1) {
-    int32_t x = t >= off ? lds[t - off] : 0;
-    __syncthreads();
-    lds[t] += x;
-    __syncthreads();
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int32_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
   }
-  const int32_t incl = lds[t];
-  block_total = lds[kScanThreads - 1];
+  if (lane == 63) lds[w] = x;
   __syncthreads();
-  return incl - v;
+  int32_t pre = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kScanWaves; ++i) {
+    const int32_t s = lds[i];
+    pre += i < w ? s : 0;
+    tot += s;
+  }
+  block_total = tot;
+  __syncthreads();  // lds may be reused by the next call
+  return pre + x - v;
 }
 
 __global__ void __launch_bounds__(kScanThreads)
 k_tile_sums(const int32_t *__restrict__ in, long n, long stride,
             int32_t *__restrict__ tsum, long tiles) {
-  __shared__ int32_t lds[kScanThreads];
+  __shared__ int32_t lds[kScanWaves];
   const int seg = blockIdx.y;
   const long tile = blockIdx.x;
   const int32_t *p = in + seg * stride;
@@ -46,7 +56,7 @@ k_tile_sums(const int32_t *__restrict__ in, long n, long stride,
 __global__ void __launch_bounds__(kScanThreads)
 k_scan_tile_sums(int32_t *__restrict__ tsum, long tiles,
                  int32_t *__restrict__ total) {
-  __shared__ int32_t lds[kScanThreads];
+  __shared__ int32_t lds[kScanWaves];
   const int seg = blockIdx.x;
   int32_t *p = tsum + seg * (tiles + 1);
   int32_t carry = 0;
@@ -64,7 +74,7 @@ k_scan_tile_sums(int32_t *__restrict__ tsum, long tiles,
 __global__ void __launch_bounds__(kScanThreads)
 k_tile_scan(const int32_t *__restrict__ in, int32_t *__restrict__ out, long n,
             long stride, const int32_t *__restrict__ tsum, long tiles) {
-  __shared__ int32_t lds[kScanThreads];
+  __shared__ int32_t lds[kScanWaves];
   const int seg = blockIdx.y;
   const long tile = blockIdx.x;
   const int32_t *p = in + seg * stride;
