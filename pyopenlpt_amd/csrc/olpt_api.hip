@@ -711,6 +711,7 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   const long grid_cells = (long)c->grid_n * c->grid_n * c->grid_n;
   long scan_n = std::max<long>(std::max<long>(M, (long)nc * cap), std::max<long>(ncell, grid_cells));
   scan_n = std::max<long>(scan_n, nchunk);
+  scan_n = std::max<long>(scan_n, (long)c->bin_rows * c->bin_cols * nc);
   A(w.scan_scratch, int32_t, scan_scratch_ints(scan_n, nc) + 64);
   b.scan_scratch = w.scan_scratch;
   A(w.totals, int32_t, 64);
@@ -728,12 +729,12 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   A(w.n_keep, int32_t, 1);
   const long nbin = (long)c->bin_rows * c->bin_cols;
   A(w.bin_cnt, int32_t, nbin * nc);
-  A(w.bin_start, int32_t, nbin * nc);
+  A(w.bin_start, int32_t, nbin * nc + 1);  // + the total: counts are start differences
   A(w.bin_obj, int32_t, (long)O * nc);
   A(w.obj_bin, int32_t, (long)O * nc);
   A(w.obj_rank, int32_t, (long)O * nc);
   A(w.g_cnt, int32_t, grid_cells);
-  A(w.g_start, int32_t, grid_cells);
+  A(w.g_start, int32_t, grid_cells + 1);
   A(w.g_obj, int32_t, O);
   A(w.g_rank, int32_t, O);
   A(w.g_cellof, int32_t, O);
@@ -746,6 +747,10 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   // another context's old values)
   fill_i32(w.flags, 16, 0, nullptr);
   fill_i32(w.npts, OLPT_MAX_CAM, 0, nullptr);
+  // count arrays of the count -> scan -> fill passes: zero here, and each
+  // pass's scan leaves them zero again
+  fill_i32(w.bin_cnt, nbin * nc, 0, nullptr);
+  fill_i32(w.g_cnt, grid_cells, 0, nullptr);
   return OLPT_OK;
 }
 

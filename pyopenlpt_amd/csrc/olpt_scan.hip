@@ -71,13 +71,17 @@ k_scan_tile_sums(int32_t *__restrict__ tsum, long tiles,
   if (threadIdx.x == 0 && total) total[seg] = carry;
 }
 
+// CLEAR: the input is zeroed once read (in and out distinct), so a count
+// array is ready for its next round of atomic increments without a launch
+// of its own.
+template <bool CLEAR>
 __global__ void __launch_bounds__(kScanThreads)
-k_tile_scan(const int32_t *__restrict__ in, int32_t *__restrict__ out, long n,
+k_tile_scan(int32_t *__restrict__ in, int32_t *__restrict__ out, long n,
             long stride, const int32_t *__restrict__ tsum, long tiles) {
   __shared__ int32_t lds[kScanWaves];
   const int seg = blockIdx.y;
   const long tile = blockIdx.x;
-  const int32_t *p = in + seg * stride;
+  int32_t *p = in + seg * stride;
   int32_t *q = out + seg * stride;
   const long base = tile * kScanTile + (long)threadIdx.x * kScanPerThread;
   int32_t v[kScanPerThread];
@@ -87,6 +91,11 @@ k_tile_scan(const int32_t *__restrict__ in, int32_t *__restrict__ out, long n,
     const long i = base + k;
     v[k] = i < n ? p[i] : 0;
     s += v[k];
+  }
+  if constexpr (CLEAR) {
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; ++k)
+      if (base + k < n) p[base + k] = 0;
   }
   int32_t tot;
   int32_t run = block_excl_scan(s, lds, tot) + tsum[seg * (tiles + 1) + tile];
@@ -106,7 +115,18 @@ void scan_exclusive(const int32_t *in, int32_t *out, long n, int segments,
   dim3 g((unsigned)tiles, (unsigned)segments);
   k_tile_sums<<<g, kScanThreads, 0, st>>>(in, n, seg_stride, scratch, tiles);
   k_scan_tile_sums<<<segments, kScanThreads, 0, st>>>(scratch, tiles, total);
-  k_tile_scan<<<g, kScanThreads, 0, st>>>(in, out, n, seg_stride, scratch, tiles);
+  k_tile_scan<false><<<g, kScanThreads, 0, st>>>(const_cast<int32_t *>(in), out, n, seg_stride,
+                                                 scratch, tiles);
+}
+
+void scan_exclusive_clear(int32_t *in, int32_t *out, long n, int32_t *total, int32_t *scratch,
+                          hipStream_t st) {
+  if (n <= 0) return;
+  const long tiles = (n + kScanTile - 1) / kScanTile;
+  dim3 g((unsigned)tiles, 1);
+  k_tile_sums<<<g, kScanThreads, 0, st>>>(in, n, n, scratch, tiles);
+  k_scan_tile_sums<<<1, kScanThreads, 0, st>>>(scratch, tiles, total);
+  k_tile_scan<true><<<g, kScanThreads, 0, st>>>(in, out, n, n, scratch, tiles);
 }
 
 } // namespace olpt

@@ -25,4 +25,11 @@ void scan_exclusive(const int32_t *in, int32_t *out, long n, int segments,
                     long seg_stride, int32_t *total, int32_t *scratch,
                     hipStream_t st);
 
+// One segment, and in[] is left all zero (in and out must not alias): the
+// counts of a count -> scan -> fill pass are ready for the next pass.
+// total may be null; pointing it at out + n makes out[i + 1] - out[i] the
+// count of entry i for every i < n.
+void scan_exclusive_clear(int32_t *in, int32_t *out, long n, int32_t *total, int32_t *scratch,
+                          hipStream_t st);
+
 } // namespace olpt

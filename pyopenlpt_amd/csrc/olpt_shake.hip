@@ -39,16 +39,6 @@ static int grid_for(long n, int threads, int max_blocks) {
   return (int)g;
 }
 
-__global__ void k_zero_i32_s(int32_t *p, long n) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x)
-    p[i] = 0;
-}
-static void zero_i32(int32_t *p, long n, hipStream_t st) {
-  if (n <= 0) return;
-  k_zero_i32_s<<<grid_for(n, 256, 4096), 256, 0, st>>>(p, n);
-}
-
 // per (object, camera): u, v, box, OTF params
 template <typename T> struct ObjCamRec {
   T u, v;
@@ -56,12 +46,16 @@ template <typename T> struct ObjCamRec {
   int32_t rmin, rmax, cmin, cmax;
 };
 
-template <typename T>
+// BIN: also count the object's box in its anchor bin (the box's top-left
+// 8x8-pixel cell); bin_cnt is all zero on entry (the scan that turns the
+// counts into offsets clears it again).
+template <typename T, bool BIN>
 __global__ void k_obj_project(ShakeBuffers<T> b, const double *__restrict__ xyz,
                               const int32_t *__restrict__ n_ptr, ObjCamRec<T> *rec,
                               T *uv_out, T r_px) {
   const int n = *n_ptr;
   const int nc = b.n_cam;
+  const long nbin = (long)b.bin_rows * b.bin_cols;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const T X[3] = {(T)xyz[3L * i], (T)xyz[3L * i + 1], (T)xyz[3L * i + 2]};
     for (int k = 0; k < nc; ++k) {
@@ -78,27 +72,16 @@ __global__ void k_obj_project(ShakeBuffers<T> b, const double *__restrict__ xyz,
         uv_out[((long)i * nc + k) * 2] = r.u;
         uv_out[((long)i * nc + k) * 2 + 1] = r.v;
       }
+      if constexpr (BIN) {
+        int bin = -1, rank = 0;
+        if (!pr.empty()) {
+          bin = (pr.rmin / b.bin_px) * b.bin_cols + (pr.cmin / b.bin_px);
+          rank = atomicAdd(&b.bin_cnt[k * nbin + bin], 1);
+        }
+        b.obj_bin[(long)k * b.cap_obj + i] = bin;
+        b.obj_rank[(long)k * b.cap_obj + i] = rank;
+      }
     }
-  }
-}
-
-template <typename T>
-__global__ void k_bin_count(ShakeBuffers<T> b, const int32_t *__restrict__ n_ptr,
-                            const ObjCamRec<T> *__restrict__ rec) {
-  const int n = *n_ptr;
-  const int nc = b.n_cam;
-  const long nbin = (long)b.bin_rows * b.bin_cols;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < (long)n * nc;
-       t += (long)gridDim.x * blockDim.x) {
-    const int i = (int)(t / nc), k = (int)(t % nc);
-    const ObjCamRec<T> &r = rec[t];
-    int bin = -1, rank = 0;
-    if (!(r.rmax < r.rmin || r.cmax < r.cmin)) {
-      bin = (r.rmin / b.bin_px) * b.bin_cols + (r.cmin / b.bin_px);
-      rank = atomicAdd(&b.bin_cnt[k * nbin + bin], 1);
-    }
-    b.obj_bin[(long)k * b.cap_obj + i] = bin;
-    b.obj_rank[(long)k * b.cap_obj + i] = rank;
   }
 }
 
@@ -112,8 +95,7 @@ __global__ void k_bin_fill(ShakeBuffers<T> b, const int32_t *__restrict__ n_ptr)
     const int i = (int)(t / nc), k = (int)(t % nc);
     const int bin = b.obj_bin[(long)k * b.cap_obj + i];
     if (bin < 0) continue;
-    b.bin_obj[(long)k * b.cap_obj + b.bin_start[k * nbin + bin] +
-              b.obj_rank[(long)k * b.cap_obj + i]] = i;
+    b.bin_obj[b.bin_start[k * nbin + bin] + b.obj_rank[(long)k * b.cap_obj + i]] = i;
   }
 }
 
@@ -162,9 +144,8 @@ k_resid_gather(ShakeBuffers<T> b, const T *__restrict__ orig, T *__restrict__ ou
   const int tid = threadIdx.x;
   const long HW = (long)b.H * b.W;
   const long nbin = (long)b.bin_rows * b.bin_cols;
-  const int32_t *bstart = b.bin_start + k * nbin;
-  const int32_t *bcnt = b.bin_cnt + k * nbin;
-  const int32_t *bobj = b.bin_obj + (long)k * b.cap_obj;
+  const int32_t *bstart = b.bin_start + k * nbin;  // offsets into bin_obj, all cameras
+  const int32_t *bobj = b.bin_obj;
   for (int q = tid; q < NPX; q += kResThreads) s_img[q] = 0;
   // anchors (box top-left) that can cover this tile, and their bins
   const int ar0 = max(0, r0 - box_w + 1), ar1 = min(b.H - 1, r0 + kResTile - 1);
@@ -178,7 +159,7 @@ k_resid_gather(ShakeBuffers<T> b, const T *__restrict__ orig, T *__restrict__ ou
     if (tid < nb) {
       const int bin = (br0 + tid / nbc) * b.bin_cols + bc0 + tid % nbc;
       s_bstart[tid] = bstart[bin];
-      cnt = bcnt[bin];
+      cnt = bstart[bin + 1] - bstart[bin];
     }
 #pragma unroll
     for (int off = 1; off < kWave; off <<= 1) {
@@ -251,11 +232,12 @@ static void residual_pass(const ShakeBuffers<T> &b, const T *orig, T *out,
                           T r_px, hipStream_t st, const double *amp = nullptr) {
   const int nc = b.n_cam;
   const long nbin = (long)b.bin_rows * b.bin_cols;
-  k_obj_project<T><<<grid_for(b.cap_obj, 256, 2048), 256, 0, st>>>(b, xyz, n_ptr, rec, uv_out, r_px);
+  k_obj_project<T, true><<<grid_for(b.cap_obj, 256, 2048), 256, 0, st>>>(b, xyz, n_ptr, rec,
+                                                                         uv_out, r_px);
   if (amp) k_set_amp<T><<<grid_for((long)b.cap_obj * nc, 256, 2048), 256, 0, st>>>(rec, amp, n_ptr, nc);
-  zero_i32(b.bin_cnt, nbin * nc, st);
-  k_bin_count<T><<<grid_for((long)b.cap_obj * nc, 256, 4096), 256, 0, st>>>(b, n_ptr, rec);
-  scan_exclusive(b.bin_cnt, b.bin_start, nbin, nc, nbin, nullptr, b.scan_scratch, st);
+  // one scan over all cameras' bins; bin_start[nc * nbin] receives the total
+  scan_exclusive_clear(b.bin_cnt, b.bin_start, nbin * nc, b.bin_start + nbin * nc,
+                       b.scan_scratch, st);
   k_bin_fill<T><<<grid_for((long)b.cap_obj * nc, 256, 4096), 256, 0, st>>>(b, n_ptr);
   const int box_w = (int)lround(2.0 * (double)r_px);
   dim3 g((b.W + kResTile - 1) / kResTile, (b.H + kResTile - 1) / kResTile, cams.n);
@@ -888,7 +870,7 @@ __device__ __forceinline__ int count_nb(const ShakeBuffers<T> &b, int i, double 
         const int x = cx + dx, y = cy + dy, z = cz + dz;
         if (x < 0 || y < 0 || z < 0 || x >= G || y >= G || z >= G) continue;
         const int cell = (x * G + y) * G + z;
-        const int s0 = b.g_start[cell], ncell = b.g_cnt[cell];
+        const int s0 = b.g_start[cell], ncell = b.g_start[cell + 1] - s0;
         for (int t = 0; t < ncell; ++t) {
           const int j = b.g_obj[s0 + t];
           if (j == i) continue;
@@ -1067,14 +1049,13 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
     if (delta < dmin) delta = dmin;
   }
   if (p.n_shake_loop <= 0) {
-    k_obj_project<T><<<gO, 256, 0, st>>>(b, b.xyz, b.n_obj, rec, b.uv, r_px);
+    k_obj_project<T, false><<<gO, 256, 0, st>>>(b, b.xyz, b.n_obj, rec, b.uv, r_px);
   }
   // markRepeatedObj
   const double t2 = p.tol3d * p.tol3d;
   const long ncell = (long)b.grid_n * b.grid_n * b.grid_n;
-  zero_i32(b.g_cnt, ncell, st);
-  k_grid_count<T><<<gO, 256, 0, st>>>(b);
-  scan_exclusive(b.g_cnt, b.g_start, ncell, 1, ncell, nullptr, b.scan_scratch, st);
+  k_grid_count<T><<<gO, 256, 0, st>>>(b);  // g_cnt is all zero here; the scan clears it
+  scan_exclusive_clear(b.g_cnt, b.g_start, ncell, b.g_start + ncell, b.scan_scratch, st);
   k_grid_fill<T><<<gO, 256, 0, st>>>(b);
   k_has_nb<T><<<gO, 256, 0, st>>>(b, t2);
   scan_exclusive(b.has_nb, b.g_rank, b.cap_obj, 1, b.cap_obj, b.totals, b.scan_scratch, st);
