@@ -208,6 +208,21 @@ __device__ __forceinline__ bool triangulate_n(const double (*pt)[3], const doubl
   return ok;
 }
 
+// triangulate_n<n> for a runtime n <= MC on arrays of MC rows (rows past MC
+// are never named, so register arrays of MC rows stay in registers)
+template <int MC>
+__device__ __forceinline__ bool triangulate_upto(const double (*pt)[3], const double (*u)[3],
+                                                 int n, double X[3], double &err) {
+  if (n == 2) return triangulate_n<2>(pt, u, X, err);
+  if constexpr (MC >= 3) if (n == 3) return triangulate_n<3>(pt, u, X, err);
+  if constexpr (MC >= 4) if (n == 4) return triangulate_n<4>(pt, u, X, err);
+  if constexpr (MC >= 5) if (n == 5) return triangulate_n<5>(pt, u, X, err);
+  if constexpr (MC >= 6) if (n == 6) return triangulate_n<6>(pt, u, X, err);
+  if constexpr (MC >= 7) if (n == 7) return triangulate_n<7>(pt, u, X, err);
+  if constexpr (MC >= 8) if (n == 8) return triangulate_n<8>(pt, u, X, err);
+  return false;
+}
+
 __device__ __forceinline__ bool triangulate(const double (*pt)[3], const double (*u)[3],
                                             int n, double X[3], double &err) {
   switch (n) {

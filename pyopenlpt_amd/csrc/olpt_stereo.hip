@@ -822,11 +822,13 @@ __global__ void __launch_bounds__(256) k_b_extend(StereoBuffers b, StereoParams 
     if (terminal) {
       CandRec cr;
       if (res == 1) {
-        for (int c = 0; c < nc; ++c) {
+        // compile-time camera index: the record stays in registers
+#pragma unroll
+        for (int c = 0; c < kMaxCam; ++c) {
           int id = -1;
 #pragma unroll
           for (int j = 0; j <= K; ++j) id = P.cam[j] == c ? P.pid[j] : id;
-          cr.ids[c] = id;
+          cr.ids[c] = c < nc ? id : -1;
         }
         cr.e = e;
       }
@@ -1321,7 +1323,11 @@ __global__ void k_replace_check(StereoBuffers b, StereoParams p) {
   }
 }
 
-// triangulateMatch: camera-order LOS, FP64, gate on tol_3d
+// triangulateMatch: camera-order LOS, FP64, gate on tol_3d.  A candidate's
+// cameras are gathered into slots 0..n-1 in camera order by selects over
+// compile-time slot and camera indices (MC >= n_cam), so the LOS arrays stay
+// in registers.
+template <int MC>
 __global__ void __launch_bounds__(256) k_tri(StereoBuffers b, StereoParams p) {
   const int M = min(*b.n_cand, b.cap_cand);
   const int nc = b.n_cam;
@@ -1332,20 +1338,38 @@ __global__ void __launch_bounds__(256) k_tri(StereoBuffers b, StereoParams p) {
     int keep = 0;
     if (j < M && b.state[j] == 1) {
       ++nsel;
-      double pt[kMaxCam][3], u[kMaxCam][3];
+      int pidc[MC];
+#pragma unroll
+      for (int c = 0; c < MC; ++c) pidc[c] = c < nc ? b.cand_ids_s[(long)j * nc + c] : -1;
+      double pt[MC][3], u[MC][3];
+#pragma unroll
+      for (int s = 0; s < MC; ++s)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) pt[s][d] = u[s][d] = 0.0;
       int n = 0;
-      for (int c = 0; c < nc; ++c) {
-        const int pid = b.cand_ids_s[(long)j * nc + c];
-        if (pid < 0) continue;
-        const double *uu = b.los + ((long)c * b.cap + pid) * 4;
+#pragma unroll
+      for (int c = 0; c < MC; ++c) {
+        if (pidc[c] < 0) continue;
+        const double *uu = b.los + ((long)c * b.cap + pidc[c]) * 4;
+        double tc[3], uc[3];
+#pragma unroll
         for (int d = 0; d < 3; ++d) {
-          pt[n][d] = b.cams[c].Tinv[d];
-          u[n][d] = uu[d];
+          tc[d] = b.cams[c].Tinv[d];
+          uc[d] = uu[d];
         }
+#pragma unroll
+        for (int s = 0; s <= c; ++s)
+          if (s == n) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              pt[s][d] = tc[d];
+              u[s][d] = uc[d];
+            }
+          }
         ++n;
       }
       double X[3], e;
-      if (n >= 2 && triangulate(pt, u, n, X, e) && !(e > p.tol3d)) {
+      if (n >= 2 && triangulate_upto<MC>(pt, u, n, X, e) && !(e > p.tol3d)) {
         keep = 1;
         b.tri_xyz[(long)j * 3 + 0] = X[0];
         b.tri_xyz[(long)j * 3 + 1] = X[1];
@@ -1460,7 +1484,10 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   k_mis_finish<<<1, kMisFinishThreads, 0, st>>>(b, p);
   k_replace_check<<<gM, T, 0, st>>>(b, p);
   // triangulate + compact in candidate order
-  k_tri<<<gM, T, 0, st>>>(b, p);
+  if (b.n_cam <= 4)
+    k_tri<4><<<gM, T, 0, st>>>(b, p);
+  else
+    k_tri<kMaxCam><<<gM, T, 0, st>>>(b, p);
   scan_exclusive(b.keep, b.keep_off, b.cap_cand, 1, b.cap_cand, b.totals,
                  b.scan_scratch, st);
   k_obj_write<<<gM, T, 0, st>>>(b);
