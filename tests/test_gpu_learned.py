@@ -77,6 +77,42 @@ def test_render_matches_numpy(setup):
     assert np.abs(got - want).max() <= 1e-9
 
 
+@pytest.mark.parametrize("r_px", [2.5, 12.0])
+def test_render_partial_tiles_dense_cluster(gpu_available, r_px):
+    """The residual/render kernel on a 250 x 270 frame (partial 32-pixel
+    tiles at the right and bottom edges), boxes up to the largest supported
+    (24 px at r = 12), and a cluster dense enough that the objects reaching
+    one tile exceed one LDS batch of 256."""
+    from pyopenlpt_amd import synth as S
+    from pyopenlpt_amd.context import Context
+    H, W = 250, 270
+    cams = S.make_rig(H, W)
+    ctx = Context(cams, H, W, precision="f64", max_2d=1 << 14, max_cand=1 << 16,
+                  max_obj=1 << 12)
+    otf, keep = S.make_otf(len(cams), (125.0, 1.5, 1.5, 0.0))
+    ctx.set_otf(otf)
+    ctx._keep = keep
+    rng = np.random.default_rng(11)
+    xyz = np.concatenate([rng.uniform(-2.0, 2.0, (400, 3)) + np.array([5.0, -3.0, 2.0]),
+                          rng.uniform(-21.0, 21.0, (300, 3))])
+    amp = rng.uniform(80, 160, len(xyz))
+    try:
+        got = ctx.render(torch.from_numpy(xyz).cuda(), torch.from_numpy(amp).cuda(), r_px=r_px)
+        got = got.cpu().numpy()
+    finally:
+        ctx.close()
+    want = np.zeros((len(cams), H, W))
+    for k, cam in enumerate(cams):
+        for i in range(len(xyz)):
+            pb = proj_box(cam, xyz[i], amp[i], 1.5, 1.5, r_px, H, W)
+            if pb is None:
+                continue
+            rr, cc, val, _ = pb
+            want[k, rr, cc] = np.maximum(want[k, rr, cc], val)
+    assert (want > 0).sum() > 0.05 * want.size
+    assert np.abs(got - want).max() <= 1e-9
+
+
 def test_patches_match_numpy(setup):
     case, ctx = setup
     from pyopenlpt_amd import _lib
