@@ -3,9 +3,11 @@
 // Per shake iteration (Shake::runShake, Shake.cpp:36-163; Jacobi: every
 // object sees the residual built from the start-of-iteration positions):
 //   k_obj_project    2D projection of every object in every camera plus its
-//                    OTF parameters at the 3D centre (getOTFParam)
-//   k_bin_*          CSR bins of the objects' radius boxes (calROIBound with
-//                    r_px) keyed by the box's top-left 8x8-pixel cell
+//                    OTF parameters at the 3D centre (getOTFParam), counted
+//                    into the bin of its radius box (calROIBound with r_px),
+//                    keyed by the box's top-left 8x8-pixel cell
+//   scan, k_bin_fill CSR bins over all cameras (the scan leaves the counts
+//                    zero for the next pass)
 //   k_resid_gather   32x32 pixel tile per workgroup: the objects whose boxes
 //                    can touch the tile are staged in LDS and scatter their
 //                    box rows into an LDS image by integer max; every pixel
