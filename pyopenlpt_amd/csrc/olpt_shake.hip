@@ -269,9 +269,29 @@ __global__ void k_copy_cam(const T *__restrict__ src, T *__restrict__ dst, long 
 // holds the same value and the branchy per-axis search of shakeOneObject
 // stays uniform per object.  Several objects per wave share the per-camera
 // projection and OTF work of a lane over more pixels.
+// Lane xor 1 or 2 inside each quad of lanes as a DPP quad permutation (a
+// register move, where __shfl_xor goes through the LDS crossbar).
+template <int X> __device__ __forceinline__ int quad_xor(int v) {
+  static_assert(X == 1 || X == 2, "quad permutations only");
+  constexpr int ctrl = X == 1 ? 0xB1 : 0x4E;  // quad_perm [1,0,3,2] / [2,3,0,1]
+  return __builtin_amdgcn_update_dpp(0, v, ctrl, 0xF, 0xF, false);
+}
+template <int X> __device__ __forceinline__ float quad_xor(float v) {
+  return __int_as_float(quad_xor<X>(__float_as_int(v)));
+}
+template <int X> __device__ __forceinline__ double quad_xor(double v) {
+  const long long u = __double_as_longlong(v);
+  const int lo = quad_xor<X>((int)(u & 0xffffffffll)), hi = quad_xor<X>((int)(u >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the LPG lanes of a group (every lane of the group active); the
+// additions are v += v[lane ^ off] for off = LPG/2 .. 1 in that order.
 template <int LPG, typename T> __device__ __forceinline__ T group_sum(T v) {
 #pragma unroll
-  for (int off = LPG / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  for (int off = LPG / 2; off > 2; off >>= 1) v += __shfl_xor(v, off, kWave);
+  if constexpr (LPG >= 4) v += quad_xor<2>(v);
+  if constexpr (LPG >= 2) v += quad_xor<1>(v);
   return v;
 }
 template <int LPG, typename T> __device__ __forceinline__ T group_max(T v) {
