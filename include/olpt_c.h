@@ -35,7 +35,12 @@ enum {
   OLPT_E_NO_ENOUGH_DATA = 5,   /* ErrorCode::NoEnoughData */
   OLPT_E_GPU = 100,            /* HIP runtime error */
   OLPT_E_ALLOC = 101,          /* device allocation failed */
-  OLPT_E_CAPACITY = 102        /* a workspace capacity was exceeded */
+  OLPT_E_CAPACITY = 102,       /* a workspace capacity was exceeded */
+  /* pruneMatch (StereoMatch.cpp:631-980) on this input depends on what the
+   * library does not replay: eq_eps-equal values that are not transitive
+   * (the reference's order is then whatever std::stable_sort produces) or
+   * a firing replacement branch.  Outputs are not valid. */
+  OLPT_E_PRUNE_TIES = 103
 };
 
 enum { OLPT_U8 = 0, OLPT_U16 = 1, OLPT_F32 = 2, OLPT_F64 = 3 };
@@ -85,9 +90,10 @@ olpt_status olpt_sync(olpt_ctx *ctx);
 void *olpt_stream(olpt_ctx *ctx);
 const char *olpt_last_error(void);
 /* Device-side diagnostic counters of the last stereo/shake run:
- * [0] candidate overflow, [1] build list overflow, [2] unused,
- * [3] prune replacement cases, [4] object overflow, [5] prune rounds
- * exhausted, [6] checkMatch triangulation failures, [7] peak overflow. */
+ * [0] candidate overflow, [1] build list overflow, [2] prune keys with
+ * non-transitive eq_eps ties, [3] prune replacement cases, [4] object
+ * overflow, [5] prune rounds exhausted, [6] checkMatch triangulation
+ * failures, [7] peak overflow. */
 olpt_status olpt_diagnostics(olpt_ctx *ctx, int32_t out[8]);
 
 /* Stage timing with HIP events on the context stream (eager runs only).
@@ -133,6 +139,18 @@ olpt_status olpt_stereo_match(olpt_ctx *ctx, const int32_t *active,
                               uint8_t *d_sel, int cap_cand, double *d_obj_xyz,
                               int32_t *d_obj_ids, int cap_obj,
                               int32_t *counts);
+
+/* StereoMatch::pruneMatch (StereoMatch.cpp:631-980) on given candidates
+ * (match() calls it on its id-sorted candidates, :153-156): d_cand_ids
+ * [n_cand][n_cam] (-1 = camera unused), d_cand_e [n_cand] their e_check,
+ * n_pts (host) the 2D point count per camera, active (host, NULL = all
+ * active).  d_sel [n_cand] = 1 for the selected candidates (the reference
+ * returns them in this index order, :973-978).  OLPT_E_PRUNE_TIES when the
+ * selection would depend on non-transitive eq_eps ties or the replacement
+ * branch. */
+olpt_status olpt_prune_match(olpt_ctx *ctx, const int32_t *active, const int32_t *n_pts,
+                             const olpt_ipr_params *params, const int32_t *d_cand_ids,
+                             const double *d_cand_e, int n_cand, uint8_t *d_sel);
 
 /* Shake(cams, cfg).runShake(objs, imgs) (Shake.cpp:36-163) on n_obj
  * objects, then calResidualImage of the unflagged objects (IPR mode,

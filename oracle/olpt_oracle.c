@@ -1956,6 +1956,22 @@ int orc_stereo_match(const olpt_camera *cams, int n_cam, const int32_t *active,
   return OK;
 }
 
+/* StereoMatch::pruneMatch alone (StereoMatch.cpp:631-980) on given
+ * candidates: ids [M][n_cam] (-1 = camera unused), e [M]; n_pts[c] 2D
+ * points per camera; sel[j] = 1 for the selected candidates. */
+int orc_prune(int n_cam, const int32_t *active, const int32_t *n_pts, const int32_t *ids,
+              const double *e, int M, uint8_t *sel) {
+  smctx S;
+  memset(&S, 0, sizeof(S));
+  if (n_cam < 1 || n_cam > OLPT_MAX_CAM) return ERR_ARG;
+  S.n_cam = n_cam;
+  for (int c = 0; c < n_cam; ++c) {
+    S.active[c] = active ? active[c] != 0 : 1;
+    S.npts[c] = n_pts[c];
+  }
+  return prune_match(&S, ids, e, M, sel);
+}
+
 int orc_shake(const olpt_camera *cams, int n_cam, const int32_t *active,
               const double *imgs, int H, int W, const olpt_ipr_params *p,
               const olpt_otf *otf, const double *xyz_in, int n_obj,

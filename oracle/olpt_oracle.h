@@ -38,6 +38,8 @@ int orc_stereo_match(const olpt_camera *cams, int n_cam, const int32_t *active,
                      const olpt_ipr_params *p, int cap_cand, int32_t *cand_ids,
                      double *cand_e, int32_t *sel_idx, int cap_obj,
                      double *obj_xyz, int32_t *obj_ids, int32_t *counts);
+int orc_prune(int n_cam, const int32_t *active, const int32_t *n_pts, const int32_t *ids,
+              const double *e, int M, uint8_t *sel);
 int orc_shake(const olpt_camera *cams, int n_cam, const int32_t *active,
               const double *imgs, int H, int W, const olpt_ipr_params *p,
               const olpt_otf *otf, const double *xyz_in, int n_obj,

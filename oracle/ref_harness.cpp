@@ -395,6 +395,36 @@ int ref_stereo_match(const olpt_camera *cams, int n_cam,
   return 0;
 }
 
+// StereoMatch::pruneMatch alone on given candidates: ids [M][n_cam] (-1 =
+// camera unused; the id vectors must be distinct), e [M], n_pts[c] 2D
+// points per camera (their positions play no part); sel[j] = 1 for the
+// selected candidates.
+int ref_prune(const olpt_camera *cams, int n_cam, const int32_t *active,
+              const int32_t *n_pts, const olpt_ipr_params *p, const int32_t *ids,
+              const double *e, int M, uint8_t *sel) {
+  CoutCapture quiet;
+  std::vector<Camera> cv = makeCameras(cams, n_cam, active);
+  TracerConfig cfg;
+  fillConfig(cfg, *p, nullptr, cv);
+  size_t tot = 0;
+  for (int c = 0; c < n_cam; ++c) tot += (size_t)n_pts[c];
+  std::vector<double> xy(2 * tot + 2, 0.0);
+  auto o2d = makeObj2d(xy.data(), n_pts, n_cam);
+  StereoMatch sm(cv, o2d, cfg);
+  std::vector<std::vector<int>> cands((size_t)M, std::vector<int>(n_cam, -1));
+  std::vector<double> es(e, e + M);
+  for (int j = 0; j < M; ++j)
+    for (int c = 0; c < n_cam; ++c) cands[j][c] = ids[(size_t)j * n_cam + c];
+  auto out = DebugAccess_StereoMatch::prune(sm, cands, es);
+  // the output lists the selected candidates in ascending index order
+  size_t k = 0;
+  for (int j = 0; j < M; ++j) {
+    sel[j] = (k < out.size() && out[k] == cands[j]) ? 1 : 0;
+    if (sel[j]) ++k;
+  }
+  return k == out.size() ? 0 : 1;
+}
+
 // Shake::runShake on given objects, then the IPR-mode residual of the kept
 // objects (IPR.cpp:178-185).
 int ref_shake(const olpt_camera *cams, int n_cam, const int32_t *active,

@@ -250,6 +250,22 @@ class Context:
                     sel=sel[:M].cpu().numpy().astype(bool), obj_xyz=ox[:no].cpu().numpy(),
                     obj_ids=oi[:no].cpu().numpy(), counts=counts)
 
+    def prune_match(self, cand_ids: np.ndarray, cand_e: np.ndarray, n_pts, params: OlptIprParams,
+                    active=None) -> np.ndarray:
+        """StereoMatch::pruneMatch on given candidates ([n][n_cam] ids, -1 =
+        unused; e_check [n]); returns the selection mask [n]."""
+        n = len(cand_e)
+        dev = torch.device("cuda", self.device)
+        ids = torch.from_numpy(np.ascontiguousarray(cand_ids, dtype=np.int32)).to(dev)
+        e = torch.from_numpy(np.ascontiguousarray(cand_e, dtype=np.float64)).to(dev)
+        sel = torch.zeros(max(n, 1), dtype=torch.uint8, device=dev)
+        npts = np.ascontiguousarray(n_pts, dtype=np.int32)
+        act = None if active is None else np.asarray(active, np.int32)
+        check(self.L.olpt_prune_match(self.h, None if act is None else act.ctypes.data_as(_lib._I32P),
+                                      npts.ctypes.data_as(_lib._I32P), ctypes.byref(params),
+                                      ptr(ids), ptr(e), n, ptr(sel)))
+        return sel[:n].cpu().numpy().astype(bool)
+
     def shake(self, imgs: torch.Tensor, dtype: int, params: OlptIprParams,
               xyz: np.ndarray, active=None, want_residual: bool = True):
         n = len(xyz)

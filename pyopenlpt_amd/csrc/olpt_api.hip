@@ -42,6 +42,14 @@ olpt_status fail(olpt_status s, const std::string &msg) {
 
 constexpr int kMaxRounds = 64;
 
+std::string prune_ties_msg(const int32_t *fl) {
+  return std::string("pruneMatch: ") +
+         (fl[2] ? "values within the eq_eps tolerance (rtol 1e-9) are not transitive here, so "
+                  "the reference's order is the one its std::stable_sort happens to produce"
+                : "the replacement branch (StereoMatch.cpp:906-968) would fire") +
+         "; this selection is not reproduced";
+}
+
 } // namespace
 
 // Per-stream workspace: everything one frame's IPR writes.  Frames of one
@@ -700,6 +708,10 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   A(b.keep, int32_t, M);
   A(b.keep_off, int32_t, M);
   A(b.tri_xyz, double, (long)M * 3);
+  b.hz_size = stereo_hz_size(M);
+  A(b.hz_key, long long, b.hz_size);
+  A(b.hz_val, unsigned long long, b.hz_size * kHzVals);
+  A(b.hz_slot, int32_t, 2L * M);
   b.cap_obj = O;
   A(b.obj_xyz, double, (long)O * 3);
   A(b.obj_ids, int32_t, (long)O * nc);
@@ -747,6 +759,7 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   // another context's old values)
   fill_i32(w.flags, 16, 0, nullptr);
   fill_i32(w.npts, OLPT_MAX_CAM, 0, nullptr);
+  stereo_init_tables(b, nullptr);
   // count arrays of the count -> scan -> fill passes: zero here, and each
   // pass's scan leaves them zero again
   fill_i32(w.bin_cnt, nbin * nc, 0, nullptr);
@@ -1039,6 +1052,7 @@ olpt_status olpt_stereo_match(olpt_ctx *c, const int32_t *active, const double *
   if (fl[1]) return fail(OLPT_E_CAPACITY, "stereo build list capacity exceeded");
   if (fl[4]) return fail(OLPT_E_CAPACITY, "stereo object capacity exceeded");
   if (fl[5]) return fail(OLPT_E_CAPACITY, "prune rounds exhausted");
+  if (fl[2] || fl[3]) return fail(OLPT_E_PRUNE_TIES, prune_ties_msg(fl));
   const int M = h[0];
   if (d_cand_ids && d_cand_e) {
     if (M > cap_cand) return fail(OLPT_E_CAPACITY, "candidate output capacity too small");
@@ -1060,6 +1074,51 @@ olpt_status olpt_stereo_match(olpt_ctx *c, const int32_t *active, const double *
                              hipMemcpyDeviceToDevice, st));
   }
   HIP_TRY(hipStreamSynchronize(st));
+  return OLPT_OK;
+}
+
+olpt_status olpt_prune_match(olpt_ctx *c, const int32_t *active, const int32_t *n_pts,
+                             const olpt_ipr_params *p, const int32_t *d_cand_ids,
+                             const double *d_cand_e, int n_cand, uint8_t *d_sel) {
+  olpt_status s = check_ready(c);
+  if (s) return s;
+  if (!n_pts || !p || (n_cand > 0 && (!d_cand_ids || !d_cand_e || !d_sel)))
+    return fail(OLPT_E_INVALID_ARGUMENT, "null argument");
+  if (n_cand < 0) return fail(OLPT_E_INVALID_ARGUMENT, "negative candidate count");
+  if (n_cand > c->cap_cand) return fail(OLPT_E_CAPACITY, "more candidates than the workspace holds");
+  for (int k = 0; k < c->n_cam; ++k)
+    if (n_pts[k] < 0 || n_pts[k] > c->cap2d)
+      return fail(OLPT_E_CAPACITY, "2D point count exceeds the workspace capacity");
+  hipStream_t st = c->st;
+  Workspace &w = c->ws[0];
+  StereoParams sp = stereo_params(c, active, p, p->tol_2d_px, &s);
+  if (s) return s;
+  const size_t nid = (size_t)n_cand * c->n_cam;
+  // ids index the flat point arrays: check them on the host
+  std::vector<int32_t> h_ids(nid);
+  if (nid) HIP_TRY(hipMemcpy(h_ids.data(), d_cand_ids, sizeof(int32_t) * nid, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < nid; ++i) {
+    const int cam = (int)(i % c->n_cam);
+    if (h_ids[i] < -1 || h_ids[i] >= n_pts[cam])
+      return fail(OLPT_E_OUT_OF_RANGE, "candidate id outside its camera's 2D points");
+  }
+  fill_i32(w.flags, 16, 0, st);
+  if (nid) {
+    HIP_TRY(hipMemcpyAsync(w.sb.cand_ids_s, d_cand_ids, sizeof(int32_t) * nid,
+                           hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(w.sb.cand_e_s, d_cand_e, sizeof(double) * (size_t)n_cand,
+                           hipMemcpyDeviceToDevice, st));
+  }
+  int32_t nc32 = n_cand;
+  HIP_TRY(hipMemcpyAsync(w.sb.n_cand, &nc32, sizeof(int32_t), hipMemcpyHostToDevice, st));
+  launch_prune(w.sb, sp, kMaxRounds, st);
+  if (n_cand) k_sel_flags<<<256, 256, 0, st>>>(w.sb.state, w.sb.n_cand, c->cap_cand, d_sel);
+  HIP_TRY(hipGetLastError());
+  int32_t fl[8];
+  HIP_TRY(hipMemcpyAsync(fl, w.flags, sizeof(fl), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (fl[5]) return fail(OLPT_E_CAPACITY, "prune rounds exhausted");
+  if (fl[2] || fl[3]) return fail(OLPT_E_PRUNE_TIES, prune_ties_msg(fl));
   return OLPT_OK;
 }
 
@@ -1190,6 +1249,7 @@ olpt_status ipr_run_impl(olpt_ctx *c, const void *d_imgs, int dtype, int n_frame
   if (fl[1]) return fail(OLPT_E_CAPACITY, "stereo build list capacity exceeded");
   if (fl[4]) return fail(OLPT_E_CAPACITY, "object capacity exceeded");
   if (fl[5]) return fail(OLPT_E_CAPACITY, "prune rounds exhausted");
+  if (fl[2] || fl[3]) return fail(OLPT_E_PRUNE_TIES, prune_ties_msg(fl));
   (void)n_loop;
   return OLPT_OK;
 }

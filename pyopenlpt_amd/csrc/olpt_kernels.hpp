@@ -72,6 +72,8 @@ struct SurvRec {
   int16_t pad;
 };
 
+constexpr int kHzVals = 4;   // distinct values kept per eq_eps check bucket
+
 struct StereoBuffers {
   int n_cam;
   int cap;               // per-camera 2D capacity
@@ -129,9 +131,16 @@ struct StereoBuffers {
   int32_t *obj_ids;      // [cap_obj][n_cam]
   int32_t *n_obj;        // device
   int32_t *n_sel;        // device
-  int32_t *flags;        // [8]: 0 cand overflow, 1 list overflow, 2 pool
-                         //      overflow, 3 replacement, 4 obj overflow,
-                         //      5 rounds exhausted, 6 triangulation throw
+  // eq_eps class check of the prune keys (k_hz_*): an open-addressing table
+  // of value buckets, each holding up to kHzVals distinct values (bits)
+  long long *hz_key;          // [hz_size] bucket key, -1 = empty
+  unsigned long long *hz_val; // [hz_size][kHzVals], all ones = empty
+  int32_t *hz_slot;           // [2 * cap_cand] slot of each inserted value
+  long hz_size;               // power of two
+  int32_t *flags;        // [8]: 0 cand overflow, 1 list overflow, 2 eq_eps
+                         //      ties not transitive, 3 replacement,
+                         //      4 obj overflow, 5 rounds exhausted,
+                         //      6 triangulation throw
   int32_t *work;         // dynamic work counter
   int32_t *scan_scratch;
   int32_t *totals;       // small scratch for scan totals
@@ -149,6 +158,19 @@ struct StereoParams {
 
 void launch_stereo(const StereoBuffers &b, const StereoParams &p,
                    int max_rounds, hipStream_t st);
+// pruneMatch alone on candidates already in cand_ids_s / cand_e_s (count
+// in n_cand): state[j] == 1 marks the selection, flags as launch_stereo.
+void launch_prune(const StereoBuffers &b, const StereoParams &p, int max_rounds,
+                  hipStream_t st);
+// Empty the eq_eps check table (once, after allocation; every stereo call
+// leaves it empty again).
+void stereo_init_tables(const StereoBuffers &b, hipStream_t st);
+// Number of slots of the eq_eps check table for a candidate capacity.
+inline long stereo_hz_size(long cap_cand) {
+  long n = 1024;
+  while (n < 4 * cap_cand) n <<= 1;
+  return n;
+}
 
 // ---------------- shake (Shake::runShake) ---------------------------------
 template <typename T> struct ShakeBuffers {

@@ -36,10 +36,12 @@
 //                 computed in rounds over the priority-sorted point lists: a
 //                 candidate is selected once every entry ahead of it on its
 //                 points is rejected and rejected once one of them is
-//                 selected.  The reference's replacement branch is checked
-//                 afterwards for every rejected candidate (flags[3] counts
-//                 cases where it would have fired; with the sort order it
-//                 cannot).
+//                 selected.  This equals the reference when its comparator
+//                 is a strict total order, which k_hz_* verify (flags[2]
+//                 otherwise), and when the replacement branch never fires,
+//                 which k_replace_check verifies for every rejected
+//                 candidate against the owners it met (flags[3] otherwise);
+//                 either flag makes the caller report an error.
 #include "olpt_kernels.hpp"
 #include "olpt_scan.hpp"
 
@@ -1323,6 +1325,162 @@ __global__ void k_replace_check(StereoBuffers b, StereoParams p) {
   }
 }
 
+// --------------------------------------------------------------------------
+// eq_eps class check.  pruneMatch orders candidates with a comparator that
+// treats values within rtol 1e-9 / atol 1e-12 as equal (StereoMatch.cpp:
+// 820-851, the percentile blocks of :742-790 likewise).  That relation is an
+// equivalence on a set of non-negative values exactly when, in sorted order,
+// every run of eq_eps-adjacent values has eq_eps end points (a run whose
+// ends are within the tolerance is a clique, and values of different runs
+// are never within it).  Then the comparator is the strict total order
+// (pct class, e class, index), every sort yields it, and the per-point
+// ranks above are its restriction.  Otherwise std::stable_sort's result
+// depends on the comparisons it happens to make, which this path does not
+// replay: flags[2] is set and the caller reports an error.
+//
+// Without a global sort: g(v) = log(v + 1e-3) grows by at most ~1e-9
+// between eq_eps values (v - u <= 1e-9 (v + 1e-3)), so buckets of width 2e-9
+// in g put every eq_eps pair in the same or adjacent buckets and never
+// relate values two buckets apart.  A bucket keeps up to kHzVals distinct
+// values; each value's thread sorts the values of its bucket and the next
+// two and checks every run in that window (a run reaching beyond it spans
+// more than two buckets, so its visible part already fails).  A fuller
+// bucket or a full table sets the flag without a claim either way.  The
+// values checked are the finite percentile scores (type 0) and their
+// candidates' e_check (type 1): the candidates the greedy considers; the
+// others sort after them whatever their order.
+// --------------------------------------------------------------------------
+constexpr double kHzWidth = 2e-9;
+constexpr long long kHzEmpty = -1;
+constexpr unsigned long long kHzNoVal = ~0ULL;   // a NaN pattern: never a value
+
+__device__ __forceinline__ long long hz_key(int type, double v) {
+  const double g = log(v + 1e-3);
+  const long long k = (long long)floor(g / kHzWidth);
+  return ((long long)type << 42) + k + (1LL << 41);
+}
+
+__device__ __forceinline__ unsigned long long hz_hash(long long key) {
+  unsigned long long h = (unsigned long long)key * 0x9E3779B97F4A7C15ULL;
+  return h ^ (h >> 29);
+}
+
+// slot of key, or -1 when absent (find) / when the table is full (insert)
+__device__ __forceinline__ long hz_find(const StereoBuffers &b, long long key) {
+  const long mask = b.hz_size - 1;
+  long s = (long)(hz_hash(key) & (unsigned long long)mask);
+  for (long t = 0; t < b.hz_size; ++t) {
+    const long long k = __atomic_load_n(&b.hz_key[s], __ATOMIC_RELAXED);
+    if (k == key) return s;
+    if (k == kHzEmpty) return -1;
+    s = (s + 1) & mask;
+  }
+  return -1;
+}
+
+__device__ __forceinline__ long hz_insert(const StereoBuffers &b, long long key) {
+  const long mask = b.hz_size - 1;
+  long s = (long)(hz_hash(key) & (unsigned long long)mask);
+  for (long t = 0; t < b.hz_size; ++t) {
+    const long long k = atomicCAS((unsigned long long *)&b.hz_key[s],
+                                  (unsigned long long)kHzEmpty, (unsigned long long)key);
+    if (k == kHzEmpty || k == key) return s;
+    s = (s + 1) & mask;
+  }
+  return -1;
+}
+
+__device__ __forceinline__ double hz_val(unsigned long long bits) {
+  return __longlong_as_double((long long)bits);
+}
+
+__global__ void k_hz_insert(StereoBuffers b) {
+  const int M = min(*b.n_cand, b.cap_cand);
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
+    const double pv[2] = {b.pct[j], b.cand_e_s[j]};
+    for (int t = 0; t < 2; ++t) {
+      b.hz_slot[2 * j + t] = -1;
+      if (!isfinite(b.pct[j]) || !isfinite(pv[t])) continue;
+      const double v = pv[t] + 0.0;  // -0 -> +0
+      if (v < 0.0) {
+        atomicOr(&b.flags[2], 1);
+        continue;
+      }
+      const long s = hz_insert(b, hz_key(t, v));
+      if (s < 0) {
+        atomicOr(&b.flags[2], 1);   // table full
+        continue;
+      }
+      const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+      bool placed = false;
+      for (int u = 0; u < kHzVals && !placed; ++u) {
+        const unsigned long long o = atomicCAS(&b.hz_val[s * kHzVals + u], kHzNoVal, bits);
+        placed = o == kHzNoVal || o == bits;
+      }
+      if (!placed) atomicOr(&b.flags[2], 1);   // bucket full
+      b.hz_slot[2 * j + t] = (int32_t)s;
+    }
+  }
+}
+
+__global__ void k_hz_check(StereoBuffers b) {
+  const int M = min(*b.n_cand, b.cap_cand);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * M; i += gridDim.x * blockDim.x) {
+    const long s = b.hz_slot[i];
+    if (s < 0) continue;
+    const long long key = b.hz_key[s];
+    // the distinct values of buckets key, key + 1, key + 2, sorted
+    double w[3 * kHzVals];
+    int n = 0;
+    for (int d = 0; d < 3; ++d) {
+      const long sd = d == 0 ? s : hz_find(b, key + d);
+      if (sd < 0) continue;
+      for (int u = 0; u < kHzVals; ++u) {
+        const unsigned long long bits = b.hz_val[sd * kHzVals + u];
+        if (bits == kHzNoVal) continue;
+        const double v = hz_val(bits);
+        int k = n++;
+        while (k > 0 && w[k - 1] > v) {
+          w[k] = w[k - 1];
+          --k;
+        }
+        w[k] = v;
+      }
+    }
+    bool bad = false;
+    int r0 = 0;
+    for (int k = 1; k <= n; ++k)
+      if (k == n || !eq_eps(w[k - 1], w[k])) {
+        bad = bad || !eq_eps(w[r0], w[k - 1]);
+        r0 = k;
+      }
+    if (bad) atomicOr(&b.flags[2], 1);
+  }
+}
+
+// empty the slots used by this call (every used slot holds an inserted value)
+__global__ void k_hz_clear(StereoBuffers b) {
+  const int M = min(*b.n_cand, b.cap_cand);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * M; i += gridDim.x * blockDim.x) {
+    const long s = b.hz_slot[i];
+    if (s < 0) continue;
+    b.hz_key[s] = kHzEmpty;
+    for (int u = 0; u < kHzVals; ++u) b.hz_val[s * kHzVals + u] = kHzNoVal;
+  }
+}
+
+__global__ void k_hz_init(StereoBuffers b) {
+  for (long s = blockIdx.x * (long)blockDim.x + threadIdx.x; s < b.hz_size;
+       s += (long)gridDim.x * blockDim.x) {
+    b.hz_key[s] = kHzEmpty;
+    for (int u = 0; u < kHzVals; ++u) b.hz_val[s * kHzVals + u] = kHzNoVal;
+  }
+}
+
+void stereo_init_tables(const StereoBuffers &b, hipStream_t st) {
+  k_hz_init<<<1024, 256, 0, st>>>(b);
+}
+
 // triangulateMatch: camera-order LOS, FP64, gate on tol_3d.  A candidate's
 // cameras are gathered into slots 0..n-1 in camera order by selects over
 // compile-time slot and camera indices (MC >= n_cam), so the LOS arrays stay
@@ -1421,6 +1579,37 @@ static void build_level(const StereoBuffers &b, const StereoParams &p, hipStream
   k_b_extend<K><<<b.grid_build, 256, 0, st>>>(b, p);
 }
 
+// pruneMatch on the sorted candidates (cand_ids_s, cand_e_s, n_cand):
+// per-point percentile and priority orders, the eq_eps check, the greedy
+// rounds and the replacement check; state[j] = 1 marks the selection.
+static void prune_section(const StereoBuffers &b, const StereoParams &p, int gM,
+                          hipStream_t st) {
+  const int T = 256;
+  const long nflat = (long)b.n_cam * b.cap;
+  zero_i32(b.pt_count, nflat, st);
+  zero_i32(b.pt_flag, nflat, st);
+  zero_i32(b.head, nflat, st);
+  k_flat<<<gM, T, 0, st>>>(b, p);
+  scan_exclusive(b.pt_count, b.pt_start, nflat, 1, nflat, b.n_entries, b.scan_scratch, st);
+  k_pt_fill<<<gM, T, 0, st>>>(b, p);
+  const int gE = grid_for((long)b.cap_cand * 4, T, 4096);
+  k_zero_entries<<<gE, T, 0, st>>>(b);
+  k_pct_rank<<<gE, T, 0, st>>>(b);
+  k_pct_fallback<<<grid_for(nflat, T, 2048), T, 0, st>>>(b);
+  k_pct_final<<<gM, T, 0, st>>>(b, p);
+  k_hz_insert<<<gM, T, 0, st>>>(b);
+  k_hz_check<<<gM, T, 0, st>>>(b);
+  k_hz_clear<<<gM, T, 0, st>>>(b);
+  k_prio_rank<<<gE, T, 0, st>>>(b);
+  k_prio_fallback<<<grid_for(nflat, T, 2048), T, 0, st>>>(b);
+  // the first rounds decide almost every candidate; one workgroup finishes
+  // the few left
+  for (int r = 0; r < kMisFull; ++r) k_mis_round<<<gM, T, 0, st>>>(b, p, r);
+  k_mis_compact<<<gM, T, 0, st>>>(b);
+  k_mis_finish<<<1, kMisFinishThreads, 0, st>>>(b, p);
+  k_replace_check<<<gM, T, 0, st>>>(b, p);
+}
+
 void launch_stereo(const StereoBuffers &b, const StereoParams &p,
                    int max_rounds, hipStream_t st) {
   const long ncell = (long)b.rows_cell * b.cols_cell;
@@ -1462,27 +1651,7 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   scan_exclusive(b.key_cnt, b.key_start, b.cap, 1, b.cap, nullptr, b.scan_scratch, st);
   k_key_scatter<<<gM, T, 0, st>>>(b, p);
   k_bucket_rank<<<gM, T, 0, st>>>(b, p);
-  // prune
-  const long nflat = (long)b.n_cam * b.cap;
-  zero_i32(b.pt_count, nflat, st);
-  zero_i32(b.pt_flag, nflat, st);
-  zero_i32(b.head, nflat, st);
-  k_flat<<<gM, T, 0, st>>>(b, p);
-  scan_exclusive(b.pt_count, b.pt_start, nflat, 1, nflat, b.n_entries, b.scan_scratch, st);
-  k_pt_fill<<<gM, T, 0, st>>>(b, p);
-  const int gE = grid_for((long)b.cap_cand * 4, T, 4096);
-  k_zero_entries<<<gE, T, 0, st>>>(b);
-  k_pct_rank<<<gE, T, 0, st>>>(b);
-  k_pct_fallback<<<grid_for(nflat, T, 2048), T, 0, st>>>(b);
-  k_pct_final<<<gM, T, 0, st>>>(b, p);
-  k_prio_rank<<<gE, T, 0, st>>>(b);
-  k_prio_fallback<<<grid_for(nflat, T, 2048), T, 0, st>>>(b);
-  // the first rounds decide almost every candidate; one workgroup finishes
-  // the few left
-  for (int r = 0; r < kMisFull; ++r) k_mis_round<<<gM, T, 0, st>>>(b, p, r);
-  k_mis_compact<<<gM, T, 0, st>>>(b);
-  k_mis_finish<<<1, kMisFinishThreads, 0, st>>>(b, p);
-  k_replace_check<<<gM, T, 0, st>>>(b, p);
+  prune_section(b, p, gM, st);
   // triangulate + compact in candidate order
   if (b.n_cam <= 4)
     k_tri<4><<<gM, T, 0, st>>>(b, p);
@@ -1493,6 +1662,12 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   k_obj_write<<<gM, T, 0, st>>>(b);
   k_obj_count<<<1, 64, 0, st>>>(b);
   stage_end(ST_PRUNE, st);
+}
+
+void launch_prune(const StereoBuffers &b, const StereoParams &p, int max_rounds,
+                  hipStream_t st) {
+  zero_i32(b.undecided, max_rounds, st);
+  prune_section(b, p, grid_for(b.cap_cand, 256, b.grid_prune), st);
 }
 
 } // namespace olpt

@@ -435,3 +435,22 @@ def render_sequence(cams, seq, amp_range=(105.0, 145.0), b=1.5, c=1.5, noise=2.0
     amp = rng.uniform(amp_range[0], amp_range[1], max_id)
     return np.stack([render(cams, p, amp[ids], b, c, 0.0, noise, seed + 1 + f, dtype)
                      for f, (p, ids) in enumerate(seq)])
+
+
+# ---------------------------------------------------------------------------
+# Quality against ground truth
+# ---------------------------------------------------------------------------
+def ghost_eval(xyz: np.ndarray, truth: np.ndarray, tol_mm: float) -> dict:
+    """Reconstruction quality with find_ghosts semantics
+    (demo/openLPT_utils/ghost_eval.py:72-92): each reconstruction's nearest
+    ground-truth particle; within tol_mm (inclusive) it is real, otherwise a
+    ghost.  recon_rate = distinct particles matched / N_truth, ghost_rate =
+    ghosts / N_rec, rmse_mm over the real reconstructions."""
+    from scipy.spatial import cKDTree
+    if len(xyz) == 0:
+        return dict(rmse_mm=None, recon_rate=0.0, ghost_rate=None, n_rec=0)
+    d, i = cKDTree(truth).query(np.asarray(xyz, np.float64))
+    real = d <= tol_mm
+    return dict(rmse_mm=float(np.sqrt(np.mean(d[real] ** 2))) if real.any() else None,
+                recon_rate=float(len(np.unique(i[real])) / len(truth)),
+                ghost_rate=float(1.0 - real.mean()), n_rec=int(len(xyz)))

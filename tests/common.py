@@ -148,3 +148,33 @@ def oracle_ipr(cams, images, params, abca):
                        256, O.iptr(lt), O.iptr(cnt))
     assert rc == 0
     return ox[: cnt[0]], ou[: cnt[0]], lt[: cnt[1]]
+
+
+# --------------------------------------------------------------------------
+# FP32-mode position comparison
+# --------------------------------------------------------------------------
+def f32_position_stats(got: np.ndarray, got_loops, want: np.ndarray, want_loops,
+                       tol_mm: float = 1e-4, near_mm: float = 0.0) -> dict:
+    """Object-by-object comparison of an FP32-mode IPR output with the FP64
+    reference.  Loop 1's objects come from the same (FP64) stereo candidates in
+    both, so when loop 1 kept the same number of objects they correspond by
+    index: `loop1_share` is the share beyond tol_mm, `loop1_q99` the 99th
+    percentile and `loop1_max` the largest deviation.  Later loops run on
+    residual images that differ in the last bits, so their objects are
+    compared by nearest neighbour: `nn_share` is the share of all output
+    objects with no reference object within tol_mm, `far_share` the share
+    with none within near_mm."""
+    from scipy.spatial import cKDTree
+    out = dict(n_got=len(got), n_want=len(want))
+    n1g, n1w = int(got_loops[0]), int(want_loops[0])
+    out["loop1_equal"] = n1g == n1w
+    if n1g == n1w and n1g:
+        d = np.abs(got[:n1g] - want[:n1w]).max(axis=1)
+        out["loop1_share"] = float((d > tol_mm).mean())
+        out["loop1_q99"] = float(np.quantile(d, 0.99))
+        out["loop1_max"] = float(d.max())
+    if len(got) and len(want):
+        d, _ = cKDTree(want).query(got)
+        out["nn_share"] = float((d > tol_mm).mean())
+        out["far_share"] = float((d > near_mm).mean())
+    return out

@@ -1,9 +1,12 @@
 """Pins the CPU oracle against golden vectors written by the reference's own
 C++ IPR path (tests/golden/make_golden.py) - stage by stage, no GPU."""
+import os
+
 import numpy as np
 import pytest
 
-from common import (GOLDEN_CASES, Case, oracle_ipr, oracle_peaks, oracle_shake,
+import oracle as O
+from common import (GOLDEN, GOLDEN_CASES, Case, oracle_ipr, oracle_peaks, oracle_shake,
                     oracle_stereo)
 
 
@@ -57,3 +60,19 @@ def test_run_ipr_bit_exact(case):
     assert np.array_equal(loops, case.z["ipr_loops"])
     assert np.array_equal(xyz, case.z["ipr_xyz"])
     assert np.array_equal(uv, case.z["ipr_uv"])
+
+
+@pytest.mark.parametrize("name", ["random", "dense"])
+def test_oracle_prune_matches_reference(name):
+    """pruneMatch alone on hand-built candidate sets (exact and one-ulp e_check
+    ties, shared points): the oracle's selection equals the compiled
+    reference's (tests/golden/make_golden_prune.py)."""
+    z = np.load(os.path.join(GOLDEN, "prune_cases.npz"))
+    ids = np.ascontiguousarray(z[f"{name}_ids"])
+    e = np.ascontiguousarray(z[f"{name}_e"])
+    npts = np.ascontiguousarray(z[f"{name}_npts"])
+    sel = np.zeros(len(e), np.uint8)
+    rc = O.load_port().orc_prune(ids.shape[1], None, O.iptr(npts), O.iptr(ids), O.dptr(e),
+                                 len(e), O.u8ptr(sel))
+    assert rc == 0
+    assert np.array_equal(sel, z[f"{name}_sel"])
