@@ -1781,6 +1781,53 @@ static int next_combination(int *c, int K, int N) {
   return 1;
 }
 
+/* limit2DObjectNumber (IPR.cpp:19-39): each active camera's list longer
+ * than cap is shuffled with one std::default_random_engine(1234) shared
+ * across the cameras (in camera order) and cut to cap.  The engine and
+ * std::shuffle are restated from GCC 11's libstdc++, the reference's
+ * toolchain: minstd_rand0 (x <- 16807 x mod 2^31-1), uniform_int_
+ * distribution by downscaling with two divisions, and shuffle's two-
+ * positions-per-draw path while n^2 fits the engine's range. */
+typedef struct { uint64_t x; } lcg0;
+
+static uint64_t lcg0_next(lcg0 *g) {
+  g->x = (g->x * 16807u) % 2147483647u;
+  return g->x;
+}
+
+static uint64_t lcg0_upto(lcg0 *g, uint64_t b) {
+  const uint64_t R = 2147483645u;
+  if (b == R) return lcg0_next(g) - 1;
+  const uint64_t er = b + 1, sc = R / er, past = er * sc;
+  uint64_t r;
+  do r = lcg0_next(g) - 1; while (r >= past);
+  return r / sc;
+}
+
+static void swap_pt(double *xy, int64_t i, int64_t j) {
+  const double x = xy[2 * i], y = xy[2 * i + 1];
+  xy[2 * i] = xy[2 * j];
+  xy[2 * i + 1] = xy[2 * j + 1];
+  xy[2 * j] = x;
+  xy[2 * j + 1] = y;
+}
+
+static void shuffle_pts(double *xy, int64_t n, lcg0 *g) {
+  const uint64_t R = 2147483645u;
+  if (n <= 0) return;
+  if (R / (uint64_t)n >= (uint64_t)n) {
+    int64_t i = 1;
+    if (n % 2 == 0) swap_pt(xy, i++, (int64_t)lcg0_upto(g, 1));
+    for (; i != n; i += 2) {
+      const uint64_t m = (uint64_t)i + 2, x = lcg0_upto(g, (m - 1) * m - 1);
+      swap_pt(xy, i, (int64_t)(x / m));
+      swap_pt(xy, i + 1, (int64_t)(x % m));
+    }
+    return;
+  }
+  for (int64_t i = 1; i < n; ++i) swap_pt(xy, i, (int64_t)lcg0_upto(g, (uint64_t)i));
+}
+
 /* runSingleIPRIteration (IPR.cpp:70-193).  images (n_cam*H*W) is replaced
  * by the residual; kept objects are appended to (xyz, uv). */
 static int ipr_iteration(const olpt_camera *cams, int n_cam,
@@ -1797,10 +1844,14 @@ static int ipr_iteration(const olpt_camera *cams, int n_cam,
     rc = find_peaks(images + HW * c, H, W, (double)p->min_obj_int, &pk[c]);
     if (rc) return rc;
     npts[c] = pk[c].n / 2;
-    /* limit2DObjectNumber (IPR.cpp:19-39) uses libstdc++'s shuffle; the
-     * oracle supports only lists within the cap. */
-    if (p->n_obj2d_process_max > 0 && npts[c] > p->n_obj2d_process_max)
-      return ERR_UNSUP;
+  }
+  if (p->n_obj2d_process_max > 0) {
+    lcg0 g = {1234};
+    for (int c = 0; c < n_cam; ++c)
+      if (active[c] && npts[c] > p->n_obj2d_process_max) {
+        shuffle_pts(pk[c].v, npts[c], &g);
+        npts[c] = p->n_obj2d_process_max;
+      }
   }
   int tot = 0;
   for (int c = 0; c < n_cam; ++c) tot += npts[c];

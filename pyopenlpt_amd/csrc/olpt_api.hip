@@ -67,6 +67,9 @@ struct Workspace {
   StereoBuffers sb{};
   double *xy = nullptr;
   int32_t *npts = nullptr;
+  // limit2DObjectNumber work buffers
+  int32_t *lim_perm = nullptr, *lim_flag = nullptr;
+  double *lim_xy = nullptr;
   // shake
   void *sh_uv = nullptr, *keep_uv = nullptr;
   double *sh_xyz = nullptr, *keep_xyz = nullptr, *sh_score = nullptr;
@@ -391,10 +394,11 @@ StereoParams stereo_params(olpt_ctx *c, const int32_t *active, const olpt_ipr_pa
   sp.build_count = bc;
   if (bc != sp.n_active) {
     // with check cameras the reference's checkMatch leaves e_check NaN
-    // (StereoMatch.cpp:522, :603-604) and match() aborts (:126-127)
-    *st = fail(OLPT_E_UNSUPPORTED,
-               "match_cam_count below the number of active cameras is undefined in the reference "
-               "(checkMatch leaves e_check NaN and match() aborts)");
+    // (StereoMatch.cpp:522, :603-604) and match() raises InvalidArgument on
+    // the first candidate that passes it (REQUIRE, :126-127)
+    *st = fail(OLPT_E_INVALID_ARGUMENT,
+               "StereoMatch::match: checkMatch returned NaN e_check. (match_cam_count below the "
+               "number of active cameras)");
   }
   return sp;
 }
@@ -410,6 +414,11 @@ ShakeParams shake_params(olpt_ctx *c, const int32_t *active, const olpt_ipr_para
   sp.r_px = p->radius_obj;
   sp.n_shake_loop = p->n_shake_loop;
   sp.ratio_augimg = p->ratio_augimg;
+  {
+    const double d = std::max(0.0, p->radius_obj);   // calROISize, Shake.cpp:294-296
+    sp.box_w = box_width(d);
+    sp.roi_w = box_width(d * (double)p->ratio_augimg);
+  }
   sp.mlp = c->mlp;
   sp.voxel_mm = c->mlp_voxel_mm;
   sp.max_step_vox = c->mlp_max_step_vox;
@@ -427,8 +436,8 @@ olpt_status check_params(olpt_ctx *c, const olpt_ipr_params *p) {
   if (!p) return fail(OLPT_E_INVALID_ARGUMENT, "null params");
   if (p->radius_obj <= 0 || p->ratio_augimg <= 0)
     return fail(OLPT_E_INVALID_ARGUMENT, "radius_obj and ratio_augimg must be positive");
-  const int box = (int)std::lround(2.0 * p->radius_obj);
-  const int roi = (int)std::floor(2.0 * p->radius_obj * p->ratio_augimg + 0.5 + 1e-3);
+  const int box = box_width(p->radius_obj);
+  const int roi = box_width(p->radius_obj * (double)p->ratio_augimg);
   if (roi * roi > 4 * 64 || box > 24)
     return fail(OLPT_E_UNSUPPORTED, "ROI larger than 16x16 pixels is not supported");
   if (c->mlp && roi > 8)
@@ -458,6 +467,10 @@ olpt_status ipr_loop(olpt_ctx *c, Workspace &w, T *img, T *img_next, const int32
                        w.pk_off, w.scan_scratch, w.totals, w.xy, c->cap2d, w.npts,
                        w.flags + 8, st);
   k_peak_overflow<<<1, 1, 0, st>>>(w.flags + 8, w.flags);
+  // 1.1) limit2DObjectNumber (IPR.cpp:105-107)
+  if (p->n_obj2d_process_max > 0 && p->n_obj2d_process_max < c->cap2d)
+    launch_limit2d(w.xy, w.npts, cl, p->n_obj2d_process_max, c->cap2d, w.lim_perm, w.lim_flag,
+                   w.lim_xy, st);
   // 2) StereoMatch
   launch_stereo(w.sb, sp, kMaxRounds, st);
   // 3) Shake the matched objects in place
@@ -656,6 +669,9 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   A(w.pk_off, int32_t, nchunk * nc);
   A(w.xy, double, (long)nc * cap * 2);
   A(w.npts, int32_t, OLPT_MAX_CAM);
+  A(w.lim_perm, int32_t, (long)nc * cap);
+  A(w.lim_flag, int32_t, OLPT_MAX_CAM);
+  A(w.lim_xy, double, (long)nc * cap * 2);
   StereoBuffers &b = w.sb;
   b = StereoBuffers{};
   b.n_cam = nc;
@@ -826,7 +842,11 @@ olpt_status olpt_ctx_reserve(olpt_ctx *c, int n_cam, int H, int W, int max_2d, i
   c->n_cam = n_cam;
   c->H = H;
   c->W = W;
-  c->cap2d = max_2d > 0 ? max_2d : std::min(1 << 17, std::max(1024, H * W / 8));
+  // default: every possible local maximum of an image up to 724^2 pixels
+  // (at most one in two pixels), and 262,144 above that - more than the
+  // reference's default n_obj2d_process_max (200,000), so its
+  // limit2DObjectNumber subsampling runs as the reference's does
+  c->cap2d = max_2d > 0 ? max_2d : std::min(1 << 18, std::max(1024, H * W / 2));
   c->cap_cand = max_cand > 0 ? max_cand : (1 << 21);
   c->cap_obj = max_obj > 0 ? max_obj : (1 << 18);
   c->bin_rows = (H + 7) / 8;

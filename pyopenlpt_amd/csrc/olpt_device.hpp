@@ -9,6 +9,8 @@
 // reference's rounding (the reference is built without FMA contraction).
 #pragma once
 
+#include <cmath>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -377,14 +379,15 @@ struct PRange {
   __device__ bool empty() const { return rmax < rmin || cmax < cmin; }
 };
 
+// The box size is decided on the host, in the reference's double
+// arithmetic: width = round(2 dx), height = round(2 dy) (box_width below);
+// a negative width or height stands for dx <= 0 or dy <= 0 (empty).
 template <typename T>
-__device__ __forceinline__ PRange roi_bound(int H, int W, T cx, T cy, T dx, T dy) {
+__device__ __forceinline__ PRange roi_bound(int H, int W, T cx, T cy, int width, int height) {
   PRange e{1, 0, 1, 0};
-  if (H <= 0 || W <= 0 || dx <= T(0) || dy <= T(0) || cx <= T(0) || cy <= T(0) ||
+  if (H <= 0 || W <= 0 || width < 0 || height < 0 || cx <= T(0) || cy <= T(0) ||
       cx >= T(W) || cy >= T(H))
     return e;
-  const int width = (int)round(T(2) * dx);
-  const int height = (int)round(T(2) * dy);
   int cmin = (int)round(cx - T(width - 1) / T(2));
   int rmin = (int)round(cy - T(height - 1) / T(2));
   int cmax = cmin + width, rmax = rmin + height;
@@ -395,6 +398,10 @@ __device__ __forceinline__ PRange roi_bound(int H, int W, T cx, T cy, T dx, T dy
   if (rmax < rmin || cmax < cmin) return e;
   return PRange{rmin, rmax, cmin, cmax};
 }
+
+// calROIBound's box size for half-size d (Shake.cpp:251-257): round(2 d)
+// with C++ std::round (half away from zero), -1 when d <= 0.
+inline int box_width(double d) { return d > 0.0 ? (int)std::round(2.0 * d) : -1; }
 
 // --------------------------------------------------------------------------
 // Wavefront helpers (wave64).

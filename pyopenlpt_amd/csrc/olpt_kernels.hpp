@@ -56,6 +56,12 @@ void launch_find_peaks(const TIn *img, long cam_stride, int H, int W,
                        int32_t *totals, double *xy, int cap, int32_t *npts,
                        int32_t *overflow, hipStream_t st);
 
+// limit2DObjectNumber (IPR.cpp:19-39) on the peak lists of `cams` (xy and
+// npts as launch_find_peaks writes them; a no-op unless 0 < cap < cap2d).
+// Work buffers: perm [n_cam][cap2d] int32, lim [n_cam], tmp [n_cam][cap2d][2].
+void launch_limit2d(double *xy, int32_t *npts, CamList cams, int cap, int cap2d,
+                    int32_t *perm, int32_t *lim, double *tmp, hipStream_t st);
+
 inline long peak_chunks(int H, int W) {
   return (long)(H > 2 ? H - 2 : 0) * ((W - 2 + 63) / 64);
 }
@@ -214,6 +220,9 @@ struct ShakeParams {
   const void *mlp;            // learned proposal network (olpt_mlp), or null
   double voxel_mm, max_step_vox;
   int n_shake_loop, ratio_augimg;
+  // calROIBound box edges (box_width): the object box (r_px) and the shake
+  // ROI (r_px * ratio_augimg), fixed on the host in double arithmetic
+  int box_w, roi_w;
 };
 
 template <typename T>

@@ -12,8 +12,11 @@
 // writes the peaks in the reference's row-major scan order with no atomics,
 // fitting the separable three-point log-parabola in FP64 (the reference's
 // absolute-coordinate formula, :67-97).
+#include <algorithm>
+
 #include "olpt_kernels.hpp"
 #include "olpt_scan.hpp"
+#include "olpt_shuffle.hpp"
 
 namespace olpt {
 
@@ -247,6 +250,62 @@ void launch_find_peaks(const TIn *img, long cam_stride, int H, int W,
                                                      mask, off, nchunk, xy, cap);
   k_peak_finish<<<1, 64, 0, st>>>(cams, totals, cap, npts, overflow);
   stage_end(ST_PEAKS, st);
+}
+
+// limit2DObjectNumber (IPR.cpp:19-39): the listed cameras' peak lists
+// longer than cap are shuffled with one engine seeded 1234, shared across
+// the cameras in order, and cut to cap.  The draws are sequential, so one
+// thread computes each camera's permutation (it only runs when a list is
+// over the cap); a grid gathers the kept points.
+__global__ void k_limit_perm(const int32_t *npts, CamList cams, int cap, int cap2d,
+                             int32_t *perm, int32_t *lim) {
+  if (threadIdx.x || blockIdx.x) return;
+  Minstd0 g{1234};
+  for (int q = 0; q < cams.n; ++q) {
+    const int k = cams.id[q];
+    const int n = npts[k];
+    lim[k] = 0;
+    if (n <= cap) continue;
+    int32_t *a = perm + (long)k * cap2d;
+    for (int i = 0; i < n; ++i) a[i] = i;
+    shuffle_like_libstdcxx(a, (int64_t)n, g);
+    lim[k] = 1;
+  }
+}
+
+__global__ void k_limit_gather(const double *__restrict__ xy, const int32_t *__restrict__ perm,
+                               const int32_t *__restrict__ lim, CamList cams, int cap, int cap2d,
+                               double *__restrict__ tmp) {
+  const int k = cams.id[blockIdx.y];
+  if (!lim[k]) return;
+  const long o = (long)k * cap2d;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) {
+    const long src = o + perm[o + i];
+    tmp[2 * (o + i)] = xy[2 * src];
+    tmp[2 * (o + i) + 1] = xy[2 * src + 1];
+  }
+}
+
+__global__ void k_limit_store(double *__restrict__ xy, int32_t *__restrict__ npts,
+                              const int32_t *__restrict__ lim, CamList cams, int cap, int cap2d,
+                              const double *__restrict__ tmp) {
+  const int k = cams.id[blockIdx.y];
+  if (!lim[k]) return;
+  const long o = (long)k * cap2d;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) {
+    xy[2 * (o + i)] = tmp[2 * (o + i)];
+    xy[2 * (o + i) + 1] = tmp[2 * (o + i) + 1];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) npts[k] = cap;
+}
+
+void launch_limit2d(double *xy, int32_t *npts, CamList cams, int cap, int cap2d,
+                    int32_t *perm, int32_t *lim, double *tmp, hipStream_t st) {
+  if (cams.n <= 0 || cap <= 0 || cap >= cap2d) return;
+  k_limit_perm<<<1, 64, 0, st>>>(npts, cams, cap, cap2d, perm, lim);
+  dim3 g((unsigned)std::min(1024, (cap + 255) / 256), (unsigned)cams.n);
+  k_limit_gather<<<g, 256, 0, st>>>(xy, perm, lim, cams, cap, cap2d, tmp);
+  k_limit_store<<<g, 256, 0, st>>>(xy, npts, lim, cams, cap, cap2d, tmp);
 }
 
 #define INST(T)                                                                \

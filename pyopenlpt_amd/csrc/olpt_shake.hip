@@ -54,7 +54,7 @@ template <typename T> struct ObjCamRec {
 template <typename T, bool BIN>
 __global__ void k_obj_project(ShakeBuffers<T> b, const double *__restrict__ xyz,
                               const int32_t *__restrict__ n_ptr, ObjCamRec<T> *rec,
-                              T *uv_out, T r_px) {
+                              T *uv_out, int box_w) {
   const int n = *n_ptr;
   const int nc = b.n_cam;
   const long nbin = (long)b.bin_rows * b.bin_cols;
@@ -64,7 +64,7 @@ __global__ void k_obj_project(ShakeBuffers<T> b, const double *__restrict__ xyz,
       ObjCamRec<T> r;
       project(b.cams[k], X, r.u, r.v);
       otf_param(b.otf, k, X, r.P);
-      const PRange pr = roi_bound<T>(b.H, b.W, r.u, r.v, r_px, r_px);
+      const PRange pr = roi_bound<T>(b.H, b.W, r.u, r.v, box_w, box_w);
       r.rmin = pr.rmin;
       r.rmax = pr.rmax;
       r.cmin = pr.cmin;
@@ -231,17 +231,17 @@ template <typename T>
 static void residual_pass(const ShakeBuffers<T> &b, const T *orig, T *out,
                           const double *xyz, const int32_t *n_ptr,
                           ObjCamRec<T> *rec, T *uv_out, CamList cams, int mode,
-                          T r_px, hipStream_t st, const double *amp = nullptr) {
+                          double r_px, hipStream_t st, const double *amp = nullptr) {
   const int nc = b.n_cam;
   const long nbin = (long)b.bin_rows * b.bin_cols;
+  const int box_w = box_width(r_px);
   k_obj_project<T, true><<<grid_for(b.cap_obj, 256, 2048), 256, 0, st>>>(b, xyz, n_ptr, rec,
-                                                                         uv_out, r_px);
+                                                                         uv_out, box_w);
   if (amp) k_set_amp<T><<<grid_for((long)b.cap_obj * nc, 256, 2048), 256, 0, st>>>(rec, amp, n_ptr, nc);
   // one scan over all cameras' bins; bin_start[nc * nbin] receives the total
   scan_exclusive_clear(b.bin_cnt, b.bin_start, nbin * nc, b.bin_start + nbin * nc,
                        b.scan_scratch, st);
   k_bin_fill<T><<<grid_for((long)b.cap_obj * nc, 256, 4096), 256, 0, st>>>(b, n_ptr);
-  const int box_w = (int)lround(2.0 * (double)r_px);
   dim3 g((b.W + kResTile - 1) / kResTile, (b.H + kResTile - 1) / kResTile, cams.n);
   stage_begin(ST_RESID, st);
   k_resid_gather<T><<<g, kResThreads, 0, st>>>(b, orig, out, rec, cams, mode, box_w);
@@ -478,8 +478,7 @@ __device__ __forceinline__ void build_roi(const ShakeBuffers<T> &b, const ShakeP
   for (int sl = 0; sl < SLOTS; ++sl) c.aug[sl] = T(0);
   if (!c.act) return;
   const ObjCamRec<T> &r = rec[(long)i * b.n_cam + c.k];
-  const PRange roi = roi_bound<T>(b.H, b.W, r.u, r.v, dpos * (T)p.ratio_augimg,
-                                  dpos * (T)p.ratio_augimg);
+  const PRange roi = roi_bound<T>(b.H, b.W, r.u, r.v, p.roi_w, p.roi_w);
   c.roi = roi;
   if (roi.empty()) return;
   c.roiw = roi.cmax - roi.cmin;
@@ -672,7 +671,7 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
     bool usek = false;
     PRange rect{1, 0, 1, 0};
     if (c.act && dpos > T(0)) {
-      rect = roi_bound<T>(b.H, b.W, uf, vf, dpos, dpos);
+      rect = roi_bound<T>(b.H, b.W, uf, vf, p.box_w, p.box_w);
       usek = !rect.empty();
     }
     const int rw = usek ? rect.cmax - rect.cmin : 0;
@@ -1034,11 +1033,9 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
   for (int k = 0; k < nc; ++k)
     if (!p.active[k]) inact.id[inact.n++] = k;
   ObjCamRec<T> *rec = reinterpret_cast<ObjCamRec<T> *>(b.rec);
-  const T r_px = (T)p.r_px;
-  // ROI edge round(2 r ratio); the margin covers FP32 rounding of the
-  // device-side product at a .5 boundary, so the slots always hold the ROI
-  const int roi_w = (int)std::floor(2.0 * p.r_px * p.ratio_augimg + 0.5 + 1e-3);
-  const int roi_px = roi_w * roi_w;
+  const double r_px = p.r_px;
+  // the ROI edge, as every kernel uses it (calROIBound, decided on the host)
+  const int roi_px = p.roi_w > 0 ? p.roi_w * p.roi_w : 0;
   const int gO = grid_for(b.cap_obj, 256, 2048);
   k_init_score<T><<<gO, 256, 0, st>>>(b);
   if (p.mlp && b.patches && b.prop && nc == 4 && roi_px <= 64) {
@@ -1071,7 +1068,7 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
     if (delta < dmin) delta = dmin;
   }
   if (p.n_shake_loop <= 0) {
-    k_obj_project<T, false><<<gO, 256, 0, st>>>(b, b.xyz, b.n_obj, rec, b.uv, r_px);
+    k_obj_project<T, false><<<gO, 256, 0, st>>>(b, b.xyz, b.n_obj, rec, b.uv, p.box_w);
   }
   // markRepeatedObj
   const double t2 = p.tol3d * p.tol3d;
@@ -1139,8 +1136,7 @@ void launch_residual_ipr(const ShakeBuffers<T> &b, const int32_t *n_keep,
   CamList all{0, {0}};
   for (int k = 0; k < b.n_cam; ++k) all.id[all.n++] = k;
   ObjCamRec<T> *rec = reinterpret_cast<ObjCamRec<T> *>(b.rec);
-  residual_pass<T>(b, b.img, b.img_next, xyz_keep, n_keep, rec, uv_keep, all, 1,
-                   (T)r_px, st);
+  residual_pass<T>(b, b.img, b.img_next, xyz_keep, n_keep, rec, uv_keep, all, 1, r_px, st);
 }
 
 template <typename T>
@@ -1149,7 +1145,7 @@ void launch_patches(const ShakeBuffers<T> &b, const ShakeParams &p, void *X, hip
   for (int k = 0; k < b.n_cam; ++k)
     if (p.active[k]) act.id[act.n++] = k;
   ObjCamRec<T> *rec = reinterpret_cast<ObjCamRec<T> *>(b.rec);
-  residual_pass<T>(b, b.img, b.res, b.xyz, b.n_obj, rec, b.uv, act, 0, (T)p.r_px, st);
+  residual_pass<T>(b, b.img, b.res, b.xyz, b.n_obj, rec, b.uv, act, 0, p.r_px, st);
   k_patch_gather<T><<<grid_for((long)b.cap_obj * kPatchIn, 256, 8192), 256, 0, st>>>(
       b, p, rec, (__bf16 *)X);
 }
@@ -1160,7 +1156,7 @@ void launch_render(const ShakeBuffers<T> &b, const double *xyz, const double *am
   CamList all{0, {0}};
   for (int k = 0; k < b.n_cam; ++k) all.id[all.n++] = k;
   ObjCamRec<T> *rec = reinterpret_cast<ObjCamRec<T> *>(b.rec);
-  residual_pass<T>(b, out, out, xyz, n_ptr, rec, nullptr, all, 2, (T)r_px, st, amp);
+  residual_pass<T>(b, out, out, xyz, n_ptr, rec, nullptr, all, 2, r_px, st, amp);
 }
 
 size_t shake_rec_bytes(bool f64) {

@@ -385,6 +385,25 @@ def workload_c3_frame(frame: int, seed: int = 1234, ppp: float = 0.05,
                     (1000.0, 1.2, 1.2, 0.0), (-20, 20) * 3, vox)
 
 
+def workload_noise_limit(seed: int = 4242, size: int = 512, cap: int = 3000) -> Workload:
+    """3,000 particles (a = 125, b = c = 1.5) rendered on 4 x 512^2 u8 under
+    uniform integer noise in [0, 70]: about one pixel in five is a local
+    maximum, so each camera has ~49,000 peaks, above the 46,340 up to which
+    std::shuffle draws two positions at a time; limit2DObjectNumber
+    (n_obj2d_process_max = cap) keeps `cap` of them, and which ones decides
+    what is reconstructed."""
+    cams = make_rig(size, size, max_intensity=255)
+    pts = uniform_particles(3000, seed)
+    img = render(cams, pts, 125.0, 1.5, 1.5, 0.0, 2.0, seed + 1, np.uint8)
+    noise = np.random.default_rng(seed + 2).integers(0, 71, size=img.shape)
+    img = np.clip(img.astype(np.int32) + noise, 0, 255).astype(np.uint8)
+    vox = 40.0 / (0.95 * size)
+    p = default_params(vox, min_obj_int=30)
+    p.n_obj2d_process_max = cap
+    return Workload(f"noise_{size}_u8_limit{cap}", cams, img, pts, p,
+                    (125.0, 1.5, 1.5, 0.0), (-20, 20) * 3, vox)
+
+
 # ---------------------------------------------------------------------------
 # Tracking sequences: particles advected by an ABC flow
 # ---------------------------------------------------------------------------

@@ -195,6 +195,11 @@ def main():
     w.params.n_loop_ipr = 2
     w.params.n_loop_ipr_reduced = 2
     run_case("golden_reduced_256", w, (125.0, 1.5, 1.5, 0.0), reduced=1)
+    # limit2DObjectNumber (IPR.cpp:19-39): ~740 peaks per camera cut to 500
+    # by the reference's std::shuffle (the two-positions-per-draw branch)
+    w = S.workload_c1(seed=61, n=800, size=256)
+    w.params.n_obj2d_process_max = 500
+    run_case("golden_limit_256", w, (125.0, 1.5, 1.5, 0.0))
     # non-uniform OTF: a 3 x 4 x 2 grid (unequal axes, so a swapped index
     # order changes the interpolated values) whose box [-18,18] x [-20,20] x
     # [-15,15] is smaller than the particle volume (clamping at the box);

@@ -9,6 +9,10 @@ seeded workloads and stores what the GPU tests compare against:
   ref_c3_f1.npz   the same workload, frame 1
   ref_c2.npz      1024^2 u16, 20,000 uniform particles, all cameras
   ref_c2_red1.npz the same with n_cam_reduced = 1 (12 IPR loops)
+  ref_noise_limit.npz  3,000 particles on 4 x 512^2 u8 under uniform noise
+                  (~50,000 peaks per camera) with n_obj2d_process_max =
+                  3,000: limit2DObjectNumber's shuffle on its one-position-
+                  per-draw branch
 
 Each file holds the workload's name and seed, a SHA-256 of the rendered
 images (the GPU test regenerates them from the seed and checks the hash
@@ -66,6 +70,7 @@ def run(name: str, w: S.Workload, *, exact: bool, seed: int, frame: int = -1):
     n = int(cnt[0])
     xyz = ox[:n].copy()
     q = S.ghost_eval(xyz, w.truth, w.voxel_mm)
+    q = {k: (np.nan if v is None else v) for k, v in q.items()}
     out = dict(workload=np.array(w.name), seed=np.int64(seed), frame=np.int64(frame),
                image_sha256=np.array(image_hash(w.images)),
                n_cam_reduced=np.int32(w.params.n_cam_reduced),
@@ -92,6 +97,7 @@ def main():
     w.params.n_cam_reduced = 1
     w.params.n_loop_ipr_reduced = 2
     run("ref_c2_red1", w, exact=False, seed=1234)
+    run("ref_noise_limit", S.workload_noise_limit(), exact=True, seed=4242)
 
 
 if __name__ == "__main__":

@@ -45,7 +45,8 @@ def test_stereo_needs_two_active_cameras(case):
 
 def test_check_cameras_configuration_is_rejected(case):
     """match_cam_count below the active count: the reference's checkMatch
-    leaves e_check NaN and match() aborts; the library refuses up front."""
+    leaves e_check NaN and match() raises InvalidArgument (REQUIRE,
+    StereoMatch.cpp:126-127); the library returns that status up front."""
     from pyopenlpt_amd._lib import FatalError
     ctx = make_ctx(case)
     p = case.params
@@ -54,7 +55,7 @@ def test_check_cameras_configuration_is_rejected(case):
     try:
         with pytest.raises(FatalError) as e:
             ctx.stereo_match(case.peaks(), p)
-        assert e.value.status == 3
+        assert e.value.status == 1
     finally:
         p.match_cam_count = old
     ctx.close()
@@ -241,4 +242,35 @@ def test_peaks_match_oracle_on_odd_widths(shape):
     assert got.shape == want.shape
     if len(want):
         assert np.abs(got - want).max() <= 1e-9
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_residual_box_width_rounds_in_double(case, precision):
+    """calROIBound's box edge round(2 r) is taken in double on the host for
+    every kernel: at r = 1.2499999999 it is 2 (in FP32, 2 * (float)r = 2.5
+    rounds to 3).  The residual then equals the oracle's (FP64 to 1e-9, FP32
+    to 2e-3 of the 255 scale)."""
+    import ctypes
+    import oracle as O
+    from pyopenlpt_amd.context import to_device_image
+    ctx = make_ctx(case, precision)
+    d, code = to_device_image(case.images)
+    p = case.params
+    old = p.radius_obj
+    p.radius_obj = 1.2499999999
+    try:
+        xyz = case.z["ipr_xyz"][:300]
+        got = ctx.residual(d, code, p, xyz)
+        otf, _k = case.otf()
+        want = np.zeros((case.n_cam, case.H, case.W))
+        O.load_port().orc_residual(O.cams_array(case.cams), case.n_cam,
+                                   O.dptr(case.images.astype(np.float64)), case.H, case.W,
+                                   ctypes.byref(p), ctypes.byref(otf),
+                                   O.dptr(np.ascontiguousarray(xyz)), len(xyz), O.dptr(want))
+    finally:
+        p.radius_obj = old
+    err = np.abs(got - want).max()
+    print(f"r = 1.2499999999 residual ({precision}): max |diff| {err:.3g}")
+    assert err <= (1e-9 if precision == "f64" else 2e-3)
     ctx.close()

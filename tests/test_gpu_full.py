@@ -3,7 +3,8 @@
 Fixtures (tests/golden/make_golden_full.py, written on the CPU from the
 compiled reference sources): the ppp 0.05 clustered workload (1024^2 u16,
 52,429 particles, frames 0 and 1), and the 20,000-particle uniform workload
-with all cameras and with n_cam_reduced = 1.  The frames are regenerated
+with all cameras and with n_cam_reduced = 1; 3,000 particles under noise with
+limit2DObjectNumber subsampling ~49,000 peaks per camera.  The frames are regenerated
 here from their seeds and checked against the stored image hash first.
 
 Bars:
@@ -48,6 +49,8 @@ class Full:
         self.name = name
         if name.startswith("ref_c3"):
             w = S.workload_c3_frame(int(z["frame"]), seed=int(z["seed"]))
+        elif name == "ref_noise_limit":
+            w = S.workload_noise_limit(seed=int(z["seed"]))
         else:
             w = S.workload_c2(seed=int(z["seed"]))
         w.params.n_cam_reduced = int(z["n_cam_reduced"])
@@ -80,6 +83,17 @@ class Full:
 @pytest.fixture(scope="module", params=FULL)
 def full(request, gpu_available):
     return Full(request.param)
+
+
+def test_limit2d_subsampling_matches_reference(gpu_available):
+    """limit2DObjectNumber (IPR.cpp:19-39) on ~49,000 peaks per camera cut to
+    3,000 (std::shuffle's one-position-per-draw branch, run by one GPU
+    thread): FP64 per-loop counts and positions equal the reference's."""
+    f = Full("ref_noise_limit")
+    got, loops, _diag = f.run("f64")
+    assert np.array_equal(loops, f.loops), (loops.tolist(), f.loops.tolist())
+    assert len(got) == len(f.xyz)
+    assert np.abs(got - f.xyz).max(initial=0) <= 1e-9
 
 
 def test_full_f64_matches_reference(full):

@@ -76,3 +76,17 @@ def test_oracle_prune_matches_reference(name):
                                  len(e), O.u8ptr(sel))
     assert rc == 0
     assert np.array_equal(sel, z[f"{name}_sel"])
+
+
+def test_oracle_limit2d_matches_reference():
+    """limit2DObjectNumber's std::shuffle restated in C, on its one-position-
+    per-draw branch (~49,000 peaks per camera cut to 3,000): runIPR equals the
+    compiled reference's (tests/golden/make_golden_full.py)."""
+    import hashlib
+    from pyopenlpt_amd import synth as S
+    z = np.load(os.path.join(GOLDEN, "ref_noise_limit.npz"))
+    w = S.workload_noise_limit(seed=int(z["seed"]))
+    assert hashlib.sha256(np.ascontiguousarray(w.images).tobytes()).hexdigest() == str(z["image_sha256"])
+    xyz, _uv, loops = oracle_ipr(w.cams, w.images, w.params, w.otf_abca)
+    assert np.array_equal(loops, z["loops"])
+    assert np.array_equal(xyz, z["xyz"])
