@@ -123,6 +123,24 @@ olpt_status olpt_find_peaks(olpt_ctx *ctx, const void *d_img, int dtype,
                             const olpt_ipr_params *params, double *d_xy,
                             int cap, int32_t *n_out);
 
+/* No counterpart in the reference: olpt_find_peaks (ObjectFinder2D::
+ * findObject2D, tracer branch) for callers that cannot allocate device
+ * memory - code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  It takes no context and allocates nothing: its
+ * work memory (chunk masks, counts, offsets, scan partials, flags) is a
+ * fixed array of the library, one per GPU (about 4.2 MB, held until the
+ * process ends), so images are limited to 4096 x 4096 (larger: status
+ * OLPT_E_CAPACITY).  Calls on one GPU share that array and must not
+ * overlap: issue them on one stream, or order them.  stream: a hipStream_t
+ * of the current device (NULL = its null stream).  The work is enqueued and
+ * the call returns: *d_n_out (device) receives the total number of peaks,
+ * of which the first min(total, cap) are written to d_xy as (x, y) pairs in
+ * row-major scan order - the values olpt_find_peaks returns. */
+olpt_status olpt_find_peaks_static(void *stream, const void *d_img, int dtype,
+                                   int n_row, int n_col,
+                                   const olpt_ipr_params *params,
+                                   double *d_xy, int cap, int32_t *d_n_out);
+
 /* StereoMatch(cams, obj2d_list, cfg).match() (StereoMatch.cpp:65-178).
  * d_xy: per camera, cap_2d (x, y) pairs ([n_cam][cap_2d][2]); n_pts: host
  * counts; active: host flags (NULL = all active).  Outputs (device):

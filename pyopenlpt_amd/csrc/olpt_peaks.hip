@@ -13,6 +13,8 @@
 // fitting the separable three-point log-parabola in FP64 (the reference's
 // absolute-coordinate formula, :67-97).
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 #include "olpt_kernels.hpp"
 #include "olpt_scan.hpp"
@@ -321,3 +323,112 @@ INST(double)
 #undef INST
 
 } // namespace olpt
+
+// ===========================================================================
+// olpt_find_peaks_static: the peak kernels on a fixed per-device work array
+// (no context, no allocation).
+// ===========================================================================
+namespace {
+
+using namespace olpt;
+
+constexpr int kStaticMaxSide = 4096;
+constexpr long kStaticMaxChunks = (long)(kStaticMaxSide - 2) * ((kStaticMaxSide - 2 + 63) / 64);
+
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+// layout: mask u64 | cnt i32 | off i32 | scan partials | totals[64] |
+// npts[OLPT_MAX_CAM] | flags[8]
+constexpr size_t kOffMask = 0;
+constexpr size_t kOffCnt = kOffMask + align256(8 * (size_t)kStaticMaxChunks);
+constexpr size_t kOffOff = kOffCnt + align256(4 * (size_t)kStaticMaxChunks);
+constexpr size_t kOffScan = kOffOff + align256(4 * (size_t)kStaticMaxChunks);
+constexpr size_t kScanInts = ((size_t)kStaticMaxChunks + kScanTile - 1) / kScanTile + 1;
+constexpr size_t kOffTotals = kOffScan + align256(4 * kScanInts);
+constexpr size_t kOffNpts = kOffTotals + align256(4 * 64);
+constexpr size_t kOffFlags = kOffNpts + align256(4 * OLPT_MAX_CAM);
+constexpr size_t kStaticBytes = kOffFlags + align256(4 * 8);   // ~4.2 MB
+
+// The work memory of olpt_find_peaks_static: one copy per GPU, held until
+// the process ends.  Host code uses only the device address returned by
+// hipGetSymbolAddress (per device), never this name.
+__device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+__global__ void k_static_clear(int32_t *p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+__global__ void k_static_count(const int32_t *__restrict__ totals, int32_t *__restrict__ n_out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *n_out = totals[0];
+}
+
+thread_local std::string g_static_err;
+
+olpt_status static_fail(olpt_status s, const std::string &m) {
+  g_static_err = m;
+  return olpt::set_error(s, g_static_err.c_str());
+}
+
+} // namespace
+
+extern "C" olpt_status olpt_find_peaks_static(void *stream, const void *d_img, int dtype, int n_row,
+                                              int n_col, const olpt_ipr_params *params, double *d_xy,
+                                              int cap, int32_t *d_n_out) {
+  if (!d_img || !params || !d_n_out || (cap > 0 && !d_xy))
+    return static_fail(OLPT_E_INVALID_ARGUMENT, "null argument");
+  if (n_row < 3 || n_col < 3 || cap < 0)
+    return static_fail(OLPT_E_INVALID_ARGUMENT, "image must be at least 3 x 3 and cap >= 0");
+  if (n_row > kStaticMaxSide || n_col > kStaticMaxSide)
+    return static_fail(OLPT_E_CAPACITY, "olpt_find_peaks_static takes images up to 4096 x 4096");
+  if (dtype < OLPT_U8 || dtype > OLPT_F64) return static_fail(OLPT_E_UNSUPPORTED, "unsupported image dtype");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return static_fail(OLPT_E_GPU, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return static_fail(OLPT_E_GPU, "device index out of range");
+  hipStream_t st = (hipStream_t)stream;
+  if (st) {
+    int sdev = -1;
+    e = hipStreamGetDevice(st, &sdev);
+    if (e != hipSuccess) return static_fail(OLPT_E_GPU, std::string("hipStreamGetDevice: ") + hipGetErrorString(e));
+    if (sdev != dev) return static_fail(OLPT_E_INVALID_ARGUMENT, "the stream belongs to another device than the current one");
+  }
+  unsigned char *base;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+      void *p = nullptr;
+      e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p)
+        return static_fail(OLPT_E_GPU, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      g_scratch_ptr[dev] = p;
+    }
+    base = (unsigned char *)g_scratch_ptr[dev];
+  }
+  uint64_t *mask = (uint64_t *)(base + kOffMask);
+  int32_t *cnt = (int32_t *)(base + kOffCnt);
+  int32_t *off = (int32_t *)(base + kOffOff);
+  int32_t *scan = (int32_t *)(base + kOffScan);
+  int32_t *totals = (int32_t *)(base + kOffTotals);
+  int32_t *npts = (int32_t *)(base + kOffNpts);
+  int32_t *flags = (int32_t *)(base + kOffFlags);
+  // the overflow flag is or-ed into: clear it (a kernel, so the call can be
+  // captured into a graph); every other word is written before it is read
+  k_static_clear<<<1, 64, 0, st>>>(flags, 8);
+  CamList cl{1, {0}};
+  const double mi = (double)params->min_obj_int;
+  double *xy = d_xy;
+  const int c = cap;
+  switch (dtype) {
+  case OLPT_U8: launch_find_peaks<uint8_t>((const uint8_t *)d_img, 0, n_row, n_col, mi, cl, mask, cnt, off, scan, totals, xy, c, npts, flags, st); break;
+  case OLPT_U16: launch_find_peaks<uint16_t>((const uint16_t *)d_img, 0, n_row, n_col, mi, cl, mask, cnt, off, scan, totals, xy, c, npts, flags, st); break;
+  case OLPT_F32: launch_find_peaks<float>((const float *)d_img, 0, n_row, n_col, mi, cl, mask, cnt, off, scan, totals, xy, c, npts, flags, st); break;
+  default: launch_find_peaks<double>((const double *)d_img, 0, n_row, n_col, mi, cl, mask, cnt, off, scan, totals, xy, c, npts, flags, st); break;
+  }
+  k_static_count<<<1, 64, 0, st>>>(totals, d_n_out);
+  e = hipGetLastError();
+  if (e != hipSuccess) return static_fail(OLPT_E_GPU, std::string("launch: ") + hipGetErrorString(e));
+  return OLPT_OK;
+}
