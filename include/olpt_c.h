@@ -86,6 +86,10 @@ olpt_status olpt_get_tuning(olpt_ctx *ctx, olpt_tuning *tuning);
 olpt_status olpt_set_cameras(olpt_ctx *ctx, const olpt_camera *cams, int n_cam);
 /* Replaces TracerConfig::_otf (OTF.h:33-78); arrays are copied. */
 olpt_status olpt_set_otf(olpt_ctx *ctx, const olpt_otf *otf);
+/* OTF::getOTFParam (OTF.cpp:273-338) of the context's OTF, FP64: for camera
+ * cam at n world points d_xyz [n][3], d_out [n][5] = (a, b, c, cos alpha,
+ * sin alpha) (device buffers; returns when done). */
+olpt_status olpt_otf_param(olpt_ctx *ctx, int cam, const double *d_xyz, int n, double *d_out);
 olpt_status olpt_sync(olpt_ctx *ctx);
 void *olpt_stream(olpt_ctx *ctx);
 const char *olpt_last_error(void);

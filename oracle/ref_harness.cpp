@@ -23,6 +23,8 @@
 #include <string>
 #include <vector>
 
+#include <cmath>
+
 #include "Config.h"
 #include "IPR.h"
 #include "ObjectFinder.h"
@@ -392,6 +394,31 @@ int ref_stereo_match(const olpt_camera *cams, int n_cam,
       }
     }
   }
+  return 0;
+}
+
+// IPR::saveObjInfo (IPR.cpp:307-343) of n tracers: xyz [n][3], uv
+// [n][n_cam][2] (a NaN x leaves that camera without an observation).
+int ref_save_obj_info(const olpt_camera *cams, int n_cam, const double *xyz, const double *uv,
+                      int n, const char *filename) {
+  std::vector<Camera> cv = makeCameras(cams, n_cam, nullptr);
+  TracerConfig cfg;
+  std::vector<std::unique_ptr<Object3D>> objs;
+  for (int i = 0; i < n; ++i) {
+    auto t = std::make_unique<Tracer3D>();
+    t->_pt_center = Pt3D(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    t->_obj2d_list.resize(n_cam);
+    for (int c = 0; c < n_cam; ++c) {
+      const double u = uv[((size_t)i * n_cam + c) * 2], v = uv[((size_t)i * n_cam + c) * 2 + 1];
+      if (std::isnan(u)) continue;
+      auto o = std::make_unique<Tracer2D>();
+      o->_pt_center = Pt2D(u, v);
+      t->_obj2d_list[c] = std::move(o);
+    }
+    objs.push_back(std::move(t));
+  }
+  IPR ipr(cv);
+  ipr.saveObjInfo(filename, objs, cfg);
   return 0;
 }
 

@@ -70,6 +70,7 @@ def lib() -> ctypes.CDLL:
     L.olpt_stereo_match.argtypes = [_P, _I32P, _P, _I, _I32P,
                                     ctypes.POINTER(OlptIprParams), _P, _P, _P, _I,
                                     _P, _P, _I, _I32P]
+    L.olpt_otf_param.argtypes = [_P, _I, _P, _I, _P]
     L.olpt_find_peaks_static.argtypes = [_P, _P, _I, _I, _I, ctypes.POINTER(OlptIprParams),
                                          _P, _I, _P]
     L.olpt_prune_match.argtypes = [_P, _I32P, _I32P, ctypes.POINTER(OlptIprParams), _P, _P,
@@ -131,7 +132,7 @@ def lib() -> ctypes.CDLL:
                  "olpt_get_tuning", "olpt_set_cameras",
                  "olpt_set_otf",
                  "olpt_sync", "olpt_diagnostics", "olpt_find_peaks",
-                 "olpt_stereo_match", "olpt_prune_match", "olpt_find_peaks_static", "olpt_shake", "olpt_residual",
+                 "olpt_stereo_match", "olpt_prune_match", "olpt_find_peaks_static", "olpt_otf_param", "olpt_shake", "olpt_residual",
                  "olpt_ipr_run", "olpt_ipr_graph_capture",
                  "olpt_ipr_graph_launch", "olpt_stage_timing",
                  "olpt_stage_times", "olpt_stage_stats"):

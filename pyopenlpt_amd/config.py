@@ -226,6 +226,20 @@ class TracerConfig:
                 self.otf.set_uniform(k, abca[k])
         return [bool(v) for v in ok]
 
+    @classmethod
+    def from_ipr_params(cls, p: S.OlptIprParams, otf: "Otf | None" = None) -> "TracerConfig":
+        """A configuration holding the IPR / stereo / shake fields of p (the
+        inverse of ipr_params) and the given OTF."""
+        cfg = cls()
+        for f in ("n_cam_reduced", "n_loop_ipr", "n_loop_ipr_reduced", "n_obj2d_process_max",
+                  "match_cam_count", "idmap_cell_px", "tol_2d_px", "tol_3d_mm", "shake_width",
+                  "shakewidth_min", "n_shake_loop", "ratio_augimg", "thred_ghost", "min_obj_int",
+                  "radius_obj"):
+            setattr(cfg, f, getattr(p, f))
+        cfg.limit = tuple(float(v) for v in p.limit)
+        cfg.otf = otf
+        return cfg
+
     def ipr_params(self) -> S.OlptIprParams:
         p = S.OlptIprParams()
         p.n_cam_reduced = self.n_cam_reduced
@@ -404,7 +418,9 @@ def ctypes_ptr(t) -> ctypes.c_void_p:
 def save_obj_info(filename: str, xyz: np.ndarray, uv: np.ndarray) -> None:
     """IPR::saveObjInfo (IPR.cpp:307-343) for tracers: header with
     Error,Ncam columns that the rows do not fill (as the reference writes
-    it), rows X,Y,Z then (x, y) per camera in the default stream format."""
+    it), rows X,Y,Z then (x, y) per camera in the default stream format
+    (Tracer3D::saveObject3D, ObjectInfo.cpp:76-94); a camera without an
+    observation (NaN x) keeps its two columns empty."""
     xyz = np.asarray(xyz, np.float64).reshape(-1, 3)
     n_cam = uv.shape[1] if uv.ndim == 3 else 0
     with open(filename, "w") as f:
@@ -415,7 +431,8 @@ def save_obj_info(filename: str, xyz: np.ndarray, uv: np.ndarray) -> None:
         for j in range(len(xyz)):
             row = [_g(v) for v in xyz[j]]
             for i in range(n_cam):
-                row += [_g(uv[j, i, 0]), _g(uv[j, i, 1])]
+                row += (["", ""] if np.isnan(uv[j, i, 0])
+                        else [_g(uv[j, i, 0]), _g(uv[j, i, 1])])
             f.write(",".join(row) + "\n")
 
 

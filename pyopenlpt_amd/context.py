@@ -250,6 +250,15 @@ class Context:
                     sel=sel[:M].cpu().numpy().astype(bool), obj_xyz=ox[:no].cpu().numpy(),
                     obj_ids=oi[:no].cpu().numpy(), counts=counts)
 
+    def otf_param(self, cam: int, xyz: np.ndarray) -> np.ndarray:
+        """OTF::getOTFParam at points xyz [n][3]: [n][5] = (a, b, c, cos
+        alpha, sin alpha), FP64."""
+        dev = torch.device("cuda", self.device)
+        x = torch.from_numpy(np.ascontiguousarray(np.atleast_2d(xyz), dtype=np.float64)).to(dev)
+        out = torch.empty((max(len(x), 1), 5), dtype=torch.float64, device=dev)
+        check(self.L.olpt_otf_param(self.h, int(cam), ptr(x), len(x), ptr(out)))
+        return out[: len(x)].cpu().numpy()
+
     def prune_match(self, cand_ids: np.ndarray, cand_e: np.ndarray, n_pts, params: OlptIprParams,
                     active=None) -> np.ndarray:
         """StereoMatch::pruneMatch on given candidates ([n][n_cam] ids, -1 =

@@ -576,6 +576,17 @@ olpt_status ipr_frame(olpt_ctx *c, Workspace &w, const olpt_ipr_params *p, doubl
   return OLPT_OK;
 }
 
+// OTF::getOTFParam at n points (FP64)
+__global__ void k_otf_param(OtfT<double> o, int cam, const double *__restrict__ xyz, int n,
+                            double *__restrict__ out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double X[3] = {xyz[3L * i], xyz[3L * i + 1], xyz[3L * i + 2]};
+    double P[5];
+    otf_param(o, cam, X, P);
+    for (int q = 0; q < 5; ++q) out[5L * i + q] = P[q];
+  }
+}
+
 size_t tsize(olpt_ctx *c) { return c->prec == OLPT_PREC_F64 ? 8 : 4; }
 
 } // namespace
@@ -1094,6 +1105,19 @@ olpt_status olpt_stereo_match(olpt_ctx *c, const int32_t *active, const double *
                              hipMemcpyDeviceToDevice, st));
   }
   HIP_TRY(hipStreamSynchronize(st));
+  return OLPT_OK;
+}
+
+olpt_status olpt_otf_param(olpt_ctx *c, int cam, const double *d_xyz, int n, double *d_out) {
+  if (!c || !c->n_ws) return fail(OLPT_E_INVALID_ARGUMENT, "workspace not reserved");
+  if (!c->otf_set) return fail(OLPT_E_INVALID_ARGUMENT, "OTF not set");
+  if (cam < 0 || cam >= c->n_cam) return fail(OLPT_E_OUT_OF_RANGE, "camera index out of range");
+  if (n < 0 || (n > 0 && (!d_xyz || !d_out))) return fail(OLPT_E_INVALID_ARGUMENT, "null argument");
+  if (n == 0) return OLPT_OK;
+  k_otf_param<<<std::min(4096, (n + 255) / 256), 256, 0, c->st>>>(dotf<double>(c), cam, d_xyz, n,
+                                                                     d_out);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->st));
   return OLPT_OK;
 }
 

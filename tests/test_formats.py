@@ -177,3 +177,19 @@ def test_abc_sequence_bounds_and_steps():
         assert np.all(np.abs(p1) <= 20.0)
         same = i0 == i1
         assert np.linalg.norm(p1[same] - p0[same], axis=1).max() <= 0.3 + 1e-12
+
+
+def test_save_obj_info_matches_reference_bytes(tmp_path):
+    """save_obj_info writes the file the reference's IPR::saveObjInfo wrote
+    for the same objects (tests/golden/make_golden_objinfo.py): header,
+    default-format doubles, empty columns for a camera without an
+    observation - byte for byte."""
+    import os
+    import numpy as np
+    from pyopenlpt_amd import config as C
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    z = np.load(os.path.join(here, "objinfo_u8_256.npz"))
+    out = tmp_path / "objs.csv"
+    C.save_obj_info(str(out), z["xyz"], z["uv"])
+    want = open(os.path.join(here, "objinfo_u8_256.csv"), "rb").read()
+    assert out.read_bytes() == want
