@@ -33,6 +33,7 @@
 #include "StereoMatch.h"
 
 #include "olpt_types.h"
+#include "ref_sink.hpp"
 
 // ---- reference debug hooks (declared friends when OPENLPT_EXPOSE_PRIVATE) ----
 struct DebugAccess_StereoMatch {
@@ -61,13 +62,6 @@ struct DebugAccess_Shake {
 
 namespace {
 
-// Silence (or capture) std::cout while the reference runs.
-struct CoutCapture {
-  std::ostringstream buf;
-  std::streambuf *old;
-  CoutCapture() : old(std::cout.rdbuf(buf.rdbuf())) {}
-  ~CoutCapture() { std::cout.rdbuf(old); }
-};
 
 Camera makeCamera(const olpt_camera &c, bool active) {
   Camera cam;
@@ -304,7 +298,7 @@ int ref_stereo_match(const olpt_camera *cams, int n_cam,
                      int cap_cand, int32_t *cand_ids, double *cand_e,
                      int32_t *sel_idx, int cap_obj, double *obj_xyz,
                      int32_t *obj_ids, int32_t *counts) {
-  CoutCapture quiet;
+  CoutRedirect quiet(false);
   std::vector<Camera> cv = makeCameras(cams, n_cam, active);
   TracerConfig cfg;
   fillConfig(cfg, *p, nullptr, cv);
@@ -429,7 +423,7 @@ int ref_save_obj_info(const olpt_camera *cams, int n_cam, const double *xyz, con
 int ref_prune(const olpt_camera *cams, int n_cam, const int32_t *active,
               const int32_t *n_pts, const olpt_ipr_params *p, const int32_t *ids,
               const double *e, int M, uint8_t *sel) {
-  CoutCapture quiet;
+  CoutRedirect quiet(false);
   std::vector<Camera> cv = makeCameras(cams, n_cam, active);
   TracerConfig cfg;
   fillConfig(cfg, *p, nullptr, cv);
@@ -459,7 +453,7 @@ int ref_shake(const olpt_camera *cams, int n_cam, const int32_t *active,
               const olpt_otf *otf, const double *xyz_in, int n_obj,
               double *xyz_out, uint8_t *flags, double *score,
               double *res_out) {
-  CoutCapture quiet;
+  CoutRedirect quiet(false);
   std::vector<Camera> cv = makeCameras(cams, n_cam, active);
   TracerConfig cfg;
   fillConfig(cfg, *p, otf, cv);
@@ -501,10 +495,10 @@ int ref_run_ipr(const olpt_camera *cams, int n_cam, const double *imgs,
   std::vector<std::unique_ptr<Object3D>> objs;
   std::string log;
   {
-    CoutCapture cap_out;
+    CoutRedirect cap_out(true);
     IPR ipr(cv);
     objs = ipr.runIPR(cfg, std::move(images));
-    log = cap_out.buf.str();
+    log = cap_out.text();
   }
   int n_loop = 0;
   const std::string key = "TOTAL OBJECTS = ";

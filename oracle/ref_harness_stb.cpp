@@ -26,17 +26,26 @@
 #include <string>
 #include <vector>
 
-// PredField keeps its field private and has no debug hook: its header is
-// read first with its members public, so the harness can copy the field.
-#define private public
-#include "PredField.h"
-#undef private
-
 #include "Config.h"
 #include "ImageIO.h"
 #include "PredField.h"
 #include "STB.h"
 #include "Track.h"
+#include "ref_sink.hpp"
+
+// PredField keeps its field private and has no debug hook.  An explicit
+// instantiation may name a private member ([temp.spec.general]: access
+// checks do not apply to its template arguments), so a pointer to
+// PredField::_disp_field is taken that way: no redefined keyword, one
+// definition of the class everywhere.
+template <typename Tag, typename Tag::type M> struct MemberPtr {
+  friend typename Tag::type member_ptr(Tag) { return M; }
+};
+struct PredFieldDisp {
+  typedef Matrix<double> PredField::*type;
+  friend type member_ptr(PredFieldDisp);
+};
+template struct MemberPtr<PredFieldDisp, &PredField::_disp_field>;
 
 struct DebugAccess_STB {
   static ObjectConfig &cfg(STB &s) { return *s._obj_config; }
@@ -54,11 +63,8 @@ struct RefSTB {
   std::vector<Image> imgs;
 };
 
-struct CoutMute {
-  std::streambuf *old;
-  std::ostringstream sink;
-  CoutMute() : old(std::cout.rdbuf(sink.rdbuf())) {}
-  ~CoutMute() { std::cout.rdbuf(old); }
+struct CoutMute : CoutRedirect {
+  CoutMute() : CoutRedirect(false) {}
 };
 
 std::deque<Track> *deque_of(STB &s, int which) {
@@ -224,7 +230,7 @@ int ref_pred_field(void *hp, const double *prev, int n_prev, const double *curr,
   pf.calPredField(a, b);
   const PFParam &pp = cfg._pf_param;
   const int ng = pp.nx * pp.ny * pp.nz;
-  const Matrix<double> &f = pf._disp_field;
+  const Matrix<double> &f = pf.*member_ptr(PredFieldDisp());
   for (int g = 0; g < ng; ++g)
     for (int k = 0; k < 3; ++k) field[3 * g + k] = f(g, k);
   for (int i = 0; i < n_q; ++i) {
