@@ -70,15 +70,20 @@ olpt_status olpt_set_concurrency(olpt_ctx *ctx, int n_streams);
  * the shake and proposal kernels (default 4096); build_blocks: 256-thread
  * workgroups of the stereo build kernels (default 1024); prune_blocks: cap
  * on the 256-thread workgroups of the sort/prune/triangulate kernels
- * (default 512); n_streams: as olpt_set_concurrency.  Each field must lie
- * in its range (blocks 64..65536, streams 1..8); a captured graph is
- * dropped.  Python's Context.autotune times candidates on the caller's
+ * (default 512), each per frame; n_streams: as olpt_set_concurrency;
+ * frames_per_batch (default 1): the frames every kernel launch of
+ * olpt_ipr_run covers - its grid takes the frame from blockIdx.z and each
+ * workspace holds that many frame slices (memory grows with it; changing it
+ * reallocates the workspaces), so one launch sequence serves a whole batch.
+ * Each field must lie in its range (blocks 64..65536, streams 1..8, frames
+ * 1..1024); a captured graph is dropped.  Python's Context.autotune times candidates on the caller's
  * frames and keeps the fastest. */
 typedef struct {
   int32_t n_streams;
   int32_t shake_blocks;
   int32_t build_blocks;
   int32_t prune_blocks;
+  int32_t frames_per_batch;
 } olpt_tuning;
 olpt_status olpt_set_tuning(olpt_ctx *ctx, const olpt_tuning *tuning);
 olpt_status olpt_get_tuning(olpt_ctx *ctx, olpt_tuning *tuning);
@@ -197,7 +202,7 @@ olpt_status olpt_residual(olpt_ctx *ctx, const void *d_imgs, int dtype,
  * [f][cap_obj_frame][3] and their final projections to d_obj_uv
  * [f][cap_obj_frame][n_cam][2] (may be NULL); d_n_obj[f] holds the count.
  * loop_total (host, optional, [n_frames][cap_loop]) receives the
- * cumulative count after each IPR loop. */
+ * cumulative count after each IPR loop (0 past the last loop). */
 olpt_status olpt_ipr_run(olpt_ctx *ctx, const void *d_imgs, int dtype,
                          int n_frames, const olpt_ipr_params *params,
                          double *d_obj_xyz, double *d_obj_uv,

@@ -42,6 +42,8 @@ olpt_status fail(olpt_status s, const std::string &msg) {
 
 constexpr int kMaxRounds = 64;
 
+olpt_status read_flags(olpt_ctx *c, int nw, hipStream_t st, int32_t out[8]);
+
 std::string prune_ties_msg(const int32_t *fl) {
   return std::string("pruneMatch: ") +
          (fl[2] ? "values within the eq_eps tolerance (rtol 1e-9) are not transitive here, so "
@@ -58,6 +60,12 @@ std::string prune_ties_msg(const int32_t *fl) {
 struct Workspace {
   hipStream_t st = nullptr;     // own stream (workspace 0: the context stream)
   hipEvent_t done = nullptr;
+  // one arena of nf_cap frame slices, fstride bytes each; every pointer
+  // below is frame 0's (frame f's buffer lies f * fstride bytes further)
+  void *arena = nullptr;
+  int nf_cap = 0;
+  long fstride = 0;
+  long lstride = 0, pstride = 0;  // learned-proposal buffers (own allocations)
   // images (sized for FP64)
   void *img_a = nullptr, *img_b = nullptr, *img_res = nullptr;
   // peaks
@@ -90,6 +98,7 @@ struct Workspace {
 };
 
 constexpr int kMaxWorkspaces = 8;
+constexpr int kMaxBatch = 1024;
 
 struct olpt_ctx {
   int device = 0;
@@ -139,7 +148,8 @@ struct olpt_ctx {
   double host_ms = 0.0;
   int32_t host_calls = 0;
   // launch shapes (olpt_set_tuning)
-  olpt_tuning tune = {1, 4096, 1024, 512};
+  olpt_tuning tune = {1, 4096, 1024, 512, 1};
+  int batch = 1;  // frames per workspace launch (arena slices)
 };
 
 namespace {
@@ -186,52 +196,82 @@ void free_graphs(olpt_ctx *c) {
   c->n_graph = 0;
 }
 
+void free_workspace_memory(olpt_ctx *c) {
+  for (int k = 0; k < kMaxWorkspaces; ++k) {
+    Workspace &w = c->ws[k];
+    if (w.arena) (void)hipFree(w.arena);
+    if (w.patches) (void)hipFree(w.patches);
+    if (w.prop) (void)hipFree(w.prop);
+    w.arena = nullptr;
+    w.patches = nullptr;
+    w.prop = nullptr;
+  }
+}
+
 void free_all(olpt_ctx *c) {
   free_graphs(c);
+  free_workspace_memory(c);
   for (void *p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
 }
 
 template <typename TIn, typename T>
-__global__ void k_convert(const TIn *__restrict__ src, long n, T *__restrict__ dst) {
+__global__ void k_convert(const TIn *__restrict__ src, long n, T *__restrict__ dst, long sstride,
+                          long dstride) {
+  src = fshift(src, (long)blockIdx.z * sstride);
+  dst = fshift(dst, (long)blockIdx.z * dstride);
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x)
     dst[i] = (T)src[i];
 }
 
+// n pixels of each of nf frames: source frames sstride bytes apart,
+// destination frames dstride bytes apart
 template <typename T>
-olpt_status convert_image(const void *src, int dtype, long n, T *dst, hipStream_t st) {
+olpt_status convert_image(const void *src, int dtype, long n, T *dst, hipStream_t st, int nf = 1,
+                          long sstride = 0, long dstride = 0) {
   const int threads = 256;
   long blocks = (n + threads - 1) / threads;
   if (blocks > 8192) blocks = 8192;
+  const dim3 g((unsigned)blocks, 1, (unsigned)nf);
   switch (dtype) {
-  case OLPT_U8: k_convert<uint8_t, T><<<blocks, threads, 0, st>>>((const uint8_t *)src, n, dst); break;
-  case OLPT_U16: k_convert<uint16_t, T><<<blocks, threads, 0, st>>>((const uint16_t *)src, n, dst); break;
-  case OLPT_F32: k_convert<float, T><<<blocks, threads, 0, st>>>((const float *)src, n, dst); break;
-  case OLPT_F64: k_convert<double, T><<<blocks, threads, 0, st>>>((const double *)src, n, dst); break;
+  case OLPT_U8: k_convert<uint8_t, T><<<g, threads, 0, st>>>((const uint8_t *)src, n, dst, sstride, dstride); break;
+  case OLPT_U16: k_convert<uint16_t, T><<<g, threads, 0, st>>>((const uint16_t *)src, n, dst, sstride, dstride); break;
+  case OLPT_F32: k_convert<float, T><<<g, threads, 0, st>>>((const float *)src, n, dst, sstride, dstride); break;
+  case OLPT_F64: k_convert<double, T><<<g, threads, 0, st>>>((const double *)src, n, dst, sstride, dstride); break;
   default: return fail(OLPT_E_UNSUPPORTED, "unsupported image dtype");
   }
   return OLPT_OK;
 }
 
-__global__ void k_fill_i32(int32_t *p, long n, int32_t v) {
+__global__ void k_fill_i32(int32_t *p, long n, int32_t v, long fstride) {
+  p = fshift(p, (long)blockIdx.z * fstride);
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x)
     p[i] = v;
 }
-void fill_i32(int32_t *p, long n, int32_t v, hipStream_t st) {
+// n ints at p set to v, in each of nf frames fstride bytes apart
+void fill_i32(int32_t *p, long n, int32_t v, hipStream_t st, int nf = 1, long fstride = 0) {
   long blocks = (n + 255) / 256;
   if (blocks < 1) blocks = 1;
   if (blocks > 4096) blocks = 4096;
-  k_fill_i32<<<blocks, 256, 0, st>>>(p, n, v);
+  k_fill_i32<<<dim3((unsigned)blocks, 1, (unsigned)nf), 256, 0, st>>>(p, n, v, fstride);
 }
 
-// append kept objects of one loop to the frame output
+// append kept objects of one loop to the frame output (frame blockIdx.z:
+// workspace buffers fstride bytes apart, outputs cap objects apart)
 template <typename T>
 __global__ void k_append(const double *__restrict__ kx, const T *__restrict__ kuv,
                          const int32_t *__restrict__ n_keep, int n_cam,
                          double *__restrict__ ox, double *__restrict__ ouv, int cap,
-                         const int32_t *__restrict__ n_out) {
+                         const int32_t *__restrict__ n_out, long fstride) {
+  const long f = blockIdx.z;
+  kx = fshift(kx, f * fstride);
+  kuv = fshift(kuv, f * fstride);
+  n_keep = fshift(n_keep, f * fstride);
+  ox += f * (long)cap * 3;
+  if (ouv) ouv += f * (long)cap * n_cam * 2;
+  n_out += f;
   const int base = *n_out;
   const int n = *n_keep;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -244,8 +284,14 @@ __global__ void k_append(const double *__restrict__ kx, const T *__restrict__ ku
 }
 
 __global__ void k_append_count(const int32_t *n_keep, int cap, int32_t *n_out,
-                               int32_t *loop_tot, int32_t *flags) {
+                               int32_t *loop_tot, int loop_stride, int32_t *flags,
+                               long fstride) {
   if (threadIdx.x || blockIdx.x) return;
+  const long f = blockIdx.z;
+  n_keep = fshift(n_keep, f * fstride);
+  flags = fshift(flags, f * fstride);
+  n_out += f;
+  if (loop_tot) loop_tot += f * loop_stride;
   int t = *n_out + *n_keep;
   if (t > cap) {
     t = cap;
@@ -262,7 +308,9 @@ __global__ void k_sel_flags(const int8_t *state, const int32_t *n_cand, int cap,
     sel[j] = state[j] == 1 ? 1 : 0;
 }
 
-__global__ void k_peak_overflow(const int32_t *pk_flag, int32_t *flags) {
+__global__ void k_peak_overflow(const int32_t *pk_flag, int32_t *flags, long fstride) {
+  pk_flag = fshift(pk_flag, (long)blockIdx.z * fstride);
+  flags = fshift(flags, (long)blockIdx.z * fstride);
   if (threadIdx.x == 0 && blockIdx.x == 0 && pk_flag[0]) atomicOr(&flags[7], 1);
 }
 
@@ -304,8 +352,12 @@ template <typename T> OtfT<T> dotf(olpt_ctx *c) {
 }
 
 template <typename T>
-ShakeBuffers<T> shake_buffers(olpt_ctx *c, Workspace &w, T *img, T *img_next) {
+ShakeBuffers<T> shake_buffers(olpt_ctx *c, Workspace &w, T *img, T *img_next, int nf = 1) {
   ShakeBuffers<T> b{};
+  b.nf = nf;
+  b.fstride = w.fstride;
+  b.lstride = w.lstride;
+  b.pstride = w.pstride;
   b.grid_shake = c->tune.shake_blocks;
   b.n_cam = c->n_cam;
   b.H = c->H;
@@ -448,12 +500,16 @@ olpt_status check_params(olpt_ctx *c, const olpt_ipr_params *p) {
   return OLPT_OK;
 }
 
-// One IPR loop on the active subset: img (loop input, T) -> img_next.
+// One IPR loop on the active subset for the nf frames of a workspace batch:
+// img (loop input, T) -> img_next.  Outputs of frame f: out_xyz + f * cap_out
+// objects, d_n_out[f], d_loop_tot[f * loop_stride].
 template <typename T>
-olpt_status ipr_loop(olpt_ctx *c, Workspace &w, T *img, T *img_next, const int32_t *active,
-                     const olpt_ipr_params *p, double tol2d, double *out_xyz,
-                     double *out_uv, int cap_out, int32_t *d_n_out, int32_t *d_loop_tot) {
+olpt_status ipr_loop(olpt_ctx *c, Workspace &w, int nf, T *img, T *img_next,
+                     const int32_t *active, const olpt_ipr_params *p, double tol2d,
+                     double *out_xyz, double *out_uv, int cap_out, int32_t *d_n_out,
+                     int32_t *d_loop_tot, int loop_stride) {
   hipStream_t st = w.st;
+  const long fs = w.fstride;
   olpt_status s;
   StereoParams sp = stereo_params(c, active, p, tol2d, &s);
   if (s) return s;
@@ -462,19 +518,21 @@ olpt_status ipr_loop(olpt_ctx *c, Workspace &w, T *img, T *img_next, const int32
     if (sp.active[k]) cl.id[cl.n++] = k;
   const long HW = (long)c->H * c->W;
   // 1) ObjectFinder2D on the active cameras (inactive slots stay empty)
-  fill_i32(w.npts, c->n_cam, 0, st);
+  fill_i32(w.npts, c->n_cam, 0, st, nf, fs);
   launch_find_peaks<T>(img, HW, c->H, c->W, (double)p->min_obj_int, cl, w.pk_mask, w.pk_cnt,
                        w.pk_off, w.scan_scratch, w.totals, w.xy, c->cap2d, w.npts,
-                       w.flags + 8, st);
-  k_peak_overflow<<<1, 1, 0, st>>>(w.flags + 8, w.flags);
+                       w.flags + 8, st, nf, fs);
+  k_peak_overflow<<<dim3(1, 1, nf), 1, 0, st>>>(w.flags + 8, w.flags, fs);
   // 1.1) limit2DObjectNumber (IPR.cpp:105-107)
   if (p->n_obj2d_process_max > 0 && p->n_obj2d_process_max < c->cap2d)
     launch_limit2d(w.xy, w.npts, cl, p->n_obj2d_process_max, c->cap2d, w.lim_perm, w.lim_flag,
-                   w.lim_xy, st);
+                   w.lim_xy, st, nf, fs);
   // 2) StereoMatch
-  launch_stereo(w.sb, sp, kMaxRounds, st);
+  StereoBuffers sb = w.sb;
+  sb.nf = nf;
+  launch_stereo(sb, sp, kMaxRounds, st);
   // 3) Shake the matched objects in place
-  ShakeBuffers<T> b = shake_buffers<T>(c, w, img, img_next);
+  ShakeBuffers<T> b = shake_buffers<T>(c, w, img, img_next, nf);
   b.xyz = w.sb.obj_xyz;
   b.n_obj = w.sb.n_obj;
   if constexpr (sizeof(T) == 8)
@@ -488,13 +546,16 @@ olpt_status ipr_loop(olpt_ctx *c, Workspace &w, T *img, T *img_next, const int32
   launch_residual_ipr<T>(b, w.n_keep, w.keep_xyz, (T *)w.keep_uv, p->radius_obj, st);
   // 5) append
   if (out_xyz) {
-    k_append<T><<<256, 256, 0, st>>>(w.keep_xyz, (const T *)w.keep_uv, w.n_keep, c->n_cam,
-                                     out_xyz, out_uv, cap_out, d_n_out);
-    k_append_count<<<1, 1, 0, st>>>(w.n_keep, cap_out, d_n_out, d_loop_tot, w.flags);
+    k_append<T><<<dim3(256, 1, nf), 256, 0, st>>>(w.keep_xyz, (const T *)w.keep_uv, w.n_keep,
+                                                   c->n_cam, out_xyz, out_uv, cap_out, d_n_out,
+                                                   fs);
+    k_append_count<<<dim3(1, 1, nf), 1, 0, st>>>(w.n_keep, cap_out, d_n_out, d_loop_tot,
+                                                  loop_stride, w.flags, fs);
   }
   HIP_TRY(hipGetLastError());
   if (c->stats) {
-    // Algorithmic bytes per launch (what each stage must move at minimum):
+    // Algorithmic bytes per launch (what each stage must move at minimum),
+    // summed over the batch's frames:
     //  peaks:    read the loop image of the active cameras + write the peaks
     //  build:    read 2D points + LOS of the active cameras + the cell CSR,
     //            write the checked candidates
@@ -502,40 +563,45 @@ olpt_status ipr_loop(olpt_ctx *c, Workspace &w, T *img, T *img_next, const int32
     //  shake:    per iteration, the N x C x P^2 patch gathers of the
     //            reference ROI (P = 2 round(r ratio)) + positions
     //  residual: read the loop image and write the residual per camera
-    int32_t np[OLPT_MAX_CAM], cnt[3];
-    HIP_TRY(hipMemcpyAsync(np, w.npts, sizeof(np), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(&cnt[0], w.sb.n_cand, 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(&cnt[1], w.sb.n_obj, 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(&cnt[2], w.n_keep, 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    double n2d = 0;
-    for (int k = 0; k < c->n_cam; ++k) n2d += np[k];
     const double s_t = sizeof(T);
     const double na = cl.n, nc = c->n_cam;
     const double ncell = (double)w.sb.rows_cell * w.sb.cols_cell;
-    const double M = cnt[0], N = cnt[1];
     const double P = std::round(2.0 * p->radius_obj * p->ratio_augimg);
-    c->st_bytes[ST_PEAKS] += na * HW * s_t + n2d * 16.0;
+    for (int f = 0; f < nf; ++f) {
+      int32_t np[OLPT_MAX_CAM], cnt[3];
+      const long o = (long)f * fs;
+      HIP_TRY(hipMemcpyAsync(np, fshift(w.npts, o), sizeof(np), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipMemcpyAsync(&cnt[0], fshift(w.sb.n_cand, o), 4, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipMemcpyAsync(&cnt[1], fshift(w.sb.n_obj, o), 4, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipMemcpyAsync(&cnt[2], fshift(w.n_keep, o), 4, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      double n2d = 0;
+      for (int k = 0; k < c->n_cam; ++k) n2d += np[k];
+      const double M = cnt[0], N = cnt[1];
+      c->st_bytes[ST_PEAKS] += na * HW * s_t + n2d * 16.0;
+      c->st_bytes[ST_BUILD] += n2d * 48.0 + na * ncell * 8.0 + M * (4.0 * nc + 8.0);
+      c->st_bytes[ST_PRUNE] += 2.0 * M * (4.0 * nc + 8.0) + 2.0 * M * na * 4.0 + N * (24.0 + 4.0 * nc);
+      c->st_bytes[ST_SHAKE] += p->n_shake_loop * (N * na * P * P * s_t + N * 48.0);
+      c->st_bytes[ST_RESID] += p->n_shake_loop * (na * HW * 2.0 * s_t) + nc * HW * 2.0 * s_t;
+    }
     c->st_launch[ST_PEAKS] += 1;
-    c->st_bytes[ST_BUILD] += n2d * 48.0 + na * ncell * 8.0 + M * (4.0 * nc + 8.0);
     c->st_launch[ST_BUILD] += 1;
-    c->st_bytes[ST_PRUNE] += 2.0 * M * (4.0 * nc + 8.0) + 2.0 * M * na * 4.0 + N * (24.0 + 4.0 * nc);
     c->st_launch[ST_PRUNE] += 1;
-    c->st_bytes[ST_SHAKE] += p->n_shake_loop * (N * na * P * P * s_t + N * 48.0);
     c->st_launch[ST_SHAKE] += p->n_shake_loop;
-    c->st_bytes[ST_RESID] += p->n_shake_loop * (na * HW * 2.0 * s_t) + nc * HW * 2.0 * s_t;
     c->st_launch[ST_RESID] += p->n_shake_loop + 1;
   }
   return OLPT_OK;
 }
 
-// IPR::runIPR on one frame already converted into img_a (T)
+// IPR::runIPR on the nf frames of a workspace batch, already converted into
+// img_a (T); frame f's outputs at out_xyz + f * cap_out objects, d_n_out[f]
+// and d_loop_tot[f * cap_loop + loop].
 template <typename T>
-olpt_status ipr_frame(olpt_ctx *c, Workspace &w, const olpt_ipr_params *p, double *out_xyz,
-                      double *out_uv, int cap_out, int32_t *d_n_out, int32_t *d_loop_tot,
-                      int cap_loop, int *n_loop_out) {
+olpt_status ipr_frame(olpt_ctx *c, Workspace &w, int nf, const olpt_ipr_params *p,
+                      double *out_xyz, double *out_uv, int cap_out, int32_t *d_n_out,
+                      int32_t *d_loop_tot, int cap_loop, int *n_loop_out) {
   const int n_cam = c->n_cam;
-  fill_i32(d_n_out, 1, 0, w.st);
+  fill_i32(d_n_out, 1, 0, w.st, nf, (long)sizeof(int32_t));
   int n_loop = 0;
   if (n_cam < 2) { // IPR.cpp:200-204
     *n_loop_out = 0;
@@ -558,8 +624,8 @@ olpt_status ipr_frame(olpt_ctx *c, Workspace &w, const olpt_ipr_params *p, doubl
       double tol = q.tol_2d_px;
       for (int loop = 0; loop < loops; ++loop) {
         int32_t *lt = (d_loop_tot && n_loop < cap_loop) ? d_loop_tot + n_loop : nullptr;
-        olpt_status s = ipr_loop<T>(c, w, img, nxt, act, &q, tol, out_xyz, out_uv, cap_out,
-                                    d_n_out, lt);
+        olpt_status s = ipr_loop<T>(c, w, nf, img, nxt, act, &q, tol, out_xyz, out_uv, cap_out,
+                                    d_n_out, lt, cap_loop);
         if (s) return s;
         std::swap(img, nxt);
         ++n_loop;
@@ -660,20 +726,29 @@ olpt_status olpt_set_precision(olpt_ctx *c, int prec) {
 
 namespace {
 
-// All buffers of one workspace, sized from the context's reserve.
+// Buffer offsets inside one frame slice of a workspace arena (256-byte
+// aligned); resolved to pointers once the arena is allocated.
+struct SliceLayout {
+  size_t off = 0;
+  std::vector<std::pair<void **, size_t>> at;
+  void add(void **p, size_t bytes) {
+    at.emplace_back(p, off);
+    off += (std::max<size_t>(bytes, 16) + 255) / 256 * 256;
+  }
+};
+
+// All buffers of one workspace, sized from the context's reserve, for
+// c->batch frames: one arena of identical frame slices.
 olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   const int H = c->H, W = c->W;
   const long HW = (long)H * W;
   const int nc = c->n_cam, cap = c->cap2d, M = c->cap_cand, O = c->cap_obj;
-  olpt_status s;
-#define A(ptr, T, n) if ((s = dalloc_t(c, &(ptr), (size_t)(n)))) return s
-  void *tmp;
-  if ((s = dalloc(c, &tmp, 8 * (size_t)nc * HW))) return s;
-  w.img_a = tmp;
-  if ((s = dalloc(c, &tmp, 8 * (size_t)nc * HW))) return s;
-  w.img_b = tmp;
-  if ((s = dalloc(c, &tmp, 8 * (size_t)nc * HW))) return s;
-  w.img_res = tmp;
+  const int F = c->batch;
+  SliceLayout L;
+#define A(ptr, T, n) L.add(reinterpret_cast<void **>(&(ptr)), sizeof(T) * (size_t)(n))
+  L.add(&w.img_a, 8 * (size_t)nc * HW);
+  L.add(&w.img_b, 8 * (size_t)nc * HW);
+  L.add(&w.img_res, 8 * (size_t)nc * HW);
   const long nchunk = peak_chunks(H, W);
   A(w.pk_mask, uint64_t, nchunk * nc);
   A(w.pk_cnt, int32_t, nchunk * nc);
@@ -758,8 +833,8 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   b.grid_build = c->tune.build_blocks;
   b.grid_prune = c->tune.prune_blocks;
   // shake
-  if ((s = dalloc(c, &w.sh_uv, 8 * (size_t)O * nc * 2))) return s;
-  if ((s = dalloc(c, &w.keep_uv, 8 * (size_t)O * nc * 2))) return s;
+  L.add(&w.sh_uv, 8 * (size_t)O * nc * 2);
+  L.add(&w.keep_uv, 8 * (size_t)O * nc * 2);
   A(w.sh_xyz, double, (long)O * 3);
   A(w.keep_xyz, double, (long)O * 3);
   A(w.sh_score, double, O);
@@ -779,26 +854,51 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   A(w.g_cellof, int32_t, O);
   A(w.has_nb, int32_t, O);
   A(w.sh_keep_off, int32_t, O);
-  if ((s = dalloc(c, &w.rec, shake_rec_bytes(true) * (size_t)O * nc))) return s;
+  L.add(&w.rec, shake_rec_bytes(true) * (size_t)O * nc);
   A(w.partial, double, 1024);
 #undef A
+  // the arena, then every buffer at its offset in frame 0's slice
+  w.fstride = (long)L.off;
+  w.nf_cap = F;
+  {
+    hipError_t e = hipMalloc(&w.arena, (size_t)F * L.off);
+    if (e != hipSuccess) {
+      w.arena = nullptr;
+      return fail(OLPT_E_ALLOC, "hipMalloc(" + std::to_string((size_t)F * L.off) +
+                                    ") for a workspace of " + std::to_string(F) +
+                                    " frames: " + hipGetErrorString(e));
+    }
+  }
+  for (auto &a : L.at) *a.first = (char *)w.arena + a.second;
+  // stereo views of the arena (pointers set above through the layout)
+  b.xy = w.xy;
+  b.npts = w.npts;
+  b.flags = w.flags;
+  b.scan_scratch = w.scan_scratch;
+  b.totals = w.totals;
+  b.nf = 1;
+  b.fstride = w.fstride;
   // device counters and flags start at zero (fresh allocations may hold
-  // another context's old values)
-  fill_i32(w.flags, 16, 0, nullptr);
-  fill_i32(w.npts, OLPT_MAX_CAM, 0, nullptr);
-  stereo_init_tables(b, nullptr);
+  // another context's old values), in every frame slice
+  const long fs = w.fstride;
+  fill_i32(w.flags, 16, 0, nullptr, F, fs);
+  fill_i32(w.npts, OLPT_MAX_CAM, 0, nullptr, F, fs);
+  StereoBuffers bf = b;
+  bf.nf = F;
+  stereo_init_tables(bf, nullptr);
   // count arrays of the count -> scan -> fill passes: zero here, and each
   // pass's scan leaves them zero again
-  fill_i32(w.bin_cnt, nbin * nc, 0, nullptr);
-  fill_i32(w.g_cnt, grid_cells, 0, nullptr);
+  fill_i32(w.bin_cnt, nbin * nc, 0, nullptr, F, fs);
+  fill_i32(w.g_cnt, grid_cells, 0, nullptr, F, fs);
   return OLPT_OK;
 }
 
 olpt_status alloc_learned(olpt_ctx *c, Workspace &w) {
   if (w.patches) return OLPT_OK;
-  olpt_status s;
-  if ((s = dalloc(c, &w.patches, (size_t)c->cap_obj * 512 * 2))) return s;
-  if ((s = dalloc_t(c, &w.prop, (size_t)c->cap_obj * 4))) return s;
+  w.lstride = (long)c->cap_obj * 512 * 2;
+  w.pstride = (long)c->cap_obj * 4 * 4;
+  HIP_TRY(hipMalloc(&w.patches, (size_t)w.nf_cap * w.lstride));
+  HIP_TRY(hipMalloc((void **)&w.prop, (size_t)w.nf_cap * w.pstride));
   return OLPT_OK;
 }
 
@@ -915,7 +1015,25 @@ olpt_status olpt_set_tuning(olpt_ctx *c, const olpt_tuning *t) {
       return fail(OLPT_E_INVALID_ARGUMENT, "block counts must be within [64, 65536]");
   if (t->n_streams < 1 || t->n_streams > kMaxWorkspaces)
     return fail(OLPT_E_INVALID_ARGUMENT, "n_streams must be within [1, 8]");
+  if (t->frames_per_batch < 1 || t->frames_per_batch > kMaxBatch)
+    return fail(OLPT_E_INVALID_ARGUMENT, "frames_per_batch must be within [1, 1024]");
   free_graphs(c);
+  if (t->frames_per_batch != c->batch) {
+    // workspaces hold that many frame slices: reallocate them
+    HIP_TRY(hipDeviceSynchronize());
+    free_workspace_memory(c);
+    c->batch = t->frames_per_batch;
+    c->n_ws_want = t->n_streams;
+    for (int k = 0; k < c->n_ws; ++k) {
+      Workspace &w = c->ws[k];
+      hipStream_t keep_st = w.st;
+      hipEvent_t keep_ev = w.done;
+      w = Workspace{};
+      w.st = keep_st;
+      w.done = keep_ev;
+    }
+    c->n_ws = 0;
+  }
   c->tune = *t;
   for (int k = 0; k < c->n_ws; ++k) {
     c->ws[k].sb.grid_build = t->build_blocks;
@@ -928,6 +1046,7 @@ olpt_status olpt_get_tuning(olpt_ctx *c, olpt_tuning *t) {
   if (!c || !t) return fail(OLPT_E_INVALID_ARGUMENT, "null argument");
   *t = c->tune;
   t->n_streams = c->n_ws_active;
+  t->frames_per_batch = c->batch;
   return OLPT_OK;
 }
 
@@ -1010,13 +1129,7 @@ olpt_status olpt_set_otf(olpt_ctx *c, const olpt_otf *o) {
 olpt_status olpt_diagnostics(olpt_ctx *c, int32_t out[8]) {
   if (!c || !c->n_ws) return fail(OLPT_E_INVALID_ARGUMENT, "no workspace");
   HIP_TRY(hipDeviceSynchronize());
-  for (int i = 0; i < 8; ++i) out[i] = 0;
-  for (int k = 0; k < c->n_ws; ++k) {
-    int32_t f[8];
-    HIP_TRY(hipMemcpy(f, c->ws[k].flags, sizeof(f), hipMemcpyDeviceToHost));
-    for (int i = 0; i < 8; ++i) out[i] = (i == 3) ? out[i] + f[i] : (out[i] | f[i]);
-  }
-  return OLPT_OK;
+  return read_flags(c, c->n_ws, c->st, out);
 }
 
 olpt_status olpt_find_peaks(olpt_ctx *c, const void *d_img, int dtype, int H, int W,
@@ -1027,7 +1140,7 @@ olpt_status olpt_find_peaks(olpt_ctx *c, const void *d_img, int dtype, int H, in
   if (H != c->H || W != c->W) return fail(OLPT_E_INVALID_ARGUMENT, "image size differs from reserve");
   CamList cl{1, {0}};
   hipStream_t st = c->st;
-  fill_i32(w.flags + 8, 8, 0, st);
+  fill_i32(w.flags, 16, 0, st, w.nf_cap, w.fstride);
   fill_i32(w.npts, OLPT_MAX_CAM, 0, st);
   const double mi = (double)p->min_obj_int;
   switch (dtype) {
@@ -1066,7 +1179,7 @@ olpt_status olpt_stereo_match(olpt_ctx *c, const int32_t *active, const double *
   HIP_TRY(hipMemcpyAsync(w.npts, np, sizeof(np), hipMemcpyHostToDevice, st));
   StereoParams sp = stereo_params(c, active, p, p->tol_2d_px, &s);
   if (s) return s;
-  fill_i32(w.flags, 16, 0, st);
+  fill_i32(w.flags, 16, 0, st, w.nf_cap, w.fstride);
   launch_stereo(w.sb, sp, kMaxRounds, st);
   HIP_TRY(hipGetLastError());
   int32_t h[3];
@@ -1146,7 +1259,7 @@ olpt_status olpt_prune_match(olpt_ctx *c, const int32_t *active, const int32_t *
     if (h_ids[i] < -1 || h_ids[i] >= n_pts[cam])
       return fail(OLPT_E_OUT_OF_RANGE, "candidate id outside its camera's 2D points");
   }
-  fill_i32(w.flags, 16, 0, st);
+  fill_i32(w.flags, 16, 0, st, w.nf_cap, w.fstride);
   if (nid) {
     HIP_TRY(hipMemcpyAsync(w.sb.cand_ids_s, d_cand_ids, sizeof(int32_t) * nid,
                            hipMemcpyDeviceToDevice, st));
@@ -1226,14 +1339,49 @@ olpt_status residual_impl(olpt_ctx *c, const void *d_imgs, int dtype, const olpt
   return OLPT_OK;
 }
 
+// Read and merge the 8 diagnostic flags of every frame slice of the first
+// nw workspaces (flags[3] is summed, the others or-ed).
+olpt_status read_flags(olpt_ctx *c, int nw, hipStream_t st, int32_t out[8]) {
+  for (int i = 0; i < 8; ++i) out[i] = 0;
+  std::vector<int32_t> fw;
+  for (int k = 0; k < nw; ++k) {
+    const Workspace &w = c->ws[k];
+    if (!w.arena) continue;
+    fw.assign(8 * (size_t)w.nf_cap, 0);
+    HIP_TRY(hipMemcpy2DAsync(fw.data(), 8 * sizeof(int32_t), w.flags, (size_t)w.fstride,
+                             8 * sizeof(int32_t), (size_t)w.nf_cap, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int f = 0; f < w.nf_cap; ++f)
+      for (int i = 0; i < 8; ++i)
+        out[i] = i == 3 ? out[i] + fw[8 * f + i] : (out[i] | fw[8 * f + i]);
+  }
+  return OLPT_OK;
+}
+
+// Frames of one call: chunks of up to c->batch consecutive frames, chunk j
+// on workspace j % nw (each workspace's chunks in order on its stream).
+template <typename T>
+olpt_status run_chunk(olpt_ctx *c, Workspace &w, int f0, int nb, const void *d_imgs, int dtype,
+                      const olpt_ipr_params *p, double *d_obj_xyz, double *d_obj_uv, int cap,
+                      int32_t *d_n_obj, int32_t *d_loop_tot, int cap_loop, int *n_loop) {
+  const long n = (long)c->n_cam * c->H * c->W;
+  const size_t in_bytes = dtype == OLPT_U8 ? 1 : dtype == OLPT_U16 ? 2 : dtype == OLPT_F32 ? 4 : 8;
+  const char *src = (const char *)d_imgs + (size_t)f0 * n * in_bytes;
+  olpt_status s = convert_image<T>(src, dtype, n, (T *)w.img_a, w.st, nb, (long)(n * in_bytes),
+                                   w.fstride);
+  if (s) return s;
+  return ipr_frame<T>(c, w, nb, p, d_obj_xyz + (size_t)f0 * cap * 3,
+                      d_obj_uv ? d_obj_uv + (size_t)f0 * cap * c->n_cam * 2 : nullptr, cap,
+                      d_n_obj + f0, d_loop_tot ? d_loop_tot + (size_t)f0 * cap_loop : nullptr,
+                      cap_loop, n_loop);
+}
+
 template <typename T>
 olpt_status ipr_run_impl(olpt_ctx *c, const void *d_imgs, int dtype, int n_frames,
                          const olpt_ipr_params *p, double *d_obj_xyz, double *d_obj_uv, int cap,
                          int32_t *d_n_obj, int32_t *loop_total, int cap_loop, bool sync) {
   hipStream_t st = c->st;
   TimerScope ts(c, !sync);
-  const long n = (long)c->n_cam * c->H * c->W;
-  const size_t in_bytes = dtype == OLPT_U8 ? 1 : dtype == OLPT_U16 ? 2 : dtype == OLPT_F32 ? 4 : 8;
   if (loop_total) {
     const int need = std::max(1, n_frames * cap_loop);
     if (c->loop_tot_cap < need) {
@@ -1242,29 +1390,27 @@ olpt_status ipr_run_impl(olpt_ctx *c, const void *d_imgs, int dtype, int n_frame
       c->loop_tot_cap = need;
     }
   }
-  // frames round-robin over the workspaces; the other streams start after
+  const int B = std::max(1, c->batch);
+  const int n_chunks = (n_frames + B - 1) / B;
+  // chunks round-robin over the workspaces; the other streams start after
   // the work already queued on the context stream and join back at the end
-  const int nw = std::max(1, std::min({c->n_ws_active, c->n_ws, n_frames}));
+  const int nw = std::max(1, std::min({c->n_ws_active, c->n_ws, n_chunks}));
+  // loop totals past a frame's last loop read 0 (before the fork: every
+  // stream writes into them)
+  if (loop_total) fill_i32(c->loop_tot, (long)n_frames * cap_loop, 0, st);
   if (nw > 1) {
     HIP_TRY(hipEventRecord(c->ev_fork, st));
     for (int k = 1; k < nw; ++k) HIP_TRY(hipStreamWaitEvent(c->ws[k].st, c->ev_fork, 0));
   }
-  for (int k = 0; k < nw; ++k) fill_i32(c->ws[k].flags, 16, 0, c->ws[k].st);
-  // frame f goes to workspace f % nw, in frame order on that stream
-  auto run_frame = [&](int f, int *n_loop) -> olpt_status {
-    Workspace &w = c->ws[f % nw];
-    const char *src = (const char *)d_imgs + (size_t)f * n * in_bytes;
-    olpt_status s = convert_image<T>(src, dtype, n, (T *)w.img_a, w.st);
-    if (s) return s;
-    return ipr_frame<T>(c, w, p, d_obj_xyz + (size_t)f * cap * 3,
-                        d_obj_uv ? d_obj_uv + (size_t)f * cap * c->n_cam * 2 : nullptr, cap,
-                        d_n_obj + f, loop_total ? c->loop_tot + (size_t)f * cap_loop : nullptr,
-                        cap_loop, n_loop);
-  };
+  for (int k = 0; k < nw; ++k)
+    fill_i32(c->ws[k].flags, 16, 0, c->ws[k].st, c->ws[k].nf_cap, c->ws[k].fstride);
   int n_loop = 0;
   const auto t_issue = std::chrono::steady_clock::now();
-  for (int f = 0; f < n_frames; ++f) {
-    olpt_status s = run_frame(f, &n_loop);
+  for (int j = 0; j < n_chunks; ++j) {
+    const int f0 = j * B, nb = std::min(B, n_frames - f0);
+    olpt_status s = run_chunk<T>(c, c->ws[j % nw], f0, nb, d_imgs, dtype, p, d_obj_xyz, d_obj_uv,
+                                 cap, d_n_obj, loop_total ? c->loop_tot : nullptr, cap_loop,
+                                 &n_loop);
     if (s) return s;
   }
   if (nw > 1) {
@@ -1276,18 +1422,13 @@ olpt_status ipr_run_impl(olpt_ctx *c, const void *d_imgs, int dtype, int n_frame
   if (!sync) return OLPT_OK;
   c->host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_issue).count();
   c->host_calls += 1;
-  int32_t fl[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  std::vector<int32_t> fw(8 * nw);
-  for (int k = 0; k < nw; ++k)
-    HIP_TRY(hipMemcpyAsync(fw.data() + 8 * k, c->ws[k].flags, 8 * sizeof(int32_t),
-                           hipMemcpyDeviceToHost, st));
   if (loop_total) {
     HIP_TRY(hipMemcpyAsync(loop_total, c->loop_tot, sizeof(int32_t) * (size_t)n_frames * cap_loop,
                            hipMemcpyDeviceToHost, st));
   }
-  HIP_TRY(hipStreamSynchronize(st));
-  for (int k = 0; k < nw; ++k)
-    for (int i = 0; i < 8; ++i) fl[i] |= fw[8 * k + i];
+  int32_t fl[8];
+  olpt_status s = read_flags(c, nw, st, fl);
+  if (s) return s;
   if (fl[7]) return fail(OLPT_E_CAPACITY, "2D peak capacity exceeded");
   if (fl[0]) return fail(OLPT_E_CAPACITY, "stereo candidate capacity exceeded");
   if (fl[1]) return fail(OLPT_E_CAPACITY, "stereo build list capacity exceeded");
@@ -1298,25 +1439,22 @@ olpt_status ipr_run_impl(olpt_ctx *c, const void *d_imgs, int dtype, int n_frame
   return OLPT_OK;
 }
 
-// The frames of workspace k (f = k, k + nw, ...) on its stream, for capture:
+// The chunks of workspace k (j = k, k + nw, ...) on its stream, for capture:
 // the same kernel sequence olpt_ipr_run issues for them, without host reads.
 template <typename T>
 olpt_status capture_slot(olpt_ctx *c, int k, int nw, const void *d_imgs, int dtype, int n_frames,
                          const olpt_ipr_params *p, double *d_obj_xyz, double *d_obj_uv, int cap,
                          int32_t *d_n_obj) {
   TimerScope ts(c, true);
-  const long n = (long)c->n_cam * c->H * c->W;
-  const size_t in_bytes = dtype == OLPT_U8 ? 1 : dtype == OLPT_U16 ? 2 : dtype == OLPT_F32 ? 4 : 8;
   Workspace &w = c->ws[k];
-  fill_i32(w.flags, 16, 0, w.st);
+  fill_i32(w.flags, 16, 0, w.st, w.nf_cap, w.fstride);
+  const int B = std::max(1, c->batch);
+  const int n_chunks = (n_frames + B - 1) / B;
   int n_loop = 0;
-  for (int f = k; f < n_frames; f += nw) {
-    const char *src = (const char *)d_imgs + (size_t)f * n * in_bytes;
-    olpt_status s = convert_image<T>(src, dtype, n, (T *)w.img_a, w.st);
-    if (s) return s;
-    s = ipr_frame<T>(c, w, p, d_obj_xyz + (size_t)f * cap * 3,
-                     d_obj_uv ? d_obj_uv + (size_t)f * cap * c->n_cam * 2 : nullptr, cap,
-                     d_n_obj + f, nullptr, 0, &n_loop);
+  for (int j = k; j < n_chunks; j += nw) {
+    const int f0 = j * B, nb = std::min(B, n_frames - f0);
+    olpt_status s = run_chunk<T>(c, w, f0, nb, d_imgs, dtype, p, d_obj_xyz, d_obj_uv, cap, d_n_obj,
+                                 nullptr, 0, &n_loop);
     if (s) return s;
   }
   return OLPT_OK;
@@ -1384,7 +1522,8 @@ olpt_status olpt_ipr_graph_capture(olpt_ctx *c, const void *d_imgs, int dtype, i
   // graphs are launched on their own streams, so frames overlap as in the
   // eager run while the host submits one graph instead of ~1000 kernels
   // per frame.
-  const int nw = std::max(1, std::min({c->n_ws_active, c->n_ws, n_frames}));
+  const int n_chunks = (n_frames + std::max(1, c->batch) - 1) / std::max(1, c->batch);
+  const int nw = std::max(1, std::min({c->n_ws_active, c->n_ws, n_chunks}));
   for (int k = 0; k < nw; ++k) HIP_TRY(hipStreamSynchronize(c->ws[k].st));
   c->capturing = true;
   for (int k = 0; k < nw; ++k) {

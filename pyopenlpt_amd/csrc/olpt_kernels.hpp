@@ -38,6 +38,14 @@ inline void stage_end(int slot, hipStream_t st) {
 }
 enum { ST_PEAKS = 0, ST_BUILD = 1, ST_PRUNE = 2, ST_SHAKE = 3, ST_RESID = 4, ST_OTHER = 5 };
 
+// Frames of one batch share a workspace layout: frame f's buffers lie f *
+// fstride bytes after frame 0's (one arena per workspace), so a kernel
+// launched over a batch takes its frame from blockIdx.z and shifts the
+// workspace pointers it was given for frame 0.  Null stays null.
+template <typename P> __host__ __device__ __forceinline__ P *fshift(P *p, long off) {
+  return p ? (P *)((char *)p + off) : p;
+}
+
 // Which cameras a stage works on (host-side list, passed by value).
 struct CamList {
   int32_t n;
@@ -47,20 +55,23 @@ struct CamList {
 // ---------------- peaks (ObjectFinder2D::findTracer2D) --------------------
 // img: [n_cam][H][W] (element stride cam_stride); output xy as (x,y) double
 // pairs at xy[(cam*cap + i)*2]; npts[cam] = min(count, cap); overflow[0] is
-// set to 1 if any camera had more than cap peaks.  Work buffers:
+// set to 1 if any camera had more than cap peaks.  Over nf frames every
+// pointer (img included) is frame 0's and moves fstride bytes per frame.  Work buffers:
 // mask/cnt/off hold (H-2)*ceil((W-2)/64) entries per listed camera.
 template <typename TIn>
 void launch_find_peaks(const TIn *img, long cam_stride, int H, int W,
                        double min_int, CamList cams, uint64_t *mask,
                        int32_t *cnt, int32_t *off, int32_t *scan_scratch,
                        int32_t *totals, double *xy, int cap, int32_t *npts,
-                       int32_t *overflow, hipStream_t st);
+                       int32_t *overflow, hipStream_t st, int nf = 1,
+                       long fstride = 0);
 
 // limit2DObjectNumber (IPR.cpp:19-39) on the peak lists of `cams` (xy and
 // npts as launch_find_peaks writes them; a no-op unless 0 < cap < cap2d).
 // Work buffers: perm [n_cam][cap2d] int32, lim [n_cam], tmp [n_cam][cap2d][2].
 void launch_limit2d(double *xy, int32_t *npts, CamList cams, int cap, int cap2d,
-                    int32_t *perm, int32_t *lim, double *tmp, hipStream_t st);
+                    int32_t *perm, int32_t *lim, double *tmp, hipStream_t st, int nf = 1,
+                    long fstride = 0);
 
 inline long peak_chunks(int H, int W) {
   return (long)(H > 2 ? H - 2 : 0) * ((W - 2 + 63) / 64);
@@ -150,8 +161,68 @@ struct StereoBuffers {
   int32_t *work;         // dynamic work counter
   int32_t *scan_scratch;
   int32_t *totals;       // small scratch for scan totals
-  int grid_build;        // workgroups of the build kernels
-  int grid_prune;        // cap on workgroups of the sort/prune kernels
+  int grid_build;        // workgroups of the build kernels (per frame)
+  int grid_prune;        // cap on workgroups of the sort/prune kernels (per frame)
+  int nf;                // frames of this batch (grid z of every launch)
+  long fstride;          // bytes between consecutive frames' buffers
+  // the buffers of frame f (cameras are shared)
+  __host__ __device__ StereoBuffers frame(long f) const {
+    StereoBuffers o = *this;
+    const long d = f * fstride;
+    o.xy = fshift(xy, d);
+    o.npts = fshift(npts, d);
+    o.los = fshift(los, d);
+    o.lines = fshift(lines, d);
+    o.t1 = fshift(t1, d);
+    o.pairs = fshift(pairs, d);
+    o.n_pairs = fshift(n_pairs, d);
+    o.surv = fshift(surv, d);
+    o.n_surv = fshift(n_surv, d);
+    o.cell_cnt = fshift(cell_cnt, d);
+    o.cell_start = fshift(cell_start, d);
+    o.cell_pid = fshift(cell_pid, d);
+    o.pt_cell = fshift(pt_cell, d);
+    o.pt_rank = fshift(pt_rank, d);
+    o.cand_ids = fshift(cand_ids, d);
+    o.cand_e = fshift(cand_e, d);
+    o.n_cand = fshift(n_cand, d);
+    o.cand_ids_s = fshift(cand_ids_s, d);
+    o.cand_e_s = fshift(cand_e_s, d);
+    o.key_cnt = fshift(key_cnt, d);
+    o.key_start = fshift(key_start, d);
+    o.cand_rank = fshift(cand_rank, d);
+    o.bucket_idx = fshift(bucket_idx, d);
+    o.pt_count = fshift(pt_count, d);
+    o.pt_start = fshift(pt_start, d);
+    o.pt_list = fshift(pt_list, d);
+    o.pt_of = fshift(pt_of, d);
+    o.pt_sorted = fshift(pt_sorted, d);
+    o.pt_occ = fshift(pt_occ, d);
+    o.ppos = fshift(ppos, d);
+    o.pt_flag = fshift(pt_flag, d);
+    o.head = fshift(head, d);
+    o.n_entries = fshift(n_entries, d);
+    o.flat_rank = fshift(flat_rank, d);
+    o.pv = fshift(pv, d);
+    o.pct = fshift(pct, d);
+    o.state = fshift(state, d);
+    o.undecided = fshift(undecided, d);
+    o.keep = fshift(keep, d);
+    o.keep_off = fshift(keep_off, d);
+    o.tri_xyz = fshift(tri_xyz, d);
+    o.obj_xyz = fshift(obj_xyz, d);
+    o.obj_ids = fshift(obj_ids, d);
+    o.n_obj = fshift(n_obj, d);
+    o.n_sel = fshift(n_sel, d);
+    o.hz_key = fshift(hz_key, d);
+    o.hz_val = fshift(hz_val, d);
+    o.hz_slot = fshift(hz_slot, d);
+    o.flags = fshift(flags, d);
+    o.work = fshift(work, d);
+    o.scan_scratch = fshift(scan_scratch, d);
+    o.totals = fshift(totals, d);
+    return o;
+  }
 };
 
 struct StereoParams {
@@ -211,7 +282,45 @@ template <typename T> struct ShakeBuffers {
   // learned proposal (optional)
   void *patches;      // [cap_obj][512] bf16 network input
   float *prop;        // [cap_obj][4] network output
-  int grid_shake;     // one-wave workgroups of the shake/proposal kernels
+  int grid_shake;     // one-wave workgroups of the shake/proposal kernels (per frame)
+  int nf;             // frames of this batch (grid z of every launch)
+  long fstride;       // bytes between consecutive frames' workspace buffers
+  long lstride;       // bytes between consecutive frames' patches
+  long pstride;       // bytes between consecutive frames' network outputs
+  // the buffers of frame f (cameras and OTF are shared)
+  __host__ __device__ ShakeBuffers frame(long f) const {
+    ShakeBuffers o = *this;
+    const long d = f * fstride;
+    o.img = fshift(img, d);
+    o.res = fshift(res, d);
+    o.img_next = fshift(img_next, d);
+    o.xyz = fshift(xyz, d);
+    o.uv = fshift(uv, d);
+    o.score = fshift(score, d);
+    o.flag = fshift(flag, d);
+    o.n_obj = fshift(n_obj, d);
+    o.bin_cnt = fshift(bin_cnt, d);
+    o.bin_start = fshift(bin_start, d);
+    o.bin_obj = fshift(bin_obj, d);
+    o.obj_bin = fshift(obj_bin, d);
+    o.obj_rank = fshift(obj_rank, d);
+    o.g_cnt = fshift(g_cnt, d);
+    o.g_start = fshift(g_start, d);
+    o.g_obj = fshift(g_obj, d);
+    o.g_rank = fshift(g_rank, d);
+    o.g_cellof = fshift(g_cellof, d);
+    o.has_nb = fshift(has_nb, d);
+    o.rec = fshift(rec, d);
+    o.partial = fshift(partial, d);
+    o.keep_off = fshift(keep_off, d);
+    o.n_keep = fshift(n_keep, d);
+    o.scan_scratch = fshift(scan_scratch, d);
+    o.totals = fshift(totals, d);
+    o.work = fshift(work, d);
+    o.patches = fshift(patches, f * lstride);
+    o.prop = fshift(prop, f * pstride);
+    return o;
+  }
 };
 
 struct ShakeParams {
@@ -254,9 +363,10 @@ void launch_patches(const ShakeBuffers<T> &b, const ShakeParams &p, void *X, hip
 template <typename T>
 void launch_render(const ShakeBuffers<T> &b, const double *xyz, const double *amp,
                    const int32_t *n_ptr, double r_px, T *out, hipStream_t st);
-// MLP forward on a device-resident row count (olpt_mlp.hip).
+// MLP forward on a device-resident row count (olpt_mlp.hip), over nf frames
+// whose inputs, counts and outputs lie xs, ns, ys bytes apart.
 void mlp_infer_device(const void *mlp, const void *X, const int32_t *d_n, int cap, float *Y,
-                      hipStream_t st);
+                      hipStream_t st, int nf = 1, long xs = 0, long ns = 0, long ys = 0);
 
 template <typename T>
 void launch_convert_image(const void *src, int dtype, long n, T *dst,

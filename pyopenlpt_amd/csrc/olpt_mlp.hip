@@ -104,8 +104,14 @@ struct TrainBuf {
 // 64-row tiles.
 template <bool TRAIN>
 __global__ void __launch_bounds__(256) k_fwd(Net net, const bf16 *X, int n_host, const int32_t *n_dev,
-                                             float *Y, TrainBuf tb) {
+                                             float *Y, TrainBuf tb, long xs = 0, long ns = 0,
+                                             long ys = 0) {
   __shared__ bf16 hA[kRows * kLdsLd], hB[kRows * kLdsLd];
+  // inference over a batch of frames: frame blockIdx.z's rows, count and
+  // outputs lie xs, ns and ys bytes per frame after frame 0's
+  X = (const bf16 *)((const char *)X + (long)blockIdx.z * xs);
+  if (n_dev) n_dev = (const int32_t *)((const char *)n_dev + (long)blockIdx.z * ns);
+  Y = (float *)((char *)Y + (long)blockIdx.z * ys);
   const int n = n_dev ? *n_dev : n_host;
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
   for (int m0 = blockIdx.x * kRows; m0 < n; m0 += gridDim.x * kRows) {
@@ -351,13 +357,14 @@ namespace olpt {
 // Forward pass for rows counted on the device (inside an IPR run, on the
 // caller's stream; inference only reads the weights).
 void mlp_infer_device(const void *mlp, const void *X, const int32_t *d_n, int cap, float *Y,
-                      hipStream_t st) {
+                      hipStream_t st, int nf, long xs, long ns, long ys) {
   const olpt_mlp *m = static_cast<const olpt_mlp *>(mlp);
   int blocks = (cap + kRows - 1) / kRows;
   if (blocks > 1024) blocks = 1024;
   if (blocks < 1) blocks = 1;
   TrainBuf none{};
-  k_fwd<false><<<blocks, 256, 0, st>>>(net_of(m), (const bf16 *)X, 0, d_n, Y, none);
+  k_fwd<false><<<dim3(blocks, 1, nf), 256, 0, st>>>(net_of(m), (const bf16 *)X, 0, d_n, Y, none,
+                                                    xs, ns, ys);
 }
 }  // namespace olpt
 

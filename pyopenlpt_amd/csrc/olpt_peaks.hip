@@ -113,8 +113,12 @@ template <typename TIn>
 __global__ void __launch_bounds__(kPeakWaves * kWave)
 k_peak_mask(const TIn *__restrict__ img, long cam_stride, int H, int W,
             double min_int, CamList cams, uint64_t *__restrict__ mask,
-            int32_t *__restrict__ cnt, long nchunk) {
+            int32_t *__restrict__ cnt, long nchunk, long fstride) {
   __shared__ unsigned long long s_vm[kPeakWaves][kPeakGroup];
+  const long fo = (long)blockIdx.z * fstride;
+  img = fshift(img, fo);
+  mask = fshift(mask, fo);
+  cnt = fshift(cnt, fo);
   const int s = blockIdx.y;
   const int wv = threadIdx.x / kWave;
   const int lane = lane_id();
@@ -176,7 +180,12 @@ __global__ void __launch_bounds__(kPeakWaves * kWave)
 k_peak_emit(const TIn *__restrict__ img, long cam_stride, int H, int W,
             CamList cams, const uint64_t *__restrict__ mask,
             const int32_t *__restrict__ off, long nchunk,
-            double *__restrict__ xy, int cap) {
+            double *__restrict__ xy, int cap, long fstride) {
+  const long fo = (long)blockIdx.z * fstride;
+  img = fshift(img, fo);
+  mask = fshift(mask, fo);
+  off = fshift(off, fo);
+  xy = fshift(xy, fo);
   const int s = blockIdx.y;
   const int wv = threadIdx.x / kWave;
   const int lane = lane_id();
@@ -224,7 +233,11 @@ k_peak_emit(const TIn *__restrict__ img, long cam_stride, int H, int W,
 // Per-camera counts and capacity check.
 __global__ void k_peak_finish(CamList cams, const int32_t *__restrict__ totals,
                               int cap, int32_t *__restrict__ npts,
-                              int32_t *__restrict__ overflow) {
+                              int32_t *__restrict__ overflow, long fstride) {
+  const long fo = (long)blockIdx.z * fstride;
+  totals = fshift(totals, fo);
+  npts = fshift(npts, fo);
+  overflow = fshift(overflow, fo);
   const int s = threadIdx.x;
   if (s >= cams.n) return;
   const int t = totals[s];
@@ -237,20 +250,20 @@ void launch_find_peaks(const TIn *img, long cam_stride, int H, int W,
                        double min_int, CamList cams, uint64_t *mask,
                        int32_t *cnt, int32_t *off, int32_t *scan_scratch,
                        int32_t *totals, double *xy, int cap, int32_t *npts,
-                       int32_t *overflow, hipStream_t st) {
+                       int32_t *overflow, hipStream_t st, int nf, long fstride) {
   const long nchunk = peak_chunks(H, W);
-  if (nchunk <= 0 || cams.n <= 0) return;
+  if (nchunk <= 0 || cams.n <= 0 || nf <= 0) return;
   const int cpr = (W - 2 + 63) / 64;
   const long ngroup = (long)(H - 2) * ((cpr + kPeakGroup - 1) / kPeakGroup);
-  dim3 g((unsigned)((ngroup + kPeakWaves - 1) / kPeakWaves), (unsigned)cams.n);
+  dim3 g((unsigned)((ngroup + kPeakWaves - 1) / kPeakWaves), (unsigned)cams.n, (unsigned)nf);
   stage_begin(ST_PEAKS, st);
   k_peak_mask<TIn><<<g, kPeakWaves * kWave, 0, st>>>(img, cam_stride, H, W,
                                                      min_int, cams, mask, cnt,
-                                                     nchunk);
-  scan_exclusive(cnt, off, nchunk, cams.n, nchunk, totals, scan_scratch, st);
+                                                     nchunk, fstride);
+  scan_exclusive(cnt, off, nchunk, cams.n, nchunk, totals, scan_scratch, st, nf, fstride);
   k_peak_emit<TIn><<<g, kPeakWaves * kWave, 0, st>>>(img, cam_stride, H, W, cams,
-                                                     mask, off, nchunk, xy, cap);
-  k_peak_finish<<<1, 64, 0, st>>>(cams, totals, cap, npts, overflow);
+                                                     mask, off, nchunk, xy, cap, fstride);
+  k_peak_finish<<<dim3(1, 1, nf), 64, 0, st>>>(cams, totals, cap, npts, overflow, fstride);
   stage_end(ST_PEAKS, st);
 }
 
@@ -260,8 +273,12 @@ void launch_find_peaks(const TIn *img, long cam_stride, int H, int W,
 // thread computes each camera's permutation (it only runs when a list is
 // over the cap); a grid gathers the kept points.
 __global__ void k_limit_perm(const int32_t *npts, CamList cams, int cap, int cap2d,
-                             int32_t *perm, int32_t *lim) {
+                             int32_t *perm, int32_t *lim, long fstride) {
   if (threadIdx.x || blockIdx.x) return;
+  const long fo = (long)blockIdx.z * fstride;
+  npts = fshift(npts, fo);
+  perm = fshift(perm, fo);
+  lim = fshift(lim, fo);
   Minstd0 g{1234};
   for (int q = 0; q < cams.n; ++q) {
     const int k = cams.id[q];
@@ -277,7 +294,12 @@ __global__ void k_limit_perm(const int32_t *npts, CamList cams, int cap, int cap
 
 __global__ void k_limit_gather(const double *__restrict__ xy, const int32_t *__restrict__ perm,
                                const int32_t *__restrict__ lim, CamList cams, int cap, int cap2d,
-                               double *__restrict__ tmp) {
+                               double *__restrict__ tmp, long fstride) {
+  const long fo = (long)blockIdx.z * fstride;
+  xy = fshift(xy, fo);
+  perm = fshift(perm, fo);
+  lim = fshift(lim, fo);
+  tmp = fshift(tmp, fo);
   const int k = cams.id[blockIdx.y];
   if (!lim[k]) return;
   const long o = (long)k * cap2d;
@@ -290,7 +312,12 @@ __global__ void k_limit_gather(const double *__restrict__ xy, const int32_t *__r
 
 __global__ void k_limit_store(double *__restrict__ xy, int32_t *__restrict__ npts,
                               const int32_t *__restrict__ lim, CamList cams, int cap, int cap2d,
-                              const double *__restrict__ tmp) {
+                              const double *__restrict__ tmp, long fstride) {
+  const long fo = (long)blockIdx.z * fstride;
+  xy = fshift(xy, fo);
+  npts = fshift(npts, fo);
+  lim = fshift(lim, fo);
+  tmp = fshift(tmp, fo);
   const int k = cams.id[blockIdx.y];
   if (!lim[k]) return;
   const long o = (long)k * cap2d;
@@ -302,12 +329,13 @@ __global__ void k_limit_store(double *__restrict__ xy, int32_t *__restrict__ npt
 }
 
 void launch_limit2d(double *xy, int32_t *npts, CamList cams, int cap, int cap2d,
-                    int32_t *perm, int32_t *lim, double *tmp, hipStream_t st) {
-  if (cams.n <= 0 || cap <= 0 || cap >= cap2d) return;
-  k_limit_perm<<<1, 64, 0, st>>>(npts, cams, cap, cap2d, perm, lim);
-  dim3 g((unsigned)std::min(1024, (cap + 255) / 256), (unsigned)cams.n);
-  k_limit_gather<<<g, 256, 0, st>>>(xy, perm, lim, cams, cap, cap2d, tmp);
-  k_limit_store<<<g, 256, 0, st>>>(xy, npts, lim, cams, cap, cap2d, tmp);
+                    int32_t *perm, int32_t *lim, double *tmp, hipStream_t st, int nf,
+                    long fstride) {
+  if (cams.n <= 0 || cap <= 0 || cap >= cap2d || nf <= 0) return;
+  k_limit_perm<<<dim3(1, 1, nf), 64, 0, st>>>(npts, cams, cap, cap2d, perm, lim, fstride);
+  dim3 g((unsigned)std::min(1024, (cap + 255) / 256), (unsigned)cams.n, (unsigned)nf);
+  k_limit_gather<<<g, 256, 0, st>>>(xy, perm, lim, cams, cap, cap2d, tmp, fstride);
+  k_limit_store<<<g, 256, 0, st>>>(xy, npts, lim, cams, cap, cap2d, tmp, fstride);
 }
 
 #define INST(T)                                                                \
@@ -315,7 +343,7 @@ void launch_limit2d(double *xy, int32_t *npts, CamList cams, int cap, int cap2d,
                                      CamList, uint64_t *, int32_t *,           \
                                      int32_t *, int32_t *, int32_t *,          \
                                      double *, int, int32_t *, int32_t *,      \
-                                     hipStream_t);
+                                     hipStream_t, int, long);
 INST(uint8_t)
 INST(uint16_t)
 INST(float)
@@ -422,10 +450,10 @@ extern "C" olpt_status olpt_find_peaks_static(void *stream, const void *d_img, i
   double *xy = d_xy;
   const int c = cap;
   switch (dtype) {
-  case OLPT_U8: launch_find_peaks<uint8_t>((const uint8_t *)d_img, 0, n_row, n_col, mi, cl, mask, cnt, off, scan, totals, xy, c, npts, flags, st); break;
-  case OLPT_U16: launch_find_peaks<uint16_t>((const uint16_t *)d_img, 0, n_row, n_col, mi, cl, mask, cnt, off, scan, totals, xy, c, npts, flags, st); break;
-  case OLPT_F32: launch_find_peaks<float>((const float *)d_img, 0, n_row, n_col, mi, cl, mask, cnt, off, scan, totals, xy, c, npts, flags, st); break;
-  default: launch_find_peaks<double>((const double *)d_img, 0, n_row, n_col, mi, cl, mask, cnt, off, scan, totals, xy, c, npts, flags, st); break;
+  case OLPT_U8: launch_find_peaks<uint8_t>((const uint8_t *)d_img, 0, n_row, n_col, mi, cl, mask, cnt, off, scan, totals, xy, c, npts, flags, st, 1, 0); break;
+  case OLPT_U16: launch_find_peaks<uint16_t>((const uint16_t *)d_img, 0, n_row, n_col, mi, cl, mask, cnt, off, scan, totals, xy, c, npts, flags, st, 1, 0); break;
+  case OLPT_F32: launch_find_peaks<float>((const float *)d_img, 0, n_row, n_col, mi, cl, mask, cnt, off, scan, totals, xy, c, npts, flags, st, 1, 0); break;
+  default: launch_find_peaks<double>((const double *)d_img, 0, n_row, n_col, mi, cl, mask, cnt, off, scan, totals, xy, c, npts, flags, st, 1, 0); break;
   }
   k_static_count<<<1, 64, 0, st>>>(totals, d_n_out);
   e = hipGetLastError();

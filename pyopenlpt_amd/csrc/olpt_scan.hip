@@ -3,6 +3,7 @@
 // 256-thread workgroup (16 contiguous ints per thread, wave-shuffle scan of the
 // per-thread partials), so a 4M-entry array is 1024 tiles.
 #include "olpt_scan.hpp"
+#include "olpt_kernels.hpp"
 
 namespace olpt {
 
@@ -35,8 +36,11 @@ __device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t *lds,
 
 __global__ void __launch_bounds__(kScanThreads)
 k_tile_sums(const int32_t *__restrict__ in, long n, long stride,
-            int32_t *__restrict__ tsum, long tiles) {
+            int32_t *__restrict__ tsum, long tiles, long fstride) {
   __shared__ int32_t lds[kScanWaves];
+  const long fo = (long)blockIdx.z * fstride;
+  in = fshift(in, fo);
+  tsum = fshift(tsum, fo);
   const int seg = blockIdx.y;
   const long tile = blockIdx.x;
   const int32_t *p = in + seg * stride;
@@ -55,8 +59,11 @@ k_tile_sums(const int32_t *__restrict__ in, long n, long stride,
 // one workgroup per segment scans its tile sums in place
 __global__ void __launch_bounds__(kScanThreads)
 k_scan_tile_sums(int32_t *__restrict__ tsum, long tiles,
-                 int32_t *__restrict__ total) {
+                 int32_t *__restrict__ total, long fstride) {
   __shared__ int32_t lds[kScanWaves];
+  const long fo = (long)blockIdx.z * fstride;
+  tsum = fshift(tsum, fo);
+  total = fshift(total, fo);
   const int seg = blockIdx.x;
   int32_t *p = tsum + seg * (tiles + 1);
   int32_t carry = 0;
@@ -77,8 +84,12 @@ k_scan_tile_sums(int32_t *__restrict__ tsum, long tiles,
 template <bool CLEAR>
 __global__ void __launch_bounds__(kScanThreads)
 k_tile_scan(int32_t *__restrict__ in, int32_t *__restrict__ out, long n,
-            long stride, const int32_t *__restrict__ tsum, long tiles) {
+            long stride, const int32_t *__restrict__ tsum, long tiles, long fstride) {
   __shared__ int32_t lds[kScanWaves];
+  const long fo = (long)blockIdx.z * fstride;
+  in = fshift(in, fo);
+  out = fshift(out, fo);
+  tsum = fshift(tsum, fo);
   const int seg = blockIdx.y;
   const long tile = blockIdx.x;
   int32_t *p = in + seg * stride;
@@ -109,24 +120,25 @@ k_tile_scan(int32_t *__restrict__ in, int32_t *__restrict__ out, long n,
 
 void scan_exclusive(const int32_t *in, int32_t *out, long n, int segments,
                     long seg_stride, int32_t *total, int32_t *scratch,
-                    hipStream_t st) {
-  if (n <= 0 || segments <= 0) return;
+                    hipStream_t st, int nf, long fstride) {
+  if (n <= 0 || segments <= 0 || nf <= 0) return;
   const long tiles = (n + kScanTile - 1) / kScanTile;
-  dim3 g((unsigned)tiles, (unsigned)segments);
-  k_tile_sums<<<g, kScanThreads, 0, st>>>(in, n, seg_stride, scratch, tiles);
-  k_scan_tile_sums<<<segments, kScanThreads, 0, st>>>(scratch, tiles, total);
+  dim3 g((unsigned)tiles, (unsigned)segments, (unsigned)nf);
+  k_tile_sums<<<g, kScanThreads, 0, st>>>(in, n, seg_stride, scratch, tiles, fstride);
+  k_scan_tile_sums<<<dim3(segments, 1, nf), kScanThreads, 0, st>>>(scratch, tiles, total,
+                                                                    fstride);
   k_tile_scan<false><<<g, kScanThreads, 0, st>>>(const_cast<int32_t *>(in), out, n, seg_stride,
-                                                 scratch, tiles);
+                                                 scratch, tiles, fstride);
 }
 
 void scan_exclusive_clear(int32_t *in, int32_t *out, long n, int32_t *total, int32_t *scratch,
-                          hipStream_t st) {
-  if (n <= 0) return;
+                          hipStream_t st, int nf, long fstride) {
+  if (n <= 0 || nf <= 0) return;
   const long tiles = (n + kScanTile - 1) / kScanTile;
-  dim3 g((unsigned)tiles, 1);
-  k_tile_sums<<<g, kScanThreads, 0, st>>>(in, n, n, scratch, tiles);
-  k_scan_tile_sums<<<1, kScanThreads, 0, st>>>(scratch, tiles, total);
-  k_tile_scan<true><<<g, kScanThreads, 0, st>>>(in, out, n, n, scratch, tiles);
+  dim3 g((unsigned)tiles, 1, (unsigned)nf);
+  k_tile_sums<<<g, kScanThreads, 0, st>>>(in, n, n, scratch, tiles, fstride);
+  k_scan_tile_sums<<<dim3(1, 1, nf), kScanThreads, 0, st>>>(scratch, tiles, total, fstride);
+  k_tile_scan<true><<<g, kScanThreads, 0, st>>>(in, out, n, n, scratch, tiles, fstride);
 }
 
 } // namespace olpt

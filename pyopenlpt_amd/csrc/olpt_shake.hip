@@ -52,9 +52,16 @@ template <typename T> struct ObjCamRec {
 // 8x8-pixel cell); bin_cnt is all zero on entry (the scan that turns the
 // counts into offsets clears it again).
 template <typename T, bool BIN>
-__global__ void k_obj_project(ShakeBuffers<T> b, const double *__restrict__ xyz,
+__global__ void k_obj_project(ShakeBuffers<T> bb, const double *__restrict__ xyz,
                               const int32_t *__restrict__ n_ptr, ObjCamRec<T> *rec,
                               T *uv_out, int box_w) {
+  const long F = (long)blockIdx.z;
+  const ShakeBuffers<T> b = bb.frame(F);
+  const long fo = F * bb.fstride;
+  xyz = fshift(xyz, fo);
+  n_ptr = fshift(n_ptr, fo);
+  rec = fshift(rec, fo);
+  uv_out = fshift(uv_out, fo);
   const int n = *n_ptr;
   const int nc = b.n_cam;
   const long nbin = (long)b.bin_rows * b.bin_cols;
@@ -88,7 +95,11 @@ __global__ void k_obj_project(ShakeBuffers<T> b, const double *__restrict__ xyz,
 }
 
 template <typename T>
-__global__ void k_bin_fill(ShakeBuffers<T> b, const int32_t *__restrict__ n_ptr) {
+__global__ void k_bin_fill(ShakeBuffers<T> bb, const int32_t *__restrict__ n_ptr) {
+  const long F = (long)blockIdx.z;
+  const ShakeBuffers<T> b = bb.frame(F);
+  const long fo = F * bb.fstride;
+  n_ptr = fshift(n_ptr, fo);
   const int n = *n_ptr;
   const int nc = b.n_cam;
   const long nbin = (long)b.bin_rows * b.bin_cols;
@@ -134,14 +145,20 @@ template <> struct MaxBits<double> {
 
 template <typename T>
 __global__ void __launch_bounds__(kResThreads)
-k_resid_gather(ShakeBuffers<T> b, const T *__restrict__ orig, T *__restrict__ out,
+k_resid_gather(ShakeBuffers<T> bb, const T *__restrict__ orig, T *__restrict__ out,
                const ObjCamRec<T> *__restrict__ rec, CamList cams, int mode, int box_w) {
+  const long F = (long)(blockIdx.z / cams.n);
+  const ShakeBuffers<T> b = bb.frame(F);
+  const long fo = F * bb.fstride;
+  orig = fshift(orig, fo);
+  out = fshift(out, fo);
+  rec = fshift(rec, fo);
   using U = typename MaxBits<T>::U;
   constexpr int NPX = kResTile * kResTile;
   __shared__ U s_img[NPX];
   __shared__ ObjCamRec<T> s_obj[kTileObjCap];
   __shared__ int32_t s_bstart[kTileBinCap], s_bpre[kTileBinCap + 1];
-  const int k = cams.id[blockIdx.z];
+  const int k = cams.id[blockIdx.z % cams.n];
   const int r0 = blockIdx.y * kResTile, c0 = blockIdx.x * kResTile;
   const int tid = threadIdx.x;
   const long HW = (long)b.H * b.W;
@@ -235,14 +252,14 @@ static void residual_pass(const ShakeBuffers<T> &b, const T *orig, T *out,
   const int nc = b.n_cam;
   const long nbin = (long)b.bin_rows * b.bin_cols;
   const int box_w = box_width(r_px);
-  k_obj_project<T, true><<<grid_for(b.cap_obj, 256, 2048), 256, 0, st>>>(b, xyz, n_ptr, rec,
+  k_obj_project<T, true><<<dim3(grid_for(b.cap_obj, 256, 2048), 1, b.nf), 256, 0, st>>>(b, xyz, n_ptr, rec,
                                                                          uv_out, box_w);
   if (amp) k_set_amp<T><<<grid_for((long)b.cap_obj * nc, 256, 2048), 256, 0, st>>>(rec, amp, n_ptr, nc);
   // one scan over all cameras' bins; bin_start[nc * nbin] receives the total
   scan_exclusive_clear(b.bin_cnt, b.bin_start, nbin * nc, b.bin_start + nbin * nc,
-                       b.scan_scratch, st);
-  k_bin_fill<T><<<grid_for((long)b.cap_obj * nc, 256, 4096), 256, 0, st>>>(b, n_ptr);
-  dim3 g((b.W + kResTile - 1) / kResTile, (b.H + kResTile - 1) / kResTile, cams.n);
+                       b.scan_scratch, st, b.nf, b.fstride);
+  k_bin_fill<T><<<dim3(grid_for((long)b.cap_obj * nc, 256, 4096), 1, b.nf), 256, 0, st>>>(b, n_ptr);
+  dim3 g((b.W + kResTile - 1) / kResTile, (b.H + kResTile - 1) / kResTile, cams.n * b.nf);
   stage_begin(ST_RESID, st);
   k_resid_gather<T><<<g, kResThreads, 0, st>>>(b, orig, out, rec, cams, mode, box_w);
   stage_end(ST_RESID, st);
@@ -250,7 +267,9 @@ static void residual_pass(const ShakeBuffers<T> &b, const T *orig, T *out,
 
 template <typename T>
 __global__ void k_copy_cam(const T *__restrict__ src, T *__restrict__ dst, long HW,
-                           CamList cams) {
+                           CamList cams, long fstride) {
+  src = fshift(src, (long)blockIdx.z * fstride);
+  dst = fshift(dst, (long)blockIdx.z * fstride);
   const int k = cams.id[blockIdx.y];
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < HW;
        i += (long)gridDim.x * blockDim.x)
@@ -503,8 +522,12 @@ __device__ __forceinline__ void build_roi(const ShakeBuffers<T> &b, const ShakeP
 
 template <typename T, int NC, int OPW, int SLOTS>
 __global__ void __launch_bounds__(kWave)
-k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
+k_shake(ShakeBuffers<T> bb, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
         T delta) {
+  const long F = (long)blockIdx.z;
+  const ShakeBuffers<T> b = bb.frame(F);
+  const long fo = F * bb.fstride;
+  rec = fshift(rec, fo);
   constexpr int LPO = kWave / OPW;   // lanes per object
   constexpr int LPG = LPO / NC;      // lanes per camera
   const int n = *b.n_obj;
@@ -775,8 +798,13 @@ k_shake(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
 constexpr int kPatch = 11, kPatchHalf = 5, kPatchIn = 512;
 
 template <typename T>
-__global__ void k_patch_gather(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
+__global__ void k_patch_gather(ShakeBuffers<T> bb, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
                                __bf16 *__restrict__ X) {
+  const long F = (long)blockIdx.z;
+  const ShakeBuffers<T> b = bb.frame(F);
+  const long fo = F * bb.fstride;
+  rec = fshift(rec, fo);
+  X = fshift(X, (long)F * bb.lstride);
   const long n = *b.n_obj;
   const int nc = b.n_cam;
   const long HW = (long)b.H * b.W;
@@ -809,8 +837,13 @@ __global__ void k_patch_gather(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec
 // residue (calShakeResidue on the same ROI); otherwise keep X.
 template <typename T, int NC, int OPW, int SLOTS>
 __global__ void __launch_bounds__(kWave)
-k_propose(ShakeBuffers<T> b, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
+k_propose(ShakeBuffers<T> bb, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
           const float *__restrict__ prop) {
+  const long F = (long)blockIdx.z;
+  const ShakeBuffers<T> b = bb.frame(F);
+  const long fo = F * bb.fstride;
+  rec = fshift(rec, fo);
+  prop = fshift(prop, (long)F * bb.pstride);
   constexpr int LPO = kWave / OPW;
   const int n = *b.n_obj;
   const int lane = lane_id();
@@ -854,7 +887,9 @@ __device__ __forceinline__ int grid_cell_of(const ShakeBuffers<T> &b, const doub
 }
 
 template <typename T>
-__global__ void k_grid_count(ShakeBuffers<T> b) {
+__global__ void k_grid_count(ShakeBuffers<T> bb) {
+  const long F = (long)blockIdx.z;
+  const ShakeBuffers<T> b = bb.frame(F);
   const int n = *b.n_obj;
   const int G = b.grid_n;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -866,7 +901,9 @@ __global__ void k_grid_count(ShakeBuffers<T> b) {
 }
 
 template <typename T>
-__global__ void k_grid_fill(ShakeBuffers<T> b) {
+__global__ void k_grid_fill(ShakeBuffers<T> bb) {
+  const long F = (long)blockIdx.z;
+  const ShakeBuffers<T> b = bb.frame(F);
   const int n = *b.n_obj;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     b.g_obj[b.g_start[b.g_cellof[i]] + b.g_rank[i]] = i;
@@ -905,7 +942,9 @@ __device__ __forceinline__ int count_nb(const ShakeBuffers<T> &b, int i, double 
 }
 
 template <typename T>
-__global__ void k_has_nb(ShakeBuffers<T> b, double t2) {
+__global__ void k_has_nb(ShakeBuffers<T> bb, double t2) {
+  const long F = (long)blockIdx.z;
+  const ShakeBuffers<T> b = bb.frame(F);
   const int n = *b.n_obj;
   // entries past n are cleared: has_nb is scanned over cap_obj
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.cap_obj;
@@ -923,7 +962,9 @@ __global__ void k_has_nb(ShakeBuffers<T> b, double t2) {
 // last object is never a query (Shake.cpp:797: i < n_obj - 1).  Marks are
 // written as OLPT_FLAG_REPEATED into flag[].
 template <typename T>
-__global__ void k_repeat_greedy(ShakeBuffers<T> b, double t2) {
+__global__ void k_repeat_greedy(ShakeBuffers<T> bb, double t2) {
+  const long F = (long)blockIdx.z;
+  const ShakeBuffers<T> b = bb.frame(F);
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const int n = *b.n_obj;
   const int m = b.totals[0];
@@ -938,7 +979,9 @@ __global__ void k_repeat_greedy(ShakeBuffers<T> b, double t2) {
 
 // compact indices of objects with neighbours (order preserved)
 template <typename T>
-__global__ void k_nb_list(ShakeBuffers<T> b) {
+__global__ void k_nb_list(ShakeBuffers<T> bb) {
+  const long F = (long)blockIdx.z;
+  const ShakeBuffers<T> b = bb.frame(F);
   const int n = *b.n_obj;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     if (b.has_nb[i]) b.keep_off[b.g_rank[i]] = i;
@@ -948,7 +991,9 @@ __global__ void k_nb_list(ShakeBuffers<T> b) {
 constexpr int kRedThreads = 256;
 constexpr int kRedBlocks = 256;
 template <typename T>
-__global__ void __launch_bounds__(kRedThreads) k_score_sum(ShakeBuffers<T> b) {
+__global__ void __launch_bounds__(kRedThreads) k_score_sum(ShakeBuffers<T> bb) {
+  const long F = (long)blockIdx.z;
+  const ShakeBuffers<T> b = bb.frame(F);
   __shared__ double sd[kRedThreads];
   __shared__ int sc[kRedThreads];
   const int n = *b.n_obj;
@@ -978,7 +1023,9 @@ __global__ void __launch_bounds__(kRedThreads) k_score_sum(ShakeBuffers<T> b) {
 // Ghost: score < thred_ghost * mean(non-repeated) or score == 0
 // (Shake.cpp:118-136); keep_off <- 1 for unflagged objects.
 template <typename T>
-__global__ void k_flags(ShakeBuffers<T> b, double thr) {
+__global__ void k_flags(ShakeBuffers<T> bb, double thr) {
+  const long F = (long)blockIdx.z;
+  const ShakeBuffers<T> b = bb.frame(F);
   double sum = 0.0, cnt = 0.0;
   for (int k = 0; k < kRedBlocks; ++k) {
     sum += b.partial[2 * k];
@@ -995,7 +1042,9 @@ __global__ void k_flags(ShakeBuffers<T> b, double thr) {
 }
 
 template <typename T>
-__global__ void k_init_score(ShakeBuffers<T> b) {
+__global__ void k_init_score(ShakeBuffers<T> bb) {
+  const long F = (long)blockIdx.z;
+  const ShakeBuffers<T> b = bb.frame(F);
   const int n = *b.n_obj;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     b.score[i] = 1.0;
@@ -1004,7 +1053,9 @@ __global__ void k_init_score(ShakeBuffers<T> b) {
 }
 
 template <typename T>
-__global__ void k_zero_score(ShakeBuffers<T> b) {
+__global__ void k_zero_score(ShakeBuffers<T> bb) {
+  const long F = (long)blockIdx.z;
+  const ShakeBuffers<T> b = bb.frame(F);
   const int n = *b.n_obj;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     b.score[i] = 0.0;
@@ -1014,7 +1065,7 @@ template <typename T, int NC, int OPW, int SLOTS>
 static void shake_iter(const ShakeBuffers<T> &b, const ShakeParams &p,
                        const ObjCamRec<T> *rec, T delta, hipStream_t st) {
   stage_begin(ST_SHAKE, st);
-  k_shake<T, NC, OPW, SLOTS><<<b.grid_shake, kWave, 0, st>>>(b, p, rec, delta);
+  k_shake<T, NC, OPW, SLOTS><<<dim3(b.grid_shake, 1, b.nf), kWave, 0, st>>>(b, p, rec, delta);
   stage_end(ST_SHAKE, st);
 }
 
@@ -1037,15 +1088,16 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
   // the ROI edge, as every kernel uses it (calROIBound, decided on the host)
   const int roi_px = p.roi_w > 0 ? p.roi_w * p.roi_w : 0;
   const int gO = grid_for(b.cap_obj, 256, 2048);
-  k_init_score<T><<<gO, 256, 0, st>>>(b);
+  k_init_score<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b);
   if (p.mlp && b.patches && b.prop && nc == 4 && roi_px <= 64) {
     // learned proposal from the matched positions, then the plain iterations
     residual_pass<T>(b, b.img, b.res, b.xyz, b.n_obj, rec, b.uv, act, 0, r_px, st);
-    k_patch_gather<T><<<grid_for((long)b.cap_obj * kPatchIn, 256, 8192), 256, 0, st>>>(
+    k_patch_gather<T><<<dim3(grid_for((long)b.cap_obj * kPatchIn, 256, 8192), 1, b.nf), 256, 0, st>>>(
         b, p, rec, (__bf16 *)b.patches);
-    mlp_infer_device(p.mlp, b.patches, b.n_obj, b.cap_obj, b.prop, st);
+    mlp_infer_device(p.mlp, b.patches, b.n_obj, b.cap_obj, b.prop, st, b.nf, b.lstride,
+                     b.fstride, b.pstride);
     constexpr int O = shake_opw<T>();
-    k_propose<T, 4, O, 4 * O><<<b.grid_shake, kWave, 0, st>>>(b, p, rec, b.prop);
+    k_propose<T, 4, O, 4 * O><<<dim3(b.grid_shake, 1, b.nf), kWave, 0, st>>>(b, p, rec, b.prop);
   }
   double delta = p.shake_width;
   double dmin = p.shakewidth_min > 0 ? p.shakewidth_min : delta / 20;
@@ -1054,7 +1106,7 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
     // projection pass also refreshes uv (runShake reprojects first)
     residual_pass<T>(b, b.img, b.res, b.xyz, b.n_obj, rec, b.uv, act, 0, r_px, st);
     if (nc < 2) {
-      k_zero_score<T><<<gO, 256, 0, st>>>(b);
+      k_zero_score<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b);
     } else if (nc <= 4 && roi_px <= 64) {
       shake_iter<T, 4, shake_opw<T>(), 4 * shake_opw<T>()>(b, p, rec, (T)delta, st);
     } else if (nc <= 4) {
@@ -1068,21 +1120,21 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
     if (delta < dmin) delta = dmin;
   }
   if (p.n_shake_loop <= 0) {
-    k_obj_project<T, false><<<gO, 256, 0, st>>>(b, b.xyz, b.n_obj, rec, b.uv, p.box_w);
+    k_obj_project<T, false><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b, b.xyz, b.n_obj, rec, b.uv, p.box_w);
   }
   // markRepeatedObj
   const double t2 = p.tol3d * p.tol3d;
   const long ncell = (long)b.grid_n * b.grid_n * b.grid_n;
-  k_grid_count<T><<<gO, 256, 0, st>>>(b);  // g_cnt is all zero here; the scan clears it
-  scan_exclusive_clear(b.g_cnt, b.g_start, ncell, b.g_start + ncell, b.scan_scratch, st);
-  k_grid_fill<T><<<gO, 256, 0, st>>>(b);
-  k_has_nb<T><<<gO, 256, 0, st>>>(b, t2);
-  scan_exclusive(b.has_nb, b.g_rank, b.cap_obj, 1, b.cap_obj, b.totals, b.scan_scratch, st);
-  k_nb_list<T><<<gO, 256, 0, st>>>(b);
-  k_repeat_greedy<T><<<1, 64, 0, st>>>(b, t2);
+  k_grid_count<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b);  // g_cnt is all zero here; the scan clears it
+  scan_exclusive_clear(b.g_cnt, b.g_start, ncell, b.g_start + ncell, b.scan_scratch, st, b.nf, b.fstride);
+  k_grid_fill<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b);
+  k_has_nb<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b, t2);
+  scan_exclusive(b.has_nb, b.g_rank, b.cap_obj, 1, b.cap_obj, b.totals, b.scan_scratch, st, b.nf, b.fstride);
+  k_nb_list<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b);
+  k_repeat_greedy<T><<<dim3(1, 1, b.nf), 64, 0, st>>>(b, t2);
   // ghost threshold: mean over non-repeated scores
-  k_score_sum<T><<<kRedBlocks, kRedThreads, 0, st>>>(b);
-  k_flags<T><<<gO, 256, 0, st>>>(b, p.thred_ghost);
+  k_score_sum<T><<<dim3(kRedBlocks, 1, b.nf), kRedThreads, 0, st>>>(b);
+  k_flags<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b, p.thred_ghost);
   (void)inact;
 }
 
@@ -1090,9 +1142,16 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
 // filter + IPR-mode residual
 // --------------------------------------------------------------------------
 template <typename T>
-__global__ void k_keep_write(ShakeBuffers<T> b, const int32_t *__restrict__ keep,
+__global__ void k_keep_write(ShakeBuffers<T> bb, const int32_t *__restrict__ keep,
                              const int32_t *__restrict__ off, double *xyz_keep,
                              T *uv_keep) {
+  const long F = (long)blockIdx.z;
+  const ShakeBuffers<T> b = bb.frame(F);
+  const long fo = F * bb.fstride;
+  keep = fshift(keep, fo);
+  off = fshift(off, fo);
+  xyz_keep = fshift(xyz_keep, fo);
+  uv_keep = fshift(uv_keep, fo);
   const int n = *b.n_obj;
   const int nc = b.n_cam;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -1107,14 +1166,18 @@ __global__ void k_keep_write(ShakeBuffers<T> b, const int32_t *__restrict__ keep
 }
 
 template <typename T>
-__global__ void k_keep_flags(ShakeBuffers<T> b) {
+__global__ void k_keep_flags(ShakeBuffers<T> bb) {
+  const long F = (long)blockIdx.z;
+  const ShakeBuffers<T> b = bb.frame(F);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.cap_obj;
        i += gridDim.x * blockDim.x)
     b.has_nb[i] = (i < *b.n_obj && b.flag[i] == OLPT_FLAG_NONE) ? 1 : 0;
 }
 
 template <typename T>
-__global__ void k_keep_count(const int32_t *tot, int32_t *n_keep) {
+__global__ void k_keep_count(const int32_t *tot, int32_t *n_keep, long fstride) {
+  tot = fshift(tot, (long)blockIdx.z * fstride);
+  n_keep = fshift(n_keep, (long)blockIdx.z * fstride);
   if (threadIdx.x == 0 && blockIdx.x == 0) *n_keep = tot[0];
 }
 
@@ -1122,11 +1185,11 @@ template <typename T>
 void launch_filter(const ShakeBuffers<T> &b, double *xyz_keep, T *uv_keep,
                    int32_t *n_keep, hipStream_t st) {
   const int gO = grid_for(b.cap_obj, 256, 2048);
-  k_keep_flags<T><<<gO, 256, 0, st>>>(b);
+  k_keep_flags<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b);
   scan_exclusive(b.has_nb, b.keep_off, b.cap_obj, 1, b.cap_obj, b.totals,
-                 b.scan_scratch, st);
-  k_keep_write<T><<<gO, 256, 0, st>>>(b, b.has_nb, b.keep_off, xyz_keep, uv_keep);
-  k_keep_count<T><<<1, 64, 0, st>>>(b.totals, n_keep);
+                 b.scan_scratch, st, b.nf, b.fstride);
+  k_keep_write<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b, b.has_nb, b.keep_off, xyz_keep, uv_keep);
+  k_keep_count<T><<<dim3(1, 1, b.nf), 64, 0, st>>>(b.totals, n_keep, b.fstride);
 }
 
 template <typename T>
@@ -1146,7 +1209,7 @@ void launch_patches(const ShakeBuffers<T> &b, const ShakeParams &p, void *X, hip
     if (p.active[k]) act.id[act.n++] = k;
   ObjCamRec<T> *rec = reinterpret_cast<ObjCamRec<T> *>(b.rec);
   residual_pass<T>(b, b.img, b.res, b.xyz, b.n_obj, rec, b.uv, act, 0, p.r_px, st);
-  k_patch_gather<T><<<grid_for((long)b.cap_obj * kPatchIn, 256, 8192), 256, 0, st>>>(
+  k_patch_gather<T><<<dim3(grid_for((long)b.cap_obj * kPatchIn, 256, 8192), 1, b.nf), 256, 0, st>>>(
       b, p, rec, (__bf16 *)X);
 }
 

@@ -54,20 +54,23 @@ namespace olpt {
 // --------------------------------------------------------------------------
 // small kernels
 // --------------------------------------------------------------------------
-__global__ void k_zero_i32(int32_t *p, long n) {
+__global__ void k_zero_i32(int32_t *p, long n, long fstride) {
+  p = fshift(p, (long)blockIdx.z * fstride);
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x)
     p[i] = 0;
 }
 
-static void zero_i32(int32_t *p, long n, hipStream_t st) {
+// zero n ints at p in every frame of b's batch (p: frame 0's buffer)
+static void zero_i32(const StereoBuffers &b, int32_t *p, long n, hipStream_t st) {
   if (n <= 0) return;
   long blocks = (n + 255) / 256;
   if (blocks > 4096) blocks = 4096;
-  k_zero_i32<<<(unsigned)blocks, 256, 0, st>>>(p, n);
+  k_zero_i32<<<dim3((unsigned)blocks, 1, b.nf), 256, 0, st>>>(p, n, b.fstride);
 }
 
-__global__ void k_cell_count(StereoBuffers b, StereoParams p) {
+__global__ void k_cell_count(StereoBuffers bb, StereoParams p) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const int cam = blockIdx.y;
   if (!p.active[cam]) return;
   const int n = b.npts[cam];
@@ -87,7 +90,8 @@ __global__ void k_cell_count(StereoBuffers b, StereoParams p) {
   }
 }
 
-__global__ void k_cell_fill(StereoBuffers b, StereoParams p) {
+__global__ void k_cell_fill(StereoBuffers bb, StereoParams p) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const int cam = blockIdx.y;
   if (!p.active[cam]) return;
   const int n = b.npts[cam];
@@ -122,7 +126,8 @@ __device__ __forceinline__ Line2 make_line2d(const CamT<double> &c,
   return Line2{a[0], a[1], d0 / n, d1 / n};
 }
 
-__global__ void k_los(StereoBuffers b, StereoParams p) {
+__global__ void k_los(StereoBuffers bb, StereoParams p) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const int cam = blockIdx.y;
   if (!p.active[cam]) return;
   const int n = b.npts[cam];
@@ -612,7 +617,8 @@ __device__ int extend(const StereoBuffers &b, const StereoParams &p, Path &P, in
 
 // Second build camera of every reference observation
 // (selectSecondCameraByLineLength, :1307-1370): shortest in-image segment.
-__global__ void __launch_bounds__(256) k_b_second(StereoBuffers b, StereoParams p) {
+__global__ void __launch_bounds__(256) k_b_second(StereoBuffers bb, StereoParams p) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   int nref;
   const int ref = ref_camera(b, p, nref);
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nref;
@@ -634,7 +640,8 @@ __global__ void __launch_bounds__(256) k_b_second(StereoBuffers b, StereoParams 
 // Level-1 band: one thread per (reference observation, run of kBandRows
 // cell rows of its second camera); accepted points become (r, pid) pairs.
 constexpr int kBandRows = 4;
-__global__ void __launch_bounds__(256) k_b_band(StereoBuffers b, StereoParams p) {
+__global__ void __launch_bounds__(256) k_b_band(StereoBuffers bb, StereoParams p) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   int nref;
   const int ref = ref_camera(b, p, nref);
   const int rows = b.rows_cell, cols = b.cols_cell;
@@ -698,7 +705,8 @@ __global__ void __launch_bounds__(256) k_b_band(StereoBuffers b, StereoParams p)
 // Deeper bands: one thread per survivor of level K-1 (a path of length K),
 // enumerating the cell rows of its K-strip intersection on its next camera.
 template <int K>
-__global__ void __launch_bounds__(256) k_b_enum(StereoBuffers b, StereoParams p) {
+__global__ void __launch_bounds__(256) k_b_enum(StereoBuffers bb, StereoParams p) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   int nref;
   const int ref = ref_camera(b, p, nref);
   const int rows = b.rows_cell, cols = b.cols_cell;
@@ -783,7 +791,8 @@ __global__ void __launch_bounds__(256) k_b_enum(StereoBuffers b, StereoParams p)
 // Extension of every (path of length K, point) pair: a survivor goes to
 // level K, a complete match (K + 1 == build_count) to the candidate list.
 template <int K>
-__global__ void __launch_bounds__(256) k_b_extend(StereoBuffers b, StereoParams p) {
+__global__ void __launch_bounds__(256) k_b_extend(StereoBuffers bb, StereoParams p) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   int nref;
   const int ref = ref_camera(b, p, nref);
   const int n = min(*b.n_pairs, (int)b.cap_pairs);
@@ -856,7 +865,8 @@ __device__ __forceinline__ int primary_cam(const StereoParams &p, int n_cam) {
   return 0;
 }
 
-__global__ void k_key_hist(StereoBuffers b, StereoParams p) {
+__global__ void k_key_hist(StereoBuffers bb, StereoParams p) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
   const int pc = primary_cam(p, b.n_cam);
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
@@ -865,7 +875,8 @@ __global__ void k_key_hist(StereoBuffers b, StereoParams p) {
   }
 }
 
-__global__ void k_key_scatter(StereoBuffers b, StereoParams p) {
+__global__ void k_key_scatter(StereoBuffers bb, StereoParams p) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
   const int pc = primary_cam(p, b.n_cam);
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
@@ -883,7 +894,8 @@ __device__ __forceinline__ bool ids_less(const int32_t *a, const int32_t *c, int
 // Final position of each candidate = bucket start + number of bucket members
 // that precede it lexicographically (equal vectors, which the build never
 // emits, by emission index so positions stay a permutation).
-__global__ void k_bucket_rank(StereoBuffers b, StereoParams p) {
+__global__ void k_bucket_rank(StereoBuffers bb, StereoParams p) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
   const int pc = primary_cam(p, b.n_cam);
   const int nc = b.n_cam;
@@ -971,7 +983,8 @@ __device__ __forceinline__ bool less_order(const double *pct, const double *e,
 __device__ __forceinline__ int ent_cand(int32_t v) { return v >> 3; }
 __device__ __forceinline__ int ent_slot(int32_t v) { return v & 7; }
 
-__global__ void k_flat(StereoBuffers b, StereoParams p) {
+__global__ void k_flat(StereoBuffers bb, StereoParams p) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
   const int nc = b.n_cam;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
@@ -987,7 +1000,8 @@ __global__ void k_flat(StereoBuffers b, StereoParams p) {
   }
 }
 
-__global__ void k_pt_fill(StereoBuffers b, StereoParams p) {
+__global__ void k_pt_fill(StereoBuffers bb, StereoParams p) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
   const int nc = b.n_cam;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
@@ -1005,7 +1019,8 @@ __global__ void k_pt_fill(StereoBuffers b, StereoParams p) {
   }
 }
 
-__global__ void k_zero_entries(StereoBuffers b) {
+__global__ void k_zero_entries(StereoBuffers bb) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const int n = *b.n_entries;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
     b.pt_occ[q] = 0;
@@ -1017,7 +1032,8 @@ __global__ void k_zero_entries(StereoBuffers b) {
 // One thread per list entry counts the entries below it.  An entry with no
 // eq_eps partner forms a block of its own at that rank; a point holding any
 // eq_eps pair is flagged and redone by k_pct_fallback.
-__global__ void k_pct_rank(StereoBuffers b) {
+__global__ void k_pct_rank(StereoBuffers bb) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const int ne = *b.n_entries;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ne; q += gridDim.x * blockDim.x) {
     const int f = b.pt_of[q];
@@ -1056,7 +1072,8 @@ __global__ void k_pct_rank(StereoBuffers b) {
 
 // Sequential statement of the percentile for flagged points: insertion sort
 // by (e eq_eps, index), then eq_eps blocks from each block's first entry.
-__global__ void k_pct_fallback(StereoBuffers b) {
+__global__ void k_pct_fallback(StereoBuffers bb) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const long total = (long)b.n_cam * b.cap;
   for (long f = blockIdx.x * (long)blockDim.x + threadIdx.x; f < total;
        f += (long)gridDim.x * blockDim.x) {
@@ -1092,7 +1109,8 @@ __global__ void k_pct_fallback(StereoBuffers b) {
   }
 }
 
-__global__ void k_pct_final(StereoBuffers b, StereoParams p) {
+__global__ void k_pct_final(StereoBuffers bb, StereoParams p) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
   const int nc = b.n_cam;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
@@ -1112,7 +1130,8 @@ __global__ void k_pct_final(StereoBuffers b, StereoParams p) {
 // of StereoMatch.cpp:820-851 restricted to the point): position = number of
 // entries that precede the entry.  A point whose positions collide (the
 // pairwise order is not transitive there) is flagged for k_prio_fallback.
-__global__ void k_prio_rank(StereoBuffers b) {
+__global__ void k_prio_rank(StereoBuffers bb) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const int ne = *b.n_entries;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ne; q += gridDim.x * blockDim.x) {
     const int f = b.pt_of[q];
@@ -1142,7 +1161,8 @@ __global__ void k_prio_rank(StereoBuffers b) {
   }
 }
 
-__global__ void k_prio_fallback(StereoBuffers b) {
+__global__ void k_prio_fallback(StereoBuffers bb) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const long total = (long)b.n_cam * b.cap;
   for (long f = blockIdx.x * (long)blockDim.x + threadIdx.x; f < total;
        f += (long)gridDim.x * blockDim.x) {
@@ -1228,7 +1248,8 @@ __device__ __forceinline__ int mis_decide(const StereoBuffers &b, const StereoPa
   return 0;
 }
 
-__global__ void k_mis_round(StereoBuffers b, StereoParams p, int round) {
+__global__ void k_mis_round(StereoBuffers bb, StereoParams p, int round) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   if (round > 0 && b.undecided[round - 1] == 0) return;
   const int M = min(*b.n_cand, b.cap_cand);
   int und = 0;
@@ -1249,7 +1270,8 @@ constexpr int kMisFull = 10;
 constexpr int kMisFinishThreads = 1024;
 constexpr int kMisFinishRounds = 1 << 14;
 
-__global__ void k_mis_compact(StereoBuffers b) {
+__global__ void k_mis_compact(StereoBuffers bb) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   if (b.undecided[kMisFull - 1] == 0) return;
   const int M = min(*b.n_cand, b.cap_cand);
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x)
@@ -1257,7 +1279,8 @@ __global__ void k_mis_compact(StereoBuffers b) {
 }
 
 __global__ void __launch_bounds__(kMisFinishThreads)
-k_mis_finish(StereoBuffers b, StereoParams p) {
+k_mis_finish(StereoBuffers bb, StereoParams p) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   __shared__ int s_und[kMisFinishThreads / 64];
   const int n = b.undecided[kMisFull];
   if (n == 0) return;
@@ -1282,7 +1305,8 @@ k_mis_finish(StereoBuffers b, StereoParams p) {
 // Replacement branch of the reference (StereoMatch.cpp:906-968), evaluated
 // for each rejected candidate against the owners it met at its turn: the
 // selected entries that precede it on its points (at most one per point).
-__global__ void k_replace_check(StereoBuffers b, StereoParams p) {
+__global__ void k_replace_check(StereoBuffers bb, StereoParams p) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
   const int nc = b.n_cam;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
@@ -1394,7 +1418,8 @@ __device__ __forceinline__ double hz_val(unsigned long long bits) {
   return __longlong_as_double((long long)bits);
 }
 
-__global__ void k_hz_insert(StereoBuffers b) {
+__global__ void k_hz_insert(StereoBuffers bb) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
     const double pv[2] = {b.pct[j], b.cand_e_s[j]};
@@ -1423,7 +1448,8 @@ __global__ void k_hz_insert(StereoBuffers b) {
   }
 }
 
-__global__ void k_hz_check(StereoBuffers b) {
+__global__ void k_hz_check(StereoBuffers bb) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * M; i += gridDim.x * blockDim.x) {
     const long s = b.hz_slot[i];
@@ -1459,7 +1485,8 @@ __global__ void k_hz_check(StereoBuffers b) {
 }
 
 // empty the slots used by this call (every used slot holds an inserted value)
-__global__ void k_hz_clear(StereoBuffers b) {
+__global__ void k_hz_clear(StereoBuffers bb) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * M; i += gridDim.x * blockDim.x) {
     const long s = b.hz_slot[i];
@@ -1469,7 +1496,8 @@ __global__ void k_hz_clear(StereoBuffers b) {
   }
 }
 
-__global__ void k_hz_init(StereoBuffers b) {
+__global__ void k_hz_init(StereoBuffers bb) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   for (long s = blockIdx.x * (long)blockDim.x + threadIdx.x; s < b.hz_size;
        s += (long)gridDim.x * blockDim.x) {
     b.hz_key[s] = kHzEmpty;
@@ -1478,7 +1506,7 @@ __global__ void k_hz_init(StereoBuffers b) {
 }
 
 void stereo_init_tables(const StereoBuffers &b, hipStream_t st) {
-  k_hz_init<<<1024, 256, 0, st>>>(b);
+  k_hz_init<<<dim3(1024, 1, b.nf), 256, 0, st>>>(b);
 }
 
 // triangulateMatch: camera-order LOS, FP64, gate on tol_3d.  A candidate's
@@ -1486,7 +1514,8 @@ void stereo_init_tables(const StereoBuffers &b, hipStream_t st) {
 // compile-time slot and camera indices (MC >= n_cam), so the LOS arrays stay
 // in registers.
 template <int MC>
-__global__ void __launch_bounds__(256) k_tri(StereoBuffers b, StereoParams p) {
+__global__ void __launch_bounds__(256) k_tri(StereoBuffers bb, StereoParams p) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
   const int nc = b.n_cam;
   int nsel = 0;
@@ -1540,7 +1569,8 @@ __global__ void __launch_bounds__(256) k_tri(StereoBuffers b, StereoParams p) {
   if (lane_id() == 0 && nsel) atomicAdd(b.n_sel, nsel);
 }
 
-__global__ void k_obj_write(StereoBuffers b) {
+__global__ void k_obj_write(StereoBuffers bb) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
   const int nc = b.n_cam;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
@@ -1555,7 +1585,8 @@ __global__ void k_obj_write(StereoBuffers b) {
   }
 }
 
-__global__ void k_obj_count(StereoBuffers b) {
+__global__ void k_obj_count(StereoBuffers bb) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     const int t = b.totals[0];
     *b.n_obj = t < b.cap_obj ? t : b.cap_obj;
@@ -1574,9 +1605,9 @@ static int grid_for(long n, int threads, int max_blocks) {
 
 template <int K>
 static void build_level(const StereoBuffers &b, const StereoParams &p, hipStream_t st) {
-  zero_i32(b.n_pairs, 1, st);
-  k_b_enum<K><<<b.grid_build, 256, 0, st>>>(b, p);
-  k_b_extend<K><<<b.grid_build, 256, 0, st>>>(b, p);
+  zero_i32(b, b.n_pairs, 1, st);
+  k_b_enum<K><<<dim3(b.grid_build, 1, b.nf), 256, 0, st>>>(b, p);
+  k_b_extend<K><<<dim3(b.grid_build, 1, b.nf), 256, 0, st>>>(b, p);
 }
 
 // pruneMatch on the sorted candidates (cand_ids_s, cand_e_s, n_cand):
@@ -1586,28 +1617,28 @@ static void prune_section(const StereoBuffers &b, const StereoParams &p, int gM,
                           hipStream_t st) {
   const int T = 256;
   const long nflat = (long)b.n_cam * b.cap;
-  zero_i32(b.pt_count, nflat, st);
-  zero_i32(b.pt_flag, nflat, st);
-  zero_i32(b.head, nflat, st);
-  k_flat<<<gM, T, 0, st>>>(b, p);
-  scan_exclusive(b.pt_count, b.pt_start, nflat, 1, nflat, b.n_entries, b.scan_scratch, st);
-  k_pt_fill<<<gM, T, 0, st>>>(b, p);
+  zero_i32(b, b.pt_count, nflat, st);
+  zero_i32(b, b.pt_flag, nflat, st);
+  zero_i32(b, b.head, nflat, st);
+  k_flat<<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p);
+  scan_exclusive(b.pt_count, b.pt_start, nflat, 1, nflat, b.n_entries, b.scan_scratch, st, b.nf, b.fstride);
+  k_pt_fill<<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p);
   const int gE = grid_for((long)b.cap_cand * 4, T, 4096);
-  k_zero_entries<<<gE, T, 0, st>>>(b);
-  k_pct_rank<<<gE, T, 0, st>>>(b);
-  k_pct_fallback<<<grid_for(nflat, T, 2048), T, 0, st>>>(b);
-  k_pct_final<<<gM, T, 0, st>>>(b, p);
-  k_hz_insert<<<gM, T, 0, st>>>(b);
-  k_hz_check<<<gM, T, 0, st>>>(b);
-  k_hz_clear<<<gM, T, 0, st>>>(b);
-  k_prio_rank<<<gE, T, 0, st>>>(b);
-  k_prio_fallback<<<grid_for(nflat, T, 2048), T, 0, st>>>(b);
+  k_zero_entries<<<dim3(gE, 1, b.nf), T, 0, st>>>(b);
+  k_pct_rank<<<dim3(gE, 1, b.nf), T, 0, st>>>(b);
+  k_pct_fallback<<<dim3(grid_for(nflat, T, 2048), 1, b.nf), T, 0, st>>>(b);
+  k_pct_final<<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p);
+  k_hz_insert<<<dim3(gM, 1, b.nf), T, 0, st>>>(b);
+  k_hz_check<<<dim3(gM, 1, b.nf), T, 0, st>>>(b);
+  k_hz_clear<<<dim3(gM, 1, b.nf), T, 0, st>>>(b);
+  k_prio_rank<<<dim3(gE, 1, b.nf), T, 0, st>>>(b);
+  k_prio_fallback<<<dim3(grid_for(nflat, T, 2048), 1, b.nf), T, 0, st>>>(b);
   // the first rounds decide almost every candidate; one workgroup finishes
   // the few left
-  for (int r = 0; r < kMisFull; ++r) k_mis_round<<<gM, T, 0, st>>>(b, p, r);
-  k_mis_compact<<<gM, T, 0, st>>>(b);
-  k_mis_finish<<<1, kMisFinishThreads, 0, st>>>(b, p);
-  k_replace_check<<<gM, T, 0, st>>>(b, p);
+  for (int r = 0; r < kMisFull; ++r) k_mis_round<<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p, r);
+  k_mis_compact<<<dim3(gM, 1, b.nf), T, 0, st>>>(b);
+  k_mis_finish<<<dim3(1, 1, b.nf), kMisFinishThreads, 0, st>>>(b, p);
+  k_replace_check<<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p);
 }
 
 void launch_stereo(const StereoBuffers &b, const StereoParams &p,
@@ -1615,27 +1646,27 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   const long ncell = (long)b.rows_cell * b.cols_cell;
   const int T = 256;
   // counters
-  zero_i32(b.n_cand, 1, st);
-  zero_i32(b.n_sel, 1, st);
-  zero_i32(b.work, 1, st);
-  zero_i32(b.undecided, max_rounds, st);
-  zero_i32(b.cell_cnt, ncell * b.n_cam, st);
+  zero_i32(b, b.n_cand, 1, st);
+  zero_i32(b, b.n_sel, 1, st);
+  zero_i32(b, b.work, 1, st);
+  zero_i32(b, b.undecided, max_rounds, st);
+  zero_i32(b, b.cell_cnt, ncell * b.n_cam, st);
   // LOS + cell index
-  dim3 gc(grid_for(b.cap, T, 1024), b.n_cam);
+  dim3 gc(grid_for(b.cap, T, 1024), b.n_cam, b.nf);
   k_los<<<gc, T, 0, st>>>(b, p);
   k_cell_count<<<gc, T, 0, st>>>(b, p);
   scan_exclusive(b.cell_cnt, b.cell_start, ncell, b.n_cam, ncell, nullptr,
-                 b.scan_scratch, st);
+                 b.scan_scratch, st, b.nf, b.fstride);
   k_cell_fill<<<gc, T, 0, st>>>(b, p);
   // build: level 1 from every reference observation, then one enumerate +
   // extend pair of kernels per deeper build camera
   const int G = b.grid_build;
   stage_begin(ST_BUILD, st);
-  zero_i32(b.n_surv, kMaxCam, st);
-  zero_i32(b.n_pairs, 1, st);
-  k_b_second<<<grid_for(b.cap, T, 1024), T, 0, st>>>(b, p);
-  k_b_band<<<G, T, 0, st>>>(b, p);
-  k_b_extend<1><<<G, T, 0, st>>>(b, p);
+  zero_i32(b, b.n_surv, kMaxCam, st);
+  zero_i32(b, b.n_pairs, 1, st);
+  k_b_second<<<dim3(grid_for(b.cap, T, 1024), 1, b.nf), T, 0, st>>>(b, p);
+  k_b_band<<<dim3(G, 1, b.nf), T, 0, st>>>(b, p);
+  k_b_extend<1><<<dim3(G, 1, b.nf), T, 0, st>>>(b, p);
   if (p.build_count > 2) build_level<2>(b, p, st);
   if (p.build_count > 3) build_level<3>(b, p, st);
   if (p.build_count > 4) build_level<4>(b, p, st);
@@ -1646,27 +1677,27 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   stage_begin(ST_PRUNE, st);
   // sort
   const int gM = grid_for(b.cap_cand, T, b.grid_prune);
-  zero_i32(b.key_cnt, b.cap, st);
-  k_key_hist<<<gM, T, 0, st>>>(b, p);
-  scan_exclusive(b.key_cnt, b.key_start, b.cap, 1, b.cap, nullptr, b.scan_scratch, st);
-  k_key_scatter<<<gM, T, 0, st>>>(b, p);
-  k_bucket_rank<<<gM, T, 0, st>>>(b, p);
+  zero_i32(b, b.key_cnt, b.cap, st);
+  k_key_hist<<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p);
+  scan_exclusive(b.key_cnt, b.key_start, b.cap, 1, b.cap, nullptr, b.scan_scratch, st, b.nf, b.fstride);
+  k_key_scatter<<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p);
+  k_bucket_rank<<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p);
   prune_section(b, p, gM, st);
   // triangulate + compact in candidate order
   if (b.n_cam <= 4)
-    k_tri<4><<<gM, T, 0, st>>>(b, p);
+    k_tri<4><<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p);
   else
-    k_tri<kMaxCam><<<gM, T, 0, st>>>(b, p);
+    k_tri<kMaxCam><<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p);
   scan_exclusive(b.keep, b.keep_off, b.cap_cand, 1, b.cap_cand, b.totals,
-                 b.scan_scratch, st);
-  k_obj_write<<<gM, T, 0, st>>>(b);
-  k_obj_count<<<1, 64, 0, st>>>(b);
+                 b.scan_scratch, st, b.nf, b.fstride);
+  k_obj_write<<<dim3(gM, 1, b.nf), T, 0, st>>>(b);
+  k_obj_count<<<dim3(1, 1, b.nf), 64, 0, st>>>(b);
   stage_end(ST_PRUNE, st);
 }
 
 void launch_prune(const StereoBuffers &b, const StereoParams &p, int max_rounds,
                   hipStream_t st) {
-  zero_i32(b.undecided, max_rounds, st);
+  zero_i32(b, b.undecided, max_rounds, st);
   prune_section(b, p, grid_for(b.cap_cand, 256, b.grid_prune), st);
 }
 

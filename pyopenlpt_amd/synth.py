@@ -88,7 +88,8 @@ class OlptStbParams(ctypes.Structure):
 class OlptTuning(ctypes.Structure):
     """olpt_tuning (include/olpt_c.h): result-neutral launch shapes."""
     _fields_ = [("n_streams", ctypes.c_int32), ("shake_blocks", ctypes.c_int32),
-                ("build_blocks", ctypes.c_int32), ("prune_blocks", ctypes.c_int32)]
+                ("build_blocks", ctypes.c_int32), ("prune_blocks", ctypes.c_int32),
+                ("frames_per_batch", ctypes.c_int32)]
 
     def as_dict(self) -> dict:
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

@@ -85,10 +85,13 @@ def test_reduced_subsets_match_reference():
     ctx.close()
 
 
-@pytest.mark.parametrize("streams", [1, 3])
-def test_batched_frames_equal_single_frames(case, streams):
-    """Frames of one call, on one stream or spread over several workspaces
-    and streams, give the single-frame results."""
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("streams,batch", [(1, 1), (3, 1), (1, 5), (1, 3), (2, 2)])
+def test_batched_frames_equal_single_frames(case, streams, batch, precision):
+    """Frames of one call - on one stream, spread over several workspaces
+    and streams, or several frames per kernel launch (frames_per_batch,
+    chunks of 5, 3 + 2 and 2 + 2 + 1 frames) - give the single-frame results
+    bit for bit."""
     from pyopenlpt_amd.context import to_device_image
     cases = [Case(n) for n in ("golden_u8_256", "golden_reduced_256")]
     cams_ok = all(np.array_equal(cases[0].z["cameras"], c.z["cameras"]) for c in cases)
@@ -96,17 +99,22 @@ def test_batched_frames_equal_single_frames(case, streams):
         cases = [case, case]
     cases = cases + cases[::-1] + cases[:1]          # 5 frames: uneven over 3 streams
     imgs = np.stack([c.images for c in cases])
-    ctx = make_ctx(case, "f64")
-    ctx.set_concurrency(streams)
+    ctx = make_ctx(case, precision)
+    ctx.set_tuning(n_streams=streams, frames_per_batch=batch)
     d, code = to_device_image(imgs)
-    (ox, _u, n_obj), _lt = ctx.ipr_run(d, code, case.params, n_frames=len(cases),
-                                       cap_obj=1 << 14)
+    (ox, ou, n_obj), lt = ctx.ipr_run(d, code, case.params, n_frames=len(cases),
+                                      cap_obj=1 << 14)
+    single = make_ctx(case, precision)
     for f, c in enumerate(cases):
         d1, _ = to_device_image(c.images)
-        (o1, _u1, n1), _ = ctx.ipr_run(d1, code, case.params, cap_obj=1 << 14)
+        (o1, u1, n1), lt1 = single.ipr_run(d1, code, case.params, cap_obj=1 << 14)
         k = int(n1[0].item())
-        assert int(n_obj[f].item()) == k
-        assert torch.equal(ox[f, :k], o1[0, :k])
+        assert int(n_obj[f].item()) == k, f
+        assert torch.equal(ox[f, :k], o1[0, :k]), f
+        assert torch.equal(ou[f, :k], u1[0, :k]), f
+        assert np.array_equal(lt[f], lt1[0]), f
+    assert ctx.diagnostics().tolist() == single.diagnostics().tolist() == [0] * 8
+    single.close()
     ctx.close()
 
 
@@ -137,16 +145,18 @@ def test_graph_replays_match_eager(case):
     ctx.close()
 
 
-def test_graph_with_streams_matches_eager(case):
-    """A captured multi-frame run that forks over two streams replays to the
-    eager result, with eager work between replays."""
+@pytest.mark.parametrize("batch", [1, 2])
+def test_graph_with_streams_matches_eager(case, batch):
+    """A captured multi-frame run that forks over two streams (one or two
+    frames per launch) replays to the eager result, with eager work between
+    replays."""
     from pyopenlpt_amd.context import to_device_image
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
         imgs = np.stack([case.images, case.images[:, ::-1, :].copy(), case.images])
         d, code = to_device_image(imgs)
         ctx = make_ctx(case, "f32", stream=s)
-        ctx.set_concurrency(2)
+        ctx.set_tuning(n_streams=2, frames_per_batch=batch)
         outs_e = ctx.alloc_outputs(3, 1 << 14, d.device)
         ctx.ipr_run(d, code, case.params, n_frames=3, cap_obj=1 << 14, outputs=outs_e)
         s.synchronize()
@@ -179,7 +189,9 @@ def test_tuning_leaves_results_unchanged(case, precision):
     ref = None
     for t in ({}, dict(shake_blocks=64, build_blocks=64, prune_blocks=64),
               dict(n_streams=3, shake_blocks=16384, build_blocks=65536, prune_blocks=8192),
-              dict(n_streams=2, shake_blocks=1000, build_blocks=777, prune_blocks=129)):
+              dict(n_streams=1, frames_per_batch=3),
+              dict(n_streams=2, shake_blocks=1000, build_blocks=777, prune_blocks=129,
+                   frames_per_batch=2)):
         ctx.set_tuning(**t)
         (ox, _u, n_obj), lt = ctx.ipr_run(d, code, case.params, n_frames=3, cap_obj=1 << 14)
         torch.cuda.synchronize()
@@ -193,7 +205,7 @@ def test_tuning_leaves_results_unchanged(case, precision):
             assert torch.equal(got[1][f], ref[1][f]), (t, f)
         assert np.array_equal(got[2], ref[2]), t
     assert ctx.get_tuning() == dict(n_streams=2, shake_blocks=1000, build_blocks=777,
-                                    prune_blocks=129)
+                                    prune_blocks=129, frames_per_batch=2)
     ctx.close()
 
 
@@ -202,7 +214,8 @@ def test_tuning_rejects_out_of_range_and_autotune_keeps_a_candidate(case):
     from pyopenlpt_amd.context import to_device_image
     ctx = make_ctx(case, "f32")
     before = ctx.get_tuning()
-    for bad in (dict(shake_blocks=0), dict(build_blocks=1 << 20), dict(n_streams=9)):
+    for bad in (dict(shake_blocks=0), dict(build_blocks=1 << 20), dict(n_streams=9),
+                dict(frames_per_batch=0)):
         with pytest.raises(FatalError) as e:
             ctx.set_tuning(**bad)
         assert e.value.status == 1
