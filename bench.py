@@ -5,12 +5,16 @@ Workload: the ppp = 0.05 configuration - 4 pinhole cameras, 1024 x 1024
 uint16 images, 52,429 particles per frame drawn from 8 Gaussian clusters
 (sigma 5 mm) in a 40 mm cube, OTF a ~ U(800, 1200), b = c = 1.2, noise
 sigma 10; 4 IPR loops.  A "step" runs IPR on a batch of 64 frames resident
-in HBM, spread over 4 HIP streams.  Two legs are timed: plain shake (the
-reference schedule, 6 iterations from 0.25 voxel) and the learned proposal
-(the BF16 MLP trained here on separately seeded frames, then 1 plain
-iteration from 0.03 voxel).  value = reconstructed particles per second of
-the learned leg when its 3D RMSE against ground truth is not above the plain
-leg's, else of the plain leg; both legs are reported in config.
+in HBM: every kernel launch covers 16 frames (the frame is blockIdx.z) and
+the four 16-frame batches run on 4 HIP streams; the step is replayed from a
+captured hipGraph.  Legs: plain shake (the reference schedule, 6 iterations
+from 0.25 voxel) in FP32 and in FP64, the plain step issued one frame per
+call on one stream (how a drop-in runIPR replacement is driven), and the
+learned proposal (the BF16 MLP trained here on separately seeded frames,
+then 1 plain iteration from 0.03 voxel).  value = reconstructed particles
+per second of the learned leg when its 3D RMSE against ground truth is not
+above the plain leg's, else of the plain leg; every leg is reported in
+config.
 
 One JSON line is printed on stdout; progress goes to stderr.
 """
@@ -43,26 +47,25 @@ def _frame(args):
 
 
 def make_frames(n_frames: int, ppp: float, size: int):
-    """Seeded frames (seed 1234 + f), rendered in a process pool."""
+    """Seeded frames (seed 1234 + f), rendered in a pool of spawned workers
+    (joined before returning)."""
     import concurrent.futures as cf
+    import multiprocessing as mp
     workers = max(1, min(16, os.cpu_count() or 1, n_frames))
     if workers == 1:
         return [_frame((f, ppp, size)) for f in range(n_frames)]
-    with cf.ProcessPoolExecutor(max_workers=workers) as ex:
+    ex = cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn"))
+    try:
         return list(ex.map(_frame, [(f, ppp, size) for f in range(n_frames)]))
+    finally:
+        ex.shutdown(wait=True)
 
 
 def quality(xyz: np.ndarray, truth: np.ndarray, vox: float) -> dict:
-    """find_ghosts semantics (demo/openLPT_utils/ghost_eval.py:72-92): a
-    reconstruction within 1 voxel of a ground-truth particle is real."""
-    from scipy.spatial import cKDTree
-    if len(xyz) == 0:
-        return dict(rmse_mm=None, recon_rate=0.0, ghost_rate=None)
-    d, i = cKDTree(truth).query(xyz)
-    real = d < vox
-    return dict(rmse_mm=float(np.sqrt(np.mean(d[real] ** 2))) if real.any() else None,
-                recon_rate=float(len(np.unique(i[real])) / len(truth)),
-                ghost_rate=float(1.0 - real.mean()))
+    """find_ghosts semantics (demo/openLPT_utils/ghost_eval.py:72-92) at a
+    1-voxel tolerance."""
+    from pyopenlpt_amd import synth as S
+    return S.ghost_eval(xyz, truth, vox)
 
 
 CPU_CHILD = r"""
@@ -71,7 +74,7 @@ import numpy as np
 sys.path.insert(0, sys.argv[1])
 import oracle as O
 from pyopenlpt_amd import synth as S
-ppp, size, kind = float(sys.argv[2]), int(sys.argv[3]), sys.argv[4]
+ppp, size, kind, reps = float(sys.argv[2]), int(sys.argv[3]), sys.argv[4], int(sys.argv[5])
 w = S.workload_c3_frame(0, ppp=ppp, size=size)
 H, W = w.images.shape[1:]
 cams = O.cams_array(w.cams)
@@ -84,29 +87,53 @@ if kind == "reference":
     lib = O.load_ref(); fn = lib.ref_run_ipr
 else:
     lib = O.load_port(); fn = lib.orc_run_ipr
-t = time.perf_counter()
-rc = fn(cams, 4, O.dptr(imgs), H, W, ctypes.byref(w.params), ctypes.byref(otf), cap,
-        O.dptr(ox), O.dptr(ou), 64, O.iptr(lt), O.iptr(cnt))
-el = time.perf_counter() - t
-print(json.dumps(dict(rc=int(rc), n=int(cnt[0]), seconds=el)))
+times = []
+for r in range(1 + reps):                      # 1 warm-up, then reps timed runs
+    t = time.perf_counter()
+    rc = fn(cams, 4, O.dptr(imgs), H, W, ctypes.byref(w.params), ctypes.byref(otf), cap,
+            O.dptr(ox), O.dptr(ou), 64, O.iptr(lt), O.iptr(cnt))
+    if r:
+        times.append(time.perf_counter() - t)
+    assert rc == 0
+print(json.dumps(dict(n=int(cnt[0]), times=times)))
 """
 
 
-def cpu_baseline(ppp: float, size: int, threads: int) -> dict | None:
+def cpu_model() -> str:
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return "unknown"
+
+
+def cpu_baseline(ppp: float, size: int, threads: int, reps: int = 3) -> dict | None:
+    """The reference's IPR::runIPR (or the C restatement when the reference
+    build is absent) on frame 0 of the workload with OpenMP on `threads`
+    host threads: 1 warm-up, then the median of `reps` runs (wall time
+    around runIPR only)."""
     import oracle as O
+    import statistics
     kind = "reference" if O.ref_available() else "port"
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
-    log(f"cpu baseline: {kind} IPR, 1 frame, {threads} threads")
-    r = subprocess.run([sys.executable, "-c", CPU_CHILD, ROOT, str(ppp), str(size), kind],
-                       capture_output=True, text=True, env=env, timeout=900)
+    log(f"cpu baseline: {kind} IPR, frame 0, {threads} threads, 1 warm-up + {reps} runs")
+    r = subprocess.run([sys.executable, "-c", CPU_CHILD, ROOT, str(ppp), str(size), kind, str(reps)],
+                       capture_output=True, text=True, env=env, timeout=1800)
     if r.returncode != 0:
         log("cpu baseline failed:", r.stderr[-2000:])
         return None
     res = json.loads(r.stdout.strip().splitlines()[-1])
-    return {"value": round(res["n"] / res["seconds"], 1), "unit": "particles/s",
-            "cores": threads, "kind": kind,
-            "sample": (f"1 frame of the same workload (frame 0, {res['n']} objects, "
-                       f"{res['seconds']:.2f} s wall, IPR::runIPR only)")}
+    med = statistics.median(res["times"])
+    return {"value": round(res["n"] / med, 1), "unit": "particles/s", "cores": threads,
+            "kind": kind,
+            "sample": (f"frame 0 of the same workload ({res['n']} objects), IPR::runIPR wall time, "
+                       f"median of {reps} after 1 warm-up: {med:.2f} s (runs "
+                       f"{', '.join(f'{t:.2f}' for t in res['times'])} s)"),
+            "host": {"nproc": os.cpu_count(), "cpu_model": cpu_model(),
+                     "omp_threads": threads}}
 
 
 def traffic_per_launch(workload: str):
@@ -289,20 +316,29 @@ def main() -> int:
     ap.add_argument("--ppp", type=float, default=0.05)
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
-    ap.add_argument("--graph", action="store_true", help="replay a captured hipGraph")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch the kernels each step instead of replaying a captured hipGraph")
     ap.add_argument("--streams", type=int, default=4,
-                    help="frames of a step run concurrently on this many HIP streams")
+                    help="batches of a step run concurrently on this many HIP streams")
+    ap.add_argument("--batch", type=int, default=16,
+                    help="frames covered by every kernel launch (frames_per_batch)")
+    ap.add_argument("--no-fp64-leg", dest="fp64_leg", action="store_false",
+                    help="skip the FP64 timing of the plain step")
+    ap.add_argument("--no-per-frame-leg", dest="per_frame_leg", action="store_false",
+                    help="skip the one-frame-per-call timing of the plain step")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes instead of timing candidates "
                          "on the first frames (untimed) and keeping the fastest")
-    ap.add_argument("--autotune-frames", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--roofline-steps", type=int, default=1,
                     help="single-stream steps timed per k_shake launch for the roofline")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="after the run, an elementwise kernel that reads and writes this many "
                          "MiB (counter calibration under rocprofv3 --pmc)")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int,
+                    default=int(os.environ.get("OMP_NUM_THREADS", 0)) or (os.cpu_count() or 1),
+                    help="OpenMP threads of the CPU baseline (default: OMP_NUM_THREADS, the "
+                         "host's CPU allotment, else every core)")
     ap.add_argument("--shake", default="both", choices=["plain", "both"],
                     help="both: also train the learned proposal and time its leg")
     ap.add_argument("--train-particles", type=int, default=6_000_000)
@@ -323,7 +359,7 @@ def main() -> int:
     from pyopenlpt_amd import synth as S
     from pyopenlpt_amd.context import Context, to_device_image
 
-    if args.graph:
+    if not args.eager:
         # graph capture needs a created stream; make it torch's current one
         torch.cuda.set_stream(torch.cuda.Stream())
 
@@ -332,97 +368,151 @@ def main() -> int:
     frames = make_frames(args.frames, args.ppp, args.size)
     w0 = frames[0]
     H, W = w0.images.shape[1:]
-    batch = np.stack([f.images for f in frames])          # [F][4][H][W] uint16
-    dimg, code = to_device_image(batch)
+    batch_imgs = np.stack([f.images for f in frames])     # [F][4][H][W] uint16
+    dimg, code = to_device_image(batch_imgs)
     otf, _keep = S.make_otf(4, w0.otf_abca)
-    ctx = Context(w0.cams, H, W, precision=args.precision)
+    # capacities for this workload: ~35k peaks per camera, ~490k stereo
+    # candidates and ~21k objects per loop
+    caps = dict(max_2d=1 << 16, max_cand=1 << 20, max_obj=1 << 17)
+    ctx = Context(w0.cams, H, W, precision=args.precision, **caps)
     ctx.set_otf(otf)
-    ctx.set_concurrency(args.streams)
+    ctx.set_tuning(n_streams=args.streams, frames_per_batch=args.batch)
     cap_obj = 1 << 17
     outs = ctx.alloc_outputs(args.frames, cap_obj, dimg.device)
     stream = torch.cuda.current_stream()
     tune_log = None
     if not args.no_autotune:
-        nt = min(args.frames, args.autotune_frames)
         t0 = time.perf_counter()
-        best, tune_log = ctx.autotune(dimg[:nt], code, w0.params, nt)
+        space = {k: v for k, v in Context.TUNING_SPACE.items() if k != "n_streams"}
+        best, tune_log = ctx.autotune(dimg, code, w0.params, args.frames, space=space, reps=1)
         for t, sec in tune_log:
             log(f"  {t}: {1000 * sec:.1f} ms")
-        log(f"autotune over {len(tune_log)} shapes on {nt} frames "
+        log(f"autotune over {len(tune_log)} shapes on {args.frames} frames "
             f"({time.perf_counter() - t0:.1f} s): {best}")
     tuning = ctx.get_tuning()
 
-    def timed_leg(params, label):
-        """Warm-up, then args.steps timed steps with events on the context
+    def timed_leg(c, params, label, steps, graph):
+        """Warm-up, then `steps` timed steps with events on the context
         stream; returns ms/step, particles/step and quality over the batch."""
         def step():
-            if args.graph:
-                ctx.graph_launch()
+            if graph:
+                c.graph_launch()
             else:
-                ctx.ipr_run(dimg, code, params, n_frames=args.frames, cap_obj=cap_obj,
-                            outputs=outs)
-        if args.graph:
-            ctx.graph_capture(dimg, code, params, args.frames, outs, cap_obj)
+                c.ipr_run(dimg, code, params, n_frames=args.frames, cap_obj=cap_obj, outputs=outs)
+        if graph:
+            c.graph_capture(dimg, code, params, args.frames, outs, cap_obj)
         for _ in range(args.warmup):
             step()
         torch.cuda.synchronize()
-        ctx.stage_times()      # reset the host-issue counter
+        c.stage_times()      # reset the host-issue counter
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t0 = time.perf_counter()
         e0.record(stream)
-        for _ in range(args.steps):
+        for _ in range(steps):
             step()
         e1.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        h_ms, h_calls = ctx.stage_times()["host_issue"]
-        ms = e0.elapsed_time(e1) / args.steps
+        h_ms, h_calls = c.stage_times()["host_issue"]
+        ms = e0.elapsed_time(e1) / steps
         n_obj = outs[2].cpu().numpy()
         per_step = int(n_obj.sum())
         rate = per_step / (ms / 1000.0)
-        log(f"{label}: {args.steps} steps: {ms:.2f} ms/step (wall {1000 * wall / args.steps:.2f}, "
+        log(f"{label}: {steps} steps: {ms:.2f} ms/step (wall {1000 * wall / steps:.2f}, "
             f"host issue {h_ms / max(1, h_calls):.1f}), {per_step} particles/step -> "
             f"{rate:.0f} particles/s")
         q = [quality(outs[0][f, : n_obj[f]].cpu().numpy(), frames[f].truth, frames[f].voxel_mm)
              for f in range(args.frames)]
         rmse = [x["rmse_mm"] for x in q if x["rmse_mm"] is not None]
+        recon = float(np.mean([x["recon_rate"] for x in q]))
         qual = dict(rmse_mm=float(np.mean(rmse)) if rmse else None,
                     rmse_px=float(np.mean(rmse) / w0.voxel_mm) if rmse else None,
-                    recon_rate=float(np.mean([x["recon_rate"] for x in q])),
+                    recon_rate=recon,
                     ghost_rate=float(np.mean([x["ghost_rate"] for x in q])))
+        true_rate = recon * sum(len(f.truth) for f in frames) / (ms / 1000.0)
         return dict(particles_per_s=round(rate, 1), ms_per_step=round(ms, 3),
+                    true_particles_per_s=round(true_rate, 1), graph=bool(graph),
                     host_issue_ms_per_step=round(h_ms / h_calls, 3) if h_calls else None,
                     particles_per_step=per_step, shake_loops=int(params.n_shake_loop),
                     quality=qual)
 
-    # ---- plain shake: the reference schedule (6 iterations from 0.25 vox)
-    plain = timed_leg(w0.params, "plain")
-
-    # algorithmic bytes per stage launch (instrumented extra run, untimed)
-    ctx.stage_stats(True)
-    ctx.ipr_run(dimg, code, w0.params, n_frames=args.frames, cap_obj=cap_obj, outputs=outs)
-    stats = ctx.stage_stats(False)
-
-    # Roofline pass: the kernel's own launch time.  Concurrent streams share
-    # the GPU, so per-launch event intervals of the throughput run include
-    # waiting on the other streams; this timed region runs the same step on
-    # one stream with HIP events around every k_shake launch.
-    times = {}
-    if not args.graph and args.roofline_steps > 0:
-        ctx.set_concurrency(1)
-        ctx.stage_timing(True)
-        ctx.stage_times()
+    def shake_roofline(c, params, label):
+        """k_shake's own launch time: one eager single-stream step with HIP
+        events around every k_shake launch (concurrent streams would add
+        waiting time to the intervals), and its algorithmic bytes per launch
+        from an instrumented step."""
+        c.stage_stats(True)
+        c.ipr_run(dimg, code, params, n_frames=args.frames, cap_obj=cap_obj, outputs=outs)
+        stats = c.stage_stats(False)
+        c.set_tuning(n_streams=1)
+        c.stage_timing(True)
+        c.stage_times()
         torch.cuda.synchronize()
         for _ in range(args.roofline_steps):
-            ctx.ipr_run(dimg, code, w0.params, n_frames=args.frames, cap_obj=cap_obj,
-                        outputs=outs)
+            c.ipr_run(dimg, code, params, n_frames=args.frames, cap_obj=cap_obj, outputs=outs)
         torch.cuda.synchronize()
-        times = ctx.stage_times()
-        ctx.stage_timing(False)
-        ctx.set_concurrency(tuning["n_streams"])
+        times = c.stage_times()
+        c.stage_timing(False)
+        c.set_tuning(n_streams=tuning["n_streams"])
+        if not (times.get("shake", (0, 0))[1] and stats.get("shake", (0, 0))[1]):
+            return None
+        ms_dom, calls = times["shake"]
+        bytes_dom, launches = stats["shake"]
+        per_launch = bytes_dom / launches
+        avg_ms = ms_dom / calls
+        achieved = per_launch / (avg_ms / 1000.0) / 1e9
+        return {"bound": "hbm", "kernel": "k_shake", "leg": label,
+                "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 6),
+                "traffic": traffic_per_launch(workload),
+                "bytes_per_launch": int(per_launch), "avg_launch_ms": round(avg_ms, 4),
+                "frames_per_launch": int(tuning["frames_per_batch"]),
+                "measured": f"{args.roofline_steps} eager single-stream step(s) of the {label} leg, "
+                            "HIP events around each k_shake launch",
+                "stage_ms_per_step_1stream": {k: round(v[0] / args.roofline_steps, 3)
+                                              for k, v in times.items()
+                                              if v[1] and k != "host_issue"}}
+
+    graph = not args.eager
+    # ---- plain shake: the reference schedule (6 iterations from 0.25 vox)
+    plain = timed_leg(ctx, w0.params, "plain", args.steps, graph)
+
+    # ---- the same step in FP64 (the reference's arithmetic): a
+    # same-precision figure beside the CPU baseline
+    plain_f64 = None
+    if args.fp64_leg and args.precision == "f32":
+        ctx.set_precision("f64")
+        plain_f64 = timed_leg(ctx, w0.params, "plain fp64", max(1, args.steps // 2), graph)
+        ctx.set_precision(args.precision)
+
+    # ---- the drop-in pattern: one frame per call, one stream (what a
+    # runIPR replacement does frame by frame), same kernels
+    per_frame = None
+    if args.per_frame_leg:
+        ctx.set_tuning(n_streams=1, frames_per_batch=1)
+        t0 = time.perf_counter()
+        for f in range(args.frames):
+            ctx.ipr_run(dimg[f:f + 1], code, w0.params, cap_obj=cap_obj)
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        n_pf = 0
+        for f in range(args.frames):
+            (_x, _u, n1), _ = ctx.ipr_run(dimg[f:f + 1], code, w0.params, cap_obj=cap_obj)
+            n_pf += int(n1[0].item())
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1)
+        per_frame = dict(particles_per_s=round(n_pf / (ms / 1000.0), 1), ms_per_step=round(ms, 3),
+                         particles_per_step=n_pf,
+                         how="olpt_ipr_run once per frame, one stream, one frame per launch")
+        log(f"plain per-frame: {ms:.1f} ms for {args.frames} frames -> "
+            f"{per_frame['particles_per_s']:.0f} particles/s")
+        ctx.set_tuning(n_streams=tuning["n_streams"], frames_per_batch=tuning["frames_per_batch"])
 
     # ---- learned proposal + 1 plain iteration (from 0.03 vox)
     learned = None
+    lp = None
     if args.shake == "both":
         from pyopenlpt_amd.learned import ShakeNet, make_training_set, train
         t0 = time.perf_counter()
@@ -443,8 +533,7 @@ def main() -> int:
         lp.n_shake_loop = 1
         lp.shake_width = 0.03 * w0.voxel_mm
         ctx.set_shake_model(net, w0.voxel_mm, 0.6)
-        learned = timed_leg(lp, "learned")
-        ctx.set_shake_model(None)
+        learned = timed_leg(ctx, lp, "learned", args.steps, graph)
         learned["train"] = dict(samples=tstats["samples"], frames=tstats["frames"],
                                 particles=tstats["particles"], steps=args.train_steps,
                                 seconds=round(t_train, 2), seeds="9001.. (eval frames 1234..)")
@@ -458,29 +547,19 @@ def main() -> int:
     value = best["particles_per_s"]
     ms_per_step = best["ms_per_step"]
 
-    # Roofline of the dominant kernel: k_shake (the "shake" stage brackets
-    # exactly its launches).  Algorithmic bytes per launch: the per-object
-    # ROI patch gathers N x C x P^2 x s (P = 8, s = 4 bytes in FP32) plus
-    # positions; HBM traffic per launch from the committed rocprofv3
-    # FETCH_SIZE / WRITE_SIZE summary of this workload, when present.
+    # Roofline of k_shake on the leg `value` reports.  Algorithmic bytes per
+    # launch: the per-object ROI patch gathers N x C x P^2 x s (P = 8, s = 4
+    # bytes in FP32) plus positions, over the launch's frames; HBM traffic per
+    # launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
+    # this workload, when present.
     roofline = None
-    if times and times.get("shake", (0, 0))[1] and stats.get("shake", (0, 0))[1]:
-        ms_dom, calls = times["shake"]
-        bytes_dom, launches = stats["shake"]
-        per_launch_bytes = bytes_dom / launches
-        avg_ms = ms_dom / calls
-        achieved = per_launch_bytes / (avg_ms / 1000.0) / 1e9
-        roofline = {"bound": "hbm", "kernel": "k_shake", "achieved": round(achieved, 3),
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 6),
-                    "traffic": traffic_per_launch(workload),
-                    "bytes_per_launch": int(per_launch_bytes),
-                    "avg_launch_ms": round(avg_ms, 4),
-                    "measured": f"{args.roofline_steps} single-stream step(s) of the plain leg, "
-                                "HIP events around each k_shake launch",
-                    "stage_ms_per_step_1stream": {k: round(v[0] / args.roofline_steps, 3)
-                                                  for k, v in times.items()
-                                                  if v[1] and k != "host_issue"}}
+    if args.roofline_steps > 0:
+        if use_learned:
+            roofline = shake_roofline(ctx, lp, "learned")
+        else:
+            ctx.set_shake_model(None)
+            roofline = shake_roofline(ctx, w0.params, "plain")
+    ctx.set_shake_model(None)
 
     if args.copy_calib_mib > 0:
         nel = args.copy_calib_mib * (1 << 20) // 4
@@ -491,6 +570,8 @@ def main() -> int:
         torch.cuda.synchronize()
         log(f"calibration: 3 x neg over {args.copy_calib_mib} MiB")
 
+    ratio = (lambda a, b: round(a["particles_per_s"] / b["particles_per_s"], 3)
+             if a and b else None)
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "particles/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -501,11 +582,14 @@ def main() -> int:
                    "frames_per_step": args.frames, "particles_per_frame": len(w0.truth),
                    "ipr_loops": int(w0.params.n_loop_ipr),
                    "shake": "learned" if use_learned else "plain",
-                   "graph": bool(args.graph), "streams": tuning["n_streams"],
-                   "tuning": tuning, "autotuned": not args.no_autotune,
-                   "plain": plain, "learned": learned,
-                   "learned_over_plain": (round(learned["particles_per_s"] / plain["particles_per_s"], 3)
-                                          if learned else None)},
+                   "graph": graph, "streams": tuning["n_streams"],
+                   "frames_per_launch": tuning["frames_per_batch"],
+                   "tuning": tuning, "autotuned": not args.no_autotune, "capacities": caps,
+                   "plain": plain, "plain_fp64": plain_f64, "plain_per_frame": per_frame,
+                   "learned": learned,
+                   "learned_over_plain": ratio(learned, plain),
+                   "learned_batched_over_plain_per_frame": ratio(learned, per_frame),
+                   "plain_batched_over_plain_per_frame": ratio(plain, per_frame)},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -245,7 +245,7 @@ void stereo_init_tables(const StereoBuffers &b, hipStream_t st);
 // Number of slots of the eq_eps check table for a candidate capacity.
 inline long stereo_hz_size(long cap_cand) {
   long n = 1024;
-  while (n < 4 * cap_cand) n <<= 1;
+  while (n < 2 * cap_cand) n <<= 1;
   return n;
 }
 

@@ -1368,8 +1368,9 @@ __global__ void k_replace_check(StereoBuffers bb, StereoParams p) {
 // relate values two buckets apart.  A bucket keeps up to kHzVals distinct
 // values; each value's thread sorts the values of its bucket and the next
 // two and checks every run in that window (a run reaching beyond it spans
-// more than two buckets, so its visible part already fails).  A fuller
-// bucket or a full table sets the flag without a claim either way.  The
+// more than two buckets, so its visible part already fails).  Each distinct
+// value is checked once, by the thread that stored it.  A fuller bucket or
+// a full table sets the flag without a claim either way.  The
 // values checked are the finite percentile scores (type 0) and their
 // candidates' e_check (type 1): the candidates the greedy considers; the
 // others sort after them whatever their order.
@@ -1406,8 +1407,12 @@ __device__ __forceinline__ long hz_insert(const StereoBuffers &b, long long key)
   const long mask = b.hz_size - 1;
   long s = (long)(hz_hash(key) & (unsigned long long)mask);
   for (long t = 0; t < b.hz_size; ++t) {
-    const long long k = atomicCAS((unsigned long long *)&b.hz_key[s],
-                                  (unsigned long long)kHzEmpty, (unsigned long long)key);
+    // a plain read first: most keys are already there (repeated values hit
+    // the same few slots, where atomics would serialise)
+    long long k = __atomic_load_n(&b.hz_key[s], __ATOMIC_RELAXED);
+    if (k == kHzEmpty)
+      k = atomicCAS((unsigned long long *)&b.hz_key[s], (unsigned long long)kHzEmpty,
+                    (unsigned long long)key);
     if (k == kHzEmpty || k == key) return s;
     s = (s + 1) & mask;
   }
@@ -1437,13 +1442,17 @@ __global__ void k_hz_insert(StereoBuffers bb) {
         continue;
       }
       const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
-      bool placed = false;
+      // the thread that stores a value owns it: it alone checks and clears
+      // the bucket for it (repeated values - most percentile scores - do not)
+      bool placed = false, owner = false;
       for (int u = 0; u < kHzVals && !placed; ++u) {
-        const unsigned long long o = atomicCAS(&b.hz_val[s * kHzVals + u], kHzNoVal, bits);
-        placed = o == kHzNoVal || o == bits;
+        unsigned long long o = __atomic_load_n(&b.hz_val[s * kHzVals + u], __ATOMIC_RELAXED);
+        if (o == kHzNoVal) o = atomicCAS(&b.hz_val[s * kHzVals + u], kHzNoVal, bits);
+        owner = o == kHzNoVal;
+        placed = owner || o == bits;
       }
       if (!placed) atomicOr(&b.flags[2], 1);   // bucket full
-      b.hz_slot[2 * j + t] = (int32_t)s;
+      if (owner) b.hz_slot[2 * j + t] = (int32_t)s;
     }
   }
 }
@@ -1484,7 +1493,8 @@ __global__ void k_hz_check(StereoBuffers bb) {
   }
 }
 
-// empty the slots used by this call (every used slot holds an inserted value)
+// empty the slots used by this call (every used slot holds a value whose
+// owner recorded it)
 __global__ void k_hz_clear(StereoBuffers bb) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
