@@ -792,6 +792,7 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   A(b.key_start, int32_t, cap);
   A(b.cand_rank, int32_t, M);
   A(b.bucket_idx, int32_t, M);
+  A(b.bucket_ids, int32_t, (long)M * nc);
   A(b.pt_count, int32_t, (long)nc * cap);
   A(b.pt_start, int32_t, (long)nc * cap);
   A(b.pt_list, int32_t, (long)M * nc);
@@ -803,6 +804,10 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   A(b.head, int32_t, (long)nc * cap);
   A(b.n_entries, int32_t, 1);
   A(b.flat_rank, int32_t, (long)M * nc);
+  A(b.e_ent, double, (long)M * nc);
+  A(b.p_ent, double, (long)M * nc);
+  A(b.ent_q, int32_t, (long)M * nc);
+  A(b.st_ent, int32_t, (long)M * nc);
   A(b.pv, double, (long)M * nc);
   A(b.pct, double, M);
   A(b.state, int8_t, M);
@@ -811,8 +816,7 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   A(b.keep_off, int32_t, M);
   A(b.tri_xyz, double, (long)M * 3);
   b.hz_size = stereo_hz_size(M);
-  A(b.hz_key, long long, b.hz_size);
-  A(b.hz_val, unsigned long long, b.hz_size * kHzVals);
+  A(b.hz_tab, unsigned long long, b.hz_size * kHzWords);
   A(b.hz_slot, int32_t, 2L * M);
   b.cap_obj = O;
   A(b.obj_xyz, double, (long)O * 3);

@@ -90,6 +90,7 @@ struct SurvRec {
 };
 
 constexpr int kHzVals = 4;   // distinct values kept per eq_eps check bucket
+constexpr int kHzWords = 8;  // 64-bit words per table slot (key, values, padding)
 
 struct StereoBuffers {
   int n_cam;
@@ -123,6 +124,7 @@ struct StereoBuffers {
   int32_t *key_start;    // [cap]
   int32_t *cand_rank;    // [cap_cand]
   int32_t *bucket_idx;   // [cap_cand] raw candidate index, grouped by key
+  int32_t *bucket_ids;   // [cap_cand][n_cam] id vectors in bucket_idx order
   // prune
   int32_t *pt_count;     // [n_cam*cap] candidates per flat point
   int32_t *pt_start;     // [n_cam*cap]
@@ -135,6 +137,15 @@ struct StereoBuffers {
   int32_t *head;         // [n_cam*cap] first entry not known to be rejected
   int32_t *n_entries;    // device: total list entries
   int32_t *flat_rank;    // [cap_cand][n_cam]
+  // per list entry, in pt_list order: the entry's candidate's e_check and
+  // percentile score, so that a point's rank loops read its own contiguous
+  // range; ent_q: the pt_list position of candidate j's entry in slot k
+  double *e_ent;         // [cap_cand*n_cam]
+  double *p_ent;         // [cap_cand*n_cam]
+  int32_t *ent_q;        // [cap_cand][n_cam]
+  // state of the candidate at each pt_sorted position (the greedy's scans
+  // read a point's run of states contiguously)
+  int32_t *st_ent;       // [cap_cand*n_cam]
   double *pv;            // [cap_cand][n_cam] percentile per used point
   double *pct;           // [cap_cand]
   int8_t *state;         // [cap_cand] 0 undecided, 1 selected, 2 rejected
@@ -149,9 +160,10 @@ struct StereoBuffers {
   int32_t *n_obj;        // device
   int32_t *n_sel;        // device
   // eq_eps class check of the prune keys (k_hz_*): an open-addressing table
-  // of value buckets, each holding up to kHzVals distinct values (bits)
-  long long *hz_key;          // [hz_size] bucket key, -1 = empty
-  unsigned long long *hz_val; // [hz_size][kHzVals], all ones = empty
+  // of value buckets, one 64-byte slot each: word 0 the bucket key (-1 =
+  // empty), words 1..kHzVals up to kHzVals distinct values (bits, all ones =
+  // empty)
+  unsigned long long *hz_tab; // [hz_size][kHzWords]
   int32_t *hz_slot;           // [2 * cap_cand] slot of each inserted value
   long hz_size;               // power of two
   int32_t *flags;        // [8]: 0 cand overflow, 1 list overflow, 2 eq_eps
@@ -192,6 +204,7 @@ struct StereoBuffers {
     o.key_start = fshift(key_start, d);
     o.cand_rank = fshift(cand_rank, d);
     o.bucket_idx = fshift(bucket_idx, d);
+    o.bucket_ids = fshift(bucket_ids, d);
     o.pt_count = fshift(pt_count, d);
     o.pt_start = fshift(pt_start, d);
     o.pt_list = fshift(pt_list, d);
@@ -203,6 +216,10 @@ struct StereoBuffers {
     o.head = fshift(head, d);
     o.n_entries = fshift(n_entries, d);
     o.flat_rank = fshift(flat_rank, d);
+    o.e_ent = fshift(e_ent, d);
+    o.p_ent = fshift(p_ent, d);
+    o.ent_q = fshift(ent_q, d);
+    o.st_ent = fshift(st_ent, d);
     o.pv = fshift(pv, d);
     o.pct = fshift(pct, d);
     o.state = fshift(state, d);
@@ -214,8 +231,7 @@ struct StereoBuffers {
     o.obj_ids = fshift(obj_ids, d);
     o.n_obj = fshift(n_obj, d);
     o.n_sel = fshift(n_sel, d);
-    o.hz_key = fshift(hz_key, d);
-    o.hz_val = fshift(hz_val, d);
+    o.hz_tab = fshift(hz_tab, d);
     o.hz_slot = fshift(hz_slot, d);
     o.flags = fshift(flags, d);
     o.work = fshift(work, d);

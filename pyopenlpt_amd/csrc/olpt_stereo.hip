@@ -879,9 +879,12 @@ __global__ void k_key_scatter(StereoBuffers bb, StereoParams p) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
   const int pc = primary_cam(p, b.n_cam);
+  const int nc = b.n_cam;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
-    const int key = b.cand_ids[(long)j * b.n_cam + pc];
-    b.bucket_idx[b.key_start[key] + b.cand_rank[j]] = j;
+    const int key = b.cand_ids[(long)j * nc + pc];
+    const int q = b.key_start[key] + b.cand_rank[j];
+    b.bucket_idx[q] = j;
+    for (int c = 0; c < nc; ++c) b.bucket_ids[(long)q * nc + c] = b.cand_ids[(long)j * nc + c];
   }
 }
 
@@ -893,29 +896,34 @@ __device__ __forceinline__ bool ids_less(const int32_t *a, const int32_t *c, int
 
 // Final position of each candidate = bucket start + number of bucket members
 // that precede it lexicographically (equal vectors, which the build never
-// emits, by emission index so positions stay a permutation).
+// emits, by emission index so positions stay a permutation).  One thread per
+// bucket slot: the members' id vectors sit contiguously in bucket_ids, so
+// the threads of one bucket read the same lines.
 __global__ void k_bucket_rank(StereoBuffers bb, StereoParams p) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
   const int pc = primary_cam(p, b.n_cam);
   const int nc = b.n_cam;
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < M; q += gridDim.x * blockDim.x) {
+    const int j = b.bucket_idx[q];
     int32_t idj[kMaxCam];
-    for (int c = 0; c < nc; ++c) idj[c] = b.cand_ids[(long)j * nc + c];
+    for (int c = 0; c < nc; ++c) idj[c] = b.bucket_ids[(long)q * nc + c];
     const int key = idj[pc];
     const int s = b.key_start[key], n = b.key_cnt[key];
     int r = 0;
     if (nc == 4) {
       // four cameras: the members' id vectors are loaded whole, a batch at
       // a time, and compared without branches
+      const int4 *ids4 = reinterpret_cast<const int4 *>(b.bucket_ids);
       for (int t0 = 0; t0 < n; t0 += kRankBatch) {
         int m[kRankBatch];
         int4 im[kRankBatch];
 #pragma unroll
-        for (int u = 0; u < kRankBatch; ++u) m[u] = t0 + u < n ? b.bucket_idx[s + t0 + u] : j;
-#pragma unroll
-        for (int u = 0; u < kRankBatch; ++u)
-          im[u] = *reinterpret_cast<const int4 *>(b.cand_ids + (long)m[u] * 4);
+        for (int u = 0; u < kRankBatch; ++u) {
+          const int t = t0 + u < n ? s + t0 + u : q;
+          m[u] = t == q ? j : b.bucket_idx[t];
+          im[u] = ids4[t];
+        }
 #pragma unroll
         for (int u = 0; u < kRankBatch; ++u) {
           const int32_t a[4] = {im[u].x, im[u].y, im[u].z, im[u].w};
@@ -927,10 +935,10 @@ __global__ void k_bucket_rank(StereoBuffers bb, StereoParams p) {
         }
       }
     } else {
-      for (int t = 0; t < n; ++t) {
-        const int m = b.bucket_idx[s + t];
-        if (m == j) continue;
-        const int32_t *idm = b.cand_ids + (long)m * nc;
+      for (int t = s; t < s + n; ++t) {
+        if (t == q) continue;
+        const int m = b.bucket_idx[t];
+        const int32_t *idm = b.bucket_ids + (long)t * nc;
         int cmp = 0;
         for (int c = 0; c < nc && cmp == 0; ++c)
           cmp = idm[c] < idj[c] ? -1 : (idm[c] > idj[c] ? 1 : 0);
@@ -1005,6 +1013,7 @@ __global__ void k_pt_fill(StereoBuffers bb, StereoParams p) {
   const int M = min(*b.n_cand, b.cap_cand);
   const int nc = b.n_cam;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
+    const double e = b.cand_e_s[j];
     int k = 0;
     for (int c = 0; c < nc; ++c) {
       if (!p.active[c]) continue;
@@ -1014,6 +1023,8 @@ __global__ void k_pt_fill(StereoBuffers bb, StereoParams p) {
       const int q = b.pt_start[f] + b.flat_rank[(long)j * nc + k];
       b.pt_list[q] = (j << 3) | k;
       b.pt_of[q] = (int32_t)f;
+      b.e_ent[q] = e;
+      b.ent_q[(long)j * nc + k] = q;
       ++k;
     }
   }
@@ -1022,8 +1033,10 @@ __global__ void k_pt_fill(StereoBuffers bb, StereoParams p) {
 __global__ void k_zero_entries(StereoBuffers bb) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   const int n = *b.n_entries;
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
     b.pt_occ[q] = 0;
+    b.st_ent[q] = 0;
+  }
 }
 
 // Percentile of each candidate on each used 2D point (StereoMatch.cpp:
@@ -1038,22 +1051,18 @@ __global__ void k_pct_rank(StereoBuffers bb) {
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ne; q += gridDim.x * blockDim.x) {
     const int f = b.pt_of[q];
     const int s0 = b.pt_start[f], n = b.pt_count[f];
-    const int32_t v = b.pt_list[q];
-    const int j = ent_cand(v);
-    const double ej = b.cand_e_s[j];
+    const double ej = b.e_ent[q];
     int r = 0;
     bool tie = false;
+    // the point's entries are e_ent[s0, s0 + n); a candidate holds at most
+    // one entry per point, so the entry itself is skipped by position
     for (int t0 = 0; t0 < n; t0 += kRankBatch) {
-      int m[kRankBatch];
       double em[kRankBatch];
 #pragma unroll
-      for (int u = 0; u < kRankBatch; ++u)
-        m[u] = t0 + u < n ? ent_cand(b.pt_list[s0 + t0 + u]) : j;
-#pragma unroll
-      for (int u = 0; u < kRankBatch; ++u) em[u] = b.cand_e_s[m[u]];
+      for (int u = 0; u < kRankBatch; ++u) em[u] = b.e_ent[t0 + u < n ? s0 + t0 + u : q];
 #pragma unroll
       for (int u = 0; u < kRankBatch; ++u) {
-        if (m[u] == j) continue;
+        if (t0 + u >= n || s0 + t0 + u == q) continue;
         if (eq_eps(em[u], ej))
           tie = true;
         else
@@ -1064,9 +1073,10 @@ __global__ void k_pct_rank(StereoBuffers bb) {
       atomicOr(&b.pt_flag[f], 1);
       continue;
     }
+    const int32_t v = b.pt_list[q];
     const double denom = (double)(n - 1);
     const double avg_rank = 0.5 * (double)(r + r);
-    b.pv[(long)j * b.n_cam + ent_slot(v)] = (denom > 0.0) ? (avg_rank / denom) : 0.0;
+    b.pv[(long)ent_cand(v) * b.n_cam + ent_slot(v)] = (denom > 0.0) ? (avg_rank / denom) : 0.0;
   }
 }
 
@@ -1079,7 +1089,10 @@ __global__ void k_pct_fallback(StereoBuffers bb) {
        f += (long)gridDim.x * blockDim.x) {
     if (!(b.pt_flag[f] & 1)) continue;
     const int n = b.pt_count[f];
-    int32_t *lst = b.pt_list + b.pt_start[f];
+    // sorted in a copy (pt_sorted is scratch until k_prio_rank fills it):
+    // pt_list keeps the order e_ent and ent_q refer to
+    int32_t *lst = b.pt_sorted + b.pt_start[f];
+    for (int i = 0; i < n; ++i) lst[i] = b.pt_list[b.pt_start[f] + i];
     for (int i = 1; i < n; ++i) {
       const int32_t cur = lst[i];
       const double ce = b.cand_e_s[ent_cand(cur)];
@@ -1121,8 +1134,10 @@ __global__ void k_pct_final(StereoBuffers bb, StereoParams p) {
         s += b.pv[(long)j * nc + nf];
         ++nf;
       }
-    b.pct[j] = nf > 0 ? s / (double)nf : INFINITY;
+    const double pj = nf > 0 ? s / (double)nf : INFINITY;
+    b.pct[j] = pj;
     b.state[j] = 0;
+    for (int k = 0; k < nf; ++k) b.p_ent[b.ent_q[(long)j * nc + k]] = pj;
   }
 }
 
@@ -1138,18 +1153,19 @@ __global__ void k_prio_rank(StereoBuffers bb) {
     const int s0 = b.pt_start[f], n = b.pt_count[f];
     const int32_t v = b.pt_list[q];
     const int j = ent_cand(v);
-    const double pj = b.pct[j], ej = b.cand_e_s[j];
+    const double pj = b.p_ent[q], ej = b.e_ent[q];
     int r = 0;
+    // the point's keys are the contiguous ranges [s0, s0 + n) of pt_list,
+    // p_ent and e_ent
     for (int t0 = 0; t0 < n; t0 += kRankBatch) {
       int m[kRankBatch];
       double pm[kRankBatch], em[kRankBatch];
 #pragma unroll
-      for (int u = 0; u < kRankBatch; ++u)
-        m[u] = t0 + u < n ? ent_cand(b.pt_list[s0 + t0 + u]) : j;
-#pragma unroll
       for (int u = 0; u < kRankBatch; ++u) {
-        pm[u] = b.pct[m[u]];
-        em[u] = b.cand_e_s[m[u]];
+        const int t = t0 + u < n ? s0 + t0 + u : q;
+        m[u] = ent_cand(b.pt_list[t]);
+        pm[u] = b.p_ent[t];
+        em[u] = b.e_ent[t];
       }
 #pragma unroll
       for (int u = 0; u < kRankBatch; ++u)
@@ -1193,12 +1209,17 @@ __global__ void k_prio_fallback(StereoBuffers bb) {
 // undecided entry ahead only delays a decision.
 // One candidate's turn: selected (1) when every entry ahead of it on each of
 // its points is rejected, rejected (2) when one is selected, else still
-// undecided (0).  States and heads are read with device-coherent loads.
+// undecided (0).  A decision is written to state[j] and to st_ent at the
+// candidate's position in each of its points' sorted lists, where the scans
+// of later candidates read it; states and heads are read with
+// device-coherent loads.
 __device__ __forceinline__ int mis_decide(const StereoBuffers &b, const StereoParams &p, int j) {
   const int nc = b.n_cam;
   bool out = false, blocked = false;
+  int at[kMaxCam];   // st_ent index of each of the candidate's entries
+  int na = 0;
   int k = 0;
-  for (int c = 0; c < nc && !out; ++c) {
+  for (int c = 0; c < nc; ++c) {
     if (!p.active[c]) continue;
     const int pid = b.cand_ids_s[(long)j * nc + c];
     if (pid < 0) continue;
@@ -1206,19 +1227,19 @@ __device__ __forceinline__ int mis_decide(const StereoBuffers &b, const StereoPa
     const int pos = b.ppos[(long)j * nc + k];
     ++k;
     const int s0 = b.pt_start[f];
+    at[na++] = s0 + pos;
+    if (out) continue;
     const int h0 = __atomic_load_n(&b.head[f], __ATOMIC_RELAXED);
     // first entry ahead of pos that is not rejected, kRankBatch entries per
-    // step (their list and state loads in flight together).  States only
-    // leave 0 for a final 1 or 2, so a batch's snapshot at worst defers a
-    // decision to a later round.
+    // step (their state loads in flight together).  States only leave 0 for
+    // a final 1 or 2, so a batch's snapshot at worst defers a decision to a
+    // later round.
     int h = h0, st = 2;
     while (h < pos) {
       int sv[kRankBatch];
 #pragma unroll
-      for (int u = 0; u < kRankBatch; ++u) {
-        const int e = h + u < pos ? b.pt_sorted[s0 + h + u] : -1;
-        sv[u] = e >= 0 ? (int)__atomic_load_n(&b.state[ent_cand(e)], __ATOMIC_RELAXED) : 2;
-      }
+      for (int u = 0; u < kRankBatch; ++u)
+        sv[u] = h + u < pos ? __atomic_load_n(&b.st_ent[s0 + h + u], __ATOMIC_RELAXED) : 2;
       int adv = kRankBatch;
 #pragma unroll
       for (int u = kRankBatch - 1; u >= 0; --u)
@@ -1237,15 +1258,12 @@ __device__ __forceinline__ int mis_decide(const StereoBuffers &b, const StereoPa
         blocked = true;
     }
   }
-  if (out) {
-    b.state[j] = 2;
-    return 2;
+  const int d = out ? 2 : (blocked ? 0 : 1);
+  if (d != 0) {
+    b.state[j] = (int8_t)d;
+    for (int u = 0; u < na; ++u) __atomic_store_n(&b.st_ent[at[u]], d, __ATOMIC_RELAXED);
   }
-  if (!blocked) {
-    b.state[j] = 1;
-    return 1;
-  }
-  return 0;
+  return d;
 }
 
 __global__ void k_mis_round(StereoBuffers bb, StereoParams p, int round) {
@@ -1322,8 +1340,8 @@ __global__ void k_replace_check(StereoBuffers bb, StereoParams p) {
       ++k;
       const int s0 = b.pt_start[f];
       for (int h = b.head[f]; h < pos; ++h) {
+        if (b.st_ent[s0 + h] != 1) continue;
         const int m = ent_cand(b.pt_sorted[s0 + h]);
-        if (b.state[m] != 1) continue;
         bool seen = false;
         for (int q = 0; q < ns; ++q) seen = seen || S[q] == m;
         if (!seen) S[ns++] = m;
@@ -1376,7 +1394,7 @@ __global__ void k_replace_check(StereoBuffers bb, StereoParams p) {
 // others sort after them whatever their order.
 // --------------------------------------------------------------------------
 constexpr double kHzWidth = 2e-9;
-constexpr long long kHzEmpty = -1;
+constexpr unsigned long long kHzEmpty = ~0ULL;   // key -1: no bucket
 constexpr unsigned long long kHzNoVal = ~0ULL;   // a NaN pattern: never a value
 
 __device__ __forceinline__ long long hz_key(int type, double v) {
@@ -1390,13 +1408,18 @@ __device__ __forceinline__ unsigned long long hz_hash(long long key) {
   return h ^ (h >> 29);
 }
 
+// word w of table slot s (word 0 the key, 1..kHzVals the values)
+__device__ __forceinline__ unsigned long long *hz_word(const StereoBuffers &b, long s, int w) {
+  return b.hz_tab + s * kHzWords + w;
+}
+
 // slot of key, or -1 when absent (find) / when the table is full (insert)
 __device__ __forceinline__ long hz_find(const StereoBuffers &b, long long key) {
   const long mask = b.hz_size - 1;
   long s = (long)(hz_hash(key) & (unsigned long long)mask);
   for (long t = 0; t < b.hz_size; ++t) {
-    const long long k = __atomic_load_n(&b.hz_key[s], __ATOMIC_RELAXED);
-    if (k == key) return s;
+    const unsigned long long k = __atomic_load_n(hz_word(b, s, 0), __ATOMIC_RELAXED);
+    if (k == (unsigned long long)key) return s;
     if (k == kHzEmpty) return -1;
     s = (s + 1) & mask;
   }
@@ -1409,11 +1432,9 @@ __device__ __forceinline__ long hz_insert(const StereoBuffers &b, long long key)
   for (long t = 0; t < b.hz_size; ++t) {
     // a plain read first: most keys are already there (repeated values hit
     // the same few slots, where atomics would serialise)
-    long long k = __atomic_load_n(&b.hz_key[s], __ATOMIC_RELAXED);
-    if (k == kHzEmpty)
-      k = atomicCAS((unsigned long long *)&b.hz_key[s], (unsigned long long)kHzEmpty,
-                    (unsigned long long)key);
-    if (k == kHzEmpty || k == key) return s;
+    unsigned long long k = __atomic_load_n(hz_word(b, s, 0), __ATOMIC_RELAXED);
+    if (k == kHzEmpty) k = atomicCAS(hz_word(b, s, 0), kHzEmpty, (unsigned long long)key);
+    if (k == kHzEmpty || k == (unsigned long long)key) return s;
     s = (s + 1) & mask;
   }
   return -1;
@@ -1430,7 +1451,7 @@ __global__ void k_hz_insert(StereoBuffers bb) {
     const double pv[2] = {b.pct[j], b.cand_e_s[j]};
     for (int t = 0; t < 2; ++t) {
       b.hz_slot[2 * j + t] = -1;
-      if (!isfinite(b.pct[j]) || !isfinite(pv[t])) continue;
+      if (!isfinite(pv[0]) || !isfinite(pv[t])) continue;
       const double v = pv[t] + 0.0;  // -0 -> +0
       if (v < 0.0) {
         atomicOr(&b.flags[2], 1);
@@ -1445,9 +1466,9 @@ __global__ void k_hz_insert(StereoBuffers bb) {
       // the thread that stores a value owns it: it alone checks and clears
       // the bucket for it (repeated values - most percentile scores - do not)
       bool placed = false, owner = false;
-      for (int u = 0; u < kHzVals && !placed; ++u) {
-        unsigned long long o = __atomic_load_n(&b.hz_val[s * kHzVals + u], __ATOMIC_RELAXED);
-        if (o == kHzNoVal) o = atomicCAS(&b.hz_val[s * kHzVals + u], kHzNoVal, bits);
+      for (int u = 1; u <= kHzVals && !placed; ++u) {
+        unsigned long long o = __atomic_load_n(hz_word(b, s, u), __ATOMIC_RELAXED);
+        if (o == kHzNoVal) o = atomicCAS(hz_word(b, s, u), kHzNoVal, bits);
         owner = o == kHzNoVal;
         placed = owner || o == bits;
       }
@@ -1463,15 +1484,15 @@ __global__ void k_hz_check(StereoBuffers bb) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * M; i += gridDim.x * blockDim.x) {
     const long s = b.hz_slot[i];
     if (s < 0) continue;
-    const long long key = b.hz_key[s];
+    const long long key = (long long)*hz_word(b, s, 0);
     // the distinct values of buckets key, key + 1, key + 2, sorted
     double w[3 * kHzVals];
     int n = 0;
     for (int d = 0; d < 3; ++d) {
       const long sd = d == 0 ? s : hz_find(b, key + d);
       if (sd < 0) continue;
-      for (int u = 0; u < kHzVals; ++u) {
-        const unsigned long long bits = b.hz_val[sd * kHzVals + u];
+      for (int u = 1; u <= kHzVals; ++u) {
+        const unsigned long long bits = *hz_word(b, sd, u);
         if (bits == kHzNoVal) continue;
         const double v = hz_val(bits);
         int k = n++;
@@ -1501,18 +1522,15 @@ __global__ void k_hz_clear(StereoBuffers bb) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * M; i += gridDim.x * blockDim.x) {
     const long s = b.hz_slot[i];
     if (s < 0) continue;
-    b.hz_key[s] = kHzEmpty;
-    for (int u = 0; u < kHzVals; ++u) b.hz_val[s * kHzVals + u] = kHzNoVal;
+    for (int u = 0; u <= kHzVals; ++u) *hz_word(b, s, u) = kHzEmpty;
   }
 }
 
 __global__ void k_hz_init(StereoBuffers bb) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   for (long s = blockIdx.x * (long)blockDim.x + threadIdx.x; s < b.hz_size;
-       s += (long)gridDim.x * blockDim.x) {
-    b.hz_key[s] = kHzEmpty;
-    for (int u = 0; u < kHzVals; ++u) b.hz_val[s * kHzVals + u] = kHzNoVal;
-  }
+       s += (long)gridDim.x * blockDim.x)
+    for (int u = 0; u < kHzWords; ++u) *hz_word(b, s, u) = kHzEmpty;
 }
 
 void stereo_init_tables(const StereoBuffers &b, hipStream_t st) {

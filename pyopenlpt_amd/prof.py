@@ -10,6 +10,21 @@
       (KiB per dispatch), each scaled by what the same counter reads for a
       plain elementwise kernel that reads N bytes and writes N bytes.
 
+  python -m pyopenlpt_amd.prof table --stats <run_kernel_stats.csv>
+      --fetch <csv> --write <csv> [--sq <csv>] --calib-kernel neg --calib-bytes N
+      per-kernel table: time share, average duration, HBM bytes per dispatch
+      (calibrated FETCH_SIZE + WRITE_SIZE) and the achieved GB/s they give;
+      with --sq, VALU instructions per wave and the share of wave cycles spent
+      waiting (SQ counters of a third run)
+
+  python -m pyopenlpt_amd.prof step [--frames 16] [--batch 16] [--steps 1]
+      [--shake-loops 6] [--calib-mib 0]
+      the program the profiles are taken on: the ppp 0.05 clustered frames
+      (1024^2 u16, 4 cameras) resident in HBM, one eager olpt_ipr_run over
+      all of them per step on one stream, `batch` frames per launch; then,
+      with --calib-mib, three launches of an elementwise kernel that reads
+      and writes that many MiB (the counters' calibration)
+
 Both print text for profiles/ and, for pmc, the key lines bench.py reads:
   fetch_bytes_per_dispatch: <float>
   write_bytes_per_dispatch: <float>
@@ -85,6 +100,103 @@ def pmc(fetch: str, write: str, kernel: str, calib_kernel: str, calib_bytes: flo
     return "\n".join(lines)
 
 
+def _dispatch_means(path: str, counter: str) -> dict[str, tuple[float, int]]:
+    vals: dict[str, list[float]] = {}
+    for r in csv.DictReader(open(path)):
+        if r.get("Counter_Name") != counter:
+            continue
+        vals.setdefault(_short(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+    return {k: (statistics.mean(v), len(v)) for k, v in vals.items()}
+
+
+def table(stats_csv: str, fetch: str, write: str, sq: str | None, calib_kernel: str,
+          calib_bytes: float, min_pct: float) -> str:
+    rows = list(csv.DictReader(open(stats_csv)))
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    f = _dispatch_means(fetch, "FETCH_SIZE")
+    w = _dispatch_means(write, "WRITE_SIZE")
+    cf = [v for k, v in f.items() if re.search(calib_kernel, k)]
+    cw = [v for k, v in w.items() if re.search(calib_kernel, k)]
+    if not cf or not cw:
+        raise SystemExit(f"calibration kernel {calib_kernel!r} not in the counter runs")
+    scale_f = calib_bytes / cf[0][0]
+    scale_w = calib_bytes / cw[0][0]
+    sqv = {}
+    if sq:
+        for name in ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_WAVE_CYCLES", "SQ_WAIT_INST_ANY",
+                     "SQ_INSTS_VMEM_RD", "SQ_INSTS_LDS"):
+            sqv[name] = _dispatch_means(sq, name)
+    out = [f"calibration: {calib_kernel} reads and writes {calib_bytes:.0f} B per dispatch; "
+           f"FETCH_SIZE x {scale_f:.1f}, WRITE_SIZE x {scale_w:.1f} give bytes", "",
+           f"kernels above {min_pct:g}% of the kernel time ({tot / 1e6:.2f} ms in all):", "",
+           "| kernel | % time | calls | avg us | HBM MB/dispatch (read + write) | achieved GB/s |"
+           + (" VALU insts/wave | wait % of wave cycles |" if sq else ""),
+           "|---|---:|---:|---:|---:|---:|" + ("---:|---:|" if sq else "")]
+    for r in rows:
+        pct = 100.0 * float(r["TotalDurationNs"]) / tot
+        if pct < min_pct:
+            continue
+        k = _short(r["Name"])
+        avg_ns = float(r["AverageNs"])
+        fb = f[k][0] * scale_f if k in f else float("nan")
+        wb = w[k][0] * scale_w if k in w else float("nan")
+        line = (f"| `{k}` | {pct:.1f} | {int(r['Calls'])} | {avg_ns / 1e3:.1f} | "
+                f"{fb / 1e6:.2f} + {wb / 1e6:.2f} | {(fb + wb) / avg_ns:.0f} |")
+        if sq:
+            waves = sqv["SQ_WAVES"].get(k, (float("nan"), 0))[0]
+            valu = sqv["SQ_INSTS_VALU"].get(k, (float("nan"), 0))[0]
+            cyc = sqv["SQ_WAVE_CYCLES"].get(k, (float("nan"), 0))[0]
+            wait = sqv["SQ_WAIT_INST_ANY"].get(k, (float("nan"), 0))[0]
+            line += f" {valu / waves:.0f} | {100.0 * wait / cyc:.0f} |" if waves and cyc else " | |"
+        out.append(line)
+    return "\n".join(out)
+
+
+def step(frames: int, batch: int, steps: int, shake_loops: int, calib_mib: int) -> int:
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    import time
+
+    import numpy as np
+    import torch
+
+    from . import synth as S
+    from .context import Context, to_device_image
+
+    ex = cf.ProcessPoolExecutor(max_workers=min(16, frames),
+                                mp_context=mp.get_context("spawn"))
+    try:
+        ws = list(ex.map(S.workload_c3_frame, range(frames)))
+    finally:
+        ex.shutdown(wait=True)
+    w0 = ws[0]
+    H, W = w0.images.shape[1:]
+    d, code = to_device_image(np.stack([w.images for w in ws]))
+    otf, _keep = S.make_otf(len(w0.cams), w0.otf_abca)
+    ctx = Context(w0.cams, H, W, precision="f32", max_2d=1 << 16, max_cand=1 << 20,
+                  max_obj=1 << 17)
+    ctx.set_otf(otf)
+    ctx.set_tuning(n_streams=1, frames_per_batch=batch)
+    params = S.OlptIprParams.from_buffer_copy(w0.params)
+    params.n_shake_loop = shake_loops
+    outs = ctx.alloc_outputs(frames, 1 << 17, d.device)
+    for s in range(steps):
+        t0 = time.perf_counter()
+        ctx.ipr_run(d, code, params, n_frames=frames, cap_obj=1 << 17, outputs=outs)
+        torch.cuda.synchronize()
+        print(f"step {s}: {1000 * (time.perf_counter() - t0):.1f} ms, "
+              f"{int(outs[2].sum())} objects over {frames} frames", flush=True)
+    if calib_mib > 0:
+        src = torch.ones(calib_mib * (1 << 20) // 4, dtype=torch.float32, device=d.device)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        print(f"calibration: 3 x neg over {calib_mib} MiB", flush=True)
+    ctx.close()
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -97,11 +209,30 @@ def main(argv=None) -> int:
     b.add_argument("--kernel", required=True)
     b.add_argument("--calib-kernel", required=True)
     b.add_argument("--calib-bytes", type=float, required=True)
+    c = sub.add_parser("table")
+    c.add_argument("--stats", required=True)
+    c.add_argument("--fetch", required=True)
+    c.add_argument("--write", required=True)
+    c.add_argument("--sq")
+    c.add_argument("--calib-kernel", required=True)
+    c.add_argument("--calib-bytes", type=float, required=True)
+    c.add_argument("--min-pct", type=float, default=2.0)
+    d = sub.add_parser("step")
+    d.add_argument("--frames", type=int, default=16)
+    d.add_argument("--batch", type=int, default=16)
+    d.add_argument("--steps", type=int, default=1)
+    d.add_argument("--shake-loops", type=int, default=6)
+    d.add_argument("--calib-mib", type=int, default=0)
     args = ap.parse_args(argv)
     if args.cmd == "stats":
         print(stats(args.csv, args.top))
-    else:
+    elif args.cmd == "pmc":
         print(pmc(args.fetch, args.write, args.kernel, args.calib_kernel, args.calib_bytes))
+    elif args.cmd == "table":
+        print(table(args.stats, args.fetch, args.write, args.sq, args.calib_kernel,
+                    args.calib_bytes, args.min_pct))
+    else:
+        return step(args.frames, args.batch, args.steps, args.shake_loops, args.calib_mib)
     return 0
 
 
