@@ -111,6 +111,7 @@ struct StereoBuffers {
   int32_t *cell_cnt;     // [n_cam][ncell]
   int32_t *cell_start;   // [n_cam][ncell]
   int32_t *cell_pid;     // [n_cam][cap]
+  double2 *cell_xy;      // [n_cam][cap] the points' coordinates in cell_pid order
   int32_t *pt_cell;      // [n_cam][cap]
   int32_t *pt_rank;      // [n_cam][cap]
   // candidates
@@ -193,6 +194,7 @@ struct StereoBuffers {
     o.cell_cnt = fshift(cell_cnt, d);
     o.cell_start = fshift(cell_start, d);
     o.cell_pid = fshift(cell_pid, d);
+    o.cell_xy = fshift(cell_xy, d);
     o.pt_cell = fshift(pt_cell, d);
     o.pt_rank = fshift(pt_rank, d);
     o.cand_ids = fshift(cand_ids, d);

@@ -1069,10 +1069,11 @@ static void shake_iter(const ShakeBuffers<T> &b, const ShakeParams &p,
   stage_end(ST_SHAKE, st);
 }
 
-// Objects per wave for up to 4 cameras and an 8x8 ROI: 4 in FP32 (measured
-// fastest; the per-camera projection and OTF work covers 16 pixels per
-// lane), 2 in FP64, whose doubled registers would otherwise spill.
-template <typename T> constexpr int shake_opw() { return sizeof(T) == 4 ? 4 : 2; }
+// Objects per wave for up to 4 cameras and an 8x8 ROI: 8 in FP32 (measured
+// fastest with 16-frame launches: the per-camera projection and OTF work
+// covers 32 pixels per lane; 4 objects per wave took 23% longer, 16 need
+// 277 registers), 2 in FP64, whose doubled registers would otherwise spill.
+template <typename T> constexpr int shake_opw() { return sizeof(T) == 4 ? 8 : 2; }
 
 template <typename T>
 void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st) {

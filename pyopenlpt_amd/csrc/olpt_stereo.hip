@@ -102,6 +102,8 @@ __global__ void k_cell_fill(StereoBuffers bb, StereoParams p) {
     if (cell < 0) continue;
     const int pos = b.cell_start[cam * ncell + cell] + b.pt_rank[(long)cam * b.cap + i];
     b.cell_pid[(long)cam * b.cap + pos] = i;
+    const double *q = b.xy + ((long)cam * b.cap + i) * 2;
+    b.cell_xy[(long)cam * b.cap + pos] = make_double2(q[0], q[1]);
   }
 }
 
@@ -355,19 +357,6 @@ __device__ __forceinline__ void pair_rows(const Strip &si, const Strip &sj, int 
   if (r1 > rows - 1) r1 = rows - 1;
 }
 
-// Per-lane cursor over the contiguous point range of one cell row: within a
-// row, cells cx in [xa, xb] are consecutive in the CSR index.
-struct RowCursor {
-  int kk, kend;
-};
-
-__device__ __forceinline__ void row_range(const int32_t *cstart, const int32_t *ccnt,
-                                          int cy, int cols, int xa, int xb, RowCursor &c) {
-  const int c0 = cy * cols + xa, c1 = cy * cols + xb;
-  c.kk = cstart[c0];
-  c.kend = cstart[c1] + ccnt[c1];
-}
-
 // Block queue: a workgroup collects its output records in LDS and claims
 // room in the global list with one atomic per flush.  Global atomics on one
 // address serialise across the XCDs' L2 slices, so a per-wave atomic on the
@@ -423,6 +412,57 @@ __device__ __forceinline__ void bq_flush(BlockQueue<Rec, CAP> &q, int32_t *count
   }
 }
 
+// bq_push for lanes in divergent control flow (per-lane loops): the
+// pushing lanes of a wave take slots with one LDS atomic; a record that
+// finds the queue full goes straight to the global list.  Drained by
+// bq_drain at the block's barrier points.
+template <typename Rec, int CAP, typename Out>
+__device__ __forceinline__ void bq_push_any(BlockQueue<Rec, CAP> &q, bool want, const Rec &r,
+                                            int32_t *counter, long cap, int32_t *overflow,
+                                            Out out) {
+  const uint64_t m = ballot(want);
+  if (m == 0) return;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  int base = 0;
+  if (lane_id() == leader) base = atomicAdd(&q.n, __popcll(m));
+  base = __shfl(base, leader, kWave);
+  if (want) {
+    const int slot = base + mbcnt(m);
+    if (slot < CAP) {
+      q.buf[slot] = r;
+    } else {
+      const int g = atomicAdd(counter, 1);
+      if (g < cap)
+        out((long)g, r);
+      else
+        atomicOr(overflow, 1);
+    }
+  }
+}
+
+// Write the queued records out once half the queue is used (or any, with
+// `all`); every thread of the block calls it at the same point.
+template <typename Rec, int CAP, typename Out>
+__device__ __forceinline__ void bq_drain(BlockQueue<Rec, CAP> &q, int32_t *counter, long cap,
+                                         int32_t *overflow, bool all, Out out) {
+  __syncthreads();
+  const int n = min(q.n, CAP);
+  if (all ? n > 0 : n >= CAP / 2) {
+    if (threadIdx.x == 0) q.base = atomicAdd(counter, n);
+    __syncthreads();
+    const int base = q.base;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      if ((long)base + i < cap)
+        out((long)base + i, q.buf[i]);
+      else
+        atomicOr(overflow, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) q.n = 0;
+    __syncthreads();
+  }
+}
+
 struct CandRec {
   int32_t ids[kMaxCam];
   double e;
@@ -430,6 +470,7 @@ struct CandRec {
 
 constexpr int kQueue = 2048;      // pair queue capacity (records)
 constexpr int kPtBatch = 4;       // points tested per thread between flushes
+constexpr int kRowsPass = 4;      // cell rows whose ranges are loaded together
 // The rank kernels (bucket, percentile and priority ranks) compare one
 // element with every other member of its list; they take kRankBatch members
 // per step so that the members' dependent loads are in flight together.
@@ -637,16 +678,90 @@ __global__ void __launch_bounds__(256) k_b_second(StereoBuffers bb, StereoParams
   }
 }
 
+// The points of cell rows r0..r1 of camera t that lie in all K strips, each
+// passed to push(accepted, pid).  kRowsPass rows at a time: their spans
+// (FP64 arithmetic), then their CSR ranges loaded together (within a row,
+// the cells of a span are consecutive in the CSR index), then the points in
+// kPtBatch batches from the cell-ordered ids and coordinates - two
+// dependent loads per pass rather than three per row.
+template <int K, typename Push>
+__device__ __forceinline__ void enum_rows(const StereoBuffers &b, int t, const Strip *S,
+                                          const StripRows *SR, int r0, int r1, double tol2,
+                                          Push push) {
+  const int cols = b.cols_cell;
+  const long ncell = (long)b.rows_cell * cols;
+  const int32_t *cstart = b.cell_start + t * ncell;
+  const int32_t *ccnt = b.cell_cnt + t * ncell;
+  const int32_t *cpid = b.cell_pid + (long)t * b.cap;
+  const double2 *cxy = b.cell_xy + (long)t * b.cap;
+  for (int rb = r0; rb <= r1; rb += kRowsPass) {
+    int c0[kRowsPass], c1[kRowsPass];
+#pragma unroll
+    for (int u = 0; u < kRowsPass; ++u) {
+      const int cy = rb + u;
+      c0[u] = -1;
+      c1[u] = -1;
+      if (cy <= r1) {
+        int xa = 0, xb = cols - 1;
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+          int a, bb;
+          if (ok) {
+            ok = row_span(SR[j], cy, cols, a, bb);
+            xa = a > xa ? a : xa;
+            xb = bb < xb ? bb : xb;
+            ok = ok && xa <= xb;
+          }
+        }
+        if (ok) {
+          c0[u] = cy * cols + xa;
+          c1[u] = cy * cols + xb;
+        }
+      }
+    }
+    int ka[kRowsPass], ke[kRowsPass];
+#pragma unroll
+    for (int u = 0; u < kRowsPass; ++u) {
+      ka[u] = 0;
+      ke[u] = 0;
+      if (c0[u] >= 0) {
+        ka[u] = cstart[c0[u]];
+        ke[u] = cstart[c1[u]] + ccnt[c1[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kRowsPass; ++u) {
+      for (int k = ka[u]; k < ke[u]; k += kPtBatch) {
+        int pid[kPtBatch];
+        double2 q[kPtBatch];
+#pragma unroll
+        for (int v = 0; v < kPtBatch; ++v) {
+          const bool in = k + v < ke[u];
+          pid[v] = in ? cpid[k + v] : -1;
+          q[v] = in ? cxy[k + v] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int v = 0; v < kPtBatch; ++v) {
+          bool acc = pid[v] >= 0;
+#pragma unroll
+          for (int j = 0; j < K; ++j) acc = acc && in_strip(S[j], q[v].x, q[v].y, tol2);
+          push(acc, pid[v]);
+        }
+      }
+    }
+  }
+}
+
 // Level-1 band: one thread per (reference observation, run of kBandRows
 // cell rows of its second camera); accepted points become (r, pid) pairs.
-constexpr int kBandRows = 4;
+constexpr int kBandRows = kRowsPass;
 __global__ void __launch_bounds__(256) k_b_band(StereoBuffers bb, StereoParams p) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   int nref;
   const int ref = ref_camera(b, p, nref);
-  const int rows = b.rows_cell, cols = b.cols_cell;
+  const int rows = b.rows_cell;
   const int nrun = (rows + kBandRows - 1) / kBandRows;
-  const long ncell = (long)rows * cols;
   const long total = (long)nref * nrun;
   const double tolg = p.tol2d + 1e-6;
   const double tol2 = p.tol2d * p.tol2d;
@@ -658,48 +773,20 @@ __global__ void __launch_bounds__(256) k_b_band(StereoBuffers bb, StereoParams p
     const long item = base + threadIdx.x;
     const int r = (int)(item / nrun);
     const int t1 = item < total ? (int)b.t1[r] : -1;
-    int cy = (int)(item % nrun) * kBandRows - 1;
-    const int cy_end = min(cy + kBandRows, rows - 1);
-    bool live = t1 >= 0;
-    RowCursor cur{0, 0};
-    Strip s{};
-    StripRows sr{};
-    const double *xy = b.xy;
-    const int32_t *cpid = b.cell_pid;
-    const int32_t *cstart = b.cell_start, *ccnt = b.cell_cnt;
-    if (live) {
-      s = strip_of(line_of(b, ref, t1, r));
-      sr = strip_rows(s, b.cell_px, tolg);
-      xy += (long)t1 * b.cap * 2;
-      cpid += (long)t1 * b.cap;
-      cstart += t1 * ncell;
-      ccnt += t1 * ncell;
+    if (t1 >= 0) {
+      const int r0 = (int)(item % nrun) * kBandRows;
+      const int r1 = min(r0 + kBandRows - 1, rows - 1);
+      Strip S[1];
+      StripRows SR[1];
+      S[0] = strip_of(line_of(b, ref, t1, r));
+      SR[0] = strip_rows(S[0], b.cell_px, tolg);
+      enum_rows<1>(b, t1, S, SR, r0, r1, tol2, [&](bool acc, int pid) {
+        bq_push_any(q, acc, make_int2(r, pid), b.n_pairs, b.cap_pairs, &b.flags[1], out);
+      });
     }
-    // per barrier: up to kPtBatch points of the current row segment (their
-    // loads in flight together), then at most one cell row advanced
-    while (__syncthreads_or(live)) {
-      int pid[kPtBatch];
-#pragma unroll
-      for (int u = 0; u < kPtBatch; ++u)
-        pid[u] = live && cur.kk + u < cur.kend ? cpid[cur.kk + u] : -1;
-      if (live) cur.kk = min(cur.kk + kPtBatch, cur.kend);
-#pragma unroll
-      for (int u = 0; u < kPtBatch; ++u) {
-        const bool acc = pid[u] >= 0 && in_strip(s, xy[2 * pid[u]], xy[2 * pid[u] + 1], tol2);
-        bq_push(q, acc, make_int2(r, pid[u]));
-      }
-      if (live && cur.kk == cur.kend) {
-        if (++cy > cy_end) {
-          live = false;
-        } else {
-          int xa, xb;
-          if (row_span(sr, cy, cols, xa, xb)) row_range(cstart, ccnt, cy, cols, xa, xb, cur);
-        }
-      }
-      bq_flush<kPtBatch>(q, b.n_pairs, b.cap_pairs, &b.flags[1], false, out);
-    }
+    bq_drain(q, b.n_pairs, b.cap_pairs, &b.flags[1], false, out);
   }
-  bq_flush(q, b.n_pairs, b.cap_pairs, &b.flags[1], true, out);
+  bq_drain(q, b.n_pairs, b.cap_pairs, &b.flags[1], true, out);
 }
 
 // Deeper bands: one thread per survivor of level K-1 (a path of length K),
@@ -709,8 +796,7 @@ __global__ void __launch_bounds__(256) k_b_enum(StereoBuffers bb, StereoParams p
   const StereoBuffers b = bb.frame(blockIdx.z);
   int nref;
   const int ref = ref_camera(b, p, nref);
-  const int rows = b.rows_cell, cols = b.cols_cell;
-  const long ncell = (long)rows * cols;
+  const int rows = b.rows_cell;
   const double tolg = p.tol2d + 1e-6;
   const double tol2 = p.tol2d * p.tol2d;
   const int n_in = min(b.n_surv[K - 1], (int)b.cap_pairs);
@@ -720,78 +806,39 @@ __global__ void __launch_bounds__(256) k_b_enum(StereoBuffers bb, StereoParams p
   auto out = [&](long i, const int2 &v) { b.pairs[i] = v; };
   for (int base = blockIdx.x * blockDim.x; base < n_in; base += stride) {
     const int s = base + threadIdx.x;
-    bool live = s < n_in;
-    Strip S[K];
-    StripRows SR[K];
-    int t = 0, cy = 0, r1 = -1;
-    RowCursor cur{0, 0};
-    if (live) {
-      t = surv_at(b, K - 1, s).next;
+    if (s < n_in) {
+      const int t = surv_at(b, K - 1, s).next;
       int cam[kMaxCam], pid[kMaxCam];
       path_ids<K>(b, ref, s, cam, pid);
+      Strip S[K];
+      StripRows SR[K];
 #pragma unroll
       for (int j = 0; j < K; ++j) {
         S[j] = strip_of(line_of(b, cam[j], t, pid[j]));
         SR[j] = strip_rows(S[j], b.cell_px, tolg);
       }
-      int r0 = 0;
-      r1 = rows - 1;
+      int r0 = 0, r1 = rows - 1;
 #pragma unroll
       for (int j = 1; j < K; ++j) pair_rows(S[0], S[j], b.cell_px, tolg, rows, r0, r1);
-      cy = r0 - 1;
+      enum_rows<K>(b, t, S, SR, r0, r1, tol2, [&](bool acc, int pid_t) {
+        bq_push_any(q, acc, make_int2(s, pid_t), b.n_pairs, b.cap_pairs, &b.flags[1], out);
+      });
     }
-    const int32_t *cstart = b.cell_start + t * ncell;
-    const int32_t *ccnt = b.cell_cnt + t * ncell;
-    const int32_t *cpid = b.cell_pid + (long)t * b.cap;
-    const double *xy = b.xy + (long)t * b.cap * 2;
-    // per barrier: up to kPtBatch points of the current row segment (their
-    // loads in flight together), then at most one cell row advanced
-    while (__syncthreads_or(live)) {
-      int pid[kPtBatch];
-#pragma unroll
-      for (int u = 0; u < kPtBatch; ++u)
-        pid[u] = live && cur.kk + u < cur.kend ? cpid[cur.kk + u] : -1;
-      if (live) cur.kk = min(cur.kk + kPtBatch, cur.kend);
-#pragma unroll
-      for (int u = 0; u < kPtBatch; ++u) {
-        bool acc = false;
-        if (pid[u] >= 0) {
-          const double qx = xy[2 * pid[u]], qy = xy[2 * pid[u] + 1];
-          acc = true;
-#pragma unroll
-          for (int j = 0; j < K; ++j) acc = acc && in_strip(S[j], qx, qy, tol2);
-        }
-        bq_push(q, acc, make_int2(s, pid[u]));
-      }
-      if (live && cur.kk == cur.kend) {
-        if (++cy > r1) {
-          live = false;
-        } else {
-          int xa = 0, xb = cols - 1;
-          bool ok = true;
-#pragma unroll
-          for (int j = 0; j < K; ++j) {
-            int a, bb;
-            if (ok) {
-              ok = row_span(SR[j], cy, cols, a, bb);
-              xa = a > xa ? a : xa;
-              xb = bb < xb ? bb : xb;
-              ok = ok && xa <= xb;
-            }
-          }
-          if (ok) row_range(cstart, ccnt, cy, cols, xa, xb, cur);
-        }
-      }
-      bq_flush<kPtBatch>(q, b.n_pairs, b.cap_pairs, &b.flags[1], false, out);
-    }
+    bq_drain(q, b.n_pairs, b.cap_pairs, &b.flags[1], false, out);
   }
-  bq_flush(q, b.n_pairs, b.cap_pairs, &b.flags[1], true, out);
+  bq_drain(q, b.n_pairs, b.cap_pairs, &b.flags[1], true, out);
 }
 
 // Extension of every (path of length K, point) pair: a survivor goes to
 // level K, a complete match (K + 1 == build_count) to the candidate list.
+// Levels 1 and 2 are held to 168 registers (3 waves per SIMD: measured
+// faster than the unconstrained 2 waves, while 4 waves spill); deeper
+// levels keep 2 waves.
+template <int K> constexpr int extend_waves() { return K <= 2 ? 3 : 2; }
+
 template <int K>
-__global__ void __launch_bounds__(256) k_b_extend(StereoBuffers bb, StereoParams p) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(extend_waves<K>())))
+k_b_extend(StereoBuffers bb, StereoParams p) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   int nref;
   const int ref = ref_camera(b, p, nref);
