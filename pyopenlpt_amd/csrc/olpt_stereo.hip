@@ -1038,20 +1038,46 @@ __device__ __forceinline__ bool less_order(const double *pct, const double *e,
 __device__ __forceinline__ int ent_cand(int32_t v) { return v >> 3; }
 __device__ __forceinline__ int ent_slot(int32_t v) { return v & 7; }
 
+// The flat point (camera * cap + pid) of each used camera of candidate j in
+// camera order, -1 past the used ones; returns their number.  Unrolled over
+// the camera bound so that the id loads (and what depends on them) issue
+// together.
+__device__ __forceinline__ int cand_points(const StereoBuffers &b, const StereoParams &p, int j,
+                                           long f[kMaxCam]) {
+  const int nc = b.n_cam;
+  int pid[kMaxCam];
+#pragma unroll
+  for (int c = 0; c < kMaxCam; ++c)
+    pid[c] = c < nc && p.active[c] ? b.cand_ids_s[(long)j * nc + c] : -1;
+  int k = 0;
+#pragma unroll
+  for (int c = 0; c < kMaxCam; ++c) f[c] = -1;
+#pragma unroll
+  for (int c = 0; c < kMaxCam; ++c) {
+    if (pid[c] >= 0) {
+#pragma unroll
+      for (int u = 0; u <= c; ++u)
+        if (u == k) f[u] = (long)c * b.cap + pid[c];
+      ++k;
+    }
+  }
+  return k;
+}
+
 __global__ void k_flat(StereoBuffers bb, StereoParams p) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
   const int nc = b.n_cam;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
-    int k = 0;
-    for (int c = 0; c < nc; ++c) {
-      if (!p.active[c]) continue;
-      const int pid = b.cand_ids_s[(long)j * nc + c];
-      if (pid < 0) continue;
-      const long f = (long)c * b.cap + pid;
-      b.flat_rank[(long)j * nc + k] = atomicAdd(&b.pt_count[f], 1);
-      ++k;
-    }
+    long f[kMaxCam];
+    cand_points(b, p, j, f);
+    // the counting atomics of all slots in flight together
+    int r[kMaxCam];
+#pragma unroll
+    for (int k = 0; k < kMaxCam; ++k) r[k] = f[k] >= 0 ? atomicAdd(&b.pt_count[f[k]], 1) : 0;
+#pragma unroll
+    for (int k = 0; k < kMaxCam; ++k)
+      if (f[k] >= 0) b.flat_rank[(long)j * nc + k] = r[k];
   }
 }
 
@@ -1061,18 +1087,19 @@ __global__ void k_pt_fill(StereoBuffers bb, StereoParams p) {
   const int nc = b.n_cam;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
     const double e = b.cand_e_s[j];
-    int k = 0;
-    for (int c = 0; c < nc; ++c) {
-      if (!p.active[c]) continue;
-      const int pid = b.cand_ids_s[(long)j * nc + c];
-      if (pid < 0) continue;
-      const long f = (long)c * b.cap + pid;
-      const int q = b.pt_start[f] + b.flat_rank[(long)j * nc + k];
-      b.pt_list[q] = (j << 3) | k;
-      b.pt_of[q] = (int32_t)f;
-      b.e_ent[q] = e;
-      b.ent_q[(long)j * nc + k] = q;
-      ++k;
+    long f[kMaxCam];
+    cand_points(b, p, j, f);
+    int q[kMaxCam];
+#pragma unroll
+    for (int k = 0; k < kMaxCam; ++k)
+      q[k] = f[k] >= 0 ? b.pt_start[f[k]] + b.flat_rank[(long)j * nc + k] : -1;
+#pragma unroll
+    for (int k = 0; k < kMaxCam; ++k) {
+      if (q[k] < 0) continue;
+      b.pt_list[q[k]] = (j << 3) | k;
+      b.pt_of[q[k]] = (int32_t)f[k];
+      b.e_ent[q[k]] = e;
+      b.ent_q[(long)j * nc + k] = q[k];
     }
   }
 }
@@ -1262,43 +1289,47 @@ __global__ void k_prio_fallback(StereoBuffers bb) {
 // device-coherent loads.
 __device__ __forceinline__ int mis_decide(const StereoBuffers &b, const StereoParams &p, int j) {
   const int nc = b.n_cam;
+  long f[kMaxCam];
+  cand_points(b, p, j, f);
+  // every point's position, list start and head loaded together
+  int pos[kMaxCam], s0[kMaxCam], h0[kMaxCam];
+#pragma unroll
+  for (int k = 0; k < kMaxCam; ++k) {
+    pos[k] = 0;
+    s0[k] = 0;
+    h0[k] = 0;
+    if (f[k] >= 0) {
+      pos[k] = b.ppos[(long)j * nc + k];
+      s0[k] = b.pt_start[f[k]];
+      h0[k] = __atomic_load_n(&b.head[f[k]], __ATOMIC_RELAXED);
+    }
+  }
   bool out = false, blocked = false;
-  int at[kMaxCam];   // st_ent index of each of the candidate's entries
-  int na = 0;
-  int k = 0;
-  for (int c = 0; c < nc; ++c) {
-    if (!p.active[c]) continue;
-    const int pid = b.cand_ids_s[(long)j * nc + c];
-    if (pid < 0) continue;
-    const long f = (long)c * b.cap + pid;
-    const int pos = b.ppos[(long)j * nc + k];
-    ++k;
-    const int s0 = b.pt_start[f];
-    at[na++] = s0 + pos;
-    if (out) continue;
-    const int h0 = __atomic_load_n(&b.head[f], __ATOMIC_RELAXED);
+#pragma unroll
+  for (int k = 0; k < kMaxCam; ++k) {
+    if (f[k] < 0 || out) continue;
     // first entry ahead of pos that is not rejected, kRankBatch entries per
     // step (their state loads in flight together).  States only leave 0 for
     // a final 1 or 2, so a batch's snapshot at worst defers a decision to a
     // later round.
-    int h = h0, st = 2;
-    while (h < pos) {
+    int h = h0[k], st = 2;
+    while (h < pos[k]) {
       int sv[kRankBatch];
 #pragma unroll
       for (int u = 0; u < kRankBatch; ++u)
-        sv[u] = h + u < pos ? __atomic_load_n(&b.st_ent[s0 + h + u], __ATOMIC_RELAXED) : 2;
+        sv[u] = h + u < pos[k] ? __atomic_load_n(&b.st_ent[s0[k] + h + u], __ATOMIC_RELAXED) : 2;
       int adv = kRankBatch;
 #pragma unroll
       for (int u = kRankBatch - 1; u >= 0; --u)
         if (sv[u] != 2) adv = u;
-      h = min(h + adv, pos);
+      h = min(h + adv, pos[k]);
       if (adv < kRankBatch) {
         st = sv[adv];
         break;
       }
     }
-    if (h > h0) atomicMax(&b.head[f], h);
-    if (h < pos) {
+    if (h > h0[k]) atomicMax(&b.head[f[k]], h);
+    if (h < pos[k]) {
       if (st == 1)
         out = true;
       else
@@ -1308,7 +1339,9 @@ __device__ __forceinline__ int mis_decide(const StereoBuffers &b, const StereoPa
   const int d = out ? 2 : (blocked ? 0 : 1);
   if (d != 0) {
     b.state[j] = (int8_t)d;
-    for (int u = 0; u < na; ++u) __atomic_store_n(&b.st_ent[at[u]], d, __ATOMIC_RELAXED);
+#pragma unroll
+    for (int k = 0; k < kMaxCam; ++k)
+      if (f[k] >= 0) __atomic_store_n(&b.st_ent[s0[k] + pos[k]], d, __ATOMIC_RELAXED);
   }
   return d;
 }
