@@ -422,31 +422,64 @@ __device__ __forceinline__ T eval_residue(const ShakeBuffers<T> &b, const CamLan
     const int roih = c.roi.rmax - c.roi.rmin;
     // FP32: exp2 with log2(e) folded into b and c once per evaluation
     const T b2 = P[1] * T(1.4426950408889634), c2 = P[2] * T(1.4426950408889634);
+    // FP32 without rotation: the Gaussian is a product of a column factor
+    // and a row factor, each zero outside the object's window and the ROI
+    // (and where the intensity is not positive), so a slot costs one
+    // product: PW / LPG column and PW row exponentials per lane instead of
+    // one per slot.  The product rounds differently from the exponential of
+    // the sum, by an ulp or two.
+    bool sep = false;
+    if constexpr (sizeof(T) == 4) sep = P[3] == T(1) && P[4] == T(0);
+    if (sep) {
+      constexpr int NCOL = PW / LPG;
+      T ex[NCOL], ey[PW];
 #pragma unroll
-    for (int sl = 0; sl < SLOTS; ++sl) {
-      const int ro = sl * LPG / PW, co = sl * LPG % PW + sub;
-      const int row = c.roi.rmin + ro;
-      const int col = c.roi.cmin + co;
-      const bool valid = co < c.roiw && ro < roih;
-      const bool win = row >= prmin && row < prmax && col >= pcmin && col < pcmax;
-      T pred = T(0);
-      if constexpr (sizeof(T) == 4) {
-        // FP32: the intensity only where some lane of the wave needs it (a
-        // wave-uniform branch: the ROI's edge rows mostly lie outside every
-        // object's window), selected per lane without branches
-        if (ballot(win) != 0) {
-          const T dx = T(col) - ue, dy = T(row) - ve;
-          const T xx = mad_t(dx, P[3], dy * P[4]);
-          const T yy = mad_t(dy, P[3], -dx * P[4]);
-          const T g = P[0] * __builtin_amdgcn_exp2f(-mad_t(b2 * xx, xx, c2 * yy * yy));
-          pred = win && g > T(0) ? g : T(0);
-        }
-      } else {
-        if (win) pred = gauss_int<T>(P, ue, ve, row, col);
+      for (int q = 0; q < NCOL; ++q) {
+        const int co = q * LPG + sub, col = c.roi.cmin + co;
+        const T dx = T(col) - ue;
+        const T e = __builtin_amdgcn_exp2f(-(b2 * dx * dx));
+        ex[q] = co < c.roiw && col >= pcmin && col < pcmax && e > T(0) ? e : T(0);
       }
-      // empty slots add an exact zero
-      const T d = valid ? c.aug[sl] - pred : T(0);
-      sse = mad_t(d, d, sse);
+#pragma unroll
+      for (int ro = 0; ro < PW; ++ro) {
+        const int row = c.roi.rmin + ro;
+        const T dy = T(row) - ve;
+        const T g = P[0] * __builtin_amdgcn_exp2f(-(c2 * dy * dy));
+        ey[ro] = ro < roih && row >= prmin && row < prmax && g > T(0) ? g : T(0);
+      }
+      // empty slots hold aug = 0 and a zero factor: they add an exact zero
+#pragma unroll
+      for (int sl = 0; sl < SLOTS; ++sl) {
+        const T d = c.aug[sl] - ey[sl * LPG / PW] * ex[(sl * LPG % PW) / LPG];
+        sse = mad_t(d, d, sse);
+      }
+    } else {
+#pragma unroll
+      for (int sl = 0; sl < SLOTS; ++sl) {
+        const int ro = sl * LPG / PW, co = sl * LPG % PW + sub;
+        const int row = c.roi.rmin + ro;
+        const int col = c.roi.cmin + co;
+        const bool valid = co < c.roiw && ro < roih;
+        const bool win = row >= prmin && row < prmax && col >= pcmin && col < pcmax;
+        T pred = T(0);
+        if constexpr (sizeof(T) == 4) {
+          // FP32: the intensity only where some lane of the wave needs it (a
+          // wave-uniform branch: the ROI's edge rows mostly lie outside every
+          // object's window), selected per lane without branches
+          if (ballot(win) != 0) {
+            const T dx = T(col) - ue, dy = T(row) - ve;
+            const T xx = mad_t(dx, P[3], dy * P[4]);
+            const T yy = mad_t(dy, P[3], -dx * P[4]);
+            const T g = P[0] * __builtin_amdgcn_exp2f(-mad_t(b2 * xx, xx, c2 * yy * yy));
+            pred = win && g > T(0) ? g : T(0);
+          }
+        } else {
+          if (win) pred = gauss_int<T>(P, ue, ve, row, col);
+        }
+        // empty slots add an exact zero
+        const T d = valid ? c.aug[sl] - pred : T(0);
+        sse = mad_t(d, d, sse);
+      }
     }
   }
   sse = group_sum<LPG>(sse);
