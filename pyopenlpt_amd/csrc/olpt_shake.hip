@@ -398,7 +398,46 @@ template <typename T, int SLOTS> struct CamLane {
   PRange roi;      // augmented ROI (zero range for inactive cameras)
   int roiw, npix;
   T aug[SLOTS];    // this lane's augmented pixels
+  T R[9], Tv[3], fx, fy, cx, cy;   // the lane camera's pinhole parameters
 };
+
+// The lane camera's pinhole parameters, held in registers for the whole
+// kernel (each lane evaluates only its own camera).
+template <typename T, int SLOTS>
+__device__ __forceinline__ void cam_lane_load(const ShakeBuffers<T> &b, int k,
+                                              CamLane<T, SLOTS> &c) {
+  const CamT<T> &cm = b.cams[k];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) c.R[i] = cm.R[i];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) c.Tv[i] = cm.Tv[i];
+  c.fx = cm.fx;
+  c.fy = cm.fy;
+  c.cx = cm.cx;
+  c.cy = cm.cy;
+}
+
+// Camera::project with the lane's registers (the distorted model reads the
+// camera record), the same operations as project().
+template <typename T, int SLOTS>
+__device__ __forceinline__ void project_lane(const ShakeBuffers<T> &b, const CamLane<T, SLOTS> &c,
+                                             const T X[3], T &u, T &v) {
+  const CamT<T> &cam = b.cams[c.k];
+  if (cam.is_distorted) {
+    project(cam, X, u, v);
+    return;
+  }
+  T t[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    t[i] = mad_t(c.R[3 * i + 2], X[2], mad_t(c.R[3 * i + 1], X[1], c.R[3 * i] * X[0]));
+#pragma unroll
+  for (int i = 0; i < 3; ++i) t[i] = t[i] + c.Tv[i];
+  const T x = t[2] != T(0) ? t[0] / t[2] : t[0];
+  const T y = t[2] != T(0) ? t[1] / t[2] : t[1];
+  u = mad_t(x, c.fx, c.cx);
+  v = mad_t(y, c.fy, c.cy);
+}
 
 // calShakeResidue (Shake.cpp:968-1045) at Xe; (ue, ve) receive the lane
 // camera's projection.  Wave-uniform result.
@@ -412,7 +451,7 @@ __device__ __forceinline__ T eval_residue(const ShakeBuffers<T> &b, const CamLan
   T sse = T(0);
   ue = T(0);
   ve = T(0);
-  if (c.cam_ok) project(b.cams[c.k], Xe, ue, ve);
+  if (c.cam_ok) project_lane(b, c, Xe, ue, ve);
   const bool use = c.act && c.roi.rmax - c.roi.rmin > 0 && c.roi.cmax - c.roi.cmin > 0;
   if (use) {
     T P[5];
@@ -506,7 +545,7 @@ __device__ __forceinline__ bool any_out(const ShakeBuffers<T> &b, const CamLane<
   bool out = false;
   if (c.cam_ok) {
     T u, v;
-    project(b.cams[c.k], Xt, u, v);
+    project_lane(b, c, Xt, u, v);
     out = !in_roi_d(c.roi, u, v);
   }
   return (ballot(out) & c.omask) != 0;
@@ -577,6 +616,7 @@ k_shake(ShakeBuffers<T> bb, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
   c.cam_ok = c.k < nc;
   c.act = c.cam_ok && p.active[c.k];
   const int ksafe = c.cam_ok ? c.k : 0;
+  cam_lane_load(b, ksafe, c);
   for (int i = blockIdx.x * OPW + lane / LPO; i < n; i += gridDim.x * OPW) {
     T X[3] = {(T)b.xyz[3L * i], (T)b.xyz[3L * i + 1], (T)b.xyz[3L * i + 2]};
     build_roi<T, LPG, SLOTS>(b, p, rec, i, c);
@@ -684,12 +724,46 @@ k_shake(ShakeBuffers<T> bb, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
           }
         }
       }
-      double dcx[5], dcf[5], coef[3];
-      for (int t = 0; t < ncand; ++t) {
-        dcx[t] = (double)cx[t];
-        dcf[t] = (double)cf[t];
+      double coef[3];
+      if constexpr (sizeof(T) == 4) {
+        // FP32 mode: the vertex of the least-squares parabola from the
+        // normal equations in offsets scaled by 1/delta (well conditioned),
+        // by cofactors: only c1 / c2 and the sign of c2 are needed, and the
+        // determinant of the Gram matrix of >= 3 distinct offsets is > 0
+        const double sc = 1.0 / (double)delta;
+        double s1 = 0, s2 = 0, s3 = 0, s4 = 0, t0 = 0, t1 = 0, t2 = 0;
+#pragma unroll
+        for (int t = 0; t < 5; ++t) {
+          if (t < ncand) {
+            const double x = (double)cx[t] * sc, y = (double)cf[t];
+            const double x2 = x * x;
+            s1 += x;
+            s2 += x2;
+            s3 += x2 * x;
+            s4 += x2 * x2;
+            t0 += y;
+            t1 += x * y;
+            t2 += x2 * y;
+          }
+        }
+        const double s0 = (double)ncand;
+        const double c01 = s2 * s3 - s1 * s4, c02 = s1 * s3 - s2 * s2;
+        const double c11 = s0 * s4 - s2 * s2, c12 = s1 * s2 - s0 * s3;
+        const double c22 = s0 * s2 - s1 * s1;
+        const double n1 = c01 * t0 + c11 * t1 + c12 * t2;
+        const double n2 = c02 * t0 + c12 * t1 + c22 * t2;
+        // c1 and c2 of the unscaled fit times the (positive) determinant
+        coef[0] = 0.0;
+        coef[1] = n1 * sc;
+        coef[2] = n2 * sc * sc;
+      } else {
+        double dcx[5], dcf[5];
+        for (int t = 0; t < ncand; ++t) {
+          dcx[t] = (double)cx[t];
+          dcf[t] = (double)cf[t];
+        }
+        polyfit2(dcx, dcf, ncand, coef);
       }
-      polyfit2(dcx, dcf, ncand, coef);
       if (allow_xhat && coef[2] > 0.0) {
         const T xh = (T)(-coef[1] / (2.0 * coef[2]));
         T xmin = cx[0], xmax = cx[0];
@@ -721,7 +795,7 @@ k_shake(ShakeBuffers<T> bb, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
     // final projection and calObjectScore (:668-782)
     T uf = T(0), vf = T(0), Pf[5] = {T(0), T(0), T(0), T(0), T(0)};
     if (c.cam_ok) {
-      project(b.cams[c.k], X, uf, vf);
+      project_lane(b, c, X, uf, vf);
       otf_param(b.otf, c.k, X, Pf);
     }
     bool usek = false;
@@ -887,6 +961,7 @@ k_propose(ShakeBuffers<T> bb, ShakeParams p, const ObjCamRec<T> *__restrict__ re
   c.k = (lane % LPO) / (LPO / NC);
   c.cam_ok = c.k < b.n_cam;
   c.act = c.cam_ok && p.active[c.k];
+  cam_lane_load(b, c.cam_ok ? c.k : 0, c);
   const T lim = (T)p.max_step_vox, vox = (T)p.voxel_mm;
   for (int i = blockIdx.x * OPW + lane / LPO; i < n; i += gridDim.x * OPW) {
     T X[3] = {(T)b.xyz[3L * i], (T)b.xyz[3L * i + 1], (T)b.xyz[3L * i + 2]};
