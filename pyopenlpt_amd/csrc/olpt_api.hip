@@ -83,11 +83,11 @@ struct Workspace {
   double *sh_xyz = nullptr, *keep_xyz = nullptr, *sh_score = nullptr;
   uint8_t *sh_flag = nullptr;
   int32_t *sh_n = nullptr, *n_keep = nullptr;
-  int32_t *bin_cnt = nullptr, *bin_start = nullptr, *bin_obj = nullptr, *obj_bin = nullptr,
+  int32_t *bin_cnt = nullptr, *bin_start = nullptr, *obj_bin = nullptr,
           *obj_rank = nullptr;
   int32_t *g_cnt = nullptr, *g_start = nullptr, *g_obj = nullptr, *g_rank = nullptr,
           *g_cellof = nullptr, *has_nb = nullptr, *sh_keep_off = nullptr;
-  void *rec = nullptr;
+  void *rec = nullptr, *bin_rec = nullptr;
   double *partial = nullptr;
   void *patches = nullptr;   // learned proposal: bf16 [cap_obj][512]
   float *prop = nullptr;     // [cap_obj][4]
@@ -378,7 +378,7 @@ ShakeBuffers<T> shake_buffers(olpt_ctx *c, Workspace &w, T *img, T *img_next, in
   b.bin_cols = c->bin_cols;
   b.bin_cnt = w.bin_cnt;
   b.bin_start = w.bin_start;
-  b.bin_obj = w.bin_obj;
+  b.bin_rec = w.bin_rec;
   b.obj_bin = w.obj_bin;
   b.obj_rank = w.obj_rank;
   b.grid_n = c->grid_n;
@@ -849,7 +849,7 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   const long nbin = (long)c->bin_rows * c->bin_cols;
   A(w.bin_cnt, int32_t, nbin * nc);
   A(w.bin_start, int32_t, nbin * nc + 1);  // + the total: counts are start differences
-  A(w.bin_obj, int32_t, (long)O * nc);
+  L.add(&w.bin_rec, shake_rec_bytes(true) * (size_t)O * nc);
   A(w.obj_bin, int32_t, (long)O * nc);
   A(w.obj_rank, int32_t, (long)O * nc);
   A(w.g_cnt, int32_t, grid_cells);

@@ -283,7 +283,8 @@ template <typename T> struct ShakeBuffers {
   const int32_t *n_obj;
   // binning of 4x4 projection boxes (residual gather)
   int bin_px, bin_rows, bin_cols;
-  int32_t *bin_cnt, *bin_start, *bin_obj, *obj_bin, *obj_rank; // per cam
+  int32_t *bin_cnt, *bin_start, *obj_bin, *obj_rank; // per cam
+  void *bin_rec;      // [n_cam * cap_obj] the projection records in bin order
   // repeated-object grid
   int grid_n;         // cells per axis
   double grid_lo[3], grid_cell;
@@ -319,7 +320,7 @@ template <typename T> struct ShakeBuffers {
     o.n_obj = fshift(n_obj, d);
     o.bin_cnt = fshift(bin_cnt, d);
     o.bin_start = fshift(bin_start, d);
-    o.bin_obj = fshift(bin_obj, d);
+    o.bin_rec = fshift(bin_rec, d);
     o.obj_bin = fshift(obj_bin, d);
     o.obj_rank = fshift(obj_rank, d);
     o.g_cnt = fshift(g_cnt, d);

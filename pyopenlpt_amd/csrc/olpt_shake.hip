@@ -94,12 +94,17 @@ __global__ void k_obj_project(ShakeBuffers<T> bb, const double *__restrict__ xyz
   }
 }
 
+// Copies every binned record to its place in bin order, so that a tile
+// reads its objects' records contiguously.
 template <typename T>
-__global__ void k_bin_fill(ShakeBuffers<T> bb, const int32_t *__restrict__ n_ptr) {
+__global__ void k_bin_fill(ShakeBuffers<T> bb, const int32_t *__restrict__ n_ptr,
+                           const ObjCamRec<T> *__restrict__ rec) {
   const long F = (long)blockIdx.z;
   const ShakeBuffers<T> b = bb.frame(F);
   const long fo = F * bb.fstride;
   n_ptr = fshift(n_ptr, fo);
+  rec = fshift(rec, fo);
+  ObjCamRec<T> *brec = reinterpret_cast<ObjCamRec<T> *>(b.bin_rec);
   const int n = *n_ptr;
   const int nc = b.n_cam;
   const long nbin = (long)b.bin_rows * b.bin_cols;
@@ -108,7 +113,7 @@ __global__ void k_bin_fill(ShakeBuffers<T> bb, const int32_t *__restrict__ n_ptr
     const int i = (int)(t / nc), k = (int)(t % nc);
     const int bin = b.obj_bin[(long)k * b.cap_obj + i];
     if (bin < 0) continue;
-    b.bin_obj[b.bin_start[k * nbin + bin] + b.obj_rank[(long)k * b.cap_obj + i]] = i;
+    brec[b.bin_start[k * nbin + bin] + b.obj_rank[(long)k * b.cap_obj + i]] = rec[t];
   }
 }
 
@@ -163,9 +168,18 @@ k_resid_gather(ShakeBuffers<T> bb, const T *__restrict__ orig, T *__restrict__ o
   const int tid = threadIdx.x;
   const long HW = (long)b.H * b.W;
   const long nbin = (long)b.bin_rows * b.bin_cols;
-  const int32_t *bstart = b.bin_start + k * nbin;  // offsets into bin_obj, all cameras
-  const int32_t *bobj = b.bin_obj;
+  const int32_t *bstart = b.bin_start + k * nbin;  // offsets into bin_rec, all cameras
+  const ObjCamRec<T> *brec = reinterpret_cast<const ObjCamRec<T> *>(b.bin_rec);
   for (int q = tid; q < NPX; q += kResThreads) s_img[q] = 0;
+  // the tile's original pixels, loaded now and used at the end
+  constexpr int PPT = NPX / kResThreads;
+  T o_px[PPT];
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int q = tid + u * kResThreads;
+    const int row = r0 + q / kResTile, col = c0 + q % kResTile;
+    o_px[u] = row < b.H && col < b.W ? orig[k * HW + (long)row * b.W + col] : T(0);
+  }
   // anchors (box top-left) that can cover this tile, and their bins
   const int ar0 = max(0, r0 - box_w + 1), ar1 = min(b.H - 1, r0 + kResTile - 1);
   const int ac0 = max(0, c0 - box_w + 1), ac1 = min(b.W - 1, c0 + kResTile - 1);
@@ -203,8 +217,7 @@ k_resid_gather(ShakeBuffers<T> bb, const T *__restrict__ orig, T *__restrict__ o
         else
           hi = mid - 1;
       }
-      const int i = bobj[s_bstart[lo] + (t - s_bpre[lo])];
-      s_obj[tid] = rec[(long)i * b.n_cam + k];
+      s_obj[tid] = brec[s_bstart[lo] + (t - s_bpre[lo])];
     }
     __syncthreads();
     for (int it = tid; it < n * box_w; it += kResThreads) {
@@ -214,20 +227,36 @@ k_resid_gather(ShakeBuffers<T> bb, const T *__restrict__ orig, T *__restrict__ o
       if (row < r.rmax && row >= r0 && row < r0 + kResTile) {
         const int cb = max(r.cmin, c0), ce = min(r.cmax, c0 + kResTile);
         U *dst = s_img + (row - r0) * kResTile - c0;
-        for (int col = cb; col < ce; ++col) {
-          const T pv = gauss_int<T>(r.P, r.u, r.v, row, col);
-          if (pv > T(0)) atomicMax(dst + col, MaxBits<T>::enc(pv));
+        bool sep = false;
+        if constexpr (sizeof(T) == 4) sep = r.P[3] == T(1) && r.P[4] == T(0);
+        if (sep) {
+          // FP32 without rotation: the row factor once, then one
+          // exponential per column (rounds like k_shake's product form)
+          const T dy = T(row) - r.v;
+          const T gy = r.P[0] * gexp(-(r.P[2] * dy * dy));
+          for (int col = cb; col < ce; ++col) {
+            const T dx = T(col) - r.u;
+            const T pv = gy * gexp(-(r.P[1] * dx * dx));
+            if (pv > T(0)) atomicMax(dst + col, MaxBits<T>::enc(pv));
+          }
+        } else {
+          for (int col = cb; col < ce; ++col) {
+            const T pv = gauss_int<T>(r.P, r.u, r.v, row, col);
+            if (pv > T(0)) atomicMax(dst + col, MaxBits<T>::enc(pv));
+          }
         }
       }
     }
     __syncthreads();
   }
-  for (int q = tid; q < NPX; q += kResThreads) {
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int q = tid + u * kResThreads;
     const int row = r0 + q / kResTile, col = c0 + q % kResTile;
     if (row < b.H && col < b.W) {
       const long pix = (long)row * b.W + col;
       const T maxp = MaxBits<T>::dec(s_img[q]);
-      T v = orig[k * HW + pix] - maxp;
+      T v = o_px[u] - maxp;
       if (mode == 1 && v < T(0)) v = T(0);
       if (mode == 2) v = maxp;  // rendering: max-fusion of the projections
       out[k * HW + pix] = v;
@@ -258,7 +287,8 @@ static void residual_pass(const ShakeBuffers<T> &b, const T *orig, T *out,
   // one scan over all cameras' bins; bin_start[nc * nbin] receives the total
   scan_exclusive_clear(b.bin_cnt, b.bin_start, nbin * nc, b.bin_start + nbin * nc,
                        b.scan_scratch, st, b.nf, b.fstride);
-  k_bin_fill<T><<<dim3(grid_for((long)b.cap_obj * nc, 256, 4096), 1, b.nf), 256, 0, st>>>(b, n_ptr);
+  k_bin_fill<T><<<dim3(grid_for((long)b.cap_obj * nc, 256, 4096), 1, b.nf), 256, 0, st>>>(b, n_ptr,
+                                                                                        rec);
   dim3 g((b.W + kResTile - 1) / kResTile, (b.H + kResTile - 1) / kResTile, cams.n * b.nf);
   stage_begin(ST_RESID, st);
   k_resid_gather<T><<<g, kResThreads, 0, st>>>(b, orig, out, rec, cams, mode, box_w);
