@@ -1524,72 +1524,85 @@ __device__ __forceinline__ double hz_val(unsigned long long bits) {
   return __longlong_as_double((long long)bits);
 }
 
+// One thread per value (i = 2 j + type): the two values of a candidate
+// insert independently.
 __global__ void k_hz_insert(StereoBuffers bb) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
-    const double pv[2] = {b.pct[j], b.cand_e_s[j]};
-    for (int t = 0; t < 2; ++t) {
-      b.hz_slot[2 * j + t] = -1;
-      if (!isfinite(pv[0]) || !isfinite(pv[t])) continue;
-      const double v = pv[t] + 0.0;  // -0 -> +0
-      if (v < 0.0) {
-        atomicOr(&b.flags[2], 1);
-        continue;
-      }
-      const long s = hz_insert(b, hz_key(t, v));
-      if (s < 0) {
-        atomicOr(&b.flags[2], 1);   // table full
-        continue;
-      }
-      const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
-      // the thread that stores a value owns it: it alone checks and clears
-      // the bucket for it (repeated values - most percentile scores - do not)
-      bool placed = false, owner = false;
-      for (int u = 1; u <= kHzVals && !placed; ++u) {
-        unsigned long long o = __atomic_load_n(hz_word(b, s, u), __ATOMIC_RELAXED);
-        if (o == kHzNoVal) o = atomicCAS(hz_word(b, s, u), kHzNoVal, bits);
-        owner = o == kHzNoVal;
-        placed = owner || o == bits;
-      }
-      if (!placed) atomicOr(&b.flags[2], 1);   // bucket full
-      if (owner) b.hz_slot[2 * j + t] = (int32_t)s;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * M; i += gridDim.x * blockDim.x) {
+    const int j = i >> 1, t = i & 1;
+    b.hz_slot[i] = -1;
+    const double pj = b.pct[j];
+    const double pv = t == 0 ? pj : b.cand_e_s[j];
+    if (!isfinite(pj) || !isfinite(pv)) continue;
+    const double v = pv + 0.0;  // -0 -> +0
+    if (v < 0.0) {
+      atomicOr(&b.flags[2], 1);
+      continue;
     }
+    const long s = hz_insert(b, hz_key(t, v));
+    if (s < 0) {
+      atomicOr(&b.flags[2], 1);   // table full
+      continue;
+    }
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+    // the thread that stores a value owns it: it alone checks and clears
+    // the bucket for it (repeated values - most percentile scores - do not)
+    bool placed = false, owner = false;
+    for (int u = 1; u <= kHzVals && !placed; ++u) {
+      unsigned long long o = __atomic_load_n(hz_word(b, s, u), __ATOMIC_RELAXED);
+      if (o == kHzNoVal) o = atomicCAS(hz_word(b, s, u), kHzNoVal, bits);
+      owner = o == kHzNoVal;
+      placed = owner || o == bits;
+    }
+    if (!placed) atomicOr(&b.flags[2], 1);   // bucket full
+    if (owner) b.hz_slot[i] = (int32_t)s;
+  }
+}
+
+// the kHzVals value words of slot s into w (absent values as +inf)
+__device__ __forceinline__ void hz_take(const StereoBuffers &b, long s, double *w) {
+#pragma unroll
+  for (int u = 0; u < kHzVals; ++u) {
+    const unsigned long long bits = s >= 0 ? *hz_word(b, s, 1 + u) : kHzNoVal;
+    w[u] = bits == kHzNoVal ? INFINITY : hz_val(bits);
   }
 }
 
 __global__ void k_hz_check(StereoBuffers bb) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
+  constexpr int NW = 3 * kHzVals;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * M; i += gridDim.x * blockDim.x) {
     const long s = b.hz_slot[i];
     if (s < 0) continue;
     const long long key = (long long)*hz_word(b, s, 0);
-    // the distinct values of buckets key, key + 1, key + 2, sorted
-    double w[3 * kHzVals];
-    int n = 0;
-    for (int d = 0; d < 3; ++d) {
-      const long sd = d == 0 ? s : hz_find(b, key + d);
-      if (sd < 0) continue;
-      for (int u = 1; u <= kHzVals; ++u) {
-        const unsigned long long bits = *hz_word(b, sd, u);
-        if (bits == kHzNoVal) continue;
-        const double v = hz_val(bits);
-        int k = n++;
-        while (k > 0 && w[k - 1] > v) {
-          w[k] = w[k - 1];
-          --k;
-        }
-        w[k] = v;
+    // the distinct values of buckets key, key + 1, key + 2 (absent ones as
+    // +inf), sorted by a transposition network in registers
+    double w[NW];
+    const long s1 = hz_find(b, key + 1), s2 = hz_find(b, key + 2);
+    hz_take(b, s, w);
+    hz_take(b, s1, w + kHzVals);
+    hz_take(b, s2, w + 2 * kHzVals);
+#pragma unroll
+    for (int r = 0; r < NW; ++r)
+#pragma unroll
+      for (int k = r & 1; k + 1 < NW; k += 2) {
+        const double lo = fmin(w[k], w[k + 1]), hi = fmax(w[k], w[k + 1]);
+        w[k] = lo;
+        w[k + 1] = hi;
+      }
+    // every run of eq_eps-adjacent values must have eq_eps end points
+    bool bad = false;
+    double r0 = w[0];
+#pragma unroll
+    for (int k = 1; k <= NW; ++k) {
+      const bool end = k == NW || !isfinite(w[k]) || !eq_eps(w[k - 1], w[k]);
+      if (isfinite(w[k - 1]) && end) {
+        bad = bad || !eq_eps(r0, w[k - 1]);
+        if (k < NW) r0 = w[k];
       }
     }
-    bool bad = false;
-    int r0 = 0;
-    for (int k = 1; k <= n; ++k)
-      if (k == n || !eq_eps(w[k - 1], w[k])) {
-        bad = bad || !eq_eps(w[r0], w[k - 1]);
-        r0 = k;
-      }
     if (bad) atomicOr(&b.flags[2], 1);
   }
 }
