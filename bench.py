@@ -535,6 +535,7 @@ def main() -> int:
         ctx.set_shake_model(net, w0.voxel_mm, 0.6)
         learned = timed_leg(ctx, lp, "learned", args.steps, graph)
         learned["train"] = dict(samples=tstats["samples"], frames=tstats["frames"],
+                                skipped_frames=tstats["skipped_frames"],
                                 particles=tstats["particles"], steps=args.train_steps,
                                 seconds=round(t_train, 2), seeds="9001.. (eval frames 1234..)")
 

@@ -171,11 +171,22 @@ def training_frame(ctx, params, rng: np.random.Generator, vox: float,
 
 def make_training_set(ctx, params, vox: float, n_particles: int = 2_000_000, seed: int = 9001,
                       **kw):
-    """Frames until n_particles true particles have been drawn."""
+    """Frames until n_particles true particles have been drawn.  A frame
+    whose pruneMatch the library refuses (percentile or e_check values
+    within the eq_eps tolerance that are not transitive, status 103 - about
+    one dense frame in a hundred) is skipped: the training data needs
+    matched objects, not that frame."""
+    from ._lib import FatalError
     rng = np.random.default_rng(seed)
-    xs, ts, n_seen, n_real = [], [], 0, 0
+    xs, ts, n_seen, n_real, skipped = [], [], 0, 0, 0
     while n_seen < n_particles:
-        r = training_frame(ctx, params, rng, vox, **kw)
+        try:
+            r = training_frame(ctx, params, rng, vox, **kw)
+        except FatalError as e:
+            if e.status != 103:
+                raise
+            skipped += 1
+            continue
         if r is None:
             continue
         xs.append(r[0])
@@ -183,7 +194,8 @@ def make_training_set(ctx, params, vox: float, n_particles: int = 2_000_000, see
         n_real += r[2]
         n_seen += r[3]
     return torch.cat(xs), torch.cat(ts), dict(samples=sum(len(x) for x in xs), real=n_real,
-                                              particles=n_seen, frames=len(xs))
+                                              particles=n_seen, frames=len(xs),
+                                              skipped_frames=skipped)
 
 
 def train(net: ShakeNet, X: torch.Tensor, T: torch.Tensor, steps: int, batch: int = 8192,
