@@ -136,10 +136,11 @@ def cpu_baseline(ppp: float, size: int, threads: int, reps: int = 3) -> dict | N
                      "omp_threads": threads}}
 
 
-def traffic_per_launch(workload: str):
-    """HBM bytes per k_shake launch from profiles/k_shake__<workload>.pmc.md
+def traffic_per_launch(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from profiles/<kernel>__<workload>.pmc.md
     (written by `python -m pyopenlpt_amd.prof pmc`), or None."""
-    path = os.path.join(ROOT, "profiles", f"k_shake__{workload}.pmc.md")
+    name = kernel.replace("<", "").replace(">", "")
+    path = os.path.join(ROOT, "profiles", f"{name}__{workload}.pmc.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -436,11 +437,14 @@ def main() -> int:
                     particles_per_step=per_step, shake_loops=int(params.n_shake_loop),
                     quality=qual)
 
-    def shake_roofline(c, params, label):
-        """k_shake's own launch time: one eager single-stream step with HIP
-        events around every k_shake launch (concurrent streams would add
-        waiting time to the intervals), and its algorithmic bytes per launch
-        from an instrumented step."""
+    def rooflines(c, params, label):
+        """Live rooflines of k_shake and of k_b_enum at level 2 (the build's
+        costliest kernel) on the leg `value` reports: one eager
+        single-stream step with HIP events around every launch of each
+        (concurrent streams would add waiting time to the intervals), and
+        their algorithmic bytes per launch from an instrumented step.
+        Returns (dominant, other), the dominant being the one with more
+        time per step."""
         c.stage_stats(True)
         c.ipr_run(dimg, code, params, n_frames=args.frames, cap_obj=cap_obj, outputs=outs)
         stats = c.stage_stats(False)
@@ -454,24 +458,36 @@ def main() -> int:
         times = c.stage_times()
         c.stage_timing(False)
         c.set_tuning(n_streams=tuning["n_streams"])
-        if not (times.get("shake", (0, 0))[1] and stats.get("shake", (0, 0))[1]):
-            return None
-        ms_dom, calls = times["shake"]
-        bytes_dom, launches = stats["shake"]
-        per_launch = bytes_dom / launches
-        avg_ms = ms_dom / calls
-        achieved = per_launch / (avg_ms / 1000.0) / 1e9
-        return {"bound": "hbm", "kernel": "k_shake", "leg": label,
-                "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6),
-                "traffic": traffic_per_launch(workload),
-                "bytes_per_launch": int(per_launch), "avg_launch_ms": round(avg_ms, 4),
-                "frames_per_launch": int(tuning["frames_per_batch"]),
-                "measured": f"{args.roofline_steps} eager single-stream step(s) of the {label} leg, "
-                            "HIP events around each k_shake launch",
-                "stage_ms_per_step_1stream": {k: round(v[0] / args.roofline_steps, 3)
-                                              for k, v in times.items()
-                                              if v[1] and k != "host_issue"}}
+        stage_ms = {k: round(v[0] / args.roofline_steps, 3) for k, v in times.items()
+                    if v[1] and k != "host_issue"}
+        out = []
+        for stage, kernel, what in (
+                ("shake", "k_shake",
+                 "N x C x P^2 x s patch gathers (P = 8, s = 4 bytes) + 48 bytes per object, "
+                 "summed over the launch's frames"),
+                ("enum2", "k_b_enum<2>",
+                 "per level-1 survivor its 12-byte record and two 32-byte line-table entries, "
+                 "summed over the launch's frames")):
+            if not (times.get(stage, (0, 0))[1] and stats.get(stage, (0, 0))[1]):
+                continue
+            ms_dom, calls = times[stage]
+            bytes_dom, launches = stats[stage]
+            per_launch = bytes_dom / launches
+            avg_ms = ms_dom / calls
+            achieved = per_launch / (avg_ms / 1000.0) / 1e9
+            out.append({"bound": "hbm", "kernel": kernel, "leg": label,
+                        "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 6),
+                        "traffic": traffic_per_launch(kernel, workload),
+                        "bytes_per_launch": int(per_launch), "algorithmic_bytes": what,
+                        "avg_launch_ms": round(avg_ms, 4),
+                        "ms_per_step_1stream": round(ms_dom / args.roofline_steps, 3),
+                        "frames_per_launch": int(tuning["frames_per_batch"]),
+                        "measured": f"{args.roofline_steps} eager single-stream step(s) of the "
+                                    f"{label} leg, HIP events around each {kernel} launch",
+                        "stage_ms_per_step_1stream": stage_ms})
+        out.sort(key=lambda r: -r["ms_per_step_1stream"])
+        return (out[0] if out else None), (out[1] if len(out) > 1 else None)
 
     graph = not args.eager
     # ---- plain shake: the reference schedule (6 iterations from 0.25 vox)
@@ -553,13 +569,13 @@ def main() -> int:
     # bytes in FP32) plus positions, over the launch's frames; HBM traffic per
     # launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
     # this workload, when present.
-    roofline = None
+    roofline = roofline_other = None
     if args.roofline_steps > 0:
         if use_learned:
-            roofline = shake_roofline(ctx, lp, "learned")
+            roofline, roofline_other = rooflines(ctx, lp, "learned")
         else:
             ctx.set_shake_model(None)
-            roofline = shake_roofline(ctx, w0.params, "plain")
+            roofline, roofline_other = rooflines(ctx, w0.params, "plain")
     ctx.set_shake_model(None)
 
     if args.copy_calib_mib > 0:
@@ -592,6 +608,7 @@ def main() -> int:
                    "learned_batched_over_plain_per_frame": ratio(learned, per_frame),
                    "plain_batched_over_plain_per_frame": ratio(plain, per_frame)},
         "roofline": roofline,
+        "roofline_next": roofline_other,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

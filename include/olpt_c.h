@@ -107,12 +107,14 @@ olpt_status olpt_diagnostics(olpt_ctx *ctx, int32_t out[8]);
 
 /* Stage timing with HIP events on the context stream (eager runs only).
  * Slots: 0 peaks, 1 stereo build, 2 sort+prune+triangulate, 3 shake
- * kernel, 4 residual gather, 5 other.  olpt_stage_times returns the summed
- * milliseconds and launch counts since the last read and resets them.
+ * kernel, 4 residual gather, 5 the level-2 band enumeration kernel of the
+ * build (k_b_enum, third camera; also inside slot 1).  olpt_stage_times
+ * returns the summed milliseconds and launch counts since the last read and
+ * resets them.
  * Slot 6 (always recorded): host milliseconds spent issuing the work of
  * the eager olpt_ipr_run calls since the last read, and their number. */
 enum { OLPT_STAGE_PEAKS = 0, OLPT_STAGE_BUILD = 1, OLPT_STAGE_PRUNE = 2,
-       OLPT_STAGE_SHAKE = 3, OLPT_STAGE_RESIDUAL = 4, OLPT_STAGE_OTHER = 5,
+       OLPT_STAGE_SHAKE = 3, OLPT_STAGE_RESIDUAL = 4, OLPT_STAGE_ENUM2 = 5,
        OLPT_STAGE_HOST = 6 };
 olpt_status olpt_stage_timing(olpt_ctx *ctx, int enable);
 olpt_status olpt_stage_times(olpt_ctx *ctx, double ms[8], int32_t calls[8]);

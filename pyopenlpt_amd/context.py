@@ -177,7 +177,7 @@ class Context:
         check(self.L.olpt_diagnostics(self.h, out.ctypes.data_as(_lib._I32P)))
         return out
 
-    STAGES = ("peaks", "stereo_build", "prune", "shake", "residual", "other", "host_issue")
+    STAGES = ("peaks", "stereo_build", "prune", "shake", "residual", "enum2", "host_issue")
 
     def stage_timing(self, enable: bool = True):
         check(self.L.olpt_stage_timing(self.h, 1 if enable else 0))

@@ -563,13 +563,16 @@ olpt_status ipr_loop(olpt_ctx *c, Workspace &w, int nf, T *img, T *img_next,
     //  shake:    per iteration, the N x C x P^2 patch gathers of the
     //            reference ROI (P = 2 round(r ratio)) + positions
     //  residual: read the loop image and write the residual per camera
+    //  enum2:    per level-1 survivor, its record (12 B) and the two line
+    //            table entries of its strips (2 x 32 B)
     const double s_t = sizeof(T);
     const double na = cl.n, nc = c->n_cam;
     const double ncell = (double)w.sb.rows_cell * w.sb.cols_cell;
     const double P = std::round(2.0 * p->radius_obj * p->ratio_augimg);
     for (int f = 0; f < nf; ++f) {
-      int32_t np[OLPT_MAX_CAM], cnt[3];
+      int32_t np[OLPT_MAX_CAM], cnt[3], surv1 = 0;
       const long o = (long)f * fs;
+      HIP_TRY(hipMemcpyAsync(&surv1, fshift(w.sb.n_surv, o) + 1, 4, hipMemcpyDeviceToHost, st));
       HIP_TRY(hipMemcpyAsync(np, fshift(w.npts, o), sizeof(np), hipMemcpyDeviceToHost, st));
       HIP_TRY(hipMemcpyAsync(&cnt[0], fshift(w.sb.n_cand, o), 4, hipMemcpyDeviceToHost, st));
       HIP_TRY(hipMemcpyAsync(&cnt[1], fshift(w.sb.n_obj, o), 4, hipMemcpyDeviceToHost, st));
@@ -583,12 +586,14 @@ olpt_status ipr_loop(olpt_ctx *c, Workspace &w, int nf, T *img, T *img_next,
       c->st_bytes[ST_PRUNE] += 2.0 * M * (4.0 * nc + 8.0) + 2.0 * M * na * 4.0 + N * (24.0 + 4.0 * nc);
       c->st_bytes[ST_SHAKE] += p->n_shake_loop * (N * na * P * P * s_t + N * 48.0);
       c->st_bytes[ST_RESID] += p->n_shake_loop * (na * HW * 2.0 * s_t) + nc * HW * 2.0 * s_t;
+      if (sp.build_count > 2) c->st_bytes[ST_ENUM2] += (double)surv1 * (12.0 + 2.0 * 32.0);
     }
     c->st_launch[ST_PEAKS] += 1;
     c->st_launch[ST_BUILD] += 1;
     c->st_launch[ST_PRUNE] += 1;
     c->st_launch[ST_SHAKE] += p->n_shake_loop;
     c->st_launch[ST_RESID] += p->n_shake_loop + 1;
+    if (sp.build_count > 2) c->st_launch[ST_ENUM2] += 1;
   }
   return OLPT_OK;
 }

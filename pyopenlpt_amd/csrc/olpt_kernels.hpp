@@ -36,7 +36,7 @@ inline void stage_begin(int slot, hipStream_t st) {
 inline void stage_end(int slot, hipStream_t st) {
   if (g_timer) g_timer->end(slot, st);
 }
-enum { ST_PEAKS = 0, ST_BUILD = 1, ST_PRUNE = 2, ST_SHAKE = 3, ST_RESID = 4, ST_OTHER = 5 };
+enum { ST_PEAKS = 0, ST_BUILD = 1, ST_PRUNE = 2, ST_SHAKE = 3, ST_RESID = 4, ST_ENUM2 = 5 };
 
 // Frames of one batch share a workspace layout: frame f's buffers lie f *
 // fstride bytes after frame 0's (one arena per workspace), so a kernel
@@ -290,6 +290,9 @@ template <typename T> struct ShakeBuffers {
   double grid_lo[3], grid_cell;
   int32_t *g_cnt, *g_start, *g_obj, *g_rank, *g_cellof;
   int32_t *has_nb;
+  // order in which the per-object shake kernels visit the objects (null:
+  // index order); frame 0's array, shifted like the others
+  const int32_t *order;
   // per (object, camera) projection records, reduction partials
   void *rec;          // [cap_obj][n_cam] records (sizeof depends on T)
   double *partial;    // [2*256]
@@ -329,6 +332,7 @@ template <typename T> struct ShakeBuffers {
     o.g_rank = fshift(g_rank, d);
     o.g_cellof = fshift(g_cellof, d);
     o.has_nb = fshift(has_nb, d);
+    o.order = fshift(order, d);
     o.rec = fshift(rec, d);
     o.partial = fshift(partial, d);
     o.keep_off = fshift(keep_off, d);

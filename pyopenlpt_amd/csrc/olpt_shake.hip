@@ -306,6 +306,21 @@ __global__ void k_copy_cam(const T *__restrict__ src, T *__restrict__ dst, long 
     dst[k * HW + i] = src[k * HW + i];
 }
 
+// Workgroups are handed to the 8 XCDs round-robin in launch order, each XCD
+// with its own L2.  With a grid width that is a multiple of 8, workgroup x
+// of a frame runs on XCD x % 8.  xcd_chunk numbers the chunks of grid-stride
+// pass `pass` (rem chunks left from its start, each of OPW objects) XCD by
+// XCD, so that each XCD takes a contiguous run of the (spatially ordered)
+// objects, whose ROIs then share that XCD's L2 lines; a last, partial pass
+// is spread evenly over the 8 XCDs.  Every chunk is taken once; -1: none.
+__device__ __forceinline__ int xcd_chunk(int pass, int rem) {
+  const int G = gridDim.x, x = blockIdx.x;
+  if (G % 8 != 0) return x < rem ? pass * G + x : -1;
+  const int per = rem >= G ? G / 8 : (rem + 7) / 8;
+  const int k = (x % 8) * per + x / 8;
+  return x / 8 < per && k < rem ? pass * G + k : -1;
+}
+
 // --------------------------------------------------------------------------
 // k_shake: wave per object, lanes split by camera
 // --------------------------------------------------------------------------
@@ -647,7 +662,12 @@ k_shake(ShakeBuffers<T> bb, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
   c.act = c.cam_ok && p.active[c.k];
   const int ksafe = c.cam_ok ? c.k : 0;
   cam_lane_load(b, ksafe, c);
-  for (int i = blockIdx.x * OPW + lane / LPO; i < n; i += gridDim.x * OPW) {
+  const int n_chunk = (n + OPW - 1) / OPW;
+  for (int pass = 0; pass * (int)gridDim.x < n_chunk; ++pass) {
+    const int chunk = xcd_chunk(pass, n_chunk - pass * (int)gridDim.x);
+    const int ii = chunk * OPW + lane / LPO;
+    if (chunk < 0 || ii >= n) continue;   // wave-uniform except in the last chunk
+    const int i = b.order ? b.order[ii] : ii;
     T X[3] = {(T)b.xyz[3L * i], (T)b.xyz[3L * i + 1], (T)b.xyz[3L * i + 2]};
     build_roi<T, LPG, SLOTS>(b, p, rec, i, c);
     // shakeOneObject (:409-665); tmp keeps the last evaluated coordinate on
@@ -993,7 +1013,12 @@ k_propose(ShakeBuffers<T> bb, ShakeParams p, const ObjCamRec<T> *__restrict__ re
   c.act = c.cam_ok && p.active[c.k];
   cam_lane_load(b, c.cam_ok ? c.k : 0, c);
   const T lim = (T)p.max_step_vox, vox = (T)p.voxel_mm;
-  for (int i = blockIdx.x * OPW + lane / LPO; i < n; i += gridDim.x * OPW) {
+  const int n_chunk = (n + OPW - 1) / OPW;
+  for (int pass = 0; pass * (int)gridDim.x < n_chunk; ++pass) {
+    const int chunk = xcd_chunk(pass, n_chunk - pass * (int)gridDim.x);
+    const int ii = chunk * OPW + lane / LPO;
+    if (chunk < 0 || ii >= n) continue;   // wave-uniform except in the last chunk
+    const int i = b.order ? b.order[ii] : ii;
     T X[3] = {(T)b.xyz[3L * i], (T)b.xyz[3L * i + 1], (T)b.xyz[3L * i + 2]};
     build_roi<T, LPO / NC, SLOTS>(b, p, rec, i, c);
     T ue, ve;
@@ -1045,6 +1070,37 @@ __global__ void k_grid_fill(ShakeBuffers<T> bb) {
   const int n = *b.n_obj;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     b.g_obj[b.g_start[b.g_cellof[i]] + b.g_rank[i]] = i;
+}
+
+// Processing order of the per-object shake kernels: objects sorted by the
+// Morton code of their cell in a 64^3 grid over the repeated-object grid's
+// box (a counting sort through the g_* arrays, free until markRepeatedObj
+// rebuilds them).  Objects close in 3D project close in every camera.
+constexpr int kOrderBits = 6;
+__device__ __forceinline__ unsigned spread3(unsigned v) {
+  unsigned r = 0;
+#pragma unroll
+  for (int k = 0; k < kOrderBits; ++k) r |= ((v >> k) & 1u) << (3 * k);
+  return r;
+}
+
+template <typename T>
+__global__ void k_order_count(ShakeBuffers<T> bb) {
+  const long F = (long)blockIdx.z;
+  const ShakeBuffers<T> b = bb.frame(F);
+  const int n = *b.n_obj;
+  const double cell = b.grid_cell * b.grid_n / (double)(1 << kOrderBits);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    unsigned key = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      int c = (int)floor((b.xyz[3L * i + d] - b.grid_lo[d]) / cell);
+      c = c < 0 ? 0 : (c >= (1 << kOrderBits) ? (1 << kOrderBits) - 1 : c);
+      key |= spread3((unsigned)c) << d;
+    }
+    b.g_cellof[i] = (int)key;
+    b.g_rank[i] = atomicAdd(&b.g_cnt[key], 1);
+  }
 }
 
 // nanoflann L2_Simple_Adaptor distance with diff = query - point
@@ -1228,6 +1284,17 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
   const int roi_px = p.roi_w > 0 ? p.roi_w * p.roi_w : 0;
   const int gO = grid_for(b.cap_obj, 256, 2048);
   k_init_score<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b);
+  // the spatial processing order (g_cnt is all zero here; the scan clears
+  // it), used by the per-object kernels until markRepeatedObj
+  ShakeBuffers<T> bo = b;
+  if ((long)b.grid_n * b.grid_n * b.grid_n >= (1L << (3 * kOrderBits))) {
+    const long nkey = 1L << (3 * kOrderBits);
+    k_order_count<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b);
+    scan_exclusive_clear(b.g_cnt, b.g_start, nkey, b.g_start + nkey, b.scan_scratch, st, b.nf,
+                         b.fstride);
+    k_grid_fill<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b);
+    bo.order = b.g_obj;
+  }
   if (p.mlp && b.patches && b.prop && nc == 4 && roi_px <= 64) {
     // learned proposal from the matched positions, then the plain iterations
     residual_pass<T>(b, b.img, b.res, b.xyz, b.n_obj, rec, b.uv, act, 0, r_px, st);
@@ -1236,7 +1303,7 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
     mlp_infer_device(p.mlp, b.patches, b.n_obj, b.cap_obj, b.prop, st, b.nf, b.lstride,
                      b.fstride, b.pstride);
     constexpr int O = shake_opw<T>();
-    k_propose<T, 4, O, 4 * O><<<dim3(b.grid_shake, 1, b.nf), kWave, 0, st>>>(b, p, rec, b.prop);
+    k_propose<T, 4, O, 4 * O><<<dim3(b.grid_shake, 1, b.nf), kWave, 0, st>>>(bo, p, rec, b.prop);
   }
   double delta = p.shake_width;
   double dmin = p.shakewidth_min > 0 ? p.shakewidth_min : delta / 20;
@@ -1247,13 +1314,13 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
     if (nc < 2) {
       k_zero_score<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b);
     } else if (nc <= 4 && roi_px <= 64) {
-      shake_iter<T, 4, shake_opw<T>(), 4 * shake_opw<T>()>(b, p, rec, (T)delta, st);
+      shake_iter<T, 4, shake_opw<T>(), 4 * shake_opw<T>()>(bo, p, rec, (T)delta, st);
     } else if (nc <= 4) {
-      shake_iter<T, 4, 1, 16>(b, p, rec, (T)delta, st);
+      shake_iter<T, 4, 1, 16>(bo, p, rec, (T)delta, st);
     } else if (roi_px <= 64) {
-      shake_iter<T, 8, 1, 8>(b, p, rec, (T)delta, st);
+      shake_iter<T, 8, 1, 8>(bo, p, rec, (T)delta, st);
     } else {
-      shake_iter<T, 8, 1, 32>(b, p, rec, (T)delta, st);
+      shake_iter<T, 8, 1, 32>(bo, p, rec, (T)delta, st);
     }
     delta *= 0.5;
     if (delta < dmin) delta = dmin;

@@ -1727,7 +1727,9 @@ static int grid_for(long n, int threads, int max_blocks) {
 template <int K>
 static void build_level(const StereoBuffers &b, const StereoParams &p, hipStream_t st) {
   zero_i32(b, b.n_pairs, 1, st);
+  if (K == 2) stage_begin(ST_ENUM2, st);
   k_b_enum<K><<<dim3(b.grid_build, 1, b.nf), 256, 0, st>>>(b, p);
+  if (K == 2) stage_end(ST_ENUM2, st);
   k_b_extend<K><<<dim3(b.grid_build, 1, b.nf), 256, 0, st>>>(b, p);
 }
 
