@@ -563,8 +563,8 @@ olpt_status ipr_loop(olpt_ctx *c, Workspace &w, int nf, T *img, T *img_next,
     //  shake:    per iteration, the N x C x P^2 patch gathers of the
     //            reference ROI (P = 2 round(r ratio)) + positions
     //  residual: read the loop image and write the residual per camera
-    //  enum2:    per level-1 survivor, its record (12 B) and the two line
-    //            table entries of its strips (2 x 32 B)
+    //  enum2:    per level-1 survivor its record (12 B), and the line table
+    //            once (each 2D point's lines on the other cameras, 32 B each)
     const double s_t = sizeof(T);
     const double na = cl.n, nc = c->n_cam;
     const double ncell = (double)w.sb.rows_cell * w.sb.cols_cell;
@@ -586,7 +586,8 @@ olpt_status ipr_loop(olpt_ctx *c, Workspace &w, int nf, T *img, T *img_next,
       c->st_bytes[ST_PRUNE] += 2.0 * M * (4.0 * nc + 8.0) + 2.0 * M * na * 4.0 + N * (24.0 + 4.0 * nc);
       c->st_bytes[ST_SHAKE] += p->n_shake_loop * (N * na * P * P * s_t + N * 48.0);
       c->st_bytes[ST_RESID] += p->n_shake_loop * (na * HW * 2.0 * s_t) + nc * HW * 2.0 * s_t;
-      if (sp.build_count > 2) c->st_bytes[ST_ENUM2] += (double)surv1 * (12.0 + 2.0 * 32.0);
+      if (sp.build_count > 2)
+        c->st_bytes[ST_ENUM2] += (double)surv1 * 12.0 + n2d * (na - 1.0) * 32.0;
     }
     c->st_launch[ST_PEAKS] += 1;
     c->st_launch[ST_BUILD] += 1;
