@@ -563,16 +563,20 @@ olpt_status ipr_loop(olpt_ctx *c, Workspace &w, int nf, T *img, T *img_next,
     //  shake:    per iteration, the N x C x P^2 patch gathers of the
     //            reference ROI (P = 2 round(r ratio)) + positions
     //  residual: read the loop image and write the residual per camera
-    //  enum2:    per level-1 survivor its record (12 B), and the line table
-    //            once (each 2D point's lines on the other cameras, 32 B each)
+    //  enum2:    the enumeration of the level-1 survivors' next camera:
+    //            each survivor's path record read, each level-2 item written,
+    //            and the line table once (each 2D point's lines on the other
+    //            cameras, 32 B each)
     const double s_t = sizeof(T);
     const double na = cl.n, nc = c->n_cam;
     const double ncell = (double)w.sb.rows_cell * w.sb.cols_cell;
     const double P = std::round(2.0 * p->radius_obj * p->ratio_augimg);
     for (int f = 0; f < nf; ++f) {
-      int32_t np[OLPT_MAX_CAM], cnt[3], surv1 = 0;
+      int32_t np[OLPT_MAX_CAM], cnt[3], lv[3] = {0, 0, 0};
       const long o = (long)f * fs;
-      HIP_TRY(hipMemcpyAsync(&surv1, fshift(w.sb.n_surv, o) + 1, 4, hipMemcpyDeviceToHost, st));
+      // level-1 survivors, level-2 items
+      HIP_TRY(hipMemcpyAsync(&lv[0], fshift(w.sb.n_surv, o) + 1, 4, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipMemcpyAsync(&lv[1], fshift(w.sb.n_path, o) + 2, 4, hipMemcpyDeviceToHost, st));
       HIP_TRY(hipMemcpyAsync(np, fshift(w.npts, o), sizeof(np), hipMemcpyDeviceToHost, st));
       HIP_TRY(hipMemcpyAsync(&cnt[0], fshift(w.sb.n_cand, o), 4, hipMemcpyDeviceToHost, st));
       HIP_TRY(hipMemcpyAsync(&cnt[1], fshift(w.sb.n_obj, o), 4, hipMemcpyDeviceToHost, st));
@@ -587,7 +591,8 @@ olpt_status ipr_loop(olpt_ctx *c, Workspace &w, int nf, T *img, T *img_next,
       c->st_bytes[ST_SHAKE] += p->n_shake_loop * (N * na * P * P * s_t + N * 48.0);
       c->st_bytes[ST_RESID] += p->n_shake_loop * (na * HW * 2.0 * s_t) + nc * HW * 2.0 * s_t;
       if (sp.build_count > 2)
-        c->st_bytes[ST_ENUM2] += (double)surv1 * 12.0 + n2d * (na - 1.0) * 32.0;
+        c->st_bytes[ST_ENUM2] += ((double)lv[0] + (double)lv[1]) * sizeof(PathRec<2>) +
+                                 n2d * (na - 1.0) * 32.0;
     }
     c->st_launch[ST_PEAKS] += 1;
     c->st_launch[ST_BUILD] += 1;
@@ -777,7 +782,9 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   b.cap_pairs = std::max<long>(1L << 22, 16L * M);
   A(b.pairs, int2, b.cap_pairs);
   A(b.n_pairs, int32_t, 1);
-  A(b.surv, SurvRec, std::max(1, nc - 2) * b.cap_pairs);
+  L.add(&b.path[0], path_rec_bytes(nc) * (size_t)b.cap_pairs);
+  L.add(&b.path[1], path_rec_bytes(nc) * (size_t)b.cap_pairs);
+  A(b.n_path, int32_t, kMaxCam);
   A(b.n_surv, int32_t, kMaxCam);
   b.cell_px = 4;
   b.rows_cell = std::max(1, H / 4 + (H % 4 ? 1 : 0));

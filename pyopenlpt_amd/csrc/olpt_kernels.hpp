@@ -81,13 +81,19 @@ inline long peak_chunks(int H, int W) {
 // A partial match that survived one build level: its point, the camera it
 // lies on, the camera to search next and the partial match it extends (the
 // reference observation for level 1).
-struct SurvRec {
-  int32_t parent;
-  int32_t pid;
-  int8_t cam;
-  int8_t next;
-  int16_t pad;
+// A partial match of the stereo build with the point it is being extended
+// by: the points and cameras of path slots 0..K (slot 0 the reference
+// observation, slot K the point under test).
+template <int K> struct PathRec {
+  int32_t pid[K + 1];
+  int8_t cam[K + 1];
 };
+
+// bytes of the largest path record a build with n_cam cameras lists
+inline long path_rec_bytes(int n_cam) {
+  const long k1 = n_cam > 1 ? n_cam : 2;   // K + 1 <= n_cam
+  return (k1 * 5 + 3) / 4 * 4;
+}
 
 constexpr int kHzVals = 4;   // distinct values kept per eq_eps check bucket
 constexpr int kHzWords = 8;  // 64-bit words per table slot (key, values, padding)
@@ -105,8 +111,12 @@ struct StereoBuffers {
   long cap_pairs;
   int2 *pairs;           // [cap_pairs] (partial match, point) work list
   int32_t *n_pairs;      // device
-  SurvRec *surv;         // [n_cam - 2][cap_pairs] survivors per level
-  int32_t *n_surv;       // [kMaxCam] device
+  // the build's work lists as whole paths: path[0] the (partial match,
+  // point) items of level K >= 2 (PathRec<K>), path[1] the survivors of
+  // level K (PathRec<K + 1>: K + 1 points and the next camera in slot K + 1)
+  void *path[2];         // [cap_pairs] records each
+  int32_t *n_path;       // [kMaxCam] device: items of each level
+  int32_t *n_surv;       // [kMaxCam] device: survivors of each level
   int cell_px, rows_cell, cols_cell;
   int32_t *cell_cnt;     // [n_cam][ncell]
   int32_t *cell_start;   // [n_cam][ncell]
@@ -189,7 +199,9 @@ struct StereoBuffers {
     o.t1 = fshift(t1, d);
     o.pairs = fshift(pairs, d);
     o.n_pairs = fshift(n_pairs, d);
-    o.surv = fshift(surv, d);
+    o.path[0] = fshift(path[0], d);
+    o.path[1] = fshift(path[1], d);
+    o.n_path = fshift(n_path, d);
     o.n_surv = fshift(n_surv, d);
     o.cell_cnt = fshift(cell_cnt, d);
     o.cell_start = fshift(cell_start, d);

@@ -10,16 +10,16 @@
 //                 first over all reference observations at once, one kernel
 //                 per step and one thread per work item: k_b_second picks
 //                 each observation's second camera, k_b_band enumerates its
-//                 epipolar band (thread per observation x cell row of the
+//                 epipolar band (thread per observation x 4 cell rows of the
 //                 strip, computeStripIntersection :1475-1566), k_b_extend
 //                 runs back-projection, triangulation and camera selection
-//                 on every (partial match, point) pair, k_b_enum enumerates
-//                 the strip intersection of every surviving partial match
-//                 on its next camera, but only the cell rows inside the
-//                 y-extent of the intersection (a superset of the
-//                 reference's non-empty spans).  Lists grow by wave-
-//                 aggregated appends; the emitted set does not depend on
-//                 visiting order.
+//                 on every (partial match, point) item, k_b_enum enumerates
+//                 the strip intersection of every survivor on its next
+//                 camera, but only the cell rows inside the y-extent of the
+//                 intersection (a superset of the reference's non-empty
+//                 spans).  Items and survivors carry their whole paths; lists
+//                 grow by workgroup batches in LDS; the emitted set does not
+//                 depend on visiting order.
 //   k_key_* / k_bucket_rank  lexicographic sort of the id vectors
 //                 (:153-156): counting sort on the first active camera's id,
 //                 then each candidate's place inside its (small) bucket by
@@ -477,6 +477,7 @@ constexpr int kRowsPass = 4;      // cell rows whose ranges are loaded together
 // Counts do not depend on the order of the comparisons.
 constexpr int kRankBatch = 4;
 constexpr int kSurvQueue = 1024;
+constexpr int kPathQueue = 1024;
 constexpr int kCandQueue = 512;
 
 // Path of a partial match in build order: slot 0 is the reference
@@ -502,25 +503,6 @@ __device__ __forceinline__ void path_set(Path &P, int k, const StereoBuffers &b,
     P.P[k][c] = b.cams[cam].Tinv[c];
     P.U[k][c] = u[c];
   }
-}
-
-__device__ __forceinline__ const SurvRec &surv_at(const StereoBuffers &b, int level, int s) {
-  return b.surv[(long)(level - 1) * b.cap_pairs + s];
-}
-
-// Cameras and points of survivor s of level K-1 (a path of length K).
-template <int K>
-__device__ __forceinline__ void path_ids(const StereoBuffers &b, int ref, int s,
-                                         int cam[kMaxCam], int pid[kMaxCam]) {
-#pragma unroll
-  for (int k = K - 1; k >= 1; --k) {
-    const SurvRec r = surv_at(b, k, s);
-    cam[k] = r.cam;
-    pid[k] = r.pid;
-    s = r.parent;
-  }
-  cam[0] = ref;
-  pid[0] = s;
 }
 
 // reference camera: fewest observations among active cameras, lowest id on
@@ -789,52 +771,62 @@ __global__ void __launch_bounds__(256) k_b_band(StereoBuffers bb, StereoParams p
   bq_drain(q, b.n_pairs, b.cap_pairs, &b.flags[1], true, out);
 }
 
-// Deeper bands: one thread per survivor of level K-1 (a path of length K),
-// enumerating the cell rows of its K-strip intersection on its next camera.
+// Deeper bands: one thread per survivor of level K-1 - a path of K points
+// and its next camera, read whole from one record - enumerating the cell
+// rows of its K-strip intersection on that camera; every point found
+// becomes a level-K item (the path with the point in slot K).
 template <int K>
 __global__ void __launch_bounds__(256) k_b_enum(StereoBuffers bb, StereoParams p) {
   const StereoBuffers b = bb.frame(blockIdx.z);
-  int nref;
-  const int ref = ref_camera(b, p, nref);
   const int rows = b.rows_cell;
   const double tolg = p.tol2d + 1e-6;
   const double tol2 = p.tol2d * p.tol2d;
   const int n_in = min(b.n_surv[K - 1], (int)b.cap_pairs);
   const int stride = gridDim.x * blockDim.x;
-  __shared__ BlockQueue<int2, kQueue> q;
+  const PathRec<K> *surv = reinterpret_cast<const PathRec<K> *>(b.path[1]);
+  PathRec<K> *items = reinterpret_cast<PathRec<K> *>(b.path[0]);
+  __shared__ BlockQueue<PathRec<K>, kPathQueue> q;
   bq_init(q);
-  auto out = [&](long i, const int2 &v) { b.pairs[i] = v; };
+  auto out = [&](long i, const PathRec<K> &v) { items[i] = v; };
   for (int base = blockIdx.x * blockDim.x; base < n_in; base += stride) {
     const int s = base + threadIdx.x;
     if (s < n_in) {
-      const int t = surv_at(b, K - 1, s).next;
-      int cam[kMaxCam], pid[kMaxCam];
-      path_ids<K>(b, ref, s, cam, pid);
+      const PathRec<K> rec = surv[s];
+      const int t = rec.cam[K];
       Strip S[K];
       StripRows SR[K];
 #pragma unroll
       for (int j = 0; j < K; ++j) {
-        S[j] = strip_of(line_of(b, cam[j], t, pid[j]));
+        S[j] = strip_of(line_of(b, rec.cam[j], t, rec.pid[j]));
         SR[j] = strip_rows(S[j], b.cell_px, tolg);
       }
       int r0 = 0, r1 = rows - 1;
 #pragma unroll
       for (int j = 1; j < K; ++j) pair_rows(S[0], S[j], b.cell_px, tolg, rows, r0, r1);
       enum_rows<K>(b, t, S, SR, r0, r1, tol2, [&](bool acc, int pid_t) {
-        bq_push_any(q, acc, make_int2(s, pid_t), b.n_pairs, b.cap_pairs, &b.flags[1], out);
+        PathRec<K> r = rec;
+        r.pid[K] = pid_t;
+        bq_push_any(q, acc, r, &b.n_path[K], b.cap_pairs, &b.flags[1], out);
       });
     }
-    bq_drain(q, b.n_pairs, b.cap_pairs, &b.flags[1], false, out);
+    bq_drain(q, &b.n_path[K], b.cap_pairs, &b.flags[1], false, out);
   }
-  bq_drain(q, b.n_pairs, b.cap_pairs, &b.flags[1], true, out);
+  bq_drain(q, &b.n_path[K], b.cap_pairs, &b.flags[1], true, out);
 }
 
-// Extension of every (path of length K, point) pair: a survivor goes to
-// level K, a complete match (K + 1 == build_count) to the candidate list.
-// Levels 1 and 2 are held to 168 registers (3 waves per SIMD: measured
-// faster than the unconstrained 2 waves, while 4 waves spill); deeper
-// levels keep 2 waves.
+// Extension of every level-K item (a path of K points and a point under test
+// on its next camera): a survivor goes to level K's survivor list as a whole
+// path with its next camera, a complete match (K + 1 == build_count) to the
+// candidate list.  Level 1's items are the band's (observation, point)
+// pairs.  Levels 1 and 2 are held to 168 registers (3 waves per SIMD:
+// measured faster than the unconstrained 2 waves, while 4 waves spill);
+// deeper levels keep 2 waves.
 template <int K> constexpr int extend_waves() { return K <= 2 ? 3 : 2; }
+
+template <int K> union ExtendQueues {
+  BlockQueue<CandRec, kCandQueue> c;           // terminal level
+  BlockQueue<PathRec<K + 1>, kSurvQueue> s;    // the others
+};
 
 template <int K>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(extend_waves<K>())))
@@ -842,40 +834,49 @@ k_b_extend(StereoBuffers bb, StereoParams p) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   int nref;
   const int ref = ref_camera(b, p, nref);
-  const int n = min(*b.n_pairs, (int)b.cap_pairs);
+  const int n = min(K == 1 ? *b.n_pairs : b.n_path[K], (int)b.cap_pairs);
   const bool terminal = K + 1 == p.build_count;
   const int stride = gridDim.x * blockDim.x;
-  __shared__ BlockQueue<CandRec, kCandQueue> qc;
-  __shared__ BlockQueue<SurvRec, kSurvQueue> qs;
-  bq_init(qc);
-  bq_init(qs);
+  __shared__ ExtendQueues<K> eq;
+  if (threadIdx.x == 0) {
+    if (terminal)
+      eq.c.n = 0;
+    else
+      eq.s.n = 0;
+  }
+  __syncthreads();
   const int nc = b.n_cam;
   auto out_c = [&](long i, const CandRec &v) {
     for (int c = 0; c < nc; ++c) b.cand_ids[i * nc + c] = v.ids[c];
     b.cand_e[i] = v.e;
   };
-  SurvRec *surv_out = b.surv + (long)(K - 1) * b.cap_pairs;
-  auto out_s = [&](long i, const SurvRec &v) { surv_out[i] = v; };
+  const PathRec<K> *items = reinterpret_cast<const PathRec<K> *>(b.path[0]);
+  PathRec<K + 1> *surv_out = reinterpret_cast<PathRec<K + 1> *>(b.path[1]);
+  auto out_s = [&](long i, const PathRec<K + 1> &v) { surv_out[i] = v; };
   for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
     const int q = base + threadIdx.x;
-    int res = 0, next = -1, t = 0;
+    int res = 0, next = -1;
     double e = 0.0;
     Path P;
-    int2 pr = make_int2(0, 0);
     if (q < n) {
-      pr = b.pairs[q];
-      int cam[kMaxCam], pid[kMaxCam];
+      int cam[K + 1], pid[K + 1];
       if constexpr (K == 1) {
+        const int2 pr = b.pairs[q];
         cam[0] = ref;
         pid[0] = pr.x;
-        t = b.t1[pr.x];
+        cam[1] = b.t1[pr.x];
+        pid[1] = pr.y;
       } else {
-        path_ids<K>(b, ref, pr.x, cam, pid);
-        t = surv_at(b, K - 1, pr.x).next;
+        const PathRec<K> r = items[q];
+#pragma unroll
+        for (int j = 0; j <= K; ++j) {
+          cam[j] = r.cam[j];
+          pid[j] = r.pid[j];
+        }
       }
 #pragma unroll
       for (int j = 0; j < K; ++j) path_set(P, j, b, cam[j], pid[j]);
-      res = extend<K>(b, p, P, t, pr.y, &next, &e);
+      res = extend<K>(b, p, P, cam[K], pid[K], &next, &e);
     }
     if (terminal) {
       CandRec cr;
@@ -890,17 +891,27 @@ k_b_extend(StereoBuffers bb, StereoParams p) {
         }
         cr.e = e;
       }
-      bq_push(qc, res == 1, cr);
-      bq_flush(qc, b.n_cand, b.cap_cand, &b.flags[0], false, out_c);
+      bq_push(eq.c, res == 1, cr);
+      bq_flush(eq.c, b.n_cand, b.cap_cand, &b.flags[0], false, out_c);
     } else {
-      bq_push(qs, res == 2, SurvRec{pr.x, pr.y, (int8_t)t, (int8_t)next, 0});
-      bq_flush(qs, &b.n_surv[K], b.cap_pairs, &b.flags[1], false, out_s);
+      PathRec<K + 1> sr;
+      if (res == 2) {
+#pragma unroll
+        for (int j = 0; j <= K; ++j) {
+          sr.pid[j] = P.pid[j];
+          sr.cam[j] = (int8_t)P.cam[j];
+        }
+        sr.pid[K + 1] = -1;
+        sr.cam[K + 1] = (int8_t)next;
+      }
+      bq_push(eq.s, res == 2, sr);
+      bq_flush(eq.s, &b.n_surv[K], b.cap_pairs, &b.flags[1], false, out_s);
     }
   }
   if (terminal)
-    bq_flush(qc, b.n_cand, b.cap_cand, &b.flags[0], true, out_c);
+    bq_flush(eq.c, b.n_cand, b.cap_cand, &b.flags[0], true, out_c);
   else
-    bq_flush(qs, &b.n_surv[K], b.cap_pairs, &b.flags[1], true, out_s);
+    bq_flush(eq.s, &b.n_surv[K], b.cap_pairs, &b.flags[1], true, out_s);
 }
 
 // --------------------------------------------------------------------------
@@ -1726,7 +1737,6 @@ static int grid_for(long n, int threads, int max_blocks) {
 
 template <int K>
 static void build_level(const StereoBuffers &b, const StereoParams &p, hipStream_t st) {
-  zero_i32(b, b.n_pairs, 1, st);
   if (K == 2) stage_begin(ST_ENUM2, st);
   k_b_enum<K><<<dim3(b.grid_build, 1, b.nf), 256, 0, st>>>(b, p);
   if (K == 2) stage_end(ST_ENUM2, st);
@@ -1785,6 +1795,7 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   // extend pair of kernels per deeper build camera
   const int G = b.grid_build;
   stage_begin(ST_BUILD, st);
+  zero_i32(b, b.n_path, kMaxCam, st);
   zero_i32(b, b.n_surv, kMaxCam, st);
   zero_i32(b, b.n_pairs, 1, st);
   k_b_second<<<dim3(grid_for(b.cap, T, 1024), 1, b.nf), T, 0, st>>>(b, p);
