@@ -18,7 +18,7 @@
       waiting (SQ counters of a third run)
 
   python -m pyopenlpt_amd.prof step [--frames 16] [--batch 16] [--steps 1]
-      [--shake-loops 6] [--calib-mib 0]
+      [--shake-loops 6] [--calib-mib 0] [--learned]
       the program the profiles are taken on: the ppp 0.05 clustered frames
       (1024^2 u16, 4 cameras) resident in HBM, one eager olpt_ipr_run over
       all of them per step on one stream, `batch` frames per launch; then,
@@ -152,7 +152,8 @@ def table(stats_csv: str, fetch: str, write: str, sq: str | None, calib_kernel: 
     return "\n".join(out)
 
 
-def step(frames: int, batch: int, steps: int, shake_loops: int, calib_mib: int) -> int:
+def step(frames: int, batch: int, steps: int, shake_loops: int, calib_mib: int,
+         learned: bool = False) -> int:
     import concurrent.futures as cf
     import multiprocessing as mp
     import time
@@ -179,6 +180,20 @@ def step(frames: int, batch: int, steps: int, shake_loops: int, calib_mib: int) 
     ctx.set_tuning(n_streams=1, frames_per_batch=batch)
     params = S.OlptIprParams.from_buffer_copy(w0.params)
     params.n_shake_loop = shake_loops
+    if learned:
+        # the learned leg's schedule: a briefly trained proposal network, then
+        # one plain iteration from 0.03 voxel
+        from .learned import ShakeNet, make_training_set, train
+        tctx = Context(w0.cams, H, W, precision="f32")
+        tctx.set_otf(otf)
+        X, T, _ = make_training_set(tctx, w0.params, w0.voxel_mm, n_particles=500_000, seed=9001)
+        net = ShakeNet(max_batch=8192, seed=0)
+        train(net, X, T, steps=500, batch=8192, lr=3e-3, ghost_weight=0.5)
+        tctx.close()
+        del X, T
+        ctx.set_shake_model(net, w0.voxel_mm, 0.6)
+        params.n_shake_loop = 1
+        params.shake_width = 0.03 * w0.voxel_mm
     outs = ctx.alloc_outputs(frames, 1 << 17, d.device)
     for s in range(steps):
         t0 = time.perf_counter()
@@ -223,6 +238,8 @@ def main(argv=None) -> int:
     d.add_argument("--steps", type=int, default=1)
     d.add_argument("--shake-loops", type=int, default=6)
     d.add_argument("--calib-mib", type=int, default=0)
+    d.add_argument("--learned", action="store_true",
+                   help="the learned leg's schedule (a briefly trained proposal, 1 iteration)")
     args = ap.parse_args(argv)
     if args.cmd == "stats":
         print(stats(args.csv, args.top))
@@ -232,7 +249,8 @@ def main(argv=None) -> int:
         print(table(args.stats, args.fetch, args.write, args.sq, args.calib_kernel,
                     args.calib_bytes, args.min_pct))
     else:
-        return step(args.frames, args.batch, args.steps, args.shake_loops, args.calib_mib)
+        return step(args.frames, args.batch, args.steps, args.shake_loops, args.calib_mib,
+                    args.learned)
     return 0
 
 
