@@ -817,10 +817,11 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   A(b.pt_flag, int32_t, (long)nc * cap);
   A(b.head, int32_t, (long)nc * cap);
   A(b.n_entries, int32_t, 1);
-  A(b.flat_rank, int32_t, (long)M * nc);
+  const long grp_n = grp_buckets((long)nc * cap) * (long)kGrpBlocks;
+  A(b.grp_cnt, int32_t, grp_n);
+  A(b.grp_off, int32_t, grp_n);
   A(b.e_ent, double, (long)M * nc);
   A(b.p_ent, double, (long)M * nc);
-  A(b.ent_q, int32_t, (long)M * nc);
   A(b.st_ent, int32_t, (long)M * nc);
   A(b.pv, double, (long)M * nc);
   A(b.pct, double, M);
@@ -843,6 +844,7 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   const long grid_cells = (long)c->grid_n * c->grid_n * c->grid_n;
   long scan_n = std::max<long>(std::max<long>(M, (long)nc * cap), std::max<long>(ncell, grid_cells));
   scan_n = std::max<long>(scan_n, nchunk);
+  scan_n = std::max<long>(scan_n, grp_n);
   scan_n = std::max<long>(scan_n, (long)c->bin_rows * c->bin_cols * nc);
   A(w.scan_scratch, int32_t, scan_scratch_ints(scan_n, nc) + 64);
   b.scan_scratch = w.scan_scratch;
@@ -965,6 +967,8 @@ olpt_status olpt_ctx_reserve(olpt_ctx *c, int n_cam, int H, int W, int max_2d, i
     return fail(OLPT_E_INVALID_ARGUMENT, "bad camera count or image size");
   if (max_cand >= (1 << 27))
     return fail(OLPT_E_INVALID_ARGUMENT, "max_cand must be below 2^27");
+  if (max_2d > (1 << 20))
+    return fail(OLPT_E_INVALID_ARGUMENT, "max_2d must be at most 2^20");
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipDeviceSynchronize());
   free_all(c);

@@ -95,6 +95,20 @@ inline long path_rec_bytes(int n_cam) {
   return (k1 * 5 + 3) / 4 * 4;
 }
 
+// point-list grouping (prune): buckets of 2^low consecutive flat points,
+// at most kGrpMaxBuckets of them; kGrpBlocks workgroups in the first pass
+constexpr int kGrpBlocks = 1024;
+constexpr int kGrpMaxBuckets = 4096;
+constexpr int kGrpMaxLow = 12;
+inline int grp_low(long nflat) {
+  int low = 9;
+  while ((nflat >> low) >= kGrpMaxBuckets) ++low;
+  return low;
+}
+inline long grp_buckets(long nflat) {
+  const int low = grp_low(nflat);
+  return (nflat + (1L << low) - 1) >> low;
+}
 constexpr int kHzVals = 4;   // distinct values kept per eq_eps check bucket
 constexpr int kHzWords = 8;  // 64-bit words per table slot (key, values, padding)
 
@@ -147,13 +161,13 @@ struct StereoBuffers {
   int32_t *pt_flag;      // [n_cam*cap] bit0/bit1: sequential percentile/priority
   int32_t *head;         // [n_cam*cap] first entry not known to be rejected
   int32_t *n_entries;    // device: total list entries
-  int32_t *flat_rank;    // [cap_cand][n_cam]
+  int32_t *grp_cnt;      // [grp buckets * kGrpBlocks] point-list grouping counts
+  int32_t *grp_off;      // [grp buckets * kGrpBlocks] their exclusive scan
   // per list entry, in pt_list order: the entry's candidate's e_check and
   // percentile score, so that a point's rank loops read its own contiguous
-  // range; ent_q: the pt_list position of candidate j's entry in slot k
+  // range
   double *e_ent;         // [cap_cand*n_cam]
   double *p_ent;         // [cap_cand*n_cam]
-  int32_t *ent_q;        // [cap_cand][n_cam]
   // state of the candidate at each pt_sorted position (the greedy's scans
   // read a point's run of states contiguously)
   int32_t *st_ent;       // [cap_cand*n_cam]
@@ -229,10 +243,10 @@ struct StereoBuffers {
     o.pt_flag = fshift(pt_flag, d);
     o.head = fshift(head, d);
     o.n_entries = fshift(n_entries, d);
-    o.flat_rank = fshift(flat_rank, d);
+    o.grp_cnt = fshift(grp_cnt, d);
+    o.grp_off = fshift(grp_off, d);
     o.e_ent = fshift(e_ent, d);
     o.p_ent = fshift(p_ent, d);
-    o.ent_q = fshift(ent_q, d);
     o.st_ent = fshift(st_ent, d);
     o.pv = fshift(pv, d);
     o.pct = fshift(pct, d);

@@ -1075,44 +1075,119 @@ __device__ __forceinline__ int cand_points(const StereoBuffers &b, const StereoP
   return k;
 }
 
-__global__ void k_flat(StereoBuffers bb, StereoParams p) {
+// Point lists by a two-level grouping instead of one counting atomic per
+// entry: the entries (flat point f, candidate << 3 | slot) go into buckets
+// of 2^low consecutive flat points - each workgroup histograms its own
+// contiguous run of candidates in LDS, a scan over (bucket, workgroup) gives
+// every workgroup its slot ranges, no global atomics - then one workgroup
+// per bucket counting-sorts the bucket's entries in LDS, which also gives
+// every point's list start and length.  The order inside a list is
+// arbitrary: the ranks and the greedy do not depend on it.
+struct GrpShape {
+  int low, nb, g1;   // bucket bits, buckets, counting workgroups
+};
+
+// pass 1: per (bucket, workgroup) counts of the workgroup's run
+__global__ void __launch_bounds__(256) k_grp_count(StereoBuffers bb, StereoParams p, GrpShape g) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
-  const int nc = b.n_cam;
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
+  __shared__ int h[kGrpMaxBuckets];
+  for (int i = threadIdx.x; i < g.nb; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const long per = ((long)M + g.g1 - 1) / g.g1;
+  const int lo = (int)min((long)M, blockIdx.x * per), hi = (int)min((long)M, lo + per);
+  for (int j = lo + threadIdx.x; j < hi; j += blockDim.x) {
     long f[kMaxCam];
     cand_points(b, p, j, f);
-    // the counting atomics of all slots in flight together
-    int r[kMaxCam];
-#pragma unroll
-    for (int k = 0; k < kMaxCam; ++k) r[k] = f[k] >= 0 ? atomicAdd(&b.pt_count[f[k]], 1) : 0;
 #pragma unroll
     for (int k = 0; k < kMaxCam; ++k)
-      if (f[k] >= 0) b.flat_rank[(long)j * nc + k] = r[k];
+      if (f[k] >= 0) atomicAdd(&h[f[k] >> g.low], 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < g.nb; i += blockDim.x) b.grp_cnt[(long)i * g.g1 + blockIdx.x] = h[i];
+}
+
+// pass 1b: the same runs, each entry to its slot (staging in pt_sorted /
+// pt_occ, free until the priority ranks)
+__global__ void __launch_bounds__(256) k_grp_place(StereoBuffers bb, StereoParams p, GrpShape g) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
+  const int M = min(*b.n_cand, b.cap_cand);
+  __shared__ int cur[kGrpMaxBuckets];
+  for (int i = threadIdx.x; i < g.nb; i += blockDim.x)
+    cur[i] = b.grp_off[(long)i * g.g1 + blockIdx.x];
+  __syncthreads();
+  const long per = ((long)M + g.g1 - 1) / g.g1;
+  const int lo = (int)min((long)M, blockIdx.x * per), hi = (int)min((long)M, lo + per);
+  for (int j = lo + threadIdx.x; j < hi; j += blockDim.x) {
+    long f[kMaxCam];
+    cand_points(b, p, j, f);
+#pragma unroll
+    for (int k = 0; k < kMaxCam; ++k) {
+      if (f[k] < 0) continue;
+      const int q = atomicAdd(&cur[f[k] >> g.low], 1);
+      b.pt_sorted[q] = (int32_t)f[k];
+      b.pt_occ[q] = (j << 3) | k;
+    }
   }
 }
 
-__global__ void k_pt_fill(StereoBuffers bb, StereoParams p) {
+// pass 2: one workgroup per bucket
+__global__ void __launch_bounds__(256) k_grp_sort(StereoBuffers bb, GrpShape g) {
   const StereoBuffers b = bb.frame(blockIdx.z);
-  const int M = min(*b.n_cand, b.cap_cand);
-  const int nc = b.n_cam;
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
-    const double e = b.cand_e_s[j];
-    long f[kMaxCam];
-    cand_points(b, p, j, f);
-    int q[kMaxCam];
+  const int x = blockIdx.x;
+  const long nflat = (long)b.n_cam * b.cap;
+  const int s0 = b.grp_off[(long)x * g.g1];
+  const int s1 = x + 1 < g.nb ? b.grp_off[(long)(x + 1) * g.g1] : *b.n_entries;
+  const int P = 1 << g.low;
+  __shared__ int cnt[1 << kGrpMaxLow], cur[1 << kGrpMaxLow];
+  __shared__ int wsum[256 / kWave];
+  for (int i = threadIdx.x; i < P; i += blockDim.x) cnt[i] = 0;
+  __syncthreads();
+  for (int q = s0 + threadIdx.x; q < s1; q += blockDim.x)
+    atomicAdd(&cnt[b.pt_sorted[q] & (P - 1)], 1);
+  __syncthreads();
+  // exclusive scan of cnt: each thread a contiguous run, then the runs' sums
+  const int per = P / (int)blockDim.x > 0 ? P / (int)blockDim.x : 1;
+  const int i0 = threadIdx.x * per;
+  int run = 0;
+  for (int i = i0; i < i0 + per && i < P; ++i) run += cnt[i];
+  int incl = run;
 #pragma unroll
-    for (int k = 0; k < kMaxCam; ++k)
-      q[k] = f[k] >= 0 ? b.pt_start[f[k]] + b.flat_rank[(long)j * nc + k] : -1;
-#pragma unroll
-    for (int k = 0; k < kMaxCam; ++k) {
-      if (q[k] < 0) continue;
-      b.pt_list[q[k]] = (j << 3) | k;
-      b.pt_of[q[k]] = (int32_t)f[k];
-      b.e_ent[q[k]] = e;
-      b.ent_q[(long)j * nc + k] = q[k];
-    }
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int o = __shfl_up(incl, off, kWave);
+    if (lane_id() >= off) incl += o;
   }
+  if (lane_id() == kWave - 1) wsum[threadIdx.x / kWave] = incl;
+  __syncthreads();
+  int wbase = 0;
+  for (int w = 0; w < (int)threadIdx.x / kWave; ++w) wbase += wsum[w];
+  int acc = s0 + wbase + incl - run;
+  for (int i = i0; i < i0 + per && i < P; ++i) {
+    const long f = (long)x * P + i;
+    cur[i] = acc;
+    if (f < nflat) {
+      b.pt_start[f] = acc;
+      b.pt_count[f] = cnt[i];
+    }
+    acc += cnt[i];
+  }
+  __syncthreads();
+  for (int q = s0 + threadIdx.x; q < s1; q += blockDim.x) {
+    const int f = b.pt_sorted[q];
+    const int32_t v = b.pt_occ[q];
+    const int d = atomicAdd(&cur[f & (P - 1)], 1);
+    b.pt_list[d] = v;
+    b.pt_of[d] = f;
+    b.e_ent[d] = b.cand_e_s[ent_cand(v)];
+  }
+}
+
+// each list entry's percentile score, in list order (after k_pct_final)
+__global__ void k_pct_ent(StereoBuffers bb) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
+  const int ne = *b.n_entries;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ne; q += gridDim.x * blockDim.x)
+    b.p_ent[q] = b.pct[ent_cand(b.pt_list[q])];
 }
 
 __global__ void k_zero_entries(StereoBuffers bb) {
@@ -1175,7 +1250,7 @@ __global__ void k_pct_fallback(StereoBuffers bb) {
     if (!(b.pt_flag[f] & 1)) continue;
     const int n = b.pt_count[f];
     // sorted in a copy (pt_sorted is scratch until k_prio_rank fills it):
-    // pt_list keeps the order e_ent and ent_q refer to
+    // pt_list keeps the order e_ent refers to
     int32_t *lst = b.pt_sorted + b.pt_start[f];
     for (int i = 0; i < n; ++i) lst[i] = b.pt_list[b.pt_start[f] + i];
     for (int i = 1; i < n; ++i) {
@@ -1222,7 +1297,6 @@ __global__ void k_pct_final(StereoBuffers bb, StereoParams p) {
     const double pj = nf > 0 ? s / (double)nf : INFINITY;
     b.pct[j] = pj;
     b.state[j] = 0;
-    for (int k = 0; k < nf; ++k) b.p_ent[b.ent_q[(long)j * nc + k]] = pj;
   }
 }
 
@@ -1743,6 +1817,15 @@ static void build_level(const StereoBuffers &b, const StereoParams &p, hipStream
   k_b_extend<K><<<dim3(b.grid_build, 1, b.nf), 256, 0, st>>>(b, p);
 }
 
+GrpShape grp_shape(const StereoBuffers &b) {
+  const long nflat = (long)b.n_cam * b.cap;
+  GrpShape g;
+  g.low = grp_low(nflat);
+  g.nb = (int)grp_buckets(nflat);
+  g.g1 = kGrpBlocks;
+  return g;
+}
+
 // pruneMatch on the sorted candidates (cand_ids_s, cand_e_s, n_cand):
 // per-point percentile and priority orders, the eq_eps check, the greedy
 // rounds and the replacement check; state[j] = 1 marks the selection.
@@ -1750,17 +1833,20 @@ static void prune_section(const StereoBuffers &b, const StereoParams &p, int gM,
                           hipStream_t st) {
   const int T = 256;
   const long nflat = (long)b.n_cam * b.cap;
-  zero_i32(b, b.pt_count, nflat, st);
   zero_i32(b, b.pt_flag, nflat, st);
   zero_i32(b, b.head, nflat, st);
-  k_flat<<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p);
-  scan_exclusive(b.pt_count, b.pt_start, nflat, 1, nflat, b.n_entries, b.scan_scratch, st, b.nf, b.fstride);
-  k_pt_fill<<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p);
+  const GrpShape g = grp_shape(b);
+  k_grp_count<<<dim3(g.g1, 1, b.nf), T, 0, st>>>(b, p, g);
+  scan_exclusive(b.grp_cnt, b.grp_off, (long)g.nb * g.g1, 1, (long)g.nb * g.g1, b.n_entries,
+                 b.scan_scratch, st, b.nf, b.fstride);
+  k_grp_place<<<dim3(g.g1, 1, b.nf), T, 0, st>>>(b, p, g);
+  k_grp_sort<<<dim3(g.nb, 1, b.nf), T, 0, st>>>(b, g);
   const int gE = grid_for((long)b.cap_cand * 4, T, 4096);
   k_zero_entries<<<dim3(gE, 1, b.nf), T, 0, st>>>(b);
   k_pct_rank<<<dim3(gE, 1, b.nf), T, 0, st>>>(b);
   k_pct_fallback<<<dim3(grid_for(nflat, T, 2048), 1, b.nf), T, 0, st>>>(b);
   k_pct_final<<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p);
+  k_pct_ent<<<dim3(gE, 1, b.nf), T, 0, st>>>(b);
   k_hz_insert<<<dim3(gM, 1, b.nf), T, 0, st>>>(b);
   k_hz_check<<<dim3(gM, 1, b.nf), T, 0, st>>>(b);
   k_hz_clear<<<dim3(gM, 1, b.nf), T, 0, st>>>(b);
