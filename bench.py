@@ -466,8 +466,9 @@ def main() -> int:
                  "N x C x P^2 x s patch gathers (P = 8, s = 4 bytes) + 48 bytes per object, "
                  "summed over the launch's frames"),
                 ("enum2", "k_b_enum<2>",
-                 "per level-1 survivor its 12-byte record, plus the line table once (each 2D "
-                 "point's 32-byte lines on the other cameras), summed over the launch's frames")):
+                 "per level-1 survivor its 16-byte path record read and per level-2 item its "
+                 "16-byte record written, plus the line table once (each 2D point's 32-byte "
+                 "lines on the other cameras), summed over the launch's frames")):
             if not (times.get(stage, (0, 0))[1] and stats.get(stage, (0, 0))[1]):
                 continue
             ms_dom, calls = times[stage]

@@ -831,7 +831,7 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   A(b.keep_off, int32_t, M);
   A(b.tri_xyz, double, (long)M * 3);
   b.hz_size = stereo_hz_size(M);
-  A(b.hz_tab, unsigned long long, b.hz_size * kHzWords);
+  A(b.hz_tab, unsigned long long, b.hz_size);
   A(b.hz_slot, int32_t, 2L * M);
   b.cap_obj = O;
   A(b.obj_xyz, double, (long)O * 3);

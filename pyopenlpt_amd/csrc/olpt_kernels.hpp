@@ -78,9 +78,6 @@ inline long peak_chunks(int H, int W) {
 }
 
 // ---------------- stereo matching (StereoMatch) ---------------------------
-// A partial match that survived one build level: its point, the camera it
-// lies on, the camera to search next and the partial match it extends (the
-// reference observation for level 1).
 // A partial match of the stereo build with the point it is being extended
 // by: the points and cameras of path slots 0..K (slot 0 the reference
 // observation, slot K the point under test).
@@ -109,8 +106,7 @@ inline long grp_buckets(long nflat) {
   const int low = grp_low(nflat);
   return (nflat + (1L << low) - 1) >> low;
 }
-constexpr int kHzVals = 4;   // distinct values kept per eq_eps check bucket
-constexpr int kHzWords = 8;  // 64-bit words per table slot (key, values, padding)
+constexpr int kHzWindow = 12;  // distinct values an eq_eps check window may hold
 
 struct StereoBuffers {
   int n_cam;
@@ -184,11 +180,10 @@ struct StereoBuffers {
   int32_t *obj_ids;      // [cap_obj][n_cam]
   int32_t *n_obj;        // device
   int32_t *n_sel;        // device
-  // eq_eps class check of the prune keys (k_hz_*): an open-addressing table
-  // of value buckets, one 64-byte slot each: word 0 the bucket key (-1 =
-  // empty), words 1..kHzVals up to kHzVals distinct values (bits, all ones =
-  // empty)
-  unsigned long long *hz_tab; // [hz_size][kHzWords]
+  // eq_eps class check of the prune keys (k_hz_*): a linear-probing table
+  // of distinct values, one 64-bit word each (the value's bits with its
+  // type in bit 63; all ones = empty), home slot = bucket key mod size
+  unsigned long long *hz_tab; // [hz_size]
   int32_t *hz_slot;           // [2 * cap_cand] slot of each inserted value
   long hz_size;               // power of two
   int32_t *flags;        // [8]: 0 cand overflow, 1 list overflow, 2 eq_eps
@@ -286,10 +281,12 @@ void launch_prune(const StereoBuffers &b, const StereoParams &p, int max_rounds,
 // Empty the eq_eps check table (once, after allocation; every stereo call
 // leaves it empty again).
 void stereo_init_tables(const StereoBuffers &b, hipStream_t st);
-// Number of slots of the eq_eps check table for a candidate capacity.
-inline long stereo_hz_size(long cap_cand) {
+// Number of slots of the eq_eps check table for n candidates: two values
+// each, at most a quarter of the slots in use.  The table is allocated for
+// the candidate capacity; each call uses the part its candidate count needs.
+__host__ __device__ inline long stereo_hz_size(long n_cand) {
   long n = 1024;
-  while (n < 2 * cap_cand) n <<= 1;
+  while (n < 8 * n_cand) n <<= 1;
   return n;
 }
 

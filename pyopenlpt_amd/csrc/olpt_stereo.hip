@@ -24,7 +24,7 @@
 //                 (:153-156): counting sort on the first active camera's id,
 //                 then each candidate's place inside its (small) bucket by
 //                 counting the members that precede it.
-//   k_flat / k_pct_* / k_prio_* / k_mis_round / k_replace_check / k_tri
+//   k_grp_* / k_pct_* / k_prio_* / k_mis_round / k_replace_check / k_tri
 //                 pruneMatch (:631-980) + triangulateMatch (:985-1050).
 //                 Per-point orders (by e_check for the percentiles, by the
 //                 greedy's priority for the packing) are rank counts, one
@@ -1548,19 +1548,25 @@ __global__ void k_replace_check(StereoBuffers bb, StereoParams p) {
 // Without a global sort: g(v) = log(v + 1e-3) grows by at most ~1e-9
 // between eq_eps values (v - u <= 1e-9 (v + 1e-3)), so buckets of width 2e-9
 // in g put every eq_eps pair in the same or adjacent buckets and never
-// relate values two buckets apart.  A bucket keeps up to kHzVals distinct
-// values; each value's thread sorts the values of its bucket and the next
-// two and checks every run in that window (a run reaching beyond it spans
-// more than two buckets, so its visible part already fails).  Each distinct
-// value is checked once, by the thread that stored it.  A fuller bucket or
-// a full table sets the flag without a claim either way.  The
-// values checked are the finite percentile scores (type 0) and their
-// candidates' e_check (type 1): the candidates the greedy considers; the
-// others sort after them whatever their order.
+// relate values two buckets apart.  Each distinct value is stored once in a
+// linear-probing table (k_hz_insert) whose home slot for bucket k is k mod
+// the table size, unhashed, so that buckets k, k + 1, k + 2 are neighbouring
+// slots and every value of them lies between home(k) and the first empty
+// slot at or after home(k + 2) (a value sits after its home with no empty
+// slot between).  The thread that stored a value scans that range, sorts the
+// values of the three buckets in registers and checks every run in that
+// window (a run reaching beyond it spans more than two buckets, so its
+// visible part already fails).  More than kHzWindow values in a window, or a
+// scan or probe longer than its bound, set the flag without a claim either
+// way.  The values checked are the finite percentile scores (type 0) and
+// their candidates' e_check (type 1): the candidates the greedy considers;
+// the others sort after them whatever their order.
 // --------------------------------------------------------------------------
 constexpr double kHzWidth = 2e-9;
-constexpr unsigned long long kHzEmpty = ~0ULL;   // key -1: no bucket
-constexpr unsigned long long kHzNoVal = ~0ULL;   // a NaN pattern: never a value
+constexpr unsigned long long kHzEmpty = ~0ULL;   // no tagged finite value has these bits
+constexpr unsigned long long kHzType = 1ULL << 63;
+constexpr int kHzMaxProbe = 256;   // slots an insert may probe
+constexpr int kHzMaxScan = 64;     // slots a window scan may visit
 
 __device__ __forceinline__ long long hz_key(int type, double v) {
   const double g = log(v + 1e-3);
@@ -1568,52 +1574,29 @@ __device__ __forceinline__ long long hz_key(int type, double v) {
   return ((long long)type << 42) + k + (1LL << 41);
 }
 
-__device__ __forceinline__ unsigned long long hz_hash(long long key) {
-  unsigned long long h = (unsigned long long)key * 0x9E3779B97F4A7C15ULL;
-  return h ^ (h >> 29);
+// a value >= +0 with its type in the (free) sign bit
+__device__ __forceinline__ unsigned long long hz_tag(int type, double v) {
+  return (unsigned long long)__double_as_longlong(v) | (type ? kHzType : 0ULL);
+}
+__device__ __forceinline__ int hz_tag_type(unsigned long long w) { return (int)(w >> 63); }
+__device__ __forceinline__ double hz_tag_val(unsigned long long w) {
+  return __longlong_as_double((long long)(w & ~kHzType));
 }
 
-// word w of table slot s (word 0 the key, 1..kHzVals the values)
-__device__ __forceinline__ unsigned long long *hz_word(const StereoBuffers &b, long s, int w) {
-  return b.hz_tab + s * kHzWords + w;
-}
-
-// slot of key, or -1 when absent (find) / when the table is full (insert)
-__device__ __forceinline__ long hz_find(const StereoBuffers &b, long long key) {
-  const long mask = b.hz_size - 1;
-  long s = (long)(hz_hash(key) & (unsigned long long)mask);
-  for (long t = 0; t < b.hz_size; ++t) {
-    const unsigned long long k = __atomic_load_n(hz_word(b, s, 0), __ATOMIC_RELAXED);
-    if (k == (unsigned long long)key) return s;
-    if (k == kHzEmpty) return -1;
-    s = (s + 1) & mask;
-  }
-  return -1;
-}
-
-__device__ __forceinline__ long hz_insert(const StereoBuffers &b, long long key) {
-  const long mask = b.hz_size - 1;
-  long s = (long)(hz_hash(key) & (unsigned long long)mask);
-  for (long t = 0; t < b.hz_size; ++t) {
-    // a plain read first: most keys are already there (repeated values hit
-    // the same few slots, where atomics would serialise)
-    unsigned long long k = __atomic_load_n(hz_word(b, s, 0), __ATOMIC_RELAXED);
-    if (k == kHzEmpty) k = atomicCAS(hz_word(b, s, 0), kHzEmpty, (unsigned long long)key);
-    if (k == kHzEmpty || k == (unsigned long long)key) return s;
-    s = (s + 1) & mask;
-  }
-  return -1;
-}
-
-__device__ __forceinline__ double hz_val(unsigned long long bits) {
-  return __longlong_as_double((long long)bits);
+// slot mask of this call's part of the table (its candidate count's size)
+__device__ __forceinline__ long hz_mask(const StereoBuffers &b, int M) {
+  const long n = stereo_hz_size(M);
+  return (n < b.hz_size ? n : b.hz_size) - 1;
 }
 
 // One thread per value (i = 2 j + type): the two values of a candidate
-// insert independently.
+// insert independently.  The thread whose store makes a value present owns
+// it: it alone checks and clears it (repeated values - most percentile
+// scores - find their bits already there).
 __global__ void k_hz_insert(StereoBuffers bb) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
+  const long mask = hz_mask(b, M);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * M; i += gridDim.x * blockDim.x) {
     const int j = i >> 1, t = i & 1;
     b.hz_slot[i] = -1;
@@ -1625,50 +1608,69 @@ __global__ void k_hz_insert(StereoBuffers bb) {
       atomicOr(&b.flags[2], 1);
       continue;
     }
-    const long s = hz_insert(b, hz_key(t, v));
-    if (s < 0) {
-      atomicOr(&b.flags[2], 1);   // table full
-      continue;
-    }
-    const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
-    // the thread that stores a value owns it: it alone checks and clears
-    // the bucket for it (repeated values - most percentile scores - do not)
+    const unsigned long long tag = hz_tag(t, v);
+    long s = (long)((unsigned long long)hz_key(t, v) & (unsigned long long)mask);
     bool placed = false, owner = false;
-    for (int u = 1; u <= kHzVals && !placed; ++u) {
-      unsigned long long o = __atomic_load_n(hz_word(b, s, u), __ATOMIC_RELAXED);
-      if (o == kHzNoVal) o = atomicCAS(hz_word(b, s, u), kHzNoVal, bits);
-      owner = o == kHzNoVal;
-      placed = owner || o == bits;
+    for (int n = 0; n < kHzMaxProbe && !placed; ++n) {
+      // a plain read first: repeated values meet their bits without an atomic
+      unsigned long long o = __atomic_load_n(b.hz_tab + s, __ATOMIC_RELAXED);
+      if (o == kHzEmpty) o = atomicCAS(b.hz_tab + s, kHzEmpty, tag);
+      owner = o == kHzEmpty;
+      placed = owner || o == tag;
+      if (!placed) s = (s + 1) & mask;
     }
-    if (!placed) atomicOr(&b.flags[2], 1);   // bucket full
+    if (!placed) atomicOr(&b.flags[2], 1);   // probe bound
     if (owner) b.hz_slot[i] = (int32_t)s;
   }
 }
 
-// the kHzVals value words of slot s into w (absent values as +inf)
-__device__ __forceinline__ void hz_take(const StereoBuffers &b, long s, double *w) {
-#pragma unroll
-  for (int u = 0; u < kHzVals; ++u) {
-    const unsigned long long bits = s >= 0 ? *hz_word(b, s, 1 + u) : kHzNoVal;
-    w[u] = bits == kHzNoVal ? INFINITY : hz_val(bits);
-  }
-}
+constexpr int kHzChunk = 4;   // slots a window scan loads per step
 
 __global__ void k_hz_check(StereoBuffers bb) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
-  constexpr int NW = 3 * kHzVals;
+  const long mask = hz_mask(b, M);
+  constexpr int NW = kHzWindow;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * M; i += gridDim.x * blockDim.x) {
     const long s = b.hz_slot[i];
     if (s < 0) continue;
-    const long long key = (long long)*hz_word(b, s, 0);
+    const unsigned long long tag = b.hz_tab[s];
+    const int type = hz_tag_type(tag);
+    const long long key = hz_key(type, hz_tag_val(tag));
+    const long home = (long)((unsigned long long)key & (unsigned long long)mask);
     // the distinct values of buckets key, key + 1, key + 2 (absent ones as
-    // +inf), sorted by a transposition network in registers
+    // +inf), gathered from home(key) to the first empty slot at or after
+    // home(key + 2)
     double w[NW];
-    const long s1 = hz_find(b, key + 1), s2 = hz_find(b, key + 2);
-    hz_take(b, s, w);
-    hz_take(b, s1, w + kHzVals);
-    hz_take(b, s2, w + 2 * kHzVals);
+#pragma unroll
+    for (int u = 0; u < NW; ++u) w[u] = INFINITY;
+    int n = 0;
+    bool bad = false, done = false;
+    for (int d0 = 0; d0 < kHzMaxScan && !done; d0 += kHzChunk) {
+      unsigned long long o[kHzChunk];
+#pragma unroll
+      for (int c = 0; c < kHzChunk; ++c) o[c] = b.hz_tab[(home + d0 + c) & mask];
+#pragma unroll
+      for (int c = 0; c < kHzChunk; ++c) {
+        const int d = d0 + c;
+        if (done) continue;
+        if (o[c] == kHzEmpty) {
+          done = d >= 2;
+          continue;
+        }
+        if (hz_tag_type(o[c]) != type) continue;
+        const double u = hz_tag_val(o[c]);
+        const long long ku = hz_key(type, u);
+        if (ku < key || ku > key + 2) continue;
+        bad = bad || n == NW;     // window full
+#pragma unroll
+        for (int q = 0; q < NW; ++q)
+          if (q == n) w[q] = u;
+        n += n < NW;
+      }
+    }
+    bad = bad || !done;           // scan bound
+    // sorted by a transposition network in registers
 #pragma unroll
     for (int r = 0; r < NW; ++r)
 #pragma unroll
@@ -1678,7 +1680,6 @@ __global__ void k_hz_check(StereoBuffers bb) {
         w[k + 1] = hi;
       }
     // every run of eq_eps-adjacent values must have eq_eps end points
-    bool bad = false;
     double r0 = w[0];
 #pragma unroll
     for (int k = 1; k <= NW; ++k) {
@@ -1699,8 +1700,7 @@ __global__ void k_hz_clear(StereoBuffers bb) {
   const int M = min(*b.n_cand, b.cap_cand);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * M; i += gridDim.x * blockDim.x) {
     const long s = b.hz_slot[i];
-    if (s < 0) continue;
-    for (int u = 0; u <= kHzVals; ++u) *hz_word(b, s, u) = kHzEmpty;
+    if (s >= 0) b.hz_tab[s] = kHzEmpty;
   }
 }
 
@@ -1708,7 +1708,7 @@ __global__ void k_hz_init(StereoBuffers bb) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   for (long s = blockIdx.x * (long)blockDim.x + threadIdx.x; s < b.hz_size;
        s += (long)gridDim.x * blockDim.x)
-    for (int u = 0; u < kHzWords; ++u) *hz_word(b, s, u) = kHzEmpty;
+    b.hz_tab[s] = kHzEmpty;
 }
 
 void stereo_init_tables(const StereoBuffers &b, hipStream_t st) {
