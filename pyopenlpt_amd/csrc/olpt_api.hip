@@ -816,6 +816,7 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   A(b.ppos, int32_t, (long)M * nc);
   A(b.pt_flag, int32_t, (long)nc * cap);
   A(b.head, int32_t, (long)nc * cap);
+  A(b.own, unsigned long long, (long)nc * cap);
   A(b.n_entries, int32_t, 1);
   const long grp_n = grp_buckets((long)nc * cap) * (long)kGrpBlocks;
   A(b.grp_cnt, int32_t, grp_n);

@@ -156,6 +156,9 @@ struct StereoBuffers {
   int32_t *ppos;         // [cap_cand][n_cam] position in the point's priority list
   int32_t *pt_flag;      // [n_cam*cap] bit0/bit1: sequential percentile/priority
   int32_t *head;         // [n_cam*cap] first entry not known to be rejected
+  // per flat point its selected candidate, if any: (position + 1) << 32 |
+  // candidate, 0 = none (written by the greedy as it selects)
+  unsigned long long *own; // [n_cam*cap]
   int32_t *n_entries;    // device: total list entries
   int32_t *grp_cnt;      // [grp buckets * kGrpBlocks] point-list grouping counts
   int32_t *grp_off;      // [grp buckets * kGrpBlocks] their exclusive scan
@@ -237,6 +240,7 @@ struct StereoBuffers {
     o.ppos = fshift(ppos, d);
     o.pt_flag = fshift(pt_flag, d);
     o.head = fshift(head, d);
+    o.own = fshift(own, d);
     o.n_entries = fshift(n_entries, d);
     o.grp_cnt = fshift(grp_cnt, d);
     o.grp_off = fshift(grp_off, d);

@@ -1426,7 +1426,11 @@ __device__ __forceinline__ int mis_decide(const StereoBuffers &b, const StereoPa
     b.state[j] = (int8_t)d;
 #pragma unroll
     for (int k = 0; k < kMaxCam; ++k)
-      if (f[k] >= 0) __atomic_store_n(&b.st_ent[s0[k] + pos[k]], d, __ATOMIC_RELAXED);
+      if (f[k] >= 0) {
+        __atomic_store_n(&b.st_ent[s0[k] + pos[k]], d, __ATOMIC_RELAXED);
+        // a point has at most one selected candidate
+        if (d == 1) b.own[f[k]] = ((unsigned long long)(pos[k] + 1) << 32) | (unsigned)j;
+      }
   }
   return d;
 }
@@ -1487,35 +1491,43 @@ k_mis_finish(StereoBuffers bb, StereoParams p) {
 
 // Replacement branch of the reference (StereoMatch.cpp:906-968), evaluated
 // for each rejected candidate against the owners it met at its turn: the
-// selected entries that precede it on its points (at most one per point).
+// selected candidates ahead of it on its points (at most one per point, the
+// point's own[] record).
 __global__ void k_replace_check(StereoBuffers bb, StereoParams p) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
   const int nc = b.n_cam;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
     if (b.state[j] != 2) continue;
-    int S[kMaxCam], ns = 0;
-    int k = 0;
-    for (int c = 0; c < nc; ++c) {
-      if (!p.active[c]) continue;
-      const int pid = b.cand_ids_s[(long)j * nc + c];
-      if (pid < 0) continue;
-      const long f = (long)c * b.cap + pid;
-      const int pos = b.ppos[(long)j * nc + k];
-      ++k;
-      const int s0 = b.pt_start[f];
-      for (int h = b.head[f]; h < pos; ++h) {
-        if (b.st_ent[s0 + h] != 1) continue;
-        const int m = ent_cand(b.pt_sorted[s0 + h]);
-        bool seen = false;
-        for (int q = 0; q < ns; ++q) seen = seen || S[q] == m;
-        if (!seen) S[ns++] = m;
-        break;
-      }
+    long f[kMaxCam];
+    cand_points(b, p, j, f);
+    // every point's position and owner record loaded together
+    int pos[kMaxCam];
+    unsigned long long o[kMaxCam];
+#pragma unroll
+    for (int k = 0; k < kMaxCam; ++k) {
+      pos[k] = f[k] >= 0 ? b.ppos[(long)j * nc + k] : 0;
+      o[k] = f[k] >= 0 ? b.own[f[k]] : 0ULL;
+    }
+    int S[kMaxCam] = {}, ns = 0;
+#pragma unroll
+    for (int k = 0; k < kMaxCam; ++k) {
+      if (o[k] == 0ULL || (int)(o[k] >> 32) - 1 >= pos[k]) continue;
+      const int m = (int)(o[k] & 0xffffffffULL);
+      bool seen = false;
+#pragma unroll
+      for (int q = 0; q < k; ++q) seen = seen || (q < ns && S[q] == m);
+      if (seen) continue;
+#pragma unroll
+      for (int q = 0; q <= k; ++q)
+        if (q == ns) S[q] = m;
+      ++ns;
     }
     double ap = 0.0, ae = 0.0;
     int cnt = 0;
-    for (int q = 0; q < ns; ++q) {
+#pragma unroll
+    for (int q = 0; q < kMaxCam; ++q) {
+      if (q >= ns) continue;
       const double po = b.pct[S[q]], eo = b.cand_e_s[S[q]];
       if (!isfinite(po) || !isfinite(eo)) continue;
       ap += po;
@@ -1835,6 +1847,7 @@ static void prune_section(const StereoBuffers &b, const StereoParams &p, int gM,
   const long nflat = (long)b.n_cam * b.cap;
   zero_i32(b, b.pt_flag, nflat, st);
   zero_i32(b, b.head, nflat, st);
+  zero_i32(b, (int32_t *)b.own, 2 * nflat, st);
   const GrpShape g = grp_shape(b);
   k_grp_count<<<dim3(g.g1, 1, b.nf), T, 0, st>>>(b, p, g);
   scan_exclusive(b.grp_cnt, b.grp_off, (long)g.nb * g.g1, 1, (long)g.nb * g.g1, b.n_entries,
