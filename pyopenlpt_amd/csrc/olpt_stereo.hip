@@ -1168,6 +1168,7 @@ __global__ void __launch_bounds__(256) k_grp_sort(StereoBuffers bb, GrpShape g) 
     if (f < nflat) {
       b.pt_start[f] = acc;
       b.pt_count[f] = cnt[i];
+      b.head[f] = acc;
     }
     acc += cnt[i];
   }
@@ -1330,7 +1331,7 @@ __global__ void k_prio_rank(StereoBuffers bb) {
       for (int u = 0; u < kRankBatch; ++u)
         r += m[u] != j && less_order_v(pm[u], em[u], m[u], pj, ej, j) ? 1 : 0;
     }
-    b.ppos[(long)j * b.n_cam + ent_slot(v)] = r;
+    b.ppos[(long)j * b.n_cam + ent_slot(v)] = s0 + r;
     b.pt_sorted[s0 + r] = v;
     if (atomicAdd(&b.pt_occ[s0 + r], 1) != 0) atomicOr(&b.pt_flag[f], 2);
   }
@@ -1356,7 +1357,7 @@ __global__ void k_prio_fallback(StereoBuffers bb) {
       lst[k + 1] = cur;
     }
     for (int i = 0; i < n; ++i)
-      b.ppos[(long)ent_cand(lst[i]) * b.n_cam + ent_slot(lst[i])] = i;
+      b.ppos[(long)ent_cand(lst[i]) * b.n_cam + ent_slot(lst[i])] = s0 + i;
   }
 }
 
@@ -1376,16 +1377,15 @@ __device__ __forceinline__ int mis_decide(const StereoBuffers &b, const StereoPa
   const int nc = b.n_cam;
   long f[kMaxCam];
   cand_points(b, p, j, f);
-  // every point's position, list start and head loaded together
-  int pos[kMaxCam], s0[kMaxCam], h0[kMaxCam];
+  // every point's position and head (both indices into the sorted lists)
+  // loaded together
+  int pos[kMaxCam], h0[kMaxCam];
 #pragma unroll
   for (int k = 0; k < kMaxCam; ++k) {
     pos[k] = 0;
-    s0[k] = 0;
     h0[k] = 0;
     if (f[k] >= 0) {
       pos[k] = b.ppos[(long)j * nc + k];
-      s0[k] = b.pt_start[f[k]];
       h0[k] = __atomic_load_n(&b.head[f[k]], __ATOMIC_RELAXED);
     }
   }
@@ -1402,7 +1402,7 @@ __device__ __forceinline__ int mis_decide(const StereoBuffers &b, const StereoPa
       int sv[kRankBatch];
 #pragma unroll
       for (int u = 0; u < kRankBatch; ++u)
-        sv[u] = h + u < pos[k] ? __atomic_load_n(&b.st_ent[s0[k] + h + u], __ATOMIC_RELAXED) : 2;
+        sv[u] = h + u < pos[k] ? __atomic_load_n(&b.st_ent[h + u], __ATOMIC_RELAXED) : 2;
       int adv = kRankBatch;
 #pragma unroll
       for (int u = kRankBatch - 1; u >= 0; --u)
@@ -1427,7 +1427,7 @@ __device__ __forceinline__ int mis_decide(const StereoBuffers &b, const StereoPa
 #pragma unroll
     for (int k = 0; k < kMaxCam; ++k)
       if (f[k] >= 0) {
-        __atomic_store_n(&b.st_ent[s0[k] + pos[k]], d, __ATOMIC_RELAXED);
+        __atomic_store_n(&b.st_ent[pos[k]], d, __ATOMIC_RELAXED);
         // a point has at most one selected candidate
         if (d == 1) b.own[f[k]] = ((unsigned long long)(pos[k] + 1) << 32) | (unsigned)j;
       }
@@ -1846,7 +1846,6 @@ static void prune_section(const StereoBuffers &b, const StereoParams &p, int gM,
   const int T = 256;
   const long nflat = (long)b.n_cam * b.cap;
   zero_i32(b, b.pt_flag, nflat, st);
-  zero_i32(b, b.head, nflat, st);
   zero_i32(b, (int32_t *)b.own, 2 * nflat, st);
   const GrpShape g = grp_shape(b);
   k_grp_count<<<dim3(g.g1, 1, b.nf), T, 0, st>>>(b, p, g);
