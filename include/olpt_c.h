@@ -104,6 +104,13 @@ const char *olpt_last_error(void);
  * overflow, [5] prune rounds exhausted, [6] checkMatch triangulation
  * failures, [7] peak overflow. */
 olpt_status olpt_diagnostics(olpt_ctx *ctx, int32_t out[8]);
+/* Work-list sizes of the last stereo build (frame 0 of the first
+ * workspace, as the last stereo call or IPR loop left them): [0] level-1
+ * (observation, point) pairs, [1..8] survivors of each build level,
+ * [9..16] items enumerated at each level, [17] candidates, [18] selected
+ * candidates, [19] objects.  No counterpart in the reference (its DFS keeps
+ * no such lists); for profiling the build. */
+olpt_status olpt_build_counts(olpt_ctx *ctx, int32_t out[24]);
 
 /* Stage timing with HIP events on the context stream (eager runs only).
  * Slots: 0 peaks, 1 stereo build, 2 sort+prune+triangulate, 3 shake
@@ -305,6 +312,15 @@ olpt_status olpt_stb_create(olpt_ctx *ctx, const olpt_stb_params *stb,
                             const olpt_ipr_params *ipr, const char *output_path,
                             olpt_stb **out);
 olpt_status olpt_stb_destroy(olpt_stb *stb);
+/* No counterpart in the reference: every IPR call of the tracker (the
+ * initial phase's IPR on the frames, STB.cpp:97, and the convergence
+ * phase's IPR on the residual images, :454-456) runs with the learned shake
+ * proposal mlp (as olpt_set_shake_model) and the IPR parameters ipr (for
+ * instance one plain iteration from a small shake width); the shake of the
+ * predicted particles (:406-418) keeps the configuration's plain schedule.
+ * mlp NULL restores the plain IPR.  The network is not owned. */
+olpt_status olpt_stb_set_ipr_model(olpt_stb *stb, const olpt_mlp *mlp, double voxel_mm,
+                                   double max_step_vox, const olpt_ipr_params *ipr);
 /* STB::processFrame (STB.cpp:40-89): d_imgs [n_cam][n_row][n_col]. */
 olpt_status olpt_stb_process_frame(olpt_stb *stb, int frame_id, const void *d_imgs,
                                    int dtype);

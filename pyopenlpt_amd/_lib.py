@@ -65,6 +65,7 @@ def lib() -> ctypes.CDLL:
     L.olpt_stream.argtypes = [_P]
     L.olpt_stream.restype = _P
     L.olpt_diagnostics.argtypes = [_P, _I32P]
+    L.olpt_build_counts.argtypes = [_P, _I32P]
     L.olpt_find_peaks.argtypes = [_P, _P, _I, _I, _I, ctypes.POINTER(OlptIprParams),
                                   _P, _I, _I32P]
     L.olpt_stereo_match.argtypes = [_P, _I32P, _P, _I, _I32P,
@@ -108,6 +109,7 @@ def lib() -> ctypes.CDLL:
     _IP = ctypes.POINTER(OlptIprParams)
     L.olpt_stb_create.argtypes = [_P, _SP, _IP, ctypes.c_char_p, ctypes.POINTER(_P)]
     L.olpt_stb_destroy.argtypes = [_P]
+    L.olpt_stb_set_ipr_model.argtypes = [_P, _P, ctypes.c_double, ctypes.c_double, _IP]
     L.olpt_stb_process_frame.argtypes = [_P, _I, _P, _I]
     L.olpt_stb_track_count.argtypes = [_P, _I, _I32P, _I32P]
     L.olpt_stb_get_tracks.argtypes = [_P, _I, ctypes.c_int32, _I32P, _I32P, _DP]
@@ -118,7 +120,8 @@ def lib() -> ctypes.CDLL:
     L.olpt_stb_linear_fit.argtypes = [_P, _I, _P, _P, _I, _P]
     L.olpt_pred_field.argtypes = [_P, _SP, _P, _I, _P, _I, _P, _P, _I, _P]
     L.olpt_estimate_otf.argtypes = [_P, _P, _I, _I, _IP, _DP, _I32P]
-    for name in ("olpt_stb_create", "olpt_stb_destroy", "olpt_stb_process_frame",
+    for name in ("olpt_stb_create", "olpt_stb_destroy", "olpt_stb_set_ipr_model",
+                 "olpt_stb_process_frame",
                  "olpt_stb_track_count", "olpt_stb_get_tracks", "olpt_stb_set_tracks",
                  "olpt_stb_save_tracks", "olpt_stb_load_tracks", "olpt_stb_predict",
                  "olpt_stb_linear_fit", "olpt_pred_field", "olpt_estimate_otf"):
@@ -131,7 +134,7 @@ def lib() -> ctypes.CDLL:
                  "olpt_set_precision", "olpt_set_concurrency", "olpt_set_tuning",
                  "olpt_get_tuning", "olpt_set_cameras",
                  "olpt_set_otf",
-                 "olpt_sync", "olpt_diagnostics", "olpt_find_peaks",
+                 "olpt_sync", "olpt_diagnostics", "olpt_build_counts", "olpt_find_peaks",
                  "olpt_stereo_match", "olpt_prune_match", "olpt_find_peaks_static", "olpt_otf_param", "olpt_shake", "olpt_residual",
                  "olpt_ipr_run", "olpt_ipr_graph_capture",
                  "olpt_ipr_graph_launch", "olpt_stage_timing",

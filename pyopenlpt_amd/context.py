@@ -179,6 +179,15 @@ class Context:
 
     STAGES = ("peaks", "stereo_build", "prune", "shake", "residual", "enum2", "host_issue")
 
+    def build_counts(self) -> dict:
+        """Work-list sizes of the last stereo build (frame 0 of the first
+        workspace): pairs, survivors and items per level, candidates,
+        selected candidates, objects."""
+        v = np.zeros(24, np.int32)
+        check(self.L.olpt_build_counts(self.h, v.ctypes.data_as(_lib._I32P)))
+        return dict(pairs=int(v[0]), surv=v[1:9].tolist(), items=v[9:17].tolist(),
+                    cand=int(v[17]), sel=int(v[18]), obj=int(v[19]))
+
     def stage_timing(self, enable: bool = True):
         check(self.L.olpt_stage_timing(self.h, 1 if enable else 0))
 

@@ -1155,6 +1155,21 @@ olpt_status olpt_diagnostics(olpt_ctx *c, int32_t out[8]) {
   return read_flags(c, c->n_ws, c->st, out);
 }
 
+olpt_status olpt_build_counts(olpt_ctx *c, int32_t out[24]) {
+  if (!c || !c->n_ws || !out) return fail(OLPT_E_INVALID_ARGUMENT, "no workspace");
+  HIP_TRY(hipDeviceSynchronize());
+  const StereoBuffers &b = c->ws[0].sb;
+  int32_t v[24] = {0};
+  HIP_TRY(hipMemcpy(&v[0], b.n_pairs, 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(&v[1], b.n_surv, 4 * kMaxCam, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(&v[1 + kMaxCam], b.n_path, 4 * kMaxCam, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(&v[1 + 2 * kMaxCam], b.n_cand, 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(&v[2 + 2 * kMaxCam], b.n_sel, 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(&v[3 + 2 * kMaxCam], b.n_obj, 4, hipMemcpyDeviceToHost));
+  for (int i = 0; i < 24; ++i) out[i] = v[i];
+  return OLPT_OK;
+}
+
 olpt_status olpt_find_peaks(olpt_ctx *c, const void *d_img, int dtype, int H, int W,
                             const olpt_ipr_params *p, double *d_xy, int cap, int32_t *n_out) {
   if (!c || !c->n_ws) return fail(OLPT_E_INVALID_ARGUMENT, "workspace not reserved");

@@ -966,6 +966,10 @@ struct olpt_stb {
       last, nb_pt, nb_disp, nb_len, link, kern, cand_d;
   PfWork pf;
   int ipr_cap = 0;
+  // learned shake proposal for the IPR calls (olpt_stb_set_ipr_model)
+  const olpt_mlp *ipr_mlp = nullptr;
+  double ipr_mlp_vox = 0.0, ipr_mlp_step = 0.0;
+  olpt_ipr_params ip_ipr{};
 };
 
 namespace {
@@ -1046,8 +1050,19 @@ olpt_status run_ipr(olpt_stb *s, const void *d_imgs, int dtype, std::vector<StbP
   if (!s->ipr_cap) s->ipr_cap = std::max(1 << 16, 4 * s->ci.cap_obj);
   STB_CHECK(s->obj_xyz.need(sizeof(double) * 3 * (size_t)s->ipr_cap));
   STB_CHECK(s->obj_n.need(sizeof(int32_t)));
-  STB_CHECK(olpt_ipr_run(s->ctx, d_imgs, dtype, 1, &s->ip, s->obj_xyz.as<double>(), nullptr,
-                         s->ipr_cap, s->obj_n.as<int32_t>(), nullptr, 0));
+  if (s->ipr_mlp) {
+    // the proposal serves this IPR call only: the shake of the predictions
+    // keeps the tracer configuration's plain schedule
+    STB_CHECK(olpt_set_shake_model(s->ctx, s->ipr_mlp, s->ipr_mlp_vox, s->ipr_mlp_step));
+    const olpt_status r = olpt_ipr_run(s->ctx, d_imgs, dtype, 1, &s->ip_ipr,
+                                       s->obj_xyz.as<double>(), nullptr, s->ipr_cap,
+                                       s->obj_n.as<int32_t>(), nullptr, 0);
+    STB_CHECK(olpt_set_shake_model(s->ctx, nullptr, 0.0, 0.0));
+    STB_CHECK(r);
+  } else {
+    STB_CHECK(olpt_ipr_run(s->ctx, d_imgs, dtype, 1, &s->ip, s->obj_xyz.as<double>(), nullptr,
+                           s->ipr_cap, s->obj_n.as<int32_t>(), nullptr, 0));
+  }
   int32_t n = 0;
   STB_TRY(hipMemcpyAsync(&n, s->obj_n.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   STB_TRY(hipStreamSynchronize(st));
@@ -1415,6 +1430,20 @@ olpt_status olpt_stb_create(olpt_ctx *ctx, const olpt_stb_params *sp, const olpt
     }
   }
   *out = s;
+  return OLPT_OK;
+}
+
+olpt_status olpt_stb_set_ipr_model(olpt_stb *s, const olpt_mlp *mlp, double voxel_mm,
+                                   double max_step_vox, const olpt_ipr_params *ipr) {
+  if (!s) return set_error(OLPT_E_INVALID_ARGUMENT, "null argument");
+  if (mlp && !ipr) return set_error(OLPT_E_INVALID_ARGUMENT, "null IPR parameters");
+  if (mlp && s->ci.n_cam != 4)
+    return set_error(OLPT_E_UNSUPPORTED, "the learned shake proposal takes exactly 4 cameras");
+  if (mlp && !(voxel_mm > 0)) return set_error(OLPT_E_INVALID_ARGUMENT, "voxel_mm must be positive");
+  s->ipr_mlp = mlp;
+  s->ipr_mlp_vox = voxel_mm;
+  s->ipr_mlp_step = max_step_vox;
+  if (mlp) s->ip_ipr = *ipr;
   return OLPT_OK;
 }
 

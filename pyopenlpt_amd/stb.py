@@ -64,6 +64,25 @@ class STB:
                                          out, ctypes.byref(h)))
         self.h = h
 
+    def set_ipr_model(self, net, voxel_mm: float = 0.0, max_step_vox: float = 0.6,
+                      n_shake_loop: int = 1, shake_width_vox: float = 0.03):
+        """No counterpart in the reference: run every IPR call of the
+        tracker with the learned shake proposal `net` (a learned.ShakeNet)
+        followed by n_shake_loop plain iterations from shake_width_vox; the
+        shake of the predicted particles keeps the configuration's plain
+        schedule.  net None restores the plain IPR."""
+        if net is None:
+            self._ipr_net = None
+            check(_lib.lib().olpt_stb_set_ipr_model(self.h, None, 0.0, 0.0, None))
+            return
+        vox = voxel_mm or self.setting.voxel_to_mm
+        ip = S.OlptIprParams.from_buffer_copy(self._ip)
+        ip.n_shake_loop = int(n_shake_loop)
+        ip.shake_width = float(shake_width_vox) * vox
+        self._ipr_net, self._ipr_params = net, ip
+        check(_lib.lib().olpt_stb_set_ipr_model(self.h, net.h, float(vox), float(max_step_vox),
+                                                ctypes.byref(ip)))
+
     def close(self):
         if getattr(self, "h", None):
             _lib.lib().olpt_stb_destroy(self.h)

@@ -274,3 +274,64 @@ def test_tracks_save_load_resume(tmp_path, gpu_available, gold):
     assert left_after - left_a == b.rows(2)["n_tracks"] + b.rows(3)["n_tracks"]
     a.close()
     b.close()
+
+
+def test_stb_learned_ipr_hook(tmp_path, gpu_available, gold):
+    """olpt_stb_set_ipr_model: the tracker's IPR calls run with the learned
+    proposal and the given IPR parameters.  Frames 0-3 (the initial phase:
+    IPR per frame, then tracks from its objects through the predictive
+    field) with an untrained network: every frame-0 point of the tracks is
+    an object of olpt_ipr_run with the same network and parameters on the
+    tracker's own context (same OTF estimate), and not every one is an
+    object of the plain IPR; the model is not left on the context."""
+    import torch
+    from scipy.spatial import cKDTree
+    from pyopenlpt_amd import config as C
+    from pyopenlpt_amd import synth as S
+    from pyopenlpt_amd.context import to_device_image
+    from pyopenlpt_amd.learned import ShakeNet
+    from pyopenlpt_amd.stb import STB
+    cams = [Case._cam(r) for r in gold["cameras"]]
+    tr = ast.literal_eval(str(gold["tracer"]))
+    vox = float(gold["voxel_mm"])
+    cfg = C.write_run_folder(str(tmp_path), cams, gold["frames"],
+                             axis_limit=(-20, 20, -20, 20, -20, 20), voxel_to_mm=vox, tracer=tr)
+    setting = C.BasicSetting()
+    assert setting.readConfig(cfg)
+    ios = [C.ImageIO("", p) for p in setting.image_file_paths]
+    imgs = [np.stack([io.loadImg(f) for io in ios]) for f in range(4)]
+    net = ShakeNet(max_batch=4096, seed=3)
+    stb = STB(setting, "Tracer", setting.object_config_paths[0], precision="f32",
+              write_files=False)
+    stb.set_ipr_model(net, vox, 0.6, n_shake_loop=1, shake_width_vox=0.03)
+    for f in range(4):
+        stb.process_frame(f, imgs[f])
+    pts = []
+    for which in range(4):
+        r = stb.rows(which)
+        pts.append(r["xyz"][r["frame"] == 0])
+    pts = np.concatenate(pts)
+    assert len(pts) > 100
+
+    d, code = to_device_image(imgs[0])
+    cap = 1 << 16
+
+    def ipr(params):
+        (ox, _u, n), _lt = stb.ctx.ipr_run(d, code, params, cap_obj=cap)
+        torch.cuda.synchronize()
+        return ox[0, : int(n[0].item())].cpu().numpy()
+
+    plain = ipr(stb._ip)                    # no model left on the context
+    lp = S.OlptIprParams.from_buffer_copy(stb._ipr_params)
+    stb.ctx.set_shake_model(net, vox, 0.6)
+    learned = ipr(lp)
+    stb.ctx.set_shake_model(None)
+    dl, _ = cKDTree(learned).query(pts)
+    dp, _ = cKDTree(plain).query(pts)
+    print(f"frame-0 track points {len(pts)}; learned IPR {len(learned)}, plain {len(plain)}; "
+          f"max dist to learned {dl.max():.2e}, to plain {dp.max():.2e} mm")
+    assert dl.max() <= 1e-9
+    assert dp.max() > 1e-6
+    stb.set_ipr_model(None)
+    stb.close()
+    net.close()
