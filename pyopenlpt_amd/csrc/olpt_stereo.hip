@@ -1200,44 +1200,87 @@ __global__ void k_zero_entries(StereoBuffers bb) {
   }
 }
 
+// The rank kernels below take the list entries in tiles of kRankTile
+// consecutive entries per workgroup.  A tile's entries belong to a run of
+// whole point lists [A, B) (lists are contiguous and in order); when the run
+// fits in kRankLds entries its keys are staged in LDS once and every entry
+// counts against its list there - lanes of one list read the same LDS words
+// (broadcast) - instead of re-reading the list's keys from the caches per
+// entry.  A run that does not fit (a tile inside a very long list) is read
+// from global memory as before.  Counts do not depend on where the keys are
+// read from.
+constexpr int kRankTile = 512;
+constexpr int kRankLds = 2048;
+
+// the run of lists [A, B) holding entries [a, a1) (a < a1)
+__device__ __forceinline__ void tile_lists(const StereoBuffers &b, int a, int a1, int &A,
+                                           int &B) {
+  const int f0 = b.pt_of[a], f1 = b.pt_of[a1 - 1];
+  A = b.pt_start[f0];
+  B = b.pt_start[f1] + b.pt_count[f1];
+}
+
 // Percentile of each candidate on each used 2D point (StereoMatch.cpp:
 // 727-800): the point's candidates sorted by e_check (eq_eps ties by
 // index), eq_eps blocks take their average rank, pv = rank / (n - 1).
 // One thread per list entry counts the entries below it.  An entry with no
 // eq_eps partner forms a block of its own at that rank; a point holding any
 // eq_eps pair is flagged and redone by k_pct_fallback.
-__global__ void k_pct_rank(StereoBuffers bb) {
+__global__ void __launch_bounds__(256) k_pct_rank(StereoBuffers bb) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   const int ne = *b.n_entries;
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ne; q += gridDim.x * blockDim.x) {
-    const int f = b.pt_of[q];
-    const int s0 = b.pt_start[f], n = b.pt_count[f];
-    const double ej = b.e_ent[q];
-    int r = 0;
-    bool tie = false;
-    // the point's entries are e_ent[s0, s0 + n); a candidate holds at most
-    // one entry per point, so the entry itself is skipped by position
-    for (int t0 = 0; t0 < n; t0 += kRankBatch) {
-      double em[kRankBatch];
+  __shared__ double se[kRankLds];
+  for (int a = blockIdx.x * kRankTile; a < ne; a += gridDim.x * kRankTile) {
+    const int a1 = min(a + kRankTile, ne);
+    int A, B;
+    tile_lists(b, a, a1, A, B);
+    const bool lds = B - A <= kRankLds;
+    if (lds)
+      for (int i = A + threadIdx.x; i < B; i += blockDim.x) se[i - A] = b.e_ent[i];
+    __syncthreads();
+    for (int q = a + threadIdx.x; q < a1; q += blockDim.x) {
+      const int f = b.pt_of[q];
+      const int s0 = b.pt_start[f], n = b.pt_count[f];
+      const double ej = b.e_ent[q];
+      int r = 0;
+      bool tie = false;
+      // the point's entries are e_ent[s0, s0 + n); a candidate holds at
+      // most one entry per point, so the entry itself is skipped by position
+      if (lds) {
+        const double *le = se + (s0 - A);
+        for (int t = 0; t < n; ++t) {
+          if (s0 + t == q) continue;
+          const double em = le[t];
+          if (eq_eps(em, ej))
+            tie = true;
+          else
+            r += em < ej ? 1 : 0;
+        }
+      } else {
+        for (int t0 = 0; t0 < n; t0 += kRankBatch) {
+          double em[kRankBatch];
 #pragma unroll
-      for (int u = 0; u < kRankBatch; ++u) em[u] = b.e_ent[t0 + u < n ? s0 + t0 + u : q];
+          for (int u = 0; u < kRankBatch; ++u) em[u] = b.e_ent[t0 + u < n ? s0 + t0 + u : q];
 #pragma unroll
-      for (int u = 0; u < kRankBatch; ++u) {
-        if (t0 + u >= n || s0 + t0 + u == q) continue;
-        if (eq_eps(em[u], ej))
-          tie = true;
-        else
-          r += em[u] < ej ? 1 : 0;
+          for (int u = 0; u < kRankBatch; ++u) {
+            if (t0 + u >= n || s0 + t0 + u == q) continue;
+            if (eq_eps(em[u], ej))
+              tie = true;
+            else
+              r += em[u] < ej ? 1 : 0;
+          }
+        }
       }
+      if (tie) {
+        atomicOr(&b.pt_flag[f], 1);
+        continue;
+      }
+      const int32_t v = b.pt_list[q];
+      const double denom = (double)(n - 1);
+      const double avg_rank = 0.5 * (double)(r + r);
+      b.pv[(long)ent_cand(v) * b.n_cam + ent_slot(v)] = (denom > 0.0) ? (avg_rank / denom) : 0.0;
     }
-    if (tie) {
-      atomicOr(&b.pt_flag[f], 1);
-      continue;
-    }
-    const int32_t v = b.pt_list[q];
-    const double denom = (double)(n - 1);
-    const double avg_rank = 0.5 * (double)(r + r);
-    b.pv[(long)ent_cand(v) * b.n_cam + ent_slot(v)] = (denom > 0.0) ? (avg_rank / denom) : 0.0;
+    __syncthreads();
   }
 }
 
@@ -1305,35 +1348,68 @@ __global__ void k_pct_final(StereoBuffers bb, StereoParams p) {
 // of StereoMatch.cpp:820-851 restricted to the point): position = number of
 // entries that precede the entry.  A point whose positions collide (the
 // pairwise order is not transitive there) is flagged for k_prio_fallback.
-__global__ void k_prio_rank(StereoBuffers bb) {
+__global__ void __launch_bounds__(256) k_prio_rank(StereoBuffers bb) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   const int ne = *b.n_entries;
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ne; q += gridDim.x * blockDim.x) {
-    const int f = b.pt_of[q];
-    const int s0 = b.pt_start[f], n = b.pt_count[f];
-    const int32_t v = b.pt_list[q];
-    const int j = ent_cand(v);
-    const double pj = b.p_ent[q], ej = b.e_ent[q];
-    int r = 0;
-    // the point's keys are the contiguous ranges [s0, s0 + n) of pt_list,
-    // p_ent and e_ent
-    for (int t0 = 0; t0 < n; t0 += kRankBatch) {
-      int m[kRankBatch];
-      double pm[kRankBatch], em[kRankBatch];
-#pragma unroll
-      for (int u = 0; u < kRankBatch; ++u) {
-        const int t = t0 + u < n ? s0 + t0 + u : q;
-        m[u] = ent_cand(b.pt_list[t]);
-        pm[u] = b.p_ent[t];
-        em[u] = b.e_ent[t];
+  __shared__ double sp[kRankLds], se[kRankLds];
+  __shared__ int32_t sm[kRankLds];
+  for (int a = blockIdx.x * kRankTile; a < ne; a += gridDim.x * kRankTile) {
+    const int a1 = min(a + kRankTile, ne);
+    int A, B;
+    tile_lists(b, a, a1, A, B);
+    const bool lds = B - A <= kRankLds;
+    if (lds)
+      for (int i = A + threadIdx.x; i < B; i += blockDim.x) {
+        sp[i - A] = b.p_ent[i];
+        se[i - A] = b.e_ent[i];
+        sm[i - A] = ent_cand(b.pt_list[i]);
       }
+    __syncthreads();
+    for (int q = a + threadIdx.x; q < a1; q += blockDim.x) {
+      const int f = b.pt_of[q];
+      const int s0 = b.pt_start[f], n = b.pt_count[f];
+      const int32_t v = b.pt_list[q];
+      const int j = ent_cand(v);
+      const double pj = b.p_ent[q], ej = b.e_ent[q];
+      int r = 0;
+      // the point's keys are the contiguous ranges [s0, s0 + n) of pt_list,
+      // p_ent and e_ent
+      if (lds) {
+        // percentiles lie in [0, 1], where eq_eps's tolerance is at most
+        // 1e-9 + 1e-12: a larger difference decides the order by itself
+        const int o = s0 - A;
+        for (int t = 0; t < n; ++t) {
+          const int m = sm[o + t];
+          const double pm = sp[o + t];
+          const double d = pm - pj;
+          bool lt;
+          if (fabs(d) > 1.001e-9)
+            lt = d < 0.0;
+          else
+            lt = m != j && less_order_v(pm, se[o + t], m, pj, ej, j);
+          r += lt ? 1 : 0;
+        }
+      } else {
+        for (int t0 = 0; t0 < n; t0 += kRankBatch) {
+          int m[kRankBatch];
+          double pm[kRankBatch], em[kRankBatch];
 #pragma unroll
-      for (int u = 0; u < kRankBatch; ++u)
-        r += m[u] != j && less_order_v(pm[u], em[u], m[u], pj, ej, j) ? 1 : 0;
+          for (int u = 0; u < kRankBatch; ++u) {
+            const int t = t0 + u < n ? s0 + t0 + u : q;
+            m[u] = ent_cand(b.pt_list[t]);
+            pm[u] = b.p_ent[t];
+            em[u] = b.e_ent[t];
+          }
+#pragma unroll
+          for (int u = 0; u < kRankBatch; ++u)
+            r += m[u] != j && less_order_v(pm[u], em[u], m[u], pj, ej, j) ? 1 : 0;
+        }
+      }
+      b.ppos[(long)j * b.n_cam + ent_slot(v)] = s0 + r;
+      b.pt_sorted[s0 + r] = v;
+      if (atomicAdd(&b.pt_occ[s0 + r], 1) != 0) atomicOr(&b.pt_flag[f], 2);
     }
-    b.ppos[(long)j * b.n_cam + ent_slot(v)] = s0 + r;
-    b.pt_sorted[s0 + r] = v;
-    if (atomicAdd(&b.pt_occ[s0 + r], 1) != 0) atomicOr(&b.pt_flag[f], 2);
+    __syncthreads();
   }
 }
 
@@ -1855,14 +1931,15 @@ static void prune_section(const StereoBuffers &b, const StereoParams &p, int gM,
   k_grp_sort<<<dim3(g.nb, 1, b.nf), T, 0, st>>>(b, g);
   const int gE = grid_for((long)b.cap_cand * 4, T, 4096);
   k_zero_entries<<<dim3(gE, 1, b.nf), T, 0, st>>>(b);
-  k_pct_rank<<<dim3(gE, 1, b.nf), T, 0, st>>>(b);
+  const int gR = grid_for((long)b.cap_cand * 4, kRankTile, 4096);
+  k_pct_rank<<<dim3(gR, 1, b.nf), T, 0, st>>>(b);
   k_pct_fallback<<<dim3(grid_for(nflat, T, 2048), 1, b.nf), T, 0, st>>>(b);
   k_pct_final<<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p);
   k_pct_ent<<<dim3(gE, 1, b.nf), T, 0, st>>>(b);
   k_hz_insert<<<dim3(gM, 1, b.nf), T, 0, st>>>(b);
   k_hz_check<<<dim3(gM, 1, b.nf), T, 0, st>>>(b);
   k_hz_clear<<<dim3(gM, 1, b.nf), T, 0, st>>>(b);
-  k_prio_rank<<<dim3(gE, 1, b.nf), T, 0, st>>>(b);
+  k_prio_rank<<<dim3(gR, 1, b.nf), T, 0, st>>>(b);
   k_prio_fallback<<<dim3(grid_for(nflat, T, 2048), 1, b.nf), T, 0, st>>>(b);
   // the first rounds decide almost every candidate; one workgroup finishes
   // the few left
