@@ -172,7 +172,7 @@ __device__ inline void line_of_sight(const CamT<double> &c, double u,
 // with unit direction u[i].  Returns false where the reference throws.
 // Fixed line count: every loop unrolls, so the lines stay in registers.
 template <int N>
-__device__ __forceinline__ bool triangulate_n(const double (*pt)[3], const double (*u)[3],
+__host__ __device__ __forceinline__ bool triangulate_n(const double (*pt)[3], const double (*u)[3],
                                               double X[3], double &err) {
   double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, P[3] = {0, 0, 0};
 #pragma unroll
@@ -208,6 +208,69 @@ __device__ __forceinline__ bool triangulate_n(const double (*pt)[3], const doubl
   }
   err = sqrt(e / N);
   return ok;
+}
+
+// The decision triangulate_n<N>(pt, u, X, err) && err <= tq, taken without
+// the Gauss inverse where the outcome is certain: 1 (true), 0 (false), or -1
+// when it has to be computed exactly.  The least-squares point comes from
+// the adjugate of the same normal matrix, and the squared residual sum
+// E = sum |(X - pt_i) x u_i|^2 from cross products.  triangulate_n's err
+// is sqrt(sum h_i^2 / N) with h_i^2 = |d_i|^2 - (d_i . u_i)^2 (d_i = X -
+// pt_i), whose rounding is at most 13 eps |d_i|^2 per line (the
+// cancellation of two squares of size |d_i|^2); the two least-squares
+// points differ by rounding only, which moves E in second order.  So the
+// decision is certain when E and N tq^2 differ by more than
+// N * 16 eps max|d_i|^2 plus 1e-6 N tq^2 (the final division, square roots
+// and comparison).  Taken only where the normal matrix is well conditioned
+// (every pair of lines at least 1 degree apart: max_dot below cos 1 deg)
+// and |d_i| < 1000 (then h_i^2 >= -13 eps |d_i|^2 > -1e-8, the bound below
+// which triangulate_n fails).
+template <int N>
+__host__ __device__ __forceinline__ int tri_decide(const double (*pt)[3], const double (*u)[3],
+                                          double tq, double max_dot) {
+  constexpr double kCos1Deg = 0.99984769515639124;
+  if (!(max_dot < kCos1Deg)) return -1;
+  double m00 = 0, m01 = 0, m02 = 0, m11 = 0, m12 = 0, m22 = 0, b0 = 0, b1 = 0, b2 = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const double nx = u[i][0], ny = u[i][1], nz = u[i][2];
+    const double t00 = 1 - nx * nx, t01 = -nx * ny, t02 = -nx * nz;
+    const double t11 = 1 - ny * ny, t12 = -ny * nz, t22 = 1 - nz * nz;
+    m00 += t00;
+    m01 += t01;
+    m02 += t02;
+    m11 += t11;
+    m12 += t12;
+    m22 += t22;
+    b0 += t00 * pt[i][0] + t01 * pt[i][1] + t02 * pt[i][2];
+    b1 += t01 * pt[i][0] + t11 * pt[i][1] + t12 * pt[i][2];
+    b2 += t02 * pt[i][0] + t12 * pt[i][1] + t22 * pt[i][2];
+  }
+  const double c00 = m11 * m22 - m12 * m12, c01 = m02 * m12 - m01 * m22,
+               c02 = m01 * m12 - m02 * m11, c11 = m00 * m22 - m02 * m02,
+               c12 = m01 * m02 - m00 * m12, c22 = m00 * m11 - m01 * m01;
+  const double det = m00 * c00 + m01 * c01 + m02 * c02;
+  if (!(det > 1e-12)) return -1;
+  const double inv = 1.0 / det;
+  const double X0 = (c00 * b0 + c01 * b1 + c02 * b2) * inv;
+  const double X1 = (c01 * b0 + c11 * b1 + c12 * b2) * inv;
+  const double X2 = (c02 * b0 + c12 * b1 + c22 * b2) * inv;
+  double E = 0.0, D2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const double d0 = X0 - pt[i][0], d1 = X1 - pt[i][1], d2 = X2 - pt[i][2];
+    const double q0 = d1 * u[i][2] - d2 * u[i][1];
+    const double q1 = d2 * u[i][0] - d0 * u[i][2];
+    const double q2 = d0 * u[i][1] - d1 * u[i][0];
+    E += q0 * q0 + q1 * q1 + q2 * q2;
+    D2 = fmax(D2, d0 * d0 + d1 * d1 + d2 * d2);
+  }
+  if (!(D2 < 1e6) || !(E == E)) return -1;
+  const double T = (double)N * tq * tq;
+  const double delta = (double)N * 16.0 * 2.220446049250313e-16 * D2 + 1e-6 * T;
+  if (E + delta < T) return 1;
+  if (E - delta > T) return 0;
+  return -1;
 }
 
 // triangulate_n<n> for a runtime n <= MC on arrays of MC rows (rows past MC

@@ -563,8 +563,14 @@ __device__ int extend(const StereoBuffers &b, const StereoParams &p, Path &P, in
     } else {
       tq = tri_tol(p.tol3d, KK, B, acos(mx) * 180.0 / M_PI);
     }
-    double X[3], e;
-    if (!triangulate_n<KK>(P.P, P.U, X, e) || !(e <= tq)) return 0;
+    // only the decision counts: tri_decide settles it unless the error
+    // lies within rounding of the tolerance
+    int d = KK != B ? tri_decide<KK>(P.P, P.U, tq, mx) : -1;
+    if (d < 0) {
+      double X[3], e;
+      d = triangulate_n<KK>(P.P, P.U, X, e) && e <= tq ? 1 : 0;
+    }
+    if (!d) return 0;
   }
   if (KK == B) {
     // checkMatch, all-active branch (:530-540): LOS in camera order
