@@ -70,7 +70,9 @@ olpt_status olpt_set_concurrency(olpt_ctx *ctx, int n_streams);
  * the shake and proposal kernels (default 4096); build_blocks: 256-thread
  * workgroups of the stereo build kernels (default 1024); prune_blocks: cap
  * on the 256-thread workgroups of the sort/prune/triangulate kernels
- * (default 512), each per frame; n_streams: as olpt_set_concurrency;
+ * (default 512), each per frame (a launch over few frames raises them so
+ * that it has at least 1024 workgroups, 4096 one-wave ones, in all);
+ * n_streams: as olpt_set_concurrency;
  * frames_per_batch (default 1): the frames every kernel launch of
  * olpt_ipr_run covers - its grid takes the frame from blockIdx.z and each
  * workspace holds that many frame slices (memory grows with it; changing it

@@ -81,6 +81,19 @@ inline long peak_chunks(int H, int W) {
 // A partial match of the stereo build with the point it is being extended
 // by: the points and cameras of path slots 0..K (slot 0 the reference
 // observation, slot K the point under test).
+// Workgroups per frame of a launch covering nf frames: the tuned count, but
+// at least min_total in all (kMinLaunchBlocks 256-thread workgroups, four
+// times as many one-wave ones), so that launches over few frames (one frame
+// per call) still fill the chip.  A launch shape never changes what a
+// kernel computes.
+constexpr int kMinLaunchBlocks = 1024;
+__host__ __device__ inline int launch_blocks(int tuned, int nf,
+                                             int min_total = kMinLaunchBlocks) {
+  const int need = (min_total + (nf > 0 ? nf : 1) - 1) / (nf > 0 ? nf : 1);
+  const int g = tuned > need ? tuned : need;
+  return g < 65536 ? g : 65536;
+}
+
 template <int K> struct PathRec {
   int32_t pid[K + 1];
   int8_t cam[K + 1];

@@ -1259,7 +1259,8 @@ template <typename T, int NC, int OPW, int SLOTS>
 static void shake_iter(const ShakeBuffers<T> &b, const ShakeParams &p,
                        const ObjCamRec<T> *rec, T delta, hipStream_t st) {
   stage_begin(ST_SHAKE, st);
-  k_shake<T, NC, OPW, SLOTS><<<dim3(b.grid_shake, 1, b.nf), kWave, 0, st>>>(b, p, rec, delta);
+  k_shake<T, NC, OPW, SLOTS><<<dim3(launch_blocks(b.grid_shake, b.nf, 4 * kMinLaunchBlocks), 1, b.nf), kWave, 0, st>>>(
+      b, p, rec, delta);
   stage_end(ST_SHAKE, st);
 }
 
@@ -1303,7 +1304,8 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
     mlp_infer_device(p.mlp, b.patches, b.n_obj, b.cap_obj, b.prop, st, b.nf, b.lstride,
                      b.fstride, b.pstride);
     constexpr int O = shake_opw<T>();
-    k_propose<T, 4, O, 4 * O><<<dim3(b.grid_shake, 1, b.nf), kWave, 0, st>>>(bo, p, rec, b.prop);
+    k_propose<T, 4, O, 4 * O><<<dim3(launch_blocks(b.grid_shake, b.nf, 4 * kMinLaunchBlocks), 1, b.nf), kWave, 0, st>>>(
+        bo, p, rec, b.prop);
   }
   double delta = p.shake_width;
   double dmin = p.shakewidth_min > 0 ? p.shakewidth_min : delta / 20;

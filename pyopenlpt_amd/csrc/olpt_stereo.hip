@@ -1906,9 +1906,9 @@ static int grid_for(long n, int threads, int max_blocks) {
 template <int K>
 static void build_level(const StereoBuffers &b, const StereoParams &p, hipStream_t st) {
   if (K == 2) stage_begin(ST_ENUM2, st);
-  k_b_enum<K><<<dim3(b.grid_build, 1, b.nf), 256, 0, st>>>(b, p);
+  k_b_enum<K><<<dim3(launch_blocks(b.grid_build, b.nf), 1, b.nf), 256, 0, st>>>(b, p);
   if (K == 2) stage_end(ST_ENUM2, st);
-  k_b_extend<K><<<dim3(b.grid_build, 1, b.nf), 256, 0, st>>>(b, p);
+  k_b_extend<K><<<dim3(launch_blocks(b.grid_build, b.nf), 1, b.nf), 256, 0, st>>>(b, p);
 }
 
 GrpShape grp_shape(const StereoBuffers &b) {
@@ -1974,7 +1974,7 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   k_cell_fill<<<gc, T, 0, st>>>(b, p);
   // build: level 1 from every reference observation, then one enumerate +
   // extend pair of kernels per deeper build camera
-  const int G = b.grid_build;
+  const int G = launch_blocks(b.grid_build, b.nf);
   stage_begin(ST_BUILD, st);
   zero_i32(b, b.n_path, kMaxCam, st);
   zero_i32(b, b.n_surv, kMaxCam, st);
@@ -1991,7 +1991,7 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   stage_end(ST_BUILD, st);
   stage_begin(ST_PRUNE, st);
   // sort
-  const int gM = grid_for(b.cap_cand, T, b.grid_prune);
+  const int gM = grid_for(b.cap_cand, T, launch_blocks(b.grid_prune, b.nf));
   zero_i32(b, b.key_cnt, b.cap, st);
   k_key_hist<<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p);
   scan_exclusive(b.key_cnt, b.key_start, b.cap, 1, b.cap, nullptr, b.scan_scratch, st, b.nf, b.fstride);
@@ -2013,7 +2013,7 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
 void launch_prune(const StereoBuffers &b, const StereoParams &p, int max_rounds,
                   hipStream_t st) {
   zero_i32(b, b.undecided, max_rounds, st);
-  prune_section(b, p, grid_for(b.cap_cand, 256, b.grid_prune), st);
+  prune_section(b, p, grid_for(b.cap_cand, 256, launch_blocks(b.grid_prune, b.nf)), st);
 }
 
 } // namespace olpt
