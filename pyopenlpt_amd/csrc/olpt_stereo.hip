@@ -1350,6 +1350,10 @@ __global__ void k_pct_final(StereoBuffers bb, StereoParams p) {
   }
 }
 
+// Greedy state of a candidate that k_prio_rank found below the first entry
+// of one of its lists; k_mis_top turns it back into undecided (0).
+constexpr int8_t kNotFirst = 3;
+
 // Each point's candidates in the greedy's priority order (the global sort
 // of StereoMatch.cpp:820-851 restricted to the point): position = number of
 // entries that precede the entry.  A point whose positions collide (the
@@ -1414,6 +1418,8 @@ __global__ void __launch_bounds__(256) k_prio_rank(StereoBuffers bb) {
       b.ppos[(long)j * b.n_cam + ent_slot(v)] = s0 + r;
       b.pt_sorted[s0 + r] = v;
       if (atomicAdd(&b.pt_occ[s0 + r], 1) != 0) atomicOr(&b.pt_flag[f], 2);
+      // not first on this point: the greedy's first round can skip it
+      if (r != 0) b.state[j] = kNotFirst;
     }
     __syncthreads();
   }
@@ -1528,6 +1534,23 @@ __global__ void k_mis_round(StereoBuffers bb, StereoParams p, int round) {
   }
   und = wave_sum(und);
   if (lane_id() == 0 && und) atomicAdd(&b.undecided[round], und);
+}
+
+// The greedy's first round: only a candidate that is first on every one of
+// its lists can be selected before anything else is decided.  Those that
+// k_prio_rank marked kNotFirst become undecided without a look at their
+// lists; the others take their turn (mis_decide, which also covers points
+// whose order k_prio_fallback rewrote: it reads the final positions).
+__global__ void k_mis_top(StereoBuffers bb, StereoParams p) {
+  const StereoBuffers b = bb.frame(blockIdx.z);
+  const int M = min(*b.n_cand, b.cap_cand);
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
+    const int8_t s = b.state[j];
+    if (s == kNotFirst)
+      b.state[j] = 0;
+    else if (s == 0)
+      (void)mis_decide(b, p, j);
+  }
 }
 
 // After kMisFull rounds over all candidates the few still undecided are
@@ -1949,6 +1972,7 @@ static void prune_section(const StereoBuffers &b, const StereoParams &p, int gM,
   k_prio_fallback<<<dim3(grid_for(nflat, T, 2048), 1, b.nf), T, 0, st>>>(b);
   // the first rounds decide almost every candidate; one workgroup finishes
   // the few left
+  k_mis_top<<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p);
   for (int r = 0; r < kMisFull; ++r) k_mis_round<<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p, r);
   k_mis_compact<<<dim3(gM, 1, b.nf), T, 0, st>>>(b);
   k_mis_finish<<<dim3(1, 1, b.nf), kMisFinishThreads, 0, st>>>(b, p);
