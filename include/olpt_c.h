@@ -264,6 +264,12 @@ olpt_status olpt_mlp_train_step(olpt_mlp *mlp, const void *d_x, const float *d_t
  * network is not owned; a previously captured graph keeps the old setting. */
 olpt_status olpt_set_shake_model(olpt_ctx *ctx, const olpt_mlp *mlp,
                                  double voxel_mm, double max_step_vox);
+/* No counterpart in the reference: with the learned proposal in use, an
+ * object whose ghost logit (the network's fourth output, at the matched
+ * position) exceeds logit_min is flagged a ghost after the shake, beside the
+ * score threshold (Shake.cpp:118-136): it leaves the IPR output and the
+ * loop's residual image.  INFINITY (the default) turns this off. */
+olpt_status olpt_set_ghost_logit(olpt_ctx *ctx, double logit_min);
 /* Network inputs of n_obj objects at d_xyz on images d_imgs: residual of
  * the active cameras from all n_obj objects (as at the start of a shake
  * iteration), then per camera the 11x11 window of max(0, residual + own

@@ -66,6 +66,7 @@ def lib() -> ctypes.CDLL:
     L.olpt_stream.restype = _P
     L.olpt_diagnostics.argtypes = [_P, _I32P]
     L.olpt_build_counts.argtypes = [_P, _I32P]
+    L.olpt_set_ghost_logit.argtypes = [_P, ctypes.c_double]
     L.olpt_find_peaks.argtypes = [_P, _P, _I, _I, _I, ctypes.POINTER(OlptIprParams),
                                   _P, _I, _I32P]
     L.olpt_stereo_match.argtypes = [_P, _I32P, _P, _I, _I32P,
@@ -134,7 +135,7 @@ def lib() -> ctypes.CDLL:
                  "olpt_set_precision", "olpt_set_concurrency", "olpt_set_tuning",
                  "olpt_get_tuning", "olpt_set_cameras",
                  "olpt_set_otf",
-                 "olpt_sync", "olpt_diagnostics", "olpt_build_counts", "olpt_find_peaks",
+                 "olpt_sync", "olpt_diagnostics", "olpt_build_counts", "olpt_set_ghost_logit", "olpt_find_peaks",
                  "olpt_stereo_match", "olpt_prune_match", "olpt_find_peaks_static", "olpt_otf_param", "olpt_shake", "olpt_residual",
                  "olpt_ipr_run", "olpt_ipr_graph_capture",
                  "olpt_ipr_graph_launch", "olpt_stage_timing",

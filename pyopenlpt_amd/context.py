@@ -311,12 +311,16 @@ class Context:
         return d_res.cpu().numpy()
 
     # -- learned shake proposal ----------------------------------------------
-    def set_shake_model(self, net, voxel_mm: float = 0.0, max_step_vox: float = 0.5):
+    def set_shake_model(self, net, voxel_mm: float = 0.0, max_step_vox: float = 0.5,
+                        ghost_logit: float | None = None):
         """Use a learned.ShakeNet before the plain shake iterations (None:
-        plain shake only)."""
+        plain shake only).  ghost_logit: objects whose ghost logit exceeds
+        it are flagged ghosts after the shake (None: off)."""
         self._net = net
         check(self.L.olpt_set_shake_model(self.h, net.h if net is not None else None,
                                           float(voxel_mm), float(max_step_vox)))
+        check(self.L.olpt_set_ghost_logit(
+            self.h, float("inf") if ghost_logit is None else float(ghost_logit)))
 
     def render(self, xyz: torch.Tensor, amp: torch.Tensor | None = None,
                r_px: float = 2.5) -> torch.Tensor:

@@ -125,6 +125,7 @@ struct olpt_ctx {
   // learned shake proposal (not owned)
   const olpt_mlp *mlp = nullptr;
   double mlp_voxel_mm = 0.0, mlp_max_step_vox = 0.5;
+  double mlp_ghost_logit = INFINITY;  // olpt_set_ghost_logit (off by default)
   // workspaces
   Workspace ws[kMaxWorkspaces];
   int n_ws = 0;          // allocated
@@ -474,6 +475,7 @@ ShakeParams shake_params(olpt_ctx *c, const int32_t *active, const olpt_ipr_para
   sp.mlp = c->mlp;
   sp.voxel_mm = c->mlp_voxel_mm;
   sp.max_step_vox = c->mlp_max_step_vox;
+  sp.ghost_logit = c->mlp_ghost_logit;
   return sp;
 }
 
@@ -1719,6 +1721,13 @@ olpt_status olpt_set_shake_model(olpt_ctx *c, const olpt_mlp *mlp, double voxel_
       olpt_status s = alloc_learned(c, c->ws[k]);
       if (s) return s;
     }
+  return OLPT_OK;
+}
+
+olpt_status olpt_set_ghost_logit(olpt_ctx *c, double logit_min) {
+  if (!c) return fail(OLPT_E_INVALID_ARGUMENT, "null context");
+  if (logit_min != logit_min) return fail(OLPT_E_INVALID_ARGUMENT, "logit_min is NaN");
+  c->mlp_ghost_logit = logit_min;
   return OLPT_OK;
 }
 

@@ -392,6 +392,7 @@ struct ShakeParams {
   double shake_width, shakewidth_min, thred_ghost, tol3d, r_px;
   const void *mlp;            // learned proposal network (olpt_mlp), or null
   double voxel_mm, max_step_vox;
+  double ghost_logit;         // learned proposal: ghost logit above this flags a ghost
   int n_shake_loop, ratio_augimg;
   // calROIBound box edges (box_width): the object box (r_px) and the shake
   // ROI (r_px * ratio_augimg), fixed on the host in double arithmetic

@@ -1215,9 +1215,11 @@ __global__ void __launch_bounds__(kRedThreads) k_score_sum(ShakeBuffers<T> bb) {
 }
 
 // Ghost: score < thred_ghost * mean(non-repeated) or score == 0
-// (Shake.cpp:118-136); keep_off <- 1 for unflagged objects.
+// (Shake.cpp:118-136); keep_off <- 1 for unflagged objects.  With the
+// learned proposal run in this shake (use_logit), also when its ghost logit
+// exceeds ghost_logit (no counterpart in the reference).
 template <typename T>
-__global__ void k_flags(ShakeBuffers<T> bb, double thr) {
+__global__ void k_flags(ShakeBuffers<T> bb, double thr, bool use_logit, double ghost_logit) {
   const long F = (long)blockIdx.z;
   const ShakeBuffers<T> b = bb.frame(F);
   double sum = 0.0, cnt = 0.0;
@@ -1231,6 +1233,7 @@ __global__ void k_flags(ShakeBuffers<T> bb, double thr) {
     uint8_t f = b.flag[i];
     const double sc = b.score[i];
     if (sc < mean * thr || sc == 0.0) f |= OLPT_FLAG_GHOST;
+    if (use_logit && (double)b.prop[4L * i + 3] > ghost_logit) f |= OLPT_FLAG_GHOST;
     b.flag[i] = f;
   }
 }
@@ -1296,7 +1299,8 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
     k_grid_fill<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b);
     bo.order = b.g_obj;
   }
-  if (p.mlp && b.patches && b.prop && nc == 4 && roi_px <= 64) {
+  const bool learned = p.mlp && b.patches && b.prop && nc == 4 && roi_px <= 64;
+  if (learned) {
     // learned proposal from the matched positions, then the plain iterations
     residual_pass<T>(b, b.img, b.res, b.xyz, b.n_obj, rec, b.uv, act, 0, r_px, st);
     k_patch_gather<T><<<dim3(grid_for((long)b.cap_obj * kPatchIn, 256, 8192), 1, b.nf), 256, 0, st>>>(
@@ -1342,7 +1346,9 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
   k_repeat_greedy<T><<<dim3(1, 1, b.nf), 64, 0, st>>>(b, t2);
   // ghost threshold: mean over non-repeated scores
   k_score_sum<T><<<dim3(kRedBlocks, 1, b.nf), kRedThreads, 0, st>>>(b);
-  k_flags<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b, p.thred_ghost);
+  k_flags<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b, p.thred_ghost,
+                                                  learned && p.ghost_logit < INFINITY,
+                                                  p.ghost_logit);
   (void)inact;
 }
 

@@ -182,3 +182,34 @@ def test_learned_ipr_runs_and_plain_is_restored(gpu_available):
     assert torch.equal(ox[0, :n_plain], ref)
     ctx.close()
     net.close()
+
+
+def test_ghost_logit_threshold(gpu_available):
+    """olpt_set_ghost_logit: an infinite threshold changes nothing; a
+    threshold below every logit flags every object of every loop a ghost,
+    so the IPR output is empty; a finite threshold leaves a subset."""
+    from pyopenlpt_amd.context import Context, to_device_image
+    from pyopenlpt_amd.learned import ShakeNet
+    case = Case("golden_u8_256")
+    ctx = Context(case.cams, case.H, case.W, precision="f32", max_2d=1 << 15,
+                  max_cand=1 << 18, max_obj=1 << 15)
+    otf, keep = case.otf()
+    ctx.set_otf(otf)
+    d, code = to_device_image(case.images)
+    net = ShakeNet(max_batch=1024, seed=2)
+    vox = case.params.tol_3d_mm
+
+    def run(thr):
+        ctx.set_shake_model(net, vox, 0.5, ghost_logit=thr)
+        (ox, _u, n), _ = ctx.ipr_run(d, code, case.params, cap_obj=1 << 14)
+        k = int(n[0].item())
+        return ox[0, :k].clone()
+
+    base = run(None)
+    assert len(base) > 0
+    assert torch.equal(run(float("inf")), base)
+    assert len(run(-1e30)) == 0
+    assert len(run(0.0)) <= len(base) + len(base) // 2
+    ctx.set_shake_model(None)
+    ctx.close()
+    net.close()
