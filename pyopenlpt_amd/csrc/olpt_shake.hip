@@ -954,39 +954,59 @@ k_shake(ShakeBuffers<T> bb, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
 // zeros of padding (bf16 [n][512]).
 constexpr int kPatch = 11, kPatchHalf = 5, kPatchIn = 512;
 
+// One wave per object: camera by camera the object's record is the same for
+// every lane (scalar loads), and the 121 window pixels are spread over the
+// lanes; each 512-value row is written whole by its wave.
+constexpr int kPatchWaves = 4;   // waves (objects) per workgroup
+
 template <typename T>
-__global__ void k_patch_gather(ShakeBuffers<T> bb, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
-                               __bf16 *__restrict__ X) {
+__global__ void __launch_bounds__(kPatchWaves * kWave)
+k_patch_gather(ShakeBuffers<T> bb, ShakeParams p, const ObjCamRec<T> *__restrict__ rec,
+               __bf16 *__restrict__ X) {
   const long F = (long)blockIdx.z;
   const ShakeBuffers<T> b = bb.frame(F);
   const long fo = F * bb.fstride;
   rec = fshift(rec, fo);
   X = fshift(X, (long)F * bb.lstride);
-  const long n = *b.n_obj;
+  const int n = *b.n_obj;
   const int nc = b.n_cam;
   const long HW = (long)b.H * b.W;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n * kPatchIn;
-       t += (long)gridDim.x * blockDim.x) {
-    const long i = t / kPatchIn;
-    const int c = (int)(t % kPatchIn);
-    float v = 0.0f;
-    const int k = c / (kPatch * kPatch), pix = c % (kPatch * kPatch);
-    if (k < 4 && k < nc && p.active[k]) {
-      const ObjCamRec<T> &r = rec[i * nc + k];
-      if (fabs((double)r.u) < 1e6 && fabs((double)r.v) < 1e6) {
-        const int cu = (int)round((double)r.u), cv = (int)round((double)r.v);
-        const int row = cv + pix / kPatch - kPatchHalf, col = cu + pix % kPatch - kPatchHalf;
-        if (row >= 0 && row < b.H && col >= 0 && col < b.W) {
-          T a = b.res[k * HW + (long)row * b.W + col];
-          if (row >= r.rmin && row < r.rmax && col >= r.cmin && col < r.cmax)
-            a += gauss_int<T>(r.P, r.u, r.v, row, col);
-          a = a > T(0) ? a : T(0);
-          const T amp = r.P[0] > T(1e-6) ? r.P[0] : T(1e-6);
-          v = (float)(a / amp);
+  const int lane = lane_id();
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
+  for (int i = blockIdx.x * kPatchWaves + wave; i < n; i += gridDim.x * kPatchWaves) {
+    __bf16 *row_out = X + (long)i * kPatchIn;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const bool cam = k < nc && p.active[k];
+      ObjCamRec<T> r{};
+      bool inside = false;
+      int cu = 0, cv = 0;
+      if (cam) {
+        r = rec[(long)i * nc + k];
+        inside = fabs((double)r.u) < 1e6 && fabs((double)r.v) < 1e6;
+        if (inside) {
+          cu = (int)round((double)r.u);
+          cv = (int)round((double)r.v);
         }
       }
+      const T amp = r.P[0] > T(1e-6) ? r.P[0] : T(1e-6);
+      for (int pix = lane; pix < kPatch * kPatch; pix += kWave) {
+        float v = 0.0f;
+        if (inside) {
+          const int row = cv + pix / kPatch - kPatchHalf, col = cu + pix % kPatch - kPatchHalf;
+          if (row >= 0 && row < b.H && col >= 0 && col < b.W) {
+            T a = b.res[k * HW + (long)row * b.W + col];
+            if (row >= r.rmin && row < r.rmax && col >= r.cmin && col < r.cmax)
+              a += gauss_int<T>(r.P, r.u, r.v, row, col);
+            a = a > T(0) ? a : T(0);
+            v = (float)(a / amp);
+          }
+        }
+        row_out[k * kPatch * kPatch + pix] = (__bf16)v;
+      }
     }
-    X[t] = (__bf16)v;
+    // the padding after the 4 x 121 values
+    if (lane < kPatchIn - 4 * kPatch * kPatch) row_out[4 * kPatch * kPatch + lane] = (__bf16)0.0f;
   }
 }
 
@@ -1303,8 +1323,8 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
   if (learned) {
     // learned proposal from the matched positions, then the plain iterations
     residual_pass<T>(b, b.img, b.res, b.xyz, b.n_obj, rec, b.uv, act, 0, r_px, st);
-    k_patch_gather<T><<<dim3(grid_for((long)b.cap_obj * kPatchIn, 256, 8192), 1, b.nf), 256, 0, st>>>(
-        b, p, rec, (__bf16 *)b.patches);
+    k_patch_gather<T><<<dim3(grid_for(b.cap_obj, kPatchWaves, 8192), 1, b.nf), kPatchWaves * kWave,
+                        0, st>>>(b, p, rec, (__bf16 *)b.patches);
     mlp_infer_device(p.mlp, b.patches, b.n_obj, b.cap_obj, b.prop, st, b.nf, b.lstride,
                      b.fstride, b.pstride);
     constexpr int O = shake_opw<T>();
@@ -1423,8 +1443,8 @@ void launch_patches(const ShakeBuffers<T> &b, const ShakeParams &p, void *X, hip
     if (p.active[k]) act.id[act.n++] = k;
   ObjCamRec<T> *rec = reinterpret_cast<ObjCamRec<T> *>(b.rec);
   residual_pass<T>(b, b.img, b.res, b.xyz, b.n_obj, rec, b.uv, act, 0, p.r_px, st);
-  k_patch_gather<T><<<dim3(grid_for((long)b.cap_obj * kPatchIn, 256, 8192), 1, b.nf), 256, 0, st>>>(
-      b, p, rec, (__bf16 *)X);
+  k_patch_gather<T><<<dim3(grid_for(b.cap_obj, kPatchWaves, 8192), 1, b.nf), kPatchWaves * kWave, 0,
+                      st>>>(b, p, rec, (__bf16 *)X);
 }
 
 template <typename T>
