@@ -3,6 +3,10 @@
   python -m pyopenlpt_amd.prof stats <run_kernel_stats.csv> [--top 25]
       per-kernel calls / total / average duration (from --kernel-trace --stats)
 
+  python -m pyopenlpt_amd.prof trace <run_kernel_trace.csv> [--top 25] [--after PATTERN]
+      the same from a kernel trace, optionally only the dispatches after the
+      last one matching PATTERN (the timed steps after a training phase)
+
   python -m pyopenlpt_amd.prof pmc --fetch <fetch_counter_collection.csv>
       --write <write_counter_collection.csv> --kernel k_shake
       --calib-kernel neg --calib-bytes N
@@ -54,6 +58,30 @@ def stats(path: str, top: int) -> str:
         out.append(f"| `{_short(r['Name'])}` | {int(r['Calls'])} | "
                    f"{float(r['TotalDurationNs']) / 1e6:.3f} | {float(r['Percentage']):.1f} | "
                    f"{float(r['AverageNs']) / 1e3:.1f} |")
+    return "\n".join(out)
+
+
+def trace(path: str, top: int, after: str | None) -> str:
+    """Per-kernel calls / total / average from a kernel-trace CSV; with
+    `after`, only the dispatches that follow the last one whose name matches
+    that pattern (e.g. the timed steps after a training phase)."""
+    rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Start_Timestamp"]))
+    if after:
+        last = max((i for i, r in enumerate(rows) if re.search(after, r["Kernel_Name"])),
+                   default=-1)
+        rows = rows[last + 1:]
+    tot: dict[str, float] = {}
+    calls: dict[str, int] = {}
+    for r in rows:
+        n = _short(r["Kernel_Name"])
+        tot[n] = tot.get(n, 0.0) + int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+        calls[n] = calls.get(n, 0) + 1
+    T = sum(tot.values())
+    out = [f"total kernel time {T / 1e6:.3f} ms over {len(rows)} dispatches", "",
+           "| kernel | calls | total ms | % | avg us |", "|---|---:|---:|---:|---:|"]
+    for n in sorted(tot, key=lambda k: -tot[k])[:top]:
+        out.append(f"| `{n}` | {calls[n]} | {tot[n] / 1e6:.3f} | {100 * tot[n] / T:.1f} | "
+                   f"{tot[n] / calls[n] / 1e3:.1f} |")
     return "\n".join(out)
 
 
@@ -218,6 +246,11 @@ def main(argv=None) -> int:
     a = sub.add_parser("stats")
     a.add_argument("csv")
     a.add_argument("--top", type=int, default=25)
+    t = sub.add_parser("trace")
+    t.add_argument("csv")
+    t.add_argument("--top", type=int, default=25)
+    t.add_argument("--after", default=None,
+                   help="only dispatches after the last one matching this pattern")
     b = sub.add_parser("pmc")
     b.add_argument("--fetch", required=True)
     b.add_argument("--write", required=True)
@@ -243,6 +276,8 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     if args.cmd == "stats":
         print(stats(args.csv, args.top))
+    elif args.cmd == "trace":
+        print(trace(args.csv, args.top, args.after))
     elif args.cmd == "pmc":
         print(pmc(args.fetch, args.write, args.kernel, args.calib_kernel, args.calib_bytes))
     elif args.cmd == "table":
