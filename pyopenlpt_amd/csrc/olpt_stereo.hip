@@ -10,8 +10,9 @@
 //                 first over all reference observations at once, one kernel
 //                 per step and one thread per work item: k_b_second picks
 //                 each observation's second camera, k_b_band enumerates its
-//                 epipolar band (thread per observation x 4 cell rows of the
-//                 strip, computeStripIntersection :1475-1566), k_b_extend
+//                 epipolar band (wave per observation, lanes over cell rows
+//                 and then over the rows' points, computeStripIntersection
+//                 :1475-1566), k_b_extend
 //                 runs back-projection, triangulation and camera selection
 //                 on every (partial match, point) item, k_b_enum enumerates
 //                 the strip intersection of every survivor on its next
@@ -741,36 +742,80 @@ __device__ __forceinline__ void enum_rows(const StereoBuffers &b, int t, const S
   }
 }
 
-// Level-1 band: one thread per (reference observation, run of kBandRows
-// cell rows of its second camera); accepted points become (r, pid) pairs.
-constexpr int kBandRows = kRowsPass;
+// Level-1 band: one wave per reference observation (its strip on the second
+// camera is wave-uniform).  Per pass of 64 cell rows each lane takes one row:
+// its span (FP64, computeStripIntersection's arithmetic) and that span's CSR
+// range; a wave scan lays the rows' points end to end, and the wave tests 64
+// consecutive points per step, each lane the point at its position (its row
+// found by a binary search over the lanes' prefix sums in LDS).  Consecutive
+// lanes so read consecutive points of the cell-ordered arrays - coalesced
+// loads, where one lane per row run would gather from its own cache lines -
+// and the pairs leave in reference order, row by row along the epipolar line,
+// which keeps the next stages' gathers coherent too.
 __global__ void __launch_bounds__(256) k_b_band(StereoBuffers bb, StereoParams p) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   int nref;
   const int ref = ref_camera(b, p, nref);
-  const int rows = b.rows_cell;
-  const int nrun = (rows + kBandRows - 1) / kBandRows;
-  const long total = (long)nref * nrun;
+  const int rows = b.rows_cell, cols = b.cols_cell;
+  const long ncell = (long)rows * cols;
   const double tolg = p.tol2d + 1e-6;
   const double tol2 = p.tol2d * p.tol2d;
-  const long stride = (long)gridDim.x * blockDim.x;
   __shared__ BlockQueue<int2, kQueue> q;
+  __shared__ int32_t wpre[256], wka[256];   // per lane: points before its row, its row's start
   bq_init(q);
   auto out = [&](long i, const int2 &v) { b.pairs[i] = v; };
-  for (long base = (long)blockIdx.x * blockDim.x; base < total; base += stride) {
-    const long item = base + threadIdx.x;
-    const int r = (int)(item / nrun);
-    const int t1 = item < total ? (int)b.t1[r] : -1;
+  const int lane = threadIdx.x & (kWave - 1), w0 = threadIdx.x & ~(kWave - 1);
+  const int nw = blockDim.x / kWave;
+  // block-uniform loop over groups of nw reference observations
+  for (int rbase = blockIdx.x * nw; rbase < nref; rbase += gridDim.x * nw) {
+    const int r = rbase + (int)threadIdx.x / kWave;
+    const int t1 = r < nref ? (int)b.t1[r] : -1;
     if (t1 >= 0) {
-      const int r0 = (int)(item % nrun) * kBandRows;
-      const int r1 = min(r0 + kBandRows - 1, rows - 1);
-      Strip S[1];
-      StripRows SR[1];
-      S[0] = strip_of(line_of(b, ref, t1, r));
-      SR[0] = strip_rows(S[0], b.cell_px, tolg);
-      enum_rows<1>(b, t1, S, SR, r0, r1, tol2, [&](bool acc, int pid) {
-        bq_push_any(q, acc, make_int2(r, pid), b.n_pairs, b.cap_pairs, &b.flags[1], out);
-      });
+      const Strip S = strip_of(line_of(b, ref, t1, r));
+      const StripRows SR = strip_rows(S, b.cell_px, tolg);
+      const int32_t *cstart = b.cell_start + t1 * ncell;
+      const int32_t *ccnt = b.cell_cnt + t1 * ncell;
+      const int32_t *cpid = b.cell_pid + (long)t1 * b.cap;
+      const double2 *cxy = b.cell_xy + (long)t1 * b.cap;
+      for (int rb = 0; rb < rows; rb += kWave) {
+        const int cy = rb + lane;
+        int ka = 0, cnt = 0;
+        if (cy < rows) {
+          int a, bq;
+          if (row_span(SR, cy, cols, a, bq)) {
+            ka = cstart[cy * cols + a];
+            cnt = cstart[cy * cols + bq] + ccnt[cy * cols + bq] - ka;
+          }
+        }
+        int incl = cnt;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+          const int o = __shfl_up(incl, off, kWave);
+          if (lane >= off) incl += o;
+        }
+        const int total = __shfl(incl, kWave - 1, kWave);
+        wpre[threadIdx.x] = incl - cnt;
+        wka[threadIdx.x] = ka;
+        __builtin_amdgcn_wave_barrier();
+        for (int t0 = 0; t0 < total; t0 += kWave) {
+          const int jf = t0 + lane;
+          // the row: the last lane whose prefix is <= jf
+          int o = 0;
+#pragma unroll
+          for (int step = kWave / 2; step > 0; step >>= 1)
+            if (wpre[w0 + o + step] <= jf) o += step;
+          bool acc = jf < total;
+          int pid = -1;
+          if (acc) {
+            const int k = wka[w0 + o] + jf - wpre[w0 + o];
+            pid = cpid[k];
+            const double2 xy = cxy[k];
+            acc = in_strip(S, xy.x, xy.y, tol2);
+          }
+          bq_push_any(q, acc, make_int2(r, pid), b.n_pairs, b.cap_pairs, &b.flags[1], out);
+        }
+        __builtin_amdgcn_wave_barrier();   // the next pass rewrites this wave's LDS words
+      }
     }
     bq_drain(q, b.n_pairs, b.cap_pairs, &b.flags[1], false, out);
   }
