@@ -667,6 +667,14 @@ __global__ void __launch_bounds__(256) k_b_second(StereoBuffers bb, StereoParams
   }
 }
 
+// End of the CSR range of cells c0..c1 of a camera: the next cell's start
+// (cell_start is an exclusive scan, so it usually shares c0's cache line
+// and saves the count array's), the count for the camera's last cell.
+__device__ __forceinline__ int cell_range_end(const int32_t *cstart, const int32_t *ccnt,
+                                              long ncell, int c1) {
+  return c1 + 1 < ncell ? cstart[c1 + 1] : cstart[c1] + ccnt[c1];
+}
+
 // The points of cell rows r0..r1 of camera t that lie in all K strips, each
 // passed to push(accepted, pid).  kRowsPass rows at a time: their spans
 // (FP64 arithmetic), then their CSR ranges loaded together (within a row,
@@ -716,7 +724,7 @@ __device__ __forceinline__ void enum_rows(const StereoBuffers &b, int t, const S
       ke[u] = 0;
       if (c0[u] >= 0) {
         ka[u] = cstart[c0[u]];
-        ke[u] = cstart[c1[u]] + ccnt[c1[u]];
+        ke[u] = cell_range_end(cstart, ccnt, ncell, c1[u]);
       }
     }
 #pragma unroll
@@ -784,7 +792,7 @@ __global__ void __launch_bounds__(256) k_b_band(StereoBuffers bb, StereoParams p
           int a, bq;
           if (row_span(SR, cy, cols, a, bq)) {
             ka = cstart[cy * cols + a];
-            cnt = cstart[cy * cols + bq] + ccnt[cy * cols + bq] - ka;
+            cnt = cell_range_end(cstart, ccnt, ncell, cy * cols + bq) - ka;
           }
         }
         int incl = cnt;
