@@ -730,20 +730,18 @@ __device__ __forceinline__ void enum_rows(const StereoBuffers &b, int t, const S
 #pragma unroll
     for (int u = 0; u < kRowsPass; ++u) {
       for (int k = ka[u]; k < ke[u]; k += kPtBatch) {
-        int pid[kPtBatch];
+        // coordinates first; the id only of the points that pass (most
+        // visited points do not, and each id is a gather of its own)
         double2 q[kPtBatch];
 #pragma unroll
-        for (int v = 0; v < kPtBatch; ++v) {
-          const bool in = k + v < ke[u];
-          pid[v] = in ? cpid[k + v] : -1;
-          q[v] = in ? cxy[k + v] : make_double2(0.0, 0.0);
-        }
+        for (int v = 0; v < kPtBatch; ++v)
+          q[v] = k + v < ke[u] ? cxy[k + v] : make_double2(0.0, 0.0);
 #pragma unroll
         for (int v = 0; v < kPtBatch; ++v) {
-          bool acc = pid[v] >= 0;
+          bool acc = k + v < ke[u];
 #pragma unroll
           for (int j = 0; j < K; ++j) acc = acc && in_strip(S[j], q[v].x, q[v].y, tol2);
-          push(acc, pid[v]);
+          push(acc, acc ? cpid[k + v] : -1);
         }
       }
     }
@@ -816,9 +814,9 @@ __global__ void __launch_bounds__(256) k_b_band(StereoBuffers bb, StereoParams p
           int pid = -1;
           if (acc) {
             const int k = wka[w0 + o] + jf - wpre[w0 + o];
-            pid = cpid[k];
             const double2 xy = cxy[k];
             acc = in_strip(S, xy.x, xy.y, tol2);
+            if (acc) pid = cpid[k];
           }
           bq_push_any(q, acc, make_int2(r, pid), b.n_pairs, b.cap_pairs, &b.flags[1], out);
         }
