@@ -795,6 +795,7 @@ olpt_status alloc_workspace(olpt_ctx *c, Workspace &w) {
   A(b.cell_cnt, int32_t, ncell * nc);
   A(b.cell_start, int32_t, ncell * nc);
   A(b.cell_pid, int32_t, (long)nc * cap);
+  A(b.out_cnt, int32_t, nc);
   A(b.cell_xy, double2, (long)nc * cap);
   A(b.pt_cell, int32_t, (long)nc * cap);
   A(b.pt_rank, int32_t, (long)nc * cap);

@@ -127,8 +127,11 @@ struct StereoBuffers {
   const CamT<double> *cams;
   const double *xy;      // [n_cam][cap][2]
   const int32_t *npts;   // [n_cam]
+  // the build numbers each camera's points by their position in cell order
+  // (cell_xy / cell_pid), the points outside the cell grid last; los and
+  // lines are indexed by that position
   double *los;           // [n_cam][cap][4]  unit direction (w unused)
-  double *lines;         // [n_cam][n_cam][cap][4] LOS (c, pid) on camera t
+  double *lines;         // [n_cam][n_cam][cap][4] LOS (c, pos) on camera t
   // build
   int8_t *t1;            // [cap] second build camera per reference observation
   long cap_pairs;
@@ -147,6 +150,7 @@ struct StereoBuffers {
   double2 *cell_xy;      // [n_cam][cap] the points' coordinates in cell_pid order
   int32_t *pt_cell;      // [n_cam][cap]
   int32_t *pt_rank;      // [n_cam][cap]
+  int32_t *out_cnt;      // [n_cam] points outside the cell grid
   // candidates
   int cap_cand;
   int32_t *cand_ids;     // [cap_cand][n_cam] raw (emission order)
@@ -235,6 +239,7 @@ struct StereoBuffers {
     o.cell_xy = fshift(cell_xy, d);
     o.pt_cell = fshift(pt_cell, d);
     o.pt_rank = fshift(pt_rank, d);
+    o.out_cnt = fshift(out_cnt, d);
     o.cand_ids = fshift(cand_ids, d);
     o.cand_e = fshift(cand_e, d);
     o.n_cand = fshift(n_cand, d);

@@ -81,14 +81,27 @@ __global__ void k_cell_count(StereoBuffers bb, StereoParams p) {
     const double *q = b.xy + ((long)cam * b.cap + i) * 2;
     const int cx = (int)floor(q[0] / b.cell_px);
     const int cy = (int)floor(q[1] / b.cell_px);
-    int cell = -1, rank = 0;
+    int cell = -1, rank;
     if (!(cx < 0 || cy < 0 || cx >= b.cols_cell || cy >= b.rows_cell)) {
       cell = cy * b.cols_cell + cx;
       rank = atomicAdd(&b.cell_cnt[cam * ncell + cell], 1);
+    } else {
+      rank = atomicAdd(&b.out_cnt[cam], 1);
     }
     b.pt_cell[(long)cam * b.cap + i] = cell;
     b.pt_rank[(long)cam * b.cap + i] = rank;
   }
+}
+
+// A point's position in its camera's cell order: its cell's start plus its
+// rank in the cell; points outside the grid follow the last cell's.
+__device__ __forceinline__ int point_pos(const StereoBuffers &b, int cam, int i) {
+  const long ncell = (long)b.rows_cell * b.cols_cell;
+  const int cell = b.pt_cell[(long)cam * b.cap + i];
+  const int rank = b.pt_rank[(long)cam * b.cap + i];
+  if (cell >= 0) return b.cell_start[cam * ncell + cell] + rank;
+  const long last = cam * ncell + ncell - 1;
+  return b.cell_start[last] + b.cell_cnt[last] + rank;
 }
 
 __global__ void k_cell_fill(StereoBuffers bb, StereoParams p) {
@@ -96,12 +109,9 @@ __global__ void k_cell_fill(StereoBuffers bb, StereoParams p) {
   const int cam = blockIdx.y;
   if (!p.active[cam]) return;
   const int n = b.npts[cam];
-  const long ncell = (long)b.rows_cell * b.cols_cell;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x) {
-    const int cell = b.pt_cell[(long)cam * b.cap + i];
-    if (cell < 0) continue;
-    const int pos = b.cell_start[cam * ncell + cell] + b.pt_rank[(long)cam * b.cap + i];
+    const int pos = point_pos(b, cam, i);
     b.cell_pid[(long)cam * b.cap + pos] = i;
     const double *q = b.xy + ((long)cam * b.cap + i) * 2;
     b.cell_xy[(long)cam * b.cap + pos] = make_double2(q[0], q[1]);
@@ -129,6 +139,8 @@ __device__ __forceinline__ Line2 make_line2d(const CamT<double> &c,
   return Line2{a[0], a[1], d0 / n, d1 / n};
 }
 
+// One FP64 line of sight per point and its 2D line on every other camera, in
+// the cell order of the point's camera (i is the position).
 __global__ void k_los(StereoBuffers bb, StereoParams p) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   const int cam = blockIdx.y;
@@ -137,9 +149,9 @@ __global__ void k_los(StereoBuffers bb, StereoParams p) {
   const CamT<double> c = b.cams[cam];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x) {
-    const double *q = b.xy + ((long)cam * b.cap + i) * 2;
+    const double2 q = b.cell_xy[(long)cam * b.cap + i];
     double u[3];
-    line_of_sight(c, q[0], q[1], u);
+    line_of_sight(c, q.x, q.y, u);
     double *o = b.los + ((long)cam * b.cap + i) * 4;
     o[0] = u[0];
     o[1] = u[1];
@@ -491,13 +503,14 @@ struct Path {
   double U[kMaxCam][3];  // LOS unit direction
 };
 
+// pid: the point's position in its camera's cell order
 __device__ __forceinline__ void path_set(Path &P, int k, const StereoBuffers &b, int cam,
                                          int pid) {
   P.cam[k] = cam;
   P.pid[k] = pid;
-  const double *q = b.xy + ((long)cam * b.cap + pid) * 2;
-  P.pt[k][0] = q[0];
-  P.pt[k][1] = q[1];
+  const double2 q = b.cell_xy[(long)cam * b.cap + pid];
+  P.pt[k][0] = q.x;
+  P.pt[k][1] = q.y;
   const double *u = b.los + ((long)cam * b.cap + pid) * 4;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -689,7 +702,6 @@ __device__ __forceinline__ void enum_rows(const StereoBuffers &b, int t, const S
   const long ncell = (long)b.rows_cell * cols;
   const int32_t *cstart = b.cell_start + t * ncell;
   const int32_t *ccnt = b.cell_cnt + t * ncell;
-  const int32_t *cpid = b.cell_pid + (long)t * b.cap;
   const double2 *cxy = b.cell_xy + (long)t * b.cap;
   for (int rb = r0; rb <= r1; rb += kRowsPass) {
     int c0[kRowsPass], c1[kRowsPass];
@@ -741,7 +753,7 @@ __device__ __forceinline__ void enum_rows(const StereoBuffers &b, int t, const S
           bool acc = k + v < ke[u];
 #pragma unroll
           for (int j = 0; j < K; ++j) acc = acc && in_strip(S[j], q[v].x, q[v].y, tol2);
-          push(acc, acc ? cpid[k + v] : -1);
+          push(acc, k + v);
         }
       }
     }
@@ -781,7 +793,6 @@ __global__ void __launch_bounds__(256) k_b_band(StereoBuffers bb, StereoParams p
       const StripRows SR = strip_rows(S, b.cell_px, tolg);
       const int32_t *cstart = b.cell_start + t1 * ncell;
       const int32_t *ccnt = b.cell_cnt + t1 * ncell;
-      const int32_t *cpid = b.cell_pid + (long)t1 * b.cap;
       const double2 *cxy = b.cell_xy + (long)t1 * b.cap;
       for (int rb = 0; rb < rows; rb += kWave) {
         const int cy = rb + lane;
@@ -816,7 +827,7 @@ __global__ void __launch_bounds__(256) k_b_band(StereoBuffers bb, StereoParams p
             const int k = wka[w0 + o] + jf - wpre[w0 + o];
             const double2 xy = cxy[k];
             acc = in_strip(S, xy.x, xy.y, tol2);
-            if (acc) pid = cpid[k];
+            pid = k;
           }
           bq_push_any(q, acc, make_int2(r, pid), b.n_pairs, b.cap_pairs, &b.flags[1], out);
         }
@@ -944,7 +955,8 @@ k_b_extend(StereoBuffers bb, StereoParams p) {
           int id = -1;
 #pragma unroll
           for (int j = 0; j <= K; ++j) id = P.cam[j] == c ? P.pid[j] : id;
-          cr.ids[c] = c < nc ? id : -1;
+          // the candidate carries the points' original indices
+          cr.ids[c] = c < nc && id >= 0 ? b.cell_pid[(long)c * b.cap + id] : -1;
         }
         cr.e = e;
       }
@@ -1911,7 +1923,7 @@ __global__ void __launch_bounds__(256) k_tri(StereoBuffers bb, StereoParams p) {
 #pragma unroll
       for (int c = 0; c < MC; ++c) {
         if (pidc[c] < 0) continue;
-        const double *uu = b.los + ((long)c * b.cap + pidc[c]) * 4;
+        const double *uu = b.los + ((long)c * b.cap + point_pos(b, c, pidc[c])) * 4;
         double tc[3], uc[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -2042,11 +2054,12 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   zero_i32(b, b.cell_cnt, ncell * b.n_cam, st);
   // LOS + cell index
   dim3 gc(grid_for(b.cap, T, 1024), b.n_cam, b.nf);
-  k_los<<<gc, T, 0, st>>>(b, p);
+  zero_i32(b, b.out_cnt, b.n_cam, st);
   k_cell_count<<<gc, T, 0, st>>>(b, p);
   scan_exclusive(b.cell_cnt, b.cell_start, ncell, b.n_cam, ncell, nullptr,
                  b.scan_scratch, st, b.nf, b.fstride);
   k_cell_fill<<<gc, T, 0, st>>>(b, p);
+  k_los<<<gc, T, 0, st>>>(b, p);
   // build: level 1 from every reference observation, then one enumerate +
   // extend pair of kernels per deeper build camera
   const int G = launch_blocks(b.grid_build, b.nf);
