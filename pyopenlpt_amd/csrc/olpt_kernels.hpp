@@ -172,7 +172,7 @@ struct StereoBuffers {
   int32_t *pt_occ;       // [cap_cand*n_cam] position occupancy (rank check)
   int32_t *ppos;         // [cap_cand][n_cam] index of the entry in pt_sorted (its point's
                          // priority-sorted run)
-  int32_t *pt_flag;      // [n_cam*cap] bit0/bit1: sequential percentile/priority
+  int32_t *pt_flag;      // [n_cam*cap] bit1: priority order redone sequentially
   int32_t *head;         // [n_cam*cap] first entry not known to be rejected (pt_sorted index)
   // per flat point its selected candidate, if any: (pt_sorted index + 1) << 32 |
   // candidate, 0 = none (written by the greedy as it selects)

@@ -29,9 +29,11 @@
 //                 pruneMatch (:631-980) + triangulateMatch (:985-1050).
 //                 Per-point orders (by e_check for the percentiles, by the
 //                 greedy's priority for the packing) are rank counts, one
-//                 thread per list entry; ranks that form a permutation are
-//                 the unique order, and the rare point where they do not
-//                 (eq_eps ties) is redone sequentially.  The greedy packing
+//                 thread per list entry: an eq_eps block's average rank
+//                 follows from the entry's counts of lower and tied
+//                 entries, and priority ranks that form a permutation are
+//                 the unique order (the rare point where they do not is
+//                 redone sequentially).  The greedy packing
 //                 in (pct, e_check, index) order equals the greedy maximal
 //                 independent set of the conflict graph under that priority,
 //                 computed in rounds over the priority-sorted point lists: a
@@ -1292,9 +1294,11 @@ __device__ __forceinline__ void tile_lists(const StereoBuffers &b, int a, int a1
 // Percentile of each candidate on each used 2D point (StereoMatch.cpp:
 // 727-800): the point's candidates sorted by e_check (eq_eps ties by
 // index), eq_eps blocks take their average rank, pv = rank / (n - 1).
-// One thread per list entry counts the entries below it.  An entry with no
-// eq_eps partner forms a block of its own at that rank; a point holding any
-// eq_eps pair is flagged and redone by k_pct_fallback.
+// One thread per list entry counts the entries below it (r) and its eq_eps
+// partners (t).  Where eq_eps is an equivalence on the frame's values (the
+// condition k_hz_* verify, flags[2] otherwise) the blocks are its classes:
+// the entry's block holds it and its t partners and starts after the r
+// entries of lower blocks, so its average rank is (r + (r + t)) / 2.
 __global__ void __launch_bounds__(256) k_pct_rank(StereoBuffers bb) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   const int ne = *b.n_entries;
@@ -1311,8 +1315,7 @@ __global__ void __launch_bounds__(256) k_pct_rank(StereoBuffers bb) {
       const int f = b.pt_of[q];
       const int s0 = b.pt_start[f], n = b.pt_count[f];
       const double ej = b.e_ent[q];
-      int r = 0;
-      bool tie = false;
+      int r = 0, nt = 0;
       // the point's entries are e_ent[s0, s0 + n); a candidate holds at
       // most one entry per point, so the entry itself is skipped by position
       if (lds) {
@@ -1321,7 +1324,7 @@ __global__ void __launch_bounds__(256) k_pct_rank(StereoBuffers bb) {
           if (s0 + t == q) continue;
           const double em = le[t];
           if (eq_eps(em, ej))
-            tie = true;
+            ++nt;
           else
             r += em < ej ? 1 : 0;
         }
@@ -1334,64 +1337,18 @@ __global__ void __launch_bounds__(256) k_pct_rank(StereoBuffers bb) {
           for (int u = 0; u < kRankBatch; ++u) {
             if (t0 + u >= n || s0 + t0 + u == q) continue;
             if (eq_eps(em[u], ej))
-              tie = true;
+              ++nt;
             else
               r += em[u] < ej ? 1 : 0;
           }
         }
       }
-      if (tie) {
-        atomicOr(&b.pt_flag[f], 1);
-        continue;
-      }
       const int32_t v = b.pt_list[q];
       const double denom = (double)(n - 1);
-      const double avg_rank = 0.5 * (double)(r + r);
+      const double avg_rank = 0.5 * (double)(r + (r + nt));
       b.pv[(long)ent_cand(v) * b.n_cam + ent_slot(v)] = (denom > 0.0) ? (avg_rank / denom) : 0.0;
     }
     __syncthreads();
-  }
-}
-
-// Sequential statement of the percentile for flagged points: insertion sort
-// by (e eq_eps, index), then eq_eps blocks from each block's first entry.
-__global__ void k_pct_fallback(StereoBuffers bb) {
-  const StereoBuffers b = bb.frame(blockIdx.z);
-  const long total = (long)b.n_cam * b.cap;
-  for (long f = blockIdx.x * (long)blockDim.x + threadIdx.x; f < total;
-       f += (long)gridDim.x * blockDim.x) {
-    if (!(b.pt_flag[f] & 1)) continue;
-    const int n = b.pt_count[f];
-    // sorted in a copy (pt_sorted is scratch until k_prio_rank fills it):
-    // pt_list keeps the order e_ent refers to
-    int32_t *lst = b.pt_sorted + b.pt_start[f];
-    for (int i = 0; i < n; ++i) lst[i] = b.pt_list[b.pt_start[f] + i];
-    for (int i = 1; i < n; ++i) {
-      const int32_t cur = lst[i];
-      const double ce = b.cand_e_s[ent_cand(cur)];
-      int k = i - 1;
-      while (k >= 0) {
-        const int32_t o = lst[k];
-        const double oe = b.cand_e_s[ent_cand(o)];
-        const bool less = !eq_eps(ce, oe) ? (ce < oe) : (ent_cand(cur) < ent_cand(o));
-        if (!less) break;
-        lst[k + 1] = o;
-        --k;
-      }
-      lst[k + 1] = cur;
-    }
-    int k = 0;
-    while (k < n) {
-      int k0 = k, k1 = k + 1;
-      const double e0 = b.cand_e_s[ent_cand(lst[k0])];
-      while (k1 < n && eq_eps(b.cand_e_s[ent_cand(lst[k1])], e0)) ++k1;
-      const double avg_rank = 0.5 * (double)(k0 + (k1 - 1));
-      const double denom = (double)(n - 1);
-      const double pv = (denom > 0.0) ? (avg_rank / denom) : 0.0;
-      for (int t = k0; t < k1; ++t)
-        b.pv[(long)ent_cand(lst[t]) * b.n_cam + ent_slot(lst[t])] = pv;
-      k = k1;
-    }
   }
 }
 
@@ -2025,7 +1982,6 @@ static void prune_section(const StereoBuffers &b, const StereoParams &p, int gM,
   k_zero_entries<<<dim3(gE, 1, b.nf), T, 0, st>>>(b);
   const int gR = grid_for((long)b.cap_cand * 4, kRankTile, 4096);
   k_pct_rank<<<dim3(gR, 1, b.nf), T, 0, st>>>(b);
-  k_pct_fallback<<<dim3(grid_for(nflat, T, 2048), 1, b.nf), T, 0, st>>>(b);
   k_pct_final<<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p);
   k_pct_ent<<<dim3(gE, 1, b.nf), T, 0, st>>>(b);
   k_hz_insert<<<dim3(gM, 1, b.nf), T, 0, st>>>(b);
