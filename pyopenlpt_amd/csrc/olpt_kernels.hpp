@@ -109,6 +109,8 @@ inline long path_rec_bytes(int n_cam) {
 // at most kGrpMaxBuckets of them; kGrpBlocks workgroups in the first pass
 constexpr int kGrpBlocks = 1024;
 constexpr int kGrpMaxBuckets = 4096;
+// max_2d <= 2^20 and at most 8 cameras give nflat <= 2^23, so low <= 12:
+// k_grp_sort's LDS counters are sized for that bound
 constexpr int kGrpMaxLow = 12;
 inline int grp_low(long nflat) {
   int low = 9;

@@ -1281,7 +1281,10 @@ __global__ void k_zero_entries(StereoBuffers bb) {
 // from global memory as before.  Counts do not depend on where the keys are
 // read from.
 constexpr int kRankTile = 512;
-constexpr int kRankLds = 2048;
+// k_prio_rank stages three arrays of this many entries (20 KB): eight
+// workgroups fit a CU's LDS, its thread limit (2048 entries took 40 KB and
+// halved the resident workgroups: -26% time at 1024)
+constexpr int kRankLds = 1024;
 
 // the run of lists [A, B) holding entries [a, a1) (a < a1)
 __device__ __forceinline__ void tile_lists(const StereoBuffers &b, int a, int a1, int &A,
