@@ -1859,60 +1859,91 @@ void stereo_init_tables(const StereoBuffers &b, hipStream_t st) {
 // cameras are gathered into slots 0..n-1 in camera order by selects over
 // compile-time slot and camera indices (MC >= n_cam), so the LOS arrays stay
 // in registers.
+// One selected candidate: its LOS in camera order, triangulated; keep[j]
+// and tri_xyz[j] written.
+template <int MC>
+__device__ __forceinline__ void tri_one(const StereoBuffers &b, const StereoParams &p, int j) {
+  const int nc = b.n_cam;
+  int keep = 0;
+  int pidc[MC];
+#pragma unroll
+  for (int c = 0; c < MC; ++c) pidc[c] = c < nc ? b.cand_ids_s[(long)j * nc + c] : -1;
+  double pt[MC][3], u[MC][3];
+#pragma unroll
+  for (int s = 0; s < MC; ++s)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) pt[s][d] = u[s][d] = 0.0;
+  int n = 0;
+#pragma unroll
+  for (int c = 0; c < MC; ++c) {
+    if (pidc[c] < 0) continue;
+    const double *uu = b.los + ((long)c * b.cap + point_pos(b, c, pidc[c])) * 4;
+    double tc[3], uc[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      tc[d] = b.cams[c].Tinv[d];
+      uc[d] = uu[d];
+    }
+#pragma unroll
+    for (int s = 0; s <= c; ++s)
+      if (s == n) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          pt[s][d] = tc[d];
+          u[s][d] = uc[d];
+        }
+      }
+    ++n;
+  }
+  double X[3], e;
+  if (n >= 2 && triangulate_upto<MC>(pt, u, n, X, e) && !(e > p.tol3d)) {
+    keep = 1;
+    b.tri_xyz[(long)j * 3 + 0] = X[0];
+    b.tri_xyz[(long)j * 3 + 1] = X[1];
+    b.tri_xyz[(long)j * 3 + 2] = X[2];
+  }
+  b.keep[j] = keep;
+}
+
+// A wave sweeps 64 candidates at a time: the unselected get keep = 0 (flags
+// past M too, the keep flags are scanned over cap_cand) and the selected
+// (about one in twelve) join the wave's LDS queue, which is triangulated 64
+// at a time with every lane busy instead of a few lanes of each sweep step.
 template <int MC>
 __global__ void __launch_bounds__(256) k_tri(StereoBuffers bb, StereoParams p) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   const int M = min(*b.n_cand, b.cap_cand);
-  const int nc = b.n_cam;
-  int nsel = 0;
-  // entries past M are cleared: the keep flags are scanned over cap_cand
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < b.cap_cand;
-       j += gridDim.x * blockDim.x) {
-    int keep = 0;
-    if (j < M && b.state[j] == 1) {
-      ++nsel;
-      int pidc[MC];
-#pragma unroll
-      for (int c = 0; c < MC; ++c) pidc[c] = c < nc ? b.cand_ids_s[(long)j * nc + c] : -1;
-      double pt[MC][3], u[MC][3];
-#pragma unroll
-      for (int s = 0; s < MC; ++s)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) pt[s][d] = u[s][d] = 0.0;
-      int n = 0;
-#pragma unroll
-      for (int c = 0; c < MC; ++c) {
-        if (pidc[c] < 0) continue;
-        const double *uu = b.los + ((long)c * b.cap + point_pos(b, c, pidc[c])) * 4;
-        double tc[3], uc[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          tc[d] = b.cams[c].Tinv[d];
-          uc[d] = uu[d];
-        }
-#pragma unroll
-        for (int s = 0; s <= c; ++s)
-          if (s == n) {
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-              pt[s][d] = tc[d];
-              u[s][d] = uc[d];
-            }
-          }
-        ++n;
-      }
-      double X[3], e;
-      if (n >= 2 && triangulate_upto<MC>(pt, u, n, X, e) && !(e > p.tol3d)) {
-        keep = 1;
-        b.tri_xyz[(long)j * 3 + 0] = X[0];
-        b.tri_xyz[(long)j * 3 + 1] = X[1];
-        b.tri_xyz[(long)j * 3 + 2] = X[2];
-      }
+  __shared__ int queue[256 / kWave][2 * kWave];
+  int *qw = queue[threadIdx.x / kWave];
+  const int lane = lane_id();
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const long nwave = (long)gridDim.x * blockDim.x / kWave;
+  int cnt = 0, nsel = 0;
+  for (long c0 = wave * kWave; c0 < b.cap_cand; c0 += nwave * kWave) {
+    const int j = (int)c0 + lane;
+    const bool sel = j < M && b.state[j] == 1;
+    if (j < b.cap_cand && !sel) b.keep[j] = 0;
+    const uint64_t m = ballot(sel);
+    if (sel) qw[cnt + mbcnt(m)] = j;
+    cnt += __popcll(m);
+    nsel += sel ? 1 : 0;
+    if (cnt >= kWave) {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      tri_one<MC>(b, p, qw[lane]);
+      const int rest = cnt - kWave;
+      const int v = lane < rest ? qw[kWave + lane] : 0;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (lane < rest) qw[lane] = v;
+      cnt = rest;
     }
-    b.keep[j] = keep;
   }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (lane < cnt) tri_one<MC>(b, p, qw[lane]);
   nsel = wave_sum(nsel);
-  if (lane_id() == 0 && nsel) atomicAdd(b.n_sel, nsel);
+  if (lane == 0 && nsel) atomicAdd(b.n_sel, nsel);
 }
 
 __global__ void k_obj_write(StereoBuffers bb) {
