@@ -1546,17 +1546,42 @@ __device__ __forceinline__ int mis_decide(const StereoBuffers &b, const StereoPa
   return d;
 }
 
-__global__ void k_mis_round(StereoBuffers bb, StereoParams p, int round) {
+// After the first round most candidates are decided: a wave sweeps 64
+// states at a time, queues the undecided in LDS and takes them 64 at a time,
+// every lane on a candidate, instead of stepping over decided ones.
+__global__ void __launch_bounds__(256) k_mis_round(StereoBuffers bb, StereoParams p, int round) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   if (round > 0 && b.undecided[round - 1] == 0) return;
   const int M = min(*b.n_cand, b.cap_cand);
-  int und = 0;
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < M; j += gridDim.x * blockDim.x) {
-    if (b.state[j] != 0) continue;
-    if (mis_decide(b, p, j) == 0) ++und;
+  __shared__ int queue[256 / kWave][2 * kWave];
+  int *qw = queue[threadIdx.x / kWave];
+  const int lane = lane_id();
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const long nwave = (long)gridDim.x * blockDim.x / kWave;
+  int cnt = 0, und = 0;
+  for (long c0 = wave * kWave; c0 < M; c0 += nwave * kWave) {
+    const int j = (int)c0 + lane;
+    const bool pend = j < M && b.state[j] == 0;
+    const uint64_t m = ballot(pend);
+    if (pend) qw[cnt + mbcnt(m)] = j;
+    cnt += __popcll(m);
+    if (cnt >= kWave) {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (mis_decide(b, p, qw[lane]) == 0) ++und;
+      const int rest = cnt - kWave;
+      const int v = lane < rest ? qw[kWave + lane] : 0;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (lane < rest) qw[lane] = v;
+      cnt = rest;
+    }
   }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (lane < cnt && mis_decide(b, p, qw[lane]) == 0) ++und;
   und = wave_sum(und);
-  if (lane_id() == 0 && und) atomicAdd(&b.undecided[round], und);
+  if (lane == 0 && und) atomicAdd(&b.undecided[round], und);
 }
 
 // The greedy's first round: only a candidate that is first on every one of
