@@ -36,13 +36,19 @@ __device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t *lds,
 
 __global__ void __launch_bounds__(kScanThreads)
 k_tile_sums(const int32_t *__restrict__ in, long n, long stride,
-            int32_t *__restrict__ tsum, long tiles, long fstride) {
+            int32_t *__restrict__ tsum, long tiles, long fstride, const int32_t *n_dev) {
   __shared__ int32_t lds[kScanWaves];
   const long fo = (long)blockIdx.z * fstride;
   in = fshift(in, fo);
   tsum = fshift(tsum, fo);
   const int seg = blockIdx.y;
   const long tile = blockIdx.x;
+  if (n_dev) n = min(n, (long)max(0, *fshift(n_dev, fo)));
+  if (tile * kScanTile >= n) {
+    // past the frame's count: an empty tile
+    if (threadIdx.x == 0) tsum[seg * (tiles + 1) + tile] = 0;
+    return;
+  }
   const int32_t *p = in + seg * stride;
   const long base = tile * kScanTile + (long)threadIdx.x * kScanPerThread;
   int32_t s = 0;
@@ -84,7 +90,8 @@ k_scan_tile_sums(int32_t *__restrict__ tsum, long tiles,
 template <bool CLEAR>
 __global__ void __launch_bounds__(kScanThreads)
 k_tile_scan(int32_t *__restrict__ in, int32_t *__restrict__ out, long n,
-            long stride, const int32_t *__restrict__ tsum, long tiles, long fstride) {
+            long stride, const int32_t *__restrict__ tsum, long tiles, long fstride,
+            const int32_t *n_dev) {
   __shared__ int32_t lds[kScanWaves];
   const long fo = (long)blockIdx.z * fstride;
   in = fshift(in, fo);
@@ -92,6 +99,8 @@ k_tile_scan(int32_t *__restrict__ in, int32_t *__restrict__ out, long n,
   tsum = fshift(tsum, fo);
   const int seg = blockIdx.y;
   const long tile = blockIdx.x;
+  if (n_dev) n = min(n, (long)max(0, *fshift(n_dev, fo)));
+  if (tile * kScanTile >= n) return;  // uniform: the whole workgroup leaves
   int32_t *p = in + seg * stride;
   int32_t *q = out + seg * stride;
   const long base = tile * kScanTile + (long)threadIdx.x * kScanPerThread;
@@ -120,15 +129,15 @@ k_tile_scan(int32_t *__restrict__ in, int32_t *__restrict__ out, long n,
 
 void scan_exclusive(const int32_t *in, int32_t *out, long n, int segments,
                     long seg_stride, int32_t *total, int32_t *scratch,
-                    hipStream_t st, int nf, long fstride) {
+                    hipStream_t st, int nf, long fstride, const int32_t *n_dev) {
   if (n <= 0 || segments <= 0 || nf <= 0) return;
   const long tiles = (n + kScanTile - 1) / kScanTile;
   dim3 g((unsigned)tiles, (unsigned)segments, (unsigned)nf);
-  k_tile_sums<<<g, kScanThreads, 0, st>>>(in, n, seg_stride, scratch, tiles, fstride);
+  k_tile_sums<<<g, kScanThreads, 0, st>>>(in, n, seg_stride, scratch, tiles, fstride, n_dev);
   k_scan_tile_sums<<<dim3(segments, 1, nf), kScanThreads, 0, st>>>(scratch, tiles, total,
                                                                     fstride);
   k_tile_scan<false><<<g, kScanThreads, 0, st>>>(const_cast<int32_t *>(in), out, n, seg_stride,
-                                                 scratch, tiles, fstride);
+                                                 scratch, tiles, fstride, n_dev);
 }
 
 void scan_exclusive_clear(int32_t *in, int32_t *out, long n, int32_t *total, int32_t *scratch,
@@ -136,9 +145,9 @@ void scan_exclusive_clear(int32_t *in, int32_t *out, long n, int32_t *total, int
   if (n <= 0 || nf <= 0) return;
   const long tiles = (n + kScanTile - 1) / kScanTile;
   dim3 g((unsigned)tiles, 1, (unsigned)nf);
-  k_tile_sums<<<g, kScanThreads, 0, st>>>(in, n, n, scratch, tiles, fstride);
+  k_tile_sums<<<g, kScanThreads, 0, st>>>(in, n, n, scratch, tiles, fstride, nullptr);
   k_scan_tile_sums<<<dim3(1, 1, nf), kScanThreads, 0, st>>>(scratch, tiles, total, fstride);
-  k_tile_scan<true><<<g, kScanThreads, 0, st>>>(in, out, n, n, scratch, tiles, fstride);
+  k_tile_scan<true><<<g, kScanThreads, 0, st>>>(in, out, n, n, scratch, tiles, fstride, nullptr);
 }
 
 } // namespace olpt

@@ -21,10 +21,14 @@ inline size_t scan_scratch_ints(long n, int segments) {
 // Exclusive scan of `segments` arrays of length n (stride `seg_stride`):
 //   out[s][i] = sum_{k<i} in[s][k];  total[s] = sum of all.
 // in and out may alias.  total may be null.  Over a batch of nf frames every
-// pointer is frame 0's and frame f's lies f * fstride bytes further.
+// pointer is frame 0's and frame f's lies f * fstride bytes further.  With
+// n_dev (a device count per frame, frame 0's pointer like the others) only
+// the first min(n, *n_dev) entries are read and written: the rest of out is
+// left as it was.
 void scan_exclusive(const int32_t *in, int32_t *out, long n, int segments,
                     long seg_stride, int32_t *total, int32_t *scratch,
-                    hipStream_t st, int nf = 1, long fstride = 0);
+                    hipStream_t st, int nf = 1, long fstride = 0,
+                    const int32_t *n_dev = nullptr);
 
 // One segment, and in[] is left all zero (in and out must not alias): the
 // counts of a count -> scan -> fill pass are ready for the next pass.

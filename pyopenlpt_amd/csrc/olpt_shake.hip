@@ -1368,7 +1368,8 @@ void launch_shake(const ShakeBuffers<T> &b, const ShakeParams &p, hipStream_t st
   scan_exclusive_clear(b.g_cnt, b.g_start, ncell, b.g_start + ncell, b.scan_scratch, st, b.nf, b.fstride);
   k_grid_fill<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b);
   k_has_nb<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b, t2);
-  scan_exclusive(b.has_nb, b.g_rank, b.cap_obj, 1, b.cap_obj, b.totals, b.scan_scratch, st, b.nf, b.fstride);
+  scan_exclusive(b.has_nb, b.g_rank, b.cap_obj, 1, b.cap_obj, b.totals, b.scan_scratch, st, b.nf,
+                 b.fstride, b.n_obj);
   k_nb_list<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b);
   k_repeat_greedy<T><<<dim3(1, 1, b.nf), 64, 0, st>>>(b, t2);
   // ghost threshold: mean over non-repeated scores
@@ -1428,7 +1429,7 @@ void launch_filter(const ShakeBuffers<T> &b, double *xyz_keep, T *uv_keep,
   const int gO = grid_for(b.cap_obj, 256, 2048);
   k_keep_flags<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b);
   scan_exclusive(b.has_nb, b.keep_off, b.cap_obj, 1, b.cap_obj, b.totals,
-                 b.scan_scratch, st, b.nf, b.fstride);
+                 b.scan_scratch, st, b.nf, b.fstride, b.n_obj);
   k_keep_write<T><<<dim3(gO, 1, b.nf), 256, 0, st>>>(b, b.has_nb, b.keep_off, xyz_keep, uv_keep);
   k_keep_count<T><<<dim3(1, 1, b.nf), 64, 0, st>>>(b.totals, n_keep, b.fstride);
 }

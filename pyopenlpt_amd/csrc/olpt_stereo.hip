@@ -1930,8 +1930,8 @@ __device__ __forceinline__ void tri_one(const StereoBuffers &b, const StereoPara
   b.keep[j] = keep;
 }
 
-// A wave sweeps 64 candidates at a time: the unselected get keep = 0 (flags
-// past M too, the keep flags are scanned over cap_cand) and the selected
+// A wave sweeps 64 candidates at a time: the unselected get keep = 0 (the
+// keep flags are scanned over the first n_cand entries) and the selected
 // (about one in twelve) join the wave's LDS queue, which is triangulated 64
 // at a time with every lane busy instead of a few lanes of each sweep step.
 template <int MC>
@@ -1944,10 +1944,10 @@ __global__ void __launch_bounds__(256) k_tri(StereoBuffers bb, StereoParams p) {
   const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const long nwave = (long)gridDim.x * blockDim.x / kWave;
   int cnt = 0, nsel = 0;
-  for (long c0 = wave * kWave; c0 < b.cap_cand; c0 += nwave * kWave) {
+  for (long c0 = wave * kWave; c0 < M; c0 += nwave * kWave) {
     const int j = (int)c0 + lane;
     const bool sel = j < M && b.state[j] == 1;
-    if (j < b.cap_cand && !sel) b.keep[j] = 0;
+    if (j < M && !sel) b.keep[j] = 0;
     const uint64_t m = ballot(sel);
     if (sel) qw[cnt + mbcnt(m)] = j;
     cnt += __popcll(m);
@@ -2107,7 +2107,7 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   else
     k_tri<kMaxCam><<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p);
   scan_exclusive(b.keep, b.keep_off, b.cap_cand, 1, b.cap_cand, b.totals,
-                 b.scan_scratch, st, b.nf, b.fstride);
+                 b.scan_scratch, st, b.nf, b.fstride, b.n_cand);
   k_obj_write<<<dim3(gM, 1, b.nf), T, 0, st>>>(b);
   k_obj_count<<<dim3(1, 1, b.nf), 64, 0, st>>>(b);
   stage_end(ST_PRUNE, st);
