@@ -107,7 +107,11 @@ inline long path_rec_bytes(int n_cam) {
 
 // point-list grouping (prune): buckets of 2^low consecutive flat points,
 // at most kGrpMaxBuckets of them; kGrpBlocks workgroups in the first pass
+// over a whole launch (kGrpBlocks / nf per frame, at least kGrpMinBlocks):
+// the (bucket, workgroup) counts it scans grow with the workgroups, so 16
+// frames at 64 each beat 1,024 each (-1.2% of the step)
 constexpr int kGrpBlocks = 1024;
+constexpr int kGrpMinBlocks = 64;
 constexpr int kGrpMaxBuckets = 4096;
 // max_2d <= 2^20 and at most 8 cameras give nflat <= 2^23, so low <= 12:
 // k_grp_sort's LDS counters are sized for that bound

@@ -2018,7 +2018,7 @@ GrpShape grp_shape(const StereoBuffers &b) {
   GrpShape g;
   g.low = grp_low(nflat);
   g.nb = (int)grp_buckets(nflat);
-  g.g1 = kGrpBlocks;
+  g.g1 = std::max(kGrpMinBlocks, kGrpBlocks / std::max(1, b.nf));
   return g;
 }
 
