@@ -50,11 +50,12 @@ k_tile_sums(const int32_t *__restrict__ in, long n, long stride,
     return;
   }
   const int32_t *p = in + seg * stride;
-  const long base = tile * kScanTile + (long)threadIdx.x * kScanPerThread;
+  // a sum: lanes read consecutive entries (coalesced)
+  const long base = tile * kScanTile + threadIdx.x;
   int32_t s = 0;
 #pragma unroll
   for (int k = 0; k < kScanPerThread; ++k) {
-    const long i = base + k;
+    const long i = base + (long)k * kScanThreads;
     if (i < n) s += p[i];
   }
   int32_t tot;
@@ -86,7 +87,14 @@ k_scan_tile_sums(int32_t *__restrict__ tsum, long tiles,
 
 // CLEAR: the input is zeroed once read (in and out distinct), so a count
 // array is ready for its next round of atomic increments without a launch
-// of its own.
+// of its own.  The tile goes through LDS: loaded and stored with lanes on
+// consecutive entries (coalesced), scanned with each thread on 16
+// consecutive ones; entry e sits at e + e / 16, so a thread's run starts 17
+// words after the previous thread's and the 64 lanes of a wave hit 64
+// different banks.
+constexpr int kScanPad = kScanTile + kScanTile / kScanPerThread;
+__device__ __forceinline__ int scan_slot(int e) { return e + e / kScanPerThread; }
+
 template <bool CLEAR>
 __global__ void __launch_bounds__(kScanThreads)
 k_tile_scan(int32_t *__restrict__ in, int32_t *__restrict__ out, long n,
@@ -101,29 +109,41 @@ k_tile_scan(int32_t *__restrict__ in, int32_t *__restrict__ out, long n,
   const long tile = blockIdx.x;
   if (n_dev) n = min(n, (long)max(0, *fshift(n_dev, fo)));
   if (tile * kScanTile >= n) return;  // uniform: the whole workgroup leaves
+  __shared__ int32_t buf[kScanPad];
   int32_t *p = in + seg * stride;
   int32_t *q = out + seg * stride;
-  const long base = tile * kScanTile + (long)threadIdx.x * kScanPerThread;
+  const long t0 = tile * kScanTile;
+#pragma unroll
+  for (int k = 0; k < kScanPerThread; ++k) {
+    const int e = k * kScanThreads + (int)threadIdx.x;
+    const long i = t0 + e;
+    buf[scan_slot(e)] = i < n ? p[i] : 0;
+    if constexpr (CLEAR) {
+      if (i < n) p[i] = 0;
+    }
+  }
+  __syncthreads();
+  const int e0 = (int)threadIdx.x * kScanPerThread;
   int32_t v[kScanPerThread];
   int32_t s = 0;
 #pragma unroll
   for (int k = 0; k < kScanPerThread; ++k) {
-    const long i = base + k;
-    v[k] = i < n ? p[i] : 0;
+    v[k] = buf[scan_slot(e0 + k)];
     s += v[k];
-  }
-  if constexpr (CLEAR) {
-#pragma unroll
-    for (int k = 0; k < kScanPerThread; ++k)
-      if (base + k < n) p[base + k] = 0;
   }
   int32_t tot;
   int32_t run = block_excl_scan(s, lds, tot) + tsum[seg * (tiles + 1) + tile];
 #pragma unroll
   for (int k = 0; k < kScanPerThread; ++k) {
-    const long i = base + k;
-    if (i < n) q[i] = run;
+    buf[scan_slot(e0 + k)] = run;
     run += v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kScanPerThread; ++k) {
+    const int e = k * kScanThreads + (int)threadIdx.x;
+    const long i = t0 + e;
+    if (i < n) q[i] = buf[scan_slot(e)];
   }
 }
 
