@@ -1,7 +1,8 @@
-// olpt_scan.hip - three-phase exclusive scan (tile sums, scan of tile sums,
-// tile-local scan plus carry).  Each tile is 4096 ints handled by one
-// 256-thread workgroup (16 contiguous ints per thread, wave-shuffle scan of the
-// per-thread partials), so a 4M-entry array is 1024 tiles.
+// olpt_scan.hip - two-launch exclusive scan: tile sums, then the tile-local
+// scan, whose workgroup adds up the sums of the tiles before its own for the
+// carry.  Each tile is 4096 ints handled by one 256-thread workgroup (16
+// consecutive ints per thread, wave-shuffle scan of the per-thread partials),
+// so a 4M-entry array is 1024 tiles.
 #include "olpt_scan.hpp"
 #include "olpt_kernels.hpp"
 
@@ -63,28 +64,6 @@ k_tile_sums(const int32_t *__restrict__ in, long n, long stride,
   if (threadIdx.x == 0) tsum[seg * (tiles + 1) + tile] = tot;
 }
 
-// one workgroup per segment scans its tile sums in place
-__global__ void __launch_bounds__(kScanThreads)
-k_scan_tile_sums(int32_t *__restrict__ tsum, long tiles,
-                 int32_t *__restrict__ total, long fstride) {
-  __shared__ int32_t lds[kScanWaves];
-  const long fo = (long)blockIdx.z * fstride;
-  tsum = fshift(tsum, fo);
-  total = fshift(total, fo);
-  const int seg = blockIdx.x;
-  int32_t *p = tsum + seg * (tiles + 1);
-  int32_t carry = 0;
-  for (long b = 0; b < tiles; b += kScanThreads) {
-    const long i = b + threadIdx.x;
-    const int32_t v = i < tiles ? p[i] : 0;
-    int32_t tot;
-    const int32_t ex = block_excl_scan(v, lds, tot);
-    if (i < tiles) p[i] = ex + carry;
-    carry += tot;
-  }
-  if (threadIdx.x == 0 && total) total[seg] = carry;
-}
-
 // CLEAR: the input is zeroed once read (in and out distinct), so a count
 // array is ready for its next round of atomic increments without a launch
 // of its own.  The tile goes through LDS: loaded and stored with lanes on
@@ -99,7 +78,7 @@ template <bool CLEAR>
 __global__ void __launch_bounds__(kScanThreads)
 k_tile_scan(int32_t *__restrict__ in, int32_t *__restrict__ out, long n,
             long stride, const int32_t *__restrict__ tsum, long tiles, long fstride,
-            const int32_t *n_dev) {
+            const int32_t *n_dev, int32_t *total) {
   __shared__ int32_t lds[kScanWaves];
   const long fo = (long)blockIdx.z * fstride;
   in = fshift(in, fo);
@@ -108,7 +87,21 @@ k_tile_scan(int32_t *__restrict__ in, int32_t *__restrict__ out, long n,
   const int seg = blockIdx.y;
   const long tile = blockIdx.x;
   if (n_dev) n = min(n, (long)max(0, *fshift(n_dev, fo)));
-  if (tile * kScanTile >= n) return;  // uniform: the whole workgroup leaves
+  const bool past = tile * kScanTile >= n;  // uniform over the workgroup
+  if (past && tile != tiles - 1) return;
+  // the sums of the tiles before this one, added up here (a few hundred per
+  // segment at most) rather than scanned by a launch of their own; the last
+  // tile also writes the segment's total
+  const int32_t *ts = tsum + seg * (tiles + 1);
+  int32_t carry = 0;
+  for (long b0 = 0; b0 < tile; b0 += kScanThreads) {
+    const long j = b0 + threadIdx.x;
+    int32_t tot;
+    block_excl_scan(j < tile ? ts[j] : 0, lds, tot);
+    carry += tot;
+  }
+  if (tile == tiles - 1 && total && threadIdx.x == 0) fshift(total, fo)[seg] = carry + ts[tile];
+  if (past) return;
   __shared__ int32_t buf[kScanPad];
   int32_t *p = in + seg * stride;
   int32_t *q = out + seg * stride;
@@ -132,7 +125,7 @@ k_tile_scan(int32_t *__restrict__ in, int32_t *__restrict__ out, long n,
     s += v[k];
   }
   int32_t tot;
-  int32_t run = block_excl_scan(s, lds, tot) + tsum[seg * (tiles + 1) + tile];
+  int32_t run = block_excl_scan(s, lds, tot) + carry;
 #pragma unroll
   for (int k = 0; k < kScanPerThread; ++k) {
     buf[scan_slot(e0 + k)] = run;
@@ -154,10 +147,8 @@ void scan_exclusive(const int32_t *in, int32_t *out, long n, int segments,
   const long tiles = (n + kScanTile - 1) / kScanTile;
   dim3 g((unsigned)tiles, (unsigned)segments, (unsigned)nf);
   k_tile_sums<<<g, kScanThreads, 0, st>>>(in, n, seg_stride, scratch, tiles, fstride, n_dev);
-  k_scan_tile_sums<<<dim3(segments, 1, nf), kScanThreads, 0, st>>>(scratch, tiles, total,
-                                                                    fstride);
   k_tile_scan<false><<<g, kScanThreads, 0, st>>>(const_cast<int32_t *>(in), out, n, seg_stride,
-                                                 scratch, tiles, fstride, n_dev);
+                                                 scratch, tiles, fstride, n_dev, total);
 }
 
 void scan_exclusive_clear(int32_t *in, int32_t *out, long n, int32_t *total, int32_t *scratch,
@@ -166,8 +157,8 @@ void scan_exclusive_clear(int32_t *in, int32_t *out, long n, int32_t *total, int
   const long tiles = (n + kScanTile - 1) / kScanTile;
   dim3 g((unsigned)tiles, 1, (unsigned)nf);
   k_tile_sums<<<g, kScanThreads, 0, st>>>(in, n, n, scratch, tiles, fstride, nullptr);
-  k_scan_tile_sums<<<dim3(1, 1, nf), kScanThreads, 0, st>>>(scratch, tiles, total, fstride);
-  k_tile_scan<true><<<g, kScanThreads, 0, st>>>(in, out, n, n, scratch, tiles, fstride, nullptr);
+  k_tile_scan<true><<<g, kScanThreads, 0, st>>>(in, out, n, n, scratch, tiles, fstride, nullptr,
+                                                total);
 }
 
 } // namespace olpt
