@@ -1136,9 +1136,11 @@ __device__ __forceinline__ double dist2(const double *a, const double *c) {
   return d0 * d0 + d1 * d1 + d2 * d2;
 }
 
+// Objects closer than sqrt(t2) to object i (marked with mark_val when mark
+// is given); with first, 1 at the first one (only whether there is one).
 template <typename T>
 __device__ __forceinline__ int count_nb(const ShakeBuffers<T> &b, int i, double t2,
-                                        int8_t *mark, int mark_val) {
+                                        int8_t *mark, int mark_val, bool first = false) {
   const double *X = b.xyz + 3L * i;
   const int G = b.grid_n;
   const int cx = grid_cell_of(b, X, 0), cy = grid_cell_of(b, X, 1), cz = grid_cell_of(b, X, 2);
@@ -1154,6 +1156,7 @@ __device__ __forceinline__ int count_nb(const ShakeBuffers<T> &b, int i, double 
           const int j = b.g_obj[s0 + t];
           if (j == i) continue;
           if (dist2(X, b.xyz + 3L * j) < t2) {
+            if (first) return 1;
             ++found;
             if (mark) mark[j] = (int8_t)mark_val;
           }
@@ -1167,14 +1170,9 @@ __global__ void k_has_nb(ShakeBuffers<T> bb, double t2) {
   const long F = (long)blockIdx.z;
   const ShakeBuffers<T> b = bb.frame(F);
   const int n = *b.n_obj;
-  // entries past n are cleared: has_nb is scanned over cap_obj
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.cap_obj;
-       i += gridDim.x * blockDim.x) {
-    if (i >= n) {
-      b.has_nb[i] = 0;
-      continue;
-    }
-    b.has_nb[i] = count_nb(b, i, t2, nullptr, 0) > 0 ? 1 : 0;
+  // has_nb is scanned over the first n entries only
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    b.has_nb[i] = count_nb(b, i, t2, nullptr, 0, true);
     b.flag[i] = OLPT_FLAG_NONE;
   }
 }
@@ -1411,9 +1409,10 @@ template <typename T>
 __global__ void k_keep_flags(ShakeBuffers<T> bb) {
   const long F = (long)blockIdx.z;
   const ShakeBuffers<T> b = bb.frame(F);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.cap_obj;
-       i += gridDim.x * blockDim.x)
-    b.has_nb[i] = (i < *b.n_obj && b.flag[i] == OLPT_FLAG_NONE) ? 1 : 0;
+  // scanned over the first n_obj entries only
+  const int n = *b.n_obj;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    b.has_nb[i] = b.flag[i] == OLPT_FLAG_NONE ? 1 : 0;
 }
 
 template <typename T>
