@@ -72,6 +72,33 @@ static void zero_i32(const StereoBuffers &b, int32_t *p, long n, hipStream_t st)
   k_zero_i32<<<dim3((unsigned)blocks, 1, b.nf), 256, 0, st>>>(p, n, b.fstride);
 }
 
+// Several arrays zeroed by one launch (blockIdx.y picks the array): the
+// counters and count arrays of one pass are cleared together instead of by a
+// launch each.
+constexpr int kZeroMax = 12;
+struct ZeroList {
+  int32_t *p[kZeroMax];
+  long n[kZeroMax];
+  int count;
+};
+
+__global__ void k_zero_list(ZeroList z, long fstride) {
+  const int a = blockIdx.y;
+  int32_t *p = fshift(z.p[a], (long)blockIdx.z * fstride);
+  const long n = z.n[a];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+static void zero_list(const StereoBuffers &b, const ZeroList &z, hipStream_t st) {
+  long most = 0;
+  for (int a = 0; a < z.count; ++a) most = std::max(most, z.n[a]);
+  if (z.count <= 0 || most <= 0) return;
+  const long blocks = std::min(4096L, (most + 255) / 256);
+  k_zero_list<<<dim3((unsigned)blocks, z.count, b.nf), 256, 0, st>>>(z, b.fstride);
+}
+
 __global__ void k_cell_count(StereoBuffers bb, StereoParams p) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   const int cam = blockIdx.y;
@@ -2029,8 +2056,7 @@ static void prune_section(const StereoBuffers &b, const StereoParams &p, int gM,
                           hipStream_t st) {
   const int T = 256;
   const long nflat = (long)b.n_cam * b.cap;
-  zero_i32(b, b.pt_flag, nflat, st);
-  zero_i32(b, (int32_t *)b.own, 2 * nflat, st);
+  zero_list(b, ZeroList{{b.pt_flag, (int32_t *)b.own}, {nflat, 2 * nflat}, 2}, st);
   const GrpShape g = grp_shape(b);
   k_grp_count<<<dim3(g.g1, 1, b.nf), T, 0, st>>>(b, p, g);
   scan_exclusive(b.grp_cnt, b.grp_off, (long)g.nb * g.g1, 1, (long)g.nb * g.g1, b.n_entries,
@@ -2061,15 +2087,16 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
                    int max_rounds, hipStream_t st) {
   const long ncell = (long)b.rows_cell * b.cols_cell;
   const int T = 256;
-  // counters
-  zero_i32(b, b.n_cand, 1, st);
-  zero_i32(b, b.n_sel, 1, st);
-  zero_i32(b, b.work, 1, st);
-  zero_i32(b, b.undecided, max_rounds, st);
-  zero_i32(b, b.cell_cnt, ncell * b.n_cam, st);
+  // the pass's counters and count arrays, all in one launch (nothing before
+  // their first use reads them)
+  zero_list(b,
+            ZeroList{{b.n_cand, b.n_sel, b.work, b.undecided, b.cell_cnt, b.out_cnt, b.n_path,
+                      b.n_surv, b.n_pairs, b.key_cnt},
+                     {1, 1, 1, max_rounds, ncell * b.n_cam, b.n_cam, kMaxCam, kMaxCam, 1, b.cap},
+                     10},
+            st);
   // LOS + cell index
   dim3 gc(grid_for(b.cap, T, 1024), b.n_cam, b.nf);
-  zero_i32(b, b.out_cnt, b.n_cam, st);
   k_cell_count<<<gc, T, 0, st>>>(b, p);
   scan_exclusive(b.cell_cnt, b.cell_start, ncell, b.n_cam, ncell, nullptr,
                  b.scan_scratch, st, b.nf, b.fstride);
@@ -2079,9 +2106,6 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   // extend pair of kernels per deeper build camera
   const int G = launch_blocks(b.grid_build, b.nf);
   stage_begin(ST_BUILD, st);
-  zero_i32(b, b.n_path, kMaxCam, st);
-  zero_i32(b, b.n_surv, kMaxCam, st);
-  zero_i32(b, b.n_pairs, 1, st);
   k_b_second<<<dim3(grid_for(b.cap, T, 1024), 1, b.nf), T, 0, st>>>(b, p);
   k_b_band<<<dim3(G, 1, b.nf), T, 0, st>>>(b, p);
   k_b_extend<1><<<dim3(G, 1, b.nf), T, 0, st>>>(b, p);
@@ -2095,7 +2119,6 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   stage_begin(ST_PRUNE, st);
   // sort
   const int gM = grid_for(b.cap_cand, T, launch_blocks(b.grid_prune, b.nf));
-  zero_i32(b, b.key_cnt, b.cap, st);
   k_key_hist<<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p);
   scan_exclusive(b.key_cnt, b.key_start, b.cap, 1, b.cap, nullptr, b.scan_scratch, st, b.nf, b.fstride);
   k_key_scatter<<<dim3(gM, 1, b.nf), T, 0, st>>>(b, p);
