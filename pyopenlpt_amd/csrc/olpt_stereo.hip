@@ -1040,7 +1040,10 @@ __global__ void k_key_scatter(StereoBuffers bb, StereoParams p) {
     const int key = b.cand_ids[(long)j * nc + pc];
     const int q = b.key_start[key] + b.cand_rank[j];
     b.bucket_idx[q] = j;
-    for (int c = 0; c < nc; ++c) b.bucket_ids[(long)q * nc + c] = b.cand_ids[(long)j * nc + c];
+    if (nc == 4)  // one 16-byte row load and store
+      reinterpret_cast<int4 *>(b.bucket_ids)[q] = reinterpret_cast<const int4 *>(b.cand_ids)[j];
+    else
+      for (int c = 0; c < nc; ++c) b.bucket_ids[(long)q * nc + c] = b.cand_ids[(long)j * nc + c];
   }
 }
 
@@ -1063,7 +1066,15 @@ __global__ void k_bucket_rank(StereoBuffers bb, StereoParams p) {
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < M; q += gridDim.x * blockDim.x) {
     const int j = b.bucket_idx[q];
     int32_t idj[kMaxCam];
-    for (int c = 0; c < nc; ++c) idj[c] = b.bucket_ids[(long)q * nc + c];
+    if (nc == 4) {
+      const int4 v = reinterpret_cast<const int4 *>(b.bucket_ids)[q];
+      idj[0] = v.x;
+      idj[1] = v.y;
+      idj[2] = v.z;
+      idj[3] = v.w;
+    } else {
+      for (int c = 0; c < nc; ++c) idj[c] = b.bucket_ids[(long)q * nc + c];
+    }
     const int key = idj[pc];
     const int s = b.key_start[key], n = b.key_cnt[key];
     int r = 0;
@@ -1102,7 +1113,10 @@ __global__ void k_bucket_rank(StereoBuffers bb, StereoParams p) {
       }
     }
     const int pos = s + r;
-    for (int c = 0; c < nc; ++c) b.cand_ids_s[(long)pos * nc + c] = idj[c];
+    if (nc == 4)
+      reinterpret_cast<int4 *>(b.cand_ids_s)[pos] = make_int4(idj[0], idj[1], idj[2], idj[3]);
+    else
+      for (int c = 0; c < nc; ++c) b.cand_ids_s[(long)pos * nc + c] = idj[c];
     b.cand_e_s[pos] = b.cand_e[j];
   }
 }
