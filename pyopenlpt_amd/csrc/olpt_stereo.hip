@@ -573,7 +573,9 @@ constexpr double kCosParallax = 0.9945200708509531;
 // (selectNextCameraByMaxPairAngle, :1375-1411).
 // Returns 0 reject, 1 complete match (checkMatch error in *e_out),
 // 2 survivor (next camera in *next).
-template <int K>
+// TERM: K + 1 is the build count (the host picks the instantiation), so
+// each level compiles only its own branch.
+template <int K, bool TERM>
 __device__ int extend(const StereoBuffers &b, const StereoParams &p, Path &P, int t,
                       int pid, int *next, double *e_out) {
   constexpr int KK = K + 1;
@@ -608,42 +610,52 @@ __device__ int extend(const StereoBuffers &b, const StereoParams &p, Path &P, in
     }
     // only the decision counts: tri_decide settles it unless the error
     // lies within rounding of the tolerance
-    int d = KK != B ? tri_decide<KK>(P.P, P.U, tq, mx) : -1;
-    if (d < 0) {
-      double X[3], e;
-      d = triangulate_n<KK>(P.P, P.U, X, e) && e <= tq ? 1 : 0;
-    }
-    if (!d) return 0;
-  }
-  if (KK == B) {
-    // checkMatch, all-active branch (:530-540): LOS in camera order
-    int rank[KK];
+    auto quick = [&]() {
+      int d = tri_decide<KK>(P.P, P.U, tq, mx);
+      if (d < 0) {
+        double X[3], e;
+        d = triangulate_n<KK>(P.P, P.U, X, e) && e <= tq ? 1 : 0;
+      }
+      return d;
+    };
+    if constexpr (!TERM) {
+      if (!quick()) return 0;
+    } else {
+      // checkMatch, all-active branch (:530-540): LOS in camera order
+      int rank[KK];
 #pragma unroll
-    for (int j = 0; j < KK; ++j) {
-      rank[j] = 0;
+      for (int j = 0; j < KK; ++j) {
+        rank[j] = 0;
 #pragma unroll
-      for (int i = 0; i < KK; ++i) rank[j] += P.cam[i] < P.cam[j] ? 1 : 0;
-    }
-    double cp[KK][3], cu[KK][3];
+        for (int i = 0; i < KK; ++i) rank[j] += P.cam[i] < P.cam[j] ? 1 : 0;
+      }
+      double cp[KK][3], cu[KK][3];
 #pragma unroll
-    for (int r = 0; r < KK; ++r)
+      for (int r = 0; r < KK; ++r)
 #pragma unroll
-      for (int j = 0; j < KK; ++j)
-        if (rank[j] == r) {
+        for (int j = 0; j < KK; ++j)
+          if (rank[j] == r) {
 #pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            cp[r][d] = P.P[j][d];
-            cu[r][d] = P.U[j][d];
+            for (int d = 0; d < 3; ++d) {
+              cp[r][d] = P.P[j][d];
+              cu[r][d] = P.U[j][d];
+            }
           }
-        }
-    double Xc[3], e;
-    if (!triangulate_n<KK>(cp, cu, Xc, e)) {
-      atomicOr(&b.flags[6], 1);
-      return 0;
+      // checkMatch first: its error is needed anyway, and when it rejects
+      // the quick test cannot change the outcome, so one exact
+      // triangulation per item is left; flags[6] as when the quick test
+      // runs first (only for items that pass it)
+      double Xc[3], e;
+      const bool okc = triangulate_n<KK>(cp, cu, Xc, e);
+      if (okc && (e > p.tol3d || !in_limit(p.limit, Xc))) return 0;
+      if (!quick()) return 0;
+      if (!okc) {
+        atomicOr(&b.flags[6], 1);
+        return 0;
+      }
+      *e_out = e;
+      return 1;
     }
-    if (e > p.tol3d || !in_limit(p.limit, Xc)) return 0;
-    *e_out = e;
-    return 1;
   }
   unsigned rem = 0;
   for (int c = 0; c < b.n_cam; ++c) rem |= p.active[c] ? (1u << c) : 0u;
@@ -925,18 +937,18 @@ template <int K> union ExtendQueues {
   BlockQueue<PathRec<K + 1>, kSurvQueue> s;    // the others
 };
 
-template <int K>
+template <int K, bool TERM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(extend_waves<K>())))
 k_b_extend(StereoBuffers bb, StereoParams p) {
   const StereoBuffers b = bb.frame(blockIdx.z);
   int nref;
   const int ref = ref_camera(b, p, nref);
   const int n = min(K == 1 ? *b.n_pairs : b.n_path[K], (int)b.cap_pairs);
-  const bool terminal = K + 1 == p.build_count;
+  constexpr bool terminal = TERM;
   const int stride = gridDim.x * blockDim.x;
   __shared__ ExtendQueues<K> eq;
   if (threadIdx.x == 0) {
-    if (terminal)
+    if constexpr (terminal)
       eq.c.n = 0;
     else
       eq.s.n = 0;
@@ -973,9 +985,9 @@ k_b_extend(StereoBuffers bb, StereoParams p) {
       }
 #pragma unroll
       for (int j = 0; j < K; ++j) path_set(P, j, b, cam[j], pid[j]);
-      res = extend<K>(b, p, P, cam[K], pid[K], &next, &e);
+      res = extend<K, TERM>(b, p, P, cam[K], pid[K], &next, &e);
     }
-    if (terminal) {
+    if constexpr (terminal) {
       CandRec cr;
       if (res == 1) {
         // compile-time camera index: the record stays in registers
@@ -1006,7 +1018,7 @@ k_b_extend(StereoBuffers bb, StereoParams p) {
       bq_flush(eq.s, &b.n_surv[K], b.cap_pairs, &b.flags[1], false, out_s);
     }
   }
-  if (terminal)
+  if constexpr (terminal)
     bq_flush(eq.c, b.n_cand, b.cap_cand, &b.flags[0], true, out_c);
   else
     bq_flush(eq.s, &b.n_surv[K], b.cap_pairs, &b.flags[1], true, out_s);
@@ -2051,7 +2063,10 @@ static void build_level(const StereoBuffers &b, const StereoParams &p, hipStream
   if (K == 2) stage_begin(ST_ENUM2, st);
   k_b_enum<K><<<dim3(launch_blocks(b.grid_build, b.nf), 1, b.nf), 256, 0, st>>>(b, p);
   if (K == 2) stage_end(ST_ENUM2, st);
-  k_b_extend<K><<<dim3(launch_blocks(b.grid_build, b.nf), 1, b.nf), 256, 0, st>>>(b, p);
+  if (K + 1 == p.build_count)
+    k_b_extend<K, true><<<dim3(launch_blocks(b.grid_build, b.nf), 1, b.nf), 256, 0, st>>>(b, p);
+  else
+    k_b_extend<K, false><<<dim3(launch_blocks(b.grid_build, b.nf), 1, b.nf), 256, 0, st>>>(b, p);
 }
 
 GrpShape grp_shape(const StereoBuffers &b) {
@@ -2122,7 +2137,10 @@ void launch_stereo(const StereoBuffers &b, const StereoParams &p,
   stage_begin(ST_BUILD, st);
   k_b_second<<<dim3(grid_for(b.cap, T, 1024), 1, b.nf), T, 0, st>>>(b, p);
   k_b_band<<<dim3(G, 1, b.nf), T, 0, st>>>(b, p);
-  k_b_extend<1><<<dim3(G, 1, b.nf), T, 0, st>>>(b, p);
+  if (p.build_count == 2)
+    k_b_extend<1, true><<<dim3(G, 1, b.nf), T, 0, st>>>(b, p);
+  else
+    k_b_extend<1, false><<<dim3(G, 1, b.nf), T, 0, st>>>(b, p);
   if (p.build_count > 2) build_level<2>(b, p, st);
   if (p.build_count > 3) build_level<3>(b, p, st);
   if (p.build_count > 4) build_level<4>(b, p, st);
